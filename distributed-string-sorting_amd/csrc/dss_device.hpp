@@ -1,0 +1,89 @@
+// Device-side helpers shared by the kernels of libdss_gpu (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dss {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// ---------------------------------------------------------------- RNG / shuffle
+// Counter-based generator and Feistel shuffle of the seeded input generators
+// (identical to the host specification so host and device inputs agree).
+__host__ __device__ inline uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__host__ __device__ inline uint64_t rng(uint64_t seed, uint64_t stream, uint64_t i) {
+    return mix64(mix64(seed ^ (stream * 0xD1B54A32D192ED03ull)) + i);
+}
+
+__host__ __device__ inline uint64_t shuffle_index(uint64_t x, uint64_t n, uint64_t seed) {
+    if (n <= 1) return x;
+    unsigned bits = 0;
+    while ((1ull << bits) < n) ++bits;
+    if (bits & 1) ++bits;
+    if (bits < 2) bits = 2;
+    const unsigned half = bits / 2;
+    const uint64_t mask = (1ull << half) - 1;
+    do {
+        uint64_t L = x >> half, R = x & mask;
+        for (unsigned r = 0; r < 4; ++r) {
+            uint64_t F = mix64(seed ^ ((uint64_t)(r + 1) * 0xA24BAED4963EE407ull) ^ R) & mask;
+            uint64_t t = R;
+            R = L ^ F;
+            L = t;
+        }
+        x = (L << half) | R;
+    } while (x >= n);
+    return x;
+}
+
+// ---------------------------------------------------------------- key loads
+// 8 characters of a string starting at `depth`, big endian (first char in the
+// most significant byte), zero after the end of the string: the reference's
+// get_uint64 (src/strings/stringset.hpp:213-239).  Two aligned 8-byte loads
+// and a funnel shift; the arena is padded so the second word is readable.
+__device__ inline uint64_t load_word_le(const uint8_t* __restrict__ chars, uint64_t a) {
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + (a & ~7ull));
+    const unsigned sh = (unsigned)(a & 7) * 8;
+    const uint64_t lo = w[0];
+    if (sh == 0) return lo;
+    const uint64_t hi = w[1];
+    return (lo >> sh) | (hi << (64 - sh));
+}
+
+__device__ inline uint64_t load_key8(const uint8_t* __restrict__ chars, uint64_t off,
+                                     uint32_t len, uint32_t depth) {
+    if (depth >= len) return 0;
+    uint64_t v = __builtin_bswap64(load_word_le(chars, off + depth));
+    const uint32_t rem = len - depth;
+    if (rem < 8) v &= ~0ull << (8 * (8 - rem));
+    return v;
+}
+
+// Number of equal leading characters of two big-endian 8-char keys that are
+// known to differ.
+__device__ inline uint32_t key_lcp(uint64_t a, uint64_t b) {
+    return (uint32_t)(__builtin_clzll(a ^ b) >> 3);
+}
+
+// Characters before the end of string inside a key (0..8).
+__device__ inline uint32_t key_strlen(uint64_t k) {
+    return k == 0 ? 0u : 8u - ((uint32_t)__builtin_ctzll(k) >> 3);
+}
+
+// ---------------------------------------------------------------- wave helpers
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+__device__ inline uint64_t lanemask_lt() {
+    const uint32_t l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+}  // namespace dss

@@ -1,0 +1,554 @@
+// Device string set: creation from host arenas and on-device generators,
+// PE split, download, timing and error plumbing of the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "dss_device.hpp"
+#include "dss_internal.hpp"
+
+namespace dss {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+// ------------------------------------------------------------------ timing
+hipEvent_t KernelTimer::get_event() {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    DSS_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void KernelTimer::begin(hipStream_t st, const char* name, double alg_bytes, Rec& r) {
+    r.name = name;
+    r.bytes = alg_bytes;
+    r.a = get_event();
+    r.b = get_event();
+    DSS_HIP(hipEventRecord(r.a, st));
+}
+
+void KernelTimer::end(hipStream_t st, Rec& r) {
+    (void)hipEventRecord(r.b, st);
+    pending.push_back(r);
+}
+
+void KernelTimer::collect() {
+    for (auto& r : pending) {
+        float ms = 0;
+        (void)hipEventSynchronize(r.b);
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        Stat* st = nullptr;
+        for (auto& s : stats)
+            if (s.name == r.name) st = &s;
+        if (!st) {
+            stats.push_back(Stat{r.name});
+            st = &stats.back();
+        }
+        st->launches++;
+        st->ms += ms;
+        st->bytes += r.bytes;
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    pending.clear();
+}
+
+KernelTimer::~KernelTimer() {
+    for (auto& r : pending) {
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+}
+
+// ------------------------------------------------------------------ set
+dss_set_s* new_set(uint32_t num_pes) {
+    DSS_REQUIRE(num_pes >= 1 && num_pes <= (1u << 16), DSS_EINVAL, "num_pes must be in [1, 65536]");
+    dss_set_s* s = new dss_set_s();
+    s->p = num_pes;
+    DSS_HIP(hipGetDevice(&s->device));
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        throw Error{DSS_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
+    }
+    return s;
+}
+
+// ------------------------------------------------------ arena indexing
+constexpr uint32_t kIdxThreads = 256;
+constexpr uint32_t kIdxBytesPerThread = 64;
+constexpr uint64_t kIdxChunk = (uint64_t)kIdxThreads * kIdxBytesPerThread;
+
+__global__ void k_count_nul(const uint8_t* __restrict__ chars, uint64_t nbytes,
+                            uint64_t* __restrict__ block_cnt) {
+    __shared__ uint32_t red[kIdxThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kIdxChunk + (uint64_t)threadIdx.x * kIdxBytesPerThread;
+    uint32_t c = 0;
+    for (uint32_t i = 0; i < kIdxBytesPerThread; ++i) {
+        uint64_t a = base + i;
+        if (a < nbytes && chars[a] == 0) ++c;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    if (lane_id() == 0) red[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kIdxThreads / 64; ++w) t += red[w];
+        block_cnt[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of n u64 values in place, one block of 1024 threads.
+__global__ void k_scan_single(uint64_t* __restrict__ v, uint64_t n, uint64_t* __restrict__ total) {
+    __shared__ uint64_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint64_t per = (n + 1023) / 1024;
+    const uint64_t b = t * per, e = (b + per < n) ? b + per : n;
+    uint64_t s = 0;
+    for (uint64_t i = b; i < e; ++i) s += v[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        uint64_t x = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint64_t run = t ? part[t - 1] : 0;
+    for (uint64_t i = b; i < e; ++i) {
+        uint64_t x = v[i];
+        v[i] = run;
+        run += x;
+    }
+    if (t == 1023 && total) *total = part[1023];
+}
+
+__global__ void k_write_nul_pos(const uint8_t* __restrict__ chars, uint64_t nbytes,
+                                const uint64_t* __restrict__ block_off, uint64_t* __restrict__ nul_pos) {
+    __shared__ uint32_t cnt[kIdxThreads];
+    const uint64_t base = (uint64_t)blockIdx.x * kIdxChunk + (uint64_t)threadIdx.x * kIdxBytesPerThread;
+    uint32_t c = 0;
+    for (uint32_t i = 0; i < kIdxBytesPerThread; ++i) {
+        uint64_t a = base + i;
+        if (a < nbytes && chars[a] == 0) ++c;
+    }
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t t = 0; t < kIdxThreads; ++t) {
+            uint32_t x = cnt[t];
+            cnt[t] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    uint64_t k = block_off[blockIdx.x] + cnt[threadIdx.x];
+    for (uint32_t i = 0; i < kIdxBytesPerThread; ++i) {
+        uint64_t a = base + i;
+        if (a < nbytes && chars[a] == 0) nul_pos[k++] = a;
+    }
+}
+
+__global__ void k_offsets_from_nul(uint64_t* __restrict__ off, uint32_t* __restrict__ len,
+                                   const uint64_t* __restrict__ nul_pos, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t s = i == 0 ? 0 : nul_pos[i - 1] + 1;
+    len[i] = (uint32_t)(nul_pos[i] - s);
+    // off is written in a second pass (it aliases nothing; nul_pos is separate)
+    off[i] = s;
+}
+
+__global__ void k_max_len(const uint32_t* __restrict__ len, uint64_t n, uint32_t* __restrict__ out) {
+    uint32_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        m = max(m, len[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_down(m, o));
+    if (lane_id() == 0) atomicMax(out, m);
+}
+
+// distribute_strings (src/mpi/readInput.hpp:57-103): the string starting at
+// byte s belongs to the PE whose byte slice holds s-1 (s = 0 to PE 0), so
+// pe_begin[r] = number of strings with offset <= o_r.
+__global__ void k_pe_split(const uint64_t* __restrict__ off, uint64_t n, uint64_t nbytes,
+                           uint32_t p, uint64_t* __restrict__ pe_begin) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > p) return;
+    if (r == 0) { pe_begin[0] = 0; return; }
+    if (r == p) { pe_begin[p] = n; return; }
+    const uint64_t small = nbytes / p, larger = nbytes % p;
+    const uint64_t o_r = r < larger ? r * (small + 1) : larger * (small + 1) + (r - larger) * small;
+    uint64_t lo = 0, hi = n;  // first index with off > o_r
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) / 2;
+        if (off[mid] <= o_r) lo = mid + 1; else hi = mid;
+    }
+    pe_begin[r] = lo;
+}
+
+uint64_t index_arena(dss_set_s* s) {
+    const uint64_t nb = s->nbytes;
+    if (nb == 0) {
+        s->n = 0;
+        return 0;
+    }
+    const uint64_t blocks = (nb + kIdxChunk - 1) / kIdxChunk;
+    DevBuf bcnt, tot, nul;
+    bcnt.reserve((blocks + 1) * sizeof(uint64_t));
+    tot.reserve(sizeof(uint64_t));
+    hipLaunchKernelGGL(k_count_nul, dim3(blocks), dim3(kIdxThreads), 0, s->stream,
+                       s->chars.as<uint8_t>(), nb, bcnt.as<uint64_t>());
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s->stream, bcnt.as<uint64_t>(),
+                       blocks, tot.as<uint64_t>());
+    uint64_t n = 0;
+    DSS_HIP(hipMemcpyAsync(&n, tot.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+    DSS_HIP(hipStreamSynchronize(s->stream));
+    DSS_REQUIRE(n < 0xFFFFFFFFull, DSS_ERANGE, "more than 2^32-1 strings");
+    s->n = n;
+    s->off.reserve(std::max<uint64_t>(n, 1) * sizeof(uint64_t));
+    s->len.reserve(std::max<uint64_t>(n, 1) * sizeof(uint32_t));
+    nul.reserve(std::max<uint64_t>(n, 1) * sizeof(uint64_t));
+    if (n) {
+        hipLaunchKernelGGL(k_write_nul_pos, dim3(blocks), dim3(kIdxThreads), 0, s->stream,
+                           s->chars.as<uint8_t>(), nb, bcnt.as<uint64_t>(), nul.as<uint64_t>());
+        hipLaunchKernelGGL(k_offsets_from_nul, dim3((n + 255) / 256), dim3(256), 0, s->stream,
+                           s->off.as<uint64_t>(), s->len.as<uint32_t>(), nul.as<uint64_t>(), n);
+    }
+    DSS_HIP(hipGetLastError());
+    DSS_HIP(hipStreamSynchronize(s->stream));
+    return n;
+}
+
+void compute_pe_split(dss_set_s* s) {
+    s->pe_begin_d.reserve((s->p + 1) * sizeof(uint64_t));
+    DevBuf dummy;
+    const uint64_t* off = s->off.as<uint64_t>();
+    if (!off) {
+        dummy.reserve(8);
+        off = dummy.as<uint64_t>();
+    }
+    hipLaunchKernelGGL(k_pe_split, dim3((s->p + 1 + 255) / 256), dim3(256), 0, s->stream, off, s->n,
+                       s->nbytes, s->p, s->pe_begin_d.as<uint64_t>());
+    s->pe_begin.assign(s->p + 1, 0);
+    DSS_HIP(hipMemcpyAsync(s->pe_begin.data(), s->pe_begin_d.p, (s->p + 1) * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, s->stream));
+    // max length
+    DevBuf mx;
+    mx.reserve(4);
+    DSS_HIP(hipMemsetAsync(mx.p, 0, 4, s->stream));
+    if (s->n)
+        hipLaunchKernelGGL(k_max_len, dim3(std::min<uint64_t>((s->n + 255) / 256, 4096)), dim3(256), 0,
+                           s->stream, s->len.as<uint32_t>(), s->n, mx.as<uint32_t>());
+    DSS_HIP(hipMemcpyAsync(&s->max_len, mx.p, 4, hipMemcpyDeviceToHost, s->stream));
+    DSS_HIP(hipGetLastError());
+    DSS_HIP(hipStreamSynchronize(s->stream));
+}
+
+// ------------------------------------------------------ D/N generator
+// DNRatioGenerator::getRawStringsTimoStyle (random_string_generator.hpp:188-
+// 235) with a seeded shuffle: arena position j holds string i = shuffle(j),
+// the base-26 digits of i ('A'..'Z', padded with 'A') in the first k chars,
+// then slen-k copies of the fill character.  One thread per 8-byte word.
+__global__ void k_gen_dton(uint8_t* __restrict__ out, uint64_t n, uint32_t k, uint32_t slen,
+                           uint8_t fill, uint64_t seed, uint64_t words) {
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    const uint64_t stride = slen + 1;
+    const uint64_t total = n * stride;
+    uint64_t word = 0;
+    uint64_t cur_j = ~0ull, cur_i = 0;
+    for (uint32_t b = 0; b < 8; ++b) {
+        const uint64_t a = w * 8 + b;
+        uint8_t c = 0;
+        if (a < total) {
+            const uint64_t j = a / stride;
+            const uint32_t pos = (uint32_t)(a - j * stride);
+            if (j != cur_j) {
+                cur_j = j;
+                cur_i = shuffle_index(j, n, seed);
+            }
+            if (pos == slen) c = 0;
+            else if (pos >= k) c = fill;
+            else {
+                const uint32_t e = k - 1 - pos;  // digit index from the right
+                uint64_t v = cur_i;
+                if (e >= 14) v = 0;  // 26^14 > 2^64
+                else for (uint32_t t = 0; t < e && v; ++t) v /= 26;
+                c = (uint8_t)('A' + v % 26);
+            }
+        }
+        word |= (uint64_t)c << (8 * b);
+    }
+    reinterpret_cast<uint64_t*>(out)[w] = word;
+}
+
+__global__ void k_fixed_offsets(uint64_t* __restrict__ off, uint32_t* __restrict__ len, uint64_t n,
+                                uint32_t slen) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    off[i] = i * (uint64_t)(slen + 1);
+    len[i] = slen;
+}
+
+// ------------------------------------------------------ skewed generator
+struct SkewP {
+    uint64_t n;
+    uint32_t long_len, short_len, k_long, k_short, long_permille, dup_permille;
+};
+
+__device__ inline void skew_id_len(const SkewP& P, uint64_t seed, uint64_t i, uint64_t& id,
+                                   uint32_t& len, uint32_t& k) {
+    const uint64_t d = rng(seed, 2, i) % 1000;
+    const uint64_t x = (d < P.dup_permille) ? rng(seed, 3, i) % P.n : i;
+    const bool is_long = (rng(seed, 4, x) % 1000) < P.long_permille;
+    uint32_t L = is_long ? P.long_len : P.short_len;
+    const uint32_t kk = is_long ? P.k_long : P.k_short;
+    if (L < kk) L = kk;
+    id = x;
+    len = L;
+    k = kk;
+}
+
+__global__ void k_skew_len(SkewP P, uint64_t seed, uint64_t* __restrict__ size_out,
+                           uint32_t* __restrict__ len_out) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.n) return;
+    const uint64_t i = shuffle_index(j, P.n, seed);
+    uint64_t id;
+    uint32_t L, k;
+    skew_id_len(P, seed, i, id, L, k);
+    size_out[j] = L + 1;
+    len_out[j] = L;
+}
+
+// One wave per string.
+__global__ void k_skew_fill(SkewP P, uint64_t seed, const uint64_t* __restrict__ off,
+                            uint8_t* __restrict__ out) {
+    const uint64_t j = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (j >= P.n) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t i = shuffle_index(j, P.n, seed);
+    uint64_t id;
+    uint32_t L, k;
+    skew_id_len(P, seed, i, id, L, k);
+    const uint64_t h = rng(seed, 5, id);
+    uint8_t* s = out + off[j];
+    for (uint32_t t = lane; t <= L; t += 64) {
+        uint8_t c;
+        if (t == L) c = 0;
+        else if (t >= k) c = (uint8_t)(1 + mix64(h + t) % 255);
+        else {
+            const uint32_t e = k - 1 - t;
+            uint64_t v = id;
+            if (e >= 9) v = 0;  // 255^8 > 2^64
+            else for (uint32_t q = 0; q < e && v; ++q) v /= 255;
+            c = (uint8_t)(1 + v % 255);
+        }
+        s[t] = c;
+    }
+}
+
+}  // namespace dss
+
+using namespace dss;
+
+extern "C" {
+
+const char* dss_last_error(void) { return g_err.c_str(); }
+
+int dss_device_info(char* buf, size_t buflen) {
+    return guarded([&] {
+        int dev = 0;
+        DSS_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t pr;
+        DSS_HIP(hipGetDeviceProperties(&pr, dev));
+        snprintf(buf, buflen, "%s %s CUs=%d HBM=%.1fGiB", pr.name, pr.gcnArchName,
+                 pr.multiProcessorCount, pr.totalGlobalMem / 1073741824.0);
+    });
+}
+
+int dss_set_create(const uint8_t* chars, uint64_t nbytes, uint32_t num_pes, dss_set* out) {
+    return guarded([&] {
+        DSS_REQUIRE(out != nullptr, DSS_EINVAL, "out is NULL");
+        DSS_REQUIRE(nbytes == 0 || chars != nullptr, DSS_EINVAL, "chars is NULL");
+        DSS_REQUIRE(nbytes == 0 || chars[nbytes - 1] == 0, DSS_EINVAL,
+                    "arena must end with a NUL terminator");
+        dss_set_s* s = new_set(num_pes);
+        try {
+            s->nbytes = nbytes;
+            s->chars.reserve(nbytes + kArenaPad);
+            DSS_HIP(hipMemsetAsync(s->chars.p, 0, nbytes + kArenaPad, s->stream));
+            if (nbytes)
+                DSS_HIP(hipMemcpyAsync(s->chars.p, chars, nbytes, hipMemcpyHostToDevice, s->stream));
+            index_arena(s);
+            compute_pe_split(s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int dss_set_create_dton(uint64_t n, uint32_t len, double dton, uint64_t seed, uint32_t num_pes,
+                        dss_set* out) {
+    return guarded([&] {
+        DSS_REQUIRE(out != nullptr, DSS_EINVAL, "out is NULL");
+        DSS_REQUIRE(n >= 1 && n < 0xFFFFFFFFull, DSS_ERANGE, "n must be in [1, 2^32-1)");
+        // k and stringLength exactly as random_string_generator.hpp:196-199
+        const double a = (double)len * dton;
+        const double b = std::ceil(std::log((double)n) / std::log(26.0));
+        const uint32_t k = (uint32_t)(a > b ? a : b);
+        const uint32_t slen = len > k ? len : k;
+        const uint8_t fill = (uint8_t)('A' + rng(seed, 1, 0) % 26);
+        dss_set_s* s = new_set(num_pes);
+        try {
+            s->n = n;
+            s->nbytes = n * (uint64_t)(slen + 1);
+            s->chars.reserve(s->nbytes + kArenaPad);
+            DSS_HIP(hipMemsetAsync(s->chars.as<uint8_t>() + s->nbytes, 0, kArenaPad, s->stream));
+            const uint64_t words = (s->nbytes + 7) / 8;
+            hipLaunchKernelGGL(k_gen_dton, dim3((words + 255) / 256), dim3(256), 0, s->stream,
+                               s->chars.as<uint8_t>(), n, k, slen, fill, seed, words);
+            s->off.reserve(n * sizeof(uint64_t));
+            s->len.reserve(n * sizeof(uint32_t));
+            hipLaunchKernelGGL(k_fixed_offsets, dim3((n + 255) / 256), dim3(256), 0, s->stream,
+                               s->off.as<uint64_t>(), s->len.as<uint32_t>(), n, slen);
+            DSS_HIP(hipGetLastError());
+            compute_pe_split(s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int dss_set_create_skewed(uint64_t n, uint32_t long_len, uint32_t short_len, double long_dn,
+                          double short_dn, uint32_t long_permille, uint32_t dup_permille,
+                          uint64_t seed, uint32_t num_pes, dss_set* out) {
+    return guarded([&] {
+        DSS_REQUIRE(out != nullptr, DSS_EINVAL, "out is NULL");
+        DSS_REQUIRE(n >= 1 && n < 0xFFFFFFFFull, DSS_ERANGE, "n must be in [1, 2^32-1)");
+        auto kfun = [&](uint32_t L, double dn) {
+            double a = (double)L * dn;
+            double b = std::ceil(std::log((double)(n > 1 ? n : 2)) / std::log(255.0));
+            return (uint32_t)(a > b ? a : b);
+        };
+        SkewP P{n, long_len, short_len, kfun(long_len, long_dn), kfun(short_len, short_dn),
+                long_permille, dup_permille};
+        dss_set_s* s = new_set(num_pes);
+        try {
+            s->n = n;
+            s->off.reserve(n * sizeof(uint64_t));
+            s->len.reserve(n * sizeof(uint32_t));
+            DevBuf tot;
+            tot.reserve(8);
+            hipLaunchKernelGGL(k_skew_len, dim3((n + 255) / 256), dim3(256), 0, s->stream, P, seed,
+                               s->off.as<uint64_t>(), s->len.as<uint32_t>());
+            hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s->stream, s->off.as<uint64_t>(),
+                               n, tot.as<uint64_t>());
+            DSS_HIP(hipMemcpyAsync(&s->nbytes, tot.p, 8, hipMemcpyDeviceToHost, s->stream));
+            DSS_HIP(hipStreamSynchronize(s->stream));
+            s->chars.reserve(s->nbytes + kArenaPad);
+            DSS_HIP(hipMemsetAsync(s->chars.as<uint8_t>() + s->nbytes, 0, kArenaPad, s->stream));
+            hipLaunchKernelGGL(k_skew_fill, dim3((n + 3) / 4), dim3(256), 0, s->stream, P, seed,
+                               s->off.as<uint64_t>(), s->chars.as<uint8_t>());
+            DSS_HIP(hipGetLastError());
+            compute_pe_split(s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int dss_set_destroy(dss_set s) {
+    return guarded([&] {
+        if (!s) return;
+        (void)hipStreamSynchronize(s->stream);
+        hipStream_t st = s->stream;
+        delete s;
+        (void)hipStreamDestroy(st);
+    });
+}
+
+int dss_set_info(dss_set s, dss_set_info_t* info) {
+    return guarded([&] {
+        DSS_REQUIRE(s && info, DSS_EINVAL, "NULL argument");
+        info->num_strings = s->n;
+        info->num_bytes = s->nbytes;
+        info->num_pes = s->p;
+        info->max_len = s->max_len;
+    });
+}
+
+int dss_set_pe_begin(dss_set s, uint64_t* pe_begin) {
+    return guarded([&] {
+        DSS_REQUIRE(s && pe_begin, DSS_EINVAL, "NULL argument");
+        memcpy(pe_begin, s->pe_begin.data(), (s->p + 1) * sizeof(uint64_t));
+    });
+}
+
+int dss_set_download(dss_set s, uint8_t* chars, uint64_t* offsets, uint32_t* lengths) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        if (chars && s->nbytes)
+            DSS_HIP(hipMemcpyAsync(chars, s->chars.p, s->nbytes, hipMemcpyDeviceToHost, s->stream));
+        if (offsets && s->n)
+            DSS_HIP(hipMemcpyAsync(offsets, s->off.p, s->n * 8, hipMemcpyDeviceToHost, s->stream));
+        if (lengths && s->n)
+            DSS_HIP(hipMemcpyAsync(lengths, s->len.p, s->n * 4, hipMemcpyDeviceToHost, s->stream));
+        DSS_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int dss_sync(dss_set s) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        DSS_HIP(hipStreamSynchronize(s->stream));
+        s->timer.collect();
+    });
+}
+
+int dss_set_timing(dss_set s, int enable) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        s->timer.enabled = enable != 0;
+    });
+}
+
+int dss_kernel_stats(dss_set s, dss_kernel_stat_t* stats, uint32_t cap, uint32_t* count, int reset) {
+    return guarded([&] {
+        DSS_REQUIRE(s && count, DSS_EINVAL, "NULL argument");
+        DSS_HIP(hipStreamSynchronize(s->stream));
+        s->timer.collect();
+        uint32_t k = 0;
+        for (auto& st : s->timer.stats) {
+            if (k < cap && stats) {
+                memset(&stats[k], 0, sizeof(dss_kernel_stat_t));
+                strncpy(stats[k].name, st.name.c_str(), sizeof(stats[k].name) - 1);
+                stats[k].launches = st.launches;
+                stats[k].total_ms = st.ms;
+                stats[k].alg_bytes = st.bytes;
+            }
+            ++k;
+        }
+        *count = k;
+        if (reset) s->timer.stats.clear();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,691 @@
+// (a2) Local LCP string sort of all virtual PEs of a set.
+//
+// Replaces tlx radixsort_CI3 (TLXV/sort/strings/radix_sort.hpp:539-880) at
+// src/sorter/distributed/lcp_aware_number_string_splitting.hpp:218-224 and
+// src/sorter/distributed/prefix_doubling.hpp:228.  Output per position: the
+// sorted string id and lcp[i] = LCP(s[i-1], s[i]) inside the PE, lcp at a PE
+// start = 0.  The LCP rules are those of the reference: strings that end
+// together are equal and get lcp = their length, the first string of every
+// further non-empty bucket gets lcp = the bucket depth.
+//
+// Algorithm (MSD radix over 8-char big-endian keys, all PEs in one launch
+// sequence; every PE starts as one segment):
+//   large segments (> kSmallMax strings), per pass over all of them:
+//     prefix skip  -- common prefix of the segment from its depth, found by
+//                     16-lane groups comparing 128 bytes per step against the
+//                     segment's first string (early exit once the segment
+//                     minimum is below the group's position);
+//     key gather   -- 8 chars at the new depth per string (coalesced writes);
+//     histogram    -- 256 byte-digit counters per 2048-string tile in LDS;
+//     scan/classify-- one block per segment: digit offsets, sub-bucket lists,
+//                     LCP at sub-bucket boundaries;
+//     scatter      -- stable rank inside the tile by wave ballots, writes the
+//                     next key/id buffers, the id array and LCPs of equal
+//                     strings.  Up to 8 digit passes reuse one gathered key.
+//   small segments (2..2048 strings), batched by power-of-two class into
+//     2048-slot blocks: LDS bitonic sort of (group, key) with prefix skip and
+//     iterative refinement of equal-key groups until every group is a single
+//     string or a run of equal strings.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "dss_device.hpp"
+#include "dss_internal.hpp"
+
+namespace dss {
+
+constexpr uint32_t kSmallMax = 2048;  // base-case capacity (slots per block)
+constexpr uint32_t kClasses = 12;     // class c holds sizes (2^(c-1), 2^c], c = 1..11
+constexpr uint32_t kTile = 2048;      // strings per tile in the global passes
+constexpr uint32_t kThreads = 256;
+constexpr uint32_t kItems = kTile / kThreads;
+
+struct Seg {
+    uint32_t begin, cnt;
+    uint32_t depth;   // chars known to be common to all members
+    uint32_t kdepth;  // depth of the cached keys, kNone if keys must be gathered
+    uint32_t cp;      // common prefix found by the prefix skip (from depth)
+    uint32_t tile0;   // first tile of the segment in the current pass
+};
+
+struct Tile {
+    uint32_t seg, begin, end;
+};
+
+struct SmallSeg {
+    uint32_t begin, cnt, depth;
+};
+
+struct SmallLists {
+    uint64_t off[kClasses];
+    uint64_t cap[kClasses];
+};
+
+// counters: [0] next large segments, [1] tiles, [2 + c] small class c
+constexpr uint32_t kCtrLarge = 0, kCtrTiles = 1, kCtrSmall = 2;
+
+__device__ inline void push_small(SmallSeg* small, const SmallLists& L, uint32_t* ctr,
+                                  uint32_t begin, uint32_t cnt, uint32_t depth) {
+    const uint32_t c = 32u - __builtin_clz(cnt - 1u);  // ceil(log2 cnt), cnt >= 2
+    const uint32_t k = atomicAdd(&ctr[kCtrSmall + c], 1u);
+    small[L.off[c] + k] = SmallSeg{begin, cnt, depth};
+}
+
+// ------------------------------------------------------------------ init
+__global__ void k_sort_init(uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        perm[i] = (uint32_t)i;
+        lcp[i] = 0;
+    }
+}
+
+__global__ void k_sort_seed(const uint64_t* __restrict__ pe_begin, uint32_t p, Seg* __restrict__ segs,
+                            SmallSeg* __restrict__ small, SmallLists L, uint32_t* __restrict__ ctr) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= p) return;
+    const uint32_t b = (uint32_t)pe_begin[r];
+    const uint32_t cnt = (uint32_t)(pe_begin[r + 1] - pe_begin[r]);
+    if (cnt > kSmallMax) {
+        const uint32_t k = atomicAdd(&ctr[kCtrLarge], 1u);
+        segs[k] = Seg{b, cnt, 0, kNone, kNone, 0};
+    } else if (cnt >= 2) {
+        push_small(small, L, ctr, b, cnt, 0);
+    }
+}
+
+// ------------------------------------------------------------------ tiles
+__global__ void k_build_tiles(Seg* __restrict__ segs, uint32_t nseg, Tile* __restrict__ tiles,
+                              uint32_t* __restrict__ ctr) {
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t carry;
+    const uint32_t t = threadIdx.x;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < nseg; base += 1024) {
+        const uint32_t i = base + t;
+        uint32_t nt = 0;
+        if (i < nseg) nt = (segs[i].cnt + kTile - 1) / kTile;
+        part[t] = nt;
+        __syncthreads();
+        for (uint32_t o = 1; o < 1024; o <<= 1) {
+            uint32_t x = t >= o ? part[t - o] : 0;
+            __syncthreads();
+            part[t] += x;
+            __syncthreads();
+        }
+        const uint32_t excl = carry + part[t] - nt;
+        if (i < nseg) {
+            Seg s = segs[i];
+            s.tile0 = excl;
+            if (s.kdepth == kNone) s.cp = kNone;
+            segs[i] = s;
+            for (uint32_t k = 0; k < nt; ++k) {
+                const uint32_t b = s.begin + k * kTile;
+                const uint32_t e = min(s.begin + s.cnt, b + kTile);
+                tiles[excl + k] = Tile{i, b, e};
+            }
+        }
+        __syncthreads();
+        if (t == 1023) carry += part[1023];
+        __syncthreads();
+    }
+    if (t == 0) ctr[kCtrTiles] = carry;
+}
+
+// Effective key depth and digit depth of a segment in the current pass.
+__device__ inline void seg_depths(const Seg& s, uint32_t& kd, uint32_t& dep) {
+    if (s.kdepth == kNone) {
+        kd = s.depth + s.cp;
+        dep = kd;
+    } else {
+        kd = s.kdepth;
+        dep = s.depth;
+    }
+}
+
+// ------------------------------------------------------------ prefix skip
+// Groups of 16 lanes compare one string against the segment's first string,
+// 128 bytes per step.  All loops that contain cross-lane operations have
+// block- or wave-uniform trip counts, so every shuffle runs with all lanes of
+// the wave active (a disabled source lane would read as 0).
+__global__ void __launch_bounds__(kThreads)
+k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
+              const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
+              const uint32_t* __restrict__ len) {
+    __shared__ uint32_t bmin;
+    const Tile T = tiles[blockIdx.x];
+    const Seg S = segs[T.seg];
+    if (S.kdepth != kNone) return;
+    if (threadIdx.x == 0) bmin = kNone;
+    __syncthreads();
+    const uint32_t g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const uint32_t ref = perm[S.begin];
+    const uint64_t roff = off[ref];
+    const uint32_t rlen = len[ref];
+    const uint32_t d0 = S.depth;
+    const volatile uint32_t* cpp = &segs[T.seg].cp;
+    uint32_t my_min = kNone;
+    const uint32_t rounds = (T.end - T.begin + 15) / 16;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t e = T.begin + r * 16 + g;
+        bool done = e >= T.end;
+        uint64_t so = 0;
+        uint32_t sl = 0;
+        if (!done) {
+            const uint32_t id = perm[e];
+            so = off[id];
+            sl = len[id];
+        }
+        uint32_t base = d0;
+        uint32_t res = kNone;
+        while (__ballot(!done) != 0) {  // wave-uniform condition
+            uint32_t m = kNone;
+            const uint32_t pos = base + 8 * l;
+            if (!done) {
+                const uint64_t a = load_key8(chars, so, sl, pos);
+                const uint64_t b = load_key8(chars, roff, rlen, pos);
+                if (a != b) m = pos + key_lcp(a, b);
+                else if ((a & 0xFF) == 0) m = pos + key_strlen(a);
+            }
+            for (int o = 8; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor(m, o, 16));
+            if (!done) {
+                if (m != kNone) {
+                    res = m - d0;
+                    done = true;
+                } else {
+                    base += 128;
+                    const uint32_t cur = *cpp;
+                    if (cur != kNone && base - d0 >= cur) done = true;  // cannot lower the minimum
+                }
+            }
+        }
+        if (res != kNone) my_min = min(my_min, res);
+    }
+    if (l == 0 && my_min != kNone) atomicMin(&bmin, my_min);
+    __syncthreads();
+    if (threadIdx.x == 0 && bmin != kNone) atomicMin(&segs[T.seg].cp, bmin);
+}
+
+// ------------------------------------------------------------ key gather
+__global__ void __launch_bounds__(kThreads)
+k_gather(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
+         const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
+         const uint32_t* __restrict__ len, uint64_t* __restrict__ key, uint32_t* __restrict__ pb) {
+    const Tile T = tiles[blockIdx.x];
+    const Seg S = segs[T.seg];
+    if (S.kdepth != kNone) return;
+    const uint32_t kd = S.depth + S.cp;
+    for (uint32_t e = T.begin + threadIdx.x; e < T.end; e += kThreads) {
+        const uint32_t id = perm[e];
+        key[e] = load_key8(chars, off[id], len[id], kd);
+        pb[e] = id;
+    }
+}
+
+// ------------------------------------------------------------ histogram
+__global__ void __launch_bounds__(kThreads)
+k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint64_t* __restrict__ key,
+       uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    const Tile T = tiles[blockIdx.x];
+    const Seg S = segs[T.seg];
+    uint32_t kd, dep;
+    seg_depths(S, kd, dep);
+    const uint32_t sh = 56 - 8 * (dep - kd);
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t e = T.begin + threadIdx.x; e < T.end; e += kThreads)
+        atomicAdd(&h[(uint32_t)(key[e] >> sh) & 0xFF], 1u);
+    __syncthreads();
+    hist[(uint64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+}
+
+// ------------------------------------------------------------ scan/classify
+__global__ void __launch_bounds__(kThreads)
+k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, Seg* __restrict__ next,
+                SmallSeg* __restrict__ small, SmallLists L, uint32_t* __restrict__ ctr,
+                uint32_t* __restrict__ lcp) {
+    __shared__ uint32_t sc[256];
+    __shared__ uint32_t first;
+    const Seg S = segs[blockIdx.x];
+    uint32_t kd, dep;
+    seg_depths(S, kd, dep);
+    const uint32_t d = threadIdx.x;
+    const uint32_t nt = (S.cnt + kTile - 1) / kTile;
+    uint32_t tot = 0;
+    for (uint32_t t = 0; t < nt; ++t) tot += hist[(uint64_t)(S.tile0 + t) * 256 + d];
+    sc[d] = tot;
+    if (d == 0) first = 256;
+    __syncthreads();
+    for (uint32_t o = 1; o < 256; o <<= 1) {
+        uint32_t x = d >= o ? sc[d - o] : 0;
+        __syncthreads();
+        sc[d] += x;
+        __syncthreads();
+    }
+    const uint32_t base = sc[d] - tot;
+    if (tot) atomicMin(&first, d);
+    uint32_t run = S.begin + base;
+    for (uint32_t t = 0; t < nt; ++t) {
+        uint32_t* hp = &hist[(uint64_t)(S.tile0 + t) * 256 + d];
+        const uint32_t x = *hp;
+        *hp = run;
+        run += x;
+    }
+    __syncthreads();
+    if (tot == 0) return;
+    const uint32_t b = S.begin + base;
+    if (d != first) lcp[b] = dep;  // boundary between non-empty buckets
+    if (d == 0 || tot == 1) return;  // equal strings / single string: done
+    if (tot <= kSmallMax) {
+        push_small(small, L, ctr, b, tot, dep + 1);
+    } else {
+        const uint32_t k = atomicAdd(&ctr[kCtrLarge], 1u);
+        const uint32_t nkd = (dep + 1 - kd >= 8) ? kNone : kd;
+        next[k] = Seg{b, tot, dep + 1, nkd, kNone, 0};
+    }
+}
+
+// ------------------------------------------------------------ scatter
+__global__ void __launch_bounds__(kThreads)
+k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint32_t* __restrict__ hist,
+          const uint64_t* __restrict__ key_in, const uint32_t* __restrict__ pb_in,
+          uint64_t* __restrict__ key_out, uint32_t* __restrict__ pb_out, uint32_t* __restrict__ perm,
+          uint32_t* __restrict__ lcp) {
+    __shared__ uint32_t run_base[256];
+    __shared__ uint32_t wcnt[kThreads / 64][256];
+    const Tile T = tiles[blockIdx.x];
+    const Seg S = segs[T.seg];
+    uint32_t kd, dep;
+    seg_depths(S, kd, dep);
+    const uint32_t sh = 56 - 8 * (dep - kd);
+    const uint32_t tid = threadIdx.x, w = tid >> 6;
+    run_base[tid] = hist[(uint64_t)blockIdx.x * 256 + tid];
+    for (uint32_t k = 0; k < kThreads / 64; ++k) wcnt[k][tid] = 0;
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    for (uint32_t r = 0; r < kItems; ++r) {
+        const uint32_t e = T.begin + r * kThreads + tid;
+        const bool valid = e < T.end;
+        uint64_t k = 0;
+        uint32_t id = 0, dg = 0;
+        if (valid) {
+            k = key_in[e];
+            id = pb_in[e];
+            dg = (uint32_t)(k >> sh) & 0xFF;
+        }
+        uint64_t m = __ballot(valid);
+        for (uint32_t b = 0; b < 8; ++b) {
+            const bool bit = (dg >> b) & 1;
+            const uint64_t bb = __ballot(valid && bit);
+            m &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(m & lt);
+        if (valid && rank == 0) wcnt[w][dg] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t q = run_base[dg] + rank;
+            for (uint32_t ww = 0; ww < w; ++ww) q += wcnt[ww][dg];
+            key_out[q] = k;
+            pb_out[q] = id;
+            perm[q] = id;
+            if (dg == 0 && q != S.begin) lcp[q] = dep;  // equal strings of length dep
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (uint32_t ww = 0; ww < kThreads / 64; ++ww) {
+            add += wcnt[ww][tid];
+            wcnt[ww][tid] = 0;
+        }
+        run_base[tid] += add;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ small sort
+// Block-wide exclusive max-scan helper over kThreads threads.
+__device__ inline uint32_t block_excl_max(uint32_t v, uint32_t* tmp) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o) x = max(x, y);
+    }
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t k = 0; k < w; ++k) pre = max(pre, tmp[k]);
+    uint32_t ex = __shfl_up(x, 1);
+    if (lane == 0) ex = 0;
+    __syncthreads();
+    return max(pre, ex);
+}
+
+// LCP of two strings from depth d, by 8-char words (one lane).
+__device__ inline uint32_t lcp_walk(const uint8_t* __restrict__ chars, uint64_t ao, uint32_t al,
+                                    uint64_t bo, uint32_t bl, uint32_t d) {
+    for (uint32_t pos = d;; pos += 8) {
+        const uint64_t a = load_key8(chars, ao, al, pos);
+        const uint64_t b = load_key8(chars, bo, bl, pos);
+        if (a != b) return pos - d + key_lcp(a, b);
+        if ((a & 0xFF) == 0) return pos - d + key_strlen(a);
+    }
+}
+
+constexpr uint32_t kCap = kSmallMax;
+constexpr uint32_t kPer = kCap / kThreads;  // elements per thread
+
+__global__ void __launch_bounds__(kThreads)
+k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2,
+             uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
+             const uint64_t* __restrict__ off, const uint32_t* __restrict__ len) {
+    __shared__ uint64_t K[kCap];
+    __shared__ uint32_t G[kCap];
+    __shared__ uint32_t I[kCap];
+    __shared__ uint32_t LO[kCap];  // lcp per slot position
+    __shared__ uint32_t RD[kCap];  // group depth, indexed by group start
+    __shared__ uint32_t RC[kCap];  // group common prefix / scratch
+    __shared__ uint8_t ACT[kCap];  // group active, indexed by group start
+    __shared__ uint32_t tmp[kThreads / 64];
+    const uint32_t CLS = 1u << cls_log2;
+    const uint32_t spb = kCap >> cls_log2;
+    const uint32_t s0 = blockIdx.x * spb;
+    const uint32_t tid = threadIdx.x;
+
+    for (uint32_t e = tid; e < kCap; e += kThreads) {
+        const uint32_t slot = e >> cls_log2, j = e & (CLS - 1);
+        const uint32_t sbase = slot << cls_log2;
+        LO[e] = 0;
+        ACT[e] = 0;
+        K[e] = ~0ull;
+        I[e] = kNone;
+        G[e] = sbase;
+        if (s0 + slot < nseg) {
+            const SmallSeg sg = segs[s0 + slot];
+            if (j < sg.cnt) {
+                I[e] = perm[sg.begin + j];
+                K[e] = 0;
+            } else {
+                G[e] = sbase + sg.cnt;
+            }
+            if (j == 0) {
+                RD[e] = sg.depth;
+                ACT[e] = sg.cnt >= 2;
+            }
+        }
+    }
+    __syncthreads();
+
+    for (;;) {
+        // 1. prefix skip inside every active group
+        for (uint32_t e = tid; e < kCap; e += kThreads)
+            if (I[e] != kNone && ACT[G[e]] && e == G[e]) RC[e] = kNone;
+        __syncthreads();
+        for (uint32_t e = tid; e < kCap; e += kThreads) {
+            const uint32_t id = I[e];
+            if (id == kNone) continue;
+            const uint32_t g = G[e];
+            if (!ACT[g] || e == g) continue;  // groups have >= 2 members
+            const uint32_t r = I[g];
+            const uint32_t l = lcp_walk(chars, off[id], len[id], off[r], len[r], RD[g]);
+            atomicMin(&RC[g], l);
+        }
+        __syncthreads();
+        for (uint32_t e = tid; e < kCap; e += kThreads)
+            if (I[e] != kNone && e == G[e] && ACT[e]) RD[e] += RC[e];
+        __syncthreads();
+        // 2. keys of active groups
+        for (uint32_t e = tid; e < kCap; e += kThreads) {
+            const uint32_t id = I[e];
+            if (id != kNone && ACT[G[e]]) K[e] = load_key8(chars, off[id], len[id], RD[G[e]]);
+        }
+        __syncthreads();
+        // 3. bitonic sort of (G, K) inside every aligned CLS slot
+        for (uint32_t k = 2; k <= CLS; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t t = tid; t < kCap / 2; t += kThreads) {
+                    const uint32_t i = 2 * t - (t & (j - 1));
+                    const uint32_t q = i + j;
+                    const bool asc = (k == CLS) || ((i & k) == 0);
+                    const uint32_t gi = G[i], gq = G[q];
+                    const uint64_t ki = K[i], kq = K[q];
+                    const bool gt = gi > gq || (gi == gq && ki > kq);
+                    const bool lt = gi < gq || (gi == gq && ki < kq);
+                    if (asc ? gt : lt) {
+                        G[i] = gq;
+                        G[q] = gi;
+                        K[i] = kq;
+                        K[q] = ki;
+                        const uint32_t x = I[i];
+                        I[i] = I[q];
+                        I[q] = x;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // 4. boundaries, LCPs and new groups (thread owns kPer consecutive slots)
+        uint32_t flag[kPer];
+        uint32_t oldg[kPer];
+        uint32_t last = 0;
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t e = tid * kPer + q;
+            const uint32_t g = G[e];
+            oldg[q] = g;
+            uint32_t f = (e == g);
+            if (!f && I[e] != kNone && ACT[g]) {
+                const uint64_t a = K[e - 1], b = K[e];
+                const uint32_t d = RD[g];
+                if (a != b) {
+                    f = 1;
+                    LO[e] = d + key_lcp(a, b);
+                } else if ((b & 0xFF) == 0) {
+                    LO[e] = d + key_strlen(b);
+                }
+            }
+            flag[q] = f;
+            if (f) last = e;
+        }
+        uint32_t run = block_excl_max(last, tmp);
+        uint32_t newg[kPer];
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t e = tid * kPer + q;
+            if (flag[q]) run = e;
+            newg[q] = run;
+        }
+        // old group depth per element (read before any write)
+        uint32_t od[kPer];
+        uint8_t oact[kPer];
+        for (uint32_t q = 0; q < kPer; ++q) {
+            od[q] = RD[oldg[q]];
+            oact[q] = ACT[oldg[q]];
+        }
+        __syncthreads();
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t e = tid * kPer + q;
+            G[e] = newg[q];
+            if (flag[q]) {
+                ACT[e] = 0;
+                if (oact[q]) RD[e] = od[q] + 8;
+            }
+        }
+        __syncthreads();
+        // a new group stays active if it came from an active group, has a
+        // second member and its key did not reach the end of the strings
+        int any = 0;
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t e = tid * kPer + q;
+            if (oact[q] && !flag[q] && I[e] != kNone && (K[e] & 0xFF) != 0) {
+                ACT[newg[q]] = 1;
+                any = 1;
+            }
+        }
+        if (!__syncthreads_or(any)) break;
+    }
+    // write back
+    for (uint32_t e = tid; e < kCap; e += kThreads) {
+        const uint32_t slot = e >> cls_log2, j = e & (CLS - 1);
+        if (s0 + slot >= nseg) continue;
+        const SmallSeg sg = segs[s0 + slot];
+        if (j >= sg.cnt) continue;
+        perm[sg.begin + j] = I[e];
+        if (j) lcp[sg.begin + j] = LO[e];
+    }
+}
+
+// ------------------------------------------------------------------ driver
+void sort_lcp(dss_set_s* s) {
+    const uint64_t n = s->n;
+    const hipStream_t st = s->stream;
+    s->perm.reserve(std::max<uint64_t>(n, 1) * 4);
+    s->lcp.reserve(std::max<uint64_t>(n, 1) * 4);
+    s->sorted = false;
+    if (n == 0) {
+        s->sorted = true;
+        return;
+    }
+    SortScratch& X = s->ss;
+    X.keyA.reserve(n * 8);
+    X.keyB.reserve(n * 8);
+    X.pbA.reserve(n * 4);
+    X.pbB.reserve(n * 4);
+    const uint64_t seg_cap = n / (kSmallMax + 1) + s->p + 16;
+    X.segA.reserve(seg_cap * sizeof(Seg));
+    X.segB.reserve(seg_cap * sizeof(Seg));
+    const uint64_t tile_cap = n / kTile + seg_cap + 16;
+    X.tiles.reserve(tile_cap * sizeof(Tile));
+    X.hist.reserve(tile_cap * 256 * 4);
+    SmallLists L;
+    uint64_t tot = 0;
+    for (uint32_t c = 0; c < kClasses; ++c) {
+        const uint64_t cap = c == 0 ? 0 : n / ((1ull << (c - 1)) + 1) + 1;
+        L.off[c] = tot;
+        L.cap[c] = cap;
+        tot += cap;
+    }
+    X.small.reserve(tot * sizeof(SmallSeg));
+    X.counters.reserve(64 * 4);
+    uint32_t* ctr = X.counters.as<uint32_t>();
+    DSS_HIP(hipMemsetAsync(ctr, 0, 64 * 4, st));
+
+    {
+        KScope ks(s->timer, st, "sort_init", n * 8.0);
+        hipLaunchKernelGGL(k_sort_init, dim3((n + 255) / 256), dim3(256), 0, st, s->perm.as<uint32_t>(),
+                           s->lcp.as<uint32_t>(), n);
+    }
+    hipLaunchKernelGGL(k_sort_seed, dim3((s->p + 255) / 256), dim3(256), 0, st,
+                       s->pe_begin_d.as<uint64_t>(), s->p, X.segA.as<Seg>(), X.small.as<SmallSeg>(), L, ctr);
+    DSS_HIP(hipGetLastError());
+
+    Seg* cur = X.segA.as<Seg>();
+    Seg* nxt = X.segB.as<Seg>();
+    uint64_t* kc = X.keyA.as<uint64_t>();
+    uint64_t* kn = X.keyB.as<uint64_t>();
+    uint32_t* pc = X.pbA.as<uint32_t>();
+    uint32_t* pn = X.pbB.as<uint32_t>();
+    uint32_t h_ctr[2];
+    DSS_HIP(hipMemcpyAsync(h_ctr, ctr, 4, hipMemcpyDeviceToHost, st));
+    DSS_HIP(hipStreamSynchronize(st));
+    uint32_t nlarge = h_ctr[0];
+    const bool trace = getenv("DSS_SORT_TRACE") != nullptr;
+    const uint8_t* chars = s->chars.as<uint8_t>();
+    const uint64_t* off = s->off.as<uint64_t>();
+    const uint32_t* len = s->len.as<uint32_t>();
+    uint32_t* perm = s->perm.as<uint32_t>();
+    uint32_t* lcp = s->lcp.as<uint32_t>();
+    while (nlarge) {
+        DSS_REQUIRE(nlarge <= seg_cap, DSS_ERANGE, "segment list overflow");
+        DSS_HIP(hipMemsetAsync(ctr, 0, 8, st));
+        hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, cur, nlarge, X.tiles.as<Tile>(), ctr);
+        DSS_HIP(hipMemcpyAsync(h_ctr, ctr, 8, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+        const uint32_t nt = h_ctr[1];
+        DSS_REQUIRE(nt <= tile_cap, DSS_ERANGE, "tile list overflow");
+        const Tile* tl = X.tiles.as<Tile>();
+        {
+            KScope ks(s->timer, st, "prefix_skip", 0.0);
+            hipLaunchKernelGGL(k_prefix_skip, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len);
+        }
+        {
+            KScope ks(s->timer, st, "gather", 0.0);
+            hipLaunchKernelGGL(k_gather, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc, pc);
+        }
+        {
+            KScope ks(s->timer, st, "hist", 0.0);
+            hipLaunchKernelGGL(k_hist, dim3(nt), dim3(kThreads), 0, st, tl, cur, kc, X.hist.as<uint32_t>());
+        }
+        {
+            KScope ks(s->timer, st, "scan_classify", 0.0);
+            hipLaunchKernelGGL(k_scan_classify, dim3(nlarge), dim3(kThreads), 0, st, cur, X.hist.as<uint32_t>(),
+                               nxt, X.small.as<SmallSeg>(), L, ctr, lcp);
+        }
+        {
+            KScope ks(s->timer, st, "scatter", 0.0);
+            hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kThreads), 0, st, tl, cur, X.hist.as<uint32_t>(), kc, pc,
+                               kn, pn, perm, lcp);
+        }
+        DSS_HIP(hipGetLastError());
+        DSS_HIP(hipMemcpyAsync(h_ctr, ctr, 4, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+        if (trace) {
+            std::vector<Seg> hs(std::min<uint32_t>(nlarge, 4));
+            DSS_HIP(hipMemcpy(hs.data(), cur, hs.size() * sizeof(Seg), hipMemcpyDeviceToHost));
+            fprintf(stderr, "[dss sort] pass nlarge=%u ntiles=%u next=%u", nlarge, nt, h_ctr[0]);
+            for (auto& q : hs)
+                fprintf(stderr, " {b=%u c=%u d=%u kd=%d cp=%d}", q.begin, q.cnt, q.depth, (int)q.kdepth, (int)q.cp);
+            fprintf(stderr, "\n");
+        }
+        nlarge = h_ctr[0];
+        std::swap(cur, nxt);
+        std::swap(kc, kn);
+        std::swap(pc, pn);
+    }
+    uint32_t cls[kClasses];
+    DSS_HIP(hipMemcpyAsync(cls, ctr + kCtrSmall, sizeof(cls), hipMemcpyDeviceToHost, st));
+    DSS_HIP(hipStreamSynchronize(st));
+    for (uint32_t c = 1; c < kClasses; ++c) {
+        const uint32_t cnt = cls[c];
+        if (!cnt) continue;
+        DSS_REQUIRE(cnt <= L.cap[c], DSS_ERANGE, "small list overflow");
+        const uint32_t spb = kCap >> c;
+        const uint32_t blocks = (cnt + spb - 1) / spb;
+        KScope ks(s->timer, st, "small_sort", 0.0);
+        hipLaunchKernelGGL(k_small_sort, dim3(blocks), dim3(kThreads), 0, st,
+                           X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len);
+    }
+    DSS_HIP(hipGetLastError());
+    s->sorted = true;
+}
+
+}  // namespace dss
+
+using namespace dss;
+
+extern "C" {
+
+int dss_sort_lcp(dss_set s) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        sort_lcp(s);
+    });
+}
+
+int dss_get_sorted(dss_set s, uint32_t* perm, uint32_t* lcp) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        DSS_REQUIRE(s->sorted, DSS_ESTATE, "dss_sort_lcp has not run");
+        if (perm && s->n)
+            DSS_HIP(hipMemcpyAsync(perm, s->perm.p, s->n * 4, hipMemcpyDeviceToHost, s->stream));
+        if (lcp && s->n)
+            DSS_HIP(hipMemcpyAsync(lcp, s->lcp.p, s->n * 4, hipMemcpyDeviceToHost, s->stream));
+        DSS_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+}  // extern "C"

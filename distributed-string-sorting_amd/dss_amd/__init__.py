@@ -1,0 +1,137 @@
+"""dss_amd -- MI355X-native per-PE string-sorting path.
+
+Python host side over the C ABI (include/dss_gpu.h): a device string set with
+p virtual PEs and the stages of the reference's distributed sorters
+(local LCP sort, partition, exchange, LCP merge, prefix decompression,
+prefix-doubling duplicate detection), all executed by gfx950 HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import DssError, check, lib
+
+__all__ = ["DeviceStringSet", "DssError", "device_info"]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def device_info() -> str:
+    buf = C.create_string_buffer(256)
+    check(lib().dss_device_info(buf, 256))
+    return buf.value.decode()
+
+
+class DeviceStringSet:
+    """A device string set of p virtual PEs (a StringLcpContainer per PE).
+
+    Create with from_arena / dton / skewed; stages run on the set's stream."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    # ------------------------------------------------------------ creation
+    @classmethod
+    def from_arena(cls, chars: np.ndarray, num_pes: int) -> "DeviceStringSet":
+        chars = np.ascontiguousarray(chars, dtype=np.uint8)
+        h = _lib.set_p()
+        ptr = _p(chars, _lib.u8p) if chars.size else None
+        check(lib().dss_set_create(ptr, chars.size, num_pes, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def dton(cls, n: int, length: int, dton: float, seed: int, num_pes: int) -> "DeviceStringSet":
+        h = _lib.set_p()
+        check(lib().dss_set_create_dton(n, length, dton, seed, num_pes, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def skewed(cls, n: int, seed: int, num_pes: int, long_len=1000, short_len=100,
+               long_dn=0.75, short_dn=0.25, long_permille=200,
+               dup_permille=100) -> "DeviceStringSet":
+        h = _lib.set_p()
+        check(lib().dss_set_create_skewed(n, long_len, short_len, long_dn, short_dn,
+                                          long_permille, dup_permille, seed, num_pes,
+                                          C.byref(h)))
+        return cls(h)
+
+    def close(self):
+        if self._h:
+            check(lib().dss_set_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------ info
+    def info(self) -> dict:
+        i = _lib.SetInfo()
+        check(lib().dss_set_info(self._h, C.byref(i)))
+        return dict(num_strings=i.num_strings, num_bytes=i.num_bytes, num_pes=i.num_pes,
+                    max_len=i.max_len)
+
+    @property
+    def n(self) -> int:
+        return self.info()["num_strings"]
+
+    @property
+    def p(self) -> int:
+        return self.info()["num_pes"]
+
+    def pe_begin(self) -> np.ndarray:
+        out = np.zeros(self.p + 1, dtype=np.uint64)
+        check(lib().dss_set_pe_begin(self._h, _p(out, _lib.u64p)))
+        return out
+
+    def download(self):
+        info = self.info()
+        chars = np.zeros(max(info["num_bytes"], 1), dtype=np.uint8)
+        off = np.zeros(max(info["num_strings"], 1), dtype=np.uint64)
+        ln = np.zeros(max(info["num_strings"], 1), dtype=np.uint32)
+        check(lib().dss_set_download(self._h, _p(chars, _lib.u8p), _p(off, _lib.u64p),
+                                     _p(ln, _lib.u32p)))
+        return chars[:info["num_bytes"]], off[:info["num_strings"]], ln[:info["num_strings"]]
+
+    # ------------------------------------------------------------ stages
+    def sort_lcp(self):
+        """Local LCP sort of every PE (tlx radixsort_CI3 call sites)."""
+        check(lib().dss_sort_lcp(self._h))
+
+    def get_sorted(self):
+        n = self.n
+        perm = np.zeros(max(n, 1), dtype=np.uint32)
+        lcp = np.zeros(max(n, 1), dtype=np.uint32)
+        check(lib().dss_get_sorted(self._h, _p(perm, _lib.u32p), _p(lcp, _lib.u32p)))
+        return perm[:n], lcp[:n]
+
+    # ------------------------------------------------------------ timing
+    def sync(self):
+        check(lib().dss_sync(self._h))
+
+    def set_timing(self, enable: bool):
+        check(lib().dss_set_timing(self._h, int(enable)))
+
+    def kernel_stats(self, reset: bool = False) -> dict:
+        cap = 64
+        arr = (_lib.KernelStat * cap)()
+        cnt = C.c_uint32()
+        check(lib().dss_kernel_stats(self._h, arr, cap, C.byref(cnt), int(reset)))
+        out = {}
+        for k in range(min(cnt.value, cap)):
+            s = arr[k]
+            out[s.name.decode()] = dict(launches=s.launches, ms=s.total_ms, alg_bytes=s.alg_bytes)
+        return out

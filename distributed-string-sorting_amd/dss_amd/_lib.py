@@ -1,0 +1,84 @@
+"""ctypes binding of libdss_gpu.so (the C ABI declared in include/dss_gpu.h).
+
+The library is built in-tree by distributed-string-sorting_amd/build.py.  There
+is no CPU fallback: importing this module without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdss_gpu.so")
+
+u8p = C.POINTER(C.c_uint8)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+set_p = C.c_void_p
+
+
+class SetInfo(C.Structure):
+    _fields_ = [("num_strings", C.c_uint64), ("num_bytes", C.c_uint64),
+                ("num_pes", C.c_uint32), ("max_len", C.c_uint32)]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64),
+                ("total_ms", C.c_double), ("alg_bytes", C.c_double)]
+
+
+class DssError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dss error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# name -> (restype, argtypes); every function returns int status unless noted
+_SIGS = {
+    "dss_last_error": (C.c_char_p, []),
+    "dss_device_info": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "dss_set_create": (C.c_int, [u8p, C.c_uint64, C.c_uint32, C.POINTER(set_p)]),
+    "dss_set_create_dton": (C.c_int, [C.c_uint64, C.c_uint32, C.c_double, C.c_uint64, C.c_uint32,
+                                      C.POINTER(set_p)]),
+    "dss_set_create_skewed": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                        C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
+                                        C.POINTER(set_p)]),
+    "dss_set_destroy": (C.c_int, [set_p]),
+    "dss_set_info": (C.c_int, [set_p, C.POINTER(SetInfo)]),
+    "dss_set_pe_begin": (C.c_int, [set_p, u64p]),
+    "dss_set_download": (C.c_int, [set_p, u8p, u64p, u32p]),
+    "dss_sort_lcp": (C.c_int, [set_p]),
+    "dss_get_sorted": (C.c_int, [set_p, u32p, u32p]),
+    "dss_sync": (C.c_int, [set_p]),
+    "dss_set_timing": (C.c_int, [set_p, C.c_int]),
+    "dss_kernel_stats": (C.c_int, [set_p, C.POINTER(KernelStat), C.c_uint32, u32p, C.c_int]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is not built; run `python distributed-string-sorting_amd/build.py` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def symbols():
+    return list(_SIGS)
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = lib().dss_last_error()
+        raise DssError(rc, msg.decode() if msg else "")
+    return rc

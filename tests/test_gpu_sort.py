@@ -1,0 +1,92 @@
+"""GPU parity of the local LCP sort (tlx radixsort_CI3 call sites) against the
+oracle's restatement, through the C ABI, on the project's seeded inputs and
+on the edge cases the reference's sorter has to handle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import canonical_ties
+
+pytestmark = pytest.mark.gpu
+
+
+def _dss():
+    import dss_amd
+    return dss_amd
+
+
+def check_sort(chars, p):
+    dss = _dss()
+    ss = O.StringSet(chars, p)
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        assert ds.n == ss.n
+        assert (ds.pe_begin() == ss.pe_begin).all()
+        ds.sort_lcp()
+        gperm, glcp = ds.get_sorted()
+    operm, olcp = O.local_sort(ss)
+    assert (glcp == olcp).all(), "LCP arrays differ"
+    a = canonical_ties(gperm, glcp, ss.len, ss.pe_begin)
+    b = canonical_ties(operm, olcp, ss.len, ss.pe_begin)
+    assert (a == b).all(), "sorted permutations differ"
+    return ss
+
+
+def arena(strings):
+    return np.frombuffer(b"".join(s + b"\0" for s in strings), dtype=np.uint8).copy()
+
+
+@pytest.mark.parametrize("p", [1, 4, 16])
+def test_dton_small(p):
+    check_sort(O.gen_dton(5000, 60, 0.5, 0x5EED0001), p)
+
+
+def test_dton_c1_shape_scaled():
+    # C1 construction (500 chars, D/N 0.5) at 40k strings, 4 PEs
+    check_sort(O.gen_dton(40000, 500, 0.5, 0x5EED0001), 4)
+
+
+@pytest.mark.parametrize("dn", [0.0, 0.25, 0.75, 1.0])
+def test_dton_ratios(dn):
+    check_sort(O.gen_dton(20000, 100, dn, 11), 4)
+
+
+def test_skewed_with_duplicates():
+    params = O.skew_params(30000, long_len=300, short_len=30)
+    check_sort(O.gen_skewed(params, 0x5EED0003), 16)
+
+
+def test_suffixes_high_lcp():
+    check_sort(O.gen_suffixes(30000, 0x5EED0004, repeats=20, rep_min=200, rep_max=2000, cap=500), 4)
+
+
+def test_random_large_single_pe():
+    rng = np.random.default_rng(5)
+    lens = rng.integers(0, 40, 300000)
+    strs = [bytes(rng.integers(1, 4, l, dtype=np.uint8)) for l in lens]
+    check_sort(arena(strs), 1)
+
+
+def test_edge_cases():
+    # empty strings, identical strings above and below the base-case size,
+    # one-string PEs, fewer strings than PEs, prefix chains
+    check_sort(arena([b""] * 10 + [b"a"] * 3 + [b""]), 2)
+    check_sort(arena([b"same"] * 5000), 3)
+    check_sort(arena([b"x" * 900] * 3000 + [b"x" * 899 + b"y"]), 1)
+    check_sort(arena([b"only"]), 1)
+    check_sort(arena([b"b", b"a"]), 8)
+    check_sort(arena([b"a" * k for k in range(300, 0, -1)]), 2)
+    check_sort(arena([bytes([255]) * 3, bytes([1]), bytes([255, 1]), bytes([128]) * 17]), 1)
+
+
+def test_device_generators_match_oracle():
+    dss = _dss()
+    n, L, dn, seed = 7777, 123, 0.5, 99
+    with dss.DeviceStringSet.dton(n, L, dn, seed, 4) as ds:
+        chars, off, ln = ds.download()
+    exp = O.gen_dton(n, L, dn, seed)
+    assert chars.tobytes() == exp.tobytes()
+    params = O.skew_params(5000, long_len=200, short_len=20)
+    with dss.DeviceStringSet.skewed(5000, 42, 4, long_len=200, short_len=20) as ds:
+        chars, off, ln = ds.download()
+    exp = O.gen_skewed(params, 42)
+    assert chars.tobytes() == exp.tobytes()
