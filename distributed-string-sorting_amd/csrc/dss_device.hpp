@@ -78,6 +78,25 @@ __device__ inline uint32_t key_strlen(uint64_t k) {
     return k == 0 ? 0u : 8u - ((uint32_t)__builtin_ctzll(k) >> 3);
 }
 
+// Compares two string views (offset, length) from character `from` on, the
+// way scmp does on NUL-terminated strings (src/strings/stringtools.hpp:108-
+// 115); returns <0, 0, >0 and the LCP (total, from 0) in *lcp_out.
+__device__ inline int view_cmp(const uint8_t* __restrict__ chars, uint64_t ao, uint32_t al,
+                               uint64_t bo, uint32_t bl, uint32_t from, uint32_t* lcp_out) {
+    for (uint32_t pos = from;; pos += 8) {
+        const uint64_t a = load_key8(chars, ao, al, pos);
+        const uint64_t b = load_key8(chars, bo, bl, pos);
+        if (a != b) {
+            if (lcp_out) *lcp_out = pos + key_lcp(a, b);
+            return a < b ? -1 : 1;
+        }
+        if ((a & 0xFF) == 0) {
+            if (lcp_out) *lcp_out = pos + key_strlen(a);
+            return 0;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- wave helpers
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 

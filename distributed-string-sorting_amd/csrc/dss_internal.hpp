@@ -127,6 +127,44 @@ struct SortScratch {
     uint64_t seg_cap = 0, tile_cap = 0;
 };
 
+// ------------------------------------------------------------- pipeline state
+struct PipeState {
+    // prefix doubling (per position)
+    bool has_dist = false;
+    DevBuf dist;     // u32[n] distinguishing-prefix estimates (results)
+    DevBuf rounds;   // u8[n]  rounds a position was a candidate in
+    uint32_t pd_rounds = 0;
+    DevBuf pd_cand, pd_next, pd_hash, pd_flags, pd_table, pd_tflag, pd_ctr;
+    // partition
+    bool partitioned = false;
+    std::vector<uint64_t> interval;  // p*p, [r*p+q] = strings PE r sends to PE q
+    DevBuf interval_d;
+    DevBuf samples, sample_rank, sample_sorted, splitters, wprefix;
+    // exchange (receive side, indexed by receive position)
+    bool exchanged = false;
+    int mode = 0;
+    uint64_t recv_nbytes = 0;
+    DevBuf recv_bytes;
+    DevBuf recv_byte_off, recv_lcp, recv_src_pe, recv_src_pos, recv_view_off, recv_view_len,
+        recv_from, recv_size;
+    std::vector<uint64_t> recv_begin, byte_begin;  // p+1 each
+    DevBuf layout;                                 // send_start / run offsets (u64)
+    // merge (per receive-side output position)
+    bool merged = false;
+    DevBuf m_order, m_lcp, m_saved, m_order2, m_lcp2, m_keyc, m_cp, m_work;
+    // prefix decompression
+    bool extended = false;
+    uint64_t out_nbytes = 0;
+    DevBuf out_bytes, out_off, out_psv, out_aux;
+    // generic scratch
+    DevBuf scan_tmp, scan_tot;
+};
+
+// Device-wide exclusive scan of u64 values (out may alias in); returns the
+// total when total_host != nullptr (synchronises the stream).
+void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t st,
+                        PipeState& P, uint64_t* total_host);
+
 }  // namespace dss
 
 // ------------------------------------------------------------------ the set
@@ -150,6 +188,7 @@ struct dss_set_s {
     dss::DevBuf lcp;    // u32[n]
 
     dss::SortScratch ss;
+    dss::PipeState pp;
     dss::KernelTimer timer;
 };
 
@@ -165,5 +204,11 @@ void compute_pe_split(dss_set_s* s);
 dss_set_s* new_set(uint32_t num_pes);
 
 void sort_lcp(dss_set_s* s);
+uint64_t avg_lcp(dss_set_s* s);
+void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, bool use_dist);
+void exchange(dss_set_s* s, int mode);
+void merge(dss_set_s* s);
+void extend_prefix(dss_set_s* s);
+void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth);
 
 }  // namespace dss

@@ -1,0 +1,399 @@
+// (a10-a12) Prefix doubling: distinguishing-prefix estimates by rounds of
+// prefix hashing and duplicate detection, and the two composite drivers.
+//
+// computeDistinguishingPrefixes (src/sorter/distributed/prefix_doubling.hpp:
+// 156-198) with BloomFilter::filter (src/sorter/distributed/bloomfilter.hpp:
+// 1415-1556).  Round r uses depth 8 (r = 0) or 8 + 16*2^(r-1).  For every
+// candidate position c of every PE, in local sorted order:
+//   depth > len(c)                         -> end of string: results = len;
+//   c-1 is a candidate and lcp[c] >= depth -> local LCP duplicate (stays);
+//   otherwise                              -> hashed: XXHasher::hash of the
+//                                             first depth chars % m.
+// A hashed candidate stays a candidate iff its hash occurs more than once
+// among the hashed candidates of all PEs, or the next position is a local
+// LCP duplicate (the LCP-local root).  results = depth for every candidate of
+// the round.  On one GPU the range-partitioned exchange of hashes
+// (SendOnlyHashesToFilter / FindDuplicates, :581-765) is a global hash table
+// in HBM: an equal hash of any two hashed candidates of any PEs marks both,
+// which is what the merge of the owners' sorted hash runs reports.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "dss_device.hpp"
+#include "dss_internal.hpp"
+
+namespace dss {
+
+// ------------------------------------------------------------------ XXH64
+// xxh::hash_state_t<64> with seed 0 fed by one update (src/hash/xxhash.hpp:
+// 508-640): the published XXH64 algorithm, except that the digest takes the
+// short-input branch (v3 + PRIME5) when exactly 32 bytes were hashed
+// (`total_len > 32`, :573).
+constexpr uint64_t P1 = 11400714785074694791ull;
+constexpr uint64_t P2 = 14029467366897019727ull;
+constexpr uint64_t P3 = 1609587929392839161ull;
+constexpr uint64_t P4 = 9650029242287828579ull;
+constexpr uint64_t P5 = 2870177450012600261ull;
+
+__device__ inline uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+__device__ inline uint64_t xround(uint64_t acc, uint64_t in) {
+    acc += in * P2;
+    acc = rotl64(acc, 31);
+    return acc * P1;
+}
+__device__ inline uint64_t xmerge(uint64_t acc, uint64_t v) {
+    acc ^= xround(0, v);
+    return acc * P1 + P4;
+}
+
+__device__ uint64_t xxh64_stream(const uint8_t* __restrict__ chars, uint64_t off, uint32_t len) {
+    uint64_t v1 = P1 + P2, v2 = P2, v3 = 0, v4 = (uint64_t)0 - P1;
+    uint32_t p = 0;
+    if (len >= 32) {
+        do {
+            v1 = xround(v1, load_word_le(chars, off + p));
+            v2 = xround(v2, load_word_le(chars, off + p + 8));
+            v3 = xround(v3, load_word_le(chars, off + p + 16));
+            v4 = xround(v4, load_word_le(chars, off + p + 24));
+            p += 32;
+        } while (p + 32 <= len);
+    }
+    uint64_t h;
+    if (len > 32) {
+        h = rotl64(v1, 1) + rotl64(v2, 7) + rotl64(v3, 12) + rotl64(v4, 18);
+        h = xmerge(h, v1);
+        h = xmerge(h, v2);
+        h = xmerge(h, v3);
+        h = xmerge(h, v4);
+    } else {
+        h = v3 + P5;
+    }
+    h += len;
+    while (p + 8 <= len) {
+        h ^= xround(0, load_word_le(chars, off + p));
+        h = rotl64(h, 27) * P1 + P4;
+        p += 8;
+    }
+    if (p + 4 <= len) {
+        h ^= (load_word_le(chars, off + p) & 0xFFFFFFFFull) * P1;
+        h = rotl64(h, 23) * P2 + P3;
+        p += 4;
+    }
+    while (p < len) {
+        h ^= (uint64_t)chars[off + p] * P5;
+        h = rotl64(h, 11) * P1;
+        ++p;
+    }
+    h ^= h >> 33;
+    h *= P2;
+    h ^= h >> 29;
+    h *= P3;
+    h ^= h >> 32;
+    return h;
+}
+
+// ------------------------------------------------------------------ rounds
+constexpr uint8_t kHashed = 1, kLcpDup = 2;
+constexpr uint64_t kEmpty = ~0ull;  // never a hash value: hash % m < m <= 2^64-1
+
+__device__ inline uint32_t pe_of(const uint64_t* __restrict__ pe_begin, uint32_t p, uint64_t i) {
+    uint32_t lo = 0, hi = p;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pe_begin[mid] <= i) lo = mid + 1; else hi = mid;
+    }
+    return lo - 1;
+}
+
+__global__ void k_pd_init(uint8_t* __restrict__ cand, uint8_t* __restrict__ rounds, uint32_t* __restrict__ dist,
+                          uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        cand[i] = 1;
+        rounds[i] = 0;
+        dist[i] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_pd_classify(const uint8_t* __restrict__ cand, const uint64_t* __restrict__ pe_begin, uint32_t p,
+              const uint32_t* __restrict__ perm, const uint32_t* __restrict__ lcp, const uint64_t* __restrict__ off,
+              const uint32_t* __restrict__ len, const uint8_t* __restrict__ chars, uint32_t depth, uint64_t m,
+              uint64_t n, uint8_t* __restrict__ flags, uint64_t* __restrict__ hashv, uint8_t* __restrict__ rounds,
+              uint32_t* __restrict__ dist, unsigned long long* __restrict__ nhashed) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t hashed = 0;
+    if (c < n) {
+        uint8_t f = 0;
+        if (cand[c]) {
+            rounds[c] = rounds[c] + (rounds[c] < 255);
+            const uint32_t id = perm[c];
+            const uint32_t L = len[id];
+            if (depth > L) {
+                dist[c] = L;  // end of string
+            } else {
+                dist[c] = depth;
+                const uint32_t r = pe_of(pe_begin, p, c);
+                if (c > pe_begin[r] && cand[c - 1] && lcp[c] >= depth) {
+                    f = kLcpDup;
+                } else {
+                    f = kHashed;
+                    hashed = 1;
+                    hashv[c] = xxh64_stream(chars, off[id], depth) % m;
+                }
+            }
+        }
+        flags[c] = f;
+    }
+    // one atomic per wave
+    const uint64_t b = __ballot(hashed);
+    if (lane_id() == 0 && b) atomicAdd(nhashed, (unsigned long long)__popcll(b));
+}
+
+__global__ void k_fill_u64(uint64_t* __restrict__ a, uint64_t n, uint64_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__device__ inline uint64_t slot_of(uint64_t h, uint64_t mask) {
+    h ^= h >> 31;
+    h *= 0x9E3779B97F4A7C15ull;
+    return (h >> 17) & mask;
+}
+
+__global__ void k_pd_insert(const uint8_t* __restrict__ flags, const uint64_t* __restrict__ hashv, uint64_t n,
+                            uint64_t* __restrict__ keys, uint8_t* __restrict__ dup, uint64_t mask) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || flags[c] != kHashed) return;
+    const uint64_t h = hashv[c];
+    uint64_t s = slot_of(h, mask);
+    for (uint64_t probe = 0; probe <= mask; ++probe) {
+        const unsigned long long old =
+            atomicCAS(reinterpret_cast<unsigned long long*>(&keys[s]), (unsigned long long)kEmpty, (unsigned long long)h);
+        if (old == kEmpty) return;
+        if (old == h) {
+            dup[s] = 1;
+            return;
+        }
+        s = (s + 1) & mask;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_pd_next(const uint8_t* __restrict__ cand, const uint8_t* __restrict__ flags, const uint64_t* __restrict__ hashv,
+          const uint64_t* __restrict__ pe_begin, uint32_t p, const uint32_t* __restrict__ lcp, uint32_t depth, uint64_t n,
+          const uint64_t* __restrict__ keys, const uint8_t* __restrict__ dup, uint64_t mask, uint8_t* __restrict__ next,
+          unsigned long long* __restrict__ ncand) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t keep = 0;
+    if (c < n) {
+        const uint8_t f = flags[c];
+        if (f == kLcpDup) {
+            keep = 1;
+        } else if (f == kHashed) {
+            const uint32_t r = pe_of(pe_begin, p, c);
+            if (c + 1 < pe_begin[r + 1] && cand[c + 1] && lcp[c + 1] >= depth) keep = 1;  // LCP-local root
+            if (!keep) {
+                const uint64_t h = hashv[c];
+                uint64_t s = slot_of(h, mask);
+                for (uint64_t probe = 0; probe <= mask; ++probe) {
+                    const uint64_t k = keys[s];
+                    if (k == h) {
+                        keep = dup[s];
+                        break;
+                    }
+                    if (k == kEmpty) break;
+                    s = (s + 1) & mask;
+                }
+            }
+        }
+        next[c] = (uint8_t)keep;
+    }
+    const uint64_t b = __ballot(keep);
+    if (lane_id() == 0 && b) atomicAdd(ncand, (unsigned long long)__popcll(b));
+}
+
+void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
+    DSS_REQUIRE(s->sorted, DSS_ESTATE, "dss_sort_lcp has not run");
+    DSS_REQUIRE(m >= 1, DSS_EINVAL, "hash range m must be >= 1");
+    DSS_REQUIRE(start_depth >= 1, DSS_EINVAL, "start depth must be >= 1");
+    PipeState& P = s->pp;
+    const hipStream_t st = s->stream;
+    const uint64_t n = s->n;
+    const uint32_t p = s->p;
+    const uint64_t nn = std::max<uint64_t>(n, 1);
+    P.dist.reserve(nn * 4);
+    P.rounds.reserve(nn);
+    P.pd_cand.reserve(nn);
+    P.pd_next.reserve(nn);
+    P.pd_flags.reserve(nn);
+    P.pd_hash.reserve(nn * 8);
+    P.pd_ctr.reserve(64);
+    uint8_t* cand = P.pd_cand.as<uint8_t>();
+    uint8_t* next = P.pd_next.as<uint8_t>();
+    unsigned long long* ctr = P.pd_ctr.as<unsigned long long>();
+    const dim3 grid((n + 255) / 256), blk(256);
+    if (n) hipLaunchKernelGGL(k_pd_init, grid, blk, 0, st, cand, P.rounds.as<uint8_t>(), P.dist.as<uint32_t>(), n);
+    uint64_t ncand = n;
+    uint32_t round = 0;
+    uint64_t depth = start_depth, step = (uint64_t)start_depth * 2;
+    while (ncand > 0) {
+        DSS_REQUIRE(depth < 0xFFFFFFFFull, DSS_ERANGE, "prefix doubling depth overflow");
+        DSS_HIP(hipMemsetAsync(ctr, 0, 16, st));
+        {
+            KScope ks(s->timer, st, "pd_hash", 0.0);
+            hipLaunchKernelGGL(k_pd_classify, grid, blk, 0, st, cand, s->pe_begin_d.as<uint64_t>(), p,
+                               s->perm.as<uint32_t>(), s->lcp.as<uint32_t>(), s->off.as<uint64_t>(),
+                               s->len.as<uint32_t>(), s->chars.as<uint8_t>(), (uint32_t)depth, m, n,
+                               P.pd_flags.as<uint8_t>(), P.pd_hash.as<uint64_t>(), P.rounds.as<uint8_t>(),
+                               P.dist.as<uint32_t>(), ctr);
+        }
+        uint64_t nh = 0;
+        DSS_HIP(hipMemcpyAsync(&nh, ctr, 8, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+        uint64_t T = 1024;
+        while (T < 2 * nh) T <<= 1;
+        P.pd_table.reserve(T * 8);
+        P.pd_tflag.reserve(T);
+        {
+            KScope ks(s->timer, st, "pd_dedup", 0.0);
+            hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T, kEmpty);
+            DSS_HIP(hipMemsetAsync(P.pd_tflag.p, 0, T, st));
+            if (nh)
+                hipLaunchKernelGGL(k_pd_insert, grid, blk, 0, st, P.pd_flags.as<uint8_t>(), P.pd_hash.as<uint64_t>(), n,
+                                   P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(), T - 1);
+            hipLaunchKernelGGL(k_pd_next, grid, blk, 0, st, cand, P.pd_flags.as<uint8_t>(), P.pd_hash.as<uint64_t>(),
+                               s->pe_begin_d.as<uint64_t>(), p, s->lcp.as<uint32_t>(), (uint32_t)depth, n,
+                               P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(), T - 1, next, ctr + 1);
+        }
+        DSS_HIP(hipGetLastError());
+        DSS_HIP(hipMemcpyAsync(&ncand, ctr + 1, 8, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+        std::swap(cand, next);
+        ++round;
+        depth = start_depth + step;
+        step *= 2;
+    }
+    P.pd_rounds = round;
+    P.has_dist = true;
+    P.partitioned = P.exchanged = P.merged = P.extended = false;
+}
+
+// ------------------------------------------------------------------ drivers
+__global__ void k_permutation(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ rpos,
+                              const uint32_t* __restrict__ rpe, uint64_t n, uint32_t* __restrict__ si,
+                              uint32_t* __restrict__ pi) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        const uint32_t o = ord[k];
+        si[k] = rpos[o];
+        pi[k] = rpe[o];
+    }
+}
+
+// Hash of the first min(depth, len) characters of the given strings.
+__global__ void k_hash_ids(const uint32_t* __restrict__ ids, uint64_t count, const uint64_t* __restrict__ off,
+                           const uint32_t* __restrict__ len, const uint8_t* __restrict__ chars, uint64_t n,
+                           uint32_t depth, uint64_t m, uint64_t* __restrict__ out) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t id = ids[k];
+    if (id >= n) {
+        out[k] = 0;
+        return;
+    }
+    const uint32_t L = min(depth, len[id]);
+    out[k] = xxh64_stream(chars, off[id], L) % m;
+}
+
+}  // namespace dss
+
+using namespace dss;
+
+extern "C" {
+
+int dss_prefix_doubling(dss_set s, uint64_t m, uint32_t start_depth, uint32_t* num_rounds) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        prefix_doubling(s, m, start_depth);
+        if (num_rounds) *num_rounds = s->pp.pd_rounds;
+    });
+}
+
+int dss_get_prefix_doubling(dss_set s, uint32_t* results, uint8_t* rounds) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        DSS_REQUIRE(s->pp.has_dist, DSS_ESTATE, "dss_prefix_doubling has not run");
+        if (s->n) {
+            if (results) DSS_HIP(hipMemcpyAsync(results, s->pp.dist.p, s->n * 4, hipMemcpyDeviceToHost, s->stream));
+            if (rounds) DSS_HIP(hipMemcpyAsync(rounds, s->pp.rounds.p, s->n, hipMemcpyDeviceToHost, s->stream));
+        }
+        DSS_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int dss_hash_prefixes(dss_set s, const uint32_t* ids, uint64_t count, uint32_t depth, uint64_t m, uint64_t* out) {
+    return guarded([&] {
+        DSS_REQUIRE(s && (count == 0 || (ids && out)), DSS_EINVAL, "NULL argument");
+        DSS_REQUIRE(m >= 1, DSS_EINVAL, "hash range m must be >= 1");
+        if (!count) return;
+        DevBuf buf;
+        buf.reserve(count * 12);
+        uint32_t* d_ids = reinterpret_cast<uint32_t*>(buf.as<uint8_t>() + count * 8);
+        DSS_HIP(hipMemcpyAsync(d_ids, ids, count * 4, hipMemcpyHostToDevice, s->stream));
+        hipLaunchKernelGGL(k_hash_ids, dim3((count + 255) / 256), dim3(256), 0, s->stream, d_ids, count,
+                           s->off.as<uint64_t>(), s->len.as<uint32_t>(), s->chars.as<uint8_t>(), s->n, depth, m,
+                           buf.as<uint64_t>());
+        DSS_HIP(hipGetLastError());
+        DSS_HIP(hipMemcpyAsync(out, buf.p, count * 8, hipMemcpyDeviceToHost, s->stream));
+        DSS_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int dss_merge_sort(dss_set s, int sampler, int compressed) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        sort_lcp(s);
+        if (s->p == 1) return;
+        const uint64_t avg = avg_lcp(s) + 5;  // lcpSummand, lcp_aware...:207,236
+        partition(s, sampler, 2, 2 * 100 * avg, false);
+        exchange(s, compressed ? 1 : 0);
+        merge(s);
+        if (compressed) extend_prefix(s);
+    });
+}
+
+int dss_prefix_doubling_sort(dss_set s, int sampler, uint64_t m) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        sort_lcp(s);
+        if (s->p == 1) return;
+        prefix_doubling(s, m, 8);
+        partition(s, sampler, 2, 0, true);
+        exchange(s, 2);
+        merge(s);
+    });
+}
+
+int dss_get_permutation(dss_set s, uint32_t* string_index, uint32_t* pe_index) {
+    return guarded([&] {
+        DSS_REQUIRE(s && string_index && pe_index, DSS_EINVAL, "NULL argument");
+        PipeState& P = s->pp;
+        DSS_REQUIRE(P.merged, DSS_ESTATE, "no merged output");
+        const uint64_t n = s->n;
+        if (!n) return;
+        DevBuf tmp;
+        tmp.reserve(n * 8);
+        uint32_t* si = tmp.as<uint32_t>();
+        hipLaunchKernelGGL(k_permutation, dim3((n + 255) / 256), dim3(256), 0, s->stream, P.m_order.as<uint32_t>(),
+                           P.recv_src_pos.as<uint32_t>(), P.recv_src_pe.as<uint32_t>(), n, si, si + n);
+        DSS_HIP(hipMemcpyAsync(string_index, si, n * 4, hipMemcpyDeviceToHost, s->stream));
+        DSS_HIP(hipMemcpyAsync(pe_index, si + n, n * 4, hipMemcpyDeviceToHost, s->stream));
+        DSS_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,1032 @@
+// The distributed stages after the local sort, for p virtual PEs held on one
+// GPU: global average LCP (a3), splitter sampling and bucket computation
+// (a4, a5), the string exchange as an in-HBM bucket scatter (a6, a7, a13),
+// the LCP merge of the received runs (a8) and prefix decompression (a9).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "dss_device.hpp"
+#include "dss_internal.hpp"
+
+namespace dss {
+
+// ================================================================== scan
+constexpr uint32_t kST = 256, kSI = 8, kSB = kST * kSI;
+
+__device__ inline uint64_t block_excl_sum_u64(uint64_t v, uint64_t* tmp, uint64_t* total) {
+    const uint32_t t = threadIdx.x;
+    tmp[t] = v;
+    __syncthreads();
+    for (uint32_t o = 1; o < kST; o <<= 1) {
+        const uint64_t x = t >= o ? tmp[t - o] : 0;
+        __syncthreads();
+        tmp[t] += x;
+        __syncthreads();
+    }
+    const uint64_t incl = tmp[t];
+    if (total) *total = tmp[kST - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(kST) k_scan_reduce(const uint64_t* __restrict__ in, uint64_t n,
+                                                     uint64_t* __restrict__ sums) {
+    __shared__ uint64_t tmp[kST];
+    const uint64_t b = (uint64_t)blockIdx.x * kSB;
+    uint64_t s = 0;
+    for (uint32_t k = 0; k < kSI; ++k) {
+        const uint64_t i = b + (uint64_t)k * kST + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint64_t tot;
+    block_excl_sum_u64(s, tmp, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_sums(uint64_t* __restrict__ v, uint64_t n,
+                                                    uint64_t* __restrict__ total) {
+    __shared__ uint64_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint64_t per = (n + 1023) / 1024;
+    const uint64_t b = t * per, e = (b + per < n) ? b + per : n;
+    uint64_t s = 0;
+    for (uint64_t i = b; i < e; ++i) s += v[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        const uint64_t x = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint64_t run = t ? part[t - 1] : 0;
+    for (uint64_t i = b; i < e; ++i) {
+        const uint64_t x = v[i];
+        v[i] = run;
+        run += x;
+    }
+    if (t == 1023) *total = part[1023];
+}
+
+__global__ void __launch_bounds__(kST) k_scan_apply(const uint64_t* __restrict__ in, uint64_t* out,
+                                                    uint64_t n, const uint64_t* __restrict__ sums) {
+    __shared__ uint64_t tmp[kST];
+    const uint64_t b = (uint64_t)blockIdx.x * kSB + (uint64_t)threadIdx.x * kSI;
+    uint64_t v[kSI];
+    uint64_t s = 0;
+    for (uint32_t k = 0; k < kSI; ++k) {
+        v[k] = (b + k < n) ? in[b + k] : 0;
+        s += v[k];
+    }
+    uint64_t run = sums[blockIdx.x] + block_excl_sum_u64(s, tmp, nullptr);
+    for (uint32_t k = 0; k < kSI; ++k) {
+        if (b + k < n) out[b + k] = run;
+        run += v[k];
+    }
+}
+
+void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t st, PipeState& P,
+                        uint64_t* total_host) {
+    const uint64_t nb = std::max<uint64_t>((n + kSB - 1) / kSB, 1);
+    P.scan_tmp.reserve((nb + 1) * 8);
+    P.scan_tot.reserve(8);
+    uint64_t* sums = P.scan_tmp.as<uint64_t>();
+    DSS_HIP(hipMemsetAsync(sums, 0, (nb + 1) * 8, st));
+    if (n) hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kST), 0, st, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, nb, P.scan_tot.as<uint64_t>());
+    if (n) hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kST), 0, st, in, out, n, sums);
+    DSS_HIP(hipGetLastError());
+    if (total_host) {
+        DSS_HIP(hipMemcpyAsync(total_host, P.scan_tot.p, 8, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+    }
+}
+
+// ================================================================== (a3)
+// getAvgLcp (src/sorter/distributed/misc.hpp:67-87): sum of all local LCP
+// entries over all PEs divided by the number of strings.
+__global__ void k_sum_u32(const uint32_t* __restrict__ v, uint64_t n, unsigned long long* __restrict__ out) {
+    uint64_t s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        s += v[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down((unsigned long long)s, o);
+    if (lane_id() == 0) atomicAdd(out, (unsigned long long)s);
+}
+
+uint64_t avg_lcp(dss_set_s* s) {
+    DSS_REQUIRE(s->sorted, DSS_ESTATE, "dss_sort_lcp has not run");
+    if (s->n == 0) return 0;
+    PipeState& P = s->pp;
+    P.scan_tot.reserve(8);
+    DSS_HIP(hipMemsetAsync(P.scan_tot.p, 0, 8, s->stream));
+    hipLaunchKernelGGL(k_sum_u32, dim3(std::min<uint64_t>((s->n + 255) / 256, 2048)), dim3(256), 0, s->stream,
+                       s->lcp.as<uint32_t>(), s->n, P.scan_tot.as<unsigned long long>());
+    uint64_t tot = 0;
+    DSS_HIP(hipMemcpyAsync(&tot, P.scan_tot.p, 8, hipMemcpyDeviceToHost, s->stream));
+    DSS_HIP(hipStreamSynchronize(s->stream));
+    return tot / s->n;
+}
+
+// ================================================================== (a4, a5)
+// Sample = a (possibly truncated) string of a PE plus its global index.
+struct Sample {
+    uint64_t off;   // arena offset of the string
+    uint32_t tlen;  // truncated length
+    uint32_t pad;
+    uint64_t gidx;  // localOffset + index in the PE's sorted order
+};
+
+// One thread per PE: the sampling policies of samplingStrategies.hpp.
+//  NumStrings / IndexedNumStrings (:20-90, :107-187): sample i at
+//    (size_t)(i * (double)n / (double)(nr + 1));
+//  NumChars / IndexedNumChars (:189-389): walk lengths (or distinguishing
+//    prefixes) until splitter_dist = num / (nr + 1) characters were seen;
+//    done by binary searches on the prefix sums w_prefix.
+// nr = min(factor * (p - 1), n) (getNumberSplitter, :14-18).  A sample is
+// truncated to max_len (merge sort) or to dist[idx] (prefix doubling).
+__global__ void k_sample(const uint64_t* __restrict__ pe_begin, uint32_t p, const uint32_t* __restrict__ perm,
+                         const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                         const uint32_t* __restrict__ dist, const uint64_t* __restrict__ wprefix, int by_chars,
+                         uint64_t factor, uint64_t max_len, uint32_t nr_max, Sample* __restrict__ samples,
+                         uint32_t* __restrict__ counts) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= p) return;
+    const uint64_t b = pe_begin[r], m = pe_begin[r + 1] - b;
+    uint64_t nr = factor * (uint64_t)(p - 1);
+    if (nr > m) nr = m;
+    uint32_t cnt = 0;
+    auto emit = [&](uint64_t idx) {
+        const uint32_t id = perm[b + idx];
+        const uint64_t ml = dist ? (uint64_t)dist[b + idx] : max_len;
+        const uint32_t sl = len[id];
+        Sample S;
+        S.off = off[id];
+        S.tlen = (uint32_t)(sl > ml ? ml : sl);
+        S.pad = 0;
+        S.gidx = b + idx;
+        samples[(uint64_t)r * nr_max + cnt++] = S;
+    };
+    if (!by_chars) {
+        const double sd = (double)m / (double)(nr + 1);
+        for (uint64_t i = 1; i <= nr; ++i) emit((uint64_t)((double)i * sd));
+    } else {
+        const uint64_t base = wprefix[b];
+        const uint64_t num = wprefix[b + m] - base;
+        const uint64_t sd = num / (nr + 1);
+        uint64_t si = 0;
+        for (uint64_t i = 1; i <= nr && si < m; ++i) {
+            if (sd == 0) {
+                emit(si ? si - 1 : 0);
+                continue;
+            }
+            // smallest t >= si with W(si..t) >= sd, else m - 1
+            const uint64_t start = wprefix[b + si];
+            uint64_t lo = si, hi = m;  // first t with wprefix[b+t+1]-start >= sd
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) / 2;
+                if (wprefix[b + mid + 1] - start >= sd) hi = mid; else lo = mid + 1;
+            }
+            if (lo >= m) {
+                si = m;
+                emit(m - 1);
+            } else {
+                si = lo + 1;
+                emit(lo);
+            }
+        }
+    }
+    counts[r] = cnt;
+}
+
+// Compacts the per-PE sample slots (single block).
+__global__ void k_sample_compact(const Sample* __restrict__ slots, const uint32_t* __restrict__ counts, uint32_t p,
+                                 uint32_t nr_max, Sample* __restrict__ out, uint32_t* __restrict__ total) {
+    if (threadIdx.x != 0) return;
+    uint32_t k = 0;
+    for (uint32_t r = 0; r < p; ++r)
+        for (uint32_t i = 0; i < counts[r]; ++i) out[k++] = slots[(uint64_t)r * nr_max + i];
+    *total = k;
+}
+
+// Rank of each sample in the global sample order (the result of the sample
+// sort RQuick::sort, misc.hpp:503-514): by string, then (indexed samplers,
+// IndexStringComparator misc.hpp:35-53) by global index, else by sample id.
+__global__ void __launch_bounds__(256) k_sample_rank(const Sample* __restrict__ smp, uint32_t ns, int indexed,
+                                                     const uint8_t* __restrict__ chars, uint32_t* __restrict__ sorted) {
+    __shared__ uint32_t red[4];
+    const uint32_t i = blockIdx.x;
+    const Sample a = smp[i];
+    uint32_t c = 0;
+    for (uint32_t j = threadIdx.x; j < ns; j += blockDim.x) {
+        if (j == i) continue;
+        const Sample b = smp[j];
+        const int r = view_cmp(chars, b.off, b.tlen, a.off, a.tlen, 0, nullptr);
+        if (r < 0) ++c;
+        else if (r == 0 && (indexed ? (b.gidx < a.gidx) : (j < i))) ++c;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    if (lane_id() == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) sorted[red[0] + red[1] + red[2] + red[3]] = i;
+}
+
+// Per (PE r, splitter q): binarySearch / binarySearchIndexed
+// (misc.hpp:310-333, :347-372) and bucket sizes (compute_interval_binary(_
+// index), :376-414).  One block per PE, p-1 threads.
+__global__ void k_bucket_search(const uint64_t* __restrict__ pe_begin, uint32_t p, const uint32_t* __restrict__ perm,
+                                const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                                const uint8_t* __restrict__ chars, const Sample* __restrict__ smp,
+                                const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ ns_p, int indexed,
+                                uint64_t* __restrict__ interval) {
+    extern __shared__ uint64_t pos[];
+    const uint32_t r = blockIdx.x;
+    const uint64_t b = pe_begin[r], m = pe_begin[r + 1] - b;
+    const uint32_t ns = *ns_p;
+    const uint32_t nspl = (p - 1) < ns ? (p - 1) : ns;
+    const uint32_t sdist = ns / (nspl + 1);
+    for (uint32_t q = threadIdx.x; q + 1 < p; q += blockDim.x) {
+        uint64_t res;
+        if (q >= nspl) {
+            res = m;
+        } else {
+            const Sample sp = smp[sorted[(q + 1) * sdist]];
+            uint64_t left = 0, right = m;
+            res = ~0ull;
+            while (left != right) {
+                const uint64_t d = (right - left) / 2;
+                const uint32_t id = perm[b + left + d];
+                int c = view_cmp(chars, off[id], len[id], sp.off, sp.tlen, 0, nullptr);
+                if (c == 0 && indexed) {
+                    const uint64_t gi = b + left + d;
+                    c = gi < sp.gidx ? -1 : (gi > sp.gidx ? 1 : 0);
+                }
+                if (c < 0) left = left + d + 1;
+                else if (c == 0) { res = left + d; break; }
+                else right = left + d;
+            }
+            if (res == ~0ull) res = left;
+        }
+        pos[q] = res;
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < p; q += blockDim.x) {
+        const uint64_t hi = (q + 1 < p) ? pos[q] : m;
+        const uint64_t lo = q ? pos[q - 1] : 0;
+        interval[(uint64_t)r * p + q] = hi - lo;
+    }
+}
+
+__global__ void k_weights(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ len,
+                          const uint32_t* __restrict__ dist, uint64_t n, uint64_t* __restrict__ w) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = dist ? dist[i] : len[perm[i]];
+}
+
+void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, bool use_dist) {
+    DSS_REQUIRE(s->sorted, DSS_ESTATE, "dss_sort_lcp has not run");
+    DSS_REQUIRE(sampler >= 0 && sampler <= 3, DSS_EINVAL, "sampler must be 0..3");
+    DSS_REQUIRE(!use_dist || s->pp.has_dist, DSS_ESTATE, "prefix doubling has not run");
+    DSS_REQUIRE(factor >= 1 && factor * (s->p - 1) <= (1u << 20), DSS_ERANGE, "sampling factor too large");
+    DSS_REQUIRE(s->p <= 4096, DSS_ERANGE, "partition supports at most 4096 PEs");
+    PipeState& P = s->pp;
+    const hipStream_t st = s->stream;
+    const uint32_t p = s->p;
+    const int indexed = sampler >= 2;
+    const int by_chars = sampler == 1 || sampler == 3;
+    const uint32_t nr_max = std::max<uint32_t>((uint32_t)(factor * (p - 1)), 1);
+    const uint64_t nsmax = (uint64_t)nr_max * p;
+    P.samples.reserve(nsmax * 2 * sizeof(Sample) + 64);
+    P.sample_sorted.reserve(nsmax * 4 + 64);
+    P.sample_rank.reserve((p + 2) * 4);
+    P.interval_d.reserve((uint64_t)p * p * 8);
+    Sample* slots = P.samples.as<Sample>();
+    Sample* compact = slots + nsmax;
+    uint32_t* counts = P.sample_rank.as<uint32_t>();
+    uint32_t* ns_d = counts + p;
+    const uint32_t* dist = use_dist ? P.dist.as<uint32_t>() : nullptr;
+    uint64_t* wprefix = nullptr;
+    if (by_chars && s->n) {
+        P.wprefix.reserve((s->n + 1) * 8);
+        wprefix = P.wprefix.as<uint64_t>();
+        hipLaunchKernelGGL(k_weights, dim3((s->n + 255) / 256), dim3(256), 0, st, s->perm.as<uint32_t>(),
+                           s->len.as<uint32_t>(), dist, s->n, wprefix);
+        uint64_t tot = 0;
+        exclusive_scan_u64(wprefix, wprefix, s->n, st, P, &tot);
+        DSS_HIP(hipMemcpyAsync(wprefix + s->n, &tot, 8, hipMemcpyHostToDevice, st));
+        DSS_HIP(hipStreamSynchronize(st));
+    }
+    {
+        KScope ks(s->timer, st, "partition", 0.0);
+        if (p > 1 && s->n) {
+            hipLaunchKernelGGL(k_sample, dim3((p + 63) / 64), dim3(64), 0, st, s->pe_begin_d.as<uint64_t>(), p,
+                               s->perm.as<uint32_t>(), s->off.as<uint64_t>(), s->len.as<uint32_t>(), dist, wprefix,
+                               by_chars, factor, max_len, nr_max, slots, counts);
+            hipLaunchKernelGGL(k_sample_compact, dim3(1), dim3(64), 0, st, slots, counts, p, nr_max, compact, ns_d);
+            uint32_t ns = 0;
+            DSS_HIP(hipMemcpyAsync(&ns, ns_d, 4, hipMemcpyDeviceToHost, st));
+            DSS_HIP(hipStreamSynchronize(st));
+            if (ns)
+                hipLaunchKernelGGL(k_sample_rank, dim3(ns), dim3(256), 0, st, compact, ns, indexed,
+                                   s->chars.as<uint8_t>(), P.sample_sorted.as<uint32_t>());
+            hipLaunchKernelGGL(k_bucket_search, dim3(p), dim3(std::min<uint32_t>(std::max<uint32_t>(p, 64), 256)),
+                               (size_t)p * 8, st, s->pe_begin_d.as<uint64_t>(), p, s->perm.as<uint32_t>(),
+                               s->off.as<uint64_t>(), s->len.as<uint32_t>(), s->chars.as<uint8_t>(), compact,
+                               P.sample_sorted.as<uint32_t>(), ns_d, indexed, P.interval_d.as<uint64_t>());
+        } else {
+            // one PE (or no strings): everything stays on its PE
+            std::vector<uint64_t> iv((uint64_t)p * p, 0);
+            for (uint32_t r = 0; r < p; ++r) iv[(uint64_t)r * p + r] = s->pe_begin[r + 1] - s->pe_begin[r];
+            DSS_HIP(hipMemcpyAsync(P.interval_d.p, iv.data(), iv.size() * 8, hipMemcpyHostToDevice, st));
+        }
+    }
+    DSS_HIP(hipGetLastError());
+    P.interval.assign((uint64_t)p * p, 0);
+    DSS_HIP(hipMemcpyAsync(P.interval.data(), P.interval_d.p, (uint64_t)p * p * 8, hipMemcpyDeviceToHost, st));
+    DSS_HIP(hipStreamSynchronize(st));
+    P.partitioned = true;
+    P.exchanged = P.merged = P.extended = false;
+}
+
+// ================================================================== (a6, a13)
+// Layout (host side, p^2 values): send_start[r*p+q] = first position of
+// bucket q inside PE r; run_off[q*p+r] = receive position of run r of PE q.
+__device__ inline uint32_t bucket_of(const uint64_t* __restrict__ send_start, uint32_t p, uint32_t r, uint64_t j) {
+    const uint64_t* ss = send_start + (uint64_t)r * p;
+    uint32_t lo = 0, hi = p;  // first q with ss[q] > j
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (ss[mid] <= j) lo = mid + 1; else hi = mid;
+    }
+    return lo - 1;
+}
+
+// One thread per source position: destination, size and receive metadata.
+// Encoders (src/mpi/byte_encoder.hpp): mode 0 whole string incl. NUL
+// (:81-107), mode 1 chars [lcp, len] incl. NUL (:109-137), mode 2 chars
+// [lcp, dist) then NUL (:139-172).  The LCP at every bucket start is zero
+// (setLcpAtStartOfInterval, src/mpi/alltoall.hpp:396-404).
+__global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, const uint32_t* __restrict__ perm,
+                             const uint32_t* __restrict__ lcp, const uint64_t* __restrict__ off,
+                             const uint32_t* __restrict__ len, const uint32_t* __restrict__ dist,
+                             const uint64_t* __restrict__ send_start, const uint64_t* __restrict__ run_off, int mode,
+                             uint64_t n, uint64_t* __restrict__ size, uint32_t* __restrict__ r_lcp,
+                             uint32_t* __restrict__ r_pe, uint32_t* __restrict__ r_pos, uint64_t* __restrict__ r_voff,
+                             uint32_t* __restrict__ r_vlen, uint32_t* __restrict__ r_from) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lo = 0, hi = p;  // PE of position i: last r with pe_begin[r] <= i (non-empty)
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pe_begin[mid] <= i) lo = mid + 1; else hi = mid;
+    }
+    const uint32_t r = lo - 1;
+    const uint64_t j = i - pe_begin[r];
+    const uint32_t q = bucket_of(send_start, p, r, j);
+    const uint64_t st = send_start[(uint64_t)r * p + q];
+    const uint64_t dst = run_off[(uint64_t)q * p + r] + (j - st);
+    const uint32_t l = (j == st) ? 0u : lcp[i];
+    const uint32_t id = perm[i];
+    const uint32_t L = len[id];
+    uint64_t sz;
+    uint32_t from = 0, vlen = L;
+    if (mode == 0) {
+        sz = (uint64_t)L + 1;
+    } else if (mode == 1) {
+        from = l;
+        sz = (uint64_t)L + 1 - l;
+    } else {
+        from = l;
+        vlen = dist[i];
+        sz = (uint64_t)vlen - l + 1;
+    }
+    size[dst] = sz;
+    r_lcp[dst] = l;
+    r_pe[dst] = r;
+    r_pos[dst] = (uint32_t)j;
+    r_voff[dst] = off[id];
+    r_vlen[dst] = vlen;
+    r_from[dst] = from;
+}
+
+// 16 lanes per received string copy its bytes into the receive arena.
+__global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* __restrict__ r_voff,
+                            const uint32_t* __restrict__ r_vlen, const uint32_t* __restrict__ r_from,
+                            const uint64_t* __restrict__ byte_off, int mode, uint64_t n, uint8_t* __restrict__ out) {
+    const uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const uint32_t l = threadIdx.x & 15;
+    if (k >= n) return;
+    const uint8_t* src = chars + r_voff[k];
+    const uint32_t from = r_from[k];
+    const uint32_t vlen = r_vlen[k];
+    uint8_t* dst = out + byte_off[k];
+    // modes 0/1: chars [from, vlen] incl. the NUL; mode 2: [from, vlen) + NUL
+    const uint32_t m = vlen - from;
+    for (uint32_t t = l; t <= m; t += 16) dst[t] = (t == m) ? 0 : src[from + t];
+}
+
+void exchange(dss_set_s* s, int mode) {
+    PipeState& P = s->pp;
+    DSS_REQUIRE(P.partitioned, DSS_ESTATE, "dss_partition has not run");
+    DSS_REQUIRE(mode >= 0 && mode <= 2, DSS_EINVAL, "mode must be 0..2");
+    DSS_REQUIRE(mode != 2 || P.has_dist, DSS_ESTATE, "prefix doubling has not run");
+    const hipStream_t st = s->stream;
+    const uint32_t p = s->p;
+    const uint64_t n = s->n;
+    // layout on the host (p^2 numbers, like the reference's alltoall of counts)
+    std::vector<uint64_t> lay((uint64_t)2 * p * p, 0);
+    uint64_t* send_start = lay.data();
+    uint64_t* run_off = lay.data() + (uint64_t)p * p;
+    P.recv_begin.assign(p + 1, 0);
+    for (uint32_t r = 0; r < p; ++r) {
+        uint64_t acc = 0;
+        for (uint32_t q = 0; q < p; ++q) {
+            send_start[(uint64_t)r * p + q] = acc;
+            acc += P.interval[(uint64_t)r * p + q];
+        }
+    }
+    uint64_t acc = 0;
+    for (uint32_t q = 0; q < p; ++q) {
+        P.recv_begin[q] = acc;
+        for (uint32_t r = 0; r < p; ++r) {
+            run_off[(uint64_t)q * p + r] = acc;
+            acc += P.interval[(uint64_t)r * p + q];
+        }
+    }
+    P.recv_begin[p] = acc;
+    DSS_REQUIRE(acc == n, DSS_EINVAL, "interval sizes do not cover all strings");
+    P.layout.reserve(lay.size() * 8);
+    DSS_HIP(hipMemcpyAsync(P.layout.p, lay.data(), lay.size() * 8, hipMemcpyHostToDevice, st));
+    const uint64_t nn = std::max<uint64_t>(n, 1);
+    P.recv_size.reserve(nn * 8);
+    P.recv_byte_off.reserve((nn + 1) * 8);
+    P.recv_lcp.reserve(nn * 4);
+    P.recv_src_pe.reserve(nn * 4);
+    P.recv_src_pos.reserve(nn * 4);
+    P.recv_view_off.reserve(nn * 8);
+    P.recv_view_len.reserve(nn * 4);
+    P.recv_from.reserve(nn * 4);
+    const uint64_t* ls = P.layout.as<uint64_t>();
+    if (n) {
+        KScope ks(s->timer, st, "exchange_index", n * 40.0);
+        hipLaunchKernelGGL(k_exch_index, dim3((n + 255) / 256), dim3(256), 0, st, s->pe_begin_d.as<uint64_t>(), p,
+                           s->perm.as<uint32_t>(), s->lcp.as<uint32_t>(), s->off.as<uint64_t>(),
+                           s->len.as<uint32_t>(), mode == 2 ? P.dist.as<uint32_t>() : nullptr, ls,
+                           ls + (uint64_t)p * p, mode, n, P.recv_size.as<uint64_t>(), P.recv_lcp.as<uint32_t>(),
+                           P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>(),
+                           P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                           P.recv_from.as<uint32_t>());
+    }
+    uint64_t total = 0;
+    exclusive_scan_u64(P.recv_size.as<uint64_t>(), P.recv_byte_off.as<uint64_t>(), n, st, P, &total);
+    P.recv_nbytes = total;
+    P.recv_bytes.reserve(total + kArenaPad);
+    DSS_HIP(hipMemsetAsync(P.recv_bytes.as<uint8_t>() + total, 0, kArenaPad, st));
+    if (n) {
+        KScope ks(s->timer, st, "exchange_copy", 2.0 * total + n * 24.0);
+        const uint64_t threads = n * 16;
+        hipLaunchKernelGGL(k_exch_copy, dim3((threads + 255) / 256), dim3(256), 0, st, s->chars.as<uint8_t>(),
+                           P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                           P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), mode, n,
+                           P.recv_bytes.as<uint8_t>());
+    }
+    DSS_HIP(hipGetLastError());
+    // byte_begin per receiver
+    P.byte_begin.assign(p + 1, 0);
+    std::vector<uint64_t> tmp(1);
+    for (uint32_t q = 0; q < p; ++q) {
+        if (P.recv_begin[q] < n) {
+            DSS_HIP(hipMemcpyAsync(&P.byte_begin[q], P.recv_byte_off.as<uint64_t>() + P.recv_begin[q], 8,
+                                   hipMemcpyDeviceToHost, st));
+        } else {
+            P.byte_begin[q] = total;
+        }
+    }
+    P.byte_begin[p] = total;
+    DSS_HIP(hipStreamSynchronize(st));
+    P.mode = mode;
+    P.exchanged = true;
+    P.merged = P.extended = false;
+}
+
+// ================================================================== (a8)
+// LCP merge of the p received runs of every PE: a binary merge tree of
+// merge-path partitioned two-way LCP merges.  Each thread produces kW outputs
+// of one pair of runs: it finds its start by a merge-path search, derives the
+// LCP of both heads with the previous output, and then merges with the LCP
+// rules of the loser tree (bingmann-lcp_losertree.hpp:280-333): the head with
+// the larger LCP to the last output is smaller; on equal LCPs characters are
+// compared from that LCP on.  Character comparisons first use a cached 8-char
+// key at the receiver's common prefix.  The merged LCP array is exact; the
+// saved LCP of an output is the run LCP it was received with (oldLcps).
+constexpr uint32_t kW = 8;
+
+struct MergePair {
+    uint64_t out;      // first receive position of the pair (A then B)
+    uint64_t na, nb;   // run lengths
+    uint64_t chunk0;   // first thread chunk of the pair
+    uint32_t recv;     // receiving PE
+    uint32_t pad;
+};
+
+struct ViewCtx {
+    const uint8_t* chars;
+    const uint64_t* voff;
+    const uint32_t* vlen;
+    const uint64_t* keyc;
+    uint32_t cp;
+};
+
+// Compare received strings a, b (receive indices) knowing they agree on the
+// first `from` (>= cp) characters; returns sign and LCP.
+__device__ inline int elem_cmp(const ViewCtx& V, uint32_t a, uint32_t b, uint32_t from, uint32_t& lcp) {
+    if (from < V.cp + 8) {
+        const uint64_t ka = V.keyc[a], kb = V.keyc[b];
+        if (ka != kb) {
+            lcp = V.cp + key_lcp(ka, kb);
+            return ka < kb ? -1 : 1;
+        }
+        if ((ka & 0xFF) == 0) {
+            lcp = V.cp + key_strlen(ka);
+            return 0;
+        }
+        from = V.cp + 8;
+    }
+    return view_cmp(V.chars, V.voff[a], V.vlen[a], V.voff[b], V.vlen[b], from, &lcp);
+}
+
+__global__ void k_recv_cp(const uint64_t* __restrict__ run_off, const uint64_t* __restrict__ interval, uint32_t p,
+                          const uint8_t* __restrict__ chars, const uint64_t* __restrict__ voff,
+                          const uint32_t* __restrict__ vlen, uint32_t* __restrict__ cp_out) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= p) return;
+    uint32_t cp = 0xFFFFFFFFu;
+    int64_t s0 = -1;
+    for (uint32_t r = 0; r < p; ++r) {
+        const uint64_t c = interval[(uint64_t)r * p + q];
+        if (!c) continue;
+        const uint64_t f = run_off[(uint64_t)q * p + r], l = f + c - 1;
+        if (s0 < 0) {
+            s0 = (int64_t)f;
+            cp = vlen[f];
+        }
+        uint32_t x;
+        view_cmp(chars, voff[s0], vlen[s0], voff[f], vlen[f], 0, &x);
+        cp = min(cp, x);
+        view_cmp(chars, voff[s0], vlen[s0], voff[l], vlen[l], 0, &x);
+        cp = min(cp, x);
+    }
+    cp_out[q] = s0 < 0 ? 0 : cp;
+}
+
+__global__ void k_keycache(const uint64_t* __restrict__ recv_begin, uint32_t p, const uint8_t* __restrict__ chars,
+                           const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
+                           const uint32_t* __restrict__ cp, uint64_t n, uint64_t* __restrict__ keyc) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t lo = 0, hi = p;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (recv_begin[mid] <= k) lo = mid + 1; else hi = mid;
+    }
+    keyc[k] = load_key8(chars, voff[k], vlen[k], cp[lo - 1]);
+}
+
+__global__ void __launch_bounds__(256)
+k_merge_pairs(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t nchunks, const uint32_t* __restrict__ ord,
+              const uint32_t* __restrict__ lcpa, uint32_t* __restrict__ ord_out, uint32_t* __restrict__ lcp_out,
+              const uint8_t* __restrict__ chars, const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
+              const uint64_t* __restrict__ keyc, const uint32_t* __restrict__ rcp) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunks) return;
+    uint32_t lo = 0, hi = npairs;  // pair of chunk c
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pairs[mid].chunk0 <= c) lo = mid + 1; else hi = mid;
+    }
+    const MergePair Pr = pairs[lo - 1];
+    const ViewCtx V{chars, voff, vlen, keyc, rcp[Pr.recv]};
+    const uint64_t A = Pr.out, B = Pr.out + Pr.na;
+    const uint64_t na = Pr.na, nb = Pr.nb, tot = na + nb;
+    const uint64_t d = (c - Pr.chunk0) * kW;
+    const uint64_t dend = min(d + kW, tot);
+    // merge path: i = number of A elements among the first d outputs (A wins ties)
+    uint64_t l0 = d > nb ? d - nb : 0, h0 = d < na ? d : na;
+    while (l0 < h0) {
+        const uint64_t mid = (l0 + h0) / 2;
+        uint32_t x;
+        if (elem_cmp(V, ord[A + mid], ord[B + d - mid - 1], V.cp, x) <= 0) l0 = mid + 1;
+        else h0 = mid;
+    }
+    uint64_t i = l0, j = d - l0;
+    // LCPs of the heads with the previous output
+    uint32_t la = 0, lb = 0;
+    if (d > 0) {
+        bool prev_is_a;
+        uint32_t x;
+        if (i == 0) prev_is_a = false;
+        else if (j == 0) prev_is_a = true;
+        else prev_is_a = elem_cmp(V, ord[A + i - 1], ord[B + j - 1], V.cp, x) > 0;
+        const uint32_t prev = prev_is_a ? ord[A + i - 1] : ord[B + j - 1];
+        if (i < na) {
+            if (prev_is_a) la = lcpa[A + i];
+            else elem_cmp(V, ord[A + i], prev, V.cp, la);
+        }
+        if (j < nb) {
+            if (!prev_is_a) lb = lcpa[B + j];
+            else elem_cmp(V, ord[B + j], prev, V.cp, lb);
+        }
+    } else {
+        la = lb = V.cp;
+    }
+    for (uint64_t o = d; o < dend; ++o) {
+        uint32_t take_a;
+        uint32_t lout;
+        if (i >= na) take_a = 0;
+        else if (j >= nb) take_a = 1;
+        else if (la > lb) take_a = 1;
+        else if (lb > la) take_a = 0;
+        else {
+            uint32_t x;
+            const int cr = elem_cmp(V, ord[A + i], ord[B + j], la, x);
+            take_a = cr <= 0;
+            if (take_a) lb = x; else la = x;
+        }
+        if (take_a) {
+            lout = la;
+            ord_out[A + o] = ord[A + i];
+            ++i;
+            if (i < na) la = lcpa[A + i];
+        } else {
+            lout = lb;
+            ord_out[A + o] = ord[B + j];
+            ++j;
+            if (j < nb) lb = lcpa[B + j];
+        }
+        lcp_out[A + o] = (o == 0) ? 0u : lout;
+    }
+}
+
+__global__ void k_iota_copy(uint32_t* __restrict__ ord, uint32_t* __restrict__ l, const uint32_t* __restrict__ rl,
+                            uint64_t n) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        ord[k] = (uint32_t)k;
+        l[k] = rl[k];
+    }
+}
+
+__global__ void k_gather_saved(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ rl, uint64_t n,
+                               uint32_t* __restrict__ saved) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) saved[k] = rl[ord[k]];
+}
+
+void merge(dss_set_s* s) {
+    PipeState& P = s->pp;
+    DSS_REQUIRE(P.exchanged, DSS_ESTATE, "dss_exchange has not run");
+    const hipStream_t st = s->stream;
+    const uint32_t p = s->p;
+    const uint64_t n = s->n;
+    const uint64_t nn = std::max<uint64_t>(n, 1);
+    P.m_order.reserve(nn * 4);
+    P.m_lcp.reserve(nn * 4);
+    P.m_order2.reserve(nn * 4);
+    P.m_lcp2.reserve(nn * 4);
+    P.m_saved.reserve(nn * 4);
+    P.m_keyc.reserve(nn * 8);
+    const uint64_t cp_bytes = ((uint64_t)(p + 1) * 4 + 7) / 8 * 8;
+    P.m_cp.reserve(cp_bytes + (p + 1) * 8);
+    uint32_t* cp = P.m_cp.as<uint32_t>();
+    uint64_t* rb_d = reinterpret_cast<uint64_t*>(P.m_cp.as<uint8_t>() + cp_bytes);
+    DSS_HIP(hipMemcpyAsync(rb_d, P.recv_begin.data(), (p + 1) * 8, hipMemcpyHostToDevice, st));
+    uint32_t* ord = P.m_order.as<uint32_t>();
+    uint32_t* lc = P.m_lcp.as<uint32_t>();
+    uint32_t* ord2 = P.m_order2.as<uint32_t>();
+    uint32_t* lc2 = P.m_lcp2.as<uint32_t>();
+    if (n) {
+        hipLaunchKernelGGL(k_iota_copy, dim3((n + 255) / 256), dim3(256), 0, st, ord, lc, P.recv_lcp.as<uint32_t>(), n);
+        hipLaunchKernelGGL(k_recv_cp, dim3((p + 63) / 64), dim3(64), 0, st, P.layout.as<uint64_t>() + (uint64_t)p * p,
+                           P.interval_d.as<uint64_t>(), p, s->chars.as<uint8_t>(), P.recv_view_off.as<uint64_t>(),
+                           P.recv_view_len.as<uint32_t>(), cp);
+        KScope ks(s->timer, st, "merge_keycache", n * 20.0);
+        hipLaunchKernelGGL(k_keycache, dim3((n + 255) / 256), dim3(256), 0, st, rb_d, p, s->chars.as<uint8_t>(),
+                           P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(), cp, n, P.m_keyc.as<uint64_t>());
+    }
+    // runs per receiver (host), then pairwise rounds
+    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> runs(p);
+    for (uint32_t q = 0; q < p; ++q) {
+        uint64_t pos = P.recv_begin[q];
+        for (uint32_t r = 0; r < p; ++r) {
+            const uint64_t c = P.interval[(uint64_t)r * p + q];
+            if (c) runs[q].push_back({pos, c});
+            pos += c;
+        }
+    }
+    std::vector<MergePair> pairs;
+    while (true) {
+        pairs.clear();
+        uint64_t chunks = 0;
+        bool any = false;
+        for (uint32_t q = 0; q < p; ++q) {
+            auto& R = runs[q];
+            if (R.size() <= 1) continue;
+            any = true;
+            std::vector<std::pair<uint64_t, uint64_t>> next;
+            for (size_t k = 0; k < R.size(); k += 2) {
+                MergePair mp;
+                mp.out = R[k].first;
+                mp.na = R[k].second;
+                mp.nb = k + 1 < R.size() ? R[k + 1].second : 0;
+                mp.recv = q;
+                mp.pad = 0;
+                mp.chunk0 = chunks;
+                chunks += (mp.na + mp.nb + kW - 1) / kW;
+                pairs.push_back(mp);
+                next.push_back({mp.out, mp.na + mp.nb});
+            }
+            R.swap(next);
+        }
+        if (!any) break;
+        // receivers that already have a single run also need their data in
+        // the output buffers: add them as one-run pairs
+        for (uint32_t q = 0; q < p; ++q) {
+            bool listed = false;
+            for (auto& mp : pairs) listed |= mp.recv == q;
+            if (!listed && runs[q].size() == 1) {
+                MergePair mp;
+                mp.out = runs[q][0].first;
+                mp.na = runs[q][0].second;
+                mp.nb = 0;
+                mp.recv = q;
+                mp.pad = 0;
+                mp.chunk0 = chunks;
+                chunks += (mp.na + kW - 1) / kW;
+                pairs.push_back(mp);
+            }
+        }
+        P.m_work.reserve(pairs.size() * sizeof(MergePair));
+        DSS_HIP(hipMemcpyAsync(P.m_work.p, pairs.data(), pairs.size() * sizeof(MergePair), hipMemcpyHostToDevice, st));
+        {
+            KScope ks(s->timer, st, "merge_pairs", n * 16.0);
+            hipLaunchKernelGGL(k_merge_pairs, dim3((chunks + 255) / 256), dim3(256), 0, st, P.m_work.as<MergePair>(),
+                               (uint32_t)pairs.size(), chunks, ord, lc, ord2, lc2, s->chars.as<uint8_t>(),
+                               P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                               P.m_keyc.as<uint64_t>(), cp);
+        }
+        DSS_HIP(hipGetLastError());
+        // the pair list is reused by the next round only after this round ran
+        DSS_HIP(hipStreamSynchronize(st));
+        std::swap(ord, ord2);
+        std::swap(lc, lc2);
+    }
+    // results live in (ord, lc); make them the canonical buffers
+    if (ord != P.m_order.as<uint32_t>()) {
+        std::swap(P.m_order.p, P.m_order2.p);
+        std::swap(P.m_order.bytes, P.m_order2.bytes);
+        std::swap(P.m_lcp.p, P.m_lcp2.p);
+        std::swap(P.m_lcp.bytes, P.m_lcp2.bytes);
+    }
+    if (n)
+        hipLaunchKernelGGL(k_gather_saved, dim3((n + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
+                           P.recv_lcp.as<uint32_t>(), n, P.m_saved.as<uint32_t>());
+    DSS_HIP(hipGetLastError());
+    P.merged = true;
+    P.extended = false;
+}
+
+// ================================================================== (a9)
+// StringLcpContainer::extendPrefix (src/strings/stringcontainer.hpp:247-319)
+// on the merged receive arena of every PE: output string i = the first
+// saved[i] characters of output string i-1, then its stored characters.
+// Char c < saved[i] of string i comes from the stored part of the nearest
+// k <= i-1 whose saved[k] <= c; following "previous strictly smaller saved"
+// links (psv) from i visits exactly those sources, so every string is
+// rebuilt independently: chars [saved[psv], cur) come from stored(psv).
+// psv is computed per 1024-position block with a stack, falling back to a
+// backward walk over earlier blocks (block minima) for open positions.
+constexpr uint32_t kPsvB = 1024;
+
+__global__ void k_block_min(const uint32_t* __restrict__ saved, const uint64_t* __restrict__ recv_begin, uint32_t p,
+                            uint64_t n, uint32_t* __restrict__ bmin) {
+    __shared__ uint32_t red[4];
+    const uint64_t b = (uint64_t)blockIdx.x * kPsvB;
+    uint32_t m = 0xFFFFFFFFu;
+    for (uint32_t t = threadIdx.x; t < kPsvB; t += blockDim.x) {
+        const uint64_t i = b + t;
+        if (i < n) m = min(m, saved[i]);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_down(m, o));
+    if (lane_id() == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) bmin[blockIdx.x] = min(min(red[0], red[1]), min(red[2], red[3]));
+}
+
+// psv[i] = max{k < i in the same receiver : saved[k] < saved[i]}, or ~0
+// when saved[i] == 0 or no such k exists (the receiver's first string).
+__global__ void k_psv(const uint32_t* __restrict__ saved, const uint64_t* __restrict__ recv_begin, uint32_t p,
+                      uint64_t n, const uint32_t* __restrict__ bmin, uint64_t* __restrict__ psv) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lo = 0, hi = p;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (recv_begin[mid] <= i) lo = mid + 1; else hi = mid;
+    }
+    const uint64_t rb = recv_begin[lo - 1];
+    const uint32_t v = saved[i];
+    uint64_t res = ~0ull;
+    if (v != 0 && i > rb) {
+        uint64_t k = i;
+        // walk back inside the current block
+        const uint64_t blk0 = (i / kPsvB) * kPsvB;
+        while (k > blk0 && k > rb) {
+            --k;
+            if (saved[k] < v) { res = k; break; }
+        }
+        if (res == ~0ull && k > rb) {
+            // earlier blocks: skip those whose minimum is not smaller
+            int64_t blk = (int64_t)(blk0 / kPsvB) - 1;
+            while (blk >= 0 && (uint64_t)(blk + 1) * kPsvB > rb) {
+                if (bmin[blk] < v) {
+                    uint64_t e = (uint64_t)(blk + 1) * kPsvB;
+                    const uint64_t s0 = std::max<uint64_t>((uint64_t)blk * kPsvB, rb);
+                    while (e > s0) {
+                        --e;
+                        if (saved[e] < v) { res = e; break; }
+                    }
+                    if (res != ~0ull) break;
+                }
+                --blk;
+            }
+        }
+    }
+    psv[i] = res;
+}
+
+__global__ void k_out_size(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ saved,
+                           const uint64_t* __restrict__ rsize, uint64_t n, uint64_t* __restrict__ out_size) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out_size[i] = saved[i] + rsize[ord[i]];  // prefix + stored chars incl. NUL
+}
+
+// 16 lanes per output string.
+__global__ void k_extend(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ saved,
+                         const uint64_t* __restrict__ psv, const uint64_t* __restrict__ rbyte,
+                         const uint64_t* __restrict__ rsize, const uint8_t* __restrict__ rbytes,
+                         const uint64_t* __restrict__ out_off, uint64_t n, uint8_t* __restrict__ out) {
+    const uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const uint32_t l = threadIdx.x & 15;
+    if (i >= n) return;
+    uint8_t* dst = out + out_off[i];
+    uint32_t cur = saved[i];
+    // own stored characters (incl. NUL) after the prefix
+    {
+        const uint8_t* src = rbytes + rbyte[ord[i]];
+        const uint64_t m = rsize[ord[i]];
+        for (uint64_t t = l; t < m; t += 16) dst[cur + t] = src[t];
+    }
+    uint64_t k = psv[i];
+    while (cur > 0 && k != ~0ull) {
+        const uint32_t sk = saved[k];
+        const uint8_t* src = rbytes + rbyte[ord[k]];
+        for (uint32_t t = sk + l; t < cur; t += 16) dst[t] = src[t - sk];
+        cur = sk;
+        k = psv[k];
+    }
+}
+
+void extend_prefix(dss_set_s* s) {
+    PipeState& P = s->pp;
+    DSS_REQUIRE(P.merged, DSS_ESTATE, "dss_merge has not run");
+    DSS_REQUIRE(P.mode == 1, DSS_ESTATE, "prefix decompression needs the LCP-compressed exchange (mode 1)");
+    const hipStream_t st = s->stream;
+    const uint64_t n = s->n;
+    const uint32_t p = s->p;
+    const uint64_t nn = std::max<uint64_t>(n, 1);
+    const uint64_t nblk = (n + kPsvB - 1) / kPsvB;
+    P.out_psv.reserve(nn * 8);
+    P.out_off.reserve((nn + 1) * 8);
+    P.out_aux.reserve((nblk + 1) * 4 + (p + 1) * 8 + 64);
+    uint32_t* bmin = P.out_aux.as<uint32_t>();
+    uint64_t* rb_d = reinterpret_cast<uint64_t*>(P.out_aux.as<uint8_t>() + ((nblk + 1) * 4 + 7) / 8 * 8);
+    DSS_HIP(hipMemcpyAsync(rb_d, P.recv_begin.data(), (p + 1) * 8, hipMemcpyHostToDevice, st));
+    uint64_t total = 0;
+    if (n) {
+        KScope ks(s->timer, st, "extend_psv", n * 16.0);
+        hipLaunchKernelGGL(k_block_min, dim3(nblk), dim3(256), 0, st, P.m_saved.as<uint32_t>(), rb_d, p, n, bmin);
+        hipLaunchKernelGGL(k_psv, dim3((n + 255) / 256), dim3(256), 0, st, P.m_saved.as<uint32_t>(), rb_d, p, n, bmin,
+                           P.out_psv.as<uint64_t>());
+        hipLaunchKernelGGL(k_out_size, dim3((n + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
+                           P.m_saved.as<uint32_t>(), P.recv_size.as<uint64_t>(), n, P.out_off.as<uint64_t>());
+    }
+    exclusive_scan_u64(P.out_off.as<uint64_t>(), P.out_off.as<uint64_t>(), n, st, P, &total);
+    P.out_nbytes = total;
+    P.out_bytes.reserve(total + kArenaPad);
+    if (n) {
+        KScope ks(s->timer, st, "extend_copy", (double)total * 2.0 + n * 24.0);
+        hipLaunchKernelGGL(k_extend, dim3((n * 16 + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
+                           P.m_saved.as<uint32_t>(), P.out_psv.as<uint64_t>(), P.recv_byte_off.as<uint64_t>(),
+                           P.recv_size.as<uint64_t>(), P.recv_bytes.as<uint8_t>(), P.out_off.as<uint64_t>(), n,
+                           P.out_bytes.as<uint8_t>());
+    }
+    DSS_HIP(hipGetLastError());
+    DSS_HIP(hipStreamSynchronize(st));
+    P.extended = true;
+}
+
+}  // namespace dss
+
+using namespace dss;
+
+extern "C" {
+
+int dss_avg_lcp(dss_set s, uint64_t* out) {
+    return guarded([&] {
+        DSS_REQUIRE(s && out, DSS_EINVAL, "NULL argument");
+        *out = avg_lcp(s);
+    });
+}
+
+int dss_partition(dss_set s, int sampler, uint64_t sampling_factor, uint64_t max_len, int use_dist,
+                  uint64_t* interval) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        partition(s, sampler, sampling_factor, max_len, use_dist != 0);
+        if (interval) memcpy(interval, s->pp.interval.data(), s->pp.interval.size() * 8);
+    });
+}
+
+int dss_exchange(dss_set s, int mode, uint64_t* recv_begin, uint64_t* byte_begin, uint64_t* recv_nbytes) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        exchange(s, mode);
+        if (recv_begin) memcpy(recv_begin, s->pp.recv_begin.data(), (s->p + 1) * 8);
+        if (byte_begin) memcpy(byte_begin, s->pp.byte_begin.data(), (s->p + 1) * 8);
+        if (recv_nbytes) *recv_nbytes = s->pp.recv_nbytes;
+    });
+}
+
+int dss_get_exchange(dss_set s, uint8_t* bytes, uint32_t* lcp, uint32_t* src_pe, uint32_t* src_pos) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        PipeState& P = s->pp;
+        DSS_REQUIRE(P.exchanged, DSS_ESTATE, "dss_exchange has not run");
+        const hipStream_t st = s->stream;
+        if (bytes && P.recv_nbytes)
+            DSS_HIP(hipMemcpyAsync(bytes, P.recv_bytes.p, P.recv_nbytes, hipMemcpyDeviceToHost, st));
+        if (s->n) {
+            if (lcp) DSS_HIP(hipMemcpyAsync(lcp, P.recv_lcp.p, s->n * 4, hipMemcpyDeviceToHost, st));
+            if (src_pe) DSS_HIP(hipMemcpyAsync(src_pe, P.recv_src_pe.p, s->n * 4, hipMemcpyDeviceToHost, st));
+            if (src_pos) DSS_HIP(hipMemcpyAsync(src_pos, P.recv_src_pos.p, s->n * 4, hipMemcpyDeviceToHost, st));
+        }
+        DSS_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int dss_merge_lcp(dss_set s) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        merge(s);
+    });
+}
+
+int dss_get_merged(dss_set s, uint32_t* order, uint32_t* lcp, uint32_t* saved) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        PipeState& P = s->pp;
+        DSS_REQUIRE(P.merged, DSS_ESTATE, "dss_merge_lcp has not run");
+        const hipStream_t st = s->stream;
+        if (s->n) {
+            if (order) DSS_HIP(hipMemcpyAsync(order, P.m_order.p, s->n * 4, hipMemcpyDeviceToHost, st));
+            if (lcp) DSS_HIP(hipMemcpyAsync(lcp, P.m_lcp.p, s->n * 4, hipMemcpyDeviceToHost, st));
+            if (saved) DSS_HIP(hipMemcpyAsync(saved, P.m_saved.p, s->n * 4, hipMemcpyDeviceToHost, st));
+        }
+        DSS_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int dss_extend_prefix(dss_set s, uint64_t* out_nbytes) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        extend_prefix(s);
+        if (out_nbytes) *out_nbytes = s->pp.out_nbytes;
+    });
+}
+
+int dss_get_extended(dss_set s, uint8_t* bytes, uint64_t* offsets) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        PipeState& P = s->pp;
+        DSS_REQUIRE(P.extended, DSS_ESTATE, "dss_extend_prefix has not run");
+        const hipStream_t st = s->stream;
+        if (bytes && P.out_nbytes) DSS_HIP(hipMemcpyAsync(bytes, P.out_bytes.p, P.out_nbytes, hipMemcpyDeviceToHost, st));
+        if (offsets && s->n) DSS_HIP(hipMemcpyAsync(offsets, P.out_off.p, s->n * 8, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+    });
+}
+
+}  // extern "C"

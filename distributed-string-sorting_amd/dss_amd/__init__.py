@@ -118,6 +118,102 @@ class DeviceStringSet:
         check(lib().dss_get_sorted(self._h, _p(perm, _lib.u32p), _p(lcp, _lib.u32p)))
         return perm[:n], lcp[:n]
 
+    def avg_lcp(self) -> int:
+        out = C.c_uint64()
+        check(lib().dss_avg_lcp(self._h, C.byref(out)))
+        return out.value
+
+    SAMPLERS = {"NumStrings": 0, "NumChars": 1, "IndexedNumStrings": 2, "IndexedNumChars": 3}
+    MODES = {"whole": 0, "lcp_compact": 1, "pd_prefix": 2}
+
+    def partition(self, sampler="IndexedNumStrings", max_len=0, use_dist=False, factor=2):
+        p = self.p
+        out = np.zeros(p * p, dtype=np.uint64)
+        check(lib().dss_partition(self._h, self.SAMPLERS.get(sampler, sampler), factor, max_len,
+                                  int(use_dist), _p(out, _lib.u64p)))
+        return out.reshape(p, p)
+
+    def exchange(self, mode):
+        p = self.p
+        rb = np.zeros(p + 1, dtype=np.uint64)
+        bb = np.zeros(p + 1, dtype=np.uint64)
+        nb = C.c_uint64()
+        check(lib().dss_exchange(self._h, self.MODES.get(mode, mode), _p(rb, _lib.u64p),
+                                 _p(bb, _lib.u64p), C.byref(nb)))
+        return rb, bb, nb.value
+
+    def get_exchange(self):
+        n = self.n
+        lcp = np.zeros(max(n, 1), dtype=np.uint32)
+        pe = np.zeros(max(n, 1), dtype=np.uint32)
+        pos = np.zeros(max(n, 1), dtype=np.uint32)
+        check(lib().dss_get_exchange(self._h, None, _p(lcp, _lib.u32p), _p(pe, _lib.u32p),
+                                     _p(pos, _lib.u32p)))
+        return lcp[:n], pe[:n], pos[:n]
+
+    def get_exchange_bytes(self, nbytes):
+        out = np.zeros(max(nbytes, 1), dtype=np.uint8)
+        check(lib().dss_get_exchange(self._h, _p(out, _lib.u8p), None, None, None))
+        return out[:nbytes]
+
+    def merge_lcp(self):
+        check(lib().dss_merge_lcp(self._h))
+
+    def get_merged(self):
+        n = self.n
+        o = np.zeros(max(n, 1), dtype=np.uint32)
+        l = np.zeros(max(n, 1), dtype=np.uint32)
+        sv = np.zeros(max(n, 1), dtype=np.uint32)
+        check(lib().dss_get_merged(self._h, _p(o, _lib.u32p), _p(l, _lib.u32p), _p(sv, _lib.u32p)))
+        return o[:n], l[:n], sv[:n]
+
+    def extend_prefix(self) -> int:
+        nb = C.c_uint64()
+        check(lib().dss_extend_prefix(self._h, C.byref(nb)))
+        return nb.value
+
+    def get_extended(self, nbytes):
+        n = self.n
+        out = np.zeros(max(nbytes, 1), dtype=np.uint8)
+        offs = np.zeros(max(n, 1), dtype=np.uint64)
+        check(lib().dss_get_extended(self._h, _p(out, _lib.u8p), _p(offs, _lib.u64p)))
+        return out[:nbytes], offs[:n]
+
+    def prefix_doubling(self, m=2**64 - 1, start_depth=8) -> int:
+        nr = C.c_uint32()
+        check(lib().dss_prefix_doubling(self._h, m, start_depth, C.byref(nr)))
+        return nr.value
+
+    def get_prefix_doubling(self):
+        n = self.n
+        res = np.zeros(max(n, 1), dtype=np.uint32)
+        rounds = np.zeros(max(n, 1), dtype=np.uint8)
+        check(lib().dss_get_prefix_doubling(self._h, _p(res, _lib.u32p), _p(rounds, _lib.u8p)))
+        return res[:n], rounds[:n]
+
+    def hash_prefixes(self, ids, depth, m=2**64 - 1):
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        out = np.zeros(max(ids.size, 1), dtype=np.uint64)
+        check(lib().dss_hash_prefixes(self._h, _p(ids, _lib.u32p), ids.size, depth, m,
+                                      _p(out, _lib.u64p)))
+        return out[:ids.size]
+
+    # ------------------------------------------------------------ drivers
+    def merge_sort(self, sampler="IndexedNumStrings", compressed=True):
+        """DistributedMergeSort::sort (+ extendPrefix when compressed)."""
+        check(lib().dss_merge_sort(self._h, self.SAMPLERS.get(sampler, sampler), int(compressed)))
+
+    def prefix_doubling_sort(self, sampler="IndexedNumStrings", m=2**64 - 1):
+        """DistributedPrefixDoublingSort::sort."""
+        check(lib().dss_prefix_doubling_sort(self._h, self.SAMPLERS.get(sampler, sampler), m))
+
+    def get_permutation(self):
+        n = self.n
+        si = np.zeros(max(n, 1), dtype=np.uint32)
+        pi = np.zeros(max(n, 1), dtype=np.uint32)
+        check(lib().dss_get_permutation(self._h, _p(si, _lib.u32p), _p(pi, _lib.u32p)))
+        return si[:n], pi[:n]
+
     # ------------------------------------------------------------ timing
     def sync(self):
         check(lib().dss_sync(self._h))

@@ -51,6 +51,20 @@ _SIGS = {
     "dss_set_download": (C.c_int, [set_p, u8p, u64p, u32p]),
     "dss_sort_lcp": (C.c_int, [set_p]),
     "dss_get_sorted": (C.c_int, [set_p, u32p, u32p]),
+    "dss_avg_lcp": (C.c_int, [set_p, u64p]),
+    "dss_partition": (C.c_int, [set_p, C.c_int, C.c_uint64, C.c_uint64, C.c_int, u64p]),
+    "dss_exchange": (C.c_int, [set_p, C.c_int, u64p, u64p, u64p]),
+    "dss_get_exchange": (C.c_int, [set_p, u8p, u32p, u32p, u32p]),
+    "dss_merge_lcp": (C.c_int, [set_p]),
+    "dss_get_merged": (C.c_int, [set_p, u32p, u32p, u32p]),
+    "dss_extend_prefix": (C.c_int, [set_p, u64p]),
+    "dss_get_extended": (C.c_int, [set_p, u8p, u64p]),
+    "dss_prefix_doubling": (C.c_int, [set_p, C.c_uint64, C.c_uint32, u32p]),
+    "dss_get_prefix_doubling": (C.c_int, [set_p, u32p, u8p]),
+    "dss_hash_prefixes": (C.c_int, [set_p, u32p, C.c_uint64, C.c_uint32, C.c_uint64, u64p]),
+    "dss_merge_sort": (C.c_int, [set_p, C.c_int, C.c_int]),
+    "dss_prefix_doubling_sort": (C.c_int, [set_p, C.c_int, C.c_uint64]),
+    "dss_get_permutation": (C.c_int, [set_p, u32p, u32p]),
     "dss_sync": (C.c_int, [set_p]),
     "dss_set_timing": (C.c_int, [set_p, C.c_int]),
     "dss_kernel_stats": (C.c_int, [set_p, C.POINTER(KernelStat), C.c_uint32, u32p, C.c_int]),

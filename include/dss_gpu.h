@@ -61,9 +61,12 @@ int dss_set_create(const uint8_t* chars, uint64_t nbytes, uint32_t num_pes, dss_
 int dss_set_create_dton(uint64_t n, uint32_t len, double dton, uint64_t seed,
                         uint32_t num_pes, dss_set* out);
 
-/* Builds the skewed D/N input on the device (see the oracle's generator
- * specification in DESIGN notes of the project): long_permille of the
- * strings long, dup_permille exact duplicates, bytes 1..255. */
+/* Builds the skewed D/N input on the device (modelled on
+ * SkewedDNRatioGenerator, src/util/random_string_generator.hpp:431-509):
+ * long_permille of the strings have long_len chars, the rest short_len;
+ * the first k = max(L*dn, ceil(log255 n)) chars are the base-255 digits of
+ * the string's id (bytes 1..255), the rest seeded filler; dup_permille of the
+ * strings repeat the id of another string (exact duplicates). */
 int dss_set_create_skewed(uint64_t n, uint32_t long_len, uint32_t short_len,
                           double long_dn, double short_dn, uint32_t long_permille,
                           uint32_t dup_permille, uint64_t seed, uint32_t num_pes,
@@ -96,6 +99,75 @@ int dss_set_download(dss_set s, uint8_t* chars, uint64_t* offsets, uint32_t* len
 int dss_sort_lcp(dss_set s);
 /* perm/lcp: num_strings entries each (either may be NULL). */
 int dss_get_sorted(dss_set s, uint32_t* perm, uint32_t* lcp);
+
+/* ---- (a3) getAvgLcp (src/sorter/distributed/misc.hpp:67-87) ---- */
+int dss_avg_lcp(dss_set s, uint64_t* avg);
+
+/* ---- (a4, a5) computePartition (src/sorter/distributed/misc.hpp:473-648):
+ *      splitter sampling (samplingStrategies.hpp; sampler 0 NumStrings,
+ *      1 NumChars, 2 IndexedNumStrings, 3 IndexedNumChars), global sample
+ *      order, splitter choice (getSplitters(Indexed), :89-189) and bucket
+ *      sizes by binarySearch(Indexed) (:310-414).  Samples are truncated to
+ *      max_len (the merge sort passes 200*(avgLcp+5)) or, with use_dist, to
+ *      the prefix-doubling results.  interval (optional, p*p entries):
+ *      interval[r*p+q] = strings PE r sends to PE q ---- */
+int dss_partition(dss_set s, int sampler, uint64_t sampling_factor, uint64_t max_len,
+                  int use_dist, uint64_t* interval);
+
+/* ---- (a6, a7, a13) AllToAllStringImpl<..., EmptyByteEncoderMemCpy> (mode 0),
+ *      <..., EmptyLcpByteEncoderMemCpy> (mode 1) and
+ *      AllToAllStringImplPrefixDoubling (mode 2) (src/mpi/alltoall.hpp:642-804,
+ *      902-1009, encoders src/mpi/byte_encoder.hpp:81-172), for p virtual
+ *      PEs as an in-HBM bucket scatter.  recv_begin / byte_begin (optional,
+ *      p+1 entries) give each receiving PE's strings and arena bytes ---- */
+int dss_exchange(dss_set s, int mode, uint64_t* recv_begin, uint64_t* byte_begin,
+                 uint64_t* recv_nbytes);
+/* Receive arena bytes, received LCPs (0 at every run start) and each
+ * received string's origin (source PE, position in the source's sorted
+ * order: StringIndexPEIndex); any pointer may be NULL. */
+int dss_get_exchange(dss_set s, uint8_t* bytes, uint32_t* lcp, uint32_t* src_pe,
+                     uint32_t* src_pos);
+
+/* ---- (a8) choose_merge / merge<K> with LcpStringLoserTree_
+ *      (src/sorter/distributed/merging.hpp:66-123,
+ *      src/merge/bingmann-lcp_losertree.hpp:204-480): merges every PE's
+ *      received runs with LCPs ---- */
+int dss_merge_lcp(dss_set s);
+/* Per receive-side output position: the received string (receive index),
+ * the merged LCP and the saved LCP (oldLcps); any pointer may be NULL. */
+int dss_get_merged(dss_set s, uint32_t* order, uint32_t* lcp, uint32_t* saved);
+
+/* ---- (a9) StringLcpContainer::extendPrefix
+ *      (src/strings/stringcontainer.hpp:247-319), called at
+ *      src/executables/distributed_sorter.cpp:127-131 after a mode-1
+ *      exchange ---- */
+int dss_extend_prefix(dss_set s, uint64_t* out_nbytes);
+/* Rebuilt arena (all PEs, merged order) and per-string byte offsets. */
+int dss_get_extended(dss_set s, uint8_t* bytes, uint64_t* offsets);
+
+/* ---- (a10-a12) computeDistinguishingPrefixes
+ *      (src/sorter/distributed/prefix_doubling.hpp:156-198) with
+ *      BloomFilter<...,XXHasher>::filter (bloomfilter.hpp:1415-1556); m is
+ *      the hash range (bloomFilterSize, UINT64_MAX in the reference) ---- */
+int dss_prefix_doubling(dss_set s, uint64_t m, uint32_t start_depth, uint32_t* num_rounds);
+/* Per position: results (distinguishing-prefix estimate) and the number of
+ * rounds the position was a candidate in; either pointer may be NULL. */
+int dss_get_prefix_doubling(dss_set s, uint32_t* results, uint8_t* rounds);
+/* XXHasher::hash (bloomfilter.hpp:1094-1102) of the first min(depth, len)
+ * chars of the given strings (string ids), reduced % m. */
+int dss_hash_prefixes(dss_set s, const uint32_t* ids, uint64_t count, uint32_t depth,
+                      uint64_t m, uint64_t* out);
+
+/* ---- drivers ---- */
+
+/* DistributedMergeSort::sort (lcp_aware_number_string_splitting.hpp:
+ * 189-289) plus extendPrefix when compressed (mode 1 exchange). */
+int dss_merge_sort(dss_set s, int sampler, int compressed);
+/* DistributedPrefixDoublingSort::sort (prefix_doubling.hpp:204-294). */
+int dss_prefix_doubling_sort(dss_set s, int sampler, uint64_t m);
+/* The permutation of DistributedPrefixDoublingSort (prefix_doubling.hpp:
+ * 282-290): per receive-side output position, stringIndex and PEIndex. */
+int dss_get_permutation(dss_set s, uint32_t* string_index, uint32_t* pe_index);
 
 /* ---- measurement ---- */
 

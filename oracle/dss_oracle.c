@@ -546,7 +546,8 @@ ORC_API int orc_partition(const uint8_t* chars, const uint64_t* off,
                     seen += dist ? dist[b + si] : len[perm[b + si]];
                     ++si;
                 }
-                idx_list[cnt++] = si - 1;
+                idx_list[cnt++] = si ? si - 1 : 0; /* the reference reads index -1 here
+                                                      when no characters were seen */
             }
         }
         for (uint64_t t = 0; t < cnt; ++t) {

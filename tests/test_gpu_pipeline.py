@@ -1,0 +1,228 @@
+"""GPU parity of the distributed stages (average LCP, partition, exchange,
+LCP merge, prefix decompression, prefix doubling) and of the two composite
+drivers against the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import canonical_ties
+
+pytestmark = pytest.mark.gpu
+
+SAMPLERS = ["NumStrings", "NumChars", "IndexedNumStrings", "IndexedNumChars"]
+
+
+def _dss():
+    import dss_amd
+    return dss_amd
+
+
+def arena(strings):
+    return np.frombuffer(b"".join(s + b"\0" for s in strings), dtype=np.uint8).copy()
+
+
+def inputs():
+    rng = np.random.default_rng(3)
+    small = [bytes(rng.integers(97, 100, int(l), dtype=np.uint8)) for l in rng.integers(0, 12, 3000)]
+    return {
+        "dton": O.gen_dton(6000, 80, 0.5, 0x5EED0001),
+        "skewed": O.gen_skewed(O.skew_params(6000, long_len=200, short_len=20), 0x5EED0003),
+        "suffix": O.gen_suffixes(5000, 0x5EED0004, repeats=10, rep_min=100, rep_max=800, cap=300),
+        "tiny_alphabet": arena(small),
+    }
+
+
+INPUTS = inputs()
+
+
+def eq_runs(strings, lcp):
+    """Boolean mask: position i continues a run of equal strings."""
+    n = len(strings)
+    m = np.zeros(n, bool)
+    for i in range(1, n):
+        m[i] = strings[i] == strings[i - 1]
+    return m
+
+
+def canon(keys, eqmask):
+    """Sort key tuples inside runs of equal strings."""
+    keys = list(keys)
+    i = 0
+    n = len(keys)
+    while i < n:
+        j = i + 1
+        while j < n and eqmask[j]:
+            j += 1
+        keys[i:j] = sorted(keys[i:j])
+        i = j
+    return keys
+
+
+@pytest.mark.parametrize("name", sorted(INPUTS))
+@pytest.mark.parametrize("p", [3, 8])
+def test_merge_sort_stages(name, p):
+    dss = _dss()
+    chars = INPUTS[name]
+    ss = O.StringSet(chars, p)
+    operm, olcp = O.local_sort(ss)
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        ds.sort_lcp()
+        avg = ds.avg_lcp()
+        assert avg == O.avg_lcp(olcp)
+        max_len = 2 * 100 * (avg + 5)
+        for sampler in SAMPLERS:
+            giv = ds.partition(sampler, max_len=max_len)
+            oiv = O.partition(ss, operm, sampler, max_len=max_len)
+            assert (giv == oiv).all(), sampler
+        # short truncation exercises prefix samples
+        giv = ds.partition("IndexedNumStrings", max_len=3)
+        oiv = O.partition(ss, operm, "IndexedNumStrings", max_len=3)
+        assert (giv == oiv).all()
+        iv = ds.partition("IndexedNumStrings", max_len=max_len)
+        for mode in ["whole", "lcp_compact"]:
+            rb, bb, nbytes = ds.exchange(mode)
+            ex = O.exchange(ss, operm, olcp, iv, mode)
+            assert (rb == ex["recv_begin"]).all() and (bb == ex["byte_begin"]).all()
+            gbytes = ds.get_exchange_bytes(nbytes)
+            assert gbytes.tobytes() == ex["bytes"].tobytes(), mode
+            glcp, gpe, gpos = ds.get_exchange()
+            assert (glcp == ex["lcp"]).all() and (gpe == ex["src_pe"]).all() and (gpos == ex["src_pos"]).all()
+            ds.merge_lcp()
+            gord, gml, gsv = ds.get_merged()
+            for q in range(p):
+                r0, r1 = int(rb[q]), int(rb[q + 1])
+                b0, b1 = int(bb[q]), int(bb[q + 1])
+                runs = iv[:, q].astype(np.uint64)
+                a = ex["bytes"][b0:b1]
+                oo, ol, osv = O.merge_losertree(a, ex["lcp"][r0:r1], runs, mode == "lcp_compact")
+                assert (gml[r0:r1] == ol).all(), (mode, q)
+                # strings in merged order (receive arena is identical on both sides)
+                starts = np.concatenate([[0], np.flatnonzero(a == 0)[:-1] + 1]) if r1 > r0 else []
+                recv = [a[int(s):].tobytes().split(b"\0", 1)[0] for s in starts]
+                go = [int(x) - r0 for x in gord[r0:r1]]
+                # with prefix compression a received string is a suffix; compare
+                # the sequence of (saved lcp, stored chars) after canonical ties
+                gk = [(recv[i], int(v)) for i, v in zip(go, gsv[r0:r1])]
+                ok = [(recv[int(i)], int(v)) for i, v in zip(oo, osv)]
+                if mode == "whole":
+                    # strings and LCPs in merged order are unique
+                    gstr = [recv[i] for i in go]
+                    ostr = [recv[int(i)] for i in oo]
+                    assert gstr == ostr
+                else:
+                    # saved LCPs (oldLcps) exist only with prefix compression;
+                    # their order among equal strings is the tie order
+                    assert sorted(gk) == sorted(ok)
+            if mode == "lcp_compact":
+                nb = ds.extend_prefix()
+                gout, goff = ds.get_extended(nb)
+                exp = []
+                for q in range(p):
+                    r0, r1 = int(rb[q]), int(rb[q + 1])
+                    b0, b1 = int(bb[q]), int(bb[q + 1])
+                    a = ex["bytes"][b0:b1]
+                    oo, ol, osv = O.merge_losertree(a, ex["lcp"][r0:r1], iv[:, q].astype(np.uint64), True)
+                    exp.append(O.extend_prefix(a, oo, osv).tobytes())
+                assert gout.tobytes() == b"".join(exp)
+
+
+@pytest.mark.parametrize("name", sorted(INPUTS))
+@pytest.mark.parametrize("p", [2, 5])
+def test_prefix_doubling_stages(name, p):
+    dss = _dss()
+    chars = INPUTS[name]
+    ss = O.StringSet(chars, p)
+    operm, olcp = O.local_sort(ss)
+    ores, orounds, onr = O.prefix_doubling(ss, operm, olcp)
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        ds.sort_lcp()
+        nr = ds.prefix_doubling()
+        gres, grounds = ds.get_prefix_doubling()
+        assert nr == onr
+        assert (gres == ores).all()
+        assert (grounds == orounds).all()
+        for sampler in SAMPLERS:
+            giv = ds.partition(sampler, use_dist=True)
+            oiv = O.partition(ss, operm, sampler, dist=ores)
+            assert (giv == oiv).all(), sampler
+        iv = ds.partition("IndexedNumStrings", use_dist=True)
+        rb, bb, nbytes = ds.exchange("pd_prefix")
+        ex = O.exchange(ss, operm, olcp, iv, "pd_prefix", dist=ores)
+        assert ds.get_exchange_bytes(nbytes).tobytes() == ex["bytes"].tobytes()
+
+
+@pytest.mark.parametrize("m", [2**64 - 1, 997, 64])
+def test_prefix_doubling_small_hash_range(m):
+    """Hash ranges far below 2^64 force collisions (the Bloom-sized variant)."""
+    dss = _dss()
+    chars = INPUTS["skewed"]
+    ss = O.StringSet(chars, 4)
+    operm, olcp = O.local_sort(ss)
+    ores, orounds, onr = O.prefix_doubling(ss, operm, olcp, m=m)
+    with dss.DeviceStringSet.from_arena(chars, 4) as ds:
+        ds.sort_lcp()
+        nr = ds.prefix_doubling(m=m)
+        gres, grounds = ds.get_prefix_doubling()
+    assert nr == onr and (gres == ores).all() and (grounds == orounds).all()
+
+
+def test_hash_prefixes_match_reference_hash():
+    dss = _dss()
+    rng = np.random.default_rng(9)
+    strs = [bytes(rng.integers(1, 256, l, dtype=np.uint8)) for l in range(0, 130)]
+    chars = arena(strs)
+    with dss.DeviceStringSet.from_arena(chars, 1) as ds:
+        ids = np.arange(len(strs), dtype=np.uint32)
+        for depth in [1, 7, 8, 31, 32, 33, 64, 100, 129]:
+            got = ds.hash_prefixes(ids, depth)
+            exp = [O.pd_hash(s[:depth]) for s in strs]
+            assert list(map(int, got)) == exp, depth
+
+
+@pytest.mark.parametrize("name", sorted(INPUTS))
+def test_drivers(name):
+    dss = _dss()
+    chars = INPUTS[name]
+    p = 4
+    ss = O.StringSet(chars, p)
+    # merge sort with LCP compression: the rebuilt arena is unique
+    res = O.merge_sort(ss, encoder="lcp_compact")
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        ds.merge_sort(compressed=True)
+        gord, gml, gsv = ds.get_merged()
+        exp = b"".join(o["arena"].tobytes() for o in res["out"])
+        got, _ = ds.get_extended(len(exp))
+        assert got.tobytes() == exp
+        assert (gml == np.concatenate([o["lcp"] for o in res["out"]])).all()
+    # prefix doubling: permutation modulo ties among equal strings
+    res = O.prefix_doubling_sort(ss)
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        ds.prefix_doubling_sort()
+        si, pi = ds.get_permutation()
+        gperm, glcp = ds.get_sorted()
+    exp_pairs = []
+    for o in res["out"]:
+        exp_pairs += list(zip(map(int, o["pe_index"]), map(int, o["string_index"])))
+    got_pairs = list(zip(map(int, pi), map(int, si)))
+
+    def full(pair, perm):
+        pe, idx = pair
+        return ss.string(perm[int(ss.pe_begin[pe]) + idx])
+    gs = [full(x, gperm) for x in got_pairs]
+    es = [full(x, res["perm"]) for x in exp_pairs]
+    assert gs == es
+    eqm = np.zeros(len(gs), bool)
+    eqm[1:] = [gs[i] == gs[i - 1] for i in range(1, len(gs))]
+    assert canon(got_pairs, eqm) == canon(exp_pairs, eqm)
+
+
+def test_few_strings_many_pes():
+    dss = _dss()
+    chars = arena([b"b", b"a", b"c", b"a"])
+    ss = O.StringSet(chars, 8)
+    res = O.merge_sort(ss, encoder="lcp_compact")
+    with dss.DeviceStringSet.from_arena(chars, 8) as ds:
+        ds.merge_sort(compressed=True)
+        exp = b"".join(o["arena"].tobytes() for o in res["out"])
+        got, _ = ds.get_extended(len(exp))
+        assert got.tobytes() == exp
