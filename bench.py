@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark: distinguishing-prefix GB/s of the prefix-doubling distributed
+string sort on one MI355X.
+
+Workload (one "step"): DistributedPrefixDoublingSort over 16 virtual PEs on
+the D/N-ratio input, 20,000,000 strings x 500 chars, D/N = 0.5 (generated in
+HBM before timing; the strings stay resident): local LCP sort of every PE,
+prefix doubling (XXH64-stream hashes, hash range 2^64-1 unless --bloom-bits
+is given, doubling from depth 8), splitter sampling + bucket search, the
+distinguishing-prefix exchange, the LCP merge and the output permutation.
+
+value = D / (time per step), D = sum over the sorted output of
+max(lcp_i, lcp_{i+1}) + 1 (the reference's computeD), in GB/s.
+
+The JSON line also carries
+  roofline:     the dominant kernel's algorithmic bytes per launch over its
+                HIP-event-timed average duration, against MI355X's 8 TB/s;
+  cpu_baseline: the CPU oracle (a single-thread C restatement of the same
+                pipeline) timed on a bounded sample of the same workload.
+Only one GPU is used; --gpus N > 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "distributed-string-sorting_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# Algorithmic bytes per unit of work of each kernel family:
+# returned by the library as alg_bytes accumulated over launches.
+def roofline_from_stats(stats: dict):
+    if not stats:
+        return None
+    name = max(stats, key=lambda k: stats[k]["ms"])
+    s = stats[name]
+    if s["launches"] == 0 or s["ms"] <= 0:
+        return None
+    avg_ms = s["ms"] / s["launches"]
+    bytes_per_launch = s["alg_bytes"] / s["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {
+        "kernel": name,
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(avg_ms, 4),
+        "launches_per_step": None,
+        "alg_bytes_per_launch": int(bytes_per_launch),
+    }
+
+
+def cpu_baseline(n, length, dton, pes, seed, m):
+    """Times the CPU oracle's prefix-doubling sort on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    chars = O.gen_dton(n, length, dton, seed)
+    ss = O.StringSet(chars, pes)
+    t0 = time.perf_counter()
+    res = O.prefix_doubling_sort(ss, m=m)
+    dt = time.perf_counter() - t0
+    # D of the sorted sample: merged outputs plus recomputed PE-boundary LCPs
+    lcps = []
+    prev = None
+    for q, o in enumerate(res["out"]):
+        lc = [int(x) for x in o["lcp"]]
+        if lc:
+            first = ss.string(res["perm"][int(ss.pe_begin[int(o["pe_index"][0])]) + int(o["string_index"][0])])
+            if prev is not None:
+                k = 0
+                while k < min(len(prev), len(first)) and prev[k] == first[k]:
+                    k += 1
+                lc[0] = k
+            last_pe, last_si = int(o["pe_index"][-1]), int(o["string_index"][-1])
+            prev = ss.string(res["perm"][int(ss.pe_begin[last_pe]) + last_si])
+        lcps += lc
+    import numpy as np
+    D = O.compute_d(np.array(lcps, dtype=np.uint32))
+    return {
+        "value": round(D / dt / 1e9, 5),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"D/N generator, {n} strings x {length} chars, D/N {dton}, {pes} PEs, "
+                  f"same seed; single-thread C oracle of the whole prefix-doubling sort "
+                  f"({dt:.2f} s, D = {D})",
+        "strings_per_s": round(n / dt, 1),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=20_000_000)
+    ap.add_argument("--len", type=int, default=500)
+    ap.add_argument("--dton", type=float, default=0.5)
+    ap.add_argument("--pes", type=int, default=16)
+    ap.add_argument("--seed", type=lambda x: int(x, 0), default=0x5EED0002)
+    ap.add_argument("--sampler", default="IndexedNumStrings")
+    ap.add_argument("--bloom-bits", type=int, default=0,
+                    help="hash range = bits * n (the 'Bloom filter sized b bits/string' variant); "
+                         "0 = 2^64-1 as in the reference")
+    ap.add_argument("--cpu-sample", type=int, default=10_000_000,
+                    help="strings in the CPU-oracle sample (0 disables the CPU leg)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(p virtual PEs share the one MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    import dss_amd
+
+    m = (2**64 - 1) if args.bloom_bits == 0 else args.bloom_bits * args.n
+    log("device:", dss_amd.device_info())
+    t0 = time.time()
+    ds = dss_amd.DeviceStringSet.dton(args.n, args.len, args.dton, args.seed, args.pes)
+    ds.sync()
+    info = ds.info()
+    log(f"generated {info['num_strings']} strings, {info['num_bytes']} bytes in {time.time() - t0:.2f} s")
+
+    for w in range(args.warmup):
+        ds.prefix_doubling_sort(args.sampler, m)
+        ds.sync()
+        log(f"warmup {w} done")
+    chk = ds.check()
+    log("check:", chk)
+    if not (chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0):
+        log("output check failed")
+        sys.exit(1)
+    D = chk["D"]
+
+    ds.kernel_stats(reset=True)
+    ds.set_timing(True)
+    ds.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ds.prefix_doubling_sort(args.sampler, m)
+    ds.sync()
+    t1 = time.perf_counter()
+    ds.set_timing(False)
+    stats = ds.kernel_stats(reset=True)
+    step_s = (t1 - t0) / args.steps
+    for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"]):
+        log(f"  {k:18s} launches={v['launches']:5d} ms/step={v['ms'] / args.steps:8.3f} "
+            f"GB/s={(v['alg_bytes'] / (v['ms'] * 1e-3) / 1e9) if v['ms'] > 0 else 0:8.1f}")
+    roof = roofline_from_stats(stats)
+    if roof:
+        roof["launches_per_step"] = stats[roof["kernel"]]["launches"] / args.steps
+
+    cpu = None
+    if args.cpu_sample > 0:
+        log(f"CPU oracle on {args.cpu_sample} strings ...")
+        cpu = cpu_baseline(args.cpu_sample, args.len, args.dton, args.pes, args.seed, m)
+        cpu["cores"] = 1
+
+    out = {
+        "metric": "distinguishing-prefix GB/s at 1 GPU (prefix-doubling distributed string sort)",
+        "value": round(D / step_s / 1e9, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded D/N-ratio generator, generated in HBM)",
+        "config": {
+            "workload": f"D/N-ratio input, {args.n} strings x {args.len} chars, D/N {args.dton}, "
+                        f"{args.pes} virtual PEs, prefix-doubling sort (XXH64-stream, "
+                        f"hash range {'2^64-1' if args.bloom_bits == 0 else f'{args.bloom_bits}*n'}, "
+                        f"start depth 8), sampler {args.sampler}",
+            "strings": args.n,
+            "string_length": args.len,
+            "d_to_n": args.dton,
+            "virtual_pes": args.pes,
+            "D_bytes": D,
+            "strings_per_s": round(args.n / step_s, 1),
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

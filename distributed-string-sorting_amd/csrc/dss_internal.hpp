@@ -97,6 +97,15 @@ struct KernelTimer {
     void begin(hipStream_t st, const char* name, double alg_bytes, Rec& r);
     void end(hipStream_t st, Rec& r);
     void collect();  // after a stream sync
+    // Replaces the algorithmic bytes of the most recent record of `name`
+    // (for byte counts known only after the launch).
+    void amend_last(const char* name, double bytes) {
+        for (auto it = pending.rbegin(); it != pending.rend(); ++it)
+            if (it->name == name) {
+                it->bytes = bytes;
+                return;
+            }
+    }
     ~KernelTimer();
 };
 
@@ -134,7 +143,7 @@ struct PipeState {
     DevBuf dist;     // u32[n] distinguishing-prefix estimates (results)
     DevBuf rounds;   // u8[n]  rounds a position was a candidate in
     uint32_t pd_rounds = 0;
-    DevBuf pd_cand, pd_next, pd_hash, pd_flags, pd_table, pd_tflag, pd_ctr;
+    DevBuf pd_cand, pd_next, pd_hash, pd_flags, pd_table, pd_tflag, pd_ctr, pd_slot;
     // partition
     bool partitioned = false;
     std::vector<uint64_t> interval;  // p*p, [r*p+q] = strings PE r sends to PE q

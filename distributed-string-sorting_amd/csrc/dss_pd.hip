@@ -118,39 +118,108 @@ __global__ void k_pd_init(uint8_t* __restrict__ cand, uint8_t* __restrict__ roun
     }
 }
 
+// Four lanes per position: lane j of the group accumulates XXH64 lane v_j+1
+// over the 32-byte stripes (all hashed prefixes of a round have the same
+// length, so the stripe loop is uniform); lane 0 merges and finishes the
+// tail.  Cross-lane reads happen with every lane of the wave active.
+// Sum of a per-thread count over the block, added to *out with one atomic.
+__device__ inline void block_count_add(uint32_t v, unsigned long long* out) {
+    __shared__ uint32_t part[4];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    if (lane_id() == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(out, (unsigned long long)t);
+    }
+}
+
+constexpr uint32_t kPdGrid = 4096;  // blocks of the grid-stride PD kernels
+
+// Four lanes per position: lane j of the group accumulates XXH64 lane v_j+1
+// over the 32-byte stripes (all hashed prefixes of a round have the same
+// length, so the stripe loop is uniform); lane 0 merges and finishes the
+// tail.  The grid-stride loop is block-uniform, so every cross-lane read
+// happens with all lanes of the wave active.
 __global__ void __launch_bounds__(256)
 k_pd_classify(const uint8_t* __restrict__ cand, const uint64_t* __restrict__ pe_begin, uint32_t p,
               const uint32_t* __restrict__ perm, const uint32_t* __restrict__ lcp, const uint64_t* __restrict__ off,
               const uint32_t* __restrict__ len, const uint8_t* __restrict__ chars, uint32_t depth, uint64_t m,
               uint64_t n, uint8_t* __restrict__ flags, uint64_t* __restrict__ hashv, uint8_t* __restrict__ rounds,
               uint32_t* __restrict__ dist, unsigned long long* __restrict__ nhashed) {
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t hashed = 0;
-    if (c < n) {
+    const uint32_t j = threadIdx.x & 3;
+    const uint32_t base_lane = threadIdx.x & ~3u;
+    uint32_t counted = 0;
+    for (uint64_t blk = (uint64_t)blockIdx.x * 64; blk < n; blk += (uint64_t)gridDim.x * 64) {
+        const uint64_t c = blk + (threadIdx.x >> 2);
         uint8_t f = 0;
-        if (cand[c]) {
-            rounds[c] = rounds[c] + (rounds[c] < 255);
+        uint64_t so = 0;
+        if (c < n && cand[c]) {
             const uint32_t id = perm[c];
             const uint32_t L = len[id];
-            if (depth > L) {
-                dist[c] = L;  // end of string
-            } else {
-                dist[c] = depth;
+            if (depth <= L) {
                 const uint32_t r = pe_of(pe_begin, p, c);
-                if (c > pe_begin[r] && cand[c - 1] && lcp[c] >= depth) {
-                    f = kLcpDup;
-                } else {
+                if (c > pe_begin[r] && cand[c - 1] && lcp[c] >= depth) f = kLcpDup;
+                else {
                     f = kHashed;
-                    hashed = 1;
-                    hashv[c] = xxh64_stream(chars, off[id], depth) % m;
+                    so = off[id];
                 }
             }
+            if (j == 0) {
+                rounds[c] = rounds[c] + (rounds[c] < 255);
+                dist[c] = depth > L ? L : depth;  // end of string: results = len
+                flags[c] = f;
+            }
+        } else if (c < n && j == 0) {
+            flags[c] = 0;
         }
-        flags[c] = f;
+        const bool hashed = f == kHashed;
+        uint64_t v = j == 0 ? P1 + P2 : (j == 1 ? P2 : (j == 2 ? 0ull : (uint64_t)0 - P1));
+        const uint32_t stripes = depth >= 32 ? depth / 32 : 0;
+        for (uint32_t t = 0; t < stripes; ++t)
+            if (hashed) v = xround(v, load_word_le(chars, so + (uint64_t)t * 32 + 8 * j));
+        const uint64_t v1 = __shfl(v, base_lane + 0);
+        const uint64_t v2 = __shfl(v, base_lane + 1);
+        const uint64_t v3 = __shfl(v, base_lane + 2);
+        const uint64_t v4 = __shfl(v, base_lane + 3);
+        if (hashed && j == 0) {
+            uint64_t h;
+            if (depth > 32) {
+                h = rotl64(v1, 1) + rotl64(v2, 7) + rotl64(v3, 12) + rotl64(v4, 18);
+                h = xmerge(h, v1);
+                h = xmerge(h, v2);
+                h = xmerge(h, v3);
+                h = xmerge(h, v4);
+            } else {
+                h = v3 + P5;  // includes depth == 32, where v3 consumed one stripe
+            }
+            h += depth;
+            uint32_t q = stripes * 32;
+            while (q + 8 <= depth) {
+                h ^= xround(0, load_word_le(chars, so + q));
+                h = rotl64(h, 27) * P1 + P4;
+                q += 8;
+            }
+            if (q + 4 <= depth) {
+                h ^= (load_word_le(chars, so + q) & 0xFFFFFFFFull) * P1;
+                h = rotl64(h, 23) * P2 + P3;
+                q += 4;
+            }
+            while (q < depth) {
+                h ^= (uint64_t)chars[so + q] * P5;
+                h = rotl64(h, 11) * P1;
+                ++q;
+            }
+            h ^= h >> 33;
+            h *= P2;
+            h ^= h >> 29;
+            h *= P3;
+            h ^= h >> 32;
+            hashv[c] = h % m;
+            ++counted;
+        }
     }
-    // one atomic per wave
-    const uint64_t b = __ballot(hashed);
-    if (lane_id() == 0 && b) atomicAdd(nhashed, (unsigned long long)__popcll(b));
+    block_count_add(counted, nhashed);
 }
 
 __global__ void k_fill_u64(uint64_t* __restrict__ a, uint64_t n, uint64_t v) {
@@ -165,7 +234,8 @@ __device__ inline uint64_t slot_of(uint64_t h, uint64_t mask) {
 }
 
 __global__ void k_pd_insert(const uint8_t* __restrict__ flags, const uint64_t* __restrict__ hashv, uint64_t n,
-                            uint64_t* __restrict__ keys, uint8_t* __restrict__ dup, uint64_t mask) {
+                            uint64_t* __restrict__ keys, uint8_t* __restrict__ dup, uint64_t mask,
+                            uint32_t* __restrict__ slot) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n || flags[c] != kHashed) return;
     const uint64_t h = hashv[c];
@@ -173,47 +243,35 @@ __global__ void k_pd_insert(const uint8_t* __restrict__ flags, const uint64_t* _
     for (uint64_t probe = 0; probe <= mask; ++probe) {
         const unsigned long long old =
             atomicCAS(reinterpret_cast<unsigned long long*>(&keys[s]), (unsigned long long)kEmpty, (unsigned long long)h);
-        if (old == kEmpty) return;
+        if (old == kEmpty) break;
         if (old == h) {
             dup[s] = 1;
-            return;
+            break;
         }
         s = (s + 1) & mask;
     }
+    slot[c] = (uint32_t)s;
 }
 
 __global__ void __launch_bounds__(256)
-k_pd_next(const uint8_t* __restrict__ cand, const uint8_t* __restrict__ flags, const uint64_t* __restrict__ hashv,
-          const uint64_t* __restrict__ pe_begin, uint32_t p, const uint32_t* __restrict__ lcp, uint32_t depth, uint64_t n,
-          const uint64_t* __restrict__ keys, const uint8_t* __restrict__ dup, uint64_t mask, uint8_t* __restrict__ next,
-          unsigned long long* __restrict__ ncand) {
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t keep = 0;
-    if (c < n) {
+k_pd_next(const uint8_t* __restrict__ cand, const uint8_t* __restrict__ flags, const uint64_t* __restrict__ pe_begin,
+          uint32_t p, const uint32_t* __restrict__ lcp, uint32_t depth, uint64_t n, const uint8_t* __restrict__ dup,
+          const uint32_t* __restrict__ slot, uint8_t* __restrict__ next, unsigned long long* __restrict__ ncand) {
+    uint32_t kept = 0;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t keep = 0;
         const uint8_t f = flags[c];
         if (f == kLcpDup) {
             keep = 1;
         } else if (f == kHashed) {
             const uint32_t r = pe_of(pe_begin, p, c);
             if (c + 1 < pe_begin[r + 1] && cand[c + 1] && lcp[c + 1] >= depth) keep = 1;  // LCP-local root
-            if (!keep) {
-                const uint64_t h = hashv[c];
-                uint64_t s = slot_of(h, mask);
-                for (uint64_t probe = 0; probe <= mask; ++probe) {
-                    const uint64_t k = keys[s];
-                    if (k == h) {
-                        keep = dup[s];
-                        break;
-                    }
-                    if (k == kEmpty) break;
-                    s = (s + 1) & mask;
-                }
-            }
+            else keep = dup[slot[c]];
         }
         next[c] = (uint8_t)keep;
+        kept += keep;
     }
-    const uint64_t b = __ballot(keep);
-    if (lane_id() == 0 && b) atomicAdd(ncand, (unsigned long long)__popcll(b));
+    block_count_add(kept, ncand);
 }
 
 void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
@@ -231,6 +289,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     P.pd_next.reserve(nn);
     P.pd_flags.reserve(nn);
     P.pd_hash.reserve(nn * 8);
+    P.pd_slot.reserve(nn * 4);
     P.pd_ctr.reserve(64);
     uint8_t* cand = P.pd_cand.as<uint8_t>();
     uint8_t* next = P.pd_next.as<uint8_t>();
@@ -242,10 +301,11 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     uint64_t depth = start_depth, step = (uint64_t)start_depth * 2;
     while (ncand > 0) {
         DSS_REQUIRE(depth < 0xFFFFFFFFull, DSS_ERANGE, "prefix doubling depth overflow");
+        DSS_REQUIRE(2 * ncand < (1ull << 32), DSS_ERANGE, "too many candidates for the hash table");
         DSS_HIP(hipMemsetAsync(ctr, 0, 16, st));
         {
             KScope ks(s->timer, st, "pd_hash", 0.0);
-            hipLaunchKernelGGL(k_pd_classify, grid, blk, 0, st, cand, s->pe_begin_d.as<uint64_t>(), p,
+            hipLaunchKernelGGL(k_pd_classify, dim3(std::min<uint64_t>((n + 63) / 64, kPdGrid)), blk, 0, st, cand, s->pe_begin_d.as<uint64_t>(), p,
                                s->perm.as<uint32_t>(), s->lcp.as<uint32_t>(), s->off.as<uint64_t>(),
                                s->len.as<uint32_t>(), s->chars.as<uint8_t>(), (uint32_t)depth, m, n,
                                P.pd_flags.as<uint8_t>(), P.pd_hash.as<uint64_t>(), P.rounds.as<uint8_t>(),
@@ -254,20 +314,23 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
         uint64_t nh = 0;
         DSS_HIP(hipMemcpyAsync(&nh, ctr, 8, hipMemcpyDeviceToHost, st));
         DSS_HIP(hipStreamSynchronize(st));
+        // every position: candidate flag, flags/rounds/dist; every candidate:
+        // id, offset, length, lcp; every hashed candidate: depth chars + hash
+        s->timer.amend_last("pd_hash", (double)n * 8.0 + (double)ncand * 24.0 + (double)nh * (depth + 8.0));
         uint64_t T = 1024;
         while (T < 2 * nh) T <<= 1;
         P.pd_table.reserve(T * 8);
         P.pd_tflag.reserve(T);
         {
-            KScope ks(s->timer, st, "pd_dedup", 0.0);
+            KScope ks(s->timer, st, "pd_dedup", (double)T * 9.0 + (double)n * 3.0 + (double)nh * 40.0);
             hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T, kEmpty);
             DSS_HIP(hipMemsetAsync(P.pd_tflag.p, 0, T, st));
             if (nh)
                 hipLaunchKernelGGL(k_pd_insert, grid, blk, 0, st, P.pd_flags.as<uint8_t>(), P.pd_hash.as<uint64_t>(), n,
-                                   P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(), T - 1);
-            hipLaunchKernelGGL(k_pd_next, grid, blk, 0, st, cand, P.pd_flags.as<uint8_t>(), P.pd_hash.as<uint64_t>(),
-                               s->pe_begin_d.as<uint64_t>(), p, s->lcp.as<uint32_t>(), (uint32_t)depth, n,
-                               P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(), T - 1, next, ctr + 1);
+                                   P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(), T - 1, P.pd_slot.as<uint32_t>());
+            hipLaunchKernelGGL(k_pd_next, dim3(std::min<uint64_t>((n + 255) / 256, kPdGrid)), blk, 0, st, cand, P.pd_flags.as<uint8_t>(), s->pe_begin_d.as<uint64_t>(),
+                               p, s->lcp.as<uint32_t>(), (uint32_t)depth, n, P.pd_tflag.as<uint8_t>(),
+                               P.pd_slot.as<uint32_t>(), next, ctr + 1);
         }
         DSS_HIP(hipGetLastError());
         DSS_HIP(hipMemcpyAsync(&ncand, ctr + 1, 8, hipMemcpyDeviceToHost, st));

@@ -412,12 +412,14 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
     r_from[dst] = from;
 }
 
-// 16 lanes per received string copy its bytes into the receive arena.
+// 2^lg lanes per received string copy its bytes into the receive arena.
 __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* __restrict__ r_voff,
                             const uint32_t* __restrict__ r_vlen, const uint32_t* __restrict__ r_from,
-                            const uint64_t* __restrict__ byte_off, int mode, uint64_t n, uint8_t* __restrict__ out) {
-    const uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
-    const uint32_t l = threadIdx.x & 15;
+                            const uint64_t* __restrict__ byte_off, int mode, uint64_t n, uint32_t lg,
+                            uint8_t* __restrict__ out) {
+    const uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lg;
+    const uint32_t lanes = 1u << lg;
+    const uint32_t l = threadIdx.x & (lanes - 1);
     if (k >= n) return;
     const uint8_t* src = chars + r_voff[k];
     const uint32_t from = r_from[k];
@@ -425,7 +427,7 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
     uint8_t* dst = out + byte_off[k];
     // modes 0/1: chars [from, vlen] incl. the NUL; mode 2: [from, vlen) + NUL
     const uint32_t m = vlen - from;
-    for (uint32_t t = l; t <= m; t += 16) dst[t] = (t == m) ? 0 : src[from + t];
+    for (uint32_t t = l; t <= m; t += lanes) dst[t] = (t == m) ? 0 : src[from + t];
 }
 
 void exchange(dss_set_s* s, int mode) {
@@ -487,10 +489,12 @@ void exchange(dss_set_s* s, int mode) {
     DSS_HIP(hipMemsetAsync(P.recv_bytes.as<uint8_t>() + total, 0, kArenaPad, st));
     if (n) {
         KScope ks(s->timer, st, "exchange_copy", 2.0 * total + n * 24.0);
-        const uint64_t threads = n * 16;
+        // short prefix-doubling prefixes: 4 lanes per string, whole strings: 16
+        const uint32_t lg = (mode == 2 || total < 24 * n) ? 2 : 4;
+        const uint64_t threads = n << lg;
         hipLaunchKernelGGL(k_exch_copy, dim3((threads + 255) / 256), dim3(256), 0, st, s->chars.as<uint8_t>(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
-                           P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), mode, n,
+                           P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), mode, n, lg,
                            P.recv_bytes.as<uint8_t>());
     }
     DSS_HIP(hipGetLastError());
@@ -522,12 +526,14 @@ void exchange(dss_set_s* s, int mode) {
 // compared from that LCP on.  Character comparisons first use a cached 8-char
 // key at the receiver's common prefix.  The merged LCP array is exact; the
 // saved LCP of an output is the run LCP it was received with (oldLcps).
-constexpr uint32_t kW = 8;
+constexpr uint32_t kMT = 256;               // threads per merge block
+constexpr uint32_t kMW = 4;                 // outputs per thread
+constexpr uint32_t kTileOut = kMT * kMW;    // outputs per tile
 
 struct MergePair {
     uint64_t out;      // first receive position of the pair (A then B)
     uint64_t na, nb;   // run lengths
-    uint64_t chunk0;   // first thread chunk of the pair
+    uint64_t tile0;    // first tile of the pair
     uint32_t recv;     // receiving PE
     uint32_t pad;
 };
@@ -536,15 +542,15 @@ struct ViewCtx {
     const uint8_t* chars;
     const uint64_t* voff;
     const uint32_t* vlen;
-    const uint64_t* keyc;
     uint32_t cp;
 };
 
-// Compare received strings a, b (receive indices) knowing they agree on the
-// first `from` (>= cp) characters; returns sign and LCP.
-__device__ inline int elem_cmp(const ViewCtx& V, uint32_t a, uint32_t b, uint32_t from, uint32_t& lcp) {
+// Compares received strings a and b (receive indices, cached keys ka, kb at
+// the receiver's common prefix cp) known to agree on the first `from` >= cp
+// characters; returns the sign and the LCP.
+__device__ inline int elem_cmp(const ViewCtx& V, uint32_t a, uint64_t ka, uint32_t b, uint64_t kb, uint32_t from,
+                               uint32_t& lcp) {
     if (from < V.cp + 8) {
-        const uint64_t ka = V.keyc[a], kb = V.keyc[b];
         if (ka != kb) {
             lcp = V.cp + key_lcp(ka, kb);
             return ka < kb ? -1 : 1;
@@ -558,28 +564,23 @@ __device__ inline int elem_cmp(const ViewCtx& V, uint32_t a, uint32_t b, uint32_
     return view_cmp(V.chars, V.voff[a], V.vlen[a], V.voff[b], V.vlen[b], from, &lcp);
 }
 
+// Common prefix of all strings a PE received: min over its runs r of the
+// LCP of the PE's first string with the first and last string of run r.
 __global__ void k_recv_cp(const uint64_t* __restrict__ run_off, const uint64_t* __restrict__ interval, uint32_t p,
-                          const uint8_t* __restrict__ chars, const uint64_t* __restrict__ voff,
-                          const uint32_t* __restrict__ vlen, uint32_t* __restrict__ cp_out) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= p) return;
-    uint32_t cp = 0xFFFFFFFFu;
-    int64_t s0 = -1;
-    for (uint32_t r = 0; r < p; ++r) {
-        const uint64_t c = interval[(uint64_t)r * p + q];
-        if (!c) continue;
-        const uint64_t f = run_off[(uint64_t)q * p + r], l = f + c - 1;
-        if (s0 < 0) {
-            s0 = (int64_t)f;
-            cp = vlen[f];
-        }
-        uint32_t x;
-        view_cmp(chars, voff[s0], vlen[s0], voff[f], vlen[f], 0, &x);
-        cp = min(cp, x);
-        view_cmp(chars, voff[s0], vlen[s0], voff[l], vlen[l], 0, &x);
-        cp = min(cp, x);
-    }
-    cp_out[q] = s0 < 0 ? 0 : cp;
+                          const uint64_t* __restrict__ recv_begin, const uint8_t* __restrict__ chars,
+                          const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
+                          uint32_t* __restrict__ cp_out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)p * p) return;
+    const uint32_t q = (uint32_t)(t / p), r = (uint32_t)(t % p);
+    const uint64_t c = interval[(uint64_t)r * p + q];
+    if (!c) return;
+    const uint64_t s0 = recv_begin[q];
+    const uint64_t f = run_off[(uint64_t)q * p + r], l = f + c - 1;
+    uint32_t x, y;
+    view_cmp(chars, voff[s0], vlen[s0], voff[f], vlen[f], 0, &x);
+    view_cmp(chars, voff[s0], vlen[s0], voff[l], vlen[l], 0, &y);
+    atomicMin(&cp_out[q], min(x, y));
 }
 
 __global__ void k_keycache(const uint64_t* __restrict__ recv_begin, uint32_t p, const uint8_t* __restrict__ chars,
@@ -595,78 +596,145 @@ __global__ void k_keycache(const uint64_t* __restrict__ recv_begin, uint32_t p, 
     keyc[k] = load_key8(chars, voff[k], vlen[k], cp[lo - 1]);
 }
 
-__global__ void __launch_bounds__(256)
-k_merge_pairs(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t nchunks, const uint32_t* __restrict__ ord,
-              const uint32_t* __restrict__ lcpa, uint32_t* __restrict__ ord_out, uint32_t* __restrict__ lcp_out,
-              const uint8_t* __restrict__ chars, const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
-              const uint64_t* __restrict__ keyc, const uint32_t* __restrict__ rcp) {
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nchunks) return;
-    uint32_t lo = 0, hi = npairs;  // pair of chunk c
+__device__ inline uint32_t pair_of_tile(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t t) {
+    uint32_t lo = 0, hi = npairs;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) / 2;
-        if (pairs[mid].chunk0 <= c) lo = mid + 1; else hi = mid;
+        if (pairs[mid].tile0 <= t) lo = mid + 1; else hi = mid;
     }
-    const MergePair Pr = pairs[lo - 1];
-    const ViewCtx V{chars, voff, vlen, keyc, rcp[Pr.recv]};
+    return lo - 1;
+}
+
+// Merge-path split (number of A elements among the first d outputs, A wins
+// ties) at the start of every tile, from global memory.
+__global__ void k_merge_split(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t ntiles,
+                              const uint32_t* __restrict__ ord, const uint8_t* __restrict__ chars,
+                              const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
+                              const uint64_t* __restrict__ keyc, const uint32_t* __restrict__ rcp,
+                              uint64_t* __restrict__ split) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const MergePair Pr = pairs[pair_of_tile(pairs, npairs, t)];
+    const ViewCtx V{chars, voff, vlen, rcp[Pr.recv]};
+    const uint64_t A = Pr.out, B = Pr.out + Pr.na;
+    const uint64_t d = (t - Pr.tile0) * kTileOut;
+    uint64_t lo = d > Pr.nb ? d - Pr.nb : 0, hi = d < Pr.na ? d : Pr.na;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        const uint32_t a = ord[A + mid], b = ord[B + d - mid - 1];
+        uint32_t x;
+        if (elem_cmp(V, a, keyc[a], b, keyc[b], V.cp, x) <= 0) lo = mid + 1;
+        else hi = mid;
+    }
+    split[t] = lo;
+}
+
+// One tile of kTileOut outputs per block: the tile's windows of A and B
+// (plus the element before each) are staged in LDS; every thread finds its
+// start by a merge-path search in LDS and merges kMW outputs with the LCP
+// rules of the loser tree (bingmann-lcp_losertree.hpp:280-333): the head
+// with the larger LCP to the last output is smaller, equal LCPs compare
+// characters from there on.
+__global__ void __launch_bounds__(kMT)
+k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_t* __restrict__ split,
+             const uint32_t* __restrict__ ord, const uint32_t* __restrict__ lcpa, uint32_t* __restrict__ ord_out,
+             uint32_t* __restrict__ lcp_out, const uint8_t* __restrict__ chars, const uint64_t* __restrict__ voff,
+             const uint32_t* __restrict__ vlen, const uint64_t* __restrict__ keyc, const uint32_t* __restrict__ rcp) {
+    __shared__ uint32_t sAo[kTileOut + 1], sAl[kTileOut + 1], sBo[kTileOut + 1], sBl[kTileOut + 1];
+    __shared__ uint64_t sAk[kTileOut + 1], sBk[kTileOut + 1];
+    const uint64_t t = blockIdx.x;
+    const uint32_t pi = pair_of_tile(pairs, npairs, t);
+    const MergePair Pr = pairs[pi];
+    const ViewCtx V{chars, voff, vlen, rcp[Pr.recv]};
     const uint64_t A = Pr.out, B = Pr.out + Pr.na;
     const uint64_t na = Pr.na, nb = Pr.nb, tot = na + nb;
-    const uint64_t d = (c - Pr.chunk0) * kW;
-    const uint64_t dend = min(d + kW, tot);
-    // merge path: i = number of A elements among the first d outputs (A wins ties)
-    uint64_t l0 = d > nb ? d - nb : 0, h0 = d < na ? d : na;
-    while (l0 < h0) {
-        const uint64_t mid = (l0 + h0) / 2;
-        uint32_t x;
-        if (elem_cmp(V, ord[A + mid], ord[B + d - mid - 1], V.cp, x) <= 0) l0 = mid + 1;
-        else h0 = mid;
+    const uint64_t d0 = (t - Pr.tile0) * kTileOut;
+    const uint64_t d1 = min(d0 + kTileOut, tot);
+    const uint64_t i0 = split[t];
+    const uint64_t i1 = (d1 == tot) ? na : split[t + 1];
+    const uint64_t j0 = d0 - i0, j1 = d1 - i1;
+    const uint32_t la_n = (uint32_t)(i1 - i0), lb_n = (uint32_t)(j1 - j0);
+    // stage A[i0-1 .. i1) and B[j0-1 .. j1) at LDS index (global - start + 1)
+    for (uint32_t k = threadIdx.x; k <= la_n; k += kMT) {
+        const int64_t g = (int64_t)i0 - 1 + k;
+        if (g >= 0 && (uint64_t)g < na) {
+            const uint32_t o = ord[A + g];
+            sAo[k] = o;
+            sAl[k] = lcpa[A + g];
+            sAk[k] = keyc[o];
+        }
     }
-    uint64_t i = l0, j = d - l0;
-    // LCPs of the heads with the previous output
-    uint32_t la = 0, lb = 0;
-    if (d > 0) {
+    for (uint32_t k = threadIdx.x; k <= lb_n; k += kMT) {
+        const int64_t g = (int64_t)j0 - 1 + k;
+        if (g >= 0 && (uint64_t)g < nb) {
+            const uint32_t o = ord[B + g];
+            sBo[k] = o;
+            sBl[k] = lcpa[B + g];
+            sBk[k] = keyc[o];
+        }
+    }
+    __syncthreads();
+    const uint32_t dl = threadIdx.x * kMW;
+    if (d0 + dl >= d1) return;
+    // local merge path
+    uint32_t lo = dl > lb_n ? dl - lb_n : 0, hi = dl < la_n ? dl : la_n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        const uint32_t bi = dl - mid - 1;
+        uint32_t x;
+        if (elem_cmp(V, sAo[mid + 1], sAk[mid + 1], sBo[bi + 1], sBk[bi + 1], V.cp, x) <= 0) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t il = lo, jl = dl - lo;          // local indices; LDS index = local + 1
+    const uint64_t gi = i0 + il, gj = j0 + jl;  // global indices in A and B
+    uint32_t la = V.cp, lb = V.cp;
+    if (d0 + dl > 0) {
         bool prev_is_a;
         uint32_t x;
-        if (i == 0) prev_is_a = false;
-        else if (j == 0) prev_is_a = true;
-        else prev_is_a = elem_cmp(V, ord[A + i - 1], ord[B + j - 1], V.cp, x) > 0;
-        const uint32_t prev = prev_is_a ? ord[A + i - 1] : ord[B + j - 1];
-        if (i < na) {
-            if (prev_is_a) la = lcpa[A + i];
-            else elem_cmp(V, ord[A + i], prev, V.cp, la);
+        if (gi == 0) prev_is_a = false;
+        else if (gj == 0) prev_is_a = true;
+        else prev_is_a = elem_cmp(V, sAo[il], sAk[il], sBo[jl], sBk[jl], V.cp, x) > 0;
+        const uint32_t po = prev_is_a ? sAo[il] : sBo[jl];
+        const uint64_t pk = prev_is_a ? sAk[il] : sBk[jl];
+        // heads outside this tile's windows are never compared in this tile
+        if (il < la_n) {
+            if (prev_is_a) la = sAl[il + 1];
+            else elem_cmp(V, sAo[il + 1], sAk[il + 1], po, pk, V.cp, la);
         }
-        if (j < nb) {
-            if (!prev_is_a) lb = lcpa[B + j];
-            else elem_cmp(V, ord[B + j], prev, V.cp, lb);
+        if (jl < lb_n) {
+            if (!prev_is_a) lb = sBl[jl + 1];
+            else elem_cmp(V, sBo[jl + 1], sBk[jl + 1], po, pk, V.cp, lb);
         }
-    } else {
-        la = lb = V.cp;
     }
-    for (uint64_t o = d; o < dend; ++o) {
-        uint32_t take_a;
-        uint32_t lout;
-        if (i >= na) take_a = 0;
-        else if (j >= nb) take_a = 1;
-        else if (la > lb) take_a = 1;
-        else if (lb > la) take_a = 0;
+    const uint64_t oend = min((uint64_t)dl + kMW, d1 - d0);
+    for (uint64_t o = dl; o < oend; ++o) {
+        const bool has_a = il < la_n, has_b = jl < lb_n;
+        bool take_a;
+        if (!has_a) take_a = false;
+        else if (!has_b) take_a = true;
+        else if (la > lb) take_a = true;
+        else if (lb > la) take_a = false;
         else {
             uint32_t x;
-            const int cr = elem_cmp(V, ord[A + i], ord[B + j], la, x);
+            const int cr = elem_cmp(V, sAo[il + 1], sAk[il + 1], sBo[jl + 1], sBk[jl + 1], la, x);
             take_a = cr <= 0;
             if (take_a) lb = x; else la = x;
         }
+        uint32_t lout, outo;
         if (take_a) {
             lout = la;
-            ord_out[A + o] = ord[A + i];
-            ++i;
-            if (i < na) la = lcpa[A + i];
+            outo = sAo[il + 1];
+            ++il;
+            if (il < la_n) la = sAl[il + 1];
         } else {
             lout = lb;
-            ord_out[A + o] = ord[B + j];
-            ++j;
-            if (j < nb) lb = lcpa[B + j];
+            outo = sBo[jl + 1];
+            ++jl;
+            if (jl < lb_n) lb = sBl[jl + 1];
         }
-        lcp_out[A + o] = (o == 0) ? 0u : lout;
+        const uint64_t go = d0 + o;
+        ord_out[A + go] = outo;
+        lcp_out[A + go] = (go == 0) ? 0u : lout;
     }
 }
 
@@ -709,14 +777,16 @@ void merge(dss_set_s* s) {
     uint32_t* lc2 = P.m_lcp2.as<uint32_t>();
     if (n) {
         hipLaunchKernelGGL(k_iota_copy, dim3((n + 255) / 256), dim3(256), 0, st, ord, lc, P.recv_lcp.as<uint32_t>(), n);
-        hipLaunchKernelGGL(k_recv_cp, dim3((p + 63) / 64), dim3(64), 0, st, P.layout.as<uint64_t>() + (uint64_t)p * p,
-                           P.interval_d.as<uint64_t>(), p, s->chars.as<uint8_t>(), P.recv_view_off.as<uint64_t>(),
-                           P.recv_view_len.as<uint32_t>(), cp);
-        KScope ks(s->timer, st, "merge_keycache", n * 20.0);
+        DSS_HIP(hipMemsetAsync(cp, 0xFF, (p + 1) * 4, st));
+        hipLaunchKernelGGL(k_recv_cp, dim3(((uint64_t)p * p + 255) / 256), dim3(256), 0, st,
+                           P.layout.as<uint64_t>() + (uint64_t)p * p, P.interval_d.as<uint64_t>(), p, rb_d,
+                           s->chars.as<uint8_t>(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(), cp);
+        KScope ks(s->timer, st, "merge_keycache", n * 24.0);
         hipLaunchKernelGGL(k_keycache, dim3((n + 255) / 256), dim3(256), 0, st, rb_d, p, s->chars.as<uint8_t>(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(), cp, n, P.m_keyc.as<uint64_t>());
     }
-    // runs per receiver (host), then pairwise rounds
+    // runs per receiver (host), then pairwise rounds; receivers that already
+    // hold a single run are carried along as one-run pairs
     std::vector<std::vector<std::pair<uint64_t, uint64_t>>> runs(p);
     for (uint32_t q = 0; q < p; ++q) {
         uint64_t pos = P.recv_begin[q];
@@ -728,13 +798,13 @@ void merge(dss_set_s* s) {
     }
     std::vector<MergePair> pairs;
     while (true) {
-        pairs.clear();
-        uint64_t chunks = 0;
         bool any = false;
+        for (uint32_t q = 0; q < p; ++q) any |= runs[q].size() > 1;
+        if (!any) break;
+        pairs.clear();
+        uint64_t tiles = 0;
         for (uint32_t q = 0; q < p; ++q) {
             auto& R = runs[q];
-            if (R.size() <= 1) continue;
-            any = true;
             std::vector<std::pair<uint64_t, uint64_t>> next;
             for (size_t k = 0; k < R.size(); k += 2) {
                 MergePair mp;
@@ -743,39 +813,26 @@ void merge(dss_set_s* s) {
                 mp.nb = k + 1 < R.size() ? R[k + 1].second : 0;
                 mp.recv = q;
                 mp.pad = 0;
-                mp.chunk0 = chunks;
-                chunks += (mp.na + mp.nb + kW - 1) / kW;
+                mp.tile0 = tiles;
+                tiles += (mp.na + mp.nb + kTileOut - 1) / kTileOut;
                 pairs.push_back(mp);
                 next.push_back({mp.out, mp.na + mp.nb});
             }
             R.swap(next);
         }
-        if (!any) break;
-        // receivers that already have a single run also need their data in
-        // the output buffers: add them as one-run pairs
-        for (uint32_t q = 0; q < p; ++q) {
-            bool listed = false;
-            for (auto& mp : pairs) listed |= mp.recv == q;
-            if (!listed && runs[q].size() == 1) {
-                MergePair mp;
-                mp.out = runs[q][0].first;
-                mp.na = runs[q][0].second;
-                mp.nb = 0;
-                mp.recv = q;
-                mp.pad = 0;
-                mp.chunk0 = chunks;
-                chunks += (mp.na + kW - 1) / kW;
-                pairs.push_back(mp);
-            }
-        }
-        P.m_work.reserve(pairs.size() * sizeof(MergePair));
-        DSS_HIP(hipMemcpyAsync(P.m_work.p, pairs.data(), pairs.size() * sizeof(MergePair), hipMemcpyHostToDevice, st));
+        P.m_work.reserve(pairs.size() * sizeof(MergePair) + (tiles + 1) * 8);
+        MergePair* dpairs = P.m_work.as<MergePair>();
+        uint64_t* split = reinterpret_cast<uint64_t*>(P.m_work.as<uint8_t>() + pairs.size() * sizeof(MergePair));
+        DSS_HIP(hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(MergePair), hipMemcpyHostToDevice, st));
         {
-            KScope ks(s->timer, st, "merge_pairs", n * 16.0);
-            hipLaunchKernelGGL(k_merge_pairs, dim3((chunks + 255) / 256), dim3(256), 0, st, P.m_work.as<MergePair>(),
-                               (uint32_t)pairs.size(), chunks, ord, lc, ord2, lc2, s->chars.as<uint8_t>(),
+            KScope ks(s->timer, st, "merge_pairs", n * 24.0);
+            hipLaunchKernelGGL(k_merge_split, dim3((tiles + 255) / 256), dim3(256), 0, st, dpairs,
+                               (uint32_t)pairs.size(), tiles, ord, s->chars.as<uint8_t>(),
                                P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
-                               P.m_keyc.as<uint64_t>(), cp);
+                               P.m_keyc.as<uint64_t>(), cp, split);
+            hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, dpairs, (uint32_t)pairs.size(), split, ord,
+                               lc, ord2, lc2, s->chars.as<uint8_t>(), P.recv_view_off.as<uint64_t>(),
+                               P.recv_view_len.as<uint32_t>(), P.m_keyc.as<uint64_t>(), cp);
         }
         DSS_HIP(hipGetLastError());
         // the pair list is reused by the next round only after this round ran
@@ -1030,3 +1087,105 @@ int dss_get_extended(dss_set s, uint8_t* bytes, uint64_t* offsets) {
 }
 
 }  // extern "C"
+
+// ================================================================== (a14)
+// is_complete_and_sorted / CheckerWithCompleteExchange (src/mpi/is_sorted.hpp:
+// 27-198) on the merged output of all PEs, plus D of the globally sorted
+// sequence (computeD, src/tests/DNGenerator_tests.cpp:22-45).  The global
+// order is the concatenation of the PEs' merged outputs; the LCP across a PE
+// boundary is recomputed.  Every received string must be a distinct origin
+// position (completeness) and every merged LCP must equal the recomputed one.
+namespace dss {
+
+__global__ void k_check(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ mlcp,
+                        const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
+                        const uint32_t* __restrict__ rpe, const uint32_t* __restrict__ rpos,
+                        const uint64_t* __restrict__ pe_begin, const uint64_t* __restrict__ recv_begin, uint32_t p,
+                        const uint8_t* __restrict__ chars, uint64_t n, uint32_t* __restrict__ glcp,
+                        unsigned int* __restrict__ seen, unsigned long long* __restrict__ cnt) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t a = ord[i];
+    // completeness: mark the origin position
+    const uint64_t origin = pe_begin[rpe[a]] + rpos[a];
+    const unsigned int bit = 1u << (origin & 31);
+    const unsigned int old = atomicOr(&seen[origin >> 5], bit);
+    if (old & bit) atomicAdd(&cnt[2], 1ull);  // duplicate origin
+    if (i == 0) {
+        glcp[0] = 0;
+        return;
+    }
+    const uint32_t b = ord[i - 1];
+    uint32_t l;
+    const int c = view_cmp(chars, voff[b], vlen[b], voff[a], vlen[a], 0, &l);
+    if (c > 0) atomicAdd(&cnt[0], 1ull);  // order violation
+    uint32_t lo = 0, hi = p;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (recv_begin[mid] <= i) lo = mid + 1; else hi = mid;
+    }
+    const bool pe_start = recv_begin[lo - 1] == i;
+    if (!pe_start && mlcp[i] != l) atomicAdd(&cnt[1], 1ull);  // LCP mismatch
+    glcp[i] = l;
+}
+
+__global__ void k_dsum(const uint32_t* __restrict__ glcp, uint64_t n, unsigned long long* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t t = 0;
+    if (i < n) {
+        if (i + 1 < n) t = (uint64_t)max(glcp[i], glcp[i + 1]) + 1;
+        else t = (uint64_t)glcp[i] + 1;
+    }
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_down((unsigned long long)t, o);
+    if (lane_id() == 0 && t) atomicAdd(out, (unsigned long long)t);
+}
+
+__global__ void k_popc(const unsigned int* __restrict__ w, uint64_t nw, unsigned long long* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t t = i < nw ? (uint64_t)__popc(w[i]) : 0;
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_down((unsigned long long)t, o);
+    if (lane_id() == 0 && t) atomicAdd(out, (unsigned long long)t);
+}
+
+}  // namespace dss
+
+extern "C" int dss_check(dss_set s, dss_check_t* out) {
+    return guarded([&] {
+        DSS_REQUIRE(s && out, DSS_EINVAL, "NULL argument");
+        PipeState& P = s->pp;
+        DSS_REQUIRE(P.merged, DSS_ESTATE, "no merged output to check");
+        const uint64_t n = s->n;
+        const hipStream_t st = s->stream;
+        memset(out, 0, sizeof(*out));
+        out->num_strings = n;
+        if (!n) {
+            out->sorted = out->complete = 1;
+            return;
+        }
+        const uint64_t nw = (n + 31) / 32;
+        DevBuf buf;
+        buf.reserve(n * 4 + nw * 4 + 64 + (s->p + 1) * 8);
+        uint32_t* glcp = buf.as<uint32_t>();
+        unsigned int* seen = reinterpret_cast<unsigned int*>(glcp + n);
+        unsigned long long* cnt = reinterpret_cast<unsigned long long*>(buf.as<uint8_t>() + (n * 4 + nw * 4 + 7) / 8 * 8);
+        uint64_t* rb = reinterpret_cast<uint64_t*>(cnt + 8);
+        DSS_HIP(hipMemsetAsync(seen, 0, nw * 4, st));
+        DSS_HIP(hipMemsetAsync(cnt, 0, 64, st));
+        DSS_HIP(hipMemcpyAsync(rb, P.recv_begin.data(), (s->p + 1) * 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_check, dim3((n + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
+                           P.m_lcp.as<uint32_t>(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                           P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>(), s->pe_begin_d.as<uint64_t>(),
+                           rb, s->p, s->chars.as<uint8_t>(), n, glcp, seen, cnt);
+        hipLaunchKernelGGL(k_dsum, dim3((n + 255) / 256), dim3(256), 0, st, glcp, n, cnt + 3);
+        hipLaunchKernelGGL(k_popc, dim3((nw + 255) / 256), dim3(256), 0, st, seen, nw, cnt + 4);
+        DSS_HIP(hipGetLastError());
+        unsigned long long h[5];
+        DSS_HIP(hipMemcpyAsync(h, cnt, 40, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+        out->order_errors = h[0];
+        out->lcp_errors = h[1];
+        out->sorted = h[0] == 0;
+        out->complete = h[2] == 0 && h[4] == n;
+        out->dist_prefix_size = h[3];
+    });
+}

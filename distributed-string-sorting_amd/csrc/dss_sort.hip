@@ -10,7 +10,7 @@
 //
 // Algorithm (MSD radix over 8-char big-endian keys, all PEs in one launch
 // sequence; every PE starts as one segment):
-//   large segments (> kSmallMax strings), per pass over all of them:
+//   large segments (> kLargeMin strings), per pass over all of them:
 //     prefix skip  -- common prefix of the segment from its depth, found by
 //                     16-lane groups comparing 128 bytes per step against the
 //                     segment's first string (early exit once the segment
@@ -22,7 +22,7 @@
 //     scatter      -- stable rank inside the tile by wave ballots, writes the
 //                     next key/id buffers, the id array and LCPs of equal
 //                     strings.  Up to 8 digit passes reuse one gathered key.
-//   small segments (2..2048 strings), batched by power-of-two class into
+//   small segments (2..kLargeMin strings), batched by power-of-two class into
 //     2048-slot blocks: LDS bitonic sort of (group, key) with prefix skip and
 //     iterative refinement of equal-key groups until every group is a single
 //     string or a run of equal strings.
@@ -40,6 +40,7 @@
 namespace dss {
 
 constexpr uint32_t kSmallMax = 2048;  // base-case capacity (slots per block)
+constexpr uint32_t kLargeMin = 512;   // segments above this go through global MSD passes
 constexpr uint32_t kClasses = 12;     // class c holds sizes (2^(c-1), 2^c], c = 1..11
 constexpr uint32_t kTile = 2048;      // strings per tile in the global passes
 constexpr uint32_t kThreads = 256;
@@ -67,12 +68,14 @@ struct SmallLists {
 };
 
 // counters: [0] next large segments, [1] tiles, [2 + c] small class c
-constexpr uint32_t kCtrLarge = 0, kCtrTiles = 1, kCtrSmall = 2;
+// [2 + kClasses + c] strings in small class c
+constexpr uint32_t kCtrLarge = 0, kCtrTiles = 1, kCtrSmall = 2, kCtrSmallElems = 2 + kClasses;
 
 __device__ inline void push_small(SmallSeg* small, const SmallLists& L, uint32_t* ctr,
                                   uint32_t begin, uint32_t cnt, uint32_t depth) {
     const uint32_t c = 32u - __builtin_clz(cnt - 1u);  // ceil(log2 cnt), cnt >= 2
     const uint32_t k = atomicAdd(&ctr[kCtrSmall + c], 1u);
+    atomicAdd(&ctr[kCtrSmallElems + c], cnt);
     small[L.off[c] + k] = SmallSeg{begin, cnt, depth};
 }
 
@@ -91,7 +94,7 @@ __global__ void k_sort_seed(const uint64_t* __restrict__ pe_begin, uint32_t p, S
     if (r >= p) return;
     const uint32_t b = (uint32_t)pe_begin[r];
     const uint32_t cnt = (uint32_t)(pe_begin[r + 1] - pe_begin[r]);
-    if (cnt > kSmallMax) {
+    if (cnt > kLargeMin) {
         const uint32_t k = atomicAdd(&ctr[kCtrLarge], 1u);
         segs[k] = Seg{b, cnt, 0, kNone, kNone, 0};
     } else if (cnt >= 2) {
@@ -247,12 +250,16 @@ k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint6
 }
 
 // ------------------------------------------------------------ scan/classify
+// One block per large segment: digit totals and offsets over the segment's
+// tiles, LCPs at bucket boundaries, and the sub-bucket lists.  List slots
+// are reserved with one global atomic per list per block.
 __global__ void __launch_bounds__(kThreads)
 k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, Seg* __restrict__ next,
                 SmallSeg* __restrict__ small, SmallLists L, uint32_t* __restrict__ ctr,
                 uint32_t* __restrict__ lcp) {
     __shared__ uint32_t sc[256];
     __shared__ uint32_t first;
+    __shared__ uint32_t lcnt[kClasses + 1], lelem[kClasses], lbase[kClasses + 1];
     const Seg S = segs[blockIdx.x];
     uint32_t kd, dep;
     seg_depths(S, kd, dep);
@@ -262,6 +269,8 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, Seg* 
     for (uint32_t t = 0; t < nt; ++t) tot += hist[(uint64_t)(S.tile0 + t) * 256 + d];
     sc[d] = tot;
     if (d == 0) first = 256;
+    if (d <= kClasses) lcnt[d] = 0;
+    if (d < kClasses) lelem[d] = 0;
     __syncthreads();
     for (uint32_t o = 1; o < 256; o <<= 1) {
         uint32_t x = d >= o ? sc[d - o] : 0;
@@ -279,16 +288,29 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, Seg* 
         run += x;
     }
     __syncthreads();
-    if (tot == 0) return;
     const uint32_t b = S.begin + base;
-    if (d != first) lcp[b] = dep;  // boundary between non-empty buckets
-    if (d == 0 || tot == 1) return;  // equal strings / single string: done
-    if (tot <= kSmallMax) {
-        push_small(small, L, ctr, b, tot, dep + 1);
+    if (tot && d != first) lcp[b] = dep;  // boundary between non-empty buckets
+    // d == 0: equal strings; tot == 1: a single string -- both done
+    int list = -1;  // small class c, or kClasses for the large list
+    uint32_t slot = 0;
+    if (tot > 1 && d != 0) {
+        list = tot <= kLargeMin ? (int)(32u - __builtin_clz(tot - 1u)) : (int)kClasses;
+        slot = atomicAdd(&lcnt[list], 1u);
+        if (list < (int)kClasses) atomicAdd(&lelem[list], tot);
+    }
+    __syncthreads();
+    if (d < kClasses && lcnt[d]) {
+        lbase[d] = atomicAdd(&ctr[kCtrSmall + d], lcnt[d]);
+        atomicAdd(&ctr[kCtrSmallElems + d], lelem[d]);
+    }
+    if (d == kClasses && lcnt[kClasses]) lbase[kClasses] = atomicAdd(&ctr[kCtrLarge], lcnt[kClasses]);
+    __syncthreads();
+    if (list < 0) return;
+    if (list < (int)kClasses) {
+        small[L.off[list] + lbase[list] + slot] = SmallSeg{b, tot, dep + 1};
     } else {
-        const uint32_t k = atomicAdd(&ctr[kCtrLarge], 1u);
         const uint32_t nkd = (dep + 1 - kd >= 8) ? kNone : kd;
-        next[k] = Seg{b, tot, dep + 1, nkd, kNone, 0};
+        next[lbase[kClasses] + slot] = Seg{b, tot, dep + 1, nkd, kNone, 0};
     }
 }
 
@@ -555,7 +577,7 @@ void sort_lcp(dss_set_s* s) {
     X.keyB.reserve(n * 8);
     X.pbA.reserve(n * 4);
     X.pbB.reserve(n * 4);
-    const uint64_t seg_cap = n / (kSmallMax + 1) + s->p + 16;
+    const uint64_t seg_cap = n / (kLargeMin + 1) + s->p + 16;
     X.segA.reserve(seg_cap * sizeof(Seg));
     X.segB.reserve(seg_cap * sizeof(Seg));
     const uint64_t tile_cap = n / kTile + seg_cap + 16;
@@ -608,31 +630,54 @@ void sort_lcp(dss_set_s* s) {
         const uint32_t nt = h_ctr[1];
         DSS_REQUIRE(nt <= tile_cap, DSS_ERANGE, "tile list overflow");
         const Tile* tl = X.tiles.as<Tile>();
+        // elements in large segments of this pass (for the algorithmic byte counts)
+        std::vector<Seg> hseg;
+        uint64_t cnt_large = 0;
+        if (s->timer.enabled) {
+            hseg.resize(nlarge);
+            DSS_HIP(hipMemcpy(hseg.data(), cur, nlarge * sizeof(Seg), hipMemcpyDeviceToHost));
+            for (auto& q : hseg) cnt_large += q.cnt;
+        }
         {
             KScope ks(s->timer, st, "prefix_skip", 0.0);
             hipLaunchKernelGGL(k_prefix_skip, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len);
         }
         {
-            KScope ks(s->timer, st, "gather", 0.0);
+            KScope ks(s->timer, st, "gather", 0.0);  // bytes amended after the pass
             hipLaunchKernelGGL(k_gather, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc, pc);
         }
         {
-            KScope ks(s->timer, st, "hist", 0.0);
+            KScope ks(s->timer, st, "hist", (double)cnt_large * 8.0);
             hipLaunchKernelGGL(k_hist, dim3(nt), dim3(kThreads), 0, st, tl, cur, kc, X.hist.as<uint32_t>());
         }
         {
-            KScope ks(s->timer, st, "scan_classify", 0.0);
+            KScope ks(s->timer, st, "scan_classify", (double)nt * 256 * 4 * 3);
             hipLaunchKernelGGL(k_scan_classify, dim3(nlarge), dim3(kThreads), 0, st, cur, X.hist.as<uint32_t>(),
                                nxt, X.small.as<SmallSeg>(), L, ctr, lcp);
         }
         {
-            KScope ks(s->timer, st, "scatter", 0.0);
+            KScope ks(s->timer, st, "scatter", (double)cnt_large * 28.0);
             hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kThreads), 0, st, tl, cur, X.hist.as<uint32_t>(), kc, pc,
                                kn, pn, perm, lcp);
         }
         DSS_HIP(hipGetLastError());
         DSS_HIP(hipMemcpyAsync(h_ctr, ctr, 4, hipMemcpyDeviceToHost, st));
         DSS_HIP(hipStreamSynchronize(st));
+        if (s->timer.enabled) {
+            // prefix skip: characters up to the common prefix + 1 of every
+            // string of a segment that gathers keys, plus id/offset/length;
+            // gather: id, offset, length, 8 chars, key and id writes
+            std::vector<Seg> after(nlarge);
+            DSS_HIP(hipMemcpy(after.data(), cur, nlarge * sizeof(Seg), hipMemcpyDeviceToHost));
+            double skip = 0, gath = 0;
+            for (uint32_t k = 0; k < nlarge; ++k)
+                if (hseg[k].kdepth == kNone) {
+                    skip += (double)after[k].cnt * ((double)after[k].cp + 1.0 + 16.0);
+                    gath += (double)after[k].cnt * 36.0;
+                }
+            s->timer.amend_last("prefix_skip", skip);
+            s->timer.amend_last("gather", gath);
+        }
         if (trace) {
             std::vector<Seg> hs(std::min<uint32_t>(nlarge, 4));
             DSS_HIP(hipMemcpy(hs.data(), cur, hs.size() * sizeof(Seg), hipMemcpyDeviceToHost));
@@ -646,7 +691,7 @@ void sort_lcp(dss_set_s* s) {
         std::swap(kc, kn);
         std::swap(pc, pn);
     }
-    uint32_t cls[kClasses];
+    uint32_t cls[2 * kClasses];
     DSS_HIP(hipMemcpyAsync(cls, ctr + kCtrSmall, sizeof(cls), hipMemcpyDeviceToHost, st));
     DSS_HIP(hipStreamSynchronize(st));
     for (uint32_t c = 1; c < kClasses; ++c) {
@@ -655,7 +700,8 @@ void sort_lcp(dss_set_s* s) {
         DSS_REQUIRE(cnt <= L.cap[c], DSS_ERANGE, "small list overflow");
         const uint32_t spb = kCap >> c;
         const uint32_t blocks = (cnt + spb - 1) / spb;
-        KScope ks(s->timer, st, "small_sort", 0.0);
+        // per string: id read, offset/length, one 8-char key, id + lcp writes
+        KScope ks(s->timer, st, "small_sort", (double)cls[kClasses + c] * 32.0);
         hipLaunchKernelGGL(k_small_sort, dim3(blocks), dim3(kThreads), 0, st,
                            X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len);
     }

@@ -214,6 +214,14 @@ class DeviceStringSet:
         check(lib().dss_get_permutation(self._h, _p(si, _lib.u32p), _p(pi, _lib.u32p)))
         return si[:n], pi[:n]
 
+    def check(self) -> dict:
+        """is_complete_and_sorted + exhaustive LCP check of the merged output; D."""
+        r = _lib.CheckResult()
+        check(lib().dss_check(self._h, C.byref(r)))
+        return dict(num_strings=r.num_strings, order_errors=r.order_errors,
+                    lcp_errors=r.lcp_errors, D=r.dist_prefix_size, sorted=bool(r.sorted),
+                    complete=bool(r.complete))
+
     # ------------------------------------------------------------ timing
     def sync(self):
         check(lib().dss_sync(self._h))

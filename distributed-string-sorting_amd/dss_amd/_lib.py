@@ -27,6 +27,12 @@ class KernelStat(C.Structure):
                 ("total_ms", C.c_double), ("alg_bytes", C.c_double)]
 
 
+class CheckResult(C.Structure):
+    _fields_ = [("num_strings", C.c_uint64), ("order_errors", C.c_uint64),
+                ("lcp_errors", C.c_uint64), ("dist_prefix_size", C.c_uint64),
+                ("sorted", C.c_int32), ("complete", C.c_int32)]
+
+
 class DssError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"dss error {code}: {msg}")
@@ -65,6 +71,7 @@ _SIGS = {
     "dss_merge_sort": (C.c_int, [set_p, C.c_int, C.c_int]),
     "dss_prefix_doubling_sort": (C.c_int, [set_p, C.c_int, C.c_uint64]),
     "dss_get_permutation": (C.c_int, [set_p, u32p, u32p]),
+    "dss_check": (C.c_int, [set_p, C.POINTER(CheckResult)]),
     "dss_sync": (C.c_int, [set_p]),
     "dss_set_timing": (C.c_int, [set_p, C.c_int]),
     "dss_kernel_stats": (C.c_int, [set_p, C.POINTER(KernelStat), C.c_uint32, u32p, C.c_int]),

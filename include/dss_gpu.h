@@ -169,6 +169,21 @@ int dss_prefix_doubling_sort(dss_set s, int sampler, uint64_t m);
  * 282-290): per receive-side output position, stringIndex and PEIndex. */
 int dss_get_permutation(dss_set s, uint32_t* string_index, uint32_t* pe_index);
 
+/* ---- (a14) is_complete_and_sorted + CheckerWithCompleteExchange
+ *      (src/mpi/is_sorted.hpp:27-198) on the merged output of all PEs, and
+ *      D = sum over the globally sorted output of max(lcp_i, lcp_i+1) + 1
+ *      (computeD, src/tests/DNGenerator_tests.cpp:22-45) ---- */
+typedef struct {
+    uint64_t num_strings;
+    uint64_t order_errors;      /* adjacent pairs out of order */
+    uint64_t lcp_errors;        /* merged LCPs that differ from the recomputed LCP */
+    uint64_t dist_prefix_size;  /* D */
+    int32_t sorted;             /* 1 if globally sorted */
+    int32_t complete;           /* 1 if every input string appears exactly once */
+} dss_check_t;
+
+int dss_check(dss_set s, dss_check_t* out);
+
 /* ---- measurement ---- */
 
 /* Waits for all work queued on the set's stream. */

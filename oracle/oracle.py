@@ -252,10 +252,10 @@ def _arena_pointers(arena: np.ndarray):
     """Char pointers to every NUL-terminated string of an arena."""
     ends = np.flatnonzero(arena == 0)
     starts = np.concatenate([[0], ends[:-1] + 1]) if ends.size else np.zeros(0, np.int64)
-    base = arena.ctypes.data
-    arr = (u8p * max(len(starts), 1))()
-    for k, s in enumerate(starts):
-        arr[k] = C.cast(base + int(s), u8p)
+    ptrs = np.zeros(max(len(starts), 1), dtype=np.uint64)
+    ptrs[:len(starts)] = np.uint64(arena.ctypes.data) + starts.astype(np.uint64)
+    arr = (u8p * ptrs.size).from_buffer(ptrs)
+    arr._keep = ptrs  # keep the backing buffer alive with the array
     return arr, starts
 
 
