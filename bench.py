@@ -37,8 +37,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# Algorithmic bytes per unit of work of each kernel family:
-# returned by the library as alg_bytes accumulated over launches.
+# Per-kernel statistics from the library: launches, event-timed milliseconds
+# and algorithmic bytes accumulated over the launches of the timed region.
 def roofline_from_stats(stats: dict):
     if not stats:
         return None

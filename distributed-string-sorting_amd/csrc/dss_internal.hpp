@@ -195,6 +195,8 @@ struct dss_set_s {
     bool sorted = false;
     dss::DevBuf perm;   // u32[n]
     dss::DevBuf lcp;    // u32[n]
+    dss::DevBuf s_off;  // u64[n] offset of the string at each sorted position
+    dss::DevBuf s_len;  // u32[n] length of the string at each sorted position
 
     dss::SortScratch ss;
     dss::PipeState pp;

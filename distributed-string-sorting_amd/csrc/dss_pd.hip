@@ -118,10 +118,6 @@ __global__ void k_pd_init(uint8_t* __restrict__ cand, uint8_t* __restrict__ roun
     }
 }
 
-// Four lanes per position: lane j of the group accumulates XXH64 lane v_j+1
-// over the 32-byte stripes (all hashed prefixes of a round have the same
-// length, so the stripe loop is uniform); lane 0 merges and finishes the
-// tail.  Cross-lane reads happen with every lane of the wave active.
 // Sum of a per-thread count over the block, added to *out with one atomic.
 __device__ inline void block_count_add(uint32_t v, unsigned long long* out) {
     __shared__ uint32_t part[4];
@@ -136,33 +132,57 @@ __device__ inline void block_count_add(uint32_t v, unsigned long long* out) {
 
 constexpr uint32_t kPdGrid = 4096;  // blocks of the grid-stride PD kernels
 
-// Four lanes per position: lane j of the group accumulates XXH64 lane v_j+1
-// over the 32-byte stripes (all hashed prefixes of a round have the same
-// length, so the stripe loop is uniform); lane 0 merges and finishes the
-// tail.  The grid-stride loop is block-uniform, so every cross-lane read
-// happens with all lanes of the wave active.
+// h % m; the reference's range m = 2^64-1 only maps 2^64-1 to 0.
+__device__ inline uint64_t mod_range(uint64_t h, uint64_t m) {
+    if (m == ~0ull) return h == ~0ull ? 0 : h;
+    return h % m;
+}
+
+constexpr uint32_t kPdPos = 64;            // positions per block iteration
+constexpr uint32_t kWin = 256;             // prefix bytes staged per window
+constexpr uint32_t kChunks = kWin / 16 + 1;  // 16-byte chunks per staged row
+constexpr uint32_t kRowW = kChunks * 2;    // row length in 8-byte words
+
+// 8 bytes at byte offset b of the staged rows (little endian).
+__device__ inline uint64_t lds_word(const uint64_t* rows, uint32_t b) {
+    const uint32_t w = b >> 3, sh = (b & 7) * 8;
+    const uint64_t lo = rows[w];
+    return sh ? (lo >> sh) | (rows[w + 1] << (64 - sh)) : lo;
+}
+
+// Classification and hashing of the candidates of one round.  A block takes
+// 64 positions per iteration, four lanes each.  The prefixes to hash are
+// staged through LDS in 256-byte windows with 16-byte loads (consecutive
+// lanes read consecutive chunks of one string); lane j of a group then
+// accumulates XXH64 lane v_j+1 over the window's 32-byte stripes and lane 0
+// merges and hashes the tail.  All loops are block-uniform (the hashed
+// length is the round's depth), so every shuffle runs with all lanes active.
 __global__ void __launch_bounds__(256)
 k_pd_classify(const uint8_t* __restrict__ cand, const uint64_t* __restrict__ pe_begin, uint32_t p,
-              const uint32_t* __restrict__ perm, const uint32_t* __restrict__ lcp, const uint64_t* __restrict__ off,
-              const uint32_t* __restrict__ len, const uint8_t* __restrict__ chars, uint32_t depth, uint64_t m,
+              const uint32_t* __restrict__ lcp, const uint64_t* __restrict__ s_off,
+              const uint32_t* __restrict__ s_len, const uint8_t* __restrict__ chars, uint32_t depth, uint64_t m,
               uint64_t n, uint8_t* __restrict__ flags, uint64_t* __restrict__ hashv, uint8_t* __restrict__ rounds,
               uint32_t* __restrict__ dist, unsigned long long* __restrict__ nhashed) {
-    const uint32_t j = threadIdx.x & 3;
+    __shared__ uint64_t rows[kPdPos * kRowW];
+    __shared__ uint64_t s_so[kPdPos];
+    const uint32_t j = threadIdx.x & 3, g = threadIdx.x >> 2;
     const uint32_t base_lane = threadIdx.x & ~3u;
+    const uint32_t stripes = depth / 32;
+    const uint32_t nwin = (depth + kWin - 1) / kWin;
     uint32_t counted = 0;
-    for (uint64_t blk = (uint64_t)blockIdx.x * 64; blk < n; blk += (uint64_t)gridDim.x * 64) {
-        const uint64_t c = blk + (threadIdx.x >> 2);
+    for (uint64_t blk = (uint64_t)blockIdx.x * kPdPos; blk < n; blk += (uint64_t)gridDim.x * kPdPos) {
+        const uint64_t c = blk + g;
         uint8_t f = 0;
         uint64_t so = 0;
         if (c < n && cand[c]) {
-            const uint32_t id = perm[c];
-            const uint32_t L = len[id];
+            const uint32_t L = s_len[c];
             if (depth <= L) {
-                const uint32_t r = pe_of(pe_begin, p, c);
-                if (c > pe_begin[r] && cand[c - 1] && lcp[c] >= depth) f = kLcpDup;
+                // lcp is 0 at every PE start and depth >= 1, so lcp[c] >= depth
+                // implies that c-1 is in the same PE
+                if (c > 0 && lcp[c] >= depth && cand[c - 1]) f = kLcpDup;
                 else {
                     f = kHashed;
-                    so = off[id];
+                    so = s_off[c];
                 }
             }
             if (j == 0) {
@@ -174,10 +194,34 @@ k_pd_classify(const uint8_t* __restrict__ cand, const uint64_t* __restrict__ pe_
             flags[c] = 0;
         }
         const bool hashed = f == kHashed;
+        if (j == 0) s_so[g] = hashed ? so : ~0ull;
         uint64_t v = j == 0 ? P1 + P2 : (j == 1 ? P2 : (j == 2 ? 0ull : (uint64_t)0 - P1));
-        const uint32_t stripes = depth >= 32 ? depth / 32 : 0;
-        for (uint32_t t = 0; t < stripes; ++t)
-            if (hashed) v = xround(v, load_word_le(chars, so + (uint64_t)t * 32 + 8 * j));
+        uint32_t shift = 0;
+        for (uint32_t w = 0; w < nwin; ++w) {
+            __syncthreads();  // s_so visible; previous window's rows consumed
+            const uint32_t wend = min(depth, (w + 1) * kWin);  // bytes needed: [w*kWin, wend)
+            for (uint32_t k = threadIdx.x; k < kPdPos * kChunks; k += blockDim.x) {
+                const uint32_t row = k / kChunks, q = k - row * kChunks;
+                const uint64_t o = s_so[row];
+                if (o == ~0ull) continue;
+                const uint64_t A = (o + (uint64_t)w * kWin) & ~15ull;
+                if (A + 16ull * q < o + wend) {
+                    const uint4 x = *reinterpret_cast<const uint4*>(chars + A + 16ull * q);
+                    uint64_t* dst = &rows[row * kRowW + 2 * q];
+                    dst[0] = ((uint64_t)x.y << 32) | x.x;
+                    dst[1] = ((uint64_t)x.w << 32) | x.z;
+                }
+            }
+            __syncthreads();
+            shift = (uint32_t)((so + (uint64_t)w * kWin) & 15);
+            if (hashed) {
+                const uint64_t* rw = &rows[g * kRowW];
+                for (uint32_t t = 0; t < kWin / 32; ++t) {
+                    const uint32_t st = w * (kWin / 32) + t;
+                    if (st < stripes) v = xround(v, lds_word(rw, shift + 32 * t + 8 * j));
+                }
+            }
+        }
         const uint64_t v1 = __shfl(v, base_lane + 0);
         const uint64_t v2 = __shfl(v, base_lane + 1);
         const uint64_t v3 = __shfl(v, base_lane + 2);
@@ -194,19 +238,22 @@ k_pd_classify(const uint8_t* __restrict__ cand, const uint64_t* __restrict__ pe_
                 h = v3 + P5;  // includes depth == 32, where v3 consumed one stripe
             }
             h += depth;
+            // tail bytes [stripes*32, depth) lie in the last staged window
+            const uint64_t* rw = &rows[g * kRowW];
+            const uint32_t wbase = (nwin - 1) * kWin;
             uint32_t q = stripes * 32;
             while (q + 8 <= depth) {
-                h ^= xround(0, load_word_le(chars, so + q));
+                h ^= xround(0, lds_word(rw, shift + q - wbase));
                 h = rotl64(h, 27) * P1 + P4;
                 q += 8;
             }
             if (q + 4 <= depth) {
-                h ^= (load_word_le(chars, so + q) & 0xFFFFFFFFull) * P1;
+                h ^= (lds_word(rw, shift + q - wbase) & 0xFFFFFFFFull) * P1;
                 h = rotl64(h, 23) * P2 + P3;
                 q += 4;
             }
             while (q < depth) {
-                h ^= (uint64_t)chars[so + q] * P5;
+                h ^= (lds_word(rw, shift + q - wbase) & 0xFF) * P5;
                 h = rotl64(h, 11) * P1;
                 ++q;
             }
@@ -215,9 +262,10 @@ k_pd_classify(const uint8_t* __restrict__ cand, const uint64_t* __restrict__ pe_
             h ^= h >> 29;
             h *= P3;
             h ^= h >> 32;
-            hashv[c] = h % m;
+            hashv[c] = mod_range(h, m);
             ++counted;
         }
+        __syncthreads();  // rows and s_so are rewritten by the next iteration
     }
     block_count_add(counted, nhashed);
 }
@@ -264,8 +312,9 @@ k_pd_next(const uint8_t* __restrict__ cand, const uint8_t* __restrict__ flags, c
         if (f == kLcpDup) {
             keep = 1;
         } else if (f == kHashed) {
-            const uint32_t r = pe_of(pe_begin, p, c);
-            if (c + 1 < pe_begin[r + 1] && cand[c + 1] && lcp[c + 1] >= depth) keep = 1;  // LCP-local root
+            // LCP-local root: the next position (same PE, as lcp is 0 at PE
+            // starts) is a local LCP duplicate
+            if (c + 1 < n && lcp[c + 1] >= depth && cand[c + 1]) keep = 1;
             else keep = dup[slot[c]];
         }
         next[c] = (uint8_t)keep;
@@ -306,8 +355,8 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
         {
             KScope ks(s->timer, st, "pd_hash", 0.0);
             hipLaunchKernelGGL(k_pd_classify, dim3(std::min<uint64_t>((n + 63) / 64, kPdGrid)), blk, 0, st, cand, s->pe_begin_d.as<uint64_t>(), p,
-                               s->perm.as<uint32_t>(), s->lcp.as<uint32_t>(), s->off.as<uint64_t>(),
-                               s->len.as<uint32_t>(), s->chars.as<uint8_t>(), (uint32_t)depth, m, n,
+                               s->lcp.as<uint32_t>(), s->s_off.as<uint64_t>(),
+                               s->s_len.as<uint32_t>(), s->chars.as<uint8_t>(), (uint32_t)depth, m, n,
                                P.pd_flags.as<uint8_t>(), P.pd_hash.as<uint64_t>(), P.rounds.as<uint8_t>(),
                                P.dist.as<uint32_t>(), ctr);
         }
@@ -369,7 +418,7 @@ __global__ void k_hash_ids(const uint32_t* __restrict__ ids, uint64_t count, con
         return;
     }
     const uint32_t L = min(depth, len[id]);
-    out[k] = xxh64_stream(chars, off[id], L) % m;
+    out[k] = mod_range(xxh64_stream(chars, off[id], L), m);
 }
 
 }  // namespace dss

@@ -280,10 +280,10 @@ __global__ void k_bucket_search(const uint64_t* __restrict__ pe_begin, uint32_t 
     }
 }
 
-__global__ void k_weights(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ len,
-                          const uint32_t* __restrict__ dist, uint64_t n, uint64_t* __restrict__ w) {
+__global__ void k_weights(const uint32_t* __restrict__ s_len, const uint32_t* __restrict__ dist, uint64_t n,
+                          uint64_t* __restrict__ w) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) w[i] = dist ? dist[i] : len[perm[i]];
+    if (i < n) w[i] = dist ? dist[i] : s_len[i];
 }
 
 void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, bool use_dist) {
@@ -312,8 +312,8 @@ void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, boo
     if (by_chars && s->n) {
         P.wprefix.reserve((s->n + 1) * 8);
         wprefix = P.wprefix.as<uint64_t>();
-        hipLaunchKernelGGL(k_weights, dim3((s->n + 255) / 256), dim3(256), 0, st, s->perm.as<uint32_t>(),
-                           s->len.as<uint32_t>(), dist, s->n, wprefix);
+        hipLaunchKernelGGL(k_weights, dim3((s->n + 255) / 256), dim3(256), 0, st, s->s_len.as<uint32_t>(), dist,
+                           s->n, wprefix);
         uint64_t tot = 0;
         exclusive_scan_u64(wprefix, wprefix, s->n, st, P, &tot);
         DSS_HIP(hipMemcpyAsync(wprefix + s->n, &tot, 8, hipMemcpyHostToDevice, st));
@@ -369,9 +369,9 @@ __device__ inline uint32_t bucket_of(const uint64_t* __restrict__ send_start, ui
 // (:81-107), mode 1 chars [lcp, len] incl. NUL (:109-137), mode 2 chars
 // [lcp, dist) then NUL (:139-172).  The LCP at every bucket start is zero
 // (setLcpAtStartOfInterval, src/mpi/alltoall.hpp:396-404).
-__global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, const uint32_t* __restrict__ perm,
-                             const uint32_t* __restrict__ lcp, const uint64_t* __restrict__ off,
-                             const uint32_t* __restrict__ len, const uint32_t* __restrict__ dist,
+__global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, const uint32_t* __restrict__ lcp,
+                             const uint64_t* __restrict__ s_off, const uint32_t* __restrict__ s_len,
+                             const uint32_t* __restrict__ dist,
                              const uint64_t* __restrict__ send_start, const uint64_t* __restrict__ run_off, int mode,
                              uint64_t n, uint64_t* __restrict__ size, uint32_t* __restrict__ r_lcp,
                              uint32_t* __restrict__ r_pe, uint32_t* __restrict__ r_pos, uint64_t* __restrict__ r_voff,
@@ -389,8 +389,7 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
     const uint64_t st = send_start[(uint64_t)r * p + q];
     const uint64_t dst = run_off[(uint64_t)q * p + r] + (j - st);
     const uint32_t l = (j == st) ? 0u : lcp[i];
-    const uint32_t id = perm[i];
-    const uint32_t L = len[id];
+    const uint32_t L = s_len[i];
     uint64_t sz;
     uint32_t from = 0, vlen = L;
     if (mode == 0) {
@@ -407,7 +406,7 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
     r_lcp[dst] = l;
     r_pe[dst] = r;
     r_pos[dst] = (uint32_t)j;
-    r_voff[dst] = off[id];
+    r_voff[dst] = s_off[i];
     r_vlen[dst] = vlen;
     r_from[dst] = from;
 }
@@ -428,6 +427,14 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
     // modes 0/1: chars [from, vlen] incl. the NUL; mode 2: [from, vlen) + NUL
     const uint32_t m = vlen - from;
     for (uint32_t t = l; t <= m; t += lanes) dst[t] = (t == m) ? 0 : src[from + t];
+}
+
+__global__ void k_byte_begin(const uint64_t* __restrict__ recv_begin, const uint64_t* __restrict__ byte_off,
+                             uint64_t n, uint64_t total, uint32_t p, uint64_t* __restrict__ out) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > p) return;
+    const uint64_t rb = recv_begin[q];
+    out[q] = rb < n ? byte_off[rb] : total;
 }
 
 void exchange(dss_set_s* s, int mode) {
@@ -460,7 +467,7 @@ void exchange(dss_set_s* s, int mode) {
     }
     P.recv_begin[p] = acc;
     DSS_REQUIRE(acc == n, DSS_EINVAL, "interval sizes do not cover all strings");
-    P.layout.reserve(lay.size() * 8);
+    P.layout.reserve(lay.size() * 8 + (uint64_t)2 * (p + 1) * 8);
     DSS_HIP(hipMemcpyAsync(P.layout.p, lay.data(), lay.size() * 8, hipMemcpyHostToDevice, st));
     const uint64_t nn = std::max<uint64_t>(n, 1);
     P.recv_size.reserve(nn * 8);
@@ -475,8 +482,8 @@ void exchange(dss_set_s* s, int mode) {
     if (n) {
         KScope ks(s->timer, st, "exchange_index", n * 40.0);
         hipLaunchKernelGGL(k_exch_index, dim3((n + 255) / 256), dim3(256), 0, st, s->pe_begin_d.as<uint64_t>(), p,
-                           s->perm.as<uint32_t>(), s->lcp.as<uint32_t>(), s->off.as<uint64_t>(),
-                           s->len.as<uint32_t>(), mode == 2 ? P.dist.as<uint32_t>() : nullptr, ls,
+                           s->lcp.as<uint32_t>(), s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(),
+                           mode == 2 ? P.dist.as<uint32_t>() : nullptr, ls,
                            ls + (uint64_t)p * p, mode, n, P.recv_size.as<uint64_t>(), P.recv_lcp.as<uint32_t>(),
                            P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
@@ -490,7 +497,7 @@ void exchange(dss_set_s* s, int mode) {
     if (n) {
         KScope ks(s->timer, st, "exchange_copy", 2.0 * total + n * 24.0);
         // short prefix-doubling prefixes: 4 lanes per string, whole strings: 16
-        const uint32_t lg = (mode == 2 || total < 24 * n) ? 2 : 4;
+        const uint32_t lg = (mode == 2 || total < 24 * n) ? 3 : 4;
         const uint64_t threads = n << lg;
         hipLaunchKernelGGL(k_exch_copy, dim3((threads + 255) / 256), dim3(256), 0, st, s->chars.as<uint8_t>(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
@@ -498,19 +505,17 @@ void exchange(dss_set_s* s, int mode) {
                            P.recv_bytes.as<uint8_t>());
     }
     DSS_HIP(hipGetLastError());
-    // byte_begin per receiver
-    P.byte_begin.assign(p + 1, 0);
-    std::vector<uint64_t> tmp(1);
-    for (uint32_t q = 0; q < p; ++q) {
-        if (P.recv_begin[q] < n) {
-            DSS_HIP(hipMemcpyAsync(&P.byte_begin[q], P.recv_byte_off.as<uint64_t>() + P.recv_begin[q], 8,
-                                   hipMemcpyDeviceToHost, st));
-        } else {
-            P.byte_begin[q] = total;
-        }
+    // byte_begin per receiver: the byte offset of its first received string
+    P.byte_begin.assign(p + 1, total);
+    {
+        uint64_t* rbd = P.layout.as<uint64_t>() + (uint64_t)2 * p * p;  // p+1 receive begins
+        uint64_t* bbd = rbd + (p + 1);                                   // p+1 byte begins
+        DSS_HIP(hipMemcpyAsync(rbd, P.recv_begin.data(), (p + 1) * 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_byte_begin, dim3((p + 1 + 255) / 256), dim3(256), 0, st, rbd, P.recv_byte_off.as<uint64_t>(),
+                           n, total, p, bbd);
+        DSS_HIP(hipMemcpyAsync(P.byte_begin.data(), bbd, (p + 1) * 8, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
     }
-    P.byte_begin[p] = total;
-    DSS_HIP(hipStreamSynchronize(st));
     P.mode = mode;
     P.exchanged = true;
     P.merged = P.extended = false;
@@ -796,12 +801,18 @@ void merge(dss_set_s* s) {
             pos += c;
         }
     }
-    std::vector<MergePair> pairs;
+    // all rounds' pair lists are built first and uploaded once, so the
+    // rounds run back to back without host synchronisation
+    std::vector<MergePair> all;
+    std::vector<size_t> rbeg;          // first pair of each round
+    std::vector<uint64_t> rtiles, tbeg;  // tiles of each round, first split slot
+    uint64_t tsum = 0;
     while (true) {
         bool any = false;
         for (uint32_t q = 0; q < p; ++q) any |= runs[q].size() > 1;
         if (!any) break;
-        pairs.clear();
+        rbeg.push_back(all.size());
+        tbeg.push_back(tsum);
         uint64_t tiles = 0;
         for (uint32_t q = 0; q < p; ++q) {
             auto& R = runs[q];
@@ -815,30 +826,37 @@ void merge(dss_set_s* s) {
                 mp.pad = 0;
                 mp.tile0 = tiles;
                 tiles += (mp.na + mp.nb + kTileOut - 1) / kTileOut;
-                pairs.push_back(mp);
+                all.push_back(mp);
                 next.push_back({mp.out, mp.na + mp.nb});
             }
             R.swap(next);
         }
-        P.m_work.reserve(pairs.size() * sizeof(MergePair) + (tiles + 1) * 8);
+        rtiles.push_back(tiles);
+        tsum += tiles + 1;
+    }
+    rbeg.push_back(all.size());
+    if (!all.empty()) {
+        const uint64_t pair_bytes = (all.size() * sizeof(MergePair) + 15) / 16 * 16;
+        P.m_work.reserve(pair_bytes + (tsum + 1) * 8);
         MergePair* dpairs = P.m_work.as<MergePair>();
-        uint64_t* split = reinterpret_cast<uint64_t*>(P.m_work.as<uint8_t>() + pairs.size() * sizeof(MergePair));
-        DSS_HIP(hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(MergePair), hipMemcpyHostToDevice, st));
-        {
+        uint64_t* splits = reinterpret_cast<uint64_t*>(P.m_work.as<uint8_t>() + pair_bytes);
+        DSS_HIP(hipMemcpyAsync(dpairs, all.data(), all.size() * sizeof(MergePair), hipMemcpyHostToDevice, st));
+        for (size_t r = 0; r + 1 < rbeg.size(); ++r) {
+            const uint32_t np = (uint32_t)(rbeg[r + 1] - rbeg[r]);
+            const uint64_t tiles = rtiles[r];
+            MergePair* rp = dpairs + rbeg[r];
+            uint64_t* split = splits + tbeg[r];
             KScope ks(s->timer, st, "merge_pairs", n * 24.0);
-            hipLaunchKernelGGL(k_merge_split, dim3((tiles + 255) / 256), dim3(256), 0, st, dpairs,
-                               (uint32_t)pairs.size(), tiles, ord, s->chars.as<uint8_t>(),
-                               P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+            hipLaunchKernelGGL(k_merge_split, dim3((tiles + 255) / 256), dim3(256), 0, st, rp, np, tiles, ord,
+                               s->chars.as<uint8_t>(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                                P.m_keyc.as<uint64_t>(), cp, split);
-            hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, dpairs, (uint32_t)pairs.size(), split, ord,
-                               lc, ord2, lc2, s->chars.as<uint8_t>(), P.recv_view_off.as<uint64_t>(),
-                               P.recv_view_len.as<uint32_t>(), P.m_keyc.as<uint64_t>(), cp);
+            hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, rp, np, split, ord, lc, ord2, lc2,
+                               s->chars.as<uint8_t>(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                               P.m_keyc.as<uint64_t>(), cp);
+            std::swap(ord, ord2);
+            std::swap(lc, lc2);
         }
         DSS_HIP(hipGetLastError());
-        // the pair list is reused by the next round only after this round ran
-        DSS_HIP(hipStreamSynchronize(st));
-        std::swap(ord, ord2);
-        std::swap(lc, lc2);
     }
     // results live in (ord, lc); make them the canonical buffers
     if (ord != P.m_order.as<uint32_t>()) {

@@ -39,7 +39,7 @@
 
 namespace dss {
 
-constexpr uint32_t kSmallMax = 2048;  // base-case capacity (slots per block)
+constexpr uint32_t kSmallMax = 1024;  // base-case capacity (slots per block)
 constexpr uint32_t kLargeMin = 512;   // segments above this go through global MSD passes
 constexpr uint32_t kClasses = 12;     // class c holds sizes (2^(c-1), 2^c], c = 1..11
 constexpr uint32_t kTile = 2048;      // strings per tile in the global passes
@@ -154,9 +154,11 @@ __device__ inline void seg_depths(const Seg& s, uint32_t& kd, uint32_t& dep) {
 
 // ------------------------------------------------------------ prefix skip
 // Groups of 16 lanes compare one string against the segment's first string,
-// 128 bytes per step.  All loops that contain cross-lane operations have
-// block- or wave-uniform trip counts, so every shuffle runs with all lanes of
-// the wave active (a disabled source lane would read as 0).
+// 256 bytes per step (two 8-char keys per lane).  The segment minimum found
+// so far is consulted once per string and once per further step: a string
+// whose compared prefix already exceeds it cannot lower it.  All loops that
+// contain cross-lane operations have block- or wave-uniform trip counts, so
+// every shuffle runs with all lanes of the wave active.
 __global__ void __launch_bounds__(kThreads)
 k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
               const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
@@ -185,26 +187,31 @@ k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint
             so = off[id];
             sl = len[id];
         }
+        uint32_t lim = (r & 7) == 0 ? *cpp : kNone;  // refreshed every 8 strings
         uint32_t base = d0;
         uint32_t res = kNone;
         while (__ballot(!done) != 0) {  // wave-uniform condition
             uint32_t m = kNone;
-            const uint32_t pos = base + 8 * l;
             if (!done) {
-                const uint64_t a = load_key8(chars, so, sl, pos);
-                const uint64_t b = load_key8(chars, roff, rlen, pos);
-                if (a != b) m = pos + key_lcp(a, b);
-                else if ((a & 0xFF) == 0) m = pos + key_strlen(a);
+                const uint32_t p0 = base + 8 * l, p1 = p0 + 128;
+                const uint64_t a0 = load_key8(chars, so, sl, p0), b0 = load_key8(chars, roff, rlen, p0);
+                const uint64_t a1 = load_key8(chars, so, sl, p1), b1 = load_key8(chars, roff, rlen, p1);
+                if (a0 != b0) m = p0 + key_lcp(a0, b0);
+                else if ((a0 & 0xFF) == 0) m = p0 + key_strlen(a0);
+                else if (a1 != b1) m = p1 + key_lcp(a1, b1);
+                else if ((a1 & 0xFF) == 0) m = p1 + key_strlen(a1);
             }
+            // first hit in position order: lanes 0..15 cover [base, base+128),
+            // then [base+128, base+256); the minimum is the answer
             for (int o = 8; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor(m, o, 16));
             if (!done) {
                 if (m != kNone) {
                     res = m - d0;
                     done = true;
                 } else {
-                    base += 128;
-                    const uint32_t cur = *cpp;
-                    if (cur != kNone && base - d0 >= cur) done = true;  // cannot lower the minimum
+                    base += 256;
+                    if (lim == kNone) lim = *cpp;
+                    if (lim != kNone && base - d0 >= lim) done = true;  // cannot lower the minimum
                 }
             }
         }
@@ -561,6 +568,19 @@ k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2
     }
 }
 
+// Offset and length per sorted position: every later stage streams these
+// instead of gathering through the permutation.
+__global__ void k_sorted_meta(const uint32_t* __restrict__ perm, const uint64_t* __restrict__ off,
+                              const uint32_t* __restrict__ len, uint64_t n, uint64_t* __restrict__ s_off,
+                              uint32_t* __restrict__ s_len) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const uint32_t id = perm[i];
+        s_off[i] = off[id];
+        s_len[i] = len[id];
+    }
+}
+
 // ------------------------------------------------------------------ driver
 void sort_lcp(dss_set_s* s) {
     const uint64_t n = s->n;
@@ -704,6 +724,13 @@ void sort_lcp(dss_set_s* s) {
         KScope ks(s->timer, st, "small_sort", (double)cls[kClasses + c] * 32.0);
         hipLaunchKernelGGL(k_small_sort, dim3(blocks), dim3(kThreads), 0, st,
                            X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len);
+    }
+    {
+        KScope ks(s->timer, st, "sort_meta", n * 28.0);
+        s->s_off.reserve(n * 8);
+        s->s_len.reserve(n * 4);
+        hipLaunchKernelGGL(k_sorted_meta, dim3((n + 255) / 256), dim3(256), 0, st, perm, off, len, n,
+                           s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>());
     }
     DSS_HIP(hipGetLastError());
     s->sorted = true;
