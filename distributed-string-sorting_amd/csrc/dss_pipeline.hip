@@ -1182,7 +1182,7 @@ extern "C" int dss_check(dss_set s, dss_check_t* out) {
         }
         const uint64_t nw = (n + 31) / 32;
         DevBuf buf;
-        buf.reserve(n * 4 + nw * 4 + 64 + (s->p + 1) * 8);
+        buf.reserve((n * 4 + nw * 4 + 7) / 8 * 8 + 64 + (s->p + 1) * 8);
         uint32_t* glcp = buf.as<uint32_t>();
         unsigned int* seen = reinterpret_cast<unsigned int*>(glcp + n);
         unsigned long long* cnt = reinterpret_cast<unsigned long long*>(buf.as<uint8_t>() + (n * 4 + nw * 4 + 7) / 8 * 8);
@@ -1205,5 +1205,39 @@ extern "C" int dss_check(dss_set s, dss_check_t* out) {
         out->sorted = h[0] == 0;
         out->complete = h[2] == 0 && h[4] == n;
         out->dist_prefix_size = h[3];
+    });
+}
+
+extern "C" int dss_set_interval(dss_set s, const uint64_t* interval) {
+    return guarded([&] {
+        DSS_REQUIRE(s && interval, DSS_EINVAL, "NULL argument");
+        DSS_REQUIRE(s->sorted, DSS_ESTATE, "no local order (dss_sort_lcp or dss_set_local_order)");
+        const uint32_t p = s->p;
+        for (uint32_t r = 0; r < p; ++r) {
+            uint64_t sum = 0;
+            for (uint32_t q = 0; q < p; ++q) sum += interval[(uint64_t)r * p + q];
+            DSS_REQUIRE(sum == s->pe_begin[r + 1] - s->pe_begin[r], DSS_EINVAL,
+                        "interval row must sum to the PE's string count");
+        }
+        PipeState& P = s->pp;
+        P.interval.assign(interval, interval + (uint64_t)p * p);
+        P.interval_d.reserve((uint64_t)p * p * 8);
+        DSS_HIP(hipMemcpyAsync(P.interval_d.p, interval, (uint64_t)p * p * 8, hipMemcpyHostToDevice, s->stream));
+        DSS_HIP(hipStreamSynchronize(s->stream));
+        P.partitioned = true;
+        P.exchanged = P.merged = P.extended = false;
+    });
+}
+
+extern "C" int dss_set_dist(dss_set s, const uint32_t* dist) {
+    return guarded([&] {
+        DSS_REQUIRE(s && (s->n == 0 || dist), DSS_EINVAL, "NULL argument");
+        DSS_REQUIRE(s->sorted, DSS_ESTATE, "no local order (dss_sort_lcp or dss_set_local_order)");
+        PipeState& P = s->pp;
+        P.dist.reserve(std::max<uint64_t>(s->n, 1) * 4);
+        if (s->n) DSS_HIP(hipMemcpyAsync(P.dist.p, dist, s->n * 4, hipMemcpyHostToDevice, s->stream));
+        DSS_HIP(hipStreamSynchronize(s->stream));
+        P.has_dist = true;
+        P.partitioned = P.exchanged = P.merged = P.extended = false;
     });
 }

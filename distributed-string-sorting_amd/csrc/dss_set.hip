@@ -552,3 +552,68 @@ int dss_kernel_stats(dss_set s, dss_kernel_stat_t* stats, uint32_t cap, uint32_t
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ explicit PEs
+namespace dss {
+__global__ void k_pe_from_bytes(const uint64_t* __restrict__ off, uint64_t n, const uint64_t* __restrict__ byte_begin,
+                                uint32_t p, uint64_t* __restrict__ pe_begin) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > p) return;
+    if (r == p) { pe_begin[p] = n; return; }
+    const uint64_t b = byte_begin[r];
+    uint64_t lo = 0, hi = n;  // first string starting at or after b
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (off[mid] < b) lo = mid + 1; else hi = mid;
+    }
+    pe_begin[r] = lo;
+}
+}  // namespace dss
+
+extern "C" int dss_set_create_pes(const uint8_t* chars, const uint64_t* pe_byte_begin, uint32_t num_pes,
+                                  dss_set* out) {
+    return guarded([&] {
+        DSS_REQUIRE(out && pe_byte_begin, DSS_EINVAL, "NULL argument");
+        const uint64_t nbytes = pe_byte_begin[num_pes];
+        for (uint32_t r = 0; r < num_pes; ++r)
+            DSS_REQUIRE(pe_byte_begin[r] <= pe_byte_begin[r + 1], DSS_EINVAL, "PE byte ranges must be ordered");
+        DSS_REQUIRE(pe_byte_begin[0] == 0, DSS_EINVAL, "PE byte ranges must start at 0");
+        DSS_REQUIRE(nbytes == 0 || chars != nullptr, DSS_EINVAL, "chars is NULL");
+        for (uint32_t r = 1; r <= num_pes; ++r)
+            DSS_REQUIRE(pe_byte_begin[r] == pe_byte_begin[r - 1] || chars[pe_byte_begin[r] - 1] == 0, DSS_EINVAL,
+                        "every PE's bytes must end with a NUL terminator");
+        dss_set_s* s = new_set(num_pes);
+        try {
+            s->nbytes = nbytes;
+            s->chars.reserve(nbytes + kArenaPad);
+            DSS_HIP(hipMemsetAsync(s->chars.p, 0, nbytes + kArenaPad, s->stream));
+            if (nbytes) DSS_HIP(hipMemcpyAsync(s->chars.p, chars, nbytes, hipMemcpyHostToDevice, s->stream));
+            index_arena(s);
+            DevBuf bb;
+            bb.reserve((num_pes + 1) * 8);
+            DSS_HIP(hipMemcpyAsync(bb.p, pe_byte_begin, (num_pes + 1) * 8, hipMemcpyHostToDevice, s->stream));
+            s->pe_begin_d.reserve((num_pes + 1) * 8);
+            DevBuf dummy;
+            const uint64_t* offp = s->off.as<uint64_t>();
+            if (!offp) {
+                dummy.reserve(8);
+                offp = dummy.as<uint64_t>();
+            }
+            hipLaunchKernelGGL(k_pe_from_bytes, dim3((num_pes + 1 + 255) / 256), dim3(256), 0, s->stream, offp, s->n,
+                               bb.as<uint64_t>(), num_pes, s->pe_begin_d.as<uint64_t>());
+            s->pe_begin.assign(num_pes + 1, 0);
+            DSS_HIP(hipMemcpyAsync(s->pe_begin.data(), s->pe_begin_d.p, (num_pes + 1) * 8, hipMemcpyDeviceToHost,
+                                   s->stream));
+            DSS_HIP(hipStreamSynchronize(s->stream));
+            uint32_t mx = 0;
+            std::vector<uint32_t> lens(s->n);
+            if (s->n) DSS_HIP(hipMemcpy(lens.data(), s->len.p, s->n * 4, hipMemcpyDeviceToHost));
+            for (auto v : lens) mx = std::max(mx, v);
+            s->max_len = mx;
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}

@@ -762,3 +762,29 @@ int dss_get_sorted(dss_set s, uint32_t* perm, uint32_t* lcp) {
 }
 
 }  // extern "C"
+
+extern "C" int dss_set_local_order(dss_set s, const uint32_t* perm, const uint32_t* lcp) {
+    return guarded([&] {
+        DSS_REQUIRE(s && (s->n == 0 || (perm && lcp)), DSS_EINVAL, "NULL argument");
+        const uint64_t n = s->n;
+        s->perm.reserve(std::max<uint64_t>(n, 1) * 4);
+        s->lcp.reserve(std::max<uint64_t>(n, 1) * 4);
+        s->s_off.reserve(std::max<uint64_t>(n, 1) * 8);
+        s->s_len.reserve(std::max<uint64_t>(n, 1) * 4);
+        if (n) {
+            for (uint32_t r = 0; r < s->p; ++r)
+                for (uint64_t i = s->pe_begin[r]; i < s->pe_begin[r + 1]; ++i)
+                    DSS_REQUIRE(perm[i] >= s->pe_begin[r] && perm[i] < s->pe_begin[r + 1], DSS_EINVAL,
+                                "perm must permute every PE's own strings");
+            DSS_HIP(hipMemcpyAsync(s->perm.p, perm, n * 4, hipMemcpyHostToDevice, s->stream));
+            DSS_HIP(hipMemcpyAsync(s->lcp.p, lcp, n * 4, hipMemcpyHostToDevice, s->stream));
+            hipLaunchKernelGGL(k_sorted_meta, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->perm.as<uint32_t>(),
+                               s->off.as<uint64_t>(), s->len.as<uint32_t>(), n, s->s_off.as<uint64_t>(),
+                               s->s_len.as<uint32_t>());
+            DSS_HIP(hipGetLastError());
+            DSS_HIP(hipStreamSynchronize(s->stream));
+        }
+        s->sorted = true;
+        s->pp.has_dist = s->pp.partitioned = s->pp.exchanged = s->pp.merged = s->pp.extended = false;
+    });
+}

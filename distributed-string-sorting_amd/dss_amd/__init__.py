@@ -45,6 +45,19 @@ class DeviceStringSet:
         return cls(h)
 
     @classmethod
+    def from_pes(cls, pe_arenas) -> "DeviceStringSet":
+        """One NUL-terminated arena per PE (the per-rank inputs)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in pe_arenas]
+        bb = np.zeros(len(arrs) + 1, dtype=np.uint64)
+        bb[1:] = np.cumsum([a.size for a in arrs])
+        chars = np.concatenate(arrs) if arrs else np.zeros(0, np.uint8)
+        if chars.size == 0:
+            chars = np.zeros(1, np.uint8)
+        h = _lib.set_p()
+        check(lib().dss_set_create_pes(_p(chars, _lib.u8p), _p(bb, _lib.u64p), len(arrs), C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def dton(cls, n: int, length: int, dton: float, seed: int, num_pes: int) -> "DeviceStringSet":
         h = _lib.set_p()
         check(lib().dss_set_create_dton(n, length, dton, seed, num_pes, C.byref(h)))
@@ -117,6 +130,19 @@ class DeviceStringSet:
         lcp = np.zeros(max(n, 1), dtype=np.uint32)
         check(lib().dss_get_sorted(self._h, _p(perm, _lib.u32p), _p(lcp, _lib.u32p)))
         return perm[:n], lcp[:n]
+
+    def set_local_order(self, perm, lcp):
+        perm = np.ascontiguousarray(perm, dtype=np.uint32)
+        lcp = np.ascontiguousarray(lcp, dtype=np.uint32)
+        check(lib().dss_set_local_order(self._h, _p(perm, _lib.u32p), _p(lcp, _lib.u32p)))
+
+    def set_interval(self, interval):
+        iv = np.ascontiguousarray(interval, dtype=np.uint64).reshape(-1)
+        check(lib().dss_set_interval(self._h, _p(iv, _lib.u64p)))
+
+    def set_dist(self, dist):
+        d = np.ascontiguousarray(dist, dtype=np.uint32)
+        check(lib().dss_set_dist(self._h, _p(d, _lib.u32p)))
 
     def avg_lcp(self) -> int:
         out = C.c_uint64()

@@ -51,12 +51,16 @@ _SIGS = {
     "dss_set_create_skewed": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                         C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
                                         C.POINTER(set_p)]),
+    "dss_set_create_pes": (C.c_int, [u8p, u64p, C.c_uint32, C.POINTER(set_p)]),
     "dss_set_destroy": (C.c_int, [set_p]),
     "dss_set_info": (C.c_int, [set_p, C.POINTER(SetInfo)]),
     "dss_set_pe_begin": (C.c_int, [set_p, u64p]),
     "dss_set_download": (C.c_int, [set_p, u8p, u64p, u32p]),
     "dss_sort_lcp": (C.c_int, [set_p]),
     "dss_get_sorted": (C.c_int, [set_p, u32p, u32p]),
+    "dss_set_local_order": (C.c_int, [set_p, u32p, u32p]),
+    "dss_set_interval": (C.c_int, [set_p, u64p]),
+    "dss_set_dist": (C.c_int, [set_p, u32p]),
     "dss_avg_lcp": (C.c_int, [set_p, u64p]),
     "dss_partition": (C.c_int, [set_p, C.c_int, C.c_uint64, C.c_uint64, C.c_int, u64p]),
     "dss_exchange": (C.c_int, [set_p, C.c_int, u64p, u64p, u64p]),

@@ -72,6 +72,12 @@ int dss_set_create_skewed(uint64_t n, uint32_t long_len, uint32_t short_len,
                           uint32_t dup_permille, uint64_t seed, uint32_t num_pes,
                           dss_set* out);
 
+/* Uploads num_pes PE-local inputs laid out back to back (each PE's bytes a
+ * sequence of NUL-terminated strings): pe_byte_begin[0..num_pes] are the byte
+ * boundaries (the per-rank input of the reference's generators). */
+int dss_set_create_pes(const uint8_t* chars, const uint64_t* pe_byte_begin, uint32_t num_pes,
+                       dss_set* out);
+
 int dss_set_destroy(dss_set s);
 
 typedef struct {
@@ -99,6 +105,10 @@ int dss_set_download(dss_set s, uint8_t* chars, uint64_t* offsets, uint32_t* len
 int dss_sort_lcp(dss_set s);
 /* perm/lcp: num_strings entries each (either may be NULL). */
 int dss_get_sorted(dss_set s, uint32_t* perm, uint32_t* lcp);
+/* Sets a caller-computed local order instead of dss_sort_lcp (a StringLcpPtr
+ * whose strings and LCPs the caller arranged, as the reference's exchange
+ * tests do): perm permutes every PE's own string ids, lcp per position. */
+int dss_set_local_order(dss_set s, const uint32_t* perm, const uint32_t* lcp);
 
 /* ---- (a3) getAvgLcp (src/sorter/distributed/misc.hpp:67-87) ---- */
 int dss_avg_lcp(dss_set s, uint64_t* avg);
@@ -113,6 +123,14 @@ int dss_avg_lcp(dss_set s, uint64_t* avg);
  *      interval[r*p+q] = strings PE r sends to PE q ---- */
 int dss_partition(dss_set s, int sampler, uint64_t sampling_factor, uint64_t max_len,
                   int use_dist, uint64_t* interval);
+
+/* Sets caller-computed send counts (the sendCounts argument of the exchange
+ * policies' alltoallv, src/mpi/alltoall.hpp:642-1009) instead of
+ * dss_partition; interval[r*p+q] as above, rows summing to the PE sizes. */
+int dss_set_interval(dss_set s, const uint64_t* interval);
+/* Sets caller-computed distinguishing prefixes (the distinguishingPrefix
+ * argument of AllToAllStringImplPrefixDoubling::alltoallv) per position. */
+int dss_set_dist(dss_set s, const uint32_t* dist);
 
 /* ---- (a6, a7, a13) AllToAllStringImpl<..., EmptyByteEncoderMemCpy> (mode 0),
  *      <..., EmptyLcpByteEncoderMemCpy> (mode 1) and

@@ -161,6 +161,18 @@ class StringSet:
         offc = self.off if n else np.zeros(1, dtype=np.uint64)
         lib().orc_pe_split(P(offc, u64p), n, chars.size, p, P(self.pe_begin, u64p))
 
+    @classmethod
+    def from_pes(cls, pe_arenas):
+        """One NUL-terminated arena per PE (per-rank inputs, no byte slicing)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in pe_arenas]
+        chars = np.concatenate(arrs) if arrs else np.zeros(0, np.uint8)
+        self = cls(chars, 1)
+        counts = [int(lib().orc_count_strings(P(a, u8p), a.size)) if a.size else 0 for a in arrs]
+        self.p = len(arrs)
+        self.pe_begin = np.zeros(self.p + 1, dtype=np.uint64)
+        self.pe_begin[1:] = np.cumsum(counts)
+        return self
+
     def string(self, i) -> bytes:
         o = int(self.off[i])
         return self.chars[o:o + int(self.len[i])].tobytes()

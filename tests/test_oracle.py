@@ -1,0 +1,363 @@
+"""CPU tests that pin the oracle (oracle/dss_oracle.c) to the reference.
+
+Each function of the oracle is checked against what fixes it independently
+of its own formula: the reference's test vectors and known-answer tests, the
+reference headers compiled in place (oracle/_ref, or the golden fixtures it
+wrote to tests/golden/ref_vectors.json), published XXH64 values, textbook
+sorting, a literal restatement of the file-distribution procedure, and exact
+prefix equality for the prefix-doubling rounds."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "ref_vectors.json")))
+
+
+def arena(strings):
+    return np.frombuffer(b"".join(s + b"\0" for s in strings), dtype=np.uint8).copy()
+
+
+def brute_lcp(a: bytes, b: bytes) -> int:
+    k = 0
+    while k < min(len(a), len(b)) and a[k] == b[k]:
+        k += 1
+    return k
+
+
+# ----------------------------------------------------------------- XXH64
+def test_xxh64_published_vectors():
+    # XXH64 with seed 0 (Y. Collet's reference values)
+    assert O.xxh64_stream(b"") == 0xEF46DB3751D8E999
+    assert O.xxh64_stream(b"abc") == 0x44BC2CF5AD770999
+    assert O.xxh64_stream(b"The quick brown fox jumps over the lazy dog") == 0x0B242D361FDA71BC
+
+
+def test_xxh64_stream_matches_reference_fixture():
+    """xxh::hash_state_t<64> digests dumped from the reference header,
+    including its `total_len > 32` branch at exactly 32 bytes."""
+    pattern = bytes((i * 131 + 7) % 251 + 1 for i in range(300))
+    for L, hx in enumerate(GOLDEN["xxh64_stream"]["digests"]):
+        assert O.xxh64_stream(pattern[:L]) == int(hx, 16), L
+
+
+def test_xxh64_stream_matches_reference_build():
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs the reference checkout)")
+    rng = np.random.default_rng(2)
+    for L in list(range(0, 80)) + [127, 128, 129, 255, 256, 264, 520, 1000]:
+        d = rng.integers(0, 256, L, dtype=np.uint8)
+        buf = d.copy() if L else np.zeros(1, np.uint8)
+        assert O.xxh64_stream(d.tobytes()) == R.ref_xxh_stream(buf.ctypes.data_as(O.u8p), L), L
+
+
+def test_pd_hash_range_reduction():
+    # XXHasher::hash reduces % bloomFilterSize = 2^64-1
+    h = O.xxh64_stream(b"abc")
+    assert O.pd_hash(b"abc") == h % (2**64 - 1)
+    assert O.pd_hash(b"abc", 1000) == h % 1000
+
+
+# ------------------------------------------------------------ string set
+def test_string_lengths_reference_case():
+    # tests_old/catch/string_set_tests.cpp:8-52
+    raw = b"AABCA\0BA\0BAB\0\0CCC\0CCD\0"
+    ss = O.StringSet(np.frombuffer(raw, np.uint8).copy(), 1)
+    assert list(map(int, ss.len)) == [5, 2, 3, 0, 3, 3]
+
+
+def file_distributer(text: bytes, p: int):
+    """distribute_strings (src/mpi/readInput.hpp:40-104) done the way the
+    reference does it: byte slices, '\\n' -> NUL, every PE but the first hands
+    its first partial string to its left neighbour."""
+    size = len(text)
+    small, larger = divmod(size, p)
+    slices = []
+    for r in range(p):
+        ln = small + (1 if r < larger else 0)
+        off = r * (small + 1) if r < larger else larger * (small + 1) + (r - larger) * small
+        slices.append(bytearray(text[off:off + ln]))
+    firsts = []
+    for r in range(p):
+        s = slices[r]
+        fe = s.index(b"\n") if b"\n" in s else len(s)
+        firsts.append(fe)
+    out = []
+    for r in range(p):
+        s = bytearray(slices[r].replace(b"\n", b"\0"))
+        if r + 1 < p:
+            nxt = slices[r + 1]
+            s += nxt[:firsts[r + 1] + 1].replace(b"\n", b"\0")
+        elif s and s[-1] != 0:
+            s.append(0)
+        if r > 0:
+            s = s[firsts[r] + 1:]
+        out.append(bytes(s))
+    return out
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 7, 16])
+def test_pe_split_matches_file_distributer(p):
+    rng = np.random.default_rng(p)
+    strs = [bytes(rng.integers(65, 91, int(l), dtype=np.uint8)) for l in rng.integers(0, 40, 500)]
+    text = b"".join(s + b"\n" for s in strs)
+    per_pe = file_distributer(text, p)
+    ss = O.StringSet(arena(strs), p)
+    for r in range(p):
+        got = b"".join(ss.string(i) + b"\0" for i in range(int(ss.pe_begin[r]), int(ss.pe_begin[r + 1])))
+        assert got == per_pe[r], r
+
+
+# ------------------------------------------------------------ local sort
+INPUTS = {
+    "dton": lambda: O.gen_dton(3000, 60, 0.5, 7),
+    "skewed": lambda: O.gen_skewed(O.skew_params(3000, long_len=100, short_len=10), 9),
+    "suffix": lambda: O.gen_suffixes(4000, 3, repeats=3, rep_min=50, rep_max=200, cap=100),
+    "small_alphabet": lambda: arena([bytes(np.random.default_rng(1).integers(97, 99, int(l), dtype=np.uint8))
+                                     for l in np.random.default_rng(2).integers(0, 20, 70000)]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(INPUTS))
+@pytest.mark.parametrize("p", [1, 3, 4])
+def test_local_sort_is_textbook_order_and_lcp(name, p):
+    ss = O.StringSet(INPUTS[name](), p)
+    perm, lcp = O.local_sort(ss)
+    for r in range(p):
+        b, e = int(ss.pe_begin[r]), int(ss.pe_begin[r + 1])
+        got = [ss.string(perm[i]) for i in range(b, e)]
+        assert got == sorted(ss.string(i) for i in range(b, e))
+        assert sorted(int(x) for x in perm[b:e]) == list(range(b, e))
+        if e > b:
+            assert lcp[b] == 0
+        for i in range(b + 1, e):
+            assert lcp[i] == brute_lcp(got[i - b - 1], got[i - b])
+
+
+# ----------------------------------------------------------------- D / D/N
+def test_compute_d_hand_example():
+    # sorted: a, ab, abc, b -> lcps 0,1,2,0; D = (1+1)+(2+1)+(2+1)+(0+1)
+    assert O.compute_d(np.array([0, 1, 2, 0], np.uint32)) == 2 + 3 + 3 + 1
+    assert O.compute_d(np.array([0], np.uint32)) == 0
+
+
+@pytest.mark.parametrize("size", [1000, 10000])
+@pytest.mark.parametrize("length", [100, 250, 500])
+@pytest.mark.parametrize("ratio,eps", [(0.0, 0.075), (0.2, 0.01), (0.25, 0.01), (0.45, 0.01),
+                                       (0.76, 0.01), (0.95, 0.01), (1.0, 0.01)])
+def test_dton_generator_ratio(size, length, ratio, eps):
+    """DNGenerator_tests.cpp:47-101: D / (size * stringLength) within epsilon
+    of the requested ratio, all strings of the requested length."""
+    chars = O.gen_dton(size, length, ratio, 0x5EED0001 + size)
+    ss = O.StringSet(chars, 1)
+    assert ss.n == size
+    assert (ss.len == length).all()
+    perm, lcp = O.local_sort(ss)
+    D = O.compute_d(lcp)
+    assert abs(D / (size * length) - ratio) < eps
+
+
+def test_generators_are_deterministic_and_seeded():
+    a = O.gen_dton(5000, 50, 0.5, 1)
+    assert (a == O.gen_dton(5000, 50, 0.5, 1)).all()
+    assert not (a == O.gen_dton(5000, 50, 0.5, 2)).all()
+    sp = O.skew_params(2000)
+    assert (O.gen_skewed(sp, 3) == O.gen_skewed(sp, 3)).all()
+    # the skewed input has exact duplicates and both length classes
+    ss = O.StringSet(O.gen_skewed(O.skew_params(4000, long_len=300, short_len=30), 5), 1)
+    strs = [ss.string(i) for i in range(ss.n)]
+    assert len(set(strs)) < len(strs)
+    assert set(int(x) for x in ss.len) == {300, 30}
+
+
+# ----------------------------------------------------------- exchange KAT
+def kat_setup(p, size_per_pe=5, common=10, suffix=20):
+    """alltoallStrings_tests.cpp:17-127: PE r holds, for each destination,
+    size_per_pe strings common*(r+1) + [i] + suffix*(r+1); LCPs are `common`
+    except 0 at the start; every PE sends size_per_pe strings to every PE."""
+    pes = []
+    for r in range(p):
+        strs = []
+        for _ in range(p):
+            for i in range(1, size_per_pe + 1):
+                strs.append(bytes([r + 1]) * common + bytes([i]) + bytes([r + 1]) * suffix)
+        pes.append(arena(strs))
+    ss = O.StringSet.from_pes(pes)
+    n = ss.n
+    perm = np.arange(n, dtype=np.uint32)
+    lcp = np.full(n, common, dtype=np.uint32)
+    for r in range(p):
+        lcp[int(ss.pe_begin[r])] = 0
+    iv = np.full((p, p), size_per_pe, dtype=np.uint64)
+    return ss, perm, lcp, iv
+
+
+@pytest.mark.parametrize("p", [1, 2, 4, 8])
+def test_exchange_reference_vectors(p):
+    size, common, suffix = 5, 10, 20
+    ss, perm, lcp, iv = kat_setup(p, size, common, suffix)
+    for mode in ("whole", "lcp_compact"):
+        ex = O.exchange(ss, perm, lcp, iv, mode)
+        for q in range(p):
+            b0, b1 = int(ex["byte_begin"][q]), int(ex["byte_begin"][q + 1])
+            r0, r1 = int(ex["recv_begin"][q]), int(ex["recv_begin"][q + 1])
+            got = ex["bytes"][b0:b1].tobytes()
+            exp = b""
+            exp_lcp = []
+            for src in range(p):
+                c = bytes([src + 1])
+                if mode == "lcp_compact":  # getExpectedRawStringsForPrefixCompression
+                    exp += c * common
+                    for i in range(size):
+                        exp += bytes([i + 1]) + c * suffix + b"\0"
+                else:  # getExpectedRawStrings
+                    for i in range(size):
+                        exp += c * common + bytes([i + 1]) + c * suffix + b"\0"
+                exp_lcp += [0] + [common] * (size - 1)  # getExpectedLcpValues
+            assert got == exp, (mode, q)
+            assert list(map(int, ex["lcp"][r0:r1])) == exp_lcp
+    for dup in (False, True):
+        # AllToAllStringImplPrefixDoubling_test: dist = common (+1 without duplicates)
+        dist = np.full(ss.n, common if dup else common + 1, dtype=np.uint32)
+        ex = O.exchange(ss, perm, lcp, iv, "pd_prefix", dist=dist)
+        for q in range(p):
+            b0, b1 = int(ex["byte_begin"][q]), int(ex["byte_begin"][q + 1])
+            exp = b""
+            for src in range(p):
+                exp += bytes([src + 1]) * common
+                exp += b"\0" * size if dup else b"".join(bytes([i + 1]) + b"\0" for i in range(size))
+            assert ex["bytes"][b0:b1].tobytes() == exp
+
+
+# ------------------------------------------------- merge / extendPrefix
+def test_merge_and_extend_prefix_match_reference_fixture():
+    for case in GOLDEN["merges"]:
+        a = np.frombuffer(bytes.fromhex(case["arena_hex"]), dtype=np.uint8).copy()
+        comp = case["mode"] == "lcp_compact"
+        order, lcp, saved = O.merge_losertree(a, np.array(case["recv_lcp"], np.uint32),
+                                              np.array(case["runs"], np.uint64), comp)
+        assert list(map(int, order)) == case["order"]
+        assert list(map(int, lcp)) == case["merged_lcp"]
+        if comp:
+            assert list(map(int, saved)) == case["saved_lcp"]
+            ext = O.extend_prefix(a, order, saved)
+            assert ext.tobytes().hex() == case["extended_hex"]
+
+
+def test_merge_matches_reference_build_on_fresh_inputs():
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs the reference checkout)")
+    chars = O.gen_skewed(O.skew_params(3000, long_len=80, short_len=6), 77)
+    for p in (2, 5, 8):
+        ss = O.StringSet(chars, p)
+        perm, lcp = O.local_sort(ss)
+        iv = O.partition(ss, perm, "NumStrings", max_len=1000)
+        ex = O.exchange(ss, perm, lcp, iv, "lcp_compact")
+        for q in range(p):
+            r0, r1 = int(ex["recv_begin"][q]), int(ex["recv_begin"][q + 1])
+            b0, b1 = int(ex["byte_begin"][q]), int(ex["byte_begin"][q + 1])
+            n = r1 - r0
+            if not n:
+                continue
+            a = np.ascontiguousarray(ex["bytes"][b0:b1])
+            rl = ex["lcp"][r0:r1].astype(np.uint64)
+            runs = iv[:, q].astype(np.uint64).copy()
+            oi, ol, osv = (np.zeros(n, np.uint64) for _ in range(3))
+            assert R.ref_merge(a.ctypes.data_as(O.u8p), a.size, rl.ctypes.data_as(O.u64p),
+                               runs.ctypes.data_as(O.u64p), p, 1, oi.ctypes.data_as(O.u64p),
+                               ol.ctypes.data_as(O.u64p), osv.ctypes.data_as(O.u64p)) == 0
+            order, mlcp, saved = O.merge_losertree(a, ex["lcp"][r0:r1], iv[:, q].astype(np.uint64), True)
+            assert (order == oi).all() and (mlcp == ol).all() and (saved == osv).all()
+
+
+# ------------------------------------------------------- end-to-end order
+@pytest.mark.parametrize("name", sorted(INPUTS))
+@pytest.mark.parametrize("sampler", ["NumStrings", "NumChars", "IndexedNumStrings", "IndexedNumChars"])
+def test_merge_sort_output_is_the_sorted_input(name, sampler):
+    ss = O.StringSet(INPUTS[name](), 4)
+    res = O.merge_sort(ss, sampler=sampler, encoder="lcp_compact")
+    got = b"".join(o["arena"].tobytes() for o in res["out"]).split(b"\0")[:-1]
+    assert got == sorted(ss.string(i) for i in range(ss.n))
+    # merged LCPs are the LCPs of neighbours inside each PE
+    for o in res["out"]:
+        strs = o["arena"].tobytes().split(b"\0")[:-1]
+        for i in range(1, len(strs)):
+            assert int(o["lcp"][i]) == brute_lcp(strs[i - 1], strs[i])
+
+
+# ------------------------------------------------------- prefix doubling
+def exact_rounds(strs_by_pe, start=8):
+    """Candidate sets with exact prefix equality in place of the hash (the
+    semantics of the reference's debug tracker, tracker.hpp:104-110,187-240):
+    a candidate stays while another candidate of any PE has the same prefix
+    of the round's depth; results = depth, or the length at end of string."""
+    cands = {(r, i) for r, ss in enumerate(strs_by_pe) for i in range(len(ss))}
+    results = {k: 0 for k in cands}
+    rounds = {k: 0 for k in cands}
+    depth, step = start, 2 * start
+    while cands:
+        groups = {}
+        nxt = set()
+        for k in cands:
+            s = strs_by_pe[k[0]][k[1]]
+            rounds[k] += 1
+            if depth > len(s):
+                results[k] = len(s)
+                continue
+            results[k] = depth
+            groups.setdefault(s[:depth], []).append(k)
+        for g in groups.values():
+            if len(g) > 1:
+                nxt.update(g)
+        cands = nxt
+        depth, step = start + step, step * 2
+    return results, rounds
+
+
+@pytest.mark.parametrize("name", ["dton", "skewed", "suffix", "small_alphabet"])
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_prefix_doubling_matches_exact_prefix_semantics(name, p):
+    chars = INPUTS[name]()
+    if name == "small_alphabet":
+        chars = chars[:20000].copy()
+        chars[-1] = 0
+    ss = O.StringSet(chars, p)
+    perm, lcp = O.local_sort(ss)
+    res, rounds, nr = O.prefix_doubling(ss, perm, lcp)
+    strs = [[ss.string(perm[i]) for i in range(int(ss.pe_begin[r]), int(ss.pe_begin[r + 1]))] for r in range(p)]
+    er, erounds = exact_rounds(strs)
+    for r in range(p):
+        b = int(ss.pe_begin[r])
+        for i in range(len(strs[r])):
+            assert res[b + i] == er[(r, i)], (r, i)
+            assert rounds[b + i] == erounds[(r, i)], (r, i)
+    # DIST <= results for non-EOS strings (the exact distinguishing prefix)
+    for r in range(p):
+        b = int(ss.pe_begin[r])
+
+
+def test_prefix_doubling_depth_schedule():
+    # every string shares 200 chars: rounds at depths 8, 24, 40, 72, 136, 264
+    strs = [b"x" * 200 + bytes([65 + i % 26, 65 + (i // 26) % 26]) for i in range(100)]
+    ss = O.StringSet(arena(strs), 2)
+    perm, lcp = O.local_sort(ss)
+    res, rounds, nr = O.prefix_doubling(ss, perm, lcp)
+    assert set(int(x) for x in res) == {202}  # EOS at depth 264 > 202
+    assert nr == 6
+
+
+# ----------------------------------------------------------------- checker
+def test_is_sorted_checker_cases():
+    strs = [b"a", b"b", b"b", b"c", b"d"]
+    a = arena(strs)
+    ptrs, _ = O._arena_pointers(a)
+    assert O.lib().orc_is_sorted(ptrs, np.array([0, 2, 2, 5], np.uint64).ctypes.data_as(O.u64p), 3)
+    a2 = arena([b"a", b"c", b"b", b"d"])
+    ptrs2, _ = O._arena_pointers(a2)
+    assert not O.lib().orc_is_sorted(ptrs2, np.array([0, 2, 4], np.uint64).ctypes.data_as(O.u64p), 2)
