@@ -102,6 +102,124 @@ def cpu_baseline(n, length, dton, pes, seed, m):
     }
 
 
+def batched_leg(args):
+    """Plain vs batched vs served forms on the same mixed job stream.
+
+    plain:   every job alone (a batch of one), one after the other, on jobs
+             already resident in HBM; only the sorts are timed.
+    batched: the same resident jobs through the continuous-batching server
+             with all jobs ready at once (makespan timed).
+    served:  Poisson arrivals at rate lambda; each job is generated on the
+             device when admitted (its ingestion) into a pool of bounded
+             capacity; latency = completion - arrival."""
+    import numpy as np
+    from dss_amd import JobPool
+    from dss_amd.serve import Server, fifo_latencies, job_stream, percentiles, run_plain
+
+    specs = job_stream(args.serve_jobs, args.serve_n_min, args.serve_n_max, args.serve_len_min,
+                       args.serve_len_max, rate=0.0, seed=args.serve_seed)
+    cap_bytes = int(args.serve_pool_gb * (1 << 30))
+    cap_strings = max(2 * max(s.n for s in specs), 256 << 20)
+    # jobs are processed in groups that fit the pool at once
+    groups, cur, used, used_n = [], [], 0, 0
+    for s in specs:
+        b = s.max_bytes() + 8
+        if cur and (used + b > cap_bytes or used_n + s.n > cap_strings):
+            groups.append(cur)
+            cur, used, used_n = [], 0, 0
+        cur.append(s)
+        used += b
+        used_n += s.n
+    if cur:
+        groups.append(cur)
+    server = Server(None, inflight=args.serve_inflight, max_batch_strings=args.serve_batch_strings)
+    plain_t = {}
+    gen_t = {}
+    Dp = {}
+    Db = {}
+    batched_s = 0.0
+    batch_sizes = []
+    with JobPool(cap_bytes, cap_strings) as pool:
+        server.pool = pool
+        # warm the batch buffers at the largest batch size
+        w = job_stream(1, 1000, 1000, rate=0.0, seed=1)[0]
+        j = w.add_to(pool)
+        run_plain(pool, [w], resident={w.jid: j})
+        pool.free(j)
+        for g in groups:
+            ids = {}
+            for s in g:
+                t0 = time.perf_counter()
+                ids[s.jid] = s.add_to(pool)
+                gen_t[s.jid] = time.perf_counter() - t0
+            pr = run_plain(pool, g, resident=ids)
+            for jid, (D, t, _, _) in pr.items():
+                plain_t[jid] = t
+                Dp[jid] = D
+            t0 = time.perf_counter()
+            res = server.run(g, realtime=False, resident=ids)
+            batched_s += time.perf_counter() - t0
+            for r in res:
+                Db[r.jid] = r.D
+                batch_sizes.append(r.batch_jobs)
+            for jid in ids.values():
+                pool.free(jid)
+        log(f"batched leg: {len(specs)} jobs in {len(groups)} groups, plain {sum(plain_t.values()):.3f} s, "
+            f"batched {batched_s:.3f} s, generation {sum(gen_t.values()):.3f} s")
+    mism = [j for j in Dp if Dp[j] != Db[j]]
+    if mism:
+        log(f"batched D differs from plain for jobs {mism[:10]}")
+        sys.exit(1)
+    D_total = sum(Dp.values())
+    plain_gbs = D_total / sum(plain_t.values()) / 1e9
+    batched_gbs = D_total / batched_s / 1e9
+
+    # served: Poisson arrivals at `load` times the batched rate including ingestion
+    busy = batched_s + sum(gen_t.values())
+    lam = args.serve_load * len(specs) / busy
+    sspecs = job_stream(args.serve_jobs, args.serve_n_min, args.serve_n_max, args.serve_len_min,
+                        args.serve_len_max, rate=lam, seed=args.serve_seed)
+    serve_cap = int(args.serve_served_pool_gb * (1 << 30))
+    with JobPool(serve_cap, cap_strings) as pool:
+        server.pool = pool
+        t0 = time.perf_counter()
+        res = server.run(sspecs, realtime=True)
+        wall = time.perf_counter() - t0
+    Ds = {r.jid: r.D for r in res}
+    if any(Ds[j] != Dp[j] for j in Dp):
+        log("served D differs from plain")
+        sys.exit(1)
+    lat = [r.latency for r in res]
+    span = max(r.done for r in res) - min(r.arrival for r in res)
+    plain_lat = fifo_latencies(sspecs, {j: plain_t[j] + gen_t[j] for j in plain_t})
+    n_total = sum(s.n for s in specs)
+    return {
+        "workload": f"{len(specs)} jobs, sizes log-uniform {args.serve_n_min}-{args.serve_n_max} strings, "
+                    f"lengths U[{args.serve_len_min},{args.serve_len_max}], D/N U[0.1,0.9], "
+                    f"D/N-ratio and skewed generators, seed {args.serve_seed:#x}",
+        "jobs": len(specs),
+        "strings": n_total,
+        "D_bytes": D_total,
+        "plain_gbs": round(plain_gbs, 3),
+        "batched_gbs": round(batched_gbs, 3),
+        "batched_over_plain": round(batched_gbs / plain_gbs, 3),
+        "plain_strings_per_s": round(n_total / sum(plain_t.values()), 1),
+        "batched_strings_per_s": round(n_total / batched_s, 1),
+        "mean_jobs_per_batch": round(float(np.mean(batch_sizes)), 2),
+        "served": {
+            "lambda_jobs_per_s": round(lam, 3),
+            "load": args.serve_load,
+            "pool_gib": args.serve_served_pool_gb,
+            "inflight_batches": args.serve_inflight,
+            "max_batch_strings": args.serve_batch_strings,
+            "gbs": round(D_total / span / 1e9, 3),
+            "wall_s": round(wall, 3),
+            **percentiles(lat),
+            "plain_fifo": percentiles(plain_lat),
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +236,20 @@ def main():
                          "0 = 2^64-1 as in the reference")
     ap.add_argument("--cpu-sample", type=int, default=10_000_000,
                     help="strings in the CPU-oracle sample (0 disables the CPU leg)")
+    ap.add_argument("--serve-jobs", type=int, default=1000,
+                    help="jobs of the batched/served leg (0 disables it)")
+    ap.add_argument("--serve-n-min", type=int, default=1000)
+    ap.add_argument("--serve-n-max", type=int, default=200_000)
+    ap.add_argument("--serve-len-min", type=int, default=10)
+    ap.add_argument("--serve-len-max", type=int, default=1000)
+    ap.add_argument("--serve-seed", type=lambda x: int(x, 0), default=0x5EED0005)
+    ap.add_argument("--serve-load", type=float, default=0.8)
+    ap.add_argument("--serve-inflight", type=int, default=2)
+    ap.add_argument("--serve-batch-strings", type=int, default=32 << 20)
+    ap.add_argument("--serve-pool-gb", type=float, default=48.0,
+                    help="pool capacity of the plain/batched comparison (GiB of characters)")
+    ap.add_argument("--serve-served-pool-gb", type=float, default=8.0,
+                    help="pool capacity of the served run (the admission limit)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -164,6 +296,12 @@ def main():
     if roof:
         roof["launches_per_step"] = stats[roof["kernel"]]["launches"] / args.steps
 
+    ds.close()
+    batched = None
+    if args.serve_jobs > 0:
+        log(f"batched/served leg: {args.serve_jobs} jobs ...")
+        batched = batched_leg(args)
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"CPU oracle on {args.cpu_sample} strings ...")
@@ -171,7 +309,7 @@ def main():
         cpu["cores"] = 1
 
     out = {
-        "metric": "distinguishing-prefix GB/s at 1 GPU (prefix-doubling distributed string sort)",
+        "metric": "distinguishing-prefix GB/s at 1 GPU (batched vs plain); p50/p99 job latency",
         "value": round(D / step_s / 1e9, 3),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -197,6 +335,7 @@ def main():
         },
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -76,6 +76,16 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Grid of a grid-stride kernel over `items` work items with `per_block`
+// items per block.  A dispatch covers fewer than 2^32 work items, so kernels
+// whose item count can reach that (bytes or lanes of a large arena) loop over
+// the items with a grid capped at kMaxGridBlocks.
+constexpr uint32_t kMaxGridBlocks = 1u << 20;
+inline uint32_t stride_grid(uint64_t items, uint32_t per_block) {
+    const uint64_t b = (items + per_block - 1) / per_block;
+    return (uint32_t)(b < 1 ? 1 : (b > kMaxGridBlocks ? kMaxGridBlocks : b));
+}
+
 // ------------------------------------------------------------- kernel timing
 struct KernelTimer {
     bool enabled = false;
@@ -186,7 +196,9 @@ struct dss_set_s {
     uint32_t max_len = 0;
     std::vector<uint64_t> pe_begin;  // host copy, p+1
 
-    dss::DevBuf chars;  // nbytes + padding
+    dss::DevBuf chars;  // nbytes + padding (owned arena)
+    const uint8_t* borrowed = nullptr;  // arena owned by a job pool (batches)
+    const uint8_t* arena() const { return borrowed ? borrowed : chars.as<uint8_t>(); }
     dss::DevBuf off;    // u64[n]
     dss::DevBuf len;    // u32[n]
     dss::DevBuf pe_begin_d;  // u64[p+1]
@@ -200,6 +212,7 @@ struct dss_set_s {
 
     dss::SortScratch ss;
     dss::PipeState pp;
+    dss::DevBuf aux;    // small per-call scratch of result queries
     dss::KernelTimer timer;
 };
 

@@ -356,7 +356,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
             KScope ks(s->timer, st, "pd_hash", 0.0);
             hipLaunchKernelGGL(k_pd_classify, dim3(std::min<uint64_t>((n + 63) / 64, kPdGrid)), blk, 0, st, cand, s->pe_begin_d.as<uint64_t>(), p,
                                s->lcp.as<uint32_t>(), s->s_off.as<uint64_t>(),
-                               s->s_len.as<uint32_t>(), s->chars.as<uint8_t>(), (uint32_t)depth, m, n,
+                               s->s_len.as<uint32_t>(), s->arena(), (uint32_t)depth, m, n,
                                P.pd_flags.as<uint8_t>(), P.pd_hash.as<uint64_t>(), P.rounds.as<uint8_t>(),
                                P.dist.as<uint32_t>(), ctr);
         }
@@ -457,7 +457,7 @@ int dss_hash_prefixes(dss_set s, const uint32_t* ids, uint64_t count, uint32_t d
         uint32_t* d_ids = reinterpret_cast<uint32_t*>(buf.as<uint8_t>() + count * 8);
         DSS_HIP(hipMemcpyAsync(d_ids, ids, count * 4, hipMemcpyHostToDevice, s->stream));
         hipLaunchKernelGGL(k_hash_ids, dim3((count + 255) / 256), dim3(256), 0, s->stream, d_ids, count,
-                           s->off.as<uint64_t>(), s->len.as<uint32_t>(), s->chars.as<uint8_t>(), s->n, depth, m,
+                           s->off.as<uint64_t>(), s->len.as<uint32_t>(), s->arena(), s->n, depth, m,
                            buf.as<uint64_t>());
         DSS_HIP(hipGetLastError());
         DSS_HIP(hipMemcpyAsync(out, buf.p, count * 8, hipMemcpyDeviceToHost, s->stream));

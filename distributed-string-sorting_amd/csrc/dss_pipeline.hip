@@ -331,10 +331,10 @@ void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, boo
             DSS_HIP(hipStreamSynchronize(st));
             if (ns)
                 hipLaunchKernelGGL(k_sample_rank, dim3(ns), dim3(256), 0, st, compact, ns, indexed,
-                                   s->chars.as<uint8_t>(), P.sample_sorted.as<uint32_t>());
+                                   s->arena(), P.sample_sorted.as<uint32_t>());
             hipLaunchKernelGGL(k_bucket_search, dim3(p), dim3(std::min<uint32_t>(std::max<uint32_t>(p, 64), 256)),
                                (size_t)p * 8, st, s->pe_begin_d.as<uint64_t>(), p, s->perm.as<uint32_t>(),
-                               s->off.as<uint64_t>(), s->len.as<uint32_t>(), s->chars.as<uint8_t>(), compact,
+                               s->off.as<uint64_t>(), s->len.as<uint32_t>(), s->arena(), compact,
                                P.sample_sorted.as<uint32_t>(), ns_d, indexed, P.interval_d.as<uint64_t>());
         } else {
             // one PE (or no strings): everything stays on its PE
@@ -416,17 +416,18 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
                             const uint32_t* __restrict__ r_vlen, const uint32_t* __restrict__ r_from,
                             const uint64_t* __restrict__ byte_off, int mode, uint64_t n, uint32_t lg,
                             uint8_t* __restrict__ out) {
-    const uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lg;
     const uint32_t lanes = 1u << lg;
     const uint32_t l = threadIdx.x & (lanes - 1);
-    if (k >= n) return;
-    const uint8_t* src = chars + r_voff[k];
-    const uint32_t from = r_from[k];
-    const uint32_t vlen = r_vlen[k];
-    uint8_t* dst = out + byte_off[k];
-    // modes 0/1: chars [from, vlen] incl. the NUL; mode 2: [from, vlen) + NUL
-    const uint32_t m = vlen - from;
-    for (uint32_t t = l; t <= m; t += lanes) dst[t] = (t == m) ? 0 : src[from + t];
+    for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lg; k < n;
+         k += ((uint64_t)gridDim.x * blockDim.x) >> lg) {
+        const uint8_t* src = chars + r_voff[k];
+        const uint32_t from = r_from[k];
+        const uint32_t vlen = r_vlen[k];
+        uint8_t* dst = out + byte_off[k];
+        // modes 0/1: chars [from, vlen] incl. the NUL; mode 2: [from, vlen) + NUL
+        const uint32_t m = vlen - from;
+        for (uint32_t t = l; t <= m; t += lanes) dst[t] = (t == m) ? 0 : src[from + t];
+    }
 }
 
 __global__ void k_byte_begin(const uint64_t* __restrict__ recv_begin, const uint64_t* __restrict__ byte_off,
@@ -499,7 +500,7 @@ void exchange(dss_set_s* s, int mode) {
         // short prefix-doubling prefixes: 4 lanes per string, whole strings: 16
         const uint32_t lg = (mode == 2 || total < 24 * n) ? 3 : 4;
         const uint64_t threads = n << lg;
-        hipLaunchKernelGGL(k_exch_copy, dim3((threads + 255) / 256), dim3(256), 0, st, s->chars.as<uint8_t>(),
+        hipLaunchKernelGGL(k_exch_copy, dim3(stride_grid(threads, 256)), dim3(256), 0, st, s->arena(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                            P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), mode, n, lg,
                            P.recv_bytes.as<uint8_t>());
@@ -785,9 +786,9 @@ void merge(dss_set_s* s) {
         DSS_HIP(hipMemsetAsync(cp, 0xFF, (p + 1) * 4, st));
         hipLaunchKernelGGL(k_recv_cp, dim3(((uint64_t)p * p + 255) / 256), dim3(256), 0, st,
                            P.layout.as<uint64_t>() + (uint64_t)p * p, P.interval_d.as<uint64_t>(), p, rb_d,
-                           s->chars.as<uint8_t>(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(), cp);
+                           s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(), cp);
         KScope ks(s->timer, st, "merge_keycache", n * 24.0);
-        hipLaunchKernelGGL(k_keycache, dim3((n + 255) / 256), dim3(256), 0, st, rb_d, p, s->chars.as<uint8_t>(),
+        hipLaunchKernelGGL(k_keycache, dim3((n + 255) / 256), dim3(256), 0, st, rb_d, p, s->arena(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(), cp, n, P.m_keyc.as<uint64_t>());
     }
     // runs per receiver (host), then pairwise rounds; receivers that already
@@ -848,10 +849,10 @@ void merge(dss_set_s* s) {
             uint64_t* split = splits + tbeg[r];
             KScope ks(s->timer, st, "merge_pairs", n * 24.0);
             hipLaunchKernelGGL(k_merge_split, dim3((tiles + 255) / 256), dim3(256), 0, st, rp, np, tiles, ord,
-                               s->chars.as<uint8_t>(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                               s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                                P.m_keyc.as<uint64_t>(), cp, split);
             hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, rp, np, split, ord, lc, ord2, lc2,
-                               s->chars.as<uint8_t>(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                               s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                                P.m_keyc.as<uint64_t>(), cp);
             std::swap(ord, ord2);
             std::swap(lc, lc2);
@@ -953,24 +954,25 @@ __global__ void k_extend(const uint32_t* __restrict__ ord, const uint32_t* __res
                          const uint64_t* __restrict__ psv, const uint64_t* __restrict__ rbyte,
                          const uint64_t* __restrict__ rsize, const uint8_t* __restrict__ rbytes,
                          const uint64_t* __restrict__ out_off, uint64_t n, uint8_t* __restrict__ out) {
-    const uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
     const uint32_t l = threadIdx.x & 15;
-    if (i >= n) return;
-    uint8_t* dst = out + out_off[i];
-    uint32_t cur = saved[i];
-    // own stored characters (incl. NUL) after the prefix
-    {
-        const uint8_t* src = rbytes + rbyte[ord[i]];
-        const uint64_t m = rsize[ord[i]];
-        for (uint64_t t = l; t < m; t += 16) dst[cur + t] = src[t];
-    }
-    uint64_t k = psv[i];
-    while (cur > 0 && k != ~0ull) {
-        const uint32_t sk = saved[k];
-        const uint8_t* src = rbytes + rbyte[ord[k]];
-        for (uint32_t t = sk + l; t < cur; t += 16) dst[t] = src[t - sk];
-        cur = sk;
-        k = psv[k];
+    for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < n;
+         i += ((uint64_t)gridDim.x * blockDim.x) >> 4) {
+        uint8_t* dst = out + out_off[i];
+        uint32_t cur = saved[i];
+        // own stored characters (incl. NUL) after the prefix
+        {
+            const uint8_t* src = rbytes + rbyte[ord[i]];
+            const uint64_t m = rsize[ord[i]];
+            for (uint64_t t = l; t < m; t += 16) dst[cur + t] = src[t];
+        }
+        uint64_t k = psv[i];
+        while (cur > 0 && k != ~0ull) {
+            const uint32_t sk = saved[k];
+            const uint8_t* src = rbytes + rbyte[ord[k]];
+            for (uint32_t t = sk + l; t < cur; t += 16) dst[t] = src[t - sk];
+            cur = sk;
+            k = psv[k];
+        }
     }
 }
 
@@ -1003,7 +1005,7 @@ void extend_prefix(dss_set_s* s) {
     P.out_bytes.reserve(total + kArenaPad);
     if (n) {
         KScope ks(s->timer, st, "extend_copy", (double)total * 2.0 + n * 24.0);
-        hipLaunchKernelGGL(k_extend, dim3((n * 16 + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
+        hipLaunchKernelGGL(k_extend, dim3(stride_grid(n * 16, 256)), dim3(256), 0, st, P.m_order.as<uint32_t>(),
                            P.m_saved.as<uint32_t>(), P.out_psv.as<uint64_t>(), P.recv_byte_off.as<uint64_t>(),
                            P.recv_size.as<uint64_t>(), P.recv_bytes.as<uint8_t>(), P.out_off.as<uint64_t>(), n,
                            P.out_bytes.as<uint8_t>());
@@ -1193,7 +1195,7 @@ extern "C" int dss_check(dss_set s, dss_check_t* out) {
         hipLaunchKernelGGL(k_check, dim3((n + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
                            P.m_lcp.as<uint32_t>(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                            P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>(), s->pe_begin_d.as<uint64_t>(),
-                           rb, s->p, s->chars.as<uint8_t>(), n, glcp, seen, cnt);
+                           rb, s->p, s->arena(), n, glcp, seen, cnt);
         hipLaunchKernelGGL(k_dsum, dim3((n + 255) / 256), dim3(256), 0, st, glcp, n, cnt + 3);
         hipLaunchKernelGGL(k_popc, dim3((nw + 255) / 256), dim3(256), 0, st, seen, nw, cnt + 4);
         DSS_HIP(hipGetLastError());

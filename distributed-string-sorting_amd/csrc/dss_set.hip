@@ -5,7 +5,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "dss_device.hpp"
 #include "dss_internal.hpp"
@@ -262,35 +265,37 @@ void compute_pe_split(dss_set_s* s) {
 // then slen-k copies of the fill character.  One thread per 8-byte word.
 __global__ void k_gen_dton(uint8_t* __restrict__ out, uint64_t n, uint32_t k, uint32_t slen,
                            uint8_t fill, uint64_t seed, uint64_t words) {
-    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= words) return;
+    // base-26 digit weights; string ids are < 2^32 < 26^7, so digits at
+    // weight 26^7 and above are 'A'
+    const uint32_t pow26[7] = {1u, 26u, 676u, 17576u, 456976u, 11881376u, 308915776u};
     const uint64_t stride = slen + 1;
     const uint64_t total = n * stride;
-    uint64_t word = 0;
-    uint64_t cur_j = ~0ull, cur_i = 0;
-    for (uint32_t b = 0; b < 8; ++b) {
-        const uint64_t a = w * 8 + b;
-        uint8_t c = 0;
-        if (a < total) {
-            const uint64_t j = a / stride;
-            const uint32_t pos = (uint32_t)(a - j * stride);
-            if (j != cur_j) {
-                cur_j = j;
-                cur_i = shuffle_index(j, n, seed);
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t a0 = w * 8;
+        uint64_t j = a0 / stride;
+        uint32_t pos = (uint32_t)(a0 - j * stride);
+        uint32_t id = j < n ? (uint32_t)shuffle_index(j, n, seed) : 0u;
+        uint64_t word = 0;
+        for (uint32_t b = 0; b < 8; ++b) {
+            uint8_t c = 0;
+            if (a0 + b < total) {
+                if (pos == slen) c = 0;
+                else if (pos >= k) c = fill;
+                else {
+                    const uint32_t e = k - 1 - pos;  // digit index from the right
+                    c = (uint8_t)('A' + (e < 7 ? (id / pow26[e]) % 26u : 0u));
+                }
             }
-            if (pos == slen) c = 0;
-            else if (pos >= k) c = fill;
-            else {
-                const uint32_t e = k - 1 - pos;  // digit index from the right
-                uint64_t v = cur_i;
-                if (e >= 14) v = 0;  // 26^14 > 2^64
-                else for (uint32_t t = 0; t < e && v; ++t) v /= 26;
-                c = (uint8_t)('A' + v % 26);
+            word |= (uint64_t)c << (8 * b);
+            if (++pos == stride) {
+                pos = 0;
+                ++j;
+                id = j < n ? (uint32_t)shuffle_index(j, n, seed) : 0u;
             }
         }
-        word |= (uint64_t)c << (8 * b);
+        reinterpret_cast<uint64_t*>(out)[w] = word;
     }
-    reinterpret_cast<uint64_t*>(out)[w] = word;
 }
 
 __global__ void k_fixed_offsets(uint64_t* __restrict__ off, uint32_t* __restrict__ len, uint64_t n,
@@ -335,27 +340,28 @@ __global__ void k_skew_len(SkewP P, uint64_t seed, uint64_t* __restrict__ size_o
 // One wave per string.
 __global__ void k_skew_fill(SkewP P, uint64_t seed, const uint64_t* __restrict__ off,
                             uint8_t* __restrict__ out) {
-    const uint64_t j = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    if (j >= P.n) return;
     const uint32_t lane = threadIdx.x & 63;
-    const uint64_t i = shuffle_index(j, P.n, seed);
-    uint64_t id;
-    uint32_t L, k;
-    skew_id_len(P, seed, i, id, L, k);
-    const uint64_t h = rng(seed, 5, id);
-    uint8_t* s = out + off[j];
-    for (uint32_t t = lane; t <= L; t += 64) {
-        uint8_t c;
-        if (t == L) c = 0;
-        else if (t >= k) c = (uint8_t)(1 + mix64(h + t) % 255);
-        else {
-            const uint32_t e = k - 1 - t;
-            uint64_t v = id;
-            if (e >= 9) v = 0;  // 255^8 > 2^64
-            else for (uint32_t q = 0; q < e && v; ++q) v /= 255;
-            c = (uint8_t)(1 + v % 255);
+    for (uint64_t j = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; j < P.n;
+         j += (uint64_t)gridDim.x * (blockDim.x / 64)) {
+        const uint64_t i = shuffle_index(j, P.n, seed);
+        uint64_t id;
+        uint32_t L, k;
+        skew_id_len(P, seed, i, id, L, k);
+        const uint64_t h = rng(seed, 5, id);
+        uint8_t* s = out + off[j];
+        for (uint32_t t = lane; t <= L; t += 64) {
+            uint8_t c;
+            if (t == L) c = 0;
+            else if (t >= k) c = (uint8_t)(1 + mix64(h + t) % 255);
+            else {
+                const uint32_t e = k - 1 - t;
+                uint64_t v = id;
+                if (e >= 9) v = 0;  // 255^8 > 2^64
+                else for (uint32_t q = 0; q < e && v; ++q) v /= 255;
+                c = (uint8_t)(1 + v % 255);
+            }
+            s[t] = c;
         }
-        s[t] = c;
     }
 }
 
@@ -419,7 +425,7 @@ int dss_set_create_dton(uint64_t n, uint32_t len, double dton, uint64_t seed, ui
             s->chars.reserve(s->nbytes + kArenaPad);
             DSS_HIP(hipMemsetAsync(s->chars.as<uint8_t>() + s->nbytes, 0, kArenaPad, s->stream));
             const uint64_t words = (s->nbytes + 7) / 8;
-            hipLaunchKernelGGL(k_gen_dton, dim3((words + 255) / 256), dim3(256), 0, s->stream,
+            hipLaunchKernelGGL(k_gen_dton, dim3(stride_grid(words, 256)), dim3(256), 0, s->stream,
                                s->chars.as<uint8_t>(), n, k, slen, fill, seed, words);
             s->off.reserve(n * sizeof(uint64_t));
             s->len.reserve(n * sizeof(uint32_t));
@@ -463,7 +469,7 @@ int dss_set_create_skewed(uint64_t n, uint32_t long_len, uint32_t short_len, dou
             DSS_HIP(hipStreamSynchronize(s->stream));
             s->chars.reserve(s->nbytes + kArenaPad);
             DSS_HIP(hipMemsetAsync(s->chars.as<uint8_t>() + s->nbytes, 0, kArenaPad, s->stream));
-            hipLaunchKernelGGL(k_skew_fill, dim3((n + 3) / 4), dim3(256), 0, s->stream, P, seed,
+            hipLaunchKernelGGL(k_skew_fill, dim3(stride_grid(n, 4)), dim3(256), 0, s->stream, P, seed,
                                s->off.as<uint64_t>(), s->chars.as<uint8_t>());
             DSS_HIP(hipGetLastError());
             compute_pe_split(s);
@@ -505,8 +511,9 @@ int dss_set_pe_begin(dss_set s, uint64_t* pe_begin) {
 int dss_set_download(dss_set s, uint8_t* chars, uint64_t* offsets, uint32_t* lengths) {
     return guarded([&] {
         DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        DSS_REQUIRE(!(chars && s->borrowed), DSS_EINVAL, "a batch set has no arena of its own to download");
         if (chars && s->nbytes)
-            DSS_HIP(hipMemcpyAsync(chars, s->chars.p, s->nbytes, hipMemcpyDeviceToHost, s->stream));
+            DSS_HIP(hipMemcpyAsync(chars, s->arena(), s->nbytes, hipMemcpyDeviceToHost, s->stream));
         if (offsets && s->n)
             DSS_HIP(hipMemcpyAsync(offsets, s->off.p, s->n * 8, hipMemcpyDeviceToHost, s->stream));
         if (lengths && s->n)
@@ -617,3 +624,351 @@ extern "C" int dss_set_create_pes(const uint8_t* chars, const uint64_t* pe_byte_
         *out = s;
     });
 }
+
+// ================================================================== job pool
+// Device-resident arenas of many sort jobs for batched and served execution:
+// one HBM region of characters and one of per-string offsets (absolute in the
+// pool arena) and lengths, with first-fit allocation.  A batch is a string
+// set over the pool whose PEs are the batch's jobs; the set borrows the pool
+// arena and owns copies of the jobs' offset/length arrays.
+namespace dss {
+
+struct RangeAlloc {
+    std::map<uint64_t, uint64_t> free_;  // begin -> size
+    uint64_t cap = 0;
+    void init(uint64_t c) {
+        cap = c;
+        free_.clear();
+        if (c) free_[0] = c;
+    }
+    bool alloc(uint64_t size, uint64_t& at) {
+        if (size == 0) {
+            at = 0;
+            return true;
+        }
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= size) {
+                at = it->first;
+                const uint64_t rest = it->second - size;
+                free_.erase(it);
+                if (rest) free_[at + size] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(uint64_t at, uint64_t size) {
+        if (!size) return;
+        auto it = free_.emplace(at, size).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    uint64_t used() const {
+        uint64_t f = 0;
+        for (auto& kv : free_) f += kv.second;
+        return cap - f;
+    }
+};
+
+__global__ void k_offsets_base(uint64_t* __restrict__ off, uint32_t* __restrict__ len, uint64_t n, uint32_t slen,
+                               uint64_t base) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    off[i] = base + i * (uint64_t)(slen + 1);
+    len[i] = slen;
+}
+
+__global__ void k_add_base(uint64_t* __restrict__ off, uint64_t n, uint64_t base) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) off[i] += base;
+}
+
+}  // namespace dss
+
+struct dss_pool_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    dss::DevBuf chars, off, len, tot;
+    dss::RangeAlloc bytes, strings;
+    struct Job {
+        uint64_t byte0 = 0, nbytes = 0, alloc = 0, str0 = 0, n = 0;
+        uint32_t max_len = 0;
+        bool live = false;
+    };
+    std::vector<Job> jobs;
+    std::mutex mu;
+};
+
+using namespace dss;
+
+// Character ranges are allocated in whole 8-byte words so that every job
+// starts 8-byte aligned (the generators write whole words).
+static uint32_t pool_register(dss_pool_s* P, uint64_t byte0, uint64_t nbytes, uint64_t asz, uint64_t str0,
+                              uint64_t n) {
+    dss_pool_s::Job J;
+    J.byte0 = byte0;
+    J.nbytes = nbytes;
+    J.alloc = asz;
+    J.str0 = str0;
+    J.n = n;
+    J.live = true;
+    for (uint32_t k = 0; k < P->jobs.size(); ++k)
+        if (!P->jobs[k].live) {
+            P->jobs[k] = J;
+            return k;
+        }
+    P->jobs.push_back(J);
+    return (uint32_t)P->jobs.size() - 1;
+}
+
+static uint32_t pool_new_job(dss_pool_s* P, uint64_t nbytes, uint64_t n, uint64_t& byte0, uint64_t& str0) {
+    const uint64_t asz = (nbytes + 7) / 8 * 8;
+    if (!P->bytes.alloc(asz, byte0)) throw Error{DSS_ENOMEM, "job pool: no room for the job's characters"};
+    if (!P->strings.alloc(n, str0)) {
+        P->bytes.release(byte0, asz);
+        throw Error{DSS_ENOMEM, "job pool: no room for the job's strings"};
+    }
+    return pool_register(P, byte0, nbytes, asz, str0, n);
+}
+
+// Points s at the given pool jobs (PE k = jobs[k]); keeps s's device buffers.
+static void pool_fill_batch(dss_pool_s* P, dss_set_s* s, const uint32_t* jobs, uint32_t count) {
+    std::lock_guard<std::mutex> g(P->mu);
+    std::vector<uint64_t> pb(count + 1, 0);
+    uint64_t n = 0;
+    uint32_t mx = 0;
+    for (uint32_t k = 0; k < count; ++k) {
+        DSS_REQUIRE(jobs[k] < P->jobs.size() && P->jobs[jobs[k]].live, DSS_EINVAL, "no such job");
+        for (uint32_t q = 0; q < k; ++q) DSS_REQUIRE(jobs[q] != jobs[k], DSS_EINVAL, "job listed twice");
+        n += P->jobs[jobs[k]].n;
+        pb[k + 1] = n;
+        mx = std::max(mx, P->jobs[jobs[k]].max_len);
+    }
+    DSS_REQUIRE(n < 0xFFFFFFFFull, DSS_ERANGE, "batch exceeds 2^32-1 strings");
+    s->borrowed = P->chars.as<uint8_t>();
+    s->nbytes = 0;
+    s->p = count;
+    s->pe_begin = pb;
+    s->n = n;
+    s->max_len = mx;
+    s->sorted = false;
+    s->pp.has_dist = s->pp.partitioned = s->pp.exchanged = s->pp.merged = s->pp.extended = false;
+    s->off.reserve(std::max<uint64_t>(n, 1) * 8);
+    s->len.reserve(std::max<uint64_t>(n, 1) * 4);
+    for (uint32_t k = 0; k < count; ++k) {
+        const auto& J = P->jobs[jobs[k]];
+        if (!J.n) continue;
+        DSS_HIP(hipMemcpyAsync(s->off.as<uint64_t>() + pb[k], P->off.as<uint64_t>() + J.str0, J.n * 8,
+                               hipMemcpyDeviceToDevice, s->stream));
+        DSS_HIP(hipMemcpyAsync(s->len.as<uint32_t>() + pb[k], P->len.as<uint32_t>() + J.str0, J.n * 4,
+                               hipMemcpyDeviceToDevice, s->stream));
+    }
+    s->pe_begin_d.reserve((count + 1) * 8);
+    DSS_HIP(hipMemcpyAsync(s->pe_begin_d.p, pb.data(), (count + 1) * 8, hipMemcpyHostToDevice, s->stream));
+    DSS_HIP(hipStreamSynchronize(s->stream));
+}
+
+extern "C" {
+
+int dss_pool_create(uint64_t char_capacity, uint64_t string_capacity, dss_pool* out) {
+    return guarded([&] {
+        DSS_REQUIRE(out, DSS_EINVAL, "out is NULL");
+        dss_pool_s* P = new dss_pool_s();
+        try {
+            DSS_HIP(hipGetDevice(&P->device));
+            DSS_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+            P->chars.reserve(char_capacity + kArenaPad);
+            DSS_HIP(hipMemsetAsync(P->chars.p, 0, char_capacity + kArenaPad, P->stream));
+            P->off.reserve(std::max<uint64_t>(string_capacity, 1) * 8);
+            P->len.reserve(std::max<uint64_t>(string_capacity, 1) * 4);
+            P->tot.reserve(8);
+            P->bytes.init(char_capacity);
+            P->strings.init(string_capacity);
+            DSS_HIP(hipStreamSynchronize(P->stream));
+        } catch (...) {
+            if (P->stream) (void)hipStreamDestroy(P->stream);
+            delete P;
+            throw;
+        }
+        *out = P;
+    });
+}
+
+int dss_pool_destroy(dss_pool P) {
+    return guarded([&] {
+        if (!P) return;
+        (void)hipStreamSynchronize(P->stream);
+        hipStream_t st = P->stream;
+        delete P;
+        (void)hipStreamDestroy(st);
+    });
+}
+
+int dss_pool_add_job(dss_pool P, const uint8_t* chars, uint64_t nbytes, uint32_t* job) {
+    return guarded([&] {
+        DSS_REQUIRE(P && job && (nbytes == 0 || chars), DSS_EINVAL, "NULL argument");
+        DSS_REQUIRE(nbytes == 0 || chars[nbytes - 1] == 0, DSS_EINVAL, "arena must end with a NUL terminator");
+        std::vector<uint64_t> off;
+        std::vector<uint32_t> len;
+        uint64_t s = 0;
+        uint32_t mx = 0;
+        for (uint64_t i = 0; i < nbytes; ++i)
+            if (chars[i] == 0) {
+                off.push_back(s);
+                len.push_back((uint32_t)(i - s));
+                mx = std::max(mx, (uint32_t)(i - s));
+                s = i + 1;
+            }
+        std::lock_guard<std::mutex> g(P->mu);
+        uint64_t b0, s0;
+        const uint32_t id = pool_new_job(P, nbytes, off.size(), b0, s0);
+        P->jobs[id].max_len = mx;
+        if (nbytes) DSS_HIP(hipMemcpyAsync(P->chars.as<uint8_t>() + b0, chars, nbytes, hipMemcpyHostToDevice, P->stream));
+        if (!off.empty()) {
+            for (auto& o : off) o += b0;
+            DSS_HIP(hipMemcpyAsync(P->off.as<uint64_t>() + s0, off.data(), off.size() * 8, hipMemcpyHostToDevice, P->stream));
+            DSS_HIP(hipMemcpyAsync(P->len.as<uint32_t>() + s0, len.data(), len.size() * 4, hipMemcpyHostToDevice, P->stream));
+        }
+        DSS_HIP(hipStreamSynchronize(P->stream));
+        *job = id;
+    });
+}
+
+int dss_pool_add_job_dton(dss_pool P, uint64_t n, uint32_t length, double dton, uint64_t seed, uint32_t* job) {
+    return guarded([&] {
+        DSS_REQUIRE(P && job, DSS_EINVAL, "NULL argument");
+        DSS_REQUIRE(n >= 1 && n < 0xFFFFFFFFull, DSS_ERANGE, "n must be in [1, 2^32-1)");
+        const double a = (double)length * dton;
+        const double b = std::ceil(std::log((double)n) / std::log(26.0));
+        const uint32_t k = (uint32_t)(a > b ? a : b);
+        const uint32_t slen = length > k ? length : k;
+        const uint8_t fill = (uint8_t)('A' + rng(seed, 1, 0) % 26);
+        const uint64_t nbytes = n * (uint64_t)(slen + 1);
+        std::lock_guard<std::mutex> g(P->mu);
+        uint64_t b0, s0;
+        const uint32_t id = pool_new_job(P, nbytes, n, b0, s0);
+        P->jobs[id].max_len = slen;
+        const uint64_t words = (nbytes + 7) / 8;
+        hipLaunchKernelGGL(k_gen_dton, dim3(stride_grid(words, 256)), dim3(256), 0, P->stream, P->chars.as<uint8_t>() + b0, n, k,
+                           slen, fill, seed, words);
+        hipLaunchKernelGGL(k_offsets_base, dim3((n + 255) / 256), dim3(256), 0, P->stream, P->off.as<uint64_t>() + s0,
+                           P->len.as<uint32_t>() + s0, n, slen, b0);
+        DSS_HIP(hipGetLastError());
+        DSS_HIP(hipStreamSynchronize(P->stream));
+        *job = id;
+    });
+}
+
+int dss_pool_add_job_skewed(dss_pool P, uint64_t n, uint32_t long_len, uint32_t short_len, double long_dn,
+                            double short_dn, uint32_t long_permille, uint32_t dup_permille, uint64_t seed,
+                            uint32_t* job) {
+    return guarded([&] {
+        DSS_REQUIRE(P && job, DSS_EINVAL, "NULL argument");
+        DSS_REQUIRE(n >= 1 && n < 0xFFFFFFFFull, DSS_ERANGE, "n must be in [1, 2^32-1)");
+        auto kfun = [&](uint32_t L, double dn) {
+            double a = (double)L * dn;
+            double b = std::ceil(std::log((double)(n > 1 ? n : 2)) / std::log(255.0));
+            return (uint32_t)(a > b ? a : b);
+        };
+        SkewP S{n, long_len, short_len, kfun(long_len, long_dn), kfun(short_len, short_dn), long_permille,
+                dup_permille};
+        std::lock_guard<std::mutex> g(P->mu);
+        // string slots first: sizes are generated into the job's offset array,
+        // scanned in place, and the character range is allocated afterwards
+        uint64_t s0;
+        if (!P->strings.alloc(n, s0)) throw Error{DSS_ENOMEM, "job pool: no room for the job's strings"};
+        uint64_t* off = P->off.as<uint64_t>() + s0;
+        hipLaunchKernelGGL(k_skew_len, dim3((n + 255) / 256), dim3(256), 0, P->stream, S, seed, off,
+                           P->len.as<uint32_t>() + s0);
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, P->stream, off, n, P->tot.as<uint64_t>());
+        uint64_t nbytes = 0;
+        DSS_HIP(hipMemcpyAsync(&nbytes, P->tot.p, 8, hipMemcpyDeviceToHost, P->stream));
+        DSS_HIP(hipStreamSynchronize(P->stream));
+        const uint64_t asz = (nbytes + 7) / 8 * 8;
+        uint64_t b0;
+        if (!P->bytes.alloc(asz, b0)) {
+            P->strings.release(s0, n);
+            throw Error{DSS_ENOMEM, "job pool: no room for the job's characters"};
+        }
+        const uint32_t id = pool_register(P, b0, nbytes, asz, s0, n);
+        P->jobs[id].max_len = std::max(std::max(long_len, S.k_long), std::max(short_len, S.k_short));
+        hipLaunchKernelGGL(k_add_base, dim3((n + 255) / 256), dim3(256), 0, P->stream, off, n, b0);
+        hipLaunchKernelGGL(k_skew_fill, dim3(stride_grid(n, 4)), dim3(256), 0, P->stream, S, seed, off,
+                           P->chars.as<uint8_t>());
+        DSS_HIP(hipGetLastError());
+        DSS_HIP(hipStreamSynchronize(P->stream));
+        *job = id;
+    });
+}
+
+int dss_pool_free_job(dss_pool P, uint32_t job) {
+    return guarded([&] {
+        DSS_REQUIRE(P, DSS_EINVAL, "NULL pool");
+        std::lock_guard<std::mutex> g(P->mu);
+        DSS_REQUIRE(job < P->jobs.size() && P->jobs[job].live, DSS_EINVAL, "no such job");
+        auto& J = P->jobs[job];
+        P->bytes.release(J.byte0, J.alloc);
+        P->strings.release(J.str0, J.n);
+        J.live = false;
+    });
+}
+
+int dss_pool_usage(dss_pool P, uint64_t* bytes_used, uint64_t* strings_used, uint64_t* bytes_cap,
+                   uint64_t* strings_cap) {
+    return guarded([&] {
+        DSS_REQUIRE(P, DSS_EINVAL, "NULL pool");
+        std::lock_guard<std::mutex> g(P->mu);
+        if (bytes_used) *bytes_used = P->bytes.used();
+        if (strings_used) *strings_used = P->strings.used();
+        if (bytes_cap) *bytes_cap = P->bytes.cap;
+        if (strings_cap) *strings_cap = P->strings.cap;
+    });
+}
+
+int dss_pool_job_info(dss_pool P, uint32_t job, uint64_t* num_strings, uint64_t* num_bytes) {
+    return guarded([&] {
+        DSS_REQUIRE(P, DSS_EINVAL, "NULL pool");
+        std::lock_guard<std::mutex> g(P->mu);
+        DSS_REQUIRE(job < P->jobs.size() && P->jobs[job].live, DSS_EINVAL, "no such job");
+        if (num_strings) *num_strings = P->jobs[job].n;
+        if (num_bytes) *num_bytes = P->jobs[job].nbytes;
+    });
+}
+
+int dss_pool_batch(dss_pool P, const uint32_t* jobs, uint32_t count, dss_set* out) {
+    return guarded([&] {
+        DSS_REQUIRE(P && out && jobs && count >= 1, DSS_EINVAL, "NULL argument or empty batch");
+        dss_set_s* s = new_set(count);
+        try {
+            pool_fill_batch(P, s, jobs, count);
+        } catch (...) {
+            hipStream_t st = s->stream;
+            delete s;
+            (void)hipStreamDestroy(st);
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int dss_pool_rebatch(dss_pool P, dss_set s, const uint32_t* jobs, uint32_t count) {
+    return guarded([&] {
+        DSS_REQUIRE(P && s && jobs && count >= 1, DSS_EINVAL, "NULL argument or empty batch");
+        DSS_REQUIRE(s->borrowed == P->chars.as<uint8_t>(), DSS_EINVAL, "set is not a batch of this pool");
+        DSS_REQUIRE(count <= (1u << 16), DSS_EINVAL, "at most 65536 jobs per batch");
+        DSS_HIP(hipStreamSynchronize(s->stream));
+        pool_fill_batch(P, s, jobs, count);
+    });
+}
+
+}  // extern "C"

@@ -80,6 +80,31 @@ __device__ inline void push_small(SmallSeg* small, const SmallLists& L, uint32_t
 }
 
 // ------------------------------------------------------------------ init
+// D of each PE's sorted strings (computeD, src/tests/DNGenerator_tests.cpp:
+// 22-45) over chunks of at most kDistChunk positions: chunk k covers
+// [begin, begin + cnt) of PE pe; the PE's first LCP counts as 0 and the
+// position after its last string contributes no LCP.
+constexpr uint64_t kDistChunk = 1 << 16;
+
+__global__ void k_pe_dsum(const uint64_t* __restrict__ chunks, const uint32_t* __restrict__ lcp,
+                          const uint64_t* __restrict__ pe_begin, unsigned long long* __restrict__ d) {
+    const uint64_t a = chunks[2 * blockIdx.x], b = chunks[2 * blockIdx.x + 1];
+    const uint32_t pe = (uint32_t)(a >> 40);
+    const uint64_t c0 = a & ((1ull << 40) - 1), cnt = b;
+    const uint64_t pb = pe_begin[pe], pe_end = pe_begin[pe + 1];
+    uint64_t t = 0;
+    for (uint64_t i = c0 + threadIdx.x; i < c0 + cnt; i += blockDim.x) {
+        const uint32_t x = i == pb ? 0u : lcp[i];
+        const uint32_t y = i + 1 < pe_end ? lcp[i + 1] : 0u;
+        t += (uint64_t)max(x, y) + 1;
+    }
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_down((unsigned long long)t, o);
+    __shared__ unsigned long long part[4];
+    if (lane_id() == 0) part[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(d + pe, part[0] + part[1] + part[2] + part[3]);
+}
+
 __global__ void k_sort_init(uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -316,7 +341,10 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, Seg* 
     if (list < (int)kClasses) {
         small[L.off[list] + lbase[list] + slot] = SmallSeg{b, tot, dep + 1};
     } else {
-        const uint32_t nkd = (dep + 1 - kd >= 8) ? kNone : kd;
+        // keys run out after 8 characters; a bucket that holds the whole
+        // segment (one digit shared by all) also re-runs the prefix skip,
+        // which jumps over the rest of the shared prefix at once
+        const uint32_t nkd = (dep + 1 - kd >= 8 || tot == S.cnt) ? kNone : kd;
         next[lbase[kClasses] + slot] = Seg{b, tot, dep + 1, nkd, kNone, 0};
     }
 }
@@ -475,7 +503,11 @@ k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2
             if (id != kNone && ACT[G[e]]) K[e] = load_key8(chars, off[id], len[id], RD[G[e]]);
         }
         __syncthreads();
-        // 3. bitonic sort of (G, K) inside every aligned CLS slot
+        // 3. bitonic sort of (G, K, I) inside every aligned CLS slot.  The
+        // string id makes the order total: a sorting network that is not
+        // stable then still returns equal strings in ascending id order, the
+        // order the stable radix passes keep, however many further rounds a
+        // block runs for other slots' groups.
         for (uint32_t k = 2; k <= CLS; k <<= 1) {
             for (uint32_t j = k >> 1; j > 0; j >>= 1) {
                 for (uint32_t t = tid; t < kCap / 2; t += kThreads) {
@@ -484,16 +516,16 @@ k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2
                     const bool asc = (k == CLS) || ((i & k) == 0);
                     const uint32_t gi = G[i], gq = G[q];
                     const uint64_t ki = K[i], kq = K[q];
-                    const bool gt = gi > gq || (gi == gq && ki > kq);
-                    const bool lt = gi < gq || (gi == gq && ki < kq);
+                    const uint32_t ii = I[i], iq = I[q];
+                    const bool gt = gi > gq || (gi == gq && (ki > kq || (ki == kq && ii > iq)));
+                    const bool lt = gi < gq || (gi == gq && (ki < kq || (ki == kq && ii < iq)));
                     if (asc ? gt : lt) {
                         G[i] = gq;
                         G[q] = gi;
                         K[i] = kq;
                         K[q] = ki;
-                        const uint32_t x = I[i];
-                        I[i] = I[q];
-                        I[q] = x;
+                        I[i] = iq;
+                        I[q] = ii;
                     }
                 }
                 __syncthreads();
@@ -636,7 +668,7 @@ void sort_lcp(dss_set_s* s) {
     DSS_HIP(hipStreamSynchronize(st));
     uint32_t nlarge = h_ctr[0];
     const bool trace = getenv("DSS_SORT_TRACE") != nullptr;
-    const uint8_t* chars = s->chars.as<uint8_t>();
+    const uint8_t* chars = s->arena();
     const uint64_t* off = s->off.as<uint64_t>();
     const uint32_t* len = s->len.as<uint32_t>();
     uint32_t* perm = s->perm.as<uint32_t>();
@@ -746,6 +778,52 @@ int dss_sort_lcp(dss_set s) {
     return guarded([&] {
         DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
         sort_lcp(s);
+    });
+}
+
+int dss_pe_dist_sizes(dss_set s, uint64_t* d_per_pe) {
+    return guarded([&] {
+        DSS_REQUIRE(s && d_per_pe, DSS_EINVAL, "NULL argument");
+        DSS_REQUIRE(s->sorted, DSS_ESTATE, "dss_sort_lcp has not run");
+        const uint32_t p = s->p;
+        std::vector<uint64_t> chunks;  // pairs (pe << 40 | begin, count)
+        for (uint32_t r = 0; r < p; ++r) {
+            const uint64_t b = s->pe_begin[r], e = s->pe_begin[r + 1];
+            if (e - b < 2) continue;
+            for (uint64_t c = b; c < e; c += kDistChunk) {
+                chunks.push_back(((uint64_t)r << 40) | c);
+                chunks.push_back(std::min(e, c + kDistChunk) - c);
+            }
+        }
+        DevBuf& buf = s->aux;
+        buf.reserve(p * 8 + chunks.size() * 8 + 8);
+        unsigned long long* d = buf.as<unsigned long long>();
+        DSS_HIP(hipMemsetAsync(d, 0, p * 8, s->stream));
+        if (!chunks.empty()) {
+            uint64_t* dc = buf.as<uint64_t>() + p;
+            DSS_HIP(hipMemcpyAsync(dc, chunks.data(), chunks.size() * 8, hipMemcpyHostToDevice, s->stream));
+            hipLaunchKernelGGL(k_pe_dsum, dim3(chunks.size() / 2), dim3(256), 0, s->stream, dc,
+                               s->lcp.as<uint32_t>(), s->pe_begin_d.as<uint64_t>(), d);
+            DSS_HIP(hipGetLastError());
+        }
+        DSS_HIP(hipMemcpyAsync(d_per_pe, d, p * 8, hipMemcpyDeviceToHost, s->stream));
+        DSS_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int dss_get_sorted_pe(dss_set s, uint32_t pe, uint32_t* perm, uint32_t* lcp) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        DSS_REQUIRE(s->sorted, DSS_ESTATE, "dss_sort_lcp has not run");
+        DSS_REQUIRE(pe < s->p, DSS_EINVAL, "PE out of range");
+        const uint64_t b = s->pe_begin[pe], cnt = s->pe_begin[pe + 1] - b;
+        if (perm && cnt)
+            DSS_HIP(hipMemcpyAsync(perm, s->perm.as<uint32_t>() + b, cnt * 4, hipMemcpyDeviceToHost, s->stream));
+        if (lcp && cnt)
+            DSS_HIP(hipMemcpyAsync(lcp, s->lcp.as<uint32_t>() + b, cnt * 4, hipMemcpyDeviceToHost, s->stream));
+        DSS_HIP(hipStreamSynchronize(s->stream));
+        if (perm)
+            for (uint64_t i = 0; i < cnt; ++i) perm[i] -= (uint32_t)b;
     });
 }
 

@@ -14,7 +14,7 @@ import numpy as np
 from . import _lib
 from ._lib import DssError, check, lib
 
-__all__ = ["DeviceStringSet", "DssError", "device_info"]
+__all__ = ["DeviceStringSet", "DssError", "JobPool", "device_info"]
 
 
 def _p(a, t):
@@ -265,3 +265,6 @@ class DeviceStringSet:
             s = arr[k]
             out[s.name.decode()] = dict(launches=s.launches, ms=s.total_ms, alg_bytes=s.alg_bytes)
         return out
+
+
+from .pool import Batch, JobPool  # noqa: E402

@@ -15,6 +15,7 @@ u8p = C.POINTER(C.c_uint8)
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
 set_p = C.c_void_p
+pool_p = C.c_void_p
 
 
 class SetInfo(C.Structure):
@@ -79,6 +80,19 @@ _SIGS = {
     "dss_sync": (C.c_int, [set_p]),
     "dss_set_timing": (C.c_int, [set_p, C.c_int]),
     "dss_kernel_stats": (C.c_int, [set_p, C.POINTER(KernelStat), C.c_uint32, u32p, C.c_int]),
+    "dss_pool_create": (C.c_int, [C.c_uint64, C.c_uint64, C.POINTER(pool_p)]),
+    "dss_pool_destroy": (C.c_int, [pool_p]),
+    "dss_pool_add_job": (C.c_int, [pool_p, u8p, C.c_uint64, u32p]),
+    "dss_pool_add_job_dton": (C.c_int, [pool_p, C.c_uint64, C.c_uint32, C.c_double, C.c_uint64, u32p]),
+    "dss_pool_add_job_skewed": (C.c_int, [pool_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
+                                          C.c_double, C.c_uint32, C.c_uint32, C.c_uint64, u32p]),
+    "dss_pool_free_job": (C.c_int, [pool_p, C.c_uint32]),
+    "dss_pool_job_info": (C.c_int, [pool_p, C.c_uint32, u64p, u64p]),
+    "dss_pool_usage": (C.c_int, [pool_p, u64p, u64p, u64p, u64p]),
+    "dss_pool_batch": (C.c_int, [pool_p, u32p, C.c_uint32, C.POINTER(set_p)]),
+    "dss_pool_rebatch": (C.c_int, [pool_p, set_p, u32p, C.c_uint32]),
+    "dss_pe_dist_sizes": (C.c_int, [set_p, u64p]),
+    "dss_get_sorted_pe": (C.c_int, [set_p, C.c_uint32, u32p, u32p]),
 }
 
 

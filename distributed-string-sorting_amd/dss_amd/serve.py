@@ -1,0 +1,268 @@
+"""Continuous-batching sort server on one GPU.
+
+Sort jobs arrive over time.  The server admits a job into a device JobPool as
+soon as its HBM footprint fits (its characters plus one u64 offset and one u32
+length per string: the pool's capacities are the admission limit), keeps it
+resident until its batch completes, and runs up to `inflight` batches at once,
+each on its own HIP stream and host thread.  A worker takes every admitted job
+that is waiting (first come first served, up to `max_batch_strings` strings,
+at least one job), sorts all of them in one sequence of kernel launches (job k
+of the batch is virtual PE k), computes each job's D and releases the job's
+memory, which lets the admission thread bring in the next arrivals while the
+other batch is still on the device.
+
+The job stream follows the mixed workload of the batched/served benchmark:
+sizes log-uniform, lengths uniform, D/N uniform in [0.1, 0.9], every job drawn
+from the D/N-ratio generator (src/util/random_string_generator.hpp:166-287) or
+its skewed variant (long/short strings plus exact duplicates).
+"""
+from __future__ import annotations
+
+import math
+import threading
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import DssError
+from .pool import JobPool
+
+DSS_ENOMEM = -2
+
+
+@dataclass
+class JobSpec:
+    jid: int
+    kind: str            # "dton" or "skewed"
+    n: int               # strings
+    length: int          # string length (long strings for "skewed")
+    dton: float          # D/N of the (long) strings
+    seed: int
+    arrival: float = 0.0  # seconds after the stream starts
+    short_len: int = 0   # "skewed": length and D/N of the short strings
+    short_dn: float = 0.0
+
+    def add_to(self, pool: JobPool) -> int:
+        if self.kind == "dton":
+            return pool.add_dton(self.n, self.length, self.dton, self.seed)
+        return pool.add_skewed(self.n, self.seed, long_len=self.length, short_len=self.short_len,
+                               long_dn=self.dton, short_dn=self.short_dn)
+
+    def max_bytes(self) -> int:
+        """Upper bound of the job's characters (NULs included)."""
+        if self.kind == "dton":
+            k = max(int(self.length * self.dton), math.ceil(math.log(max(self.n, 2)) / math.log(26)))
+            return self.n * (max(self.length, k) + 1)
+        k = math.ceil(math.log(max(self.n, 2)) / math.log(255))
+        return self.n * (max(self.length, int(self.length * self.dton), k) + 1)
+
+
+def job_stream(num_jobs: int, n_min: int, n_max: int, len_min: int = 10, len_max: int = 1000,
+               rate: float = 0.0, seed: int = 0x5EED0005) -> list[JobSpec]:
+    """Seeded mixed job stream; arrivals form a Poisson process of `rate`
+    jobs/s (rate 0: every job arrives at time 0)."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.random.default_rng([seed, 1])  # jobs do not depend on the rate
+    specs = []
+    t = 0.0
+    for j in range(num_jobs):
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        n = min(max(n, n_min), n_max)
+        length = int(rng.integers(len_min, len_max + 1))
+        d = float(rng.uniform(0.1, 0.9))
+        kind = ("dton", "dton", "skewed")[int(rng.integers(0, 3))]
+        if rate > 0:
+            t += float(arrivals.exponential(1.0 / rate))
+        s = JobSpec(jid=j, kind=kind, n=n, length=length, dton=d, seed=seed * 7919 + j, arrival=t)
+        if kind == "skewed":
+            s.short_len = max(len_min, length // 10)
+            s.short_dn = d / 3.0
+        specs.append(s)
+    return specs
+
+
+@dataclass
+class JobResult:
+    jid: int
+    n: int
+    D: int = 0
+    arrival: float = 0.0    # seconds since the run started
+    admitted: float = 0.0
+    started: float = 0.0
+    done: float = 0.0
+    batch_jobs: int = 0     # jobs in the batch that sorted this one
+    perm: np.ndarray | None = field(default=None, repr=False)
+    lcp: np.ndarray | None = field(default=None, repr=False)
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+class Server:
+    def __init__(self, pool: JobPool, inflight: int = 2, max_batch_strings: int = 32 << 20):
+        self.pool = pool
+        self.inflight = inflight
+        self.max_batch_strings = max_batch_strings
+
+    def run(self, specs, realtime: bool = True, keep_results: bool = False,
+            resident: dict | None = None) -> list[JobResult]:
+        """Serves the stream; with realtime=False every job arrives at once.
+
+        With `resident` (jid -> pool job id) the jobs are already in the pool:
+        they are ready at once, nothing is admitted and nothing is freed (the
+        batched form of jobs that are resident in HBM)."""
+        pool = self.pool
+        order = sorted(specs, key=lambda s: s.arrival)
+        res = {s.jid: JobResult(jid=s.jid, n=s.n) for s in order}
+        cv = threading.Condition()
+        ready = deque()          # (spec, pool job id)
+        state = {"admitting": True, "inflight_jobs": 0, "error": None}
+        t0 = time.perf_counter()
+
+        def now():
+            return time.perf_counter() - t0
+
+        def admit():
+            try:
+                for s in order:
+                    if realtime:
+                        delay = s.arrival - now()
+                        if delay > 0:
+                            time.sleep(delay)
+                    r = res[s.jid]
+                    r.arrival = s.arrival if realtime else 0.0
+                    if resident is not None:
+                        r.admitted = now()
+                        with cv:
+                            ready.append((s, resident[s.jid]))
+                            cv.notify_all()
+                        continue
+                    while True:
+                        try:
+                            job = s.add_to(pool)
+                            break
+                        except DssError as e:
+                            if e.code != DSS_ENOMEM:
+                                raise
+                            with cv:
+                                if not ready and state["inflight_jobs"] == 0:
+                                    raise DssError(e.code, f"job {s.jid} does not fit the empty pool") from e
+                                cv.wait(timeout=0.05)
+                    r.admitted = now()
+                    with cv:
+                        ready.append((s, job))
+                        cv.notify_all()
+            except BaseException as e:  # noqa: BLE001 -- reported by run()
+                state["error"] = e
+            finally:
+                with cv:
+                    state["admitting"] = False
+                    cv.notify_all()
+
+        def work():
+            batch = None
+            try:
+                while True:
+                    with cv:
+                        while not ready and state["admitting"] and state["error"] is None:
+                            cv.wait()
+                        if state["error"] is not None or (not ready and not state["admitting"]):
+                            return
+                        take = [ready.popleft()]
+                        strings = take[0][0].n
+                        while ready and strings + ready[0][0].n <= self.max_batch_strings \
+                                and len(take) < 65536:
+                            strings += ready[0][0].n
+                            take.append(ready.popleft())
+                        state["inflight_jobs"] += len(take)
+                    started = now()
+                    ids = [job for _, job in take]
+                    if batch is None:
+                        batch = pool.batch(ids)
+                    else:
+                        batch.rebatch(ids)
+                    batch.sort()
+                    D = batch.dist_sizes()
+                    outs = [batch.result(k) for k in range(len(take))] if keep_results else None
+                    if resident is None:
+                        for _, job in take:
+                            pool.free(job)
+                    done = now()
+                    with cv:
+                        for k, (s, _) in enumerate(take):
+                            r = res[s.jid]
+                            r.D = int(D[k])
+                            r.started, r.done, r.batch_jobs = started, done, len(take)
+                            if outs is not None:
+                                r.perm, r.lcp = outs[k]
+                        state["inflight_jobs"] -= len(take)
+                        cv.notify_all()
+            except BaseException as e:  # noqa: BLE001 -- reported by run()
+                with cv:
+                    state["error"] = e
+                    cv.notify_all()
+            finally:
+                if batch is not None:
+                    batch.close()
+
+        threads = [threading.Thread(target=admit, name="dss-admit")]
+        threads += [threading.Thread(target=work, name=f"dss-worker{k}") for k in range(self.inflight)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if state["error"] is not None:
+            raise state["error"]
+        return [res[s.jid] for s in specs]
+
+
+def run_plain(pool: JobPool, specs, resident: dict | None = None, keep_results: bool = False):
+    """Runs every job alone (a batch of one), one after the other.
+
+    With `resident` (jid -> pool job id of jobs already admitted) only the sort
+    is timed; otherwise each job is admitted first (untimed ingestion).
+    Returns per job (D, seconds of sort, seconds of ingestion, results)."""
+    out = {}
+    batch = None
+    try:
+        for s in specs:
+            g0 = time.perf_counter()
+            job = resident[s.jid] if resident is not None else s.add_to(pool)
+            g1 = time.perf_counter()
+            if batch is None:
+                batch = pool.batch([job])
+            else:
+                batch.rebatch([job])
+            batch.sort()
+            D = int(batch.dist_sizes()[0])
+            t1 = time.perf_counter()
+            r = batch.result(0) if keep_results else None
+            if resident is None:
+                pool.free(job)
+            out[s.jid] = (D, t1 - g1, g1 - g0, r)
+    finally:
+        if batch is not None:
+            batch.close()
+    return out
+
+
+def fifo_latencies(specs, service: dict) -> np.ndarray:
+    """Latencies of a single server that runs jobs one at a time in arrival
+    order, given each job's service time (seconds)."""
+    t = 0.0
+    lat = []
+    for s in sorted(specs, key=lambda s: s.arrival):
+        t = max(t, s.arrival) + service[s.jid]
+        lat.append(t - s.arrival)
+    return np.array(lat)
+
+
+def percentiles(x) -> dict:
+    x = np.asarray(x, dtype=np.float64)
+    if x.size == 0:
+        return {"p50_ms": None, "p99_ms": None}
+    return {"p50_ms": round(float(np.percentile(x, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(x, 99)) * 1e3, 3)}

@@ -202,6 +202,50 @@ typedef struct {
 
 int dss_check(dss_set s, dss_check_t* out);
 
+/* ---- batching and serving: a device job pool ----
+ *
+ * No counterpart in the reference (its drivers sort one input per MPI job).
+ * A dss_pool holds the character arenas and per-string offset/length arrays
+ * of many independent sort jobs in HBM, allocated first-fit from fixed
+ * capacities fixed at creation (the admission limit).  dss_pool_batch builds
+ * a dss_set over any subset of live jobs without copying characters: job k
+ * of the batch becomes virtual PE k, so dss_sort_lcp sorts every job of the
+ * batch in the same kernel launches (src/sorter/RadixSortLCP call sites, per
+ * job).  A pool may be used from several host threads; each batch set is
+ * used by one thread at a time, and a job must not be freed while a batch
+ * that holds it is alive. */
+typedef struct dss_pool_s* dss_pool;
+
+int dss_pool_create(uint64_t char_capacity, uint64_t string_capacity, dss_pool* out);
+int dss_pool_destroy(dss_pool pool);
+
+/* Adds a job from a host arena of NUL-terminated strings; *job = its id. */
+int dss_pool_add_job(dss_pool pool, const uint8_t* chars, uint64_t nbytes, uint32_t* job);
+/* Adds a job generated on the device, like dss_set_create_dton / _skewed. */
+int dss_pool_add_job_dton(dss_pool pool, uint64_t n, uint32_t length, double dton, uint64_t seed,
+                          uint32_t* job);
+int dss_pool_add_job_skewed(dss_pool pool, uint64_t n, uint32_t long_len, uint32_t short_len,
+                            double long_dn, double short_dn, uint32_t long_permille,
+                            uint32_t dup_permille, uint64_t seed, uint32_t* job);
+int dss_pool_free_job(dss_pool pool, uint32_t job);
+int dss_pool_job_info(dss_pool pool, uint32_t job, uint64_t* num_strings, uint64_t* num_bytes);
+int dss_pool_usage(dss_pool pool, uint64_t* bytes_used, uint64_t* strings_used,
+                   uint64_t* bytes_capacity, uint64_t* strings_capacity);
+
+/* A string set whose PE k is jobs[k] (characters borrowed from the pool). */
+int dss_pool_batch(dss_pool pool, const uint32_t* jobs, uint32_t count, dss_set* out);
+/* Re-points an existing batch set of the same pool at other jobs, keeping its
+ * device buffers (no allocation once they are large enough; avoids the
+ * device-wide synchronisation of freeing memory between batches). */
+int dss_pool_rebatch(dss_pool pool, dss_set batch, const uint32_t* jobs, uint32_t count);
+
+/* Per-PE results of the local sort: D of every PE's sorted strings alone
+ * (computeD, src/tests/DNGenerator_tests.cpp:22-45; 0 below two strings),
+ * and one PE's sorted order as PE-local string indices with its LCP array
+ * (the sorted StringLcpContainer of one job). */
+int dss_pe_dist_sizes(dss_set s, uint64_t* d_per_pe);
+int dss_get_sorted_pe(dss_set s, uint32_t pe, uint32_t* perm, uint32_t* lcp);
+
 /* ---- measurement ---- */
 
 /* Waits for all work queued on the set's stream. */

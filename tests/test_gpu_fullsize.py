@@ -95,3 +95,24 @@ def test_c4_suffixes_parity_slice():
         check_ok(ds.check())
         got, _ = ds.get_extended(len(exp))
     assert got.tobytes() == exp
+
+
+def test_dton_arena_beyond_32gib():
+    """40M x 900 chars (36 GB arena: more 8-byte words than one dispatch has
+    work items), D/N 0.5: generated, sorted and checked as a set and as a
+    pool job.  Each string is distinguished by its 450-char digit block, so
+    D = 450 n exactly."""
+    dss = _dss()
+    n, length, k = 40_000_000, 900, 450
+    with dss.DeviceStringSet.dton(n, length, 0.5, 0x5EED0006, 16) as ds:
+        ds.prefix_doubling_sort()
+        chk = ds.check()
+        check_ok(chk)
+        assert chk["D"] == k * n
+    with dss.JobPool(n * (length + 1) + 64, n) as pool:
+        j = pool.add_dton(n, length, 0.5, 0x5EED0006)
+        b = pool.batch([j])
+        b.sort()
+        assert int(b.dist_sizes()[0]) == k * n
+        b.close()
+        pool.free(j)
