@@ -236,19 +236,19 @@ def main():
                          "0 = 2^64-1 as in the reference")
     ap.add_argument("--cpu-sample", type=int, default=10_000_000,
                     help="strings in the CPU-oracle sample (0 disables the CPU leg)")
-    ap.add_argument("--serve-jobs", type=int, default=1000,
+    ap.add_argument("--serve-jobs", type=int, default=2000,
                     help="jobs of the batched/served leg (0 disables it)")
     ap.add_argument("--serve-n-min", type=int, default=1000)
-    ap.add_argument("--serve-n-max", type=int, default=200_000)
+    ap.add_argument("--serve-n-max", type=int, default=50_000_000)
     ap.add_argument("--serve-len-min", type=int, default=10)
     ap.add_argument("--serve-len-max", type=int, default=1000)
     ap.add_argument("--serve-seed", type=lambda x: int(x, 0), default=0x5EED0005)
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--serve-batch-strings", type=int, default=32 << 20)
-    ap.add_argument("--serve-pool-gb", type=float, default=48.0,
+    ap.add_argument("--serve-pool-gb", type=float, default=100.0,
                     help="pool capacity of the plain/batched comparison (GiB of characters)")
-    ap.add_argument("--serve-served-pool-gb", type=float, default=8.0,
+    ap.add_argument("--serve-served-pool-gb", type=float, default=64.0,
                     help="pool capacity of the served run (the admission limit)")
     args = ap.parse_args()
 

@@ -140,6 +140,7 @@ struct SortScratch {
     DevBuf segA, segB;      // large segment lists
     DevBuf tiles;           // tile list
     DevBuf hist;            // per tile x 256 digit counters / offsets
+    DevBuf csum;            // per tile chunk x 256 digit sums / offsets
     DevBuf small;           // small segment lists (all classes)
     DevBuf counters;        // device counters
     std::vector<uint64_t> small_off, small_cap;  // per class region in `small`

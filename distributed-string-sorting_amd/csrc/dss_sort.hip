@@ -45,6 +45,7 @@ constexpr uint32_t kClasses = 12;     // class c holds sizes (2^(c-1), 2^c], c =
 constexpr uint32_t kTile = 2048;      // strings per tile in the global passes
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kItems = kTile / kThreads;
+constexpr uint32_t kChunkTiles = 64;  // tiles per chunk of the two-level digit-offset scan
 
 struct Seg {
     uint32_t begin, cnt;
@@ -52,6 +53,7 @@ struct Seg {
     uint32_t kdepth;  // depth of the cached keys, kNone if keys must be gathered
     uint32_t cp;      // common prefix found by the prefix skip (from depth)
     uint32_t tile0;   // first tile of the segment in the current pass
+    uint32_t chunk0;  // first tile chunk (segments of more than kChunkTiles tiles)
 };
 
 struct Tile {
@@ -67,9 +69,10 @@ struct SmallLists {
     uint64_t cap[kClasses];
 };
 
-// counters: [0] next large segments, [1] tiles, [2 + c] small class c
-// [2 + kClasses + c] strings in small class c
-constexpr uint32_t kCtrLarge = 0, kCtrTiles = 1, kCtrSmall = 2, kCtrSmallElems = 2 + kClasses;
+// counters: [0] next large segments, [1] tiles, [2] tile chunks, [3 + c]
+// small class c, [3 + kClasses + c] strings in small class c
+constexpr uint32_t kCtrLarge = 0, kCtrTiles = 1, kCtrChunks = 2, kCtrSmall = 3,
+                   kCtrSmallElems = 3 + kClasses;
 
 __device__ inline void push_small(SmallSeg* small, const SmallLists& L, uint32_t* ctr,
                                   uint32_t begin, uint32_t cnt, uint32_t depth) {
@@ -121,49 +124,115 @@ __global__ void k_sort_seed(const uint64_t* __restrict__ pe_begin, uint32_t p, S
     const uint32_t cnt = (uint32_t)(pe_begin[r + 1] - pe_begin[r]);
     if (cnt > kLargeMin) {
         const uint32_t k = atomicAdd(&ctr[kCtrLarge], 1u);
-        segs[k] = Seg{b, cnt, 0, kNone, kNone, 0};
+        segs[k] = Seg{b, cnt, 0, kNone, kNone, 0, 0};
     } else if (cnt >= 2) {
         push_small(small, L, ctr, b, cnt, 0);
     }
 }
 
 // ------------------------------------------------------------------ tiles
-__global__ void k_build_tiles(Seg* __restrict__ segs, uint32_t nseg, Tile* __restrict__ tiles,
-                              uint32_t* __restrict__ ctr) {
-    __shared__ uint32_t part[1024];
-    __shared__ uint32_t carry;
+// Tile and chunk numbering of the large segments (one block scans the
+// per-segment counts); the tiles themselves are written by k_fill_tiles.
+// Segments of more than kChunkTiles tiles also get chunks of kChunkTiles
+// tiles for the two-level digit-offset scan.
+__device__ inline uint32_t seg_tiles(uint32_t cnt) { return (cnt + kTile - 1) / kTile; }
+__device__ inline uint32_t seg_chunks(uint32_t nt) {
+    return nt > kChunkTiles ? (nt + kChunkTiles - 1) / kChunkTiles : 0u;
+}
+
+__global__ void k_build_tiles(Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restrict__ ctr) {
+    __shared__ uint32_t part[1024], cpart[1024];
+    __shared__ uint32_t carry, ccarry;
     const uint32_t t = threadIdx.x;
-    if (t == 0) carry = 0;
+    if (t == 0) carry = ccarry = 0;
     __syncthreads();
     for (uint32_t base = 0; base < nseg; base += 1024) {
         const uint32_t i = base + t;
-        uint32_t nt = 0;
-        if (i < nseg) nt = (segs[i].cnt + kTile - 1) / kTile;
+        const uint32_t nt = i < nseg ? seg_tiles(segs[i].cnt) : 0u;
+        const uint32_t nc = seg_chunks(nt);
         part[t] = nt;
+        cpart[t] = nc;
         __syncthreads();
         for (uint32_t o = 1; o < 1024; o <<= 1) {
-            uint32_t x = t >= o ? part[t - o] : 0;
+            const uint32_t x = t >= o ? part[t - o] : 0;
+            const uint32_t y = t >= o ? cpart[t - o] : 0;
             __syncthreads();
             part[t] += x;
+            cpart[t] += y;
             __syncthreads();
         }
-        const uint32_t excl = carry + part[t] - nt;
         if (i < nseg) {
             Seg s = segs[i];
-            s.tile0 = excl;
+            s.tile0 = carry + part[t] - nt;
+            s.chunk0 = ccarry + cpart[t] - nc;
             if (s.kdepth == kNone) s.cp = kNone;
             segs[i] = s;
-            for (uint32_t k = 0; k < nt; ++k) {
-                const uint32_t b = s.begin + k * kTile;
-                const uint32_t e = min(s.begin + s.cnt, b + kTile);
-                tiles[excl + k] = Tile{i, b, e};
-            }
         }
         __syncthreads();
-        if (t == 1023) carry += part[1023];
+        if (t == 1023) {
+            carry += part[1023];
+            ccarry += cpart[1023];
+        }
         __syncthreads();
     }
-    if (t == 0) ctr[kCtrTiles] = carry;
+    if (t == 0) {
+        ctr[kCtrTiles] = carry;
+        ctr[kCtrChunks] = ccarry;
+    }
+}
+
+// Last segment whose first tile (or first chunk) is <= x: tile0 and chunk0
+// do not decrease along the segment list.
+template <bool kChunk>
+__device__ inline uint32_t seg_of(const Seg* __restrict__ segs, uint32_t nseg, uint32_t x) {
+    uint32_t lo = 0, hi = nseg;  // answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t v = kChunk ? segs[mid].chunk0 : segs[mid].tile0;
+        if (v <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_fill_tiles(const Seg* __restrict__ segs, uint32_t nseg, uint32_t ntiles,
+                             Tile* __restrict__ tiles) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const uint32_t i = seg_of<false>(segs, nseg, t);
+    const Seg s = segs[i];
+    const uint32_t b = s.begin + (t - s.tile0) * kTile;
+    tiles[t] = Tile{i, b, min(s.begin + s.cnt, b + kTile)};
+}
+
+// Per chunk and digit: the sum of the chunk's tile histograms.
+__global__ void __launch_bounds__(kThreads)
+k_chunk_sum(const Seg* __restrict__ segs, uint32_t nseg, const uint32_t* __restrict__ hist,
+            uint32_t* __restrict__ csum) {
+    const uint32_t k = blockIdx.x, d = threadIdx.x;
+    const Seg s = segs[seg_of<true>(segs, nseg, k)];
+    const uint32_t t0 = s.tile0 + (k - s.chunk0) * kChunkTiles;
+    const uint32_t t1 = min(s.tile0 + seg_tiles(s.cnt), t0 + kChunkTiles);
+    uint32_t tot = 0;
+    for (uint32_t t = t0; t < t1; ++t) tot += hist[(uint64_t)t * 256 + d];
+    csum[(uint64_t)k * 256 + d] = tot;
+}
+
+// Per chunk and digit: tile offsets from the chunk's base offset.
+__global__ void __launch_bounds__(kThreads)
+k_chunk_apply(const Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restrict__ hist,
+              const uint32_t* __restrict__ csum) {
+    const uint32_t k = blockIdx.x, d = threadIdx.x;
+    const Seg s = segs[seg_of<true>(segs, nseg, k)];
+    const uint32_t t0 = s.tile0 + (k - s.chunk0) * kChunkTiles;
+    const uint32_t t1 = min(s.tile0 + seg_tiles(s.cnt), t0 + kChunkTiles);
+    uint32_t run = csum[(uint64_t)k * 256 + d];
+    for (uint32_t t = t0; t < t1; ++t) {
+        uint32_t* hp = &hist[(uint64_t)t * 256 + d];
+        const uint32_t x = *hp;
+        *hp = run;
+        run += x;
+    }
 }
 
 // Effective key depth and digit depth of a segment in the current pass.
@@ -286,8 +355,8 @@ k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint6
 // tiles, LCPs at bucket boundaries, and the sub-bucket lists.  List slots
 // are reserved with one global atomic per list per block.
 __global__ void __launch_bounds__(kThreads)
-k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, Seg* __restrict__ next,
-                SmallSeg* __restrict__ small, SmallLists L, uint32_t* __restrict__ ctr,
+k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
+                Seg* __restrict__ next, SmallSeg* __restrict__ small, SmallLists L, uint32_t* __restrict__ ctr,
                 uint32_t* __restrict__ lcp) {
     __shared__ uint32_t sc[256];
     __shared__ uint32_t first;
@@ -296,9 +365,13 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, Seg* 
     uint32_t kd, dep;
     seg_depths(S, kd, dep);
     const uint32_t d = threadIdx.x;
-    const uint32_t nt = (S.cnt + kTile - 1) / kTile;
+    const uint32_t nt = seg_tiles(S.cnt);
+    const uint32_t nc = seg_chunks(nt);
+    // digit totals over the tiles, or over the chunk sums of a long segment
+    uint32_t* const col = nc ? csum + (uint64_t)S.chunk0 * 256 : hist + (uint64_t)S.tile0 * 256;
+    const uint32_t nrow = nc ? nc : nt;
     uint32_t tot = 0;
-    for (uint32_t t = 0; t < nt; ++t) tot += hist[(uint64_t)(S.tile0 + t) * 256 + d];
+    for (uint32_t t = 0; t < nrow; ++t) tot += col[(uint64_t)t * 256 + d];
     sc[d] = tot;
     if (d == 0) first = 256;
     if (d <= kClasses) lcnt[d] = 0;
@@ -312,9 +385,10 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, Seg* 
     }
     const uint32_t base = sc[d] - tot;
     if (tot) atomicMin(&first, d);
+    // exclusive offsets per tile (k_chunk_apply expands chunk offsets)
     uint32_t run = S.begin + base;
-    for (uint32_t t = 0; t < nt; ++t) {
-        uint32_t* hp = &hist[(uint64_t)(S.tile0 + t) * 256 + d];
+    for (uint32_t t = 0; t < nrow; ++t) {
+        uint32_t* hp = &col[(uint64_t)t * 256 + d];
         const uint32_t x = *hp;
         *hp = run;
         run += x;
@@ -345,7 +419,7 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, Seg* 
         // segment (one digit shared by all) also re-runs the prefix skip,
         // which jumps over the rest of the shared prefix at once
         const uint32_t nkd = (dep + 1 - kd >= 8 || tot == S.cnt) ? kNone : kd;
-        next[lbase[kClasses] + slot] = Seg{b, tot, dep + 1, nkd, kNone, 0};
+        next[lbase[kClasses] + slot] = Seg{b, tot, dep + 1, nkd, kNone, 0, 0};
     }
 }
 
@@ -635,6 +709,8 @@ void sort_lcp(dss_set_s* s) {
     const uint64_t tile_cap = n / kTile + seg_cap + 16;
     X.tiles.reserve(tile_cap * sizeof(Tile));
     X.hist.reserve(tile_cap * 256 * 4);
+    const uint64_t chunk_cap = tile_cap / kChunkTiles + 16;
+    X.csum.reserve(chunk_cap * 256 * 4);
     SmallLists L;
     uint64_t tot = 0;
     for (uint32_t c = 0; c < kClasses; ++c) {
@@ -663,7 +739,7 @@ void sort_lcp(dss_set_s* s) {
     uint64_t* kn = X.keyB.as<uint64_t>();
     uint32_t* pc = X.pbA.as<uint32_t>();
     uint32_t* pn = X.pbB.as<uint32_t>();
-    uint32_t h_ctr[2];
+    uint32_t h_ctr[3];
     DSS_HIP(hipMemcpyAsync(h_ctr, ctr, 4, hipMemcpyDeviceToHost, st));
     DSS_HIP(hipStreamSynchronize(st));
     uint32_t nlarge = h_ctr[0];
@@ -675,12 +751,16 @@ void sort_lcp(dss_set_s* s) {
     uint32_t* lcp = s->lcp.as<uint32_t>();
     while (nlarge) {
         DSS_REQUIRE(nlarge <= seg_cap, DSS_ERANGE, "segment list overflow");
-        DSS_HIP(hipMemsetAsync(ctr, 0, 8, st));
-        hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, cur, nlarge, X.tiles.as<Tile>(), ctr);
-        DSS_HIP(hipMemcpyAsync(h_ctr, ctr, 8, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipMemsetAsync(ctr, 0, 12, st));
+        hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, cur, nlarge, ctr);
+        DSS_HIP(hipMemcpyAsync(h_ctr, ctr, 12, hipMemcpyDeviceToHost, st));
         DSS_HIP(hipStreamSynchronize(st));
-        const uint32_t nt = h_ctr[1];
+        const uint32_t nt = h_ctr[kCtrTiles];
+        const uint32_t nchunks = h_ctr[kCtrChunks];
         DSS_REQUIRE(nt <= tile_cap, DSS_ERANGE, "tile list overflow");
+        DSS_REQUIRE(nchunks <= chunk_cap, DSS_ERANGE, "chunk list overflow");
+        hipLaunchKernelGGL(k_fill_tiles, dim3((nt + 255) / 256), dim3(256), 0, st, cur, nlarge, nt,
+                           X.tiles.as<Tile>());
         const Tile* tl = X.tiles.as<Tile>();
         // elements in large segments of this pass (for the algorithmic byte counts)
         std::vector<Seg> hseg;
@@ -704,8 +784,14 @@ void sort_lcp(dss_set_s* s) {
         }
         {
             KScope ks(s->timer, st, "scan_classify", (double)nt * 256 * 4 * 3);
+            if (nchunks)
+                hipLaunchKernelGGL(k_chunk_sum, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
+                                   X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
             hipLaunchKernelGGL(k_scan_classify, dim3(nlarge), dim3(kThreads), 0, st, cur, X.hist.as<uint32_t>(),
-                               nxt, X.small.as<SmallSeg>(), L, ctr, lcp);
+                               X.csum.as<uint32_t>(), nxt, X.small.as<SmallSeg>(), L, ctr, lcp);
+            if (nchunks)
+                hipLaunchKernelGGL(k_chunk_apply, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
+                                   X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
         }
         {
             KScope ks(s->timer, st, "scatter", (double)cnt_large * 28.0);
