@@ -484,8 +484,10 @@ ORC_API uint64_t orc_avg_lcp(const uint32_t* lcp, uint64_t n) {
 /* max_len: truncation length (merge sort: 2*100*(avgLcp+5)); dist: per-      */
 /* position distinguishing prefix (prefix doubling) or NULL.                   */
 /* Output: interval[r*p + q] = strings PE r sends to PE q.                     */
-/* Pinned by: the reference's closed-form rules, checked by brute force on    */
-/* tiny inputs (tests/test_oracle.py::test_partition_*).                      */
+/* Pinned by: an independent Python restatement (sorted() sample order, the  */
+/* reference's halving binarySearch) on inputs with and without duplicates,  */
+/* truncated and prefix-doubling splitters (NumStrings, IndexedNumStrings);  */
+/* NumChars samplers by bucket-order invariants only (tests/test_oracle.py). */
 /* ------------------------------------------------------------------------- */
 
 typedef struct {
@@ -677,7 +679,7 @@ ORC_API uint64_t orc_exchange(int mode, const uint8_t* chars, const uint64_t* of
 /* :395) when prefix compressed.  Strings are given as char pointers; with    */
 /* compression a string's stored chars start at logical position run_lcp.     */
 /* Pinned by: oracle/_ref built from the reference header (bit-exact incl.    */
-/* tie order), tests/test_ref_parity.py.                                      */
+/* tie order) and its golden vectors, tests/test_oracle.py.                   */
 /* ------------------------------------------------------------------------- */
 
 typedef struct {

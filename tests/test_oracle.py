@@ -361,3 +361,116 @@ def test_is_sorted_checker_cases():
     a2 = arena([b"a", b"c", b"b", b"d"])
     ptrs2, _ = O._arena_pointers(a2)
     assert not O.lib().orc_is_sorted(ptrs2, np.array([0, 2, 4], np.uint64).ctypes.data_as(O.u64p), 2)
+
+
+# ---------------------------------------------------------------- partition
+# A second, independent restatement of the partition rules in plain Python
+# (sorted() for the global sample order, the reference's halving binary
+# search), checked against the C oracle on small inputs with and without
+# duplicates, for truncated splitters and for prefix-doubling lengths.
+
+def _ref_binary_search(strs, key, idx_key=None, local_offset=0):
+    """binarySearch / binarySearchIndexed (misc.hpp:310-374): halving search
+    that returns the first probe that compares equal, else the insertion point."""
+    left, right = 0, len(strs)
+    while left != right:
+        dist = (right - left) // 2
+        cur = strs[left + dist]
+        if idx_key is None:
+            res = (cur > key) - (cur < key)
+        else:
+            a, b = (cur, left + dist + local_offset), (key, idx_key)
+            res = (a > b) - (a < b)
+        if res < 0:
+            left = left + dist + 1
+        elif res == 0:
+            return left + dist
+        else:
+            right = left + dist
+    return left
+
+
+def _brute_partition(pes, indexed, max_len=None, dist=None, factor=2):
+    """pes: per PE the sorted list of bytes strings."""
+    p = len(pes)
+    offsets = [0]
+    for x in pes:
+        offsets.append(offsets[-1] + len(x))
+    samples = []
+    for r, ss in enumerate(pes):
+        n = len(ss)
+        nr = min(factor * (p - 1), n)
+        sd = n / (nr + 1)
+        for i in range(1, nr + 1):
+            j = int(i * sd)
+            lim = dist[r][j] if dist is not None else max_len
+            samples.append((ss[j][:int(lim)], offsets[r] + j))
+    samples.sort(key=(lambda t: (t[0], t[1])) if indexed else (lambda t: t[0]))
+    nspl = min(p - 1, len(samples))
+    step = len(samples) // (nspl + 1) if nspl + 1 else 0
+    spl = [samples[i * step] for i in range(1, nspl + 1)]
+    out = np.zeros((p, p), dtype=np.uint64)
+    for r, ss in enumerate(pes):
+        pos = [_ref_binary_search(ss, s, i if indexed else None, offsets[r]) for s, i in spl]
+        pos = [0] + pos + [len(ss)]
+        for q in range(len(pos) - 1):  # fewer splitters than p - 1: later buckets stay empty
+            out[r, q] = pos[q + 1] - pos[q]
+    return out
+
+
+def _sorted_pes(ss):
+    perm, lcp = O.local_sort(ss)
+    pes = []
+    for r in range(ss.p):
+        b, e = int(ss.pe_begin[r]), int(ss.pe_begin[r + 1])
+        pes.append([ss.string(int(i)) for i in perm[b:e]])
+    return perm, lcp, pes
+
+
+PART_INPUTS = {
+    "distinct": lambda: O.gen_dton(900, 12, 0.5, 21),
+    "duplicates": lambda: arena([bytes([97 + (i * 7) % 3]) * (1 + (i * 5) % 4) for i in range(700)]),
+    "skewed": lambda: O.gen_skewed(O.skew_params(800, long_len=30, short_len=5, dup_permille=300), 22),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PART_INPUTS))
+@pytest.mark.parametrize("p", [2, 3, 5])
+@pytest.mark.parametrize("sampler", ["NumStrings", "IndexedNumStrings"])
+def test_partition_matches_brute_force(name, p, sampler):
+    ss = O.StringSet(PART_INPUTS[name](), p)
+    perm, lcp, pes = _sorted_pes(ss)
+    indexed = sampler.startswith("Indexed")
+    for max_len in (1000, 3):
+        got = O.partition(ss, perm, sampler, max_len=max_len)
+        exp = _brute_partition(pes, indexed, max_len=max_len)
+        assert (got == exp).all(), (name, p, sampler, max_len, got, exp)
+    # prefix-doubling form: splitters truncated to the per-position lengths
+    dist = np.array([1 + (i * 13) % 9 for i in range(ss.n)], dtype=np.uint32)
+    dl = [list(dist[int(ss.pe_begin[r]):int(ss.pe_begin[r + 1])]) for r in range(p)]
+    got = O.partition(ss, perm, sampler, dist=dist)
+    exp = _brute_partition(pes, indexed, dist=dl)
+    assert (got == exp).all(), (name, p, sampler, "dist", got, exp)
+
+
+@pytest.mark.parametrize("sampler", ["NumChars", "IndexedNumChars"])
+def test_partition_numchars_buckets_respect_the_order(sampler):
+    """NumChars samplers: every PE's buckets split its sorted strings at the
+    same splitter boundaries (strings in bucket q never exceed strings in
+    bucket q+1 of any PE), and every string is sent exactly once."""
+    ss = O.StringSet(O.gen_skewed(O.skew_params(1200, long_len=40, short_len=4), 23), 4)
+    perm, lcp, pes = _sorted_pes(ss)
+    iv = O.partition(ss, perm, sampler, max_len=1000)
+    assert (iv.sum(axis=1) == np.diff(ss.pe_begin)).all()
+    lo = [None] * 4
+    hi = [None] * 4
+    for r in range(4):
+        b = np.concatenate([[0], np.cumsum(iv[r])]).astype(int)
+        for q in range(4):
+            part = pes[r][b[q]:b[q + 1]]
+            if part:
+                lo[q] = min(part) if lo[q] is None else min(lo[q], min(part))
+                hi[q] = max(part) if hi[q] is None else max(hi[q], max(part))
+    for q in range(3):
+        if hi[q] is not None and lo[q + 1] is not None:
+            assert hi[q] <= lo[q + 1]
