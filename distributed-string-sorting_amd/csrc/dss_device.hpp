@@ -67,6 +67,31 @@ __device__ inline uint64_t load_key8(const uint8_t* __restrict__ chars, uint64_t
     return v;
 }
 
+// Four consecutive keys (32 characters) from `depth`, from five aligned
+// 8-byte loads; the arena padding keeps the fifth word readable.
+__device__ inline void load_keys4(const uint8_t* __restrict__ chars, uint64_t off, uint32_t len,
+                                  uint32_t depth, uint64_t k[4]) {
+    if (depth >= len) {
+        k[0] = k[1] = k[2] = k[3] = 0;
+        return;
+    }
+    const uint64_t a = off + depth;
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + (a & ~7ull));
+    const unsigned sh = (unsigned)(a & 7) * 8;
+    uint64_t x[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) x[i] = w[i];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint64_t v = sh ? (x[j] >> sh) | (x[j + 1] << (64 - sh)) : x[j];
+        v = __builtin_bswap64(v);
+        const uint32_t d = depth + 8 * j;
+        if (d >= len) v = 0;
+        else if (len - d < 8) v &= ~0ull << (8 * (8 - (len - d)));
+        k[j] = v;
+    }
+}
+
 // Number of equal leading characters of two big-endian 8-char keys that are
 // known to differ.
 __device__ inline uint32_t key_lcp(uint64_t a, uint64_t b) {

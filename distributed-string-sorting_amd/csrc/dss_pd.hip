@@ -99,15 +99,6 @@ __device__ uint64_t xxh64_stream(const uint8_t* __restrict__ chars, uint64_t off
 constexpr uint8_t kHashed = 1, kLcpDup = 2;
 constexpr uint64_t kEmpty = ~0ull;  // never a hash value: hash % m < m <= 2^64-1
 
-__device__ inline uint32_t pe_of(const uint64_t* __restrict__ pe_begin, uint32_t p, uint64_t i) {
-    uint32_t lo = 0, hi = p;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (pe_begin[mid] <= i) lo = mid + 1; else hi = mid;
-    }
-    return lo - 1;
-}
-
 __global__ void k_pd_init(uint8_t* __restrict__ cand, uint8_t* __restrict__ rounds, uint32_t* __restrict__ dist,
                           uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -150,50 +141,82 @@ __device__ inline uint64_t lds_word(const uint64_t* rows, uint32_t b) {
     return sh ? (lo >> sh) | (rows[w + 1] << (64 - sh)) : lo;
 }
 
-// Classification and hashing of the candidates of one round.  A block takes
-// 64 positions per iteration, four lanes each.  The prefixes to hash are
-// staged through LDS in 256-byte windows with 16-byte loads (consecutive
-// lanes read consecutive chunks of one string); lane j of a group then
-// accumulates XXH64 lane v_j+1 over the window's 32-byte stripes and lane 0
-// merges and hashes the tail.  All loops are block-uniform (the hashed
-// length is the round's depth), so every shuffle runs with all lanes active.
+// Classification of the candidates of one round, one thread per position:
+// end of string, local LCP duplicate or hashed.  Positions to hash collect
+// in LDS over kMarkIters block iterations and are appended to `list` with
+// one global atomic per flush.
+constexpr uint32_t kMarkIters = 16;
+
 __global__ void __launch_bounds__(256)
-k_pd_classify(const uint8_t* __restrict__ cand, const uint64_t* __restrict__ pe_begin, uint32_t p,
-              const uint32_t* __restrict__ lcp, const uint64_t* __restrict__ s_off,
-              const uint32_t* __restrict__ s_len, const uint8_t* __restrict__ chars, uint32_t depth, uint64_t m,
-              uint64_t n, uint8_t* __restrict__ flags, uint64_t* __restrict__ hashv, uint8_t* __restrict__ rounds,
-              uint32_t* __restrict__ dist, unsigned long long* __restrict__ nhashed) {
+k_pd_mark(const uint8_t* __restrict__ cand, const uint32_t* __restrict__ lcp, const uint32_t* __restrict__ s_len,
+          uint32_t depth, uint64_t n, uint8_t* __restrict__ flags, uint8_t* __restrict__ rounds,
+          uint32_t* __restrict__ dist, uint32_t* __restrict__ list, unsigned long long* __restrict__ nhashed) {
+    __shared__ uint32_t sl[256 * kMarkIters];
+    __shared__ uint32_t scount, base;
+    const uint32_t lane = lane_id();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (threadIdx.x == 0) scount = 0;
+    __syncthreads();
+    uint32_t it = 0;
+    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+        const uint64_t c = b0 + threadIdx.x;
+        uint8_t f = 0;
+        if (c < n) {
+            // independent loads first; lcp is 0 at every PE start and depth >= 1,
+            // so lcp[c] >= depth implies that c-1 is in the same PE
+            const uint8_t cc = cand[c];
+            const uint8_t cp = c > 0 ? cand[c - 1] : 0;
+            const uint32_t L = s_len[c];
+            const uint32_t lc = lcp[c];
+            if (cc) {
+                if (depth <= L) f = (c > 0 && lc >= depth && cp) ? kLcpDup : kHashed;
+                rounds[c] = rounds[c] + (rounds[c] < 255);
+                dist[c] = depth > L ? L : depth;  // end of string: results = len
+            }
+            flags[c] = f;
+        }
+        const uint64_t bal = __ballot(f == kHashed);
+        uint32_t wbase = 0;
+        if (lane == 0 && bal) wbase = atomicAdd(&scount, (uint32_t)__popcll(bal));
+        wbase = __shfl(wbase, 0);
+        if (f == kHashed) sl[wbase + (uint32_t)__popcll(bal & lanemask_lt())] = (uint32_t)c;
+        if (++it == kMarkIters || b0 + stride >= n) {  // block-uniform
+            it = 0;
+            __syncthreads();
+            const uint32_t cnt = scount;
+            if (threadIdx.x == 0 && cnt) base = (uint32_t)atomicAdd(nhashed, (unsigned long long)cnt);
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) list[base + k] = sl[k];
+            __syncthreads();
+            if (threadIdx.x == 0) scount = 0;
+            __syncthreads();
+        }
+    }
+}
+
+// Hashing of the listed positions.  A block takes 64 positions per
+// iteration, four lanes each.  The prefixes are staged through LDS in
+// 256-byte windows with 16-byte loads (consecutive lanes read consecutive
+// chunks of one string); lane j of a group then accumulates XXH64 lane v_j+1
+// over the window's 32-byte stripes and lane 0 merges and hashes the tail.
+// All loops are block-uniform (the hashed length is the round's depth), so
+// every shuffle runs with all lanes active.
+__global__ void __launch_bounds__(256)
+k_pd_hash(const uint32_t* __restrict__ list, const unsigned long long* __restrict__ nhashed,
+          const uint64_t* __restrict__ s_off, const uint8_t* __restrict__ chars, uint32_t depth, uint64_t m,
+          uint64_t* __restrict__ hashv) {
     __shared__ uint64_t rows[kPdPos * kRowW];
     __shared__ uint64_t s_so[kPdPos];
     const uint32_t j = threadIdx.x & 3, g = threadIdx.x >> 2;
     const uint32_t base_lane = threadIdx.x & ~3u;
     const uint32_t stripes = depth / 32;
     const uint32_t nwin = (depth + kWin - 1) / kWin;
-    uint32_t counted = 0;
-    for (uint64_t blk = (uint64_t)blockIdx.x * kPdPos; blk < n; blk += (uint64_t)gridDim.x * kPdPos) {
-        const uint64_t c = blk + g;
-        uint8_t f = 0;
-        uint64_t so = 0;
-        if (c < n && cand[c]) {
-            const uint32_t L = s_len[c];
-            if (depth <= L) {
-                // lcp is 0 at every PE start and depth >= 1, so lcp[c] >= depth
-                // implies that c-1 is in the same PE
-                if (c > 0 && lcp[c] >= depth && cand[c - 1]) f = kLcpDup;
-                else {
-                    f = kHashed;
-                    so = s_off[c];
-                }
-            }
-            if (j == 0) {
-                rounds[c] = rounds[c] + (rounds[c] < 255);
-                dist[c] = depth > L ? L : depth;  // end of string: results = len
-                flags[c] = f;
-            }
-        } else if (c < n && j == 0) {
-            flags[c] = 0;
-        }
-        const bool hashed = f == kHashed;
+    const uint64_t nh = *nhashed;
+    for (uint64_t blk = (uint64_t)blockIdx.x * kPdPos; blk < nh; blk += (uint64_t)gridDim.x * kPdPos) {
+        const uint64_t i = blk + g;
+        const bool hashed = i < nh;
+        const uint32_t c = hashed ? list[i] : 0u;
+        const uint64_t so = hashed ? s_off[c] : 0ull;
         if (j == 0) s_so[g] = hashed ? so : ~0ull;
         uint64_t v = j == 0 ? P1 + P2 : (j == 1 ? P2 : (j == 2 ? 0ull : (uint64_t)0 - P1));
         uint32_t shift = 0;
@@ -263,11 +286,9 @@ k_pd_classify(const uint8_t* __restrict__ cand, const uint64_t* __restrict__ pe_
             h *= P3;
             h ^= h >> 32;
             hashv[c] = mod_range(h, m);
-            ++counted;
         }
         __syncthreads();  // rows and s_so are rewritten by the next iteration
     }
-    block_count_add(counted, nhashed);
 }
 
 __global__ void k_fill_u64(uint64_t* __restrict__ a, uint64_t n, uint64_t v) {
@@ -281,11 +302,12 @@ __device__ inline uint64_t slot_of(uint64_t h, uint64_t mask) {
     return (h >> 17) & mask;
 }
 
-__global__ void k_pd_insert(const uint8_t* __restrict__ flags, const uint64_t* __restrict__ hashv, uint64_t n,
+__global__ void k_pd_insert(const uint32_t* __restrict__ list, uint64_t nh, const uint64_t* __restrict__ hashv,
                             uint64_t* __restrict__ keys, uint8_t* __restrict__ dup, uint64_t mask,
                             uint32_t* __restrict__ slot) {
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n || flags[c] != kHashed) return;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nh) return;
+    const uint32_t c = list[i];
     const uint64_t h = hashv[c];
     uint64_t s = slot_of(h, mask);
     for (uint64_t probe = 0; probe <= mask; ++probe) {
@@ -339,6 +361,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     P.pd_flags.reserve(nn);
     P.pd_hash.reserve(nn * 8);
     P.pd_slot.reserve(nn * 4);
+    P.pd_list.reserve(nn * 4);
     P.pd_ctr.reserve(64);
     uint8_t* cand = P.pd_cand.as<uint8_t>();
     uint8_t* next = P.pd_next.as<uint8_t>();
@@ -353,19 +376,25 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
         DSS_REQUIRE(2 * ncand < (1ull << 32), DSS_ERANGE, "too many candidates for the hash table");
         DSS_HIP(hipMemsetAsync(ctr, 0, 16, st));
         {
+            // every position: candidate flags of c and c-1, length, lcp, the
+            // flag written; every candidate: rounds and dist
+            KScope ks(s->timer, st, "pd_mark", (double)n * 11.0 + (double)ncand * 6.0);
+            hipLaunchKernelGGL(k_pd_mark, dim3(std::min<uint64_t>((n + 255) / 256, kPdGrid)), blk, 0, st, cand,
+                               s->lcp.as<uint32_t>(), s->s_len.as<uint32_t>(), (uint32_t)depth, n,
+                               P.pd_flags.as<uint8_t>(), P.rounds.as<uint8_t>(), P.dist.as<uint32_t>(),
+                               P.pd_list.as<uint32_t>(), ctr);
+        }
+        {
             KScope ks(s->timer, st, "pd_hash", 0.0);
-            hipLaunchKernelGGL(k_pd_classify, dim3(std::min<uint64_t>((n + 63) / 64, kPdGrid)), blk, 0, st, cand, s->pe_begin_d.as<uint64_t>(), p,
-                               s->lcp.as<uint32_t>(), s->s_off.as<uint64_t>(),
-                               s->s_len.as<uint32_t>(), s->arena(), (uint32_t)depth, m, n,
-                               P.pd_flags.as<uint8_t>(), P.pd_hash.as<uint64_t>(), P.rounds.as<uint8_t>(),
-                               P.dist.as<uint32_t>(), ctr);
+            hipLaunchKernelGGL(k_pd_hash, dim3(std::min<uint64_t>((ncand + kPdPos - 1) / kPdPos, kPdGrid)), blk, 0,
+                               st, P.pd_list.as<uint32_t>(), ctr, s->s_off.as<uint64_t>(), s->arena(),
+                               (uint32_t)depth, m, P.pd_hash.as<uint64_t>());
         }
         uint64_t nh = 0;
         DSS_HIP(hipMemcpyAsync(&nh, ctr, 8, hipMemcpyDeviceToHost, st));
         DSS_HIP(hipStreamSynchronize(st));
-        // every position: candidate flag, flags/rounds/dist; every candidate:
-        // id, offset, length, lcp; every hashed candidate: depth chars + hash
-        s->timer.amend_last("pd_hash", (double)n * 8.0 + (double)ncand * 24.0 + (double)nh * (depth + 8.0));
+        // every hashed candidate: list entry, offset, depth chars, hash
+        s->timer.amend_last("pd_hash", (double)nh * (depth + 20.0));
         uint64_t T = 1024;
         while (T < 2 * nh) T <<= 1;
         P.pd_table.reserve(T * 8);
@@ -375,8 +404,9 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
             hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T, kEmpty);
             DSS_HIP(hipMemsetAsync(P.pd_tflag.p, 0, T, st));
             if (nh)
-                hipLaunchKernelGGL(k_pd_insert, grid, blk, 0, st, P.pd_flags.as<uint8_t>(), P.pd_hash.as<uint64_t>(), n,
-                                   P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(), T - 1, P.pd_slot.as<uint32_t>());
+                hipLaunchKernelGGL(k_pd_insert, dim3((nh + 255) / 256), blk, 0, st, P.pd_list.as<uint32_t>(), nh,
+                                   P.pd_hash.as<uint64_t>(), P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(),
+                                   T - 1, P.pd_slot.as<uint32_t>());
             hipLaunchKernelGGL(k_pd_next, dim3(std::min<uint64_t>((n + 255) / 256, kPdGrid)), blk, 0, st, cand, P.pd_flags.as<uint8_t>(), s->pe_begin_d.as<uint64_t>(),
                                p, s->lcp.as<uint32_t>(), (uint32_t)depth, n, P.pd_tflag.as<uint8_t>(),
                                P.pd_slot.as<uint32_t>(), next, ctr + 1);

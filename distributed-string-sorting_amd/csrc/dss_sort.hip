@@ -46,6 +46,7 @@ constexpr uint32_t kTile = 2048;      // strings per tile in the global passes
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kItems = kTile / kThreads;
 constexpr uint32_t kChunkTiles = 64;  // tiles per chunk of the two-level digit-offset scan
+constexpr uint32_t kWaveClassMax = 6;  // classes up to 64 strings use the one-wave base case
 
 struct Seg {
     uint32_t begin, cnt;
@@ -247,71 +248,98 @@ __device__ inline void seg_depths(const Seg& s, uint32_t& kd, uint32_t& dep) {
 }
 
 // ------------------------------------------------------------ prefix skip
-// Groups of 16 lanes compare one string against the segment's first string,
-// 256 bytes per step (two 8-char keys per lane).  The segment minimum found
-// so far is consulted once per string and once per further step: a string
-// whose compared prefix already exceeds it cannot lower it.  All loops that
-// contain cross-lane operations have block- or wave-uniform trip counts, so
-// every shuffle runs with all lanes of the wave active.
+// Groups of 8 lanes compare one string against the segment's first string,
+// 256 bytes per step (four 8-char keys from five aligned loads per lane), so
+// a wave has 8 strings and 32 independent key loads in flight.  The tile's
+// offsets and lengths are staged in LDS first, so a step waits for one
+// dependent load; the reference's first-step keys stay in registers.  The
+// minimum found so far (the block's, and every 8 rounds the segment's)
+// bounds each further comparison: keys at or beyond it cannot lower the
+// minimum and are not loaded, and a string stops once its compared prefix
+// reaches it.  All loops that contain cross-lane operations have block- or
+// wave-uniform trip counts, so every shuffle runs with all lanes active.
+constexpr uint32_t kSkipLanes = 8;
+constexpr uint32_t kSkipGroups = kThreads / kSkipLanes;
+
 __global__ void __launch_bounds__(kThreads)
 k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
               const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
               const uint32_t* __restrict__ len) {
+    __shared__ uint64_t t_off[kTile];
+    __shared__ uint32_t t_len[kTile];
     __shared__ uint32_t bmin;
     const Tile T = tiles[blockIdx.x];
     const Seg S = segs[T.seg];
     if (S.kdepth != kNone) return;
+    const uint32_t cnt = T.end - T.begin;
     if (threadIdx.x == 0) bmin = kNone;
+    for (uint32_t k = threadIdx.x; k < cnt; k += kThreads) {
+        const uint32_t id = perm[T.begin + k];
+        t_off[k] = off[id];
+        t_len[k] = len[id];
+    }
     __syncthreads();
-    const uint32_t g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const uint32_t g = threadIdx.x / kSkipLanes, l = threadIdx.x % kSkipLanes;
     const uint32_t ref = perm[S.begin];
     const uint64_t roff = off[ref];
     const uint32_t rlen = len[ref];
     const uint32_t d0 = S.depth;
+    uint64_t rk[4];
+    load_keys4(chars, roff, rlen, d0 + 32 * l, rk);
     const volatile uint32_t* cpp = &segs[T.seg].cp;
-    uint32_t my_min = kNone;
-    const uint32_t rounds = (T.end - T.begin + 15) / 16;
+    const volatile uint32_t* vbmin = &bmin;
+    uint32_t glim = kNone;
+    const uint32_t rounds = (cnt + kSkipGroups - 1) / kSkipGroups;
     for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t e = T.begin + r * 16 + g;
-        bool done = e >= T.end;
+        const uint32_t k = r * kSkipGroups + g;
+        bool done = k >= cnt;
         uint64_t so = 0;
         uint32_t sl = 0;
         if (!done) {
-            const uint32_t id = perm[e];
-            so = off[id];
-            sl = len[id];
+            so = t_off[k];
+            sl = t_len[k];
         }
-        uint32_t lim = (r & 7) == 0 ? *cpp : kNone;  // refreshed every 8 strings
+        if ((r & 7) == 0) glim = *cpp;
+        uint32_t lim = min(glim, *vbmin);
         uint32_t base = d0;
         uint32_t res = kNone;
         while (__ballot(!done) != 0) {  // wave-uniform condition
             uint32_t m = kNone;
             if (!done) {
-                const uint32_t p0 = base + 8 * l, p1 = p0 + 128;
-                const uint64_t a0 = load_key8(chars, so, sl, p0), b0 = load_key8(chars, roff, rlen, p0);
-                const uint64_t a1 = load_key8(chars, so, sl, p1), b1 = load_key8(chars, roff, rlen, p1);
-                if (a0 != b0) m = p0 + key_lcp(a0, b0);
-                else if ((a0 & 0xFF) == 0) m = p0 + key_strlen(a0);
-                else if (a1 != b1) m = p1 + key_lcp(a1, b1);
-                else if ((a1 & 0xFF) == 0) m = p1 + key_strlen(a1);
+                const uint32_t p0 = base + 32 * l;
+                if (lim == kNone || p0 - d0 < lim) {
+                    uint64_t a[4], b[4];
+                    load_keys4(chars, so, sl, p0, a);
+                    if (base == d0) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) b[j] = rk[j];
+                    } else {
+                        load_keys4(chars, roff, rlen, p0, b);
+                    }
+#pragma unroll
+                    for (int j = 3; j >= 0; --j) {  // the first event of the lane wins
+                        if (a[j] != b[j]) m = p0 + 8 * j + key_lcp(a[j], b[j]);
+                        else if ((a[j] & 0xFF) == 0) m = p0 + 8 * j + key_strlen(a[j]);
+                    }
+                }
             }
-            // first hit in position order: lanes 0..15 cover [base, base+128),
-            // then [base+128, base+256); the minimum is the answer
-            for (int o = 8; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor(m, o, 16));
+            // lanes cover consecutive 32-byte pieces of the step: the
+            // minimum over the group is the first mismatch or end
+            for (int o = kSkipLanes / 2; o > 0; o >>= 1)
+                m = min(m, (uint32_t)__shfl_xor(m, o, kSkipLanes));
             if (!done) {
                 if (m != kNone) {
                     res = m - d0;
                     done = true;
                 } else {
-                    base += 256;
-                    if (lim == kNone) lim = *cpp;
+                    base += 32 * kSkipLanes;
+                    lim = min(lim, *vbmin);
                     if (lim != kNone && base - d0 >= lim) done = true;  // cannot lower the minimum
                 }
             }
         }
-        if (res != kNone) my_min = min(my_min, res);
+        if (l == 0 && res != kNone) atomicMin(&bmin, res);
     }
-    if (l == 0 && my_min != kNone) atomicMin(&bmin, my_min);
     __syncthreads();
     if (threadIdx.x == 0 && bmin != kNone) atomicMin(&segs[T.seg].cp, bmin);
 }
@@ -674,6 +702,124 @@ k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2
     }
 }
 
+// Small segments of at most 64 strings, one wave per 64 slots (64/CLS
+// segments per wave, one string per lane).  Each lane keeps its string's
+// 8-char key at its group's depth, the group leader lane G and the slot it
+// came from in registers; offset, length and id stay in LDS under that slot.
+// An iteration sorts (G, key, slot) with a register bitonic network inside
+// each aligned CLS-lane slot, splits groups where keys differ (LCP = depth +
+// common key bytes), and lets every group whose members still share the key
+// continue 8 characters deeper; groups of equal strings (key ended) stop.  A
+// round in which no active group has two different keys only advances the
+// depths.  Group extents come from ballots of the leader lanes; every loop
+// runs while a ballot over the wave is non-zero, so all lanes take part in
+// every shuffle.
+__global__ void __launch_bounds__(kThreads)
+k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2,
+                  uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
+                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ len) {
+    __shared__ uint64_t s_off[kThreads];
+    __shared__ uint32_t s_len[kThreads], s_id[kThreads];
+    const uint32_t lane = lane_id();
+    const uint32_t wbase = threadIdx.x & ~63u;  // this wave's LDS rows
+    const uint32_t CLS = 1u << cls_log2;
+    const uint32_t spw = 64u >> cls_log2;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t slot_start = lane & ~(CLS - 1), j = lane & (CLS - 1);
+    const uint64_t sidx = wave * spw + (lane >> cls_log2);
+    bool valid = false;
+    uint32_t sbegin = 0, depth0 = 0, scnt = 0;
+    if (sidx < nseg) {
+        const SmallSeg sg = segs[sidx];
+        sbegin = sg.begin;
+        scnt = sg.cnt;
+        depth0 = sg.depth;
+        if (j < sg.cnt) {
+            valid = true;
+            const uint32_t id = perm[sg.begin + j];
+            s_id[wbase + lane] = id;
+            s_off[wbase + lane] = off[id];
+            s_len[wbase + lane] = len[id];
+        }
+    }
+    __syncthreads();
+    // per element (moves with the sort): G, K, src (original lane)
+    uint32_t G = valid ? slot_start : lane;  // padding lanes: own inactive group, after the strings
+    uint64_t K = valid ? 0 : ~0ull;
+    uint32_t src = lane;
+    // per position: LCP with the predecessor; per leader lane: depth, active
+    uint32_t lo = 0;
+    uint32_t rd = depth0;
+    bool act = valid && j == 0 && scnt >= 2;
+    for (;;) {
+        const uint64_t leaders = __ballot(lane == G);
+        const uint32_t grd = __shfl(rd, G);
+        const bool gact = __shfl((int)act, G) != 0;
+        if (__ballot(gact) == 0) break;
+        if (gact) K = load_key8(chars, s_off[wbase + src], s_len[wbase + src], grd);
+        // group extent [G, gend) from the leader mask
+        const uint64_t above = G >= 63 ? 0ull : (leaders & (~0ull << (G + 1)));
+        const uint32_t gend = above ? (uint32_t)__builtin_ctzll(above) : 64u;
+        const uint64_t grp = (gend >= 64 ? ~0ull : ((1ull << gend) - 1)) & (~0ull << G);
+        const uint64_t leadK = __shfl((unsigned long long)K, G);
+        const bool differs = gact && K != leadK;
+        const uint64_t dmask = __ballot(differs);
+        if (__ballot(gact && (dmask & grp) != 0) == 0) {
+            // no active group has two different keys: equal keys go deeper
+            // unless they ended (then the group is a run of equal strings)
+            if (gact && lane != G && (K & 0xFF) == 0) lo = grd + key_strlen(K);
+            if (lane == G && gact) {
+                if ((K & 0xFF) == 0) act = false;
+                else rd = grd + 8;
+            }
+            continue;
+        }
+        // bitonic sort of (G, K, src) inside every aligned CLS-lane slot
+        for (uint32_t k = 2; k <= CLS; k <<= 1) {
+            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+                const uint32_t pG = __shfl_xor(G, jj);
+                const uint64_t pK = __shfl_xor((unsigned long long)K, jj);
+                const uint32_t pS = __shfl_xor(src, jj);
+                const bool asc = (k == CLS) || ((lane & k) == 0);
+                const bool lower = (lane & jj) == 0;
+                const bool pless = pG < G || (pG == G && (pK < K || (pK == K && pS < src)));
+                // the lower lane keeps the smaller element when ascending
+                if (lower == asc ? pless : !pless) {
+                    G = pG;
+                    K = pK;
+                    src = pS;
+                }
+            }
+        }
+        // boundaries and LCPs inside active groups
+        const uint64_t pK = __shfl_up((unsigned long long)K, 1);
+        bool flag = lane == G;
+        if (!flag && gact) {
+            if (pK != K) {
+                flag = true;
+                lo = grd + key_lcp(pK, K);
+            } else if ((K & 0xFF) == 0) {
+                lo = grd + key_strlen(K);
+            }
+        }
+        const uint64_t fmask = __ballot(flag);
+        const uint32_t newG = 63u - (uint32_t)__builtin_clzll(fmask & (lane == 63 ? ~0ull : ((2ull << lane) - 1)));
+        if (flag && gact) {
+            // a new group of an active group: 8 characters deeper, active if
+            // it has a second member (padding lanes are always leaders) and
+            // its key did not end
+            const bool second = lane < 63 && !((fmask >> (lane + 1)) & 1);
+            rd = grd + 8;
+            act = second && (K & 0xFF) != 0;
+        }
+        G = newG;
+    }
+    if (valid) {
+        perm[sbegin + j] = s_id[wbase + src];
+        if (j) lcp[sbegin + j] = lo;
+    }
+}
+
 // Offset and length per sorted position: every later stage streams these
 // instead of gathering through the permutation.
 __global__ void k_sorted_meta(const uint32_t* __restrict__ perm, const uint64_t* __restrict__ off,
@@ -832,16 +978,28 @@ void sort_lcp(dss_set_s* s) {
     uint32_t cls[2 * kClasses];
     DSS_HIP(hipMemcpyAsync(cls, ctr + kCtrSmall, sizeof(cls), hipMemcpyDeviceToHost, st));
     DSS_HIP(hipStreamSynchronize(st));
+    if (trace) {
+        fprintf(stderr, "[dss sort] small classes:");
+        for (uint32_t c = 1; c < kClasses; ++c)
+            if (cls[c]) fprintf(stderr, " c%u=%u segs/%u strings", c, cls[c], cls[kClasses + c]);
+        fprintf(stderr, "\n");
+    }
     for (uint32_t c = 1; c < kClasses; ++c) {
         const uint32_t cnt = cls[c];
         if (!cnt) continue;
         DSS_REQUIRE(cnt <= L.cap[c], DSS_ERANGE, "small list overflow");
-        const uint32_t spb = kCap >> c;
-        const uint32_t blocks = (cnt + spb - 1) / spb;
         // per string: id read, offset/length, one 8-char key, id + lcp writes
         KScope ks(s->timer, st, "small_sort", (double)cls[kClasses + c] * 32.0);
-        hipLaunchKernelGGL(k_small_sort, dim3(blocks), dim3(kThreads), 0, st,
-                           X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len);
+        if (c <= kWaveClassMax) {
+            const uint64_t waves = (cnt + (64u >> c) - 1) / (64u >> c);
+            hipLaunchKernelGGL(k_small_sort_wave, dim3((waves + 3) / 4), dim3(kThreads), 0, st,
+                               X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len);
+        } else {
+            const uint32_t spb = kCap >> c;
+            const uint32_t blocks = (cnt + spb - 1) / spb;
+            hipLaunchKernelGGL(k_small_sort, dim3(blocks), dim3(kThreads), 0, st,
+                               X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len);
+        }
     }
     {
         KScope ks(s->timer, st, "sort_meta", n * 28.0);
