@@ -365,11 +365,161 @@ __global__ void k_skew_fill(SkewP P, uint64_t seed, const uint64_t* __restrict__
     }
 }
 
+// ------------------------------------------------------ suffix views
+// Capped suffixes of one text without materialising them: string i is the
+// view text[i .. i + min(cap, len - i)) (the strings of SuffixGenerator,
+// random_string_generator.hpp:89-115, capped as in the suffix workload).
+__global__ void k_suffix_views(uint64_t* __restrict__ off, uint32_t* __restrict__ len, uint64_t n, uint32_t cap) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    off[i] = i;
+    len[i] = (uint32_t)min((uint64_t)cap, n - i);
+}
+
+// Materialised size of every string (length + NUL), for the byte-slice PE
+// split of the materialised arena.
+__global__ void k_len_plus1(const uint32_t* __restrict__ len, uint64_t n, uint64_t* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint64_t)len[i] + 1;
+}
+
+// Order-3 Markov text over {A,C,G,T}: the chain restarts from a seeded
+// context every 4096 characters, so one thread generates one 4096-character
+// block (the oracle's generator, same seed streams).
+struct MarkovTable {
+    uint32_t cum[64][4];
+};
+
+__global__ void k_markov_text(uint8_t* __restrict__ text, uint64_t text_len, uint64_t seed, MarkovTable T) {
+    const uint64_t blk = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t base = blk * 4096;
+    if (base >= text_len) return;
+    const uint64_t end = min(text_len, base + 4096);
+    uint32_t ctx = (uint32_t)(rng(seed, 8, base) % 64);
+    for (uint64_t i = base; i < end; ++i) {
+        const uint32_t r = (uint32_t)(rng(seed, 9, i) % T.cum[ctx][3]);
+        uint32_t sym = 0;
+        while (r >= T.cum[ctx][sym]) ++sym;
+        text[i] = "ACGT"[sym];
+        ctx = ((ctx << 2) | sym) & 63;
+    }
+}
+
+// Planted repeats, applied in order (each copy may read earlier copies):
+// one block moves each repeat with memmove semantics through LDS chunks,
+// front to back when the destination lies before the source, else back to
+// front.
+__global__ void __launch_bounds__(1024)
+k_plant_repeats(uint8_t* __restrict__ text, uint64_t text_len, uint32_t repeats, uint32_t rep_min,
+                uint32_t rep_max, uint64_t seed) {
+    __shared__ uint8_t buf[8192];
+    for (uint32_t r = 0; r < repeats; ++r) {
+        const uint64_t span = rep_max > rep_min ? (uint64_t)rep_max - rep_min + 1 : 1;
+        const uint64_t len = rep_min + rng(seed, 10, r) % span;
+        if (len * 2 >= text_len) continue;
+        const uint64_t src = rng(seed, 11, r) % (text_len - len);
+        const uint64_t dst = rng(seed, 12, r) % (text_len - len);
+        const uint64_t nchunks = (len + sizeof(buf) - 1) / sizeof(buf);
+        for (uint64_t q = 0; q < nchunks; ++q) {
+            const uint64_t k = dst <= src ? q : nchunks - 1 - q;
+            const uint64_t c0 = k * sizeof(buf), c1 = min(len, c0 + sizeof(buf));
+            for (uint64_t t = c0 + threadIdx.x; t < c1; t += blockDim.x) buf[t - c0] = text[src + t];
+            __syncthreads();
+            for (uint64_t t = c0 + threadIdx.x; t < c1; t += blockDim.x) text[dst + t] = buf[t - c0];
+            __syncthreads();
+        }
+    }
+}
+
+// Views over the set's arena (already holding the text), PE split and max length.
+static void make_suffix_views(dss_set_s* s, uint64_t text_len, uint32_t cap) {
+    const uint64_t n = text_len;
+    const hipStream_t st = s->stream;
+    s->n = n;
+    s->nbytes = text_len;
+    s->off.reserve(std::max<uint64_t>(n, 1) * 8);
+    s->len.reserve(std::max<uint64_t>(n, 1) * 4);
+    DevBuf voff;
+    voff.reserve(std::max<uint64_t>(n, 1) * 8);
+    uint64_t total = 0;
+    if (n) {
+        hipLaunchKernelGGL(k_suffix_views, dim3((n + 255) / 256), dim3(256), 0, st, s->off.as<uint64_t>(),
+                           s->len.as<uint32_t>(), n, cap);
+        hipLaunchKernelGGL(k_len_plus1, dim3((n + 255) / 256), dim3(256), 0, st, s->len.as<uint32_t>(), n,
+                           voff.as<uint64_t>());
+        exclusive_scan_u64(voff.as<uint64_t>(), voff.as<uint64_t>(), n, st, s->pp, &total);
+    }
+    s->pe_begin_d.reserve((s->p + 1) * 8);
+    hipLaunchKernelGGL(k_pe_split, dim3((s->p + 1 + 255) / 256), dim3(256), 0, st, voff.as<uint64_t>(), n, total,
+                       s->p, s->pe_begin_d.as<uint64_t>());
+    s->pe_begin.assign(s->p + 1, 0);
+    DSS_HIP(hipMemcpyAsync(s->pe_begin.data(), s->pe_begin_d.p, (s->p + 1) * 8, hipMemcpyDeviceToHost, st));
+    DSS_HIP(hipGetLastError());
+    DSS_HIP(hipStreamSynchronize(st));
+    s->max_len = (uint32_t)std::min<uint64_t>(cap, n);
+}
+
 }  // namespace dss
 
 using namespace dss;
 
 extern "C" {
+
+int dss_set_create_suffixes(const uint8_t* text, uint64_t text_len, uint32_t cap, uint32_t num_pes,
+                            dss_set* out) {
+    return guarded([&] {
+        DSS_REQUIRE(out && (text_len == 0 || text), DSS_EINVAL, "NULL argument");
+        DSS_REQUIRE(cap >= 1, DSS_EINVAL, "cap must be >= 1");
+        DSS_REQUIRE(text_len < 0xFFFFFFFFull, DSS_ERANGE, "more than 2^32-1 suffixes");
+        for (uint64_t i = 0; i < text_len; ++i)
+            DSS_REQUIRE(text[i] != 0, DSS_EINVAL, "the text must not contain NUL bytes");
+        dss_set_s* s = new_set(num_pes);
+        try {
+            s->chars.reserve(text_len + kArenaPad);
+            DSS_HIP(hipMemsetAsync(s->chars.p, 0, text_len + kArenaPad, s->stream));
+            if (text_len) DSS_HIP(hipMemcpyAsync(s->chars.p, text, text_len, hipMemcpyHostToDevice, s->stream));
+            make_suffix_views(s, text_len, cap);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int dss_set_create_markov_suffixes(uint64_t text_len, uint32_t repeats, uint32_t rep_min, uint32_t rep_max,
+                                   uint32_t cap, uint64_t seed, uint32_t num_pes, dss_set* out) {
+    return guarded([&] {
+        DSS_REQUIRE(out, DSS_EINVAL, "NULL argument");
+        DSS_REQUIRE(cap >= 1, DSS_EINVAL, "cap must be >= 1");
+        DSS_REQUIRE(text_len < 0xFFFFFFFFull, DSS_ERANGE, "more than 2^32-1 suffixes");
+        MarkovTable T;
+        for (int c = 0; c < 64; ++c) {
+            uint32_t acc = 0;
+            for (int k = 0; k < 4; ++k) {
+                acc += 1 + (uint32_t)(rng(seed, 7, (uint64_t)c * 4 + k) % 16);
+                T.cum[c][k] = acc;
+            }
+        }
+        dss_set_s* s = new_set(num_pes);
+        try {
+            s->chars.reserve(text_len + kArenaPad);
+            DSS_HIP(hipMemsetAsync(s->chars.p, 0, text_len + kArenaPad, s->stream));
+            if (text_len) {
+                const uint64_t blocks = (text_len + 4095) / 4096;
+                hipLaunchKernelGGL(k_markov_text, dim3((blocks + 255) / 256), dim3(256), 0, s->stream,
+                                   s->chars.as<uint8_t>(), text_len, seed, T);
+                hipLaunchKernelGGL(k_plant_repeats, dim3(1), dim3(1024), 0, s->stream, s->chars.as<uint8_t>(),
+                                   text_len, repeats, rep_min, rep_max, seed);
+            }
+            make_suffix_views(s, text_len, cap);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
 
 const char* dss_last_error(void) { return g_err.c_str(); }
 

@@ -73,6 +73,25 @@ class DeviceStringSet:
                                           C.byref(h)))
         return cls(h)
 
+    @classmethod
+    def suffixes(cls, text: np.ndarray, cap: int, num_pes: int) -> "DeviceStringSet":
+        """Capped suffixes of `text` as views (no materialised copies)."""
+        text = np.ascontiguousarray(text, dtype=np.uint8)
+        h = _lib.set_p()
+        check(lib().dss_set_create_suffixes(_p(text, _lib.u8p) if text.size else None, text.size, cap,
+                                            num_pes, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def markov_suffixes(cls, text_len: int, seed: int, num_pes: int, repeats=5000, rep_min=1000,
+                        rep_max=10000, cap=500) -> "DeviceStringSet":
+        """Capped suffixes of a seeded order-3 Markov ACGT text with planted
+        repeats, generated on the device."""
+        h = _lib.set_p()
+        check(lib().dss_set_create_markov_suffixes(text_len, repeats, rep_min, rep_max, cap, seed, num_pes,
+                                                   C.byref(h)))
+        return cls(h)
+
     def close(self):
         if self._h:
             check(lib().dss_set_destroy(self._h))

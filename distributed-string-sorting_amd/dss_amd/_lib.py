@@ -53,6 +53,9 @@ _SIGS = {
                                         C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
                                         C.POINTER(set_p)]),
     "dss_set_create_pes": (C.c_int, [u8p, u64p, C.c_uint32, C.POINTER(set_p)]),
+    "dss_set_create_suffixes": (C.c_int, [u8p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(set_p)]),
+    "dss_set_create_markov_suffixes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                 C.c_uint64, C.c_uint32, C.POINTER(set_p)]),
     "dss_set_destroy": (C.c_int, [set_p]),
     "dss_set_info": (C.c_int, [set_p, C.POINTER(SetInfo)]),
     "dss_set_pe_begin": (C.c_int, [set_p, u64p]),

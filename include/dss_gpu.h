@@ -78,6 +78,20 @@ int dss_set_create_skewed(uint64_t n, uint32_t long_len, uint32_t short_len,
 int dss_set_create_pes(const uint8_t* chars, const uint64_t* pe_byte_begin, uint32_t num_pes,
                        dss_set* out);
 
+/* Capped suffixes of a text as views into one arena (the suffix workload:
+ * SuffixGenerator, src/util/random_string_generator.hpp:89-115, each suffix
+ * capped at `cap` chars, without materialising them): string i =
+ * text[i .. i + min(cap, text_len - i)), PEs split by the byte-slice rule
+ * over the materialised strings.  The text must not contain NUL bytes. */
+int dss_set_create_suffixes(const uint8_t* text, uint64_t text_len, uint32_t cap, uint32_t num_pes,
+                            dss_set* out);
+/* Same over a seeded order-3 Markov text over {A,C,G,T} (chain restarted
+ * every 4096 chars) with `repeats` planted copies of rep_min..rep_max chars,
+ * generated on the device. */
+int dss_set_create_markov_suffixes(uint64_t text_len, uint32_t repeats, uint32_t rep_min,
+                                   uint32_t rep_max, uint32_t cap, uint64_t seed, uint32_t num_pes,
+                                   dss_set* out);
+
 int dss_set_destroy(dss_set s);
 
 typedef struct {

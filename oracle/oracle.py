@@ -134,6 +134,12 @@ def gen_skewed(params, seed):
     return out
 
 
+def gen_markov_text(text_len, seed, repeats=5, rep_min=1000, rep_max=10000):
+    text = np.empty(text_len, dtype=np.uint8)
+    lib().orc_gen_markov_text(text_len, seed, repeats, rep_min, rep_max, P(text, u8p))
+    return text
+
+
 def gen_suffixes(text_len, seed, repeats=5, rep_min=1000, rep_max=10000, cap=500):
     text = np.empty(text_len, dtype=np.uint8)
     lib().orc_gen_markov_text(text_len, seed, repeats, rep_min, rep_max, P(text, u8p))
