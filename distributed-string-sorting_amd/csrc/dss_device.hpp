@@ -130,4 +130,24 @@ __device__ inline uint64_t lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// Comparison of two strings by a whole wave: lane l compares the 8-char keys
+// at 8l of a 512-character window, a ballot finds the first difference or
+// end.  Every lane of the wave must call it with the same arguments (the
+// loop is controlled by the ballot).
+__device__ inline int wave_cmp(const uint8_t* __restrict__ chars, uint64_t ao, uint32_t al, uint64_t bo,
+                               uint32_t bl) {
+    const uint32_t lane = lane_id();
+    for (uint32_t base = 0;; base += 512) {
+        const uint32_t pos = base + 8 * lane;
+        const uint64_t a = load_key8(chars, ao, al, pos);
+        const uint64_t b = load_key8(chars, bo, bl, pos);
+        const uint64_t m = __ballot(a != b || (a & 0xFF) == 0);
+        if (m) {
+            const int f = __ffsll((unsigned long long)m) - 1;
+            const uint64_t fa = __shfl((unsigned long long)a, f), fb = __shfl((unsigned long long)b, f);
+            return fa < fb ? -1 : (fa > fb ? 1 : 0);
+        }
+    }
+}
+
 }  // namespace dss

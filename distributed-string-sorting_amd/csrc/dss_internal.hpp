@@ -159,7 +159,7 @@ struct PipeState {
     bool partitioned = false;
     std::vector<uint64_t> interval;  // p*p, [r*p+q] = strings PE r sends to PE q
     DevBuf interval_d;
-    DevBuf samples, sample_rank, sample_sorted, splitters, wprefix;
+    DevBuf samples, sample_rank, sample_sorted, splitters, wprefix, splitter_pos;
     // exchange (receive side, indexed by receive position)
     bool exchanged = false;
     int mode = 0;

@@ -202,14 +202,16 @@ __global__ void k_sample(const uint64_t* __restrict__ pe_begin, uint32_t p, cons
     counts[r] = cnt;
 }
 
-// Compacts the per-PE sample slots (single block).
+// Compacts the per-PE sample slots in PE order (one block; thread r copies
+// PE r's samples after the samples of PEs 0..r-1).
 __global__ void k_sample_compact(const Sample* __restrict__ slots, const uint32_t* __restrict__ counts, uint32_t p,
                                  uint32_t nr_max, Sample* __restrict__ out, uint32_t* __restrict__ total) {
-    if (threadIdx.x != 0) return;
-    uint32_t k = 0;
-    for (uint32_t r = 0; r < p; ++r)
-        for (uint32_t i = 0; i < counts[r]; ++i) out[k++] = slots[(uint64_t)r * nr_max + i];
-    *total = k;
+    for (uint32_t r = threadIdx.x; r < p; r += blockDim.x) {
+        uint32_t k = 0;
+        for (uint32_t q = 0; q < r; ++q) k += counts[q];
+        for (uint32_t i = 0; i < counts[r]; ++i) out[k + i] = slots[(uint64_t)r * nr_max + i];
+        if (r == p - 1) *total = k + counts[r];
+    }
 }
 
 // Rank of each sample in the global sample order (the result of the sample
@@ -280,6 +282,56 @@ __global__ void k_bucket_search(const uint64_t* __restrict__ pe_begin, uint32_t 
     }
 }
 
+// The same searches with one wave per (PE r, splitter q): every string
+// comparison is a 512-character wave comparison (one dependent load instead
+// of one per 8 characters).  pos[r * (p-1) + q] = boundary.
+__global__ void __launch_bounds__(64)
+k_bucket_search_wave(const uint64_t* __restrict__ pe_begin, uint32_t p, const uint32_t* __restrict__ perm,
+                     const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                     const uint8_t* __restrict__ chars, const Sample* __restrict__ smp,
+                     const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ ns_p, int indexed,
+                     uint64_t* __restrict__ pos) {
+    const uint32_t r = blockIdx.x / (p - 1), q = blockIdx.x % (p - 1);
+    const uint64_t b = pe_begin[r], m = pe_begin[r + 1] - b;
+    const uint32_t ns = *ns_p;
+    const uint32_t nspl = (p - 1) < ns ? (p - 1) : ns;
+    uint64_t res = m;
+    if (q < nspl) {  // block-uniform
+        const uint32_t sdist = ns / (nspl + 1);
+        const Sample sp = smp[sorted[(q + 1) * sdist]];
+        uint64_t left = 0, right = m;
+        res = ~0ull;
+        while (left != right) {
+            const uint64_t d = (right - left) / 2;
+            const uint32_t id = perm[b + left + d];
+            int c = wave_cmp(chars, off[id], len[id], sp.off, sp.tlen);
+            if (c == 0 && indexed) {
+                const uint64_t gi = b + left + d;
+                c = gi < sp.gidx ? -1 : (gi > sp.gidx ? 1 : 0);
+            }
+            if (c < 0) left = left + d + 1;
+            else if (c == 0) {
+                res = left + d;
+                break;
+            } else right = left + d;
+        }
+        if (res == ~0ull) res = left;
+    }
+    if (threadIdx.x == 0) pos[(uint64_t)r * (p - 1) + q] = res;
+}
+
+__global__ void k_intervals(const uint64_t* __restrict__ pe_begin, uint32_t p, const uint64_t* __restrict__ pos,
+                            uint64_t* __restrict__ interval) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (uint64_t)p * p) return;
+    const uint32_t r = (uint32_t)(k / p), q = (uint32_t)(k % p);
+    const uint64_t m = pe_begin[r + 1] - pe_begin[r];
+    const uint64_t* pr = pos + (uint64_t)r * (p - 1);
+    const uint64_t hi = (q + 1 < p) ? pr[q] : m;
+    const uint64_t lo = q ? pr[q - 1] : 0;
+    interval[k] = hi - lo;
+}
+
 __global__ void k_weights(const uint32_t* __restrict__ s_len, const uint32_t* __restrict__ dist, uint64_t n,
                           uint64_t* __restrict__ w) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -325,17 +377,28 @@ void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, boo
             hipLaunchKernelGGL(k_sample, dim3((p + 63) / 64), dim3(64), 0, st, s->pe_begin_d.as<uint64_t>(), p,
                                s->perm.as<uint32_t>(), s->off.as<uint64_t>(), s->len.as<uint32_t>(), dist, wprefix,
                                by_chars, factor, max_len, nr_max, slots, counts);
-            hipLaunchKernelGGL(k_sample_compact, dim3(1), dim3(64), 0, st, slots, counts, p, nr_max, compact, ns_d);
+            hipLaunchKernelGGL(k_sample_compact, dim3(1), dim3(256), 0, st, slots, counts, p, nr_max, compact, ns_d);
             uint32_t ns = 0;
             DSS_HIP(hipMemcpyAsync(&ns, ns_d, 4, hipMemcpyDeviceToHost, st));
             DSS_HIP(hipStreamSynchronize(st));
             if (ns)
                 hipLaunchKernelGGL(k_sample_rank, dim3(ns), dim3(256), 0, st, compact, ns, indexed,
                                    s->arena(), P.sample_sorted.as<uint32_t>());
-            hipLaunchKernelGGL(k_bucket_search, dim3(p), dim3(std::min<uint32_t>(std::max<uint32_t>(p, 64), 256)),
-                               (size_t)p * 8, st, s->pe_begin_d.as<uint64_t>(), p, s->perm.as<uint32_t>(),
-                               s->off.as<uint64_t>(), s->len.as<uint32_t>(), s->arena(), compact,
-                               P.sample_sorted.as<uint32_t>(), ns_d, indexed, P.interval_d.as<uint64_t>());
+            if ((uint64_t)p * (p - 1) <= (1u << 16)) {
+                P.splitter_pos.reserve((uint64_t)p * (p - 1) * 8);
+                hipLaunchKernelGGL(k_bucket_search_wave, dim3(p * (p - 1)), dim3(64), 0, st,
+                                   s->pe_begin_d.as<uint64_t>(), p, s->perm.as<uint32_t>(), s->off.as<uint64_t>(),
+                                   s->len.as<uint32_t>(), s->arena(), compact, P.sample_sorted.as<uint32_t>(), ns_d,
+                                   indexed, P.splitter_pos.as<uint64_t>());
+                hipLaunchKernelGGL(k_intervals, dim3(((uint64_t)p * p + 255) / 256), dim3(256), 0, st,
+                                   s->pe_begin_d.as<uint64_t>(), p, P.splitter_pos.as<uint64_t>(),
+                                   P.interval_d.as<uint64_t>());
+            } else {
+                hipLaunchKernelGGL(k_bucket_search, dim3(p), dim3(std::min<uint32_t>(std::max<uint32_t>(p, 64), 256)),
+                                   (size_t)p * 8, st, s->pe_begin_d.as<uint64_t>(), p, s->perm.as<uint32_t>(),
+                                   s->off.as<uint64_t>(), s->len.as<uint32_t>(), s->arena(), compact,
+                                   P.sample_sorted.as<uint32_t>(), ns_d, indexed, P.interval_d.as<uint64_t>());
+            }
         } else {
             // one PE (or no strings): everything stays on its PE
             std::vector<uint64_t> iv((uint64_t)p * p, 0);
