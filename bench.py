@@ -63,6 +63,24 @@ def roofline_from_stats(stats: dict):
     }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_dton_20m_pd_sort_traffic.md")
+
+
+def measured_traffic(timer_name):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by the counter's
+    reading for a plain copy of known size); None if not measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 6 and cells[0] == timer_name and cells[5].isdigit():
+                    return int(cells[5])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n, length, dton, pes, seed, m):
     """Times the CPU oracle's prefix-doubling sort on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -301,6 +319,12 @@ def main():
     roof = roofline_from_stats(stats)
     if roof:
         roof["launches_per_step"] = stats[roof["kernel"]]["launches"] / args.steps
+        default_workload = (args.n, args.len, args.dton, args.pes, args.seed, args.bloom_bits) == \
+            (20_000_000, 500, 0.5, 16, 0x5EED0002, 0)
+        if default_workload:
+            roof["traffic"] = measured_traffic(roof["kernel"])
+            if roof["traffic"] is not None:
+                roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
 
     ds.close()
     batched = None
