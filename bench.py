@@ -150,7 +150,12 @@ def batched_leg(args):
         used_n += s.n
     if cur:
         groups.append(cur)
-    server = Server(None, inflight=args.serve_inflight, max_batch_strings=args.serve_batch_strings)
+    # resident batches: FIFO batches up to the string budget (throughput);
+    # served: large jobs on one worker, small jobs batched on the others
+    server = Server(None, inflight=args.serve_inflight, max_batch_strings=args.serve_batch_strings,
+                    large_strings=None)
+    served = Server(None, inflight=args.serve_inflight, max_batch_strings=args.serve_batch_strings,
+                    large_strings=args.serve_large_strings)
     plain_t = {}
     gen_t = {}
     Dp = {}
@@ -199,9 +204,9 @@ def batched_leg(args):
                         args.serve_len_max, rate=lam, seed=args.serve_seed)
     serve_cap = int(args.serve_served_pool_gb * (1 << 30))
     with JobPool(serve_cap, cap_strings) as pool:
-        server.pool = pool
+        served.pool = pool
         t0 = time.perf_counter()
-        res = server.run(sspecs, realtime=True)
+        res = served.run(sspecs, realtime=True)
         wall = time.perf_counter() - t0
     Ds = {r.jid: r.D for r in res}
     if any(Ds[j] != Dp[j] for j in Dp):
@@ -230,6 +235,7 @@ def batched_leg(args):
             "pool_gib": args.serve_served_pool_gb,
             "inflight_batches": args.serve_inflight,
             "max_batch_strings": args.serve_batch_strings,
+            "large_job_strings": args.serve_large_strings,
             "gbs": round(D_total / span / 1e9, 3),
             "wall_s": round(wall, 3),
             **percentiles(lat),
@@ -264,6 +270,8 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--serve-batch-strings", type=int, default=32 << 20)
+    ap.add_argument("--serve-large-strings", type=int, default=4 << 20,
+                    help="served jobs of at least this many strings run alone on worker 0")
     ap.add_argument("--serve-pool-gb", type=float, default=100.0,
                     help="pool capacity of the plain/batched comparison (GiB of characters)")
     ap.add_argument("--serve-served-pool-gb", type=float, default=64.0,

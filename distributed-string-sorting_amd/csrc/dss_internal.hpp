@@ -188,6 +188,13 @@ void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream
 }  // namespace dss
 
 // ------------------------------------------------------------------ the set
+// Runtime parameters of the local sort (chosen per input regime).
+struct SortParams {
+    uint32_t large_min = 512;      // segments above go through the MSD passes
+    uint32_t wave_class_max = 6;   // base-case classes handled one segment per wave
+    uint32_t skip_lanes = 8;       // lanes per string in the prefix skip
+};
+
 struct dss_set_s {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -211,6 +218,7 @@ struct dss_set_s {
     dss::DevBuf s_off;  // u64[n] offset of the string at each sorted position
     dss::DevBuf s_len;  // u32[n] length of the string at each sorted position
 
+    SortParams sort_params;
     dss::SortScratch ss;
     dss::PipeState pp;
     dss::DevBuf aux;    // small per-call scratch of result queries

@@ -268,33 +268,45 @@ __global__ void k_gen_dton(uint8_t* __restrict__ out, uint64_t n, uint32_t k, ui
     // base-26 digit weights; string ids are < 2^32 < 26^7, so digits at
     // weight 26^7 and above are 'A'
     const uint32_t pow26[7] = {1u, 26u, 676u, 17576u, 456976u, 11881376u, 308915776u};
-    const uint64_t stride = slen + 1;
-    const uint64_t total = n * stride;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
-         w += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t a0 = w * 8;
-        uint64_t j = a0 / stride;
-        uint32_t pos = (uint32_t)(a0 - j * stride);
+    const uint32_t stride = slen + 1;
+    const uint64_t total = n * (uint64_t)stride;
+    // a wave covers a 4096-byte chunk, lane l its 64 bytes [64l, 64l + 64):
+    // one 64-bit division per chunk, string ids only where a string starts
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nchunks = (words + 511) / 512;
+    for (uint64_t ch = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; ch < nchunks;
+         ch += (uint64_t)gridDim.x * blockDim.x / 64) {
+        const uint64_t a0 = ch * 4096;
+        const uint64_t j0 = a0 / stride;
+        const uint32_t pos0 = (uint32_t)(a0 - j0 * stride);
+        const uint32_t rel = pos0 + 64 * lane;
+        uint64_t j = j0 + rel / stride;
+        uint32_t pos = rel % stride;
         uint32_t id = j < n ? (uint32_t)shuffle_index(j, n, seed) : 0u;
-        uint64_t word = 0;
-        for (uint32_t b = 0; b < 8; ++b) {
-            uint8_t c = 0;
-            if (a0 + b < total) {
-                if (pos == slen) c = 0;
-                else if (pos >= k) c = fill;
-                else {
-                    const uint32_t e = k - 1 - pos;  // digit index from the right
-                    c = (uint8_t)('A' + (e < 7 ? (id / pow26[e]) % 26u : 0u));
+        const uint64_t a = a0 + 64 * lane;
+        for (uint32_t wd = 0; wd < 8; ++wd) {
+            const uint64_t w = (a >> 3) + wd;
+            if (w >= words) break;
+            uint64_t word = 0;
+            for (uint32_t b = 0; b < 8; ++b) {
+                uint8_t c = 0;
+                if (a + 8 * wd + b < total) {
+                    if (pos == slen) c = 0;
+                    else if (pos >= k) c = fill;
+                    else {
+                        const uint32_t e = k - 1 - pos;  // digit index from the right
+                        c = (uint8_t)('A' + (e < 7 ? (id / pow26[e]) % 26u : 0u));
+                    }
+                }
+                word |= (uint64_t)c << (8 * b);
+                if (++pos == stride) {
+                    pos = 0;
+                    ++j;
+                    id = j < n ? (uint32_t)shuffle_index(j, n, seed) : 0u;
                 }
             }
-            word |= (uint64_t)c << (8 * b);
-            if (++pos == stride) {
-                pos = 0;
-                ++j;
-                id = j < n ? (uint32_t)shuffle_index(j, n, seed) : 0u;
-            }
+            reinterpret_cast<uint64_t*>(out)[w] = word;
         }
-        reinterpret_cast<uint64_t*>(out)[w] = word;
     }
 }
 
@@ -340,6 +352,8 @@ __global__ void k_skew_len(SkewP P, uint64_t seed, uint64_t* __restrict__ size_o
 // One wave per string.
 __global__ void k_skew_fill(SkewP P, uint64_t seed, const uint64_t* __restrict__ off,
                             uint8_t* __restrict__ out) {
+    // one wave per string; lane l writes bytes [8l, 8l + 8) of every 512-byte
+    // step, so one mix64 yields its 8 filler bytes
     const uint32_t lane = threadIdx.x & 63;
     for (uint64_t j = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; j < P.n;
          j += (uint64_t)gridDim.x * (blockDim.x / 64)) {
@@ -349,18 +363,22 @@ __global__ void k_skew_fill(SkewP P, uint64_t seed, const uint64_t* __restrict__
         skew_id_len(P, seed, i, id, L, k);
         const uint64_t h = rng(seed, 5, id);
         uint8_t* s = out + off[j];
-        for (uint32_t t = lane; t <= L; t += 64) {
-            uint8_t c;
-            if (t == L) c = 0;
-            else if (t >= k) c = (uint8_t)(1 + mix64(h + t) % 255);
-            else {
-                const uint32_t e = k - 1 - t;
-                uint64_t v = id;
-                if (e >= 9) v = 0;  // 255^8 > 2^64
-                else for (uint32_t q = 0; q < e && v; ++q) v /= 255;
-                c = (uint8_t)(1 + v % 255);
+        for (uint32_t t0 = 8 * lane; t0 <= L; t0 += 512) {
+            const uint64_t fill = mix64(h + (t0 >> 3));
+            for (uint32_t b = 0; b < 8 && t0 + b <= L; ++b) {
+                const uint32_t t = t0 + b;
+                uint8_t c;
+                if (t == L) c = 0;
+                else if (t >= k) c = (uint8_t)(1 + ((fill >> (8 * b)) & 0xFF) % 255);
+                else {
+                    const uint32_t e = k - 1 - t;
+                    uint64_t v = id;
+                    if (e >= 9) v = 0;  // 255^8 > 2^64
+                    else for (uint32_t q = 0; q < e && v; ++q) v /= 255;
+                    c = (uint8_t)(1 + v % 255);
+                }
+                s[t] = c;
             }
-            s[t] = c;
         }
     }
 }
@@ -575,7 +593,7 @@ int dss_set_create_dton(uint64_t n, uint32_t len, double dton, uint64_t seed, ui
             s->chars.reserve(s->nbytes + kArenaPad);
             DSS_HIP(hipMemsetAsync(s->chars.as<uint8_t>() + s->nbytes, 0, kArenaPad, s->stream));
             const uint64_t words = (s->nbytes + 7) / 8;
-            hipLaunchKernelGGL(k_gen_dton, dim3(stride_grid(words, 256)), dim3(256), 0, s->stream,
+            hipLaunchKernelGGL(k_gen_dton, dim3(stride_grid((words + 511) / 512, 4)), dim3(256), 0, s->stream,
                                s->chars.as<uint8_t>(), n, k, slen, fill, seed, words);
             s->off.reserve(n * sizeof(uint64_t));
             s->len.reserve(n * sizeof(uint32_t));
@@ -609,14 +627,9 @@ int dss_set_create_skewed(uint64_t n, uint32_t long_len, uint32_t short_len, dou
             s->n = n;
             s->off.reserve(n * sizeof(uint64_t));
             s->len.reserve(n * sizeof(uint32_t));
-            DevBuf tot;
-            tot.reserve(8);
             hipLaunchKernelGGL(k_skew_len, dim3((n + 255) / 256), dim3(256), 0, s->stream, P, seed,
                                s->off.as<uint64_t>(), s->len.as<uint32_t>());
-            hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s->stream, s->off.as<uint64_t>(),
-                               n, tot.as<uint64_t>());
-            DSS_HIP(hipMemcpyAsync(&s->nbytes, tot.p, 8, hipMemcpyDeviceToHost, s->stream));
-            DSS_HIP(hipStreamSynchronize(s->stream));
+            exclusive_scan_u64(s->off.as<uint64_t>(), s->off.as<uint64_t>(), n, s->stream, s->pp, &s->nbytes);
             s->chars.reserve(s->nbytes + kArenaPad);
             DSS_HIP(hipMemsetAsync(s->chars.as<uint8_t>() + s->nbytes, 0, kArenaPad, s->stream));
             hipLaunchKernelGGL(k_skew_fill, dim3(stride_grid(n, 4)), dim3(256), 0, s->stream, P, seed,
@@ -850,6 +863,7 @@ struct dss_pool_s {
     hipStream_t stream = nullptr;
     dss::DevBuf chars, off, len, tot;
     dss::RangeAlloc bytes, strings;
+    dss::PipeState scan;  // scratch of the device-wide scan
     struct Job {
         uint64_t byte0 = 0, nbytes = 0, alloc = 0, str0 = 0, n = 0;
         uint32_t max_len = 0;
@@ -1009,7 +1023,7 @@ int dss_pool_add_job_dton(dss_pool P, uint64_t n, uint32_t length, double dton, 
         const uint32_t id = pool_new_job(P, nbytes, n, b0, s0);
         P->jobs[id].max_len = slen;
         const uint64_t words = (nbytes + 7) / 8;
-        hipLaunchKernelGGL(k_gen_dton, dim3(stride_grid(words, 256)), dim3(256), 0, P->stream, P->chars.as<uint8_t>() + b0, n, k,
+        hipLaunchKernelGGL(k_gen_dton, dim3(stride_grid((words + 511) / 512, 4)), dim3(256), 0, P->stream, P->chars.as<uint8_t>() + b0, n, k,
                            slen, fill, seed, words);
         hipLaunchKernelGGL(k_offsets_base, dim3((n + 255) / 256), dim3(256), 0, P->stream, P->off.as<uint64_t>() + s0,
                            P->len.as<uint32_t>() + s0, n, slen, b0);
@@ -1040,10 +1054,8 @@ int dss_pool_add_job_skewed(dss_pool P, uint64_t n, uint32_t long_len, uint32_t 
         uint64_t* off = P->off.as<uint64_t>() + s0;
         hipLaunchKernelGGL(k_skew_len, dim3((n + 255) / 256), dim3(256), 0, P->stream, S, seed, off,
                            P->len.as<uint32_t>() + s0);
-        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, P->stream, off, n, P->tot.as<uint64_t>());
         uint64_t nbytes = 0;
-        DSS_HIP(hipMemcpyAsync(&nbytes, P->tot.p, 8, hipMemcpyDeviceToHost, P->stream));
-        DSS_HIP(hipStreamSynchronize(P->stream));
+        exclusive_scan_u64(off, off, n, P->stream, P->scan, &nbytes);
         const uint64_t asz = (nbytes + 7) / 8 * 8;
         uint64_t b0;
         if (!P->bytes.alloc(asz, b0)) {

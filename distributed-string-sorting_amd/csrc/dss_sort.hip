@@ -10,22 +10,26 @@
 //
 // Algorithm (MSD radix over 8-char big-endian keys, all PEs in one launch
 // sequence; every PE starts as one segment):
-//   large segments (> kLargeMin strings), per pass over all of them:
-//     prefix skip  -- common prefix of the segment from its depth, found by
-//                     16-lane groups comparing 128 bytes per step against the
-//                     segment's first string (early exit once the segment
-//                     minimum is below the group's position);
-//     key gather   -- 8 chars at the new depth per string (coalesced writes);
-//     histogram    -- 256 byte-digit counters per 2048-string tile in LDS;
-//     scan/classify-- one block per segment: digit offsets, sub-bucket lists,
-//                     LCP at sub-bucket boundaries;
+//   large segments (> large_min strings, a runtime parameter), per pass,
+//   over 2048-string tiles:
+//     prefix skip  -- common prefix of the segment from its depth: groups of
+//                     8 or 16 lanes compare each string with the segment's
+//                     first string, 32 bytes per lane and step, loads bounded
+//                     by the minimum found so far;
+//     key gather   -- 8 chars at the new depth per string;
+//     histogram    -- 256 byte-digit counters per tile in LDS;
+//     scan/classify-- digit offsets (two-level over 64-tile chunks for long
+//                     segments), sub-bucket lists, LCP at sub-bucket
+//                     boundaries;
 //     scatter      -- stable rank inside the tile by wave ballots, writes the
 //                     next key/id buffers, the id array and LCPs of equal
 //                     strings.  Up to 8 digit passes reuse one gathered key.
-//   small segments (2..kLargeMin strings), batched by power-of-two class into
-//     2048-slot blocks: LDS bitonic sort of (group, key) with prefix skip and
-//     iterative refinement of equal-key groups until every group is a single
-//     string or a run of equal strings.
+//   small segments (2..large_min strings), batched by power-of-two class:
+//     up to 64 strings one segment per wave (register bitonic sort), larger
+//     ones in 1024-slot LDS blocks; both sort (group, key, id) and refine
+//     equal-key groups 8 characters at a time until every group is a single
+//     string or a run of equal strings.  Ties are broken by id, so the order
+//     of equal strings is their input order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,13 +44,12 @@
 namespace dss {
 
 constexpr uint32_t kSmallMax = 1024;  // base-case capacity (slots per block)
-constexpr uint32_t kLargeMin = 512;   // segments above this go through global MSD passes
 constexpr uint32_t kClasses = 12;     // class c holds sizes (2^(c-1), 2^c], c = 1..11
 constexpr uint32_t kTile = 2048;      // strings per tile in the global passes
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kItems = kTile / kThreads;
 constexpr uint32_t kChunkTiles = 64;  // tiles per chunk of the two-level digit-offset scan
-constexpr uint32_t kWaveClassMax = 6;  // classes up to 64 strings use the one-wave base case
+constexpr uint32_t kWaveClassMax = 6;  // default: classes up to 64 strings use the one-wave base case
 
 struct Seg {
     uint32_t begin, cnt;
@@ -117,13 +120,14 @@ __global__ void k_sort_init(uint32_t* __restrict__ perm, uint32_t* __restrict__ 
     }
 }
 
-__global__ void k_sort_seed(const uint64_t* __restrict__ pe_begin, uint32_t p, Seg* __restrict__ segs,
-                            SmallSeg* __restrict__ small, SmallLists L, uint32_t* __restrict__ ctr) {
+__global__ void k_sort_seed(const uint64_t* __restrict__ pe_begin, uint32_t p, uint32_t large_min,
+                            Seg* __restrict__ segs, SmallSeg* __restrict__ small, SmallLists L,
+                            uint32_t* __restrict__ ctr) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= p) return;
     const uint32_t b = (uint32_t)pe_begin[r];
     const uint32_t cnt = (uint32_t)(pe_begin[r + 1] - pe_begin[r]);
-    if (cnt > kLargeMin) {
+    if (cnt > large_min) {
         const uint32_t k = atomicAdd(&ctr[kCtrLarge], 1u);
         segs[k] = Seg{b, cnt, 0, kNone, kNone, 0, 0};
     } else if (cnt >= 2) {
@@ -248,9 +252,9 @@ __device__ inline void seg_depths(const Seg& s, uint32_t& kd, uint32_t& dep) {
 }
 
 // ------------------------------------------------------------ prefix skip
-// Groups of 8 lanes compare one string against the segment's first string,
-// 256 bytes per step (four 8-char keys from five aligned loads per lane), so
-// a wave has 8 strings and 32 independent key loads in flight.  The tile's
+// Groups of 8 (or 16) lanes compare one string against the segment's first
+// string, 32 bytes per lane and step (four 8-char keys from five aligned
+// loads), so a wave has 8 (4) strings and 32 independent key loads in flight.  The tile's
 // offsets and lengths are staged in LDS first, so a step waits for one
 // dependent load; the reference's first-step keys stay in registers.  The
 // minimum found so far (the block's, and every 8 rounds the segment's)
@@ -258,9 +262,7 @@ __device__ inline void seg_depths(const Seg& s, uint32_t& kd, uint32_t& dep) {
 // minimum and are not loaded, and a string stops once its compared prefix
 // reaches it.  All loops that contain cross-lane operations have block- or
 // wave-uniform trip counts, so every shuffle runs with all lanes active.
-constexpr uint32_t kSkipLanes = 8;
-constexpr uint32_t kSkipGroups = kThreads / kSkipLanes;
-
+template <uint32_t kSkipLanes>
 __global__ void __launch_bounds__(kThreads)
 k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
               const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
@@ -279,6 +281,7 @@ k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint
         t_len[k] = len[id];
     }
     __syncthreads();
+    constexpr uint32_t kSkipGroups = kThreads / kSkipLanes;
     const uint32_t g = threadIdx.x / kSkipLanes, l = threadIdx.x % kSkipLanes;
     const uint32_t ref = perm[S.begin];
     const uint64_t roff = off[ref];
@@ -384,8 +387,8 @@ k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint6
 // are reserved with one global atomic per list per block.
 __global__ void __launch_bounds__(kThreads)
 k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
-                Seg* __restrict__ next, SmallSeg* __restrict__ small, SmallLists L, uint32_t* __restrict__ ctr,
-                uint32_t* __restrict__ lcp) {
+                Seg* __restrict__ next, SmallSeg* __restrict__ small, SmallLists L, uint32_t large_min,
+                uint32_t* __restrict__ ctr, uint32_t* __restrict__ lcp) {
     __shared__ uint32_t sc[256];
     __shared__ uint32_t first;
     __shared__ uint32_t lcnt[kClasses + 1], lelem[kClasses], lbase[kClasses + 1];
@@ -428,7 +431,7 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint3
     int list = -1;  // small class c, or kClasses for the large list
     uint32_t slot = 0;
     if (tot > 1 && d != 0) {
-        list = tot <= kLargeMin ? (int)(32u - __builtin_clz(tot - 1u)) : (int)kClasses;
+        list = tot <= large_min ? (int)(32u - __builtin_clz(tot - 1u)) : (int)kClasses;
         slot = atomicAdd(&lcnt[list], 1u);
         if (list < (int)kClasses) atomicAdd(&lelem[list], tot);
     }
@@ -849,7 +852,8 @@ void sort_lcp(dss_set_s* s) {
     X.keyB.reserve(n * 8);
     X.pbA.reserve(n * 4);
     X.pbB.reserve(n * 4);
-    const uint64_t seg_cap = n / (kLargeMin + 1) + s->p + 16;
+    const SortParams& T = s->sort_params;
+    const uint64_t seg_cap = n / (T.large_min + 1) + s->p + 16;
     X.segA.reserve(seg_cap * sizeof(Seg));
     X.segB.reserve(seg_cap * sizeof(Seg));
     const uint64_t tile_cap = n / kTile + seg_cap + 16;
@@ -876,7 +880,8 @@ void sort_lcp(dss_set_s* s) {
                            s->lcp.as<uint32_t>(), n);
     }
     hipLaunchKernelGGL(k_sort_seed, dim3((s->p + 255) / 256), dim3(256), 0, st,
-                       s->pe_begin_d.as<uint64_t>(), s->p, X.segA.as<Seg>(), X.small.as<SmallSeg>(), L, ctr);
+                       s->pe_begin_d.as<uint64_t>(), s->p, T.large_min, X.segA.as<Seg>(), X.small.as<SmallSeg>(), L,
+                       ctr);
     DSS_HIP(hipGetLastError());
 
     Seg* cur = X.segA.as<Seg>();
@@ -918,7 +923,10 @@ void sort_lcp(dss_set_s* s) {
         }
         {
             KScope ks(s->timer, st, "prefix_skip", 0.0);
-            hipLaunchKernelGGL(k_prefix_skip, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len);
+            if (T.skip_lanes == 16)
+                hipLaunchKernelGGL(k_prefix_skip<16>, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len);
+            else
+                hipLaunchKernelGGL(k_prefix_skip<8>, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len);
         }
         {
             KScope ks(s->timer, st, "gather", 0.0);  // bytes amended after the pass
@@ -934,7 +942,7 @@ void sort_lcp(dss_set_s* s) {
                 hipLaunchKernelGGL(k_chunk_sum, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
                                    X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
             hipLaunchKernelGGL(k_scan_classify, dim3(nlarge), dim3(kThreads), 0, st, cur, X.hist.as<uint32_t>(),
-                               X.csum.as<uint32_t>(), nxt, X.small.as<SmallSeg>(), L, ctr, lcp);
+                               X.csum.as<uint32_t>(), nxt, X.small.as<SmallSeg>(), L, T.large_min, ctr, lcp);
             if (nchunks)
                 hipLaunchKernelGGL(k_chunk_apply, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
                                    X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
@@ -990,7 +998,7 @@ void sort_lcp(dss_set_s* s) {
         DSS_REQUIRE(cnt <= L.cap[c], DSS_ERANGE, "small list overflow");
         // per string: id read, offset/length, one 8-char key, id + lcp writes
         KScope ks(s->timer, st, "small_sort", (double)cls[kClasses + c] * 32.0);
-        if (c <= kWaveClassMax) {
+        if (c <= T.wave_class_max) {
             const uint64_t waves = (cnt + (64u >> c) - 1) / (64u >> c);
             hipLaunchKernelGGL(k_small_sort_wave, dim3((waves + 3) / 4), dim3(kThreads), 0, st,
                                X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len);
@@ -1022,6 +1030,33 @@ int dss_sort_lcp(dss_set s) {
     return guarded([&] {
         DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
         sort_lcp(s);
+    });
+}
+
+int dss_set_sort_params(dss_set s, const dss_sort_params_t* params) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        SortParams T;  // NULL restores the defaults
+        if (params) {
+            DSS_REQUIRE(params->large_min >= 16 && params->large_min <= kSmallMax, DSS_EINVAL,
+                        "large_min must be in [16, 1024]");
+            DSS_REQUIRE(params->wave_class_max <= kWaveClassMax, DSS_EINVAL, "wave_class_max must be in [0, 6]");
+            DSS_REQUIRE(params->skip_lanes == 8 || params->skip_lanes == 16, DSS_EINVAL, "skip_lanes must be 8 or 16");
+            T.large_min = params->large_min;
+            T.wave_class_max = params->wave_class_max;
+            T.skip_lanes = params->skip_lanes;
+        }
+        s->sort_params = T;
+    });
+}
+
+int dss_get_sort_params(dss_set s, dss_sort_params_t* params) {
+    return guarded([&] {
+        DSS_REQUIRE(s && params, DSS_EINVAL, "NULL argument");
+        params->large_min = s->sort_params.large_min;
+        params->wave_class_max = s->sort_params.wave_class_max;
+        params->skip_lanes = s->sort_params.skip_lanes;
+        params->reserved = 0;
     });
 }
 

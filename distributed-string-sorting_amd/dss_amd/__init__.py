@@ -150,6 +150,22 @@ class DeviceStringSet:
         check(lib().dss_get_sorted(self._h, _p(perm, _lib.u32p), _p(lcp, _lib.u32p)))
         return perm[:n], lcp[:n]
 
+    def sort_params(self) -> dict:
+        t = _lib.SortParams()
+        check(lib().dss_get_sort_params(self._h, C.byref(t)))
+        return dict(large_min=t.large_min, wave_class_max=t.wave_class_max, skip_lanes=t.skip_lanes)
+
+    def set_sort_params(self, params=None):
+        """Local-sort tuning (large_min, wave_class_max, skip_lanes); None
+        restores the defaults.  Every setting gives the same output."""
+        if params is None:
+            check(lib().dss_set_sort_params(self._h, None))
+            return
+        cur = self.sort_params()
+        cur.update(params)
+        t = _lib.SortParams(cur["large_min"], cur["wave_class_max"], cur["skip_lanes"], 0)
+        check(lib().dss_set_sort_params(self._h, C.byref(t)))
+
     def set_local_order(self, perm, lcp):
         perm = np.ascontiguousarray(perm, dtype=np.uint32)
         lcp = np.ascontiguousarray(lcp, dtype=np.uint32)

@@ -28,6 +28,11 @@ class KernelStat(C.Structure):
                 ("total_ms", C.c_double), ("alg_bytes", C.c_double)]
 
 
+class SortParams(C.Structure):
+    _fields_ = [("large_min", C.c_uint32), ("wave_class_max", C.c_uint32),
+                ("skip_lanes", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class CheckResult(C.Structure):
     _fields_ = [("num_strings", C.c_uint64), ("order_errors", C.c_uint64),
                 ("lcp_errors", C.c_uint64), ("dist_prefix_size", C.c_uint64),
@@ -63,6 +68,8 @@ _SIGS = {
     "dss_sort_lcp": (C.c_int, [set_p]),
     "dss_get_sorted": (C.c_int, [set_p, u32p, u32p]),
     "dss_set_local_order": (C.c_int, [set_p, u32p, u32p]),
+    "dss_set_sort_params": (C.c_int, [set_p, C.POINTER(SortParams)]),
+    "dss_get_sort_params": (C.c_int, [set_p, C.POINTER(SortParams)]),
     "dss_set_interval": (C.c_int, [set_p, u64p]),
     "dss_set_dist": (C.c_int, [set_p, u32p]),
     "dss_avg_lcp": (C.c_int, [set_p, u64p]),

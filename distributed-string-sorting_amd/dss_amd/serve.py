@@ -102,10 +102,54 @@ class JobResult:
 
 
 class Server:
-    def __init__(self, pool: JobPool, inflight: int = 2, max_batch_strings: int = 32 << 20):
+    """`inflight` workers, each with its own batch set and stream.  Worker 0
+    takes the oldest waiting large job (at least `large_strings` strings) on
+    its own, else a batch of small jobs; the other workers only batch small
+    jobs, so small jobs never wait behind a large one in the same batch."""
+
+    def __init__(self, pool: JobPool, inflight: int = 2, max_batch_strings: int = 32 << 20,
+                 large_strings: int = 4 << 20):
         self.pool = pool
         self.inflight = inflight
         self.max_batch_strings = max_batch_strings
+        self.large_strings = large_strings
+
+    def _take(self, ready, worker):
+        """Removes and returns the next batch for `worker` from `ready`, or []
+        if it has nothing to take.  Without a large-job threshold (or with
+        one worker) batches are FIFO prefixes of the queue."""
+        large = self.large_strings
+        if large is None or self.inflight == 1:
+            take, strings = [], 0
+            while ready and (not take or (strings + ready[0][0].n <= self.max_batch_strings
+                                          and len(take) < 65536)):
+                if take and large is not None and ready[0][0].n >= large:
+                    break
+                item = ready.popleft()
+                take.append(item)
+                strings += item[0].n
+                if large is not None and item[0].n >= large:
+                    break
+            return take
+        if worker == 0:
+            for k, (s, _) in enumerate(ready):
+                if s.n >= large:
+                    item = ready[k]
+                    del ready[k]
+                    return [item]
+        take, strings, keep = [], 0, deque()
+        while ready:
+            item = ready.popleft()
+            s = item[0]
+            if s.n < large and (not take or (strings + s.n <= self.max_batch_strings and len(take) < 65536)):
+                take.append(item)
+                strings += s.n
+            else:
+                keep.append(item)
+        keep.extend(ready)
+        ready.clear()
+        ready.extend(keep)
+        return take
 
     def run(self, specs, realtime: bool = True, keep_results: bool = False,
             resident: dict | None = None) -> list[JobResult]:
@@ -162,21 +206,23 @@ class Server:
                     state["admitting"] = False
                     cv.notify_all()
 
-        def work():
+        def work(worker):
             batch = None
             try:
                 while True:
                     with cv:
-                        while not ready and state["admitting"] and state["error"] is None:
+                        while True:
+                            if state["error"] is not None:
+                                return
+                            take = self._take(ready, worker)
+                            if take:
+                                break
+                            if not state["admitting"] and not ready:
+                                return
+                            if not state["admitting"] and worker != 0 and self.large_strings is not None \
+                                    and self.inflight > 1 and all(s.n >= self.large_strings for s, _ in ready):
+                                return  # only large jobs remain: worker 0 runs them
                             cv.wait()
-                        if state["error"] is not None or (not ready and not state["admitting"]):
-                            return
-                        take = [ready.popleft()]
-                        strings = take[0][0].n
-                        while ready and strings + ready[0][0].n <= self.max_batch_strings \
-                                and len(take) < 65536:
-                            strings += ready[0][0].n
-                            take.append(ready.popleft())
                         state["inflight_jobs"] += len(take)
                     started = now()
                     ids = [job for _, job in take]
@@ -209,7 +255,7 @@ class Server:
                     batch.close()
 
         threads = [threading.Thread(target=admit, name="dss-admit")]
-        threads += [threading.Thread(target=work, name=f"dss-worker{k}") for k in range(self.inflight)]
+        threads += [threading.Thread(target=work, args=(k,), name=f"dss-worker{k}") for k in range(self.inflight)]
         for t in threads:
             t.start()
         for t in threads:

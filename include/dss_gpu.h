@@ -119,6 +119,23 @@ int dss_set_download(dss_set s, uint8_t* chars, uint64_t* offsets, uint32_t* len
 int dss_sort_lcp(dss_set s);
 /* perm/lcp: num_strings entries each (either may be NULL). */
 int dss_get_sorted(dss_set s, uint32_t* perm, uint32_t* lcp);
+
+/* Tuning of the local sort (no counterpart in the reference, whose tlx sort
+ * has fixed thresholds): segments of more than large_min strings (16..1024,
+ * default 512) go through the global MSD passes, smaller ones to the base
+ * case; base-case classes up to 2^wave_class_max strings (0..6, default 6)
+ * are sorted one segment per wave; skip_lanes (8 or 16, default 8) lanes
+ * compare each string in the common-prefix skip.  Every setting produces the
+ * same output.  NULL restores the defaults. */
+typedef struct {
+    uint32_t large_min;
+    uint32_t wave_class_max;
+    uint32_t skip_lanes;
+    uint32_t reserved;
+} dss_sort_params_t;
+
+int dss_set_sort_params(dss_set s, const dss_sort_params_t* params);
+int dss_get_sort_params(dss_set s, dss_sort_params_t* params);
 /* Sets a caller-computed local order instead of dss_sort_lcp (a StringLcpPtr
  * whose strings and LCPs the caller arranged, as the reference's exchange
  * tests do): perm permutes every PE's own string ids, lcp per position. */

@@ -176,8 +176,8 @@ ORC_API void orc_gen_skewed(const orc_skew_params* P, uint64_t seed,
             cur /= 255;
         }
         uint64_t h = orc_rng(seed, 5, id);
-        for (uint32_t t = k; t < len; ++t)
-            s[t] = (uint8_t)(1 + mix64(h + t) % 255);
+        for (uint32_t t = k; t < len; ++t) /* 8 filler bytes per mix64 */
+            s[t] = (uint8_t)(1 + ((mix64(h + (t >> 3)) >> (8 * (t & 7))) & 0xFF) % 255);
         s[len] = 0;
         pos += len + 1;
     }

@@ -64,3 +64,15 @@ def test_job_stream_jobs_do_not_depend_on_rate():
     b = job_stream(50, 1000, 10**6, rate=7.5, seed=3)
     strip = lambda s: {k: v for k, v in vars(s).items() if k != "arrival"}  # noqa: E731
     assert [strip(x) for x in a] == [strip(x) for x in b]
+
+
+def test_tuner_regime_and_cache(tmp_path):
+    from dss_amd.tune import Tuner, regime
+    info = dict(num_strings=1_000_000, num_bytes=501_000_000, num_pes=4, max_len=500)
+    assert regime(info) == "n2^17/len2^8/lcp?/p4"
+    assert regime(info, avg_lcp=249) == "n2^17/len2^8/lcp4/p4"
+    path = tmp_path / "tune.json"
+    t = Tuner(str(path))
+    t.table["x"] = {"large_min": 256, "wave_class_max": 5, "skip_lanes": 16}
+    t.save()
+    assert Tuner(str(path)).table == t.table

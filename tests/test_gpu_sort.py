@@ -90,3 +90,56 @@ def test_device_generators_match_oracle():
         chars, off, ln = ds.download()
     exp = O.gen_skewed(params, 42)
     assert chars.tobytes() == exp.tobytes()
+
+
+PARAM_SETS = [
+    {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8},
+    {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16},
+    {"large_min": 256, "wave_class_max": 5, "skip_lanes": 8},
+    {"large_min": 1024, "wave_class_max": 6, "skip_lanes": 16},
+]
+
+
+@pytest.mark.parametrize("name", ["dton", "skewed", "suffix"])
+def test_sort_params_do_not_change_the_output(name):
+    """Every tuning of the local sort returns the same permutation and LCPs
+    (ties by id), equal to the oracle's order and LCPs."""
+    dss = _dss()
+    chars = {"dton": lambda: O.gen_dton(30000, 80, 0.5, 31),
+             "skewed": lambda: O.gen_skewed(O.skew_params(30000, long_len=200, short_len=20), 32),
+             "suffix": lambda: O.gen_suffixes(30000, 33, repeats=10, rep_min=200, rep_max=2000, cap=300)}[name]()
+    ss = O.StringSet(chars, 4)
+    operm, olcp = O.local_sort(ss)
+    with dss.DeviceStringSet.from_arena(chars, 4) as ds:
+        ds.sort_lcp()
+        ref = ds.get_sorted()
+        for params in PARAM_SETS:
+            ds.set_sort_params(params)
+            assert ds.sort_params() == params
+            ds.sort_lcp()
+            perm, lcp = ds.get_sorted()
+            assert (perm == ref[0]).all() and (lcp == ref[1]).all(), params
+        ds.set_sort_params(None)
+        assert ds.sort_params() == {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8}
+    assert (ref[1] == olcp).all()
+    assert (canonical_ties(ref[0], ref[1], ss.len, ss.pe_begin) == canonical_ties(operm, olcp, ss.len, ss.pe_begin)).all()
+
+
+def test_autotuner_picks_and_applies_a_setting(tmp_path):
+    dss = _dss()
+    from dss_amd.tune import Tuner
+    tuner = Tuner(str(tmp_path / "t.json"))
+    with dss.DeviceStringSet.dton(200000, 100, 0.5, 34, 4) as ds:
+        out = tuner.autotune(ds, repeats=1)
+        assert out["best"] in [c for _, c in out["times_ms"]]
+        assert ds.sort_params() == out["best"]
+        tuner.save()
+    with dss.DeviceStringSet.dton(200000, 100, 0.5, 35, 4) as ds:
+        ds.sort_lcp()
+        assert Tuner(str(tmp_path / "t.json")).apply(ds)
+        assert ds.sort_params() == out["best"]
+    with dss.DeviceStringSet.from_arena(arena([b"a", b"b"]), 1) as ds:
+        with pytest.raises(dss.DssError):
+            ds.set_sort_params({"large_min": 2000})
+        with pytest.raises(dss.DssError):
+            ds.set_sort_params({"skip_lanes": 12})
