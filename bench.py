@@ -152,10 +152,11 @@ def batched_leg(args):
         groups.append(cur)
     # resident batches: FIFO batches up to the string budget (throughput);
     # served: large jobs on one worker, small jobs batched on the others
+    reserve = max(args.serve_batch_strings, max(s.n for s in specs))
     server = Server(None, inflight=args.serve_inflight, max_batch_strings=args.serve_batch_strings,
-                    large_strings=None)
+                    large_strings=None, reserve_strings=reserve)
     served = Server(None, inflight=args.serve_inflight, max_batch_strings=args.serve_batch_strings,
-                    large_strings=args.serve_large_strings)
+                    large_strings=args.serve_large_strings or None, reserve_strings=reserve)
     plain_t = {}
     gen_t = {}
     Dp = {}
@@ -164,10 +165,11 @@ def batched_leg(args):
     batch_sizes = []
     with JobPool(cap_bytes, cap_strings) as pool:
         server.pool = pool
-        # warm the batch buffers at the largest batch size
+        # first launch of every kernel (code-object loading) before any timing
         w = job_stream(1, 1000, 1000, rate=0.0, seed=1)[0]
         j = w.add_to(pool)
         run_plain(pool, [w], resident={w.jid: j})
+        server.prepare(j)
         pool.free(j)
         for g in groups:
             ids = {}
@@ -189,6 +191,7 @@ def batched_leg(args):
                 pool.free(jid)
         log(f"batched leg: {len(specs)} jobs in {len(groups)} groups, plain {sum(plain_t.values()):.3f} s, "
             f"batched {batched_s:.3f} s, generation {sum(gen_t.values()):.3f} s")
+        server.close()
     mism = [j for j in Dp if Dp[j] != Db[j]]
     if mism:
         log(f"batched D differs from plain for jobs {mism[:10]}")
@@ -205,9 +208,14 @@ def batched_leg(args):
     serve_cap = int(args.serve_served_pool_gb * (1 << 30))
     with JobPool(serve_cap, cap_strings) as pool:
         served.pool = pool
+        w = job_stream(1, 1000, 1000, rate=0.0, seed=1)[0]
+        j = w.add_to(pool)
+        served.prepare(j)
+        pool.free(j)
         t0 = time.perf_counter()
         res = served.run(sspecs, realtime=True)
         wall = time.perf_counter() - t0
+        served.close()
     Ds = {r.jid: r.D for r in res}
     if any(Ds[j] != Dp[j] for j in Dp):
         log("served D differs from plain")
@@ -271,10 +279,10 @@ def main():
     ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--serve-batch-strings", type=int, default=32 << 20)
     ap.add_argument("--serve-large-strings", type=int, default=4 << 20,
-                    help="served jobs of at least this many strings run alone on worker 0")
+                    help="served jobs of at least this many strings run alone on worker 0 (0: FIFO batches)")
     ap.add_argument("--serve-pool-gb", type=float, default=100.0,
                     help="pool capacity of the plain/batched comparison (GiB of characters)")
-    ap.add_argument("--serve-served-pool-gb", type=float, default=64.0,
+    ap.add_argument("--serve-served-pool-gb", type=float, default=160.0,
                     help="pool capacity of the served run (the admission limit)")
     args = ap.parse_args()
 

@@ -72,6 +72,25 @@ struct DevBuf {
         }
         bytes = n;
     }
+    // Grows to at least n bytes keeping the current contents (copied on st).
+    void reserve_keep(size_t n, hipStream_t st) {
+        if (n <= bytes) return;
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, n);
+        if (e != hipSuccess)
+            throw Error{DSS_ENOMEM, "hipMalloc(" + std::to_string(n) + " B) failed: " + hipGetErrorString(e)};
+        if (p && bytes) {
+            e = hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                (void)hipFree(q);
+                throw Error{DSS_EHIP, std::string("reserve_keep copy: ") + hipGetErrorString(e)};
+            }
+        }
+        release();
+        p = q;
+        bytes = n;
+    }
     template <class T>
     T* as() const { return static_cast<T*>(p); }
 };

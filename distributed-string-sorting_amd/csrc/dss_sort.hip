@@ -837,40 +837,65 @@ __global__ void k_sorted_meta(const uint32_t* __restrict__ perm, const uint64_t*
 }
 
 // ------------------------------------------------------------------ driver
+// Capacities of the sort's work lists for n strings over p PEs.
+struct SortCaps {
+    uint64_t seg_cap, tile_cap, chunk_cap;
+    SmallLists L;
+    uint64_t small_total;
+};
+
+static SortCaps sort_caps(uint64_t n, uint32_t p, const SortParams& T) {
+    SortCaps c;
+    c.seg_cap = n / (T.large_min + 1) + p + 16;
+    c.tile_cap = n / kTile + c.seg_cap + 16;
+    c.chunk_cap = c.tile_cap / kChunkTiles + 16;
+    uint64_t tot = 0;
+    for (uint32_t k = 0; k < kClasses; ++k) {
+        const uint64_t cap = k == 0 ? 0 : n / ((1ull << (k - 1)) + 1) + 1;
+        c.L.off[k] = tot;
+        c.L.cap[k] = cap;
+        tot += cap;
+    }
+    c.small_total = tot;
+    return c;
+}
+
+// Grows every buffer the local sort of n strings over p PEs uses.
+static SortCaps reserve_sort(dss_set_s* s, uint64_t n, uint32_t p) {
+    const uint64_t nn = std::max<uint64_t>(n, 1);
+    s->perm.reserve(nn * 4);
+    s->lcp.reserve(nn * 4);
+    s->s_off.reserve(nn * 8);
+    s->s_len.reserve(nn * 4);
+    SortScratch& X = s->ss;
+    X.keyA.reserve(nn * 8);
+    X.keyB.reserve(nn * 8);
+    X.pbA.reserve(nn * 4);
+    X.pbB.reserve(nn * 4);
+    const SortCaps c = sort_caps(n, p, s->sort_params);
+    X.segA.reserve(c.seg_cap * sizeof(Seg));
+    X.segB.reserve(c.seg_cap * sizeof(Seg));
+    X.tiles.reserve(c.tile_cap * sizeof(Tile));
+    X.hist.reserve(c.tile_cap * 256 * 4);
+    X.csum.reserve(c.chunk_cap * 256 * 4);
+    X.small.reserve(std::max<uint64_t>(c.small_total, 1) * sizeof(SmallSeg));
+    X.counters.reserve(64 * 4);
+    return c;
+}
+
 void sort_lcp(dss_set_s* s) {
     const uint64_t n = s->n;
     const hipStream_t st = s->stream;
-    s->perm.reserve(std::max<uint64_t>(n, 1) * 4);
-    s->lcp.reserve(std::max<uint64_t>(n, 1) * 4);
     s->sorted = false;
+    const SortCaps caps = reserve_sort(s, n, s->p);
     if (n == 0) {
         s->sorted = true;
         return;
     }
     SortScratch& X = s->ss;
-    X.keyA.reserve(n * 8);
-    X.keyB.reserve(n * 8);
-    X.pbA.reserve(n * 4);
-    X.pbB.reserve(n * 4);
     const SortParams& T = s->sort_params;
-    const uint64_t seg_cap = n / (T.large_min + 1) + s->p + 16;
-    X.segA.reserve(seg_cap * sizeof(Seg));
-    X.segB.reserve(seg_cap * sizeof(Seg));
-    const uint64_t tile_cap = n / kTile + seg_cap + 16;
-    X.tiles.reserve(tile_cap * sizeof(Tile));
-    X.hist.reserve(tile_cap * 256 * 4);
-    const uint64_t chunk_cap = tile_cap / kChunkTiles + 16;
-    X.csum.reserve(chunk_cap * 256 * 4);
-    SmallLists L;
-    uint64_t tot = 0;
-    for (uint32_t c = 0; c < kClasses; ++c) {
-        const uint64_t cap = c == 0 ? 0 : n / ((1ull << (c - 1)) + 1) + 1;
-        L.off[c] = tot;
-        L.cap[c] = cap;
-        tot += cap;
-    }
-    X.small.reserve(tot * sizeof(SmallSeg));
-    X.counters.reserve(64 * 4);
+    const uint64_t seg_cap = caps.seg_cap, tile_cap = caps.tile_cap, chunk_cap = caps.chunk_cap;
+    const SmallLists L = caps.L;
     uint32_t* ctr = X.counters.as<uint32_t>();
     DSS_HIP(hipMemsetAsync(ctr, 0, 64 * 4, st));
 
@@ -1011,8 +1036,6 @@ void sort_lcp(dss_set_s* s) {
     }
     {
         KScope ks(s->timer, st, "sort_meta", n * 28.0);
-        s->s_off.reserve(n * 8);
-        s->s_len.reserve(n * 4);
         hipLaunchKernelGGL(k_sorted_meta, dim3((n + 255) / 256), dim3(256), 0, st, perm, off, len, n,
                            s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>());
     }
@@ -1030,6 +1053,26 @@ int dss_sort_lcp(dss_set s) {
     return guarded([&] {
         DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
         sort_lcp(s);
+    });
+}
+
+int dss_set_reserve(dss_set s, uint64_t num_strings, uint32_t num_pes) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        DSS_REQUIRE(num_strings < 0xFFFFFFFFull && num_pes >= 1 && num_pes <= (1u << 16), DSS_ERANGE,
+                    "num_strings must be < 2^32-1 and num_pes in [1, 65536]");
+        const uint64_t nn = std::max<uint64_t>(num_strings, 1);
+        DSS_HIP(hipStreamSynchronize(s->stream));
+        reserve_sort(s, num_strings, num_pes);
+        s->pe_begin_d.reserve_keep((uint64_t)(num_pes + 1) * 8, s->stream);
+        s->aux.reserve((uint64_t)num_pes * 8 + (num_strings / 65536 + num_pes) * 16 + 8);
+        if (s->borrowed) {  // a batch set: its offset/length copies too
+            s->off.reserve_keep(nn * 8, s->stream);
+            s->len.reserve_keep(nn * 4, s->stream);
+        }
+        // grown result arrays hold no results: the set must be sorted again
+        s->sorted = false;
+        s->pp.has_dist = s->pp.partitioned = s->pp.exchanged = s->pp.merged = s->pp.extended = false;
     });
 }
 

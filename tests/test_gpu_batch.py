@@ -164,3 +164,33 @@ def test_pool_errors():
             rc = _lib.lib().dss_pool_rebatch(pool._h, ds._h, ids.ctypes.data_as(_lib.u32p), 1)
             assert rc == -1
         pool.free(a)
+
+
+def test_reserved_batches_give_the_same_results():
+    """Growing a batch set's buffers (dss_set_reserve) keeps its jobs'
+    metadata; served runs on pre-grown sets match the plain results."""
+    dss = _dss()
+    from dss_amd.serve import Server, job_stream, run_plain
+    specs = job_stream(30, 300, 20000, 10, 200, rate=0.0, seed=41)
+    with dss.JobPool(128 << 20, 2 << 20) as pool:
+        plain = run_plain(pool, specs, keep_results=True)
+        j = specs[0].add_to(pool)
+        b = pool.batch([j])
+        b.set.reserve(1 << 20, 64)  # grows off/len after they were filled
+        b.sort()
+        perm, lcp = b.result(0)
+        D = int(b.dist_sizes()[0])
+        b.close()
+        pool.free(j)
+        assert D == plain[specs[0].jid][0]
+        assert (perm == plain[specs[0].jid][3][0]).all() and (lcp == plain[specs[0].jid][3][1]).all()
+        with Server(pool, inflight=2, max_batch_strings=50000, reserve_strings=1 << 20) as srv:
+            w = specs[1].add_to(pool)
+            srv.prepare(w)
+            pool.free(w)
+            for _ in range(2):  # the batch sets persist across runs
+                res = srv.run(specs, realtime=False, keep_results=True)
+                for r in res:
+                    D, _, _, (pp, pl) = plain[r.jid]
+                    assert r.D == D and (r.perm == pp).all() and (r.lcp == pl).all()
+        assert pool.usage()["bytes_used"] == 0
