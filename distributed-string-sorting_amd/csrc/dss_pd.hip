@@ -323,10 +323,80 @@ __global__ void k_pd_insert(const uint32_t* __restrict__ list, uint64_t nh, cons
     slot[c] = (uint32_t)s;
 }
 
+// ------------------------------------------------ duplicate prefilter
+// Two bitmaps over 2^bits positions of mix64(hash) (about 16 bits per hashed
+// candidate, small enough to stay in the last-level cache): `seen` and
+// `again`.  A hash whose bit was set twice may occur more than once and goes
+// to the exact open-addressing table (k_pd_insert); every other hash is
+// unique.  Equal hashes always share a bit, so no duplicate is missed.
+__device__ inline uint64_t pd_bit(uint64_t h, uint32_t bits) { return mix64(h) >> (64 - bits); }
+
+__global__ void __launch_bounds__(256)
+k_pd_bits(const uint32_t* __restrict__ list, uint64_t nh, const uint64_t* __restrict__ hashv, uint32_t bits,
+          uint32_t* __restrict__ seen, uint32_t* __restrict__ again) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = pd_bit(hashv[list[i]], bits);
+        const uint32_t b = 1u << (x & 31);
+        if (atomicOr(&seen[x >> 5], b) & b) atomicOr(&again[x >> 5], b);
+    }
+}
+
+// Unique hashes get dup = 0; the others are appended to `maybe` (collected
+// in LDS over kMarkIters iterations, one global atomic per flush).
+__global__ void __launch_bounds__(256)
+k_pd_maybe(const uint32_t* __restrict__ list, uint64_t nh, const uint64_t* __restrict__ hashv, uint32_t bits,
+           const uint32_t* __restrict__ again, uint8_t* __restrict__ dup, uint32_t* __restrict__ maybe,
+           unsigned long long* __restrict__ nmaybe) {
+    __shared__ uint32_t sl[256 * kMarkIters];
+    __shared__ uint32_t scount, base;
+    const uint32_t lane = lane_id();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (threadIdx.x == 0) scount = 0;
+    __syncthreads();
+    uint32_t it = 0;
+    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < nh; b0 += stride) {
+        const uint64_t i = b0 + threadIdx.x;
+        bool m = false;
+        uint32_t c = 0;
+        if (i < nh) {
+            c = list[i];
+            const uint64_t x = pd_bit(hashv[c], bits);
+            m = (again[x >> 5] >> (x & 31)) & 1;
+            if (!m) dup[c] = 0;
+        }
+        const uint64_t bal = __ballot(m);
+        uint32_t wb = 0;
+        if (lane == 0 && bal) wb = atomicAdd(&scount, (uint32_t)__popcll(bal));
+        wb = __shfl(wb, 0);
+        if (m) sl[wb + (uint32_t)__popcll(bal & lanemask_lt())] = c;
+        if (++it == kMarkIters || b0 + stride >= nh) {  // block-uniform
+            it = 0;
+            __syncthreads();
+            const uint32_t cnt = scount;
+            if (threadIdx.x == 0 && cnt) base = (uint32_t)atomicAdd(nmaybe, (unsigned long long)cnt);
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) maybe[base + k] = sl[k];
+            __syncthreads();
+            if (threadIdx.x == 0) scount = 0;
+            __syncthreads();
+        }
+    }
+}
+
+// Exact result for the prefiltered hashes: their table slot's flag.
+__global__ void k_pd_flag(const uint32_t* __restrict__ maybe, uint64_t nm, const uint32_t* __restrict__ slot,
+                          const uint8_t* __restrict__ tdup, uint8_t* __restrict__ dup) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nm) {
+        const uint32_t c = maybe[i];
+        dup[c] = tdup[slot[c]];
+    }
+}
+
 __global__ void __launch_bounds__(256)
 k_pd_next(const uint8_t* __restrict__ cand, const uint8_t* __restrict__ flags, const uint64_t* __restrict__ pe_begin,
           uint32_t p, const uint32_t* __restrict__ lcp, uint32_t depth, uint64_t n, const uint8_t* __restrict__ dup,
-          const uint32_t* __restrict__ slot, uint8_t* __restrict__ next, unsigned long long* __restrict__ ncand) {
+          uint8_t* __restrict__ next, unsigned long long* __restrict__ ncand) {
     uint32_t kept = 0;
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t keep = 0;
@@ -337,7 +407,7 @@ k_pd_next(const uint8_t* __restrict__ cand, const uint8_t* __restrict__ flags, c
             // LCP-local root: the next position (same PE, as lcp is 0 at PE
             // starts) is a local LCP duplicate
             if (c + 1 < n && lcp[c + 1] >= depth && cand[c + 1]) keep = 1;
-            else keep = dup[slot[c]];
+            else keep = dup[c];
         }
         next[c] = (uint8_t)keep;
         kept += keep;
@@ -362,6 +432,8 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     P.pd_hash.reserve(nn * 8);
     P.pd_slot.reserve(nn * 4);
     P.pd_list.reserve(nn * 4);
+    P.pd_maybe.reserve(nn * 4);
+    P.pd_dup.reserve(nn);
     P.pd_ctr.reserve(64);
     uint8_t* cand = P.pd_cand.as<uint8_t>();
     uint8_t* next = P.pd_next.as<uint8_t>();
@@ -374,7 +446,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     while (ncand > 0) {
         DSS_REQUIRE(depth < 0xFFFFFFFFull, DSS_ERANGE, "prefix doubling depth overflow");
         DSS_REQUIRE(2 * ncand < (1ull << 32), DSS_ERANGE, "too many candidates for the hash table");
-        DSS_HIP(hipMemsetAsync(ctr, 0, 16, st));
+        DSS_HIP(hipMemsetAsync(ctr, 0, 24, st));
         {
             // every position: candidate flags of c and c-1, length, lcp, the
             // flag written; every candidate: rounds and dist
@@ -395,21 +467,45 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
         DSS_HIP(hipStreamSynchronize(st));
         // every hashed candidate: list entry, offset, depth chars, hash
         s->timer.amend_last("pd_hash", (double)nh * (depth + 20.0));
-        uint64_t T = 1024;
-        while (T < 2 * nh) T <<= 1;
-        P.pd_table.reserve(T * 8);
-        P.pd_tflag.reserve(T);
         {
-            KScope ks(s->timer, st, "pd_dedup", (double)T * 9.0 + (double)n * 3.0 + (double)nh * 40.0);
-            hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T, kEmpty);
-            DSS_HIP(hipMemsetAsync(P.pd_tflag.p, 0, T, st));
-            if (nh)
-                hipLaunchKernelGGL(k_pd_insert, dim3((nh + 255) / 256), blk, 0, st, P.pd_list.as<uint32_t>(), nh,
-                                   P.pd_hash.as<uint64_t>(), P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(),
-                                   T - 1, P.pd_slot.as<uint32_t>());
-            hipLaunchKernelGGL(k_pd_next, dim3(std::min<uint64_t>((n + 255) / 256, kPdGrid)), blk, 0, st, cand, P.pd_flags.as<uint8_t>(), s->pe_begin_d.as<uint64_t>(),
-                               p, s->lcp.as<uint32_t>(), (uint32_t)depth, n, P.pd_tflag.as<uint8_t>(),
-                               P.pd_slot.as<uint32_t>(), next, ctr + 1);
+            // hashed candidates: prefilter bitmaps (two atomics, one read),
+            // exact table only for hashes whose bit was set twice; every
+            // position: candidate flags, lcp, next flag
+            KScope ks(s->timer, st, "pd_dedup", (double)nh * 24.0 + (double)n * 7.0);
+            if (nh) {
+                uint32_t bits = 16;
+                while (bits < 32 && (1ull << bits) < 16 * nh) ++bits;
+                const uint64_t words = (1ull << bits) / 32;
+                P.pd_bits.reserve(words * 8);
+                uint32_t* seen = P.pd_bits.as<uint32_t>();
+                uint32_t* again = seen + words;
+                DSS_HIP(hipMemsetAsync(seen, 0, words * 8, st));
+                const dim3 g(std::min<uint64_t>((nh + 255) / 256, kPdGrid));
+                hipLaunchKernelGGL(k_pd_bits, g, blk, 0, st, P.pd_list.as<uint32_t>(), nh, P.pd_hash.as<uint64_t>(),
+                                   bits, seen, again);
+                hipLaunchKernelGGL(k_pd_maybe, g, blk, 0, st, P.pd_list.as<uint32_t>(), nh, P.pd_hash.as<uint64_t>(),
+                                   bits, again, P.pd_dup.as<uint8_t>(), P.pd_maybe.as<uint32_t>(), ctr + 2);
+                uint64_t nm = 0;
+                DSS_HIP(hipMemcpyAsync(&nm, ctr + 2, 8, hipMemcpyDeviceToHost, st));
+                DSS_HIP(hipStreamSynchronize(st));
+                if (nm) {
+                    uint64_t T = 1024;
+                    while (T < 2 * nm) T <<= 1;
+                    P.pd_table.reserve(T * 8);
+                    P.pd_tflag.reserve(T);
+                    hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T,
+                                       kEmpty);
+                    DSS_HIP(hipMemsetAsync(P.pd_tflag.p, 0, T, st));
+                    hipLaunchKernelGGL(k_pd_insert, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
+                                       P.pd_hash.as<uint64_t>(), P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(),
+                                       T - 1, P.pd_slot.as<uint32_t>());
+                    hipLaunchKernelGGL(k_pd_flag, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
+                                       P.pd_slot.as<uint32_t>(), P.pd_tflag.as<uint8_t>(), P.pd_dup.as<uint8_t>());
+                }
+            }
+            hipLaunchKernelGGL(k_pd_next, dim3(std::min<uint64_t>((n + 255) / 256, kPdGrid)), blk, 0, st, cand,
+                               P.pd_flags.as<uint8_t>(), s->pe_begin_d.as<uint64_t>(), p, s->lcp.as<uint32_t>(),
+                               (uint32_t)depth, n, P.pd_dup.as<uint8_t>(), next, ctr + 1);
         }
         DSS_HIP(hipGetLastError());
         DSS_HIP(hipMemcpyAsync(&ncand, ctr + 1, 8, hipMemcpyDeviceToHost, st));
