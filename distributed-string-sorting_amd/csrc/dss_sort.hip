@@ -112,11 +112,16 @@ __global__ void k_pe_dsum(const uint64_t* __restrict__ chunks, const uint32_t* _
     if (threadIdx.x == 0) atomicAdd(d + pe, part[0] + part[1] + part[2] + part[3]);
 }
 
-__global__ void k_sort_init(uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, uint64_t n) {
+// Identity order, zero LCPs, and "no metadata yet" (kNone) in the sorted
+// length array: base cases fill offset/length of the strings they place,
+// k_sorted_meta the rest.
+__global__ void k_sort_init(uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, uint32_t* __restrict__ out_len,
+                            uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         perm[i] = (uint32_t)i;
         lcp[i] = 0;
+        out_len[i] = kNone;
     }
 }
 
@@ -546,7 +551,8 @@ constexpr uint32_t kPer = kCap / kThreads;  // elements per thread
 __global__ void __launch_bounds__(kThreads)
 k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2,
              uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
-             const uint64_t* __restrict__ off, const uint32_t* __restrict__ len) {
+             const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+             uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len) {
     __shared__ uint64_t K[kCap];
     __shared__ uint32_t G[kCap];
     __shared__ uint32_t I[kCap];
@@ -700,7 +706,10 @@ k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2
         if (s0 + slot >= nseg) continue;
         const SmallSeg sg = segs[s0 + slot];
         if (j >= sg.cnt) continue;
-        perm[sg.begin + j] = I[e];
+        const uint32_t id = I[e];
+        perm[sg.begin + j] = id;
+        out_off[sg.begin + j] = off[id];
+        out_len[sg.begin + j] = len[id];
         if (j) lcp[sg.begin + j] = LO[e];
     }
 }
@@ -720,7 +729,8 @@ k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2
 __global__ void __launch_bounds__(kThreads)
 k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2,
                   uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
-                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ len) {
+                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                  uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len) {
     __shared__ uint64_t s_off[kThreads];
     __shared__ uint32_t s_len[kThreads], s_id[kThreads];
     const uint32_t lane = lane_id();
@@ -819,17 +829,21 @@ k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls
     }
     if (valid) {
         perm[sbegin + j] = s_id[wbase + src];
+        out_off[sbegin + j] = s_off[wbase + src];
+        out_len[sbegin + j] = s_len[wbase + src];
         if (j) lcp[sbegin + j] = lo;
     }
 }
 
 // Offset and length per sorted position: every later stage streams these
 // instead of gathering through the permutation.
+// With only_missing, positions the base cases already filled (s_len not
+// kNone) are skipped.
 __global__ void k_sorted_meta(const uint32_t* __restrict__ perm, const uint64_t* __restrict__ off,
                               const uint32_t* __restrict__ len, uint64_t n, uint64_t* __restrict__ s_off,
-                              uint32_t* __restrict__ s_len) {
+                              uint32_t* __restrict__ s_len, int only_missing) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+    if (i < n && (!only_missing || s_len[i] == kNone)) {
         const uint32_t id = perm[i];
         s_off[i] = off[id];
         s_len[i] = len[id];
@@ -902,7 +916,7 @@ void sort_lcp(dss_set_s* s) {
     {
         KScope ks(s->timer, st, "sort_init", n * 8.0);
         hipLaunchKernelGGL(k_sort_init, dim3((n + 255) / 256), dim3(256), 0, st, s->perm.as<uint32_t>(),
-                           s->lcp.as<uint32_t>(), n);
+                           s->lcp.as<uint32_t>(), s->s_len.as<uint32_t>(), n);
     }
     hipLaunchKernelGGL(k_sort_seed, dim3((s->p + 255) / 256), dim3(256), 0, st,
                        s->pe_begin_d.as<uint64_t>(), s->p, T.large_min, X.segA.as<Seg>(), X.small.as<SmallSeg>(), L,
@@ -1026,18 +1040,22 @@ void sort_lcp(dss_set_s* s) {
         if (c <= T.wave_class_max) {
             const uint64_t waves = (cnt + (64u >> c) - 1) / (64u >> c);
             hipLaunchKernelGGL(k_small_sort_wave, dim3((waves + 3) / 4), dim3(kThreads), 0, st,
-                               X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len);
+                               X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len,
+                               s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>());
         } else {
             const uint32_t spb = kCap >> c;
             const uint32_t blocks = (cnt + spb - 1) / spb;
             hipLaunchKernelGGL(k_small_sort, dim3(blocks), dim3(kThreads), 0, st,
-                               X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len);
+                               X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len,
+                               s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>());
         }
     }
     {
-        KScope ks(s->timer, st, "sort_meta", n * 28.0);
+        // the sorted-length check per position; positions the base cases did
+        // not place (normally few) add an id/offset/length gather
+        KScope ks(s->timer, st, "sort_meta", n * 4.0);
         hipLaunchKernelGGL(k_sorted_meta, dim3((n + 255) / 256), dim3(256), 0, st, perm, off, len, n,
-                           s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>());
+                           s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(), 1);
     }
     DSS_HIP(hipGetLastError());
     s->sorted = true;
@@ -1180,7 +1198,7 @@ extern "C" int dss_set_local_order(dss_set s, const uint32_t* perm, const uint32
             DSS_HIP(hipMemcpyAsync(s->lcp.p, lcp, n * 4, hipMemcpyHostToDevice, s->stream));
             hipLaunchKernelGGL(k_sorted_meta, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->perm.as<uint32_t>(),
                                s->off.as<uint64_t>(), s->len.as<uint32_t>(), n, s->s_off.as<uint64_t>(),
-                               s->s_len.as<uint32_t>());
+                               s->s_len.as<uint32_t>(), 0);
             DSS_HIP(hipGetLastError());
             DSS_HIP(hipStreamSynchronize(s->stream));
         }
