@@ -291,6 +291,176 @@ k_pd_hash(const uint32_t* __restrict__ list, const unsigned long long* __restric
     }
 }
 
+// ------------------------------------------------ shared-prefix hashing
+// When most positions are hashed and share long prefixes with their
+// predecessors, the XXH64 state after the first L = S-1 full 32-byte stripes
+// (S = full stripes of the depth, B = 32L characters) is the same along a
+// run of positions whose LCP with the predecessor is at least B.  The first
+// position of every run (its head: LCP below B, every PE start included)
+// computes that state once; the other hashed candidates of the run continue
+// from it with their own last stripe and tail.  The digest is unchanged.
+// A scan over positions gives every position the rank of its run head.
+constexpr uint32_t kHsT = 256, kHsI = 8, kHsB = kHsT * kHsI;
+
+__global__ void __launch_bounds__(kHsT)
+k_head_count(const uint32_t* __restrict__ lcp, uint64_t n, uint32_t B, uint32_t* __restrict__ bcnt) {
+    __shared__ uint32_t red[kHsT / 64];
+    const uint64_t b = (uint64_t)blockIdx.x * kHsB + (uint64_t)threadIdx.x * kHsI;
+    uint32_t cnt = 0;
+    for (uint32_t k = 0; k < kHsI; ++k)
+        if (b + k < n) cnt += lcp[b + k] < B;
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o);
+    if (lane_id() == 0) red[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Exclusive sum-scan of the block counts in place (one block); the total to
+// *total.
+__global__ void __launch_bounds__(1024) k_head_sums(uint32_t* __restrict__ v, uint64_t nb,
+                                                    unsigned long long* __restrict__ total) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint64_t per = (nb + 1023) / 1024;
+    const uint64_t b = t * per, e = (b + per < nb) ? b + per : nb;
+    uint32_t sum = 0;
+    for (uint64_t i = b; i < e; ++i) sum += v[i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        const uint32_t x = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (uint64_t i = b; i < e; ++i) {
+        const uint32_t x = v[i];
+        v[i] = run;
+        run += x;
+    }
+    if (t == 1023) *total = part[1023];
+}
+
+// Per position: rank of its run head (heads before or at it, minus one);
+// heads record their position under their rank.
+__global__ void __launch_bounds__(kHsT)
+k_head_rank(const uint32_t* __restrict__ lcp, uint64_t n, uint32_t B, const uint32_t* __restrict__ bcnt,
+            uint32_t* __restrict__ hrank, uint32_t* __restrict__ hpos) {
+    __shared__ uint32_t part[kHsT];
+    const uint64_t b = (uint64_t)blockIdx.x * kHsB + (uint64_t)threadIdx.x * kHsI;
+    uint32_t f[kHsI];
+    uint32_t cnt = 0;
+    for (uint32_t k = 0; k < kHsI; ++k) {
+        f[k] = b + k < n && lcp[b + k] < B;
+        cnt += f[k];
+    }
+    part[threadIdx.x] = cnt;
+    __syncthreads();
+    for (uint32_t o = 1; o < kHsT; o <<= 1) {
+        const uint32_t x = threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = bcnt[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0u);
+    for (uint32_t k = 0; k < kHsI; ++k) {
+        if (b + k >= n) break;
+        if (f[k]) hpos[run++] = (uint32_t)(b + k);
+        hrank[b + k] = run - 1;
+    }
+}
+
+// XXH64 lanes v1..v4 after the first L stripes of every run head.  A head
+// shorter than 32L characters heads a run of one (its successor shares
+// fewer characters), so its state is never used and not computed.
+__global__ void k_head_state(const uint32_t* __restrict__ hpos, uint64_t nheads, const uint64_t* __restrict__ s_off,
+                             const uint32_t* __restrict__ s_len, const uint8_t* __restrict__ chars, uint32_t L,
+                             uint64_t* __restrict__ state) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nheads * 4) return;
+    const uint64_t r = t >> 2;
+    const uint32_t j = (uint32_t)(t & 3);
+    if (s_len[hpos[r]] < 32 * L) {
+        state[t] = 0;
+        return;
+    }
+    const uint64_t o = s_off[hpos[r]];
+    uint64_t v = j == 0 ? P1 + P2 : (j == 1 ? P2 : (j == 2 ? 0ull : (uint64_t)0 - P1));
+    for (uint32_t k = 0; k < L; ++k) v = xround(v, load_word_le(chars, o + 32 * k + 8 * j));
+    state[t] = v;
+}
+
+// XXH64 of the listed candidates, four lanes per string: a run head hashes
+// its own bytes from the start, any other position continues from its head's
+// state after L stripes.
+__global__ void __launch_bounds__(256)
+k_pd_hash_shared(const uint32_t* __restrict__ list, const unsigned long long* __restrict__ nhashed,
+                 const uint64_t* __restrict__ s_off, const uint32_t* __restrict__ hrank,
+                 const uint32_t* __restrict__ hpos, const uint64_t* __restrict__ state,
+                 const uint8_t* __restrict__ chars, uint32_t depth, uint32_t L, uint64_t m,
+                 uint64_t* __restrict__ hashv) {
+    const uint32_t j = threadIdx.x & 3, g = threadIdx.x >> 2;
+    const uint32_t base_lane = threadIdx.x & ~3u;
+    const uint32_t stripes = depth / 32;
+    const uint64_t nh = *nhashed;
+    for (uint64_t blk = (uint64_t)blockIdx.x * 64; blk < nh; blk += (uint64_t)gridDim.x * 64) {
+        const uint64_t i = blk + g;
+        const bool hashed = i < nh;
+        const uint32_t c = hashed ? list[i] : 0u;
+        const uint64_t own = hashed ? s_off[c] : 0ull;
+        uint64_t v = j == 0 ? P1 + P2 : (j == 1 ? P2 : (j == 2 ? 0ull : (uint64_t)0 - P1));
+        uint32_t k0 = 0;
+        if (hashed) {
+            const uint32_t r = hrank[c];
+            if (hpos[r] != c) {
+                v = state[4 * (uint64_t)r + j];
+                k0 = L;
+            }
+            for (uint32_t k = k0; k < stripes; ++k) v = xround(v, load_word_le(chars, own + 32 * k + 8 * j));
+        }
+        const uint64_t v1 = __shfl(v, base_lane + 0);
+        const uint64_t v2 = __shfl(v, base_lane + 1);
+        const uint64_t v3 = __shfl(v, base_lane + 2);
+        const uint64_t v4 = __shfl(v, base_lane + 3);
+        if (hashed && j == 0) {
+            uint64_t h;
+            if (depth > 32) {
+                h = rotl64(v1, 1) + rotl64(v2, 7) + rotl64(v3, 12) + rotl64(v4, 18);
+                h = xmerge(h, v1);
+                h = xmerge(h, v2);
+                h = xmerge(h, v3);
+                h = xmerge(h, v4);
+            } else {
+                h = v3 + P5;
+            }
+            h += depth;
+            uint32_t q = stripes * 32;
+            while (q + 8 <= depth) {
+                h ^= xround(0, load_word_le(chars, own + q));
+                h = rotl64(h, 27) * P1 + P4;
+                q += 8;
+            }
+            if (q + 4 <= depth) {
+                h ^= (load_word_le(chars, own + q) & 0xFFFFFFFFull) * P1;
+                h = rotl64(h, 23) * P2 + P3;
+                q += 4;
+            }
+            while (q < depth) {
+                h ^= (uint64_t)chars[own + q] * P5;
+                h = rotl64(h, 11) * P1;
+                ++q;
+            }
+            h ^= h >> 33;
+            h *= P2;
+            h ^= h >> 29;
+            h *= P3;
+            h ^= h >> 32;
+            hashv[c] = mod_range(h, m);
+        }
+    }
+}
+
 __global__ void k_fill_u64(uint64_t* __restrict__ a, uint64_t n, uint64_t v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -446,7 +616,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     while (ncand > 0) {
         DSS_REQUIRE(depth < 0xFFFFFFFFull, DSS_ERANGE, "prefix doubling depth overflow");
         DSS_REQUIRE(2 * ncand < (1ull << 32), DSS_ERANGE, "too many candidates for the hash table");
-        DSS_HIP(hipMemsetAsync(ctr, 0, 24, st));
+        DSS_HIP(hipMemsetAsync(ctr, 0, 32, st));
         {
             // every position: candidate flags of c and c-1, length, lcp, the
             // flag written; every candidate: rounds and dist
@@ -456,53 +626,96 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
                                P.pd_flags.as<uint8_t>(), P.rounds.as<uint8_t>(), P.dist.as<uint32_t>(),
                                P.pd_list.as<uint32_t>(), ctr);
         }
-        {
-            KScope ks(s->timer, st, "pd_hash", 0.0);
-            hipLaunchKernelGGL(k_pd_hash, dim3(std::min<uint64_t>((ncand + kPdPos - 1) / kPdPos, kPdGrid)), blk, 0,
-                               st, P.pd_list.as<uint32_t>(), ctr, s->s_off.as<uint64_t>(), s->arena(),
-                               (uint32_t)depth, m, P.pd_hash.as<uint64_t>());
-        }
         uint64_t nh = 0;
         DSS_HIP(hipMemcpyAsync(&nh, ctr, 8, hipMemcpyDeviceToHost, st));
         DSS_HIP(hipStreamSynchronize(st));
-        // every hashed candidate: list entry, offset, depth chars, hash
-        s->timer.amend_last("pd_hash", (double)nh * (depth + 20.0));
+        // shared-prefix hashing: at least 2 full stripes, at least half of the
+        // positions hashed, and at most one run head per 4 hashed candidates
+        const uint32_t stripes = (uint32_t)(depth / 32);
+        const uint32_t L = stripes >= 2 ? stripes - 1 : 0;
+        const uint32_t B = 32 * L;
+        bool shared = L >= 1 && nh * 2 >= n;
+        uint64_t nheads = 0;
+        const uint64_t nbh = (n + kHsB - 1) / kHsB;
+        if (shared) {
+            KScope ks(s->timer, st, "pd_heads", (double)n * 4.0);
+            P.pd_hmax.reserve((nbh + 1) * 4);
+            hipLaunchKernelGGL(k_head_count, dim3(nbh), dim3(kHsT), 0, st, s->lcp.as<uint32_t>(), n, B,
+                               P.pd_hmax.as<uint32_t>());
+            hipLaunchKernelGGL(k_head_sums, dim3(1), dim3(1024), 0, st, P.pd_hmax.as<uint32_t>(), nbh, ctr + 3);
+            DSS_HIP(hipMemcpyAsync(&nheads, ctr + 3, 8, hipMemcpyDeviceToHost, st));
+            DSS_HIP(hipStreamSynchronize(st));
+            shared = nheads * 4 <= nh;
+        }
+        if (shared) {
+            // ranks: lcp read, rank written, head positions; states: L stripes per head
+            KScope ks(s->timer, st, "pd_heads", (double)n * 8.0 + (double)nheads * (B + 36.0));
+            P.pd_head.reserve(nn * 4);
+            P.pd_hpos.reserve(std::max<uint64_t>(nheads, 1) * 4);
+            P.pd_hstate.reserve(std::max<uint64_t>(nheads, 1) * 32);
+            hipLaunchKernelGGL(k_head_rank, dim3(nbh), dim3(kHsT), 0, st, s->lcp.as<uint32_t>(), n, B,
+                               P.pd_hmax.as<uint32_t>(), P.pd_head.as<uint32_t>(), P.pd_hpos.as<uint32_t>());
+            hipLaunchKernelGGL(k_head_state, dim3((nheads * 4 + 255) / 256), blk, 0, st, P.pd_hpos.as<uint32_t>(),
+                               nheads, s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(), s->arena(), L,
+                               P.pd_hstate.as<uint64_t>());
+        }
         {
-            // hashed candidates: prefilter bitmaps (two atomics, one read),
-            // exact table only for hashes whose bit was set twice; every
-            // position: candidate flags, lcp, next flag
-            KScope ks(s->timer, st, "pd_dedup", (double)nh * 24.0 + (double)n * 7.0);
-            if (nh) {
-                uint32_t bits = 16;
-                while (bits < 32 && (1ull << bits) < 16 * nh) ++bits;
-                const uint64_t words = (1ull << bits) / 32;
-                P.pd_bits.reserve(words * 8);
-                uint32_t* seen = P.pd_bits.as<uint32_t>();
-                uint32_t* again = seen + words;
-                DSS_HIP(hipMemsetAsync(seen, 0, words * 8, st));
-                const dim3 g(std::min<uint64_t>((nh + 255) / 256, kPdGrid));
+            KScope ks(s->timer, st, "pd_hash", 0.0);
+            const dim3 g(std::min<uint64_t>((nh + kPdPos - 1) / kPdPos, kPdGrid));
+            if (shared)
+                hipLaunchKernelGGL(k_pd_hash_shared, g, blk, 0, st, P.pd_list.as<uint32_t>(), ctr,
+                                   s->s_off.as<uint64_t>(), P.pd_head.as<uint32_t>(), P.pd_hpos.as<uint32_t>(),
+                                   P.pd_hstate.as<uint64_t>(), s->arena(), (uint32_t)depth, L, m,
+                                   P.pd_hash.as<uint64_t>());
+            else if (nh)
+                hipLaunchKernelGGL(k_pd_hash, g, blk, 0, st, P.pd_list.as<uint32_t>(), ctr, s->s_off.as<uint64_t>(),
+                                   s->arena(), (uint32_t)depth, m, P.pd_hash.as<uint64_t>());
+        }
+        // every hashed candidate: list entry, offset, the characters read
+        // from its own string (all of them, or beyond B), hash
+        s->timer.amend_last("pd_hash", (double)nh * ((shared ? depth - B : depth) + 24.0));
+        if (nh) {
+            uint32_t bits = 16;
+            while (bits < 32 && (1ull << bits) < 16 * nh) ++bits;
+            const uint64_t words = (1ull << bits) / 32;
+            P.pd_bits.reserve(words * 8);
+            uint32_t* seen = P.pd_bits.as<uint32_t>();
+            uint32_t* again = seen + words;
+            DSS_HIP(hipMemsetAsync(seen, 0, words * 8, st));
+            const dim3 g(std::min<uint64_t>((nh + 255) / 256, kPdGrid));
+            {
+                // hashed candidates: list entry, hash, two bitmap atomics
+                // (seen, rarely again), one bitmap read, the flag
+                KScope ks(s->timer, st, "pd_prefilter", (double)nh * 25.0);
                 hipLaunchKernelGGL(k_pd_bits, g, blk, 0, st, P.pd_list.as<uint32_t>(), nh, P.pd_hash.as<uint64_t>(),
                                    bits, seen, again);
-                hipLaunchKernelGGL(k_pd_maybe, g, blk, 0, st, P.pd_list.as<uint32_t>(), nh, P.pd_hash.as<uint64_t>(),
-                                   bits, again, P.pd_dup.as<uint8_t>(), P.pd_maybe.as<uint32_t>(), ctr + 2);
-                uint64_t nm = 0;
-                DSS_HIP(hipMemcpyAsync(&nm, ctr + 2, 8, hipMemcpyDeviceToHost, st));
-                DSS_HIP(hipStreamSynchronize(st));
-                if (nm) {
-                    uint64_t T = 1024;
-                    while (T < 2 * nm) T <<= 1;
-                    P.pd_table.reserve(T * 8);
-                    P.pd_tflag.reserve(T);
-                    hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T,
-                                       kEmpty);
-                    DSS_HIP(hipMemsetAsync(P.pd_tflag.p, 0, T, st));
-                    hipLaunchKernelGGL(k_pd_insert, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
-                                       P.pd_hash.as<uint64_t>(), P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(),
-                                       T - 1, P.pd_slot.as<uint32_t>());
-                    hipLaunchKernelGGL(k_pd_flag, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
-                                       P.pd_slot.as<uint32_t>(), P.pd_tflag.as<uint8_t>(), P.pd_dup.as<uint8_t>());
-                }
+                hipLaunchKernelGGL(k_pd_maybe, g, blk, 0, st, P.pd_list.as<uint32_t>(), nh,
+                                   P.pd_hash.as<uint64_t>(), bits, again, P.pd_dup.as<uint8_t>(),
+                                   P.pd_maybe.as<uint32_t>(), ctr + 2);
             }
+            uint64_t nm = 0;
+            DSS_HIP(hipMemcpyAsync(&nm, ctr + 2, 8, hipMemcpyDeviceToHost, st));
+            DSS_HIP(hipStreamSynchronize(st));
+            if (nm) {
+                uint64_t T = 1024;
+                while (T < 2 * nm) T <<= 1;
+                P.pd_table.reserve(T * 8);
+                P.pd_tflag.reserve(T);
+                // table fill, per possible repeat: hash, slot, flags
+                KScope ks(s->timer, st, "pd_table", (double)T * 9.0 + (double)nm * 30.0);
+                hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T,
+                                   kEmpty);
+                DSS_HIP(hipMemsetAsync(P.pd_tflag.p, 0, T, st));
+                hipLaunchKernelGGL(k_pd_insert, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
+                                   P.pd_hash.as<uint64_t>(), P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(),
+                                   T - 1, P.pd_slot.as<uint32_t>());
+                hipLaunchKernelGGL(k_pd_flag, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
+                                   P.pd_slot.as<uint32_t>(), P.pd_tflag.as<uint8_t>(), P.pd_dup.as<uint8_t>());
+            }
+        }
+        {
+            // every position: flag, candidate flags, lcp, duplicate flag, next
+            KScope ks(s->timer, st, "pd_next", (double)n * 8.0);
             hipLaunchKernelGGL(k_pd_next, dim3(std::min<uint64_t>((n + 255) / 256, kPdGrid)), blk, 0, st, cand,
                                P.pd_flags.as<uint8_t>(), s->pe_begin_d.as<uint64_t>(), p, s->lcp.as<uint32_t>(),
                                (uint32_t)depth, n, P.pd_dup.as<uint8_t>(), next, ctr + 1);

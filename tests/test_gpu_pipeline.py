@@ -226,3 +226,23 @@ def test_few_strings_many_pes():
         exp = b"".join(o["arena"].tobytes() for o in res["out"])
         got, _ = ds.get_extended(len(exp))
         assert got.tobytes() == exp
+
+
+@pytest.mark.parametrize("m", [2**64 - 1, 1 << 20, 4093])
+def test_prefix_doubling_shared_prefix_rounds(m):
+    """D/N 0.5 over 500 chars: in the depth-264 round every string is hashed
+    while sharing ~246 leading characters with its predecessor, so its first
+    224 characters are read from the head of its run.  With small hash ranges
+    the candidate sets depend on every hash value, so any difference from the
+    oracle's hashes of the strings' own bytes shows up in the results."""
+    dss = _dss()
+    chars = O.gen_dton(20000, 500, 0.5, 0x5EED0007)
+    ss = O.StringSet(chars, 4)
+    operm, olcp = O.local_sort(ss)
+    ores, orounds, onr = O.prefix_doubling(ss, operm, olcp, m=m)
+    assert onr >= 6  # the depth-264 round ran
+    with dss.DeviceStringSet.from_arena(chars, 4) as ds:
+        ds.sort_lcp()
+        nr = ds.prefix_doubling(m=m)
+        gres, grounds = ds.get_prefix_doubling()
+    assert nr == onr and (gres == ores).all() and (grounds == orounds).all()
