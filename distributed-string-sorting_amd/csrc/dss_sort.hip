@@ -73,10 +73,11 @@ struct SmallLists {
     uint64_t cap[kClasses];
 };
 
-// counters: [0] next large segments, [1] tiles, [2] tile chunks, [3 + c]
-// small class c, [3 + kClasses + c] strings in small class c
+// counters: [3 + c] small class c, [3 + kClasses + c] strings in small
+// class c; two sets of per-pass counters at kCtrPass + 4 * (pass & 1):
+// [+0] large segments of the list, [+1] tiles, [+2] tile chunks
 constexpr uint32_t kCtrLarge = 0, kCtrTiles = 1, kCtrChunks = 2, kCtrSmall = 3,
-                   kCtrSmallElems = 3 + kClasses;
+                   kCtrSmallElems = 3 + kClasses, kCtrPass = 32;
 
 __device__ inline void push_small(SmallSeg* small, const SmallLists& L, uint32_t* ctr,
                                   uint32_t begin, uint32_t cnt, uint32_t depth) {
@@ -133,7 +134,7 @@ __global__ void k_sort_seed(const uint64_t* __restrict__ pe_begin, uint32_t p, u
     const uint32_t b = (uint32_t)pe_begin[r];
     const uint32_t cnt = (uint32_t)(pe_begin[r + 1] - pe_begin[r]);
     if (cnt > large_min) {
-        const uint32_t k = atomicAdd(&ctr[kCtrLarge], 1u);
+        const uint32_t k = atomicAdd(&ctr[kCtrPass + kCtrLarge], 1u);
         segs[k] = Seg{b, cnt, 0, kNone, kNone, 0, 0};
     } else if (cnt >= 2) {
         push_small(small, L, ctr, b, cnt, 0);
@@ -150,7 +151,8 @@ __device__ inline uint32_t seg_chunks(uint32_t nt) {
     return nt > kChunkTiles ? (nt + kChunkTiles - 1) / kChunkTiles : 0u;
 }
 
-__global__ void k_build_tiles(Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restrict__ ctr) {
+__global__ void k_build_tiles(Seg* __restrict__ segs, uint32_t* __restrict__ pc) {
+    const uint32_t nseg = pc[kCtrLarge];  // written by the previous pass (or the seed)
     __shared__ uint32_t part[1024], cpart[1024];
     __shared__ uint32_t carry, ccarry;
     const uint32_t t = threadIdx.x;
@@ -186,8 +188,8 @@ __global__ void k_build_tiles(Seg* __restrict__ segs, uint32_t nseg, uint32_t* _
         __syncthreads();
     }
     if (t == 0) {
-        ctr[kCtrTiles] = carry;
-        ctr[kCtrChunks] = ccarry;
+        pc[kCtrTiles] = carry;
+        pc[kCtrChunks] = ccarry;
     }
 }
 
@@ -393,7 +395,7 @@ k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint6
 __global__ void __launch_bounds__(kThreads)
 k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
                 Seg* __restrict__ next, SmallSeg* __restrict__ small, SmallLists L, uint32_t large_min,
-                uint32_t* __restrict__ ctr, uint32_t* __restrict__ lcp) {
+                uint32_t* __restrict__ ctr, uint32_t* __restrict__ next_large, uint32_t* __restrict__ lcp) {
     __shared__ uint32_t sc[256];
     __shared__ uint32_t first;
     __shared__ uint32_t lcnt[kClasses + 1], lelem[kClasses], lbase[kClasses + 1];
@@ -445,7 +447,7 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint3
         lbase[d] = atomicAdd(&ctr[kCtrSmall + d], lcnt[d]);
         atomicAdd(&ctr[kCtrSmallElems + d], lelem[d]);
     }
-    if (d == kClasses && lcnt[kClasses]) lbase[kClasses] = atomicAdd(&ctr[kCtrLarge], lcnt[kClasses]);
+    if (d == kClasses && lcnt[kClasses]) lbase[kClasses] = atomicAdd(next_large, lcnt[kClasses]);
     __syncthreads();
     if (list < 0) return;
     if (list < (int)kClasses) {
@@ -927,28 +929,34 @@ void sort_lcp(dss_set_s* s) {
     Seg* nxt = X.segB.as<Seg>();
     uint64_t* kc = X.keyA.as<uint64_t>();
     uint64_t* kn = X.keyB.as<uint64_t>();
-    uint32_t* pc = X.pbA.as<uint32_t>();
-    uint32_t* pn = X.pbB.as<uint32_t>();
-    uint32_t h_ctr[3];
-    DSS_HIP(hipMemcpyAsync(h_ctr, ctr, 4, hipMemcpyDeviceToHost, st));
+    uint32_t* pc_ids = X.pbA.as<uint32_t>();
+    uint32_t* pn_ids = X.pbB.as<uint32_t>();
+    // every pass ends with one read-back of all counters: the next list's
+    // segment count and its tile/chunk counts (k_build_tiles of the next pass
+    // runs before the read-back), and the small-class counts
+    uint32_t h_ctr[64];
+    uint32_t par = 0;  // counter set of the current pass
+    hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, X.segA.as<Seg>(), ctr + kCtrPass);
+    DSS_HIP(hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, st));
     DSS_HIP(hipStreamSynchronize(st));
-    uint32_t nlarge = h_ctr[0];
     const bool trace = getenv("DSS_SORT_TRACE") != nullptr;
     const uint8_t* chars = s->arena();
     const uint64_t* off = s->off.as<uint64_t>();
     const uint32_t* len = s->len.as<uint32_t>();
     uint32_t* perm = s->perm.as<uint32_t>();
     uint32_t* lcp = s->lcp.as<uint32_t>();
-    while (nlarge) {
+    for (;;) {
+        uint32_t* pc = ctr + kCtrPass + 4 * par;   // this pass
+        uint32_t* pn = ctr + kCtrPass + 4 * (par ^ 1);  // the next pass
+        const uint32_t* hp = h_ctr + kCtrPass + 4 * par;
+        const uint32_t nlarge = hp[kCtrLarge];
+        if (!nlarge) break;
+        const uint32_t nt = hp[kCtrTiles];
+        const uint32_t nchunks = hp[kCtrChunks];
         DSS_REQUIRE(nlarge <= seg_cap, DSS_ERANGE, "segment list overflow");
-        DSS_HIP(hipMemsetAsync(ctr, 0, 12, st));
-        hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, cur, nlarge, ctr);
-        DSS_HIP(hipMemcpyAsync(h_ctr, ctr, 12, hipMemcpyDeviceToHost, st));
-        DSS_HIP(hipStreamSynchronize(st));
-        const uint32_t nt = h_ctr[kCtrTiles];
-        const uint32_t nchunks = h_ctr[kCtrChunks];
         DSS_REQUIRE(nt <= tile_cap, DSS_ERANGE, "tile list overflow");
         DSS_REQUIRE(nchunks <= chunk_cap, DSS_ERANGE, "chunk list overflow");
+        DSS_HIP(hipMemsetAsync(pn, 0, 16, st));
         hipLaunchKernelGGL(k_fill_tiles, dim3((nt + 255) / 256), dim3(256), 0, st, cur, nlarge, nt,
                            X.tiles.as<Tile>());
         const Tile* tl = X.tiles.as<Tile>();
@@ -969,7 +977,7 @@ void sort_lcp(dss_set_s* s) {
         }
         {
             KScope ks(s->timer, st, "gather", 0.0);  // bytes amended after the pass
-            hipLaunchKernelGGL(k_gather, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc, pc);
+            hipLaunchKernelGGL(k_gather, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc, pc_ids);
         }
         {
             KScope ks(s->timer, st, "hist", (double)cnt_large * 8.0);
@@ -981,18 +989,20 @@ void sort_lcp(dss_set_s* s) {
                 hipLaunchKernelGGL(k_chunk_sum, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
                                    X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
             hipLaunchKernelGGL(k_scan_classify, dim3(nlarge), dim3(kThreads), 0, st, cur, X.hist.as<uint32_t>(),
-                               X.csum.as<uint32_t>(), nxt, X.small.as<SmallSeg>(), L, T.large_min, ctr, lcp);
+                               X.csum.as<uint32_t>(), nxt, X.small.as<SmallSeg>(), L, T.large_min, ctr,
+                               pn + kCtrLarge, lcp);
             if (nchunks)
                 hipLaunchKernelGGL(k_chunk_apply, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
                                    X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
         }
         {
             KScope ks(s->timer, st, "scatter", (double)cnt_large * 28.0);
-            hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kThreads), 0, st, tl, cur, X.hist.as<uint32_t>(), kc, pc,
-                               kn, pn, perm, lcp);
+            hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kThreads), 0, st, tl, cur, X.hist.as<uint32_t>(), kc,
+                               pc_ids, kn, pn_ids, perm, lcp);
         }
+        hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, nxt, pn);
         DSS_HIP(hipGetLastError());
-        DSS_HIP(hipMemcpyAsync(h_ctr, ctr, 4, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, st));
         DSS_HIP(hipStreamSynchronize(st));
         if (s->timer.enabled) {
             // prefix skip: characters up to the common prefix + 1 of every
@@ -1012,19 +1022,18 @@ void sort_lcp(dss_set_s* s) {
         if (trace) {
             std::vector<Seg> hs(std::min<uint32_t>(nlarge, 4));
             DSS_HIP(hipMemcpy(hs.data(), cur, hs.size() * sizeof(Seg), hipMemcpyDeviceToHost));
-            fprintf(stderr, "[dss sort] pass nlarge=%u ntiles=%u next=%u", nlarge, nt, h_ctr[0]);
+            fprintf(stderr, "[dss sort] pass nlarge=%u ntiles=%u next=%u", nlarge, nt,
+                    h_ctr[kCtrPass + 4 * (par ^ 1) + kCtrLarge]);
             for (auto& q : hs)
                 fprintf(stderr, " {b=%u c=%u d=%u kd=%d cp=%d}", q.begin, q.cnt, q.depth, (int)q.kdepth, (int)q.cp);
             fprintf(stderr, "\n");
         }
-        nlarge = h_ctr[0];
+        par ^= 1;
         std::swap(cur, nxt);
         std::swap(kc, kn);
-        std::swap(pc, pn);
+        std::swap(pc_ids, pn_ids);
     }
-    uint32_t cls[2 * kClasses];
-    DSS_HIP(hipMemcpyAsync(cls, ctr + kCtrSmall, sizeof(cls), hipMemcpyDeviceToHost, st));
-    DSS_HIP(hipStreamSynchronize(st));
+    const uint32_t* cls = h_ctr + kCtrSmall;  // small-class segment and string counts
     if (trace) {
         fprintf(stderr, "[dss sort] small classes:");
         for (uint32_t c = 1; c < kClasses; ++c)
