@@ -213,6 +213,7 @@ struct SortParams {
     uint32_t large_min = 512;      // segments above go through the MSD passes
     uint32_t wave_class_max = 6;   // base-case classes handled one segment per wave
     uint32_t skip_lanes = 8;       // lanes per string in the prefix skip
+    uint32_t skip_strings = 4;     // strings per lane group and round in the prefix skip
 };
 
 struct dss_set_s {

@@ -64,8 +64,12 @@ struct Tile {
     uint32_t seg, begin, end;
 };
 
+// A base-case segment; `cache` says where the 8-char keys of its strings
+// already are: 0 none, else bits 0-2 the segment depth minus the keys' depth
+// (1..7: the keys' last 8 - that characters are the segment's next ones) and
+// bit 3 the key buffer (0 A, 1 B) the last MSD pass scattered them into.
 struct SmallSeg {
-    uint32_t begin, cnt, depth;
+    uint32_t begin, cnt, depth, cache;
 };
 
 struct SmallLists {
@@ -84,7 +88,7 @@ __device__ inline void push_small(SmallSeg* small, const SmallLists& L, uint32_t
     const uint32_t c = 32u - __builtin_clz(cnt - 1u);  // ceil(log2 cnt), cnt >= 2
     const uint32_t k = atomicAdd(&ctr[kCtrSmall + c], 1u);
     atomicAdd(&ctr[kCtrSmallElems + c], cnt);
-    small[L.off[c] + k] = SmallSeg{begin, cnt, depth};
+    small[L.off[c] + k] = SmallSeg{begin, cnt, depth, 0};
 }
 
 // ------------------------------------------------------------------ init
@@ -259,17 +263,19 @@ __device__ inline void seg_depths(const Seg& s, uint32_t& kd, uint32_t& dep) {
 }
 
 // ------------------------------------------------------------ prefix skip
-// Groups of 8 (or 16) lanes compare one string against the segment's first
-// string, 32 bytes per lane and step (four 8-char keys from five aligned
-// loads), so a wave has 8 (4) strings and 32 independent key loads in flight.  The tile's
-// offsets and lengths are staged in LDS first, so a step waits for one
-// dependent load; the reference's first-step keys stay in registers.  The
-// minimum found so far (the block's, and every 8 rounds the segment's)
-// bounds each further comparison: keys at or beyond it cannot lower the
-// minimum and are not loaded, and a string stops once its compared prefix
-// reaches it.  All loops that contain cross-lane operations have block- or
-// wave-uniform trip counts, so every shuffle runs with all lanes active.
-template <uint32_t kSkipLanes>
+// Groups of 8 (or 16) lanes compare kU strings at a time against the
+// segment's first string, 32 bytes per lane and step (four 8-char keys from
+// five aligned loads), so a wave keeps 8·kU (4·kU) strings and 32·kU
+// independent key loads in flight; the reference's keys of a step are
+// loaded once for the kU strings (its first-step keys stay in registers).
+// The tile's offsets and lengths are staged in LDS first, so a step waits
+// for one dependent load.  The minimum found so far (the block's, and every
+// 8 rounds the segment's) bounds each further comparison: keys at or beyond
+// it cannot lower the minimum and are not loaded, and a string stops once its
+// compared prefix reaches it.  All loops that contain cross-lane operations
+// have block- or wave-uniform trip counts, so every shuffle runs with all
+// lanes active.
+template <uint32_t kSkipLanes, uint32_t kU>
 __global__ void __launch_bounds__(kThreads)
 k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
               const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
@@ -289,6 +295,7 @@ k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint
     }
     __syncthreads();
     constexpr uint32_t kSkipGroups = kThreads / kSkipLanes;
+    constexpr uint32_t kPerRound = kSkipGroups * kU;
     const uint32_t g = threadIdx.x / kSkipLanes, l = threadIdx.x % kSkipLanes;
     const uint32_t ref = perm[S.begin];
     const uint64_t roff = off[ref];
@@ -299,56 +306,73 @@ k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint
     const volatile uint32_t* cpp = &segs[T.seg].cp;
     const volatile uint32_t* vbmin = &bmin;
     uint32_t glim = kNone;
-    const uint32_t rounds = (cnt + kSkipGroups - 1) / kSkipGroups;
+    const uint32_t rounds = (cnt + kPerRound - 1) / kPerRound;
     for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t k = r * kSkipGroups + g;
-        bool done = k >= cnt;
-        uint64_t so = 0;
-        uint32_t sl = 0;
-        if (!done) {
-            so = t_off[k];
-            sl = t_len[k];
+        uint64_t so[kU];
+        uint32_t sl[kU], res[kU];
+        bool done[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint32_t k = (r * kU + u) * kSkipGroups + g;
+            done[u] = k >= cnt;
+            so[u] = 0;
+            sl[u] = 0;
+            res[u] = kNone;
+            if (!done[u]) {
+                so[u] = t_off[k];
+                sl[u] = t_len[k];
+            }
         }
         if ((r & 7) == 0) glim = *cpp;
         uint32_t lim = min(glim, *vbmin);
-        uint32_t base = d0;
-        uint32_t res = kNone;
-        while (__ballot(!done) != 0) {  // wave-uniform condition
-            uint32_t m = kNone;
-            if (!done) {
-                const uint32_t p0 = base + 32 * l;
-                if (lim == kNone || p0 - d0 < lim) {
-                    uint64_t a[4], b[4];
-                    load_keys4(chars, so, sl, p0, a);
-                    if (base == d0) {
+        uint32_t base = d0;  // the kU strings of a group step together
+        for (;;) {
+            bool any = false;
 #pragma unroll
-                        for (int j = 0; j < 4; ++j) b[j] = rk[j];
-                    } else {
-                        load_keys4(chars, roff, rlen, p0, b);
-                    }
+            for (uint32_t u = 0; u < kU; ++u) any |= !done[u];
+            if (__ballot(any) == 0) break;  // wave-uniform condition
+            const uint32_t p0 = base + 32 * l;
+            const bool inwin = lim == kNone || p0 - d0 < lim;
+            uint64_t a[kU][4], b[4];
+#pragma unroll
+            for (uint32_t u = 0; u < kU; ++u)
+                if (!done[u] && inwin) load_keys4(chars, so[u], sl[u], p0, a[u]);
+            if (base == d0) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) b[j] = rk[j];
+            } else if (any && inwin) {
+                load_keys4(chars, roff, rlen, p0, b);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kU; ++u) {
+                uint32_t m = kNone;
+                if (!done[u] && inwin) {
 #pragma unroll
                     for (int j = 3; j >= 0; --j) {  // the first event of the lane wins
-                        if (a[j] != b[j]) m = p0 + 8 * j + key_lcp(a[j], b[j]);
-                        else if ((a[j] & 0xFF) == 0) m = p0 + 8 * j + key_strlen(a[j]);
+                        if (a[u][j] != b[j]) m = p0 + 8 * j + key_lcp(a[u][j], b[j]);
+                        else if ((a[u][j] & 0xFF) == 0) m = p0 + 8 * j + key_strlen(a[u][j]);
                     }
                 }
-            }
-            // lanes cover consecutive 32-byte pieces of the step: the
-            // minimum over the group is the first mismatch or end
-            for (int o = kSkipLanes / 2; o > 0; o >>= 1)
-                m = min(m, (uint32_t)__shfl_xor(m, o, kSkipLanes));
-            if (!done) {
-                if (m != kNone) {
-                    res = m - d0;
-                    done = true;
-                } else {
-                    base += 32 * kSkipLanes;
-                    lim = min(lim, *vbmin);
-                    if (lim != kNone && base - d0 >= lim) done = true;  // cannot lower the minimum
+                // lanes cover consecutive 32-byte pieces of the step: the
+                // minimum over the group is the first mismatch or end
+                for (int o = kSkipLanes / 2; o > 0; o >>= 1)
+                    m = min(m, (uint32_t)__shfl_xor(m, o, kSkipLanes));
+                if (!done[u] && m != kNone) {
+                    res[u] = m - d0;
+                    done[u] = true;
                 }
             }
+            base += 32 * kSkipLanes;
+            lim = min(lim, *vbmin);
+            if (lim != kNone && base - d0 >= lim) {  // cannot lower the minimum
+#pragma unroll
+                for (uint32_t u = 0; u < kU; ++u) done[u] = true;
+            }
         }
-        if (l == 0 && res != kNone) atomicMin(&bmin, res);
+        uint32_t best = kNone;
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) best = min(best, res[u]);
+        if (l == 0 && best != kNone) atomicMin(&bmin, best);
     }
     __syncthreads();
     if (threadIdx.x == 0 && bmin != kNone) atomicMin(&segs[T.seg].cp, bmin);
@@ -395,7 +419,8 @@ k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint6
 __global__ void __launch_bounds__(kThreads)
 k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
                 Seg* __restrict__ next, SmallSeg* __restrict__ small, SmallLists L, uint32_t large_min,
-                uint32_t* __restrict__ ctr, uint32_t* __restrict__ next_large, uint32_t* __restrict__ lcp) {
+                uint32_t* __restrict__ ctr, uint32_t* __restrict__ next_large, uint32_t* __restrict__ lcp,
+                uint32_t key_buf) {
     __shared__ uint32_t sc[256];
     __shared__ uint32_t first;
     __shared__ uint32_t lcnt[kClasses + 1], lelem[kClasses], lbase[kClasses + 1];
@@ -451,7 +476,9 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint3
     __syncthreads();
     if (list < 0) return;
     if (list < (int)kClasses) {
-        small[L.off[list] + lbase[list] + slot] = SmallSeg{b, tot, dep + 1};
+        // the scatter of this pass leaves the bucket's keys (depth kd) in key_buf
+        const uint32_t sh = dep + 1 - kd;
+        small[L.off[list] + lbase[list] + slot] = SmallSeg{b, tot, dep + 1, sh < 8 ? (sh | (key_buf << 3)) : 0u};
     } else {
         // keys run out after 8 characters; a bucket that holds the whole
         // segment (one digit shared by all) also re-runs the prefix skip,
@@ -716,62 +743,78 @@ k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2
     }
 }
 
+// True if a key whose first w characters are valid reaches the end of its
+// string within them (the key is zero after a string's end).
+__device__ inline bool key_ended(uint64_t k, uint32_t w) { return ((k >> (64 - 8 * w)) & 0xFF) == 0; }
+
 // Small segments of at most 64 strings, one wave per 64 slots (64/CLS
 // segments per wave, one string per lane).  Each lane keeps its string's
-// 8-char key at its group's depth, the group leader lane G and the slot it
-// came from in registers; offset, length and id stay in LDS under that slot.
-// An iteration sorts (G, key, slot) with a register bitonic network inside
-// each aligned CLS-lane slot, splits groups where keys differ (LCP = depth +
-// common key bytes), and lets every group whose members still share the key
-// continue 8 characters deeper; groups of equal strings (key ended) stop.  A
-// round in which no active group has two different keys only advances the
-// depths.  Group extents come from ballots of the leader lanes; every loop
-// runs while a ballot over the wave is non-zero, so all lanes take part in
-// every shuffle.
+// 8-char key at its group's depth, the group leader lane G and the lane the
+// string came from in registers; id, offset and length stay in the lane that
+// loaded them and are fetched across lanes when needed.  An iteration sorts
+// (G, key, lane) with a register bitonic network inside each aligned
+// CLS-lane slot, splits groups where keys differ (LCP = depth + common key
+// bytes), and lets every group whose members still share the key continue
+// deeper; groups of equal strings (key ended) stop.  A segment whose keys
+// the last MSD pass left in a key buffer starts from those (fewer than 8
+// valid characters, no string access); later keys are loaded 8 characters
+// at a time.  A round in which no active group has two different keys only
+// advances the depths.  Group extents come from ballots of the leader lanes;
+// every loop runs while a ballot over the wave is non-zero, so all lanes
+// take part in every shuffle.
 __global__ void __launch_bounds__(kThreads)
 k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2,
                   uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
                   const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                  const uint64_t* __restrict__ keyA, const uint64_t* __restrict__ keyB,
                   uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len) {
-    __shared__ uint64_t s_off[kThreads];
-    __shared__ uint32_t s_len[kThreads], s_id[kThreads];
     const uint32_t lane = lane_id();
-    const uint32_t wbase = threadIdx.x & ~63u;  // this wave's LDS rows
     const uint32_t CLS = 1u << cls_log2;
     const uint32_t spw = 64u >> cls_log2;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t slot_start = lane & ~(CLS - 1), j = lane & (CLS - 1);
     const uint64_t sidx = wave * spw + (lane >> cls_log2);
     bool valid = false;
-    uint32_t sbegin = 0, depth0 = 0, scnt = 0;
+    uint32_t sbegin = 0, depth0 = 0, scnt = 0, cache = 0;
+    uint32_t my_id = 0, my_len = 0;
+    uint64_t my_off = 0, K0 = 0;
     if (sidx < nseg) {
         const SmallSeg sg = segs[sidx];
         sbegin = sg.begin;
         scnt = sg.cnt;
         depth0 = sg.depth;
+        cache = sg.cache;
         if (j < sg.cnt) {
             valid = true;
-            const uint32_t id = perm[sg.begin + j];
-            s_id[wbase + lane] = id;
-            s_off[wbase + lane] = off[id];
-            s_len[wbase + lane] = len[id];
+            my_id = perm[sg.begin + j];
+            if (cache) K0 = ((cache & 8) ? keyB : keyA)[sg.begin + j] << (8 * (cache & 7));
+            my_off = off[my_id];
+            my_len = len[my_id];
         }
     }
-    __syncthreads();
     // per element (moves with the sort): G, K, src (original lane)
     uint32_t G = valid ? slot_start : lane;  // padding lanes: own inactive group, after the strings
     uint64_t K = valid ? 0 : ~0ull;
     uint32_t src = lane;
-    // per position: LCP with the predecessor; per leader lane: depth, active
+    // per position: LCP with the predecessor; per leader lane: depth, width
+    // of the group's next key, active
     uint32_t lo = 0;
     uint32_t rd = depth0;
+    uint32_t rw = cache ? 8 - (cache & 7) : 8;
     bool act = valid && j == 0 && scnt >= 2;
     for (;;) {
         const uint64_t leaders = __ballot(lane == G);
         const uint32_t grd = __shfl(rd, G);
+        const uint32_t gw = __shfl(rw, G);
         const bool gact = __shfl((int)act, G) != 0;
         if (__ballot(gact) == 0) break;
-        if (gact) K = load_key8(chars, s_off[wbase + src], s_len[wbase + src], grd);
+        if (__ballot(gact && gw == 8)) {  // some group reads its next key from the strings
+            const uint64_t so = __shfl((unsigned long long)my_off, src);
+            const uint32_t sl = __shfl(my_len, src);
+            if (gact && gw == 8) K = load_key8(chars, so, sl, grd);
+        }
+        // a cached key is a segment's first (nothing is sorted yet: src == lane)
+        if (gact && gw < 8) K = K0;
         // group extent [G, gend) from the leader mask
         const uint64_t above = G >= 63 ? 0ull : (leaders & (~0ull << (G + 1)));
         const uint32_t gend = above ? (uint32_t)__builtin_ctzll(above) : 64u;
@@ -782,10 +825,15 @@ k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls
         if (__ballot(gact && (dmask & grp) != 0) == 0) {
             // no active group has two different keys: equal keys go deeper
             // unless they ended (then the group is a run of equal strings)
-            if (gact && lane != G && (K & 0xFF) == 0) lo = grd + key_strlen(K);
+            const bool ended = key_ended(K, gw);
+            if (gact && lane != G && ended) lo = grd + key_strlen(K);
             if (lane == G && gact) {
-                if ((K & 0xFF) == 0) act = false;
-                else rd = grd + 8;
+                if (ended) {
+                    act = false;
+                } else {
+                    rd = grd + gw;
+                    rw = 8;
+                }
             }
             continue;
         }
@@ -813,26 +861,30 @@ k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls
             if (pK != K) {
                 flag = true;
                 lo = grd + key_lcp(pK, K);
-            } else if ((K & 0xFF) == 0) {
+            } else if (key_ended(K, gw)) {
                 lo = grd + key_strlen(K);
             }
         }
         const uint64_t fmask = __ballot(flag);
         const uint32_t newG = 63u - (uint32_t)__builtin_clzll(fmask & (lane == 63 ? ~0ull : ((2ull << lane) - 1)));
         if (flag && gact) {
-            // a new group of an active group: 8 characters deeper, active if
-            // it has a second member (padding lanes are always leaders) and
-            // its key did not end
+            // a new group of an active group: deeper by the key's width,
+            // active if it has a second member (padding lanes are always
+            // leaders) and its key did not end
             const bool second = lane < 63 && !((fmask >> (lane + 1)) & 1);
-            rd = grd + 8;
-            act = second && (K & 0xFF) != 0;
+            rd = grd + gw;
+            rw = 8;
+            act = second && !key_ended(K, gw);
         }
         G = newG;
     }
+    const uint32_t fid = __shfl(my_id, src);
+    const uint64_t fo = __shfl((unsigned long long)my_off, src);
+    const uint32_t fl = __shfl(my_len, src);
     if (valid) {
-        perm[sbegin + j] = s_id[wbase + src];
-        out_off[sbegin + j] = s_off[wbase + src];
-        out_len[sbegin + j] = s_len[wbase + src];
+        perm[sbegin + j] = fid;
+        out_off[sbegin + j] = fo;
+        out_len[sbegin + j] = fl;
         if (j) lcp[sbegin + j] = lo;
     }
 }
@@ -946,7 +998,6 @@ void sort_lcp(dss_set_s* s) {
     uint32_t* perm = s->perm.as<uint32_t>();
     uint32_t* lcp = s->lcp.as<uint32_t>();
     for (;;) {
-        uint32_t* pc = ctr + kCtrPass + 4 * par;   // this pass
         uint32_t* pn = ctr + kCtrPass + 4 * (par ^ 1);  // the next pass
         const uint32_t* hp = h_ctr + kCtrPass + 4 * par;
         const uint32_t nlarge = hp[kCtrLarge];
@@ -970,10 +1021,14 @@ void sort_lcp(dss_set_s* s) {
         }
         {
             KScope ks(s->timer, st, "prefix_skip", 0.0);
+            decltype(&k_prefix_skip<8, 1>) kern = &k_prefix_skip<8, 1>;
             if (T.skip_lanes == 16)
-                hipLaunchKernelGGL(k_prefix_skip<16>, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len);
+                kern = T.skip_strings == 4 ? &k_prefix_skip<16, 4>
+                       : T.skip_strings == 2 ? &k_prefix_skip<16, 2> : &k_prefix_skip<16, 1>;
             else
-                hipLaunchKernelGGL(k_prefix_skip<8>, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len);
+                kern = T.skip_strings == 4 ? &k_prefix_skip<8, 4>
+                       : T.skip_strings == 2 ? &k_prefix_skip<8, 2> : &k_prefix_skip<8, 1>;
+            hipLaunchKernelGGL(kern, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len);
         }
         {
             KScope ks(s->timer, st, "gather", 0.0);  // bytes amended after the pass
@@ -990,7 +1045,7 @@ void sort_lcp(dss_set_s* s) {
                                    X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
             hipLaunchKernelGGL(k_scan_classify, dim3(nlarge), dim3(kThreads), 0, st, cur, X.hist.as<uint32_t>(),
                                X.csum.as<uint32_t>(), nxt, X.small.as<SmallSeg>(), L, T.large_min, ctr,
-                               pn + kCtrLarge, lcp);
+                               pn + kCtrLarge, lcp, kn == X.keyB.as<uint64_t>() ? 1u : 0u);
             if (nchunks)
                 hipLaunchKernelGGL(k_chunk_apply, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
                                    X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
@@ -1045,12 +1100,16 @@ void sort_lcp(dss_set_s* s) {
         if (!cnt) continue;
         DSS_REQUIRE(cnt <= L.cap[c], DSS_ERANGE, "small list overflow");
         // per string: id read, offset/length, one 8-char key, id + lcp writes
-        KScope ks(s->timer, st, "small_sort", (double)cls[kClasses + c] * 32.0);
+        static const char* const kClassNames[kClasses] = {
+            "small_sort_c0", "small_sort_c1", "small_sort_c2", "small_sort_c3", "small_sort_c4", "small_sort_c5",
+            "small_sort_c6", "small_sort_c7", "small_sort_c8", "small_sort_c9", "small_sort_c10", "small_sort_c11"};
+        KScope ks(s->timer, st, trace ? kClassNames[c] : "small_sort", (double)cls[kClasses + c] * 32.0);
         if (c <= T.wave_class_max) {
             const uint64_t waves = (cnt + (64u >> c) - 1) / (64u >> c);
             hipLaunchKernelGGL(k_small_sort_wave, dim3((waves + 3) / 4), dim3(kThreads), 0, st,
                                X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len,
-                               s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>());
+                               X.keyA.as<uint64_t>(), X.keyB.as<uint64_t>(), s->s_off.as<uint64_t>(),
+                               s->s_len.as<uint32_t>());
         } else {
             const uint32_t spb = kCap >> c;
             const uint32_t blocks = (cnt + spb - 1) / spb;
@@ -1112,9 +1171,12 @@ int dss_set_sort_params(dss_set s, const dss_sort_params_t* params) {
                         "large_min must be in [16, 1024]");
             DSS_REQUIRE(params->wave_class_max <= kWaveClassMax, DSS_EINVAL, "wave_class_max must be in [0, 6]");
             DSS_REQUIRE(params->skip_lanes == 8 || params->skip_lanes == 16, DSS_EINVAL, "skip_lanes must be 8 or 16");
+            DSS_REQUIRE(params->skip_strings <= 4 && params->skip_strings != 3, DSS_EINVAL,
+                        "skip_strings must be 0 (default), 1, 2 or 4");
             T.large_min = params->large_min;
             T.wave_class_max = params->wave_class_max;
             T.skip_lanes = params->skip_lanes;
+            if (params->skip_strings) T.skip_strings = params->skip_strings;
         }
         s->sort_params = T;
     });
@@ -1126,7 +1188,7 @@ int dss_get_sort_params(dss_set s, dss_sort_params_t* params) {
         params->large_min = s->sort_params.large_min;
         params->wave_class_max = s->sort_params.wave_class_max;
         params->skip_lanes = s->sort_params.skip_lanes;
-        params->reserved = 0;
+        params->skip_strings = s->sort_params.skip_strings;
     });
 }
 

@@ -157,17 +157,19 @@ class DeviceStringSet:
     def sort_params(self) -> dict:
         t = _lib.SortParams()
         check(lib().dss_get_sort_params(self._h, C.byref(t)))
-        return dict(large_min=t.large_min, wave_class_max=t.wave_class_max, skip_lanes=t.skip_lanes)
+        return dict(large_min=t.large_min, wave_class_max=t.wave_class_max, skip_lanes=t.skip_lanes,
+                    skip_strings=t.skip_strings)
 
     def set_sort_params(self, params=None):
-        """Local-sort tuning (large_min, wave_class_max, skip_lanes); None
-        restores the defaults.  Every setting gives the same output."""
+        """Local-sort tuning (large_min, wave_class_max, skip_lanes,
+        skip_strings); None restores the defaults.  Every setting gives the
+        same output."""
         if params is None:
             check(lib().dss_set_sort_params(self._h, None))
             return
         cur = self.sort_params()
         cur.update(params)
-        t = _lib.SortParams(cur["large_min"], cur["wave_class_max"], cur["skip_lanes"], 0)
+        t = _lib.SortParams(cur["large_min"], cur["wave_class_max"], cur["skip_lanes"], cur["skip_strings"])
         check(lib().dss_set_sort_params(self._h, C.byref(t)))
 
     def set_local_order(self, perm, lcp):

@@ -15,11 +15,12 @@ import math
 import os
 import time
 
-DEFAULT = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8}
+DEFAULT = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4}
 
 CANDIDATES = [
-    {"large_min": lm, "wave_class_max": wc, "skip_lanes": sk}
-    for lm in (256, 512, 1024) for wc in (5, 6) for sk in (8, 16)
+    {"large_min": lm, "wave_class_max": wc, "skip_lanes": sk, "skip_strings": ss}
+    for lm in (256, 512, 1024) for wc in (5, 6)
+    for sk, ss in ((8, 1), (8, 2), (8, 4), (16, 1), (16, 2))
 ]
 
 

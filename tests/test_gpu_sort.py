@@ -93,11 +93,14 @@ def test_device_generators_match_oracle():
 
 
 PARAM_SETS = [
-    {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8},
-    {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16},
-    {"large_min": 256, "wave_class_max": 5, "skip_lanes": 8},
-    {"large_min": 1024, "wave_class_max": 6, "skip_lanes": 16},
+    {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8, "skip_strings": 1},
+    {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 2},
+    {"large_min": 256, "wave_class_max": 5, "skip_lanes": 8, "skip_strings": 4},
+    {"large_min": 1024, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 1},
+    {"large_min": 128, "wave_class_max": 4, "skip_lanes": 8, "skip_strings": 2},
+    {"large_min": 32, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 4},
 ]
+DEFAULT_PARAMS = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4}
 
 
 @pytest.mark.parametrize("name", ["dton", "skewed", "suffix"])
@@ -120,7 +123,7 @@ def test_sort_params_do_not_change_the_output(name):
             perm, lcp = ds.get_sorted()
             assert (perm == ref[0]).all() and (lcp == ref[1]).all(), params
         ds.set_sort_params(None)
-        assert ds.sort_params() == {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8}
+        assert ds.sort_params() == DEFAULT_PARAMS
     assert (ref[1] == olcp).all()
     assert (canonical_ties(ref[0], ref[1], ss.len, ss.pe_begin) == canonical_ties(operm, olcp, ss.len, ss.pe_begin)).all()
 
@@ -143,3 +146,5 @@ def test_autotuner_picks_and_applies_a_setting(tmp_path):
             ds.set_sort_params({"large_min": 2000})
         with pytest.raises(dss.DssError):
             ds.set_sort_params({"skip_lanes": 12})
+        with pytest.raises(dss.DssError):
+            ds.set_sort_params({"skip_strings": 3})
