@@ -62,6 +62,7 @@ struct Seg {
 
 struct Tile {
     uint32_t seg, begin, end;
+    uint32_t keyed;  // common prefix at which the prefix skip wrote the tile's keys, or kNone
 };
 
 // A base-case segment; `cache` says where the 8-char keys of its strings
@@ -218,7 +219,7 @@ __global__ void k_fill_tiles(const Seg* __restrict__ segs, uint32_t nseg, uint32
     const uint32_t i = seg_of<false>(segs, nseg, t);
     const Seg s = segs[i];
     const uint32_t b = s.begin + (t - s.tile0) * kTile;
-    tiles[t] = Tile{i, b, min(s.begin + s.cnt, b + kTile)};
+    tiles[t] = Tile{i, b, min(s.begin + s.cnt, b + kTile), kNone};
 }
 
 // Per chunk and digit: the sum of the chunk's tile histograms.
@@ -274,14 +275,22 @@ __device__ inline void seg_depths(const Seg& s, uint32_t& kd, uint32_t& dep) {
 // it cannot lower the minimum and are not loaded, and a string stops once its
 // compared prefix reaches it.  All loops that contain cross-lane operations
 // have block- or wave-uniform trip counts, so every shuffle runs with all
-// lanes active.
+// lanes active.  The lane that finds a string's first difference (or end)
+// keeps the string's 8 characters from there (with the next lane's first key
+// when the difference lies in its last one); the block then writes its
+// strings' ids and their keys at the block minimum, composed from the
+// reference's characters before the difference and the kept ones, without
+// reading the strings again (strings without kept characters are loaded).
+// The key gather keeps the keys if the block minimum is the segment's, which
+// the whole grid decides.
 template <uint32_t kSkipLanes, uint32_t kU>
 __global__ void __launch_bounds__(kThreads)
-k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
+k_prefix_skip(Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
               const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
-              const uint32_t* __restrict__ len) {
-    __shared__ uint64_t t_off[kTile];
+              const uint32_t* __restrict__ len, uint64_t* __restrict__ key, uint32_t* __restrict__ pb) {
+    __shared__ uint64_t t_off[kTile];  // offset; after a string's event its kept characters
     __shared__ uint32_t t_len[kTile];
+    __shared__ uint32_t t_res[kTile];  // event depth - segment depth if characters were kept
     __shared__ uint32_t bmin;
     const Tile T = tiles[blockIdx.x];
     const Seg S = segs[T.seg];
@@ -292,6 +301,8 @@ k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint
         const uint32_t id = perm[T.begin + k];
         t_off[k] = off[id];
         t_len[k] = len[id];
+        t_res[k] = kNone;
+        pb[T.begin + k] = id;
     }
     __syncthreads();
     constexpr uint32_t kSkipGroups = kThreads / kSkipLanes;
@@ -343,16 +354,40 @@ k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint
             } else if (any && inwin) {
                 load_keys4(chars, roff, rlen, p0, b);
             }
+            // the next lane's window starts where this lane's ends
+            const bool next_in = l + 1 < kSkipLanes && (lim == kNone || p0 + 32 - d0 < lim);
 #pragma unroll
             for (uint32_t u = 0; u < kU; ++u) {
                 uint32_t m = kNone;
+                uint64_t cap = 0;  // the string's 8 characters from m
+                bool kept = false;
+                bool want_next = false;
                 if (!done[u] && inwin) {
 #pragma unroll
                     for (int j = 3; j >= 0; --j) {  // the first event of the lane wins
-                        if (a[u][j] != b[j]) m = p0 + 8 * j + key_lcp(a[u][j], b[j]);
-                        else if ((a[u][j] & 0xFF) == 0) m = p0 + 8 * j + key_strlen(a[u][j]);
+                        if (a[u][j] != b[j]) {
+                            const uint32_t e = key_lcp(a[u][j], b[j]);
+                            m = p0 + 8 * j + e;
+                            cap = a[u][j] << (8 * e);
+                            if (e && j < 3) cap |= a[u][j + 1] >> (64 - 8 * e);
+                            kept = e == 0 || j < 3;
+                            want_next = !kept;
+                        } else if ((a[u][j] & 0xFF) == 0) {
+                            m = p0 + 8 * j + key_strlen(a[u][j]);
+                            cap = 0;  // a key is zero after its string's end
+                            kept = true;
+                            want_next = false;
+                        }
                     }
                 }
+                if (__ballot(want_next)) {  // the rest comes from the next lane's first key
+                    const uint64_t nx = __shfl_down((unsigned long long)a[u][0], 1, kSkipLanes);
+                    if (want_next && next_in) {
+                        cap |= nx >> (64 - 8 * ((m - p0) & 7));
+                        kept = true;
+                    }
+                }
+                const uint32_t mine = m;
                 // lanes cover consecutive 32-byte pieces of the step: the
                 // minimum over the group is the first mismatch or end
                 for (int o = kSkipLanes / 2; o > 0; o >>= 1)
@@ -360,6 +395,11 @@ k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint
                 if (!done[u] && m != kNone) {
                     res[u] = m - d0;
                     done[u] = true;
+                    if (mine == m && kept) {
+                        const uint32_t k = (r * kU + u) * kSkipGroups + g;
+                        t_off[k] = cap;
+                        t_res[k] = m - d0;
+                    }
                 }
             }
             base += 32 * kSkipLanes;
@@ -375,22 +415,41 @@ k_prefix_skip(const Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint
         if (l == 0 && best != kNone) atomicMin(&bmin, best);
     }
     __syncthreads();
-    if (threadIdx.x == 0 && bmin != kNone) atomicMin(&segs[T.seg].cp, bmin);
+    const uint32_t bm = bmin;
+    if (bm == kNone) return;
+    if (threadIdx.x == 0) {
+        atomicMin(&segs[T.seg].cp, bm);
+        tiles[blockIdx.x].keyed = bm;
+    }
+    // a string equals the reference before its event depth r >= bm: its key
+    // at bm is the reference's r - bm characters, then its kept ones
+    const uint64_t rkey = load_key8(chars, roff, rlen, d0 + bm);
+    for (uint32_t k = threadIdx.x; k < cnt; k += kThreads) {
+        const uint32_t r = t_res[k];
+        uint64_t kk;
+        if (r == kNone) {
+            kk = load_key8(chars, t_off[k], t_len[k], d0 + bm);
+        } else {
+            const uint32_t dl = r - bm;
+            kk = dl >= 8 ? rkey : dl ? (rkey & (~0ull << (64 - 8 * dl))) | (t_off[k] >> (8 * dl)) : t_off[k];
+        }
+        key[T.begin + k] = kk;
+    }
 }
 
 // ------------------------------------------------------------ key gather
 __global__ void __launch_bounds__(kThreads)
 k_gather(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
          const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
-         const uint32_t* __restrict__ len, uint64_t* __restrict__ key, uint32_t* __restrict__ pb) {
+         const uint32_t* __restrict__ len, uint64_t* __restrict__ key) {
     const Tile T = tiles[blockIdx.x];
     const Seg S = segs[T.seg];
-    if (S.kdepth != kNone) return;
+    // the prefix skip wrote the ids, and the keys if its block minimum is the segment's
+    if (S.kdepth != kNone || T.keyed == S.cp) return;
     const uint32_t kd = S.depth + S.cp;
     for (uint32_t e = T.begin + threadIdx.x; e < T.end; e += kThreads) {
         const uint32_t id = perm[e];
         key[e] = load_key8(chars, off[id], len[id], kd);
-        pb[e] = id;
     }
 }
 
@@ -1010,7 +1069,7 @@ void sort_lcp(dss_set_s* s) {
         DSS_HIP(hipMemsetAsync(pn, 0, 16, st));
         hipLaunchKernelGGL(k_fill_tiles, dim3((nt + 255) / 256), dim3(256), 0, st, cur, nlarge, nt,
                            X.tiles.as<Tile>());
-        const Tile* tl = X.tiles.as<Tile>();
+        Tile* tl = X.tiles.as<Tile>();
         // elements in large segments of this pass (for the algorithmic byte counts)
         std::vector<Seg> hseg;
         uint64_t cnt_large = 0;
@@ -1028,11 +1087,11 @@ void sort_lcp(dss_set_s* s) {
             else
                 kern = T.skip_strings == 4 ? &k_prefix_skip<8, 4>
                        : T.skip_strings == 2 ? &k_prefix_skip<8, 2> : &k_prefix_skip<8, 1>;
-            hipLaunchKernelGGL(kern, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len);
+            hipLaunchKernelGGL(kern, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc, pc_ids);
         }
         {
             KScope ks(s->timer, st, "gather", 0.0);  // bytes amended after the pass
-            hipLaunchKernelGGL(k_gather, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc, pc_ids);
+            hipLaunchKernelGGL(k_gather, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc);
         }
         {
             KScope ks(s->timer, st, "hist", (double)cnt_large * 8.0);
@@ -1060,17 +1119,20 @@ void sort_lcp(dss_set_s* s) {
         DSS_HIP(hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, st));
         DSS_HIP(hipStreamSynchronize(st));
         if (s->timer.enabled) {
-            // prefix skip: characters up to the common prefix + 1 of every
-            // string of a segment that gathers keys, plus id/offset/length;
-            // gather: id, offset, length, 8 chars, key and id writes
+            // prefix skip: per string of a segment that gathers keys, its
+            // characters up to the common prefix + 8 (the key), id, offset
+            // and length reads, key and id writes; gather: per string of a
+            // tile whose keys are re-gathered, id, offset, length, 8 chars
+            // and the key write
             std::vector<Seg> after(nlarge);
             DSS_HIP(hipMemcpy(after.data(), cur, nlarge * sizeof(Seg), hipMemcpyDeviceToHost));
+            std::vector<Tile> ht(nt);
+            DSS_HIP(hipMemcpy(ht.data(), tl, nt * sizeof(Tile), hipMemcpyDeviceToHost));
             double skip = 0, gath = 0;
             for (uint32_t k = 0; k < nlarge; ++k)
-                if (hseg[k].kdepth == kNone) {
-                    skip += (double)after[k].cnt * ((double)after[k].cp + 1.0 + 16.0);
-                    gath += (double)after[k].cnt * 36.0;
-                }
+                if (hseg[k].kdepth == kNone) skip += (double)after[k].cnt * ((double)after[k].cp + 36.0);
+            for (const Tile& q : ht)
+                if (hseg[q.seg].kdepth == kNone && q.keyed != after[q.seg].cp) gath += (double)(q.end - q.begin) * 32.0;
             s->timer.amend_last("prefix_skip", skip);
             s->timer.amend_last("gather", gath);
         }
