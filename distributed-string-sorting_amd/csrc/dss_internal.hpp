@@ -175,6 +175,7 @@ struct PipeState {
     uint32_t pd_rounds = 0;
     DevBuf pd_cand, pd_next, pd_hash, pd_flags, pd_table, pd_tflag, pd_ctr, pd_slot, pd_list, pd_maybe, pd_dup,
         pd_bits, pd_head, pd_hmax, pd_hpos, pd_hstate;
+    DevBuf pd_pkey, pd_pid, pd_phist, pd_pbound;  // partitioned duplicate detection
     // partition
     bool partitioned = false;
     std::vector<uint64_t> interval;  // p*p, [r*p+q] = strings PE r sends to PE q

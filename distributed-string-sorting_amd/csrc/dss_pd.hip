@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -342,8 +343,10 @@ __global__ void __launch_bounds__(1024) k_head_sums(uint32_t* __restrict__ v, ui
     if (t == 1023) *total = part[1023];
 }
 
-// Per position: rank of its run head (heads before or at it, minus one);
-// heads record their position under their rank.
+// Per position: rank of its run head (heads before or at it, minus one),
+// with kHeadBit set at the heads themselves; heads record their position
+// under their rank.
+constexpr uint32_t kHeadBit = 0x80000000u;
 __global__ void __launch_bounds__(kHsT)
 k_head_rank(const uint32_t* __restrict__ lcp, uint64_t n, uint32_t B, const uint32_t* __restrict__ bcnt,
             uint32_t* __restrict__ hrank, uint32_t* __restrict__ hpos) {
@@ -367,7 +370,7 @@ k_head_rank(const uint32_t* __restrict__ lcp, uint64_t n, uint32_t B, const uint
     for (uint32_t k = 0; k < kHsI; ++k) {
         if (b + k >= n) break;
         if (f[k]) hpos[run++] = (uint32_t)(b + k);
-        hrank[b + k] = run - 1;
+        hrank[b + k] = (run - 1) | (f[k] ? kHeadBit : 0u);
     }
 }
 
@@ -391,72 +394,110 @@ __global__ void k_head_state(const uint32_t* __restrict__ hpos, uint64_t nheads,
     state[t] = v;
 }
 
-// XXH64 of the listed candidates, four lanes per string: a run head hashes
-// its own bytes from the start, any other position continues from its head's
-// state after L stripes.
+// Digest of XXH64 (the 32-byte quirk included) from the lanes v1..v4 after
+// the full stripes and the little-endian words t0..t3 of the tail (the
+// depth - 32·stripes < 32 characters after them).
+__device__ inline uint64_t xxh64_digest(uint64_t v1, uint64_t v2, uint64_t v3, uint64_t v4, uint32_t depth,
+                                        uint64_t t0, uint64_t t1, uint64_t t2, uint64_t t3) {
+    uint64_t h;
+    if (depth > 32) {
+        h = rotl64(v1, 1) + rotl64(v2, 7) + rotl64(v3, 12) + rotl64(v4, 18);
+        h = xmerge(h, v1);
+        h = xmerge(h, v2);
+        h = xmerge(h, v3);
+        h = xmerge(h, v4);
+    } else {
+        h = v3 + P5;
+    }
+    h += depth;
+    const uint32_t tail = depth & 31;
+    uint32_t q = 0;
+    while (q + 8 <= tail) {
+        const uint64_t w = q == 0 ? t0 : (q == 8 ? t1 : t2);
+        h ^= xround(0, w);
+        h = rotl64(h, 27) * P1 + P4;
+        q += 8;
+    }
+    const uint64_t w = q == 0 ? t0 : (q == 8 ? t1 : (q == 16 ? t2 : t3));  // the partial word
+    uint32_t b = 0;
+    if (q + 4 <= tail) {
+        h ^= (w & 0xFFFFFFFFull) * P1;
+        h = rotl64(h, 23) * P2 + P3;
+        q += 4;
+        b = 4;
+    }
+    for (; q < tail; ++q, ++b) {
+        h ^= ((w >> (8 * b)) & 0xFF) * P5;
+        h = rotl64(h, 11) * P1;
+    }
+    h ^= h >> 33;
+    h *= P2;
+    h ^= h >> 29;
+    h *= P3;
+    h ^= h >> 32;
+    return h;
+}
+
+// XXH64 of the listed candidates, four lanes per string and kHashU strings
+// per lane group at a time: a run head hashes its own bytes from the start,
+// any other position continues from its head's state after L stripes.  Lane
+// j loads word j of every stripe it hashes and word j of the tail up front,
+// so a string waits for its list entry, its offset and rank, then its words
+// and head state.
+constexpr uint32_t kHashU = 2;
+
 __global__ void __launch_bounds__(256)
 k_pd_hash_shared(const uint32_t* __restrict__ list, const unsigned long long* __restrict__ nhashed,
                  const uint64_t* __restrict__ s_off, const uint32_t* __restrict__ hrank,
-                 const uint32_t* __restrict__ hpos, const uint64_t* __restrict__ state,
-                 const uint8_t* __restrict__ chars, uint32_t depth, uint32_t L, uint64_t m,
-                 uint64_t* __restrict__ hashv) {
+                 const uint64_t* __restrict__ state, const uint8_t* __restrict__ chars, uint32_t depth, uint32_t L,
+                 uint64_t m, uint64_t* __restrict__ hashv) {
     const uint32_t j = threadIdx.x & 3, g = threadIdx.x >> 2;
     const uint32_t base_lane = threadIdx.x & ~3u;
     const uint32_t stripes = depth / 32;
+    const bool tw_in = 8 * j < (depth & 31);  // lane j's tail word holds characters of the prefix
     const uint64_t nh = *nhashed;
-    for (uint64_t blk = (uint64_t)blockIdx.x * 64; blk < nh; blk += (uint64_t)gridDim.x * 64) {
-        const uint64_t i = blk + g;
-        const bool hashed = i < nh;
-        const uint32_t c = hashed ? list[i] : 0u;
-        const uint64_t own = hashed ? s_off[c] : 0ull;
-        uint64_t v = j == 0 ? P1 + P2 : (j == 1 ? P2 : (j == 2 ? 0ull : (uint64_t)0 - P1));
-        uint32_t k0 = 0;
-        if (hashed) {
-            const uint32_t r = hrank[c];
-            if (hpos[r] != c) {
-                v = state[4 * (uint64_t)r + j];
-                k0 = L;
-            }
-            for (uint32_t k = k0; k < stripes; ++k) v = xround(v, load_word_le(chars, own + 32 * k + 8 * j));
+    const uint64_t v0 = j == 0 ? P1 + P2 : (j == 1 ? P2 : (j == 2 ? 0ull : (uint64_t)0 - P1));
+    constexpr uint64_t kPer = 64 * kHashU;
+    for (uint64_t blk = (uint64_t)blockIdx.x * kPer; blk < nh; blk += (uint64_t)gridDim.x * kPer) {
+        uint32_t c[kHashU], rk[kHashU];
+        uint64_t own[kHashU], v[kHashU], tw[kHashU];
+        bool hashed[kHashU];
+#pragma unroll
+        for (uint32_t u = 0; u < kHashU; ++u) {
+            const uint64_t i = blk + 64 * u + g;
+            hashed[u] = i < nh;
+            c[u] = hashed[u] ? list[i] : 0u;
         }
-        const uint64_t v1 = __shfl(v, base_lane + 0);
-        const uint64_t v2 = __shfl(v, base_lane + 1);
-        const uint64_t v3 = __shfl(v, base_lane + 2);
-        const uint64_t v4 = __shfl(v, base_lane + 3);
-        if (hashed && j == 0) {
-            uint64_t h;
-            if (depth > 32) {
-                h = rotl64(v1, 1) + rotl64(v2, 7) + rotl64(v3, 12) + rotl64(v4, 18);
-                h = xmerge(h, v1);
-                h = xmerge(h, v2);
-                h = xmerge(h, v3);
-                h = xmerge(h, v4);
-            } else {
-                h = v3 + P5;
+#pragma unroll
+        for (uint32_t u = 0; u < kHashU; ++u) {
+            own[u] = hashed[u] ? s_off[c[u]] : 0ull;
+            rk[u] = hashed[u] ? hrank[c[u]] : kHeadBit;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kHashU; ++u) {
+            v[u] = v0;
+            tw[u] = 0;
+            if (hashed[u]) {
+                uint32_t k0 = 0;
+                if (!(rk[u] & kHeadBit)) {
+                    v[u] = state[4 * (uint64_t)rk[u] + j];
+                    k0 = L;
+                }
+                if (tw_in) tw[u] = load_word_le(chars, own[u] + 32 * stripes + 8 * j);
+                for (uint32_t k = k0; k < stripes; ++k) v[u] = xround(v[u], load_word_le(chars, own[u] + 32 * k + 8 * j));
             }
-            h += depth;
-            uint32_t q = stripes * 32;
-            while (q + 8 <= depth) {
-                h ^= xround(0, load_word_le(chars, own + q));
-                h = rotl64(h, 27) * P1 + P4;
-                q += 8;
-            }
-            if (q + 4 <= depth) {
-                h ^= (load_word_le(chars, own + q) & 0xFFFFFFFFull) * P1;
-                h = rotl64(h, 23) * P2 + P3;
-                q += 4;
-            }
-            while (q < depth) {
-                h ^= (uint64_t)chars[own + q] * P5;
-                h = rotl64(h, 11) * P1;
-                ++q;
-            }
-            h ^= h >> 33;
-            h *= P2;
-            h ^= h >> 29;
-            h *= P3;
-            h ^= h >> 32;
-            hashv[c] = mod_range(h, m);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kHashU; ++u) {
+            const uint64_t v1 = __shfl(v[u], base_lane + 0);
+            const uint64_t v2 = __shfl(v[u], base_lane + 1);
+            const uint64_t v3 = __shfl(v[u], base_lane + 2);
+            const uint64_t v4 = __shfl(v[u], base_lane + 3);
+            const uint64_t t0 = __shfl(tw[u], base_lane + 0);
+            const uint64_t t1 = __shfl(tw[u], base_lane + 1);
+            const uint64_t t2 = __shfl(tw[u], base_lane + 2);
+            const uint64_t t3 = __shfl(tw[u], base_lane + 3);
+            if (hashed[u] && j == 0) hashv[c[u]] = mod_range(xxh64_digest(v1, v2, v3, v4, depth, t0, t1, t2, t3), m);
         }
     }
 }
@@ -563,6 +604,169 @@ __global__ void k_pd_flag(const uint32_t* __restrict__ maybe, uint64_t nm, const
     }
 }
 
+// ------------------------------------------------ partitioned duplicates
+// Rounds with many hashed candidates: x = mix64(hash) (a bijection, so equal
+// x means equal hashes) is put in order of its top kb bits by two stable LSD
+// partition passes over tiles of kHpTile candidates (a per-tile digit
+// histogram, an exclusive scan in digit-major order, a scatter that ranks
+// equal digits with ballots), so that equal hashes share one bucket of about
+// nh / 2^kb candidates.  A block per bucket then inserts the bucket's hashes
+// into an LDS open-addressing table, flags slots that receive a hash twice,
+// and marks the candidates of flagged slots as duplicates.  A bucket with too
+// many distinct hashes for the table hands its candidates to the exact
+// global table instead.  Every access is a streamed read or write except the
+// duplicate flags, which only duplicates write.
+constexpr uint32_t kHpT = 256, kHpI = 16, kHpTile = kHpT * kHpI;
+constexpr uint32_t kHpMaxBits = 9;     // digit bits per pass
+constexpr uint32_t kHpCap = 2048;      // LDS table slots per bucket
+constexpr uint64_t kHpMin = 1u << 20;  // hashed candidates from which rounds partition
+
+__global__ void __launch_bounds__(kHpT)
+k_hp_hist(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv, const uint64_t* __restrict__ keys,
+          uint64_t nh, uint32_t shift, uint32_t bits, uint64_t ntiles, uint64_t* __restrict__ hist) {
+    __shared__ uint32_t h[1u << kHpMaxBits];
+    const uint32_t nb = 1u << bits, tid = threadIdx.x;
+    for (uint32_t d = tid; d < nb; d += kHpT) h[d] = 0;
+    __syncthreads();
+    const uint64_t t0 = (uint64_t)blockIdx.x * kHpTile;
+    for (uint32_t r = 0; r < kHpI; ++r) {
+        const uint64_t i = t0 + r * kHpT + tid;
+        if (i < nh) {
+            const uint64_t x = keys ? keys[i] : mix64(hashv[list[i]]);
+            atomicAdd(&h[(uint32_t)(x >> shift) & (nb - 1)], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t d = tid; d < nb; d += kHpT) hist[(uint64_t)d * ntiles + blockIdx.x] = h[d];
+}
+
+// Stable scatter of one tile by the digit (x >> shift) & (2^bits - 1): the
+// first pass reads the candidate list and hashes, the second the first
+// pass's output.
+__global__ void __launch_bounds__(kHpT)
+k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv,
+             const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ ids_in, uint64_t nh, uint32_t shift,
+             uint32_t bits, uint64_t ntiles, const uint64_t* __restrict__ off, uint64_t* __restrict__ keys_out,
+             uint32_t* __restrict__ ids_out) {
+    __shared__ uint64_t base[1u << kHpMaxBits];
+    __shared__ uint32_t wcnt[kHpT / 64][1u << kHpMaxBits];
+    const uint32_t nb = 1u << bits, tid = threadIdx.x, w = tid >> 6;
+    for (uint32_t d = tid; d < nb; d += kHpT) {
+        base[d] = off[(uint64_t)d * ntiles + blockIdx.x];
+        for (uint32_t k = 0; k < kHpT / 64; ++k) wcnt[k][d] = 0;
+    }
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    const uint64_t t0 = (uint64_t)blockIdx.x * kHpTile;
+    for (uint32_t r = 0; r < kHpI; ++r) {
+        const uint64_t i = t0 + r * kHpT + tid;
+        const bool valid = i < nh;
+        uint64_t x = 0;
+        uint32_t id = 0, dg = 0;
+        if (valid) {
+            if (keys_in) {
+                x = keys_in[i];
+                id = ids_in[i];
+            } else {
+                id = list[i];
+                x = mix64(hashv[id]);
+            }
+            dg = (uint32_t)(x >> shift) & (nb - 1);
+        }
+        uint64_t m = __ballot(valid);
+        for (uint32_t b = 0; b < bits; ++b) {
+            const bool bit = (dg >> b) & 1;
+            const uint64_t bb = __ballot(valid && bit);
+            m &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(m & lt);
+        if (valid && rank == 0) wcnt[w][dg] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint64_t q = base[dg] + rank;
+            for (uint32_t ww = 0; ww < w; ++ww) q += wcnt[ww][dg];
+            keys_out[q] = x;
+            ids_out[q] = id;
+        }
+        __syncthreads();
+        for (uint32_t d = tid; d < nb; d += kHpT) {
+            uint32_t add = 0;
+            for (uint32_t k = 0; k < kHpT / 64; ++k) {
+                add += wcnt[k][d];
+                wcnt[k][d] = 0;
+            }
+            base[d] += add;
+        }
+        __syncthreads();
+    }
+}
+
+// Bucket extents [bstart, bend) of the partitioned keys (top kb bits); empty
+// buckets keep 0, 0.
+__global__ void k_hp_bounds(const uint64_t* __restrict__ keys, uint64_t nh, uint32_t kb, uint32_t* __restrict__ bstart,
+                            uint32_t* __restrict__ bend) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nh) return;
+    const uint64_t b = keys[i] >> (64 - kb);
+    if (i == 0 || (keys[i - 1] >> (64 - kb)) != b) bstart[b] = (uint32_t)i;
+    if (i + 1 == nh || (keys[i + 1] >> (64 - kb)) != b) bend[b] = (uint32_t)(i + 1);
+}
+
+__global__ void __launch_bounds__(256)
+k_hp_dedup(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids, const uint32_t* __restrict__ bstart,
+           const uint32_t* __restrict__ bend, uint32_t nbuckets, uint32_t kb, uint32_t limit, uint8_t* __restrict__ dup,
+           uint32_t* __restrict__ maybe, unsigned long long* __restrict__ nmaybe) {
+    __shared__ unsigned long long tab[kHpCap];
+    __shared__ uint8_t fl[kHpCap];
+    __shared__ uint32_t used, over, mbase;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t b = blockIdx.x; b < nbuckets; b += gridDim.x) {
+        const uint32_t s = bstart[b], e = bend[b];
+        if (e - s < 2) continue;  // block-uniform
+        // a value outside the bucket (other top bits) marks empty slots
+        const unsigned long long empty = (unsigned long long)(b ^ 1u) << (64 - kb);
+        for (uint32_t k = tid; k < kHpCap; k += 256) {
+            tab[k] = empty;
+            fl[k] = 0;
+        }
+        if (tid == 0) used = over = 0;
+        __syncthreads();
+        for (uint32_t i = s + tid; i < e; i += 256) {
+            if (over) break;
+            const unsigned long long x = keys[i];
+            uint32_t slot = (uint32_t)x & (kHpCap - 1);
+            uint32_t probe = 0;
+            for (; probe < kHpCap; ++probe) {
+                const unsigned long long old = atomicCAS(&tab[slot], empty, x);
+                if (old == empty) {
+                    if (atomicAdd(&used, 1u) >= limit) over = 1;
+                    break;
+                }
+                if (old == x) {
+                    fl[slot] = 1;
+                    break;
+                }
+                slot = (slot + 1) & (kHpCap - 1);
+            }
+            if (probe == kHpCap) over = 1;
+        }
+        __syncthreads();
+        if (over) {
+            if (tid == 0) mbase = (uint32_t)atomicAdd(nmaybe, (unsigned long long)(e - s));
+            __syncthreads();
+            for (uint32_t i = s + tid; i < e; i += 256) maybe[mbase + (i - s)] = ids[i];
+        } else {
+            for (uint32_t i = s + tid; i < e; i += 256) {
+                const unsigned long long x = keys[i];
+                uint32_t slot = (uint32_t)x & (kHpCap - 1);
+                while (tab[slot] != x) slot = (slot + 1) & (kHpCap - 1);
+                if (fl[slot]) dup[ids[i]] = 1;
+            }
+        }
+        __syncthreads();  // the table is reset for the next bucket
+    }
+}
+
 __global__ void __launch_bounds__(256)
 k_pd_next(const uint8_t* __restrict__ cand, const uint8_t* __restrict__ flags, const uint64_t* __restrict__ pe_begin,
           uint32_t p, const uint32_t* __restrict__ lcp, uint32_t depth, uint64_t n, const uint8_t* __restrict__ dup,
@@ -583,6 +787,64 @@ k_pd_next(const uint8_t* __restrict__ cand, const uint8_t* __restrict__ flags, c
         kept += keep;
     }
     block_count_add(kept, ncand);
+}
+
+// Duplicate flags of a round's nh hashed candidates by partitioning (see
+// k_hp_dedup): dup is cleared and set for the repeats found in LDS; the
+// candidates of buckets with more than table_limit distinct hashes are
+// listed in pd_maybe for the exact table.  Returns their number.
+static uint64_t partition_duplicates(dss_set_s* s, uint64_t nh, uint32_t table_limit) {
+    PipeState& P = s->pp;
+    const hipStream_t st = s->stream;
+    unsigned long long* ctr = P.pd_ctr.as<unsigned long long>();
+    uint32_t kb = 2;  // bucket bits: about 384 candidates per bucket at most on average
+    while (kb < 2 * kHpMaxBits && (nh >> kb) > 384) ++kb;
+    const uint32_t b1 = kb / 2, b0 = kb - b1;  // pass 0: the lower b0 of the top kb bits; pass 1: the top b1
+    const uint64_t ntiles = (nh + kHpTile - 1) / kHpTile;
+    const uint32_t nbk = 1u << kb;
+    P.pd_pkey.reserve(nh * 16);
+    P.pd_pid.reserve(nh * 8);
+    P.pd_phist.reserve((1ull << b0) * ntiles * 8);
+    P.pd_pbound.reserve((uint64_t)nbk * 8);
+    uint64_t* kA = P.pd_pkey.as<uint64_t>();
+    uint64_t* kB = kA + nh;
+    uint32_t* iA = P.pd_pid.as<uint32_t>();
+    uint32_t* iB = iA + nh;
+    uint64_t* hist = P.pd_phist.as<uint64_t>();
+    const uint32_t* list = P.pd_list.as<uint32_t>();
+    const uint64_t* hash = P.pd_hash.as<uint64_t>();
+    {
+        // per candidate: list entry and hash (twice), two 12-byte scatters
+        // (read and write) and the second histogram's key read
+        KScope ks(s->timer, st, "pd_partition", (double)nh * 68.0);
+        const uint32_t sh0 = 64 - kb, sh1 = 64 - b1;
+        hipLaunchKernelGGL(k_hp_hist, dim3(ntiles), dim3(kHpT), 0, st, list, hash, nullptr, nh, sh0, b0, ntiles, hist);
+        exclusive_scan_u64(hist, hist, (1ull << b0) * ntiles, st, P, nullptr);
+        hipLaunchKernelGGL(k_hp_scatter, dim3(ntiles), dim3(kHpT), 0, st, list, hash, nullptr, nullptr, nh, sh0, b0,
+                           ntiles, hist, kA, iA);
+        hipLaunchKernelGGL(k_hp_hist, dim3(ntiles), dim3(kHpT), 0, st, nullptr, nullptr, kA, nh, sh1, b1, ntiles,
+                           hist);
+        exclusive_scan_u64(hist, hist, (1ull << b1) * ntiles, st, P, nullptr);
+        hipLaunchKernelGGL(k_hp_scatter, dim3(ntiles), dim3(kHpT), 0, st, nullptr, nullptr, kA, iA, nh, sh1, b1,
+                           ntiles, hist, kB, iB);
+    }
+    {
+        // per candidate: bucket bounds (key read), table insert and lookup
+        // (key read twice, id read)
+        KScope ks(s->timer, st, "pd_dedup", (double)nh * 28.0);
+        uint32_t* bstart = P.pd_pbound.as<uint32_t>();
+        uint32_t* bend = bstart + nbk;
+        DSS_HIP(hipMemsetAsync(bstart, 0, (uint64_t)nbk * 8, st));
+        hipLaunchKernelGGL(k_hp_bounds, dim3((nh + 255) / 256), dim3(256), 0, st, kB, nh, kb, bstart, bend);
+        DSS_HIP(hipMemsetAsync(P.pd_dup.p, 0, s->n, st));
+        hipLaunchKernelGGL(k_hp_dedup, dim3(std::min<uint32_t>(nbk, 65536)), dim3(256), 0, st, kB, iB, bstart, bend,
+                           nbk, kb, table_limit, P.pd_dup.as<uint8_t>(), P.pd_maybe.as<uint32_t>(), ctr + 2);
+    }
+    DSS_HIP(hipGetLastError());
+    uint64_t nm = 0;
+    DSS_HIP(hipMemcpyAsync(&nm, ctr + 2, 8, hipMemcpyDeviceToHost, st));
+    DSS_HIP(hipStreamSynchronize(st));
+    return nm;
 }
 
 void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
@@ -613,6 +875,14 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     uint64_t ncand = n;
     uint32_t round = 0;
     uint64_t depth = start_depth, step = (uint64_t)start_depth * 2;
+    // duplicate detection by partitioning from part_min hashed candidates on;
+    // both settings can be lowered by the environment (tests cover the
+    // partitioned path and its exact-table fallback on small inputs)
+    uint64_t part_min = kHpMin;
+    uint32_t table_limit = kHpCap / 8 * 7;
+    if (const char* e = getenv("DSS_PD_PARTITION_MIN")) part_min = std::max<uint64_t>(strtoull(e, nullptr, 10), 1);
+    if (const char* e = getenv("DSS_PD_TABLE_LIMIT"))
+        table_limit = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(strtoull(e, nullptr, 10), 1), kHpCap / 8 * 7);
     while (ncand > 0) {
         DSS_REQUIRE(depth < 0xFFFFFFFFull, DSS_ERANGE, "prefix doubling depth overflow");
         DSS_REQUIRE(2 * ncand < (1ull << 32), DSS_ERANGE, "too many candidates for the hash table");
@@ -662,11 +932,11 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
         {
             KScope ks(s->timer, st, "pd_hash", 0.0);
             const dim3 g(std::min<uint64_t>((nh + kPdPos - 1) / kPdPos, kPdGrid));
+            const dim3 gs(std::min<uint64_t>((nh + 64 * kHashU - 1) / (64 * kHashU), kPdGrid));
             if (shared)
-                hipLaunchKernelGGL(k_pd_hash_shared, g, blk, 0, st, P.pd_list.as<uint32_t>(), ctr,
-                                   s->s_off.as<uint64_t>(), P.pd_head.as<uint32_t>(), P.pd_hpos.as<uint32_t>(),
-                                   P.pd_hstate.as<uint64_t>(), s->arena(), (uint32_t)depth, L, m,
-                                   P.pd_hash.as<uint64_t>());
+                hipLaunchKernelGGL(k_pd_hash_shared, gs, blk, 0, st, P.pd_list.as<uint32_t>(), ctr,
+                                   s->s_off.as<uint64_t>(), P.pd_head.as<uint32_t>(), P.pd_hstate.as<uint64_t>(),
+                                   s->arena(), (uint32_t)depth, L, m, P.pd_hash.as<uint64_t>());
             else if (nh)
                 hipLaunchKernelGGL(k_pd_hash, g, blk, 0, st, P.pd_list.as<uint32_t>(), ctr, s->s_off.as<uint64_t>(),
                                    s->arena(), (uint32_t)depth, m, P.pd_hash.as<uint64_t>());
@@ -674,7 +944,10 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
         // every hashed candidate: list entry, offset, the characters read
         // from its own string (all of them, or beyond B), hash
         s->timer.amend_last("pd_hash", (double)nh * ((shared ? depth - B : depth) + 24.0));
-        if (nh) {
+        uint64_t nm = 0;  // possible repeats for the exact table
+        if (nh >= part_min) {
+            nm = partition_duplicates(s, nh, table_limit);
+        } else if (nh) {
             uint32_t bits = 16;
             while (bits < 32 && (1ull << bits) < 16 * nh) ++bits;
             const uint64_t words = (1ull << bits) / 32;
@@ -693,25 +966,23 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
                                    P.pd_hash.as<uint64_t>(), bits, again, P.pd_dup.as<uint8_t>(),
                                    P.pd_maybe.as<uint32_t>(), ctr + 2);
             }
-            uint64_t nm = 0;
             DSS_HIP(hipMemcpyAsync(&nm, ctr + 2, 8, hipMemcpyDeviceToHost, st));
             DSS_HIP(hipStreamSynchronize(st));
-            if (nm) {
-                uint64_t T = 1024;
-                while (T < 2 * nm) T <<= 1;
-                P.pd_table.reserve(T * 8);
-                P.pd_tflag.reserve(T);
-                // table fill, per possible repeat: hash, slot, flags
-                KScope ks(s->timer, st, "pd_table", (double)T * 9.0 + (double)nm * 30.0);
-                hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T,
-                                   kEmpty);
-                DSS_HIP(hipMemsetAsync(P.pd_tflag.p, 0, T, st));
-                hipLaunchKernelGGL(k_pd_insert, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
-                                   P.pd_hash.as<uint64_t>(), P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(),
-                                   T - 1, P.pd_slot.as<uint32_t>());
-                hipLaunchKernelGGL(k_pd_flag, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
-                                   P.pd_slot.as<uint32_t>(), P.pd_tflag.as<uint8_t>(), P.pd_dup.as<uint8_t>());
-            }
+        }
+        if (nm) {
+            uint64_t T = 1024;
+            while (T < 2 * nm) T <<= 1;
+            P.pd_table.reserve(T * 8);
+            P.pd_tflag.reserve(T);
+            // table fill, per possible repeat: hash, slot, flags
+            KScope ks(s->timer, st, "pd_table", (double)T * 9.0 + (double)nm * 30.0);
+            hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T, kEmpty);
+            DSS_HIP(hipMemsetAsync(P.pd_tflag.p, 0, T, st));
+            hipLaunchKernelGGL(k_pd_insert, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
+                               P.pd_hash.as<uint64_t>(), P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(), T - 1,
+                               P.pd_slot.as<uint32_t>());
+            hipLaunchKernelGGL(k_pd_flag, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
+                               P.pd_slot.as<uint32_t>(), P.pd_tflag.as<uint8_t>(), P.pd_dup.as<uint8_t>());
         }
         {
             // every position: flag, candidate flags, lcp, duplicate flag, next

@@ -475,10 +475,37 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
 }
 
 // 2^lg lanes per received string copy its bytes into the receive arena.
+// With lg = 0 (short copies, such as the few characters after the LCP that
+// an LCP-compressed distinguishing prefix sends) every lane copies one
+// string, 32 characters at a time from five aligned 8-byte loads, so a wave
+// has 64 strings in flight.
 __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* __restrict__ r_voff,
                             const uint32_t* __restrict__ r_vlen, const uint32_t* __restrict__ r_from,
                             const uint64_t* __restrict__ byte_off, int mode, uint64_t n, uint32_t lg,
                             uint8_t* __restrict__ out) {
+    if (lg == 0) {
+        for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+             k += (uint64_t)gridDim.x * blockDim.x) {
+            const uint32_t from = r_from[k];
+            const uint64_t a = r_voff[k] + from;
+            const uint32_t m = r_vlen[k] - from;
+            uint8_t* dst = out + byte_off[k];
+            for (uint32_t t = 0; t < m; t += 32) {
+                const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + ((a + t) & ~7ull));
+                const unsigned sh = (unsigned)((a + t) & 7) * 8;
+                uint64_t x[5], y[4];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) x[i] = w[i];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) y[i] = sh ? (x[i] >> sh) | (x[i + 1] << (64 - sh)) : x[i];
+#pragma unroll
+                for (uint32_t b = 0; b < 32; ++b)
+                    if (t + b < m) dst[t + b] = (uint8_t)(y[b >> 3] >> (8 * (b & 7)));
+            }
+            dst[m] = 0;
+        }
+        return;
+    }
     const uint32_t lanes = 1u << lg;
     const uint32_t l = threadIdx.x & (lanes - 1);
     for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lg; k < n;
@@ -560,8 +587,9 @@ void exchange(dss_set_s* s, int mode) {
     DSS_HIP(hipMemsetAsync(P.recv_bytes.as<uint8_t>() + total, 0, kArenaPad, st));
     if (n) {
         KScope ks(s->timer, st, "exchange_copy", 2.0 * total + n * 24.0);
-        // short prefix-doubling prefixes: 4 lanes per string, whole strings: 16
-        const uint32_t lg = (mode == 2 || total < 24 * n) ? 3 : 4;
+        // a lane per string for copies of at most 32 characters on average,
+        // else 8 lanes (short prefix-doubling prefixes) or 16 (whole strings)
+        const uint32_t lg = total <= 33 * n ? 0 : mode == 2 ? 3 : 4;
         const uint64_t threads = n << lg;
         hipLaunchKernelGGL(k_exch_copy, dim3(stride_grid(threads, 256)), dim3(256), 0, st, s->arena(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),

@@ -246,3 +246,27 @@ def test_prefix_doubling_shared_prefix_rounds(m):
         nr = ds.prefix_doubling(m=m)
         gres, grounds = ds.get_prefix_doubling()
     assert nr == onr and (gres == ores).all() and (grounds == orounds).all()
+
+
+@pytest.mark.parametrize("limit", [None, "8"])
+@pytest.mark.parametrize("m", [2**64 - 1, 4093, 64])
+@pytest.mark.parametrize("name", ["skewed", "dton"])
+def test_prefix_doubling_partitioned_duplicates(name, m, limit, monkeypatch):
+    """Rounds with many hashed candidates find repeated hashes by partitioning
+    them into buckets and checking each bucket in an LDS table; buckets with
+    more distinct hashes than the table allows (a limit of 8 forces that)
+    fall back to the exact global table.  Forced on for every round here,
+    both must give the oracle's results, collisions of small ranges included."""
+    dss = _dss()
+    chars = INPUTS[name] if name == "skewed" else O.gen_dton(20000, 200, 0.5, 0x5EED0008)
+    ss = O.StringSet(chars, 4)
+    operm, olcp = O.local_sort(ss)
+    ores, orounds, onr = O.prefix_doubling(ss, operm, olcp, m=m)
+    monkeypatch.setenv("DSS_PD_PARTITION_MIN", "1")
+    if limit:
+        monkeypatch.setenv("DSS_PD_TABLE_LIMIT", limit)
+    with dss.DeviceStringSet.from_arena(chars, 4) as ds:
+        ds.sort_lcp()
+        nr = ds.prefix_doubling(m=m)
+        gres, grounds = ds.get_prefix_doubling()
+    assert nr == onr and (gres == ores).all() and (grounds == orounds).all()
