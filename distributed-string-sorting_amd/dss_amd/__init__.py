@@ -14,7 +14,7 @@ import numpy as np
 from . import _lib
 from ._lib import DssError, check, lib
 
-__all__ = ["DeviceStringSet", "DssError", "JobPool", "device_info"]
+__all__ = ["DeviceStringSet", "DssError", "JobPool", "WireCodec", "device_info"]
 
 
 def _p(a, t):
@@ -309,3 +309,4 @@ class DeviceStringSet:
 
 
 from .pool import Batch, JobPool  # noqa: E402
+from .codec import WireCodec  # noqa: E402

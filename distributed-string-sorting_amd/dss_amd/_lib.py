@@ -16,6 +16,7 @@ u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
 set_p = C.c_void_p
 pool_p = C.c_void_p
+codec_p = C.c_void_p
 
 
 class SetInfo(C.Structure):
@@ -104,6 +105,25 @@ _SIGS = {
     "dss_pool_rebatch": (C.c_int, [pool_p, set_p, u32p, C.c_uint32]),
     "dss_pe_dist_sizes": (C.c_int, [set_p, u64p]),
     "dss_get_sorted_pe": (C.c_int, [set_p, C.c_uint32, u32p, u32p]),
+    "dss_codec_create": (C.c_int, [C.POINTER(codec_p)]),
+    "dss_codec_destroy": (C.c_int, [codec_p]),
+    "dss_codec_sync": (C.c_int, [codec_p]),
+    "dss_codec_set_timing": (C.c_int, [codec_p, C.c_int]),
+    "dss_codec_kernel_stats": (C.c_int, [codec_p, C.POINTER(KernelStat), C.c_uint32, u32p, C.c_int]),
+    "dss_varint_write_ranges": (C.c_int, [codec_p, u64p, u64p, C.c_uint32, u8p, C.c_uint64, u64p, u64p]),
+    "dss_varint_write_ranges_dev": (C.c_int, [codec_p, C.c_void_p, u64p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                              u64p, u64p]),
+    "dss_varint_read_ranges": (C.c_int, [codec_p, u8p, C.c_uint64, u64p, C.c_uint32, u64p]),
+    "dss_varint_read_ranges_dev": (C.c_int, [codec_p, C.c_void_p, C.c_uint64, u64p, C.c_uint32, C.c_void_p]),
+    "dss_golomb_b": (C.c_int, [C.c_uint64, C.c_uint64, u64p]),
+    "dss_golomb_encode": (C.c_int, [codec_p, C.c_int, u64p, u64p, u64p, C.c_uint32, u64p, C.c_uint64, u64p,
+                                    u64p]),
+    "dss_golomb_encode_dev": (C.c_int, [codec_p, C.c_int, C.c_void_p, u64p, u64p, C.c_uint32, C.c_void_p,
+                                        C.c_uint64, u64p, u64p]),
+    "dss_golomb_decode": (C.c_int, [codec_p, C.c_int, u64p, C.c_uint64, C.c_uint32, C.c_uint64, u64p,
+                                    C.c_uint64, u64p, u32p]),
+    "dss_golomb_decode_dev": (C.c_int, [codec_p, C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64,
+                                        C.c_void_p, C.c_uint64, u64p, u32p]),
 }
 
 

@@ -305,6 +305,71 @@ typedef struct {
 int dss_kernel_stats(dss_set s, dss_kernel_stat_t* stats, uint32_t cap, uint32_t* count,
                      int reset);
 
+
+/* ---- wire codecs of the exchange (multi-GPU wire formats; lossless) ----
+ *
+ * A dss_codec owns a HIP stream and scratch buffers.  The plain functions
+ * take host buffers; the _dev functions take device pointers for the data
+ * (values, bytes, words) and host pointers for counts and sizes.  An encoder
+ * that returns DSS_ERANGE has stored the size it needs in its total. */
+typedef struct dss_codec_s* dss_codec;
+
+int dss_codec_create(dss_codec* out);
+int dss_codec_destroy(dss_codec c);
+int dss_codec_sync(dss_codec c);
+int dss_codec_set_timing(dss_codec c, int enable);
+int dss_codec_kernel_stats(dss_codec c, dss_kernel_stat_t* stats, uint32_t cap, uint32_t* count, int reset);
+
+/* IntegerCompression::writeRanges (src/mpi/byte_encoder.hpp:31-53) over
+ * PutVarint (src/encoding/integer_compression.hpp:67-150), as sendLcps sends
+ * LCPs (src/mpi/alltoall.hpp:406-425): the values of num_ranges ranges
+ * (counts[r] each) as varints back to back; range_bytes[r] = bytes of range r. */
+int dss_varint_write_ranges(dss_codec c, const uint64_t* values, const uint64_t* counts, uint32_t num_ranges,
+                            uint8_t* out, uint64_t out_cap, uint64_t* range_bytes, uint64_t* total_bytes);
+int dss_varint_write_ranges_dev(dss_codec c, const uint64_t* values, const uint64_t* counts, uint32_t num_ranges,
+                                uint8_t* out, uint64_t out_cap, uint64_t* range_bytes, uint64_t* total_bytes);
+/* IntegerCompression::readRanges (src/mpi/byte_encoder.hpp:64-78): nbytes must
+ * hold exactly sum(counts) varints; DSS_EINVAL where GetVarint
+ * (integer_compression.hpp:209-236) throws or the input is short. */
+int dss_varint_read_ranges(dss_codec c, const uint8_t* bytes, uint64_t nbytes, const uint64_t* counts,
+                           uint32_t num_ranges, uint64_t* values);
+int dss_varint_read_ranges_dev(dss_codec c, const uint8_t* bytes, uint64_t nbytes, const uint64_t* counts,
+                               uint32_t num_ranges, uint64_t* values);
+
+/* getB (src/encoding/golomb_encoding.hpp:49-58). */
+int dss_golomb_b(uint64_t max_value, uint64_t num_values, uint64_t* b);
+
+/* Delta + Golomb streams of ascending u64 values per range
+ * (getDeltaEncoding, src/encoding/golomb_encoding.hpp:60-76), framed as
+ *   DSS_GOLOMB_BLOCKS:    AllToAllHashesGolomb (bloomfilter.hpp:164-313):
+ *                         [size][b][raw first value][codes] per non-empty
+ *                         range, nothing for an empty one;
+ *   DSS_GOLOMB_PIPELINED: AllToAllHashValuesPipeline::getEncoding
+ *                         (bloomfilter.hpp:322-331): [size][raw][codes] per
+ *                         range, [0] for an empty one.
+ * b[r] is range r's Golomb parameter (the policies' getB choices are the
+ * caller's); range_words[r] = words of range r's block. */
+#define DSS_GOLOMB_BLOCKS 0
+#define DSS_GOLOMB_PIPELINED 1
+int dss_golomb_encode(dss_codec c, int layout, const uint64_t* values, const uint64_t* counts, const uint64_t* b,
+                      uint32_t num_ranges, uint64_t* out, uint64_t out_cap, uint64_t* range_words,
+                      uint64_t* total_words);
+int dss_golomb_encode_dev(dss_codec c, int layout, const uint64_t* values, const uint64_t* counts, const uint64_t* b,
+                          uint32_t num_ranges, uint64_t* out, uint64_t out_cap, uint64_t* range_words,
+                          uint64_t* total_words);
+/* getDeltaDecoding (golomb_encoding.hpp:93-109) of every block of nwords
+ * words (the receive loops of bloomfilter.hpp:226-240 and 453-463): values
+ * of all blocks back to back, block_counts[j] = values of block j,
+ * *num_blocks = blocks found.  DSS_GOLOMB_BLOCKS reads b from each block and
+ * accepts at most max_blocks blocks; DSS_GOLOMB_PIPELINED reads exactly
+ * max_blocks blocks with parameter b_pipelined. */
+int dss_golomb_decode(dss_codec c, int layout, const uint64_t* words, uint64_t nwords, uint32_t max_blocks,
+                      uint64_t b_pipelined, uint64_t* values, uint64_t cap, uint64_t* block_counts,
+                      uint32_t* num_blocks);
+int dss_golomb_decode_dev(dss_codec c, int layout, const uint64_t* words, uint64_t nwords, uint32_t max_blocks,
+                          uint64_t b_pipelined, uint64_t* values, uint64_t cap, uint64_t* block_counts,
+                          uint32_t* num_blocks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1084,3 +1084,174 @@ ORC_API int orc_is_sorted(const uint8_t* const* strs, const uint64_t* pe_begin,
     }
     return 1;
 }
+
+/* ------------------------------------------------------------------------- */
+/* (next) Wire codecs.                                                        */
+/* ------------------------------------------------------------------------- */
+
+/* PutVarint (src/encoding/integer_compression.hpp:67-150): 7 value bits per  */
+/* byte, least significant group first, 0x80 on every byte but the last      */
+/* (at most 10 bytes).  IntegerCompression::writeRanges                       */
+/* (src/mpi/byte_encoder.hpp:31-53): the ranges are encoded back to back and  */
+/* range_bytes[r] is the byte count of range r.  Returns the total bytes.     */
+/* Pinned by: the reference's integerCompression_tests (round trip; the two  */
+/* ranges of 8 values use 37 bytes each) and byte-exact vectors from the      */
+/* reference header compiled in oracle/_ref.                                  */
+ORC_API uint64_t orc_varint_write_ranges(const uint64_t* values, const uint64_t* counts,
+                                         uint32_t num_ranges, uint8_t* out,
+                                         uint64_t* range_bytes) {
+    uint64_t o = 0, i = 0;
+    for (uint32_t r = 0; r < num_ranges; ++r) {
+        const uint64_t o0 = o;
+        for (uint64_t k = 0; k < counts[r]; ++k, ++i) {
+            uint64_t v = values[i];
+            while (v >= 128) {
+                out[o++] = (uint8_t)((v & 0x7F) | 0x80);
+                v >>= 7;
+            }
+            out[o++] = (uint8_t)v;
+        }
+        range_bytes[r] = o - o0;
+    }
+    return o;
+}
+
+/* GetVarint (src/encoding/integer_compression.hpp:209-236) over             */
+/* IntegerCompression::readRanges (src/mpi/byte_encoder.hpp:64-78): decodes   */
+/* sum(counts) values in order.  Returns the bytes consumed, or -1 when the   */
+/* input ends inside a value or a 10th byte carries bits beyond the 64th     */
+/* (the reference throws std::overflow_error there).                          */
+ORC_API int64_t orc_varint_read_ranges(const uint8_t* bytes, uint64_t nbytes,
+                                       const uint64_t* counts, uint32_t num_ranges,
+                                       uint64_t* values) {
+    uint64_t o = 0, i = 0;
+    for (uint32_t r = 0; r < num_ranges; ++r) {
+        for (uint64_t k = 0; k < counts[r]; ++k, ++i) {
+            uint64_t v = 0;
+            for (unsigned s = 0;; s += 7) {
+                if (o >= nbytes) return -1;
+                const uint8_t b = bytes[o++];
+                if (s == 63 && (b & 0xFE)) return -1;
+                v |= (uint64_t)(b & 0x7F) << s;
+                if (!(b & 0x80)) break;
+                if (s == 63) return -1;
+            }
+            values[i] = v;
+        }
+    }
+    return (int64_t)o;
+}
+
+/* getB (src/encoding/golomb_encoding.hpp:49-58): the power of two           */
+/* 2^floor(log2(max_value / num_values)) (double log2, as the reference), 1   */
+/* for no values or an average gap below 1.                                  */
+ORC_API uint64_t orc_golomb_b(uint64_t max_value, uint64_t num_values) {
+    if (num_values == 0) return 1;
+    const uint64_t gap = max_value / num_values;
+    if (gap < 1) return 1;
+    const uint64_t k = (uint64_t)floor(log2((double)gap));
+    return 1ull << k;
+}
+
+/* Bit writer of BitStreamWriter (src/encoding/bit_stream.hpp:40-110): bits   */
+/* fill 64-bit words from the most significant end.                           */
+typedef struct {
+    uint64_t* out;
+    uint64_t n;    /* words written */
+    uint64_t buf;
+    unsigned pos;  /* bits used in buf */
+} orc_bits;
+
+static void bits_put(orc_bits* w, uint64_t value, unsigned bits) {
+    for (unsigned k = bits; k > 0; --k) {  /* one bit at a time, MSB first */
+        const uint64_t bit = (value >> (k - 1)) & 1;
+        w->buf |= bit << (63 - w->pos);
+        if (++w->pos == 64) {
+            w->out[w->n++] = w->buf;
+            w->buf = 0;
+            w->pos = 0;
+        }
+    }
+}
+
+static unsigned ceil_log2_u64(uint64_t b) {  /* tlx::integer_log2_ceil */
+    unsigned k = 0;
+    while (k < 64 && (1ull << k) < b) ++k;
+    return k;
+}
+
+/* getDeltaEncoding (src/encoding/golomb_encoding.hpp:60-76) through         */
+/* DeltaStreamWriter (src/encoding/delta_stream.hpp:24-44) and                */
+/* GolombBitStreamWriter (src/encoding/golomb_bit_stream.hpp:27-136): the     */
+/* first value as one raw word, then for every further value the gap g to its */
+/* predecessor as q = g / b ones, a zero, and r = g - q*b in log2b - 1 bits   */
+/* if r < 2^log2b - b, else r + 2^log2b - b in log2b bits (log2b =            */
+/* ceil(log2 b)); the last word is filled up with ones.  Values ascending.    */
+/* Returns the words written.                                                 */
+/* Pinned by: word-exact vectors from the reference header compiled in       */
+/* oracle/_ref (several b, including non-powers of two and long unary runs),  */
+/* and the reference's pipelinedGolombEncoding_tests round trip.             */
+ORC_API uint64_t orc_golomb_encode(const uint64_t* v, uint64_t n, uint64_t b, uint64_t* out) {
+    if (n == 0) return 0;
+    orc_bits w = {out, 0, 0, 0};
+    out[w.n++] = v[0];
+    const unsigned log2b = ceil_log2_u64(b);
+    const uint64_t mlv = (log2b == 64 ? 0 : (1ull << log2b)) - b;
+    for (uint64_t i = 1; i < n; ++i) {
+        const uint64_t g = v[i] - v[i - 1];
+        uint64_t q = g / b;
+        const uint64_t r = g - q * b;
+        for (; q > 0; --q) bits_put(&w, 1, 1);
+        bits_put(&w, 0, 1);
+        if (r >= mlv)
+            bits_put(&w, r + mlv, log2b);
+        else
+            bits_put(&w, r, log2b - 1);
+    }
+    if (w.pos != 0) bits_put(&w, ~0ull >> w.pos, 64 - w.pos);
+    return w.n;
+}
+
+/* getDeltaDecoding (src/encoding/golomb_encoding.hpp:93-109) through         */
+/* GolombBitStreamReader (golomb_bit_stream.hpp:142-210) and BitStreamReader */
+/* (bit_stream.hpp:113-230): a value follows while a zero bit remains.        */
+/* Returns the number of values, or -1 if a code runs past the input or the   */
+/* output capacity.  (For b = 2 the reference reader's GetBits(0) shifts a    */
+/* 64-bit word by 64, which C++ leaves undefined; this restatement decodes    */
+/* the code the writer produced.)                                             */
+ORC_API int64_t orc_golomb_decode(const uint64_t* words, uint64_t nwords, uint64_t b,
+                                  uint64_t* out, uint64_t cap) {
+    if (nwords == 0) return 0;
+    if (cap == 0) return -1;
+    out[0] = words[0];
+    uint64_t n = 1, acc = words[0];
+    const uint64_t* bw = words + 1;
+    const uint64_t nbits = (nwords - 1) * 64;
+    const unsigned log2b = ceil_log2_u64(b);
+    const uint64_t mlv = (log2b == 64 ? 0 : (1ull << log2b)) - b;
+    uint64_t pos = 0;
+#define ORC_BIT(p) ((bw[(p) >> 6] >> (63 - ((p) & 63))) & 1)
+    for (;;) {
+        uint64_t z = pos;
+        while (z < nbits && ORC_BIT(z)) ++z;
+        if (z >= nbits) break;  /* only ones remain: the padding */
+        const uint64_t q = z - pos;
+        uint64_t p = z + 1, r = 0;
+        if (log2b > 0) {
+            if (p + (log2b - 1) > nbits) return -1;
+            for (unsigned k = 0; k + 1 < log2b; ++k) r = (r << 1) | ORC_BIT(p + k);
+            p += log2b - 1;
+            if (r >= mlv) {
+                if (p + 1 > nbits) return -1;
+                r = ((r << 1) | ORC_BIT(p)) - mlv;
+                p += 1;
+            }
+        }
+        if (n >= cap) return -1;
+        acc += q * b + r;
+        out[n++] = acc;
+        pos = p;
+    }
+#undef ORC_BIT
+    return (int64_t)n;
+}

@@ -77,6 +77,16 @@ def lib():
                                           C.c_uint64, C.c_uint32, u32p, u8p]
         L.orc_is_sorted.restype = C.c_int
         L.orc_is_sorted.argtypes = [C.POINTER(u8p), u64p, C.c_uint32]
+        L.orc_varint_write_ranges.restype = C.c_uint64
+        L.orc_varint_write_ranges.argtypes = [u64p, u64p, C.c_uint32, u8p, u64p]
+        L.orc_varint_read_ranges.restype = C.c_int64
+        L.orc_varint_read_ranges.argtypes = [u8p, C.c_uint64, u64p, C.c_uint32, u64p]
+        L.orc_golomb_b.restype = C.c_uint64
+        L.orc_golomb_b.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_golomb_encode.restype = C.c_uint64
+        L.orc_golomb_encode.argtypes = [u64p, C.c_uint64, C.c_uint64, u64p]
+        L.orc_golomb_decode.restype = C.c_int64
+        L.orc_golomb_decode.argtypes = [u64p, C.c_uint64, C.c_uint64, u64p, C.c_uint64]
         _lib = L
     return _lib
 
@@ -93,6 +103,16 @@ def ref():
                                 u64p, u64p, u64p]
         R.ref_extend_prefix.restype = C.c_uint64
         R.ref_extend_prefix.argtypes = [u8p, C.c_uint64, u64p, u64p, C.c_uint64, u8p]
+        R.ref_varint_write_ranges.restype = C.c_uint64
+        R.ref_varint_write_ranges.argtypes = [u64p, u64p, C.c_uint32, u8p, u64p]
+        R.ref_varint_read_ranges.restype = None
+        R.ref_varint_read_ranges.argtypes = [u8p, u64p, C.c_uint32, u64p]
+        R.ref_golomb_b.restype = C.c_uint64
+        R.ref_golomb_b.argtypes = [C.c_uint64, C.c_uint64]
+        R.ref_golomb_encode.restype = C.c_uint64
+        R.ref_golomb_encode.argtypes = [u64p, C.c_uint64, C.c_uint64, u64p]
+        R.ref_golomb_decode.restype = C.c_uint64
+        R.ref_golomb_decode.argtypes = [u64p, C.c_uint64, C.c_uint64, u64p]
         _ref = R
     return _ref
 
@@ -396,3 +416,116 @@ def prefix_doubling_sort(ss: StringSet, sampler="IndexedNumStrings", m=2**64 - 1
                          lcp=olcp, saved=saved))
     return dict(perm=perm, lcp=lcp, results=res, rounds=rounds, nrounds=nr,
                 interval=interval, exchange=ex, out=outs)
+
+
+# ---------------------------------------------------------------- wire codecs
+
+def varint_write_ranges(values, counts):
+    """IntegerCompression::writeRanges (src/mpi/byte_encoder.hpp:31-53):
+    (bytes, per-range byte counts)."""
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    c = np.ascontiguousarray(counts, dtype=np.uint64)
+    out = np.zeros(max(1, v.size * 10), dtype=np.uint8)
+    rb = np.zeros(max(1, c.size), dtype=np.uint64)
+    n = lib().orc_varint_write_ranges(P(v, u64p), P(c, u64p), c.size, P(out, u8p), P(rb, u64p))
+    return out[:n].copy(), rb[:c.size].copy()
+
+
+def varint_read_ranges(data, counts):
+    """IntegerCompression::readRanges (src/mpi/byte_encoder.hpp:64-78)."""
+    b = np.ascontiguousarray(data, dtype=np.uint8)
+    c = np.ascontiguousarray(counts, dtype=np.uint64)
+    out = np.zeros(max(1, int(c.sum())), dtype=np.uint64)
+    rc = lib().orc_varint_read_ranges(P(b if b.size else np.zeros(1, np.uint8), u8p), b.size,
+                                      P(c, u64p), c.size, P(out, u64p))
+    if rc < 0:
+        raise ValueError("malformed varint stream")
+    return out[:int(c.sum())].copy()
+
+
+def golomb_b(max_value, num_values):
+    """getB (src/encoding/golomb_encoding.hpp:49-58)."""
+    return int(lib().orc_golomb_b(max_value, num_values))
+
+
+def golomb_encode(values, b):
+    """getDeltaEncoding (src/encoding/golomb_encoding.hpp:60-76) of ascending
+    values: words (u64)."""
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    # raw word + per code at most (q + 1 + 64) bits, q summed over the codes
+    qsum = (int(v[-1]) - int(v[0])) // max(b, 1) if v.size else 0
+    out = np.zeros(2 + (qsum + v.size * 65) // 64 + 1, dtype=np.uint64)
+    n = lib().orc_golomb_encode(P(v if v.size else np.zeros(1, np.uint64), u64p), v.size, b,
+                                P(out, u64p))
+    return out[:n].copy()
+
+
+def golomb_decode(words, b, cap):
+    """getDeltaDecoding (src/encoding/golomb_encoding.hpp:93-109)."""
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+    out = np.zeros(max(1, cap), dtype=np.uint64)
+    n = lib().orc_golomb_decode(P(w if w.size else np.zeros(1, np.uint64), u64p), w.size, b,
+                                P(out, u64p), cap)
+    if n < 0:
+        raise ValueError("malformed Golomb stream")
+    return out[:n].copy()
+
+
+GOLOMB_ESCAPE_B = 20000000000  # b used when getB gives 1 (bloomfilter.hpp:196-197)
+
+
+def golomb_hash_b(filter_size, p, count):
+    """b of AllToAllHashesGolomb for hash intervals (bloomfilter.hpp:194-197)."""
+    b = golomb_b(filter_size // p, count)
+    return GOLOMB_ESCAPE_B if b == 1 else b
+
+
+def golomb_index_b(interval_max, count):
+    """b of AllToAllHashesGolomb for duplicate indices (bloomfilter.hpp:267-269)."""
+    return max(8, golomb_b(interval_max, count))
+
+
+def golomb_blocks(values, counts, bs):
+    """Send buffer of AllToAllHashesGolomb::alltoallv (bloomfilter.hpp:
+    164-215, 247-280): for every non-empty interval [size][b][encoding], size
+    counting b and the encoding; empty intervals send nothing.  Returns
+    (words, per-interval word counts)."""
+    out, sizes, i = [], [], 0
+    for c, b in zip(counts, bs):
+        c = int(c)
+        if c == 0:
+            sizes.append(0)
+            continue
+        enc = golomb_encode(values[i:i + c], b)
+        out += [1 + enc.size, b] + [int(x) for x in enc]
+        sizes.append(2 + enc.size)
+        i += c
+    return np.array(out, dtype=np.uint64), np.array(sizes, dtype=np.uint64)
+
+
+def golomb_blocks_decode(words, cap):
+    """Receive side of AllToAllHashesGolomb::alltoallv (bloomfilter.hpp:
+    226-240, 285-297): blocks [size][b][encoding] back to back.  Returns the
+    decoded values and the value count of every block."""
+    vals, counts, k = [], [], 0
+    w = np.asarray(words, dtype=np.uint64)
+    while k < w.size:
+        size, b = int(w[k]), int(w[k + 1])
+        d = golomb_decode(w[k + 2:k + 1 + size], b, cap)
+        vals += [int(x) for x in d]
+        counts.append(d.size)
+        k += 1 + size
+    return np.array(vals, dtype=np.uint64), np.array(counts, dtype=np.uint64)
+
+
+def golomb_pipeline_blocks(values, counts, b=1048576):
+    """AllToAllHashValuesPipeline::getEncoding per partner (bloomfilter.hpp:
+    322-331): [size][encoding] for every interval, [0] for an empty one."""
+    out, sizes, i = [], [], 0
+    for c in counts:
+        c = int(c)
+        enc = golomb_encode(values[i:i + c], b) if c else np.zeros(0, np.uint64)
+        out += [enc.size] + [int(x) for x in enc]
+        sizes.append(1 + enc.size)
+        i += c
+    return np.array(out, dtype=np.uint64), np.array(sizes, dtype=np.uint64)

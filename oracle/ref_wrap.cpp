@@ -6,7 +6,11 @@
 //     XXHasher::hash (src/sorter/distributed/bloomfilter.hpp:1094-1102);
 //   - dss_schimek::LcpStringLoserTree_ (src/merge/bingmann-lcp_losertree.hpp:
 //     204-480) driven like merge<K> (src/sorter/distributed/merging.hpp:23-63);
-//   - StringLcpContainer::extendPrefix (src/strings/stringcontainer.hpp:247-319).
+//   - StringLcpContainer::extendPrefix (src/strings/stringcontainer.hpp:247-319);
+//   - IntegerCompression::writeRanges / readRanges (src/mpi/byte_encoder.hpp:
+//     31-78) over PutVarint / GetVarint (src/encoding/integer_compression.hpp);
+//   - getB, getDeltaEncoding, getDeltaDecoding (src/encoding/golomb_encoding.hpp:
+//     49-109) over the Golomb and delta bit streams.
 // The tlx radix sort (insertion_sort.hpp needs tlx/meta/enable_if.hpp, which
 // the vendored tlx lacks) and the MPI-bound drivers are not built.
 #include <cstdint>
@@ -16,6 +20,8 @@
 #include "hash/xxhash.hpp"
 #include "merge/bingmann-lcp_losertree.hpp"
 #include "strings/stringcontainer.hpp"
+#include "encoding/golomb_encoding.hpp"
+#include "mpi/byte_encoder.hpp"
 
 using SSet = dss_schimek::UCharLengthStringSet;
 using Cont = dss_schimek::StringLcpContainer<SSet>;
@@ -105,4 +111,50 @@ extern "C" uint64_t ref_extend_prefix(const uint8_t* bytes, uint64_t nbytes,
     uint64_t total = cont.char_size();
     if (out) std::memcpy(out, cont.raw_strings().data(), total);
     return total;
+}
+
+// Varint ranges: writes the encoding into out (capacity 10 bytes per value)
+// and the byte count of every range; returns the total bytes.
+extern "C" uint64_t ref_varint_write_ranges(const uint64_t* values, const uint64_t* counts,
+                                            uint32_t num_ranges, uint8_t* out,
+                                            uint64_t* range_bytes) {
+    std::vector<uint64_t> c(counts, counts + num_ranges);
+    uint64_t n = 0;
+    for (uint64_t x : c) n += x;
+    std::vector<uint64_t> v(values, values + n);
+    auto res = dss_schimek::IntegerCompression::writeRanges(c.begin(), c.end(), v.begin());
+    uint64_t total = 0;
+    for (uint32_t r = 0; r < num_ranges; ++r) {
+        range_bytes[r] = res.counts[r];
+        total += res.counts[r];
+    }
+    std::memcpy(out, res.integers.data(), total);
+    return total;
+}
+
+extern "C" void ref_varint_read_ranges(const uint8_t* bytes, const uint64_t* counts,
+                                       uint32_t num_ranges, uint64_t* values) {
+    std::vector<uint64_t> c(counts, counts + num_ranges);
+    auto out = dss_schimek::IntegerCompression::readRanges(c.begin(), c.end(), bytes);
+    std::memcpy(values, out.data(), out.size() * sizeof(uint64_t));
+}
+
+extern "C" uint64_t ref_golomb_b(uint64_t max_value, uint64_t num_values) {
+    return getB(max_value, num_values);
+}
+
+// Delta + Golomb encoding of ascending values; returns the words written.
+extern "C" uint64_t ref_golomb_encode(const uint64_t* v, uint64_t n, uint64_t b, uint64_t* out) {
+    std::vector<size_t> in(v, v + n), enc;
+    getDeltaEncoding(in.begin(), in.end(), std::back_inserter(enc), b);
+    std::memcpy(out, enc.data(), enc.size() * sizeof(uint64_t));
+    return enc.size();
+}
+
+extern "C" uint64_t ref_golomb_decode(const uint64_t* words, uint64_t nwords, uint64_t b,
+                                      uint64_t* out) {
+    std::vector<size_t> in(words, words + nwords), dec;
+    getDeltaDecoding(in.begin(), in.end(), std::back_inserter(dec), b);
+    std::memcpy(out, dec.data(), dec.size() * sizeof(uint64_t));
+    return dec.size();
 }
