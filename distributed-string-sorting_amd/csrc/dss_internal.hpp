@@ -197,6 +197,8 @@ struct PipeState {
     bool extended = false;
     uint64_t out_nbytes = 0;
     DevBuf out_bytes, out_off, out_psv, out_aux;
+    // exact distinguishing prefixes of the merged output
+    DevBuf exact, exact_glcp, exact_pos, exact_cnt;
     // generic scratch
     DevBuf scan_tmp, scan_tot;
 };
@@ -222,6 +224,7 @@ struct dss_set_s {
     hipStream_t stream = nullptr;
     uint64_t n = 0;        // strings
     uint64_t nbytes = 0;   // arena bytes (incl. NULs)
+    uint64_t slice_bytes = 0;  // bytes the PE split slices (0: nbytes); a text file's size
     uint32_t p = 1;        // virtual PEs
     uint32_t max_len = 0;
     std::vector<uint64_t> pe_begin;  // host copy, p+1

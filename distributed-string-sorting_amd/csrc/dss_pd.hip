@@ -1078,13 +1078,16 @@ int dss_hash_prefixes(dss_set s, const uint32_t* ids, uint64_t count, uint32_t d
 int dss_merge_sort(dss_set s, int sampler, int compressed) {
     return guarded([&] {
         DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        DSS_REQUIRE(compressed >= 0 && compressed <= 2, DSS_EINVAL, "compressed must be 0, 1 or 2");
         sort_lcp(s);
         if (s->p == 1) return;
-        const uint64_t avg = avg_lcp(s) + 5;  // lcpSummand, lcp_aware...:207,236
+        // lcpSummand (lcp_aware...:207,236); the NoLcps variant sorts without
+        // LCPs (:218-224), so getAvgLcp reads the zero-initialised array
+        const uint64_t avg = (compressed == 2 ? 0 : avg_lcp(s)) + 5;
         partition(s, sampler, 2, 2 * 100 * avg, false);
-        exchange(s, compressed ? 1 : 0);
+        exchange(s, compressed == 2 ? 3 : compressed);
         merge(s);
-        if (compressed) extend_prefix(s);
+        if (compressed == 1) extend_prefix(s);
     });
 }
 

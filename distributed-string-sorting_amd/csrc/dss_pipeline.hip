@@ -455,7 +455,7 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
     const uint32_t L = s_len[i];
     uint64_t sz;
     uint32_t from = 0, vlen = L;
-    if (mode == 0) {
+    if (mode == 0 || mode == 3) {
         sz = (uint64_t)L + 1;
     } else if (mode == 1) {
         from = l;
@@ -466,7 +466,7 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
         sz = (uint64_t)vlen - l + 1;
     }
     size[dst] = sz;
-    r_lcp[dst] = l;
+    r_lcp[dst] = mode == 3 && j != st ? kNone : l;  // mode 3 sends no LCPs
     r_pe[dst] = r;
     r_pos[dst] = (uint32_t)j;
     r_voff[dst] = s_off[i];
@@ -520,6 +520,29 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
     }
 }
 
+// Mode 3 (AllToAllStringImpl<..., NoLcps>, src/mpi/alltoall.hpp:567-640)
+// sends whole strings and no LCPs; the receiver derives the LCP of every
+// received string with its predecessor in the same run from the receive
+// arena (8 characters per step), so the LCP merge can order the runs.
+__global__ void k_recv_lcp(const uint8_t* __restrict__ bytes, const uint64_t* __restrict__ byte_off,
+                           const uint32_t* __restrict__ vlen, uint64_t n, uint32_t* __restrict__ r_lcp) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+        if (r_lcp[k] != kNone) continue;  // a run start (LCP 0)
+        const uint64_t a = byte_off[k - 1], b = byte_off[k];
+        const uint32_t m = min(vlen[k - 1], vlen[k]);
+        uint32_t d = 0;
+        while (d < m) {
+            const uint64_t x = load_word_le(bytes, a + d) ^ load_word_le(bytes, b + d);
+            if (x) {
+                d += __builtin_ctzll(x) / 8;
+                break;
+            }
+            d += 8;
+        }
+        r_lcp[k] = min(d, m);
+    }
+}
+
 __global__ void k_byte_begin(const uint64_t* __restrict__ recv_begin, const uint64_t* __restrict__ byte_off,
                              uint64_t n, uint64_t total, uint32_t p, uint64_t* __restrict__ out) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -531,7 +554,7 @@ __global__ void k_byte_begin(const uint64_t* __restrict__ recv_begin, const uint
 void exchange(dss_set_s* s, int mode) {
     PipeState& P = s->pp;
     DSS_REQUIRE(P.partitioned, DSS_ESTATE, "dss_partition has not run");
-    DSS_REQUIRE(mode >= 0 && mode <= 2, DSS_EINVAL, "mode must be 0..2");
+    DSS_REQUIRE(mode >= 0 && mode <= 3, DSS_EINVAL, "mode must be 0..3");
     DSS_REQUIRE(mode != 2 || P.has_dist, DSS_ESTATE, "prefix doubling has not run");
     const hipStream_t st = s->stream;
     const uint32_t p = s->p;
@@ -595,6 +618,12 @@ void exchange(dss_set_s* s, int mode) {
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                            P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), mode, n, lg,
                            P.recv_bytes.as<uint8_t>());
+    }
+    if (n && mode == 3) {
+        KScope ks(s->timer, st, "exchange_lcp", 2.0 * total + n * 20.0);
+        hipLaunchKernelGGL(k_recv_lcp, dim3(stride_grid(n, 256)), dim3(256), 0, st, P.recv_bytes.as<uint8_t>(),
+                           P.recv_byte_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(), n,
+                           P.recv_lcp.as<uint32_t>());
     }
     DSS_HIP(hipGetLastError());
     // byte_begin per receiver: the byte offset of its first received string
@@ -1258,7 +1287,132 @@ __global__ void k_popc(const unsigned int* __restrict__ w, uint64_t nw, unsigned
     if (lane_id() == 0 && t) atomicAdd(out, (unsigned long long)t);
 }
 
+// ExcatDistinguishingPrefix::filter_exact (src/sorter/distributed/
+// bloomfilter.hpp:847-965) on the globally sorted output: the reference
+// allgathers every candidate string, sorts them and gives each string
+// 1 + the larger LCP with its neighbours in that order.  The merge already
+// holds the global order, so one lane per output position compares its full
+// string (through its origin, not the possibly truncated received view) with
+// its predecessor, and a second pass takes the maximum of both neighbours.
+__global__ void k_full_glcp(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ rpe,
+                            const uint32_t* __restrict__ rpos, const uint64_t* __restrict__ pe_begin,
+                            const uint64_t* __restrict__ s_off, const uint32_t* __restrict__ s_len,
+                            const uint8_t* __restrict__ chars, uint64_t n, uint32_t* __restrict__ glcp,
+                            uint64_t* __restrict__ gpos) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t a = ord[i];
+    const uint64_t pa = pe_begin[rpe[a]] + rpos[a];
+    gpos[i] = pa;
+    if (i == 0) {
+        glcp[0] = 0;
+        return;
+    }
+    const uint32_t b = ord[i - 1];
+    const uint64_t pb = pe_begin[rpe[b]] + rpos[b];
+    uint32_t l = 0;
+    (void)view_cmp(chars, s_off[pb], s_len[pb], s_off[pa], s_len[pa], 0, &l);
+    glcp[i] = l;
+}
+
+__global__ void k_exact_dist(const uint32_t* __restrict__ glcp, const uint64_t* __restrict__ gpos, uint64_t n,
+                             uint32_t* __restrict__ exact) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t m = 0;
+    if (i > 0) m = glcp[i];
+    if (i + 1 < n) m = max(m, glcp[i + 1]);
+    exact[gpos[i]] = n < 2 ? 0 : m + 1;  // one string: no neighbour pair
+}
+
+// Bounds of prefix doubling's results against the exact lengths: cnt[0]
+// strings with results below DIST (below DIST - 1 for a string whose result
+// is its whole length), cnt[1] strings cut before their end whose result is
+// not below 2 * DIST + 8 (the doubling schedule's bound when no hashes
+// collide), cnt[2] sum of DIST, cnt[3] sum of results.
+__global__ void k_dist_bounds(const uint32_t* __restrict__ exact, const uint32_t* __restrict__ res,
+                              const uint32_t* __restrict__ s_len, uint64_t n, unsigned long long* __restrict__ cnt) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t lo = 0, hi = 0, sd = 0, sr = 0;
+    if (i < n) {
+        const uint64_t d = exact[i], r = res[i], L = s_len[i];
+        lo = d > r + (r >= L ? 1 : 0);
+        hi = r < L && r >= 2 * d + 8;
+        sd = d;
+        sr = r;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo += __shfl_down((unsigned long long)lo, o);
+        hi += __shfl_down((unsigned long long)hi, o);
+        sd += __shfl_down((unsigned long long)sd, o);
+        sr += __shfl_down((unsigned long long)sr, o);
+    }
+    if (lane_id() == 0) {
+        if (lo) atomicAdd(&cnt[0], (unsigned long long)lo);
+        if (hi) atomicAdd(&cnt[1], (unsigned long long)hi);
+        if (sd) atomicAdd(&cnt[2], (unsigned long long)sd);
+        if (sr) atomicAdd(&cnt[3], (unsigned long long)sr);
+    }
+}
+
+static void exact_dist(dss_set_s* s) {
+    PipeState& P = s->pp;
+    DSS_REQUIRE(P.merged, DSS_ESTATE, "no merged output (dss_merge_sort / dss_prefix_doubling_sort)");
+    const uint64_t n = s->n;
+    const uint64_t nn = std::max<uint64_t>(n, 1);
+    P.exact.reserve(nn * 4);
+    P.exact_glcp.reserve(nn * 4);
+    P.exact_pos.reserve(nn * 8);
+    if (!n) return;
+    const hipStream_t st = s->stream;
+    KScope ks(s->timer, st, "exact_dist", n * 48.0);
+    hipLaunchKernelGGL(k_full_glcp, dim3((n + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
+                       P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>(), s->pe_begin_d.as<uint64_t>(),
+                       s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(), s->arena(), n, P.exact_glcp.as<uint32_t>(),
+                       P.exact_pos.as<uint64_t>());
+    hipLaunchKernelGGL(k_exact_dist, dim3((n + 255) / 256), dim3(256), 0, st, P.exact_glcp.as<uint32_t>(),
+                       P.exact_pos.as<uint64_t>(), n, P.exact.as<uint32_t>());
+    DSS_HIP(hipGetLastError());
+}
+
 }  // namespace dss
+
+extern "C" int dss_exact_dist(dss_set s, uint32_t* exact) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        exact_dist(s);
+        if (s->n && exact)
+            DSS_HIP(hipMemcpyAsync(exact, s->pp.exact.p, s->n * 4, hipMemcpyDeviceToHost, s->stream));
+        DSS_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+extern "C" int dss_check_dist(dss_set s, dss_dist_check_t* out) {
+    return guarded([&] {
+        DSS_REQUIRE(s && out, DSS_EINVAL, "NULL argument");
+        PipeState& P = s->pp;
+        DSS_REQUIRE(P.has_dist, DSS_ESTATE, "no prefix-doubling results");
+        exact_dist(s);
+        memset(out, 0, sizeof(*out));
+        const uint64_t n = s->n;
+        out->num_strings = n;
+        if (!n) return;
+        const hipStream_t st = s->stream;
+        P.exact_cnt.reserve(32);
+        DSS_HIP(hipMemsetAsync(P.exact_cnt.p, 0, 32, st));
+        hipLaunchKernelGGL(k_dist_bounds, dim3((n + 255) / 256), dim3(256), 0, st, P.exact.as<uint32_t>(),
+                           P.dist.as<uint32_t>(), s->s_len.as<uint32_t>(), n,
+                           P.exact_cnt.as<unsigned long long>());
+        DSS_HIP(hipGetLastError());
+        unsigned long long h[4];
+        DSS_HIP(hipMemcpyAsync(h, P.exact_cnt.p, 32, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+        out->lower_violations = h[0];
+        out->upper_violations = h[1];
+        out->exact_sum = h[2];
+        out->results_sum = h[3];
+    });
+}
 
 extern "C" int dss_check(dss_set s, dss_check_t* out) {
     return guarded([&] {

@@ -242,7 +242,7 @@ void compute_pe_split(dss_set_s* s) {
         off = dummy.as<uint64_t>();
     }
     hipLaunchKernelGGL(k_pe_split, dim3((s->p + 1 + 255) / 256), dim3(256), 0, s->stream, off, s->n,
-                       s->nbytes, s->p, s->pe_begin_d.as<uint64_t>());
+                       s->slice_bytes ? s->slice_bytes : s->nbytes, s->p, s->pe_begin_d.as<uint64_t>());
     s->pe_begin.assign(s->p + 1, 0);
     DSS_HIP(hipMemcpyAsync(s->pe_begin.data(), s->pe_begin_d.p, (s->p + 1) * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, s->stream));
@@ -571,6 +571,113 @@ int dss_set_create(const uint8_t* chars, uint64_t nbytes, uint32_t num_pes, dss_
             delete s;
             throw;
         }
+        *out = s;
+    });
+}
+
+// ------------------------------------------------------ text input
+// distribute_strings (src/mpi/readInput.hpp:40-104) on a '\n'-separated
+// text: every '\n' becomes a NUL (8 bytes per lane: the bytes equal to '\n'
+// are found with an exact zero-byte test on x ^ 0x0A..0A), a NUL is appended
+// when the text does not end with one, and the PE split slices the text's
+// bytes (not the appended NUL), as the MPI ranks slice the file.
+__global__ void k_newline_to_nul(uint8_t* __restrict__ a, uint64_t nbytes) {
+    const uint64_t words = nbytes / 8;
+    uint64_t* w = reinterpret_cast<uint64_t*>(a);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr uint64_t lo7 = 0x7F7F7F7F7F7F7F7Full;
+    for (uint64_t i = gid; i < words; i += stride) {
+        const uint64_t x = w[i];
+        const uint64_t t = x ^ 0x0A0A0A0A0A0A0A0Aull;
+        const uint64_t z = ~(((t & lo7) + lo7) | t | lo7);  // 0x80 in every byte equal to '\n'
+        if (z) w[i] = x & ~((z >> 7) * 0xFF);
+    }
+    for (uint64_t j = words * 8 + gid; j < nbytes; j += stride)
+        if (a[j] == '\n') a[j] = 0;
+}
+
+// The first L text bytes are in the arena, which is zeroed from L on.
+static void finish_text(dss_set_s* s, uint64_t L, uint8_t last) {
+    if (L)
+        hipLaunchKernelGGL(k_newline_to_nul, dim3(stride_grid((L + 7) / 8, 256)), dim3(256), 0, s->stream,
+                           s->chars.as<uint8_t>(), L);
+    DSS_HIP(hipGetLastError());
+    s->nbytes = L + (L && last != '\n' && last != 0 ? 1 : 0);
+    s->slice_bytes = L;
+    index_arena(s);
+    compute_pe_split(s);
+}
+
+int dss_set_create_text(const uint8_t* text, uint64_t text_len, uint64_t max_size, uint32_t num_pes,
+                        dss_set* out) {
+    return guarded([&] {
+        DSS_REQUIRE(out != nullptr, DSS_EINVAL, "out is NULL");
+        DSS_REQUIRE(text_len == 0 || text != nullptr, DSS_EINVAL, "text is NULL");
+        const uint64_t L = max_size ? std::min(max_size, text_len) : text_len;
+        dss_set_s* s = new_set(num_pes);
+        try {
+            s->chars.reserve(L + 1 + kArenaPad);
+            DSS_HIP(hipMemsetAsync(s->chars.as<uint8_t>() + L, 0, 1 + kArenaPad, s->stream));
+            if (L) DSS_HIP(hipMemcpyAsync(s->chars.p, text, L, hipMemcpyHostToDevice, s->stream));
+            finish_text(s, L, L ? text[L - 1] : 0);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int dss_set_load_file(const char* path, uint64_t max_size, uint32_t num_pes, dss_set* out) {
+    return guarded([&] {
+        DSS_REQUIRE(out != nullptr && path != nullptr, DSS_EINVAL, "NULL argument");
+        FILE* f = fopen(path, "rb");
+        DSS_REQUIRE(f != nullptr, DSS_EINVAL, std::string("cannot open ") + path);
+        dss_set_s* s = nullptr;
+        uint8_t* pin[2] = {nullptr, nullptr};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        auto cleanup = [&] {
+            for (int k = 0; k < 2; ++k) {
+                if (ev[k]) (void)hipEventDestroy(ev[k]);
+                if (pin[k]) (void)hipHostFree(pin[k]);
+            }
+            fclose(f);
+        };
+        try {
+            DSS_REQUIRE(fseeko(f, 0, SEEK_END) == 0, DSS_EINVAL, "cannot seek the input file");
+            const off_t size = ftello(f);
+            DSS_REQUIRE(size >= 0 && fseeko(f, 0, SEEK_SET) == 0, DSS_EINVAL, "cannot size the input file");
+            const uint64_t L = max_size ? std::min<uint64_t>(max_size, (uint64_t)size) : (uint64_t)size;
+            s = new_set(num_pes);
+            s->chars.reserve(L + 1 + kArenaPad);
+            DSS_HIP(hipMemsetAsync(s->chars.as<uint8_t>() + L, 0, 1 + kArenaPad, s->stream));
+            // file -> pinned staging (two 64 MiB buffers) -> HBM, reads overlapping copies
+            constexpr uint64_t kStage = 64ull << 20;
+            const uint64_t stage = std::min<uint64_t>(kStage, std::max<uint64_t>(L, 1));
+            for (int k = 0; k < 2; ++k) {
+                DSS_HIP(hipHostMalloc((void**)&pin[k], stage, hipHostMallocDefault));
+                DSS_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+            }
+            uint8_t last = 0;
+            uint64_t i = 0;
+            for (uint64_t o = 0; o < L; o += stage, ++i) {
+                const uint64_t k = std::min(stage, L - o);
+                uint8_t* b = pin[i & 1];
+                if (i >= 2) DSS_HIP(hipEventSynchronize(ev[i & 1]));
+                DSS_REQUIRE(fread(b, 1, k, f) == k, DSS_EINVAL, "short read of the input file");
+                last = b[k - 1];
+                DSS_HIP(hipMemcpyAsync(s->chars.as<uint8_t>() + o, b, k, hipMemcpyHostToDevice, s->stream));
+                DSS_HIP(hipEventRecord(ev[i & 1], s->stream));
+            }
+            DSS_HIP(hipStreamSynchronize(s->stream));
+            finish_text(s, L, last);
+        } catch (...) {
+            cleanup();
+            delete s;
+            throw;
+        }
+        cleanup();
         *out = s;
     });
 }

@@ -45,6 +45,23 @@ class DeviceStringSet:
         return cls(h)
 
     @classmethod
+    def from_text(cls, text, num_pes: int, max_size: int = 0) -> "DeviceStringSet":
+        """FileDistributer on a '\\n'-separated text held in host memory."""
+        t = np.frombuffer(text, dtype=np.uint8) if isinstance(text, (bytes, bytearray)) else \
+            np.ascontiguousarray(text, dtype=np.uint8)
+        h = _lib.set_p()
+        check(lib().dss_set_create_text(_p(t, _lib.u8p) if t.size else None, t.size, max_size, num_pes,
+                                        C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_file(cls, path: str, num_pes: int, max_size: int = 0) -> "DeviceStringSet":
+        """FileDistributer(path) (src/util/random_string_generator.hpp:50-62)."""
+        h = _lib.set_p()
+        check(lib().dss_set_load_file(path.encode(), max_size, num_pes, C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def from_pes(cls, pe_arenas) -> "DeviceStringSet":
         """One NUL-terminated arena per PE (the per-rank inputs)."""
         arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in pe_arenas]
@@ -191,7 +208,7 @@ class DeviceStringSet:
         return out.value
 
     SAMPLERS = {"NumStrings": 0, "NumChars": 1, "IndexedNumStrings": 2, "IndexedNumChars": 3}
-    MODES = {"whole": 0, "lcp_compact": 1, "pd_prefix": 2}
+    MODES = {"whole": 0, "lcp_compact": 1, "pd_prefix": 2, "no_lcps": 3}
 
     def partition(self, sampler="IndexedNumStrings", max_len=0, use_dist=False, factor=2):
         p = self.p
@@ -266,9 +283,15 @@ class DeviceStringSet:
         return out[:ids.size]
 
     # ------------------------------------------------------------ drivers
+    ENCODERS = {"whole": 0, "lcp_compact": 1, "no_lcps": 2}
+
     def merge_sort(self, sampler="IndexedNumStrings", compressed=True):
-        """DistributedMergeSort::sort (+ extendPrefix when compressed)."""
-        check(lib().dss_merge_sort(self._h, self.SAMPLERS.get(sampler, sampler), int(compressed)))
+        """DistributedMergeSort::sort (+ extendPrefix when compressed).
+        compressed: True/"lcp_compact" (EmptyLcpByteEncoderMemCpy +
+        extendPrefix), False/"whole" (EmptyByteEncoderMemCpy), "no_lcps"
+        (the NoLcps variant with noLcpMerge)."""
+        mode = self.ENCODERS[compressed] if isinstance(compressed, str) else int(bool(compressed))
+        check(lib().dss_merge_sort(self._h, self.SAMPLERS.get(sampler, sampler), mode))
 
     def prefix_doubling_sort(self, sampler="IndexedNumStrings", m=2**64 - 1):
         """DistributedPrefixDoublingSort::sort."""
@@ -288,6 +311,21 @@ class DeviceStringSet:
         return dict(num_strings=r.num_strings, order_errors=r.order_errors,
                     lcp_errors=r.lcp_errors, D=r.dist_prefix_size, sorted=bool(r.sorted),
                     complete=bool(r.complete))
+
+    def exact_dist(self):
+        """ExcatDistinguishingPrefix::filter_exact: exact distinguishing-prefix
+        length per position (index space of get_prefix_doubling's results)."""
+        out = np.zeros(max(self.n, 1), dtype=np.uint32)
+        check(lib().dss_exact_dist(self._h, _p(out, _lib.u32p)))
+        return out[:self.n]
+
+    def check_dist(self) -> dict:
+        """Bounds of the prefix-doubling results against the exact lengths."""
+        r = _lib.DistCheck()
+        check(lib().dss_check_dist(self._h, C.byref(r)))
+        return dict(num_strings=r.num_strings, lower_violations=r.lower_violations,
+                    upper_violations=r.upper_violations, exact_sum=r.exact_sum,
+                    results_sum=r.results_sum)
 
     # ------------------------------------------------------------ timing
     def sync(self):

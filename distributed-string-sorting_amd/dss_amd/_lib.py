@@ -40,6 +40,12 @@ class CheckResult(C.Structure):
                 ("sorted", C.c_int32), ("complete", C.c_int32)]
 
 
+class DistCheck(C.Structure):
+    _fields_ = [("num_strings", C.c_uint64), ("lower_violations", C.c_uint64),
+                ("upper_violations", C.c_uint64), ("exact_sum", C.c_uint64),
+                ("results_sum", C.c_uint64)]
+
+
 class DssError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"dss error {code}: {msg}")
@@ -58,6 +64,8 @@ _SIGS = {
     "dss_set_create_skewed": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                         C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
                                         C.POINTER(set_p)]),
+    "dss_set_create_text": (C.c_int, [u8p, C.c_uint64, C.c_uint64, C.c_uint32, C.POINTER(set_p)]),
+    "dss_set_load_file": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint32, C.POINTER(set_p)]),
     "dss_set_create_pes": (C.c_int, [u8p, u64p, C.c_uint32, C.POINTER(set_p)]),
     "dss_set_create_suffixes": (C.c_int, [u8p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(set_p)]),
     "dss_set_create_markov_suffixes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -89,6 +97,8 @@ _SIGS = {
     "dss_prefix_doubling_sort": (C.c_int, [set_p, C.c_int, C.c_uint64]),
     "dss_get_permutation": (C.c_int, [set_p, u32p, u32p]),
     "dss_check": (C.c_int, [set_p, C.POINTER(CheckResult)]),
+    "dss_exact_dist": (C.c_int, [set_p, u32p]),
+    "dss_check_dist": (C.c_int, [set_p, C.POINTER(DistCheck)]),
     "dss_sync": (C.c_int, [set_p]),
     "dss_set_timing": (C.c_int, [set_p, C.c_int]),
     "dss_kernel_stats": (C.c_int, [set_p, C.POINTER(KernelStat), C.c_uint32, u32p, C.c_int]),

@@ -55,6 +55,19 @@ int dss_device_info(char* buf, size_t buflen);
  * into num_pes virtual PEs by the reference's byte-slice rule. */
 int dss_set_create(const uint8_t* chars, uint64_t nbytes, uint32_t num_pes, dss_set* out);
 
+/* FileDistributer / distribute_strings (src/util/random_string_generator.hpp:
+ * 50-62, src/mpi/readInput.hpp:40-104): a '\n'-separated text (the first
+ * max_size bytes if max_size > 0), '\n' -> NUL on the device, a NUL appended
+ * to an unterminated last line, split into num_pes PEs by byte slices of the
+ * text as the MPI ranks slice the file: a string belongs to the PE whose
+ * slice holds the byte before its start.  (A slice without any '\n', i.e. a
+ * line longer than the slice, is undefined in the reference: its shift_left
+ * reads past the slice.)  _text takes a host buffer, _load_file reads the
+ * file through pinned staging buffers. */
+int dss_set_create_text(const uint8_t* text, uint64_t text_len, uint64_t max_size, uint32_t num_pes,
+                        dss_set* out);
+int dss_set_load_file(const char* path, uint64_t max_size, uint32_t num_pes, dss_set* out);
+
 /* Builds the D/N-ratio input on the device (DNRatioGenerator,
  * src/util/random_string_generator.hpp:166-287, seeded): n strings of
  * max(len, k) chars, k = max(len*dton, ceil(log26 n)). */
@@ -172,11 +185,14 @@ int dss_set_interval(dss_set s, const uint64_t* interval);
 int dss_set_dist(dss_set s, const uint32_t* dist);
 
 /* ---- (a6, a7, a13) AllToAllStringImpl<..., EmptyByteEncoderMemCpy> (mode 0),
- *      <..., EmptyLcpByteEncoderMemCpy> (mode 1) and
- *      AllToAllStringImplPrefixDoubling (mode 2) (src/mpi/alltoall.hpp:642-804,
- *      902-1009, encoders src/mpi/byte_encoder.hpp:81-172), for p virtual
- *      PEs as an in-HBM bucket scatter.  recv_begin / byte_begin (optional,
- *      p+1 entries) give each receiving PE's strings and arena bytes ---- */
+ *      <..., EmptyLcpByteEncoderMemCpy> (mode 1),
+ *      AllToAllStringImplPrefixDoubling (mode 2) and
+ *      AllToAllStringImpl<..., NoLcps> (mode 3: mode 0's bytes, no LCPs sent;
+ *      the receiver derives its runs' LCPs from the received strings)
+ *      (src/mpi/alltoall.hpp:567-804, 902-1009, encoders
+ *      src/mpi/byte_encoder.hpp:81-172), for p virtual PEs as an in-HBM
+ *      bucket scatter.  recv_begin / byte_begin (optional, p+1 entries) give
+ *      each receiving PE's strings and arena bytes ---- */
 int dss_exchange(dss_set s, int mode, uint64_t* recv_begin, uint64_t* byte_begin,
                  uint64_t* recv_nbytes);
 /* Receive arena bytes, received LCPs (0 at every run start) and each
@@ -218,7 +234,9 @@ int dss_hash_prefixes(dss_set s, const uint32_t* ids, uint64_t count, uint32_t d
 /* ---- drivers ---- */
 
 /* DistributedMergeSort::sort (lcp_aware_number_string_splitting.hpp:
- * 189-289) plus extendPrefix when compressed (mode 1 exchange). */
+ * 189-289): compressed = 0 plain exchange (EmptyByteEncoderMemCpy) and LCP
+ * merge; 1 LCP-compressed exchange and extendPrefix; 2 the NoLcps variant
+ * (no LCPs sent, noLcpMerge, :139-182, 218-224, 276-281). */
 int dss_merge_sort(dss_set s, int sampler, int compressed);
 /* DistributedPrefixDoublingSort::sort (prefix_doubling.hpp:204-294). */
 int dss_prefix_doubling_sort(dss_set s, int sampler, uint64_t m);
@@ -240,6 +258,28 @@ typedef struct {
 } dss_check_t;
 
 int dss_check(dss_set s, dss_check_t* out);
+
+/* ---- ExcatDistinguishingPrefix::filter_exact
+ *      (src/sorter/distributed/bloomfilter.hpp:847-965) on the globally
+ *      sorted output of dss_merge_sort / dss_prefix_doubling_sort ----
+ * Per position (PE r's sorted index j at pe_begin[r] + j: the index space of
+ * dss_get_prefix_doubling's results) the exact distinguishing-prefix length:
+ * 1 + the larger full-string LCP with its neighbours in the global order, 0
+ * when the set holds one string.  exact may be NULL (device only). */
+int dss_exact_dist(dss_set s, uint32_t* exact);
+
+typedef struct {
+    uint64_t num_strings;
+    uint64_t lower_violations;  /* results < DIST (< DIST - 1 where results = length) */
+    uint64_t upper_violations;  /* results < length and results >= 2 * DIST + 8 */
+    uint64_t exact_sum;         /* sum of DIST (= D of the output) */
+    uint64_t results_sum;       /* sum of results */
+} dss_dist_check_t;
+
+/* Device validator of prefix doubling's results against the exact lengths:
+ * a result is an upper bound of DIST, and without hash collisions the
+ * doubling schedule 8, 24, 40, 72, ... keeps it below 2 * DIST + 8. */
+int dss_check_dist(dss_set s, dss_dist_check_t* out);
 
 /* ---- batching and serving: a device job pool ----
  *

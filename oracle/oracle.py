@@ -370,6 +370,8 @@ def merge_sort(ss: StringSet, sampler="IndexedNumStrings", encoder="lcp_compact"
     p = ss.p
     if p == 1:
         return dict(perm=perm, lcp=lcp, local=(perm, lcp), out=None)
+    if encoder == "no_lcps":
+        return _merge_sort_no_lcps(ss, perm, lcp, sampler)
     avg = avg_lcp(lcp) + 5
     interval = partition(ss, perm, sampler, max_len=2 * 100 * avg)
     ex = exchange(ss, perm, lcp, interval, encoder)
@@ -391,6 +393,32 @@ def merge_sort(ss: StringSet, sampler="IndexedNumStrings", encoder="lcp_compact"
         outs.append(dict(arena=np.asarray(full, dtype=np.uint8), lcp=olcp, saved=saved,
                          order=order, src_pe=ex["src_pe"][r0:r1][order.astype(np.int64)],
                          src_pos=ex["src_pos"][r0:r1][order.astype(np.int64)]))
+    return dict(perm=perm, lcp=lcp, interval=interval, exchange=ex, out=outs)
+
+
+def _merge_sort_no_lcps(ss: StringSet, perm, lcp, sampler):
+    """The NoLcps variant of DistributedMergeSort::sort (lcp_aware_number_
+    string_splitting.hpp:218-224, 276-281): the local sort computes no LCPs,
+    so getAvgLcp reads the zero-initialised array (avg = 0 + 5); whole strings
+    are exchanged without LCPs (alltoall.hpp:567-640) and noLcpMerge (:139-182)
+    merges the runs with a plain loser tree (string_compare).  Per PE: the
+    merged strings (order of equal strings is the tree's, not pinned)."""
+    import heapq
+    interval = partition(ss, perm, sampler, max_len=2 * 100 * 5)
+    ex = exchange(ss, perm, lcp, interval, "whole")
+    outs = []
+    for q in range(ss.p):
+        b0, b1 = int(ex["byte_begin"][q]), int(ex["byte_begin"][q + 1])
+        a = ex["bytes"][b0:b1].tobytes()
+        strs = a.split(b"\0")[:-1] if a else []
+        runs, k = [], 0
+        for r in range(ss.p):
+            c = int(interval[r, q])
+            runs.append(strs[k:k + c])
+            k += c
+        merged = list(heapq.merge(*runs))
+        outs.append(dict(arena=np.frombuffer(b"".join(x + b"\0" for x in merged), dtype=np.uint8),
+                         strings=merged))
     return dict(perm=perm, lcp=lcp, interval=interval, exchange=ex, out=outs)
 
 
@@ -529,3 +557,27 @@ def golomb_pipeline_blocks(values, counts, b=1048576):
         sizes.append(1 + enc.size)
         i += c
     return np.array(out, dtype=np.uint64), np.array(sizes, dtype=np.uint64)
+
+
+def exact_dist(ss: StringSet, perm):
+    """ExcatDistinguishingPrefix::filter_exact (src/sorter/distributed/
+    bloomfilter.hpp:847-965): all strings sorted stably (by string, then PE
+    and local sorted index), every string gets 1 + the larger LCP with its
+    neighbours in that order (0 when there is a single string).  Indexed by
+    position pe_begin[r] + j, j = index in PE r's local sorted order."""
+    n = ss.n
+    keys = []
+    for r in range(ss.p):
+        for pos in range(int(ss.pe_begin[r]), int(ss.pe_begin[r + 1])):
+            keys.append((ss.string(int(perm[pos])), pos))
+    keys.sort(key=lambda t: t[0])  # stable: equal strings keep (PE, index) order
+    out = np.zeros(n, dtype=np.uint32)
+    for i in range(1, len(keys)):
+        a, b = keys[i - 1][0], keys[i][0]
+        k = 0
+        while k < min(len(a), len(b)) and a[k] == b[k]:
+            k += 1
+        d = 1 + k
+        out[keys[i - 1][1]] = max(out[keys[i - 1][1]], d)
+        out[keys[i][1]] = max(out[keys[i][1]], d)
+    return out

@@ -32,3 +32,33 @@ def canonical_ties(perm, lcp, lengths, pe_begin):
         perm[s:e] = np.sort(perm[s:e])
         i = j + 1
     return perm
+
+
+def file_distributer(text: bytes, p: int):
+    """distribute_strings (src/mpi/readInput.hpp:40-104) done the way the
+    reference does it: byte slices, '\\n' -> NUL, every PE but the first hands
+    its first partial string to its left neighbour."""
+    size = len(text)
+    small, larger = divmod(size, p)
+    slices = []
+    for r in range(p):
+        ln = small + (1 if r < larger else 0)
+        off = r * (small + 1) if r < larger else larger * (small + 1) + (r - larger) * small
+        slices.append(bytearray(text[off:off + ln]))
+    firsts = []
+    for r in range(p):
+        s = slices[r]
+        fe = s.index(b"\n") if b"\n" in s else len(s)
+        firsts.append(fe)
+    out = []
+    for r in range(p):
+        s = bytearray(slices[r].replace(b"\n", b"\0"))
+        if r + 1 < p:
+            nxt = slices[r + 1]
+            s += nxt[:firsts[r + 1] + 1].replace(b"\n", b"\0")
+        elif s and s[-1] != 0:
+            s.append(0)
+        if r > 0:
+            s = s[firsts[r] + 1:]
+        out.append(bytes(s))
+    return out

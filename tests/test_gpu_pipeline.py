@@ -270,3 +270,70 @@ def test_prefix_doubling_partitioned_duplicates(name, m, limit, monkeypatch):
         nr = ds.prefix_doubling(m=m)
         gres, grounds = ds.get_prefix_doubling()
     assert nr == onr and (gres == ores).all() and (grounds == orounds).all()
+
+
+@pytest.mark.parametrize("name", sorted(INPUTS))
+@pytest.mark.parametrize("p", [4, 16])
+def test_merge_sort_no_lcps(name, p):
+    """The NoLcps variant: partition with the LCP average of the never-computed
+    LCP array (0 + 5), whole strings sent without LCPs, the receiver's run
+    LCPs derived from the received strings (equal to what mode 0 would have
+    sent), merged strings per PE equal to noLcpMerge's."""
+    dss = _dss()
+    chars = INPUTS[name]
+    ss = O.StringSet(chars, p)
+    res = O.merge_sort(ss, encoder="no_lcps")
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        ds.merge_sort(compressed="no_lcps")
+        rb, bb = None, None
+        glcp, gpe, gpos = ds.get_exchange()
+        ex = res["exchange"]
+        assert (glcp == ex["lcp"]).all() and (gpe == ex["src_pe"]).all() and (gpos == ex["src_pos"]).all()
+        nbytes = int(ex["byte_begin"][-1])
+        gbytes = ds.get_exchange_bytes(nbytes)
+        assert gbytes.tobytes() == ex["bytes"].tobytes()
+        gord, gml, _ = ds.get_merged()
+        chk = ds.check()
+        assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0
+    for q in range(p):
+        r0, r1 = int(ex["recv_begin"][q]), int(ex["recv_begin"][q + 1])
+        b0, b1 = int(ex["byte_begin"][q]), int(ex["byte_begin"][q + 1])
+        a = ex["bytes"][b0:b1].tobytes()
+        recv = a.split(b"\0")[:-1] if a else []
+        got = [recv[int(i) - r0] for i in gord[r0:r1]]
+        assert got == res["out"][q]["strings"], q
+
+
+@pytest.mark.parametrize("name", sorted(INPUTS))
+def test_exact_dist_and_prefix_doubling_bounds(name):
+    """ExcatDistinguishingPrefix on the device equals the oracle's after
+    every driver; prefix doubling's results are upper bounds of the exact
+    lengths and, with the full hash range, below 2 * DIST + 8; the exact
+    lengths sum to the checker's D."""
+    dss = _dss()
+    chars = INPUTS[name]
+    p = 4
+    ss = O.StringSet(chars, p)
+    operm, _ = O.local_sort(ss)
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        for enc in ("whole", "lcp_compact", "no_lcps"):
+            ds.merge_sort(compressed=enc)
+            gperm, _ = ds.get_sorted()
+            gex = ds.exact_dist()
+            # positions hold the same strings on both sides (ties aside)
+            oex = O.exact_dist(ss, gperm)
+            assert (gex == oex).all(), enc
+            assert int(gex.sum()) == ds.check()["D"]
+        ds.prefix_doubling_sort()
+        gperm, _ = ds.get_sorted()
+        gex = ds.exact_dist()
+        assert (gex == O.exact_dist(ss, gperm)).all()
+        res, _ = ds.get_prefix_doubling()
+        c = ds.check_dist()
+        assert c["lower_violations"] == 0 and c["upper_violations"] == 0, c
+        assert c["exact_sum"] == int(gex.sum()) and c["results_sum"] == int(res.sum())
+        # a tiny hash range makes candidates survive collisions: results stay
+        # upper bounds
+        ds.prefix_doubling_sort(m=64)
+        c = ds.check_dist()
+        assert c["lower_violations"] == 0

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from helpers import file_distributer
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = json.load(open(os.path.join(HERE, "golden", "ref_vectors.json")))
@@ -69,36 +70,6 @@ def test_string_lengths_reference_case():
     raw = b"AABCA\0BA\0BAB\0\0CCC\0CCD\0"
     ss = O.StringSet(np.frombuffer(raw, np.uint8).copy(), 1)
     assert list(map(int, ss.len)) == [5, 2, 3, 0, 3, 3]
-
-
-def file_distributer(text: bytes, p: int):
-    """distribute_strings (src/mpi/readInput.hpp:40-104) done the way the
-    reference does it: byte slices, '\\n' -> NUL, every PE but the first hands
-    its first partial string to its left neighbour."""
-    size = len(text)
-    small, larger = divmod(size, p)
-    slices = []
-    for r in range(p):
-        ln = small + (1 if r < larger else 0)
-        off = r * (small + 1) if r < larger else larger * (small + 1) + (r - larger) * small
-        slices.append(bytearray(text[off:off + ln]))
-    firsts = []
-    for r in range(p):
-        s = slices[r]
-        fe = s.index(b"\n") if b"\n" in s else len(s)
-        firsts.append(fe)
-    out = []
-    for r in range(p):
-        s = bytearray(slices[r].replace(b"\n", b"\0"))
-        if r + 1 < p:
-            nxt = slices[r + 1]
-            s += nxt[:firsts[r + 1] + 1].replace(b"\n", b"\0")
-        elif s and s[-1] != 0:
-            s.append(0)
-        if r > 0:
-            s = s[firsts[r] + 1:]
-        out.append(bytes(s))
-    return out
 
 
 @pytest.mark.parametrize("p", [1, 2, 3, 4, 7, 16])
@@ -474,3 +445,45 @@ def test_partition_numchars_buckets_respect_the_order(sampler):
     for q in range(3):
         if hi[q] is not None and lo[q + 1] is not None:
             assert hi[q] <= lo[q + 1]
+
+
+@pytest.mark.parametrize("p", [1, 3, 4])
+def test_merge_sort_no_lcps_is_textbook_sort(p):
+    """NoLcps variant of the oracle: every PE's merged strings concatenated
+    in PE order are the textbook sorted input, and the buckets are those of
+    the partition with maximum splitter length 2 * 100 * (0 + 5)."""
+    chars = INPUTS["skewed"]()
+    ss = O.StringSet(chars, p)
+    res = O.merge_sort(ss, encoder="no_lcps")
+    allstr = sorted(ss.string(i) for i in range(ss.n))
+    if p == 1:
+        return
+    got = [s for o in res["out"] for s in o["strings"]]
+    assert got == allstr
+    perm, _ = O.local_sort(ss)
+    assert (res["interval"] == O.partition(ss, perm, "IndexedNumStrings", max_len=1000)).all()
+
+
+@pytest.mark.parametrize("name", sorted(INPUTS))
+def test_exact_dist_sums_to_computed(name):
+    """The exact distinguishing prefixes of all strings sum to computeD of
+    the globally sorted LCP array (DNGenerator_tests.cpp:22-45, the C
+    restatement), and a string equal to another has length + 1."""
+    chars = INPUTS[name]()
+    if O.StringSet(chars, 1).n > 20000:
+        chars = chars[:int(np.flatnonzero(chars == 0)[19999]) + 1]
+    for p in (1, 3):
+        ss = O.StringSet(chars, p)
+        perm, _ = O.local_sort(ss)
+        ex = O.exact_dist(ss, perm)
+        one = O.StringSet(chars, 1)
+        _, glcp = O.local_sort(one)
+        assert int(ex.sum()) == O.compute_d(glcp)
+        strs = [ss.string(int(perm[i])) for i in range(ss.n)]
+        count = {}
+        for s in strs:
+            count[s] = count.get(s, 0) + 1
+        for i, s in enumerate(strs):
+            if count[s] > 1:
+                assert ex[i] == len(s) + 1
+    assert list(O.exact_dist(O.StringSet(arena([b"abc"]), 1), np.zeros(1, np.uint32))) == [0]
