@@ -252,6 +252,122 @@ def batched_leg(args):
     }
 
 
+def extras_leg(args, D_c2):
+    """Measurements of the paths beside the headline sort, on the same GPU:
+    the wire codecs on C2-shaped streams (device-resident, HIP-event timed),
+    the distributed merge sort on the 1M x 500 B D/N input over 4 PEs (the
+    reference's CPU-runnable case) with each string encoder, the text loader
+    and the exact distinguishing-prefix validator."""
+    import numpy as np
+    import dss_amd
+    from dss_amd import WireCodec
+
+    out = {}
+    rng = np.random.default_rng(7)
+    # -- codecs: one PE's prefix-doubling hash stream (1.25M sorted hashes in
+    # 16 intervals) and the LCPs of a 20M-string exchange (u64 values < 512)
+    p, per = 16, args.n // 16
+    m = 2**64 - 1
+    counts = np.full(p, per // p, dtype=np.uint64)
+    hashes = np.concatenate([np.sort(rng.integers(q * (m // p), (q + 1) * (m // p) - 1, size=int(c),
+                                                  dtype=np.uint64)) for q, c in enumerate(counts)])
+    lcps = rng.integers(200, 300, size=args.n, dtype=np.uint64)
+    lcounts = np.full(p, args.n // p, dtype=np.uint64)
+    with WireCodec() as c:
+        # host forms stage the streams in the codec's HBM buffers; the kernel
+        # timer sees only the coding kernels (no PCIe copies)
+        res = {}
+        for it in range(4):  # 1 warmup + 3 timed
+            c.set_timing(it > 0)
+            words, _ = c.send_hashes(hashes, counts, m)
+            dec, bc = c.golomb_decode(words, p, hashes.size)
+            data, _ = c.write_ranges(lcps, lcounts)
+            back = c.read_ranges(data, lcounts)
+        st = c.kernel_stats(reset=True)
+        assert np.array_equal(dec, hashes) and np.array_equal(back, lcps)
+        assert int(bc.sum()) == hashes.size
+        nw, nb = words.size, data.size
+        def per_call(names, calls=3):
+            # the host form sizes its output first: code lengths and their
+            # scan run twice per encode, count them once
+            t = 0.0
+            for k in names:
+                if k in st:
+                    twice = k in ("golomb_len", "golomb_scan")
+                    t += st[k]["ms"] / (2 if twice else 1)
+            return t / calls
+        enc_ms = per_call(("golomb_len", "golomb_scan", "golomb_put"))
+        dec_ms = per_call(("golomb_spec", "golomb_sync", "golomb_count", "golomb_decode", "golomb_sum"))
+        venc_ms = per_call(("varint_len", "varint_scan", "varint_put"))
+        vdec_ms = per_call(("varint_count", "varint_decode"))
+        res["golomb_hash_stream"] = {
+            "values": int(hashes.size), "intervals": p, "words": int(nw),
+            "bits_per_value": round(nw * 64 / hashes.size, 2),
+            "encode_ms": round(enc_ms, 4), "decode_ms": round(dec_ms, 4),
+            "encode_values_per_s": round(hashes.size / enc_ms * 1e3, 1),
+            "decode_values_per_s": round(hashes.size / dec_ms * 1e3, 1)}
+        res["varint_lcps"] = {
+            "values": int(lcps.size), "bytes": int(nb),
+            "encode_ms": round(venc_ms, 4), "decode_ms": round(vdec_ms, 4),
+            "encode_GBps_in": round(lcps.size * 8 / venc_ms / 1e6, 1),
+            "decode_GBps_out": round(lcps.size * 8 / vdec_ms / 1e6, 1)}
+        res["kernels"] = {k: {"launches": v["launches"], "ms": round(v["ms"] / 3, 4)} for k, v in st.items()}
+        out["codec"] = res
+
+    # -- distributed merge sort, 1M x 500 B D/N 0.5 over 4 PEs
+    ms = {}
+    with dss_amd.DeviceStringSet.dton(1_000_000, 500, 0.5, 0x5EED0001, 4) as ds:
+        for enc in ("whole", "lcp_compact", "no_lcps"):
+            ds.merge_sort(compressed=enc)
+            ds.sync()
+            chk = ds.check()
+            assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0
+            t0 = time.perf_counter()
+            for _ in range(5):
+                ds.merge_sort(compressed=enc)
+            ds.sync()
+            dt = (time.perf_counter() - t0) / 5
+            ms[enc] = {"ms": round(dt * 1e3, 3), "dp_GBps": round(chk["D"] / dt / 1e9, 2)}
+        out["merge_sort_1m_x_500_p4"] = ms
+
+    # -- exact distinguishing prefixes and the bound check after the C2 sort
+    with dss_amd.DeviceStringSet.dton(args.n, args.len, args.dton, args.seed, args.pes) as ds:
+        ds.prefix_doubling_sort(args.sampler)
+        ds.check_dist()
+        ds.sync()
+        t0 = time.perf_counter()
+        c = ds.check_dist()
+        dt = time.perf_counter() - t0
+        out["exact_dist_c2"] = {"ms": round(dt * 1e3, 3), "lower_violations": c["lower_violations"],
+                                "upper_violations": c["upper_violations"], "exact_sum": c["exact_sum"],
+                                "results_sum": c["results_sum"], "D_check": D_c2}
+
+    # -- text loader: the 1M x 500 B input as a '\n'-separated file, 4 PEs
+    import tempfile
+    with dss_amd.DeviceStringSet.dton(1_000_000, 500, 0.5, 0x5EED0001, 4) as ds:
+        chars, _, _ = ds.download()
+        pb = ds.pe_begin()
+    text = chars.copy()
+    text[text == 0] = ord("\n")
+    with tempfile.NamedTemporaryFile(suffix=".txt", dir=os.environ.get("TMPDIR", "/tmp"), delete=False) as f:
+        f.write(text.tobytes())
+        path = f.name
+    try:
+        with dss_amd.DeviceStringSet.load_file(path, 4) as ds:  # warm the page cache and the code
+            pass
+        t0 = time.perf_counter()
+        with dss_amd.DeviceStringSet.load_file(path, 4) as ds:
+            dt = time.perf_counter() - t0
+            assert (ds.pe_begin() == pb).all() and ds.n == 1_000_000
+        out["text_loader_1m_x_500_p4"] = {"bytes": int(text.size), "ms": round(dt * 1e3, 3),
+                                          "GBps_file_to_set": round(text.size / dt / 1e9, 2),
+                                          "note": "page-cached file, pinned staging, PCIe copy, "
+                                                  "'\\n' -> NUL, index and PE split"}
+    finally:
+        os.unlink(path)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,6 +400,8 @@ def main():
                     help="pool capacity of the plain/batched comparison (GiB of characters)")
     ap.add_argument("--serve-served-pool-gb", type=float, default=160.0,
                     help="pool capacity of the served run (the admission limit)")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="measure the codecs, merge-sort encoders, exact-prefix validator and loader (0: off)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -348,6 +466,11 @@ def main():
         log(f"batched/served leg: {args.serve_jobs} jobs ...")
         batched = batched_leg(args)
 
+    extras = None
+    if args.extras:
+        log("extras: codecs, merge sort encoders, exact distinguishing prefixes, text loader ...")
+        extras = extras_leg(args, D)
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"CPU oracle on {args.cpu_sample} strings ...")
@@ -382,6 +505,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched": batched,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
 

@@ -22,7 +22,7 @@
 //                     stored whole), the last word of a stream is padded with
 //                     ones;
 //   Golomb decode  -- code boundaries are found without a sequential walk:
-//                     every 256-bit chunk is decoded speculatively from its
+//                     every 4096-bit chunk is decoded speculatively from its
 //                     first bit, recording the code starts it visits; then
 //                     every chunk is decoded again from its predecessor's exit
 //                     until that path meets one of its speculative starts
@@ -54,7 +54,7 @@ namespace dss {
 namespace {
 
 constexpr uint64_t kEnd = ~0ull;
-constexpr uint32_t kChunkBits = 256;  // speculative decode chunk
+constexpr uint32_t kChunkBits = 4096;  // speculative decode chunk (64 words)
 constexpr uint32_t kChunkWords = kChunkBits / 64;
 constexpr uint32_t kVBytes = 16;      // bytes per lane in the varint decode
 constexpr uint32_t kVThreads = 256;
@@ -316,6 +316,27 @@ __device__ inline uint32_t find_block(const GBlock* __restrict__ blk, uint32_t n
     return lo;
 }
 
+// Code starts are recorded in a bitmap with one word per stream word;
+// a lane accumulates the word it is in and stores it when it moves on (its
+// chunk's words are its own, zeroed before).
+struct StartBits {
+    uint64_t* out;
+    uint64_t w = kEnd, acc = 0;
+    __device__ explicit StartBits(uint64_t* o) : out(o) {}
+    __device__ void set(uint64_t pos) {
+        const uint64_t w2 = pos >> 6;
+        if (w2 != w) {
+            flush();
+            w = w2;
+        }
+        acc |= 1ull << (63 - (pos & 63));
+    }
+    __device__ void flush() {
+        if (w != kEnd && acc) out[w] |= acc;
+        acc = 0;
+    }
+};
+
 __global__ void k_golomb_spec(const uint64_t* __restrict__ words, const GBlock* __restrict__ blk, uint32_t nb,
                               uint64_t nchunks, uint64_t* __restrict__ S, uint64_t* __restrict__ E) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -323,7 +344,7 @@ __global__ void k_golomb_spec(const uint64_t* __restrict__ words, const GBlock* 
     const GBlock B = blk[find_block(blk, nb, c)];
     const uint64_t* W = words + B.bitw0;
     const uint64_t cs = (c - B.chunk0) * kChunkBits, ce = min<uint64_t>(cs + kChunkBits, B.nbits);
-    uint64_t sw[kChunkWords] = {0, 0, 0, 0};
+    StartBits sb(S + B.bitw0);
     uint64_t pos = cs;
     while (pos < ce) {
         uint64_t v;
@@ -332,10 +353,10 @@ __global__ void k_golomb_spec(const uint64_t* __restrict__ words, const GBlock* 
             pos = kEnd;
             break;
         }
-        sw[(pos - cs) >> 6] |= 1ull << (63 - (pos & 63));
+        sb.set(pos);
         pos = nx;
     }
-    for (uint32_t i = 0; i < kChunkWords && cs + 64 * i < B.nbits; ++i) S[B.bitw0 + cs / 64 + i] = sw[i];
+    sb.flush();
     E[c] = pos;
 }
 
@@ -348,45 +369,47 @@ __global__ void k_golomb_sync(const uint64_t* __restrict__ words, const GBlock* 
     const GBlock B = blk[find_block(blk, nb, c)];
     const uint64_t* W = words + B.bitw0;
     const uint64_t cs = (c - B.chunk0) * kChunkBits, ce = min<uint64_t>(cs + kChunkBits, B.nbits);
-    const uint64_t* Sc = S + B.bitw0 + cs / 64;
+    const uint64_t w0 = cs / 64, w1 = (ce + 63) / 64;  // the chunk's bitmap words
+    const uint64_t* Sb = S + B.bitw0;
+    uint64_t* Tb = T + B.bitw0;
     const uint64_t entry = c == B.chunk0 ? 0 : Xin[c - 1];
-    uint64_t tw[kChunkWords] = {0, 0, 0, 0};
     uint64_t x;
     if (entry == cs) {
-        for (uint32_t i = 0; i < kChunkWords && cs + 64 * i < B.nbits; ++i) tw[i] = Sc[i];
+        for (uint64_t w = w0; w < w1; ++w) Tb[w] = Sb[w];
         x = E[c];
-    } else if (entry == kEnd || entry >= ce) {
-        x = entry;  // no code starts in this chunk
     } else {
-        uint64_t pos = entry;
-        bool synced = false;
-        while (pos < ce) {
-            if ((Sc[(pos - cs) >> 6] >> (63 - (pos & 63))) & 1) {
-                synced = true;  // from here on the speculative walk is exact
-                break;
-            }
-            uint64_t v;
-            const uint64_t nx = golomb_next(W, B, pos, &v);
-            if (nx == kEnd) {
-                pos = kEnd;
-                break;
-            }
-            tw[(pos - cs) >> 6] |= 1ull << (63 - (pos & 63));
-            pos = nx;
-        }
-        if (synced) {
-            for (uint32_t i = 0; i < kChunkWords && cs + 64 * i < B.nbits; ++i) {
-                const uint64_t w0 = cs + 64 * i;
-                if (pos >= w0 + 64) continue;
-                const uint64_t m = pos <= w0 ? ~0ull : ~0ull >> (pos - w0);
-                tw[i] |= Sc[i] & m;
-            }
-            x = E[c];
+        for (uint64_t w = w0; w < w1; ++w) Tb[w] = 0;
+        if (entry == kEnd || entry >= ce) {
+            x = entry;  // no code starts in this chunk
         } else {
-            x = pos;
+            StartBits tb(Tb);
+            uint64_t pos = entry;
+            bool synced = false;
+            while (pos < ce) {
+                if ((Sb[pos >> 6] >> (63 - (pos & 63))) & 1) {
+                    synced = true;  // from here on the speculative walk is exact
+                    break;
+                }
+                uint64_t v;
+                const uint64_t nx = golomb_next(W, B, pos, &v);
+                if (nx == kEnd) {
+                    pos = kEnd;
+                    break;
+                }
+                tb.set(pos);
+                pos = nx;
+            }
+            tb.flush();
+            if (synced) {
+                const uint64_t pw = pos >> 6;
+                Tb[pw] |= Sb[pw] & (~0ull >> (pos & 63));
+                for (uint64_t w = pw + 1; w < w1; ++w) Tb[w] = Sb[w];
+                x = E[c];
+            } else {
+                x = pos;
+            }
         }
     }
-    for (uint32_t i = 0; i < kChunkWords && cs + 64 * i < B.nbits; ++i) T[B.bitw0 + cs / 64 + i] = tw[i];
     Xout[c] = x;
     if (x != Xin[c]) atomicOr(changed, 1u);
 }
@@ -513,7 +536,11 @@ static void varint_write(dss_codec_s* c, const uint64_t* values, const uint64_t*
     }
     check_launch();
     uint64_t total = 0;
-    exclusive_scan_u64(L, O, n + 1, st, c->pp, &total);
+    {
+        KScope ks(c->timer, st, "varint_scan", n * 16.0);
+        exclusive_scan_u64(L, O, n + 1, st, c->pp, nullptr);
+    }
+    DSS_HIP(hipMemcpyAsync(&total, O + n, 8, hipMemcpyDeviceToHost, st));
     c->meta.reserve(idx.size() * 16);
     uint64_t* di = c->meta.as<uint64_t>();
     uint64_t* dv = di + idx.size();
@@ -553,10 +580,14 @@ static void varint_read(dss_codec_s* c, const uint8_t* bytes, uint64_t nbytes, c
     uint64_t* base = c->pos.as<uint64_t>();
     uint32_t* err = c->flag.as<uint32_t>();
     DSS_HIP(hipMemsetAsync(err, 0, 4, st));
-    hipLaunchKernelGGL(k_vterm_count, dim3((uint32_t)nblk), dim3(kVThreads), 0, st, bytes, nbytes, cnt);
-    check_launch();
     uint64_t total = 0;
-    exclusive_scan_u64(cnt, base, nblk, st, c->pp, &total);
+    {
+        KScope ks(c->timer, st, "varint_count", (double)nbytes + nblk * 16.0);
+        hipLaunchKernelGGL(k_vterm_count, dim3((uint32_t)nblk), dim3(kVThreads), 0, st, bytes, nbytes, cnt);
+        exclusive_scan_u64(cnt, base, nblk, st, c->pp, nullptr);
+    }
+    check_launch();
+    DSS_HIP(hipMemcpyAsync(&total, c->pp.scan_tot.p, 8, hipMemcpyDeviceToHost, st));
     uint8_t last = 0;
     DSS_HIP(hipMemcpyAsync(&last, bytes + nbytes - 1, 1, hipMemcpyDeviceToHost, st));
     DSS_HIP(hipStreamSynchronize(st));
@@ -611,7 +642,10 @@ static void golomb_encode(dss_codec_s* c, int layout, const uint64_t* values, co
         hipLaunchKernelGGL(k_golomb_len, dim3(stride_grid(n + 1, 256)), dim3(256), 0, st, values, n, dR, nr, L);
     }
     check_launch();
-    exclusive_scan_u64(L, G, n + 1, st, c->pp, nullptr);
+    {
+        KScope ks(c->timer, st, "golomb_scan", n * 16.0);
+        exclusive_scan_u64(L, G, n + 1, st, c->pp, nullptr);
+    }
     std::vector<uint64_t> at(idx.size());
     if (!idx.empty()) {
         hipLaunchKernelGGL(k_gather_u64, dim3(((uint32_t)idx.size() + 255) / 256), dim3(256), 0, st, G, di,
@@ -706,6 +740,7 @@ static void golomb_decode(dss_codec_s* c, int layout, const uint64_t* words, uin
     if (nch) {
         {
             KScope ks(c->timer, st, "golomb_spec", nwords * 16.0);
+            DSS_HIP(hipMemsetAsync(S, 0, nwords * 8, st));
             hipLaunchKernelGGL(k_golomb_spec, dim3(gch), dim3(256), 0, st, words, dB, nb, nch, S, E);
         }
         check_launch();
@@ -728,10 +763,15 @@ static void golomb_decode(dss_codec_s* c, int layout, const uint64_t* words, uin
         }
     }
     const uint32_t gcb = (uint32_t)std::max<uint64_t>((std::max<uint64_t>(nch, nb) + 255) / 256, 1);
-    hipLaunchKernelGGL(k_golomb_count, dim3(gcb), dim3(256), 0, st, dB, nb, nch, T, A);
-    check_launch();
     uint64_t total = 0;
-    exclusive_scan_u64(A, Ap, nch + nb + 1, st, c->pp, &total);
+    {
+        KScope ks(c->timer, st, "golomb_count", nwords * 8.0 + (nch + nb) * 24.0);
+        hipLaunchKernelGGL(k_golomb_count, dim3(gcb), dim3(256), 0, st, dB, nb, nch, T, A);
+        exclusive_scan_u64(A, Ap, nch + nb + 1, st, c->pp, nullptr);
+    }
+    check_launch();
+    DSS_HIP(hipMemcpyAsync(&total, c->pp.scan_tot.p, 8, hipMemcpyDeviceToHost, st));
+    DSS_HIP(hipStreamSynchronize(st));
     DSS_REQUIRE(total <= cap, DSS_ERANGE,
                 "Golomb input decodes to " + std::to_string(total) + " values, capacity " + std::to_string(cap));
     {
@@ -742,8 +782,8 @@ static void golomb_decode(dss_codec_s* c, int layout, const uint64_t* words, uin
     if (total) {
         c->pos.reserve(total * 8);
         uint64_t* ex = c->pos.as<uint64_t>();
+        KScope ks(c->timer, st, "golomb_sum", total * 40.0);
         exclusive_scan_u64(values, ex, total, st, c->pp, nullptr);
-        KScope ks(c->timer, st, "golomb_sum", total * 24.0);
         hipLaunchKernelGGL(k_golomb_sum, dim3(stride_grid(total, 256)), dim3(256), 0, st, values, ex, total, bstart, nb);
     }
     check_launch();

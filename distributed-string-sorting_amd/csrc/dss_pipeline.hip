@@ -1310,9 +1310,25 @@ __global__ void k_full_glcp(const uint32_t* __restrict__ ord, const uint32_t* __
     }
     const uint32_t b = ord[i - 1];
     const uint64_t pb = pe_begin[rpe[b]] + rpos[b];
-    uint32_t l = 0;
-    (void)view_cmp(chars, s_off[pb], s_len[pb], s_off[pa], s_len[pa], 0, &l);
-    glcp[i] = l;
+    const uint64_t ao = s_off[pa], bo = s_off[pb];
+    const uint32_t al = s_len[pa], bl = s_len[pb];
+    // 32 characters per step from aligned loads (keys are zero after a
+    // string's end, so equal strings stop at their common length)
+    for (uint32_t d = 0;; d += 32) {
+        uint64_t x[4], y[4];
+        load_keys4(chars, ao, al, d, x);
+        load_keys4(chars, bo, bl, d, y);
+        uint32_t l = kNone;
+#pragma unroll
+        for (int j = 3; j >= 0; --j) {  // the first event wins
+            if (x[j] != y[j]) l = d + 8 * j + key_lcp(x[j], y[j]);
+            else if ((x[j] & 0xFF) == 0) l = d + 8 * j + key_strlen(x[j]);
+        }
+        if (l != kNone) {
+            glcp[i] = l;
+            return;
+        }
+    }
 }
 
 __global__ void k_exact_dist(const uint32_t* __restrict__ glcp, const uint64_t* __restrict__ gpos, uint64_t n,
