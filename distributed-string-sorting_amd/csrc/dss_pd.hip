@@ -100,13 +100,36 @@ __device__ uint64_t xxh64_stream(const uint8_t* __restrict__ chars, uint64_t off
 constexpr uint8_t kHashed = 1, kLcpDup = 2;
 constexpr uint64_t kEmpty = ~0ull;  // never a hash value: hash % m < m <= 2^64-1
 
+// Every position starts as a candidate; `skipped` leading rounds (the last
+// at depth `dlast`) are already applied: in them every position stayed a
+// candidate with results = depth.
 __global__ void k_pd_init(uint8_t* __restrict__ cand, uint8_t* __restrict__ rounds, uint32_t* __restrict__ dist,
-                          uint64_t n) {
+                          uint64_t n, uint32_t skipped, uint32_t dlast) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         cand[i] = 1;
-        rounds[i] = 0;
-        dist[i] = 0;
+        rounds[i] = (uint8_t)min(skipped, 255u);
+        dist[i] = skipped ? dlast : 0;
+    }
+}
+
+// min over all positions c of max(lcp[c], lcp[c+1]) (lcp is 0 at every PE
+// start, so lcp[c+1] of a PE's last position never counts).  A round at
+// depth d <= this minimum changes nothing but results and the round count:
+// every candidate c has a neighbour in its PE sharing >= d characters, which
+// makes it a local LCP duplicate or an LCP-local root, and both stay
+// candidates (and d <= the LCP <= len(c), so no string ends).
+__global__ void k_pd_min_run(const uint32_t* __restrict__ lcp, uint64_t n, unsigned int* __restrict__ out) {
+    uint32_t v = 0xFFFFFFFFu;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (uint64_t)gridDim.x * blockDim.x)
+        v = min(v, max(lcp[c], c + 1 < n ? lcp[c + 1] : 0u));
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_down(v, o));
+    __shared__ uint32_t part[4];  // one atomic per block: single-address atomics serialise
+    if (lane_id() == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        v = min(min(part[0], part[1]), min(part[2], part[3]));
+        if (v != 0xFFFFFFFFu) atomicMin(out, v);
     }
 }
 
@@ -871,10 +894,32 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     uint8_t* next = P.pd_next.as<uint8_t>();
     unsigned long long* ctr = P.pd_ctr.as<unsigned long long>();
     const dim3 grid((n + 255) / 256), blk(256);
-    if (n) hipLaunchKernelGGL(k_pd_init, grid, blk, 0, st, cand, P.rounds.as<uint8_t>(), P.dist.as<uint32_t>(), n);
     uint64_t ncand = n;
     uint32_t round = 0;
     uint64_t depth = start_depth, step = (uint64_t)start_depth * 2;
+    // leading rounds in which every string has a neighbour sharing at least
+    // the round's depth keep every candidate: they are applied at once
+    uint32_t skipped = 0, dlast = 0;
+    if (n) {
+        unsigned int minrun = 0;
+        {
+            KScope ks(s->timer, st, "pd_skip", (double)n * 4.0);
+            DSS_HIP(hipMemsetAsync(ctr, 0xFF, 4, st));
+            hipLaunchKernelGGL(k_pd_min_run, dim3(std::min<uint64_t>((n + 255) / 256, 1024)), blk, 0, st,
+                               s->lcp.as<uint32_t>(), n, reinterpret_cast<unsigned int*>(ctr));
+        }
+        DSS_HIP(hipMemcpyAsync(&minrun, ctr, 4, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+        while (depth <= minrun) {
+            dlast = (uint32_t)depth;
+            ++skipped;
+            depth = start_depth + step;
+            step *= 2;
+        }
+        round = skipped;
+        hipLaunchKernelGGL(k_pd_init, grid, blk, 0, st, cand, P.rounds.as<uint8_t>(), P.dist.as<uint32_t>(), n,
+                           skipped, dlast);
+    }
     // duplicate detection by partitioning from part_min hashed candidates on;
     // both settings can be lowered by the environment (tests cover the
     // partitioned path and its exact-table fallback on small inputs)
