@@ -150,4 +150,22 @@ __device__ inline int wave_cmp(const uint8_t* __restrict__ chars, uint64_t ao, u
     }
 }
 
+// LCP of two strings by a whole wave (same window scheme as wave_cmp; every
+// lane of the wave must call it with the same arguments).
+__device__ inline uint32_t wave_lcp(const uint8_t* __restrict__ chars, uint64_t ao, uint32_t al, uint64_t bo,
+                                    uint32_t bl) {
+    const uint32_t lane = lane_id();
+    for (uint32_t base = 0;; base += 512) {
+        const uint32_t pos = base + 8 * lane;
+        const uint64_t a = load_key8(chars, ao, al, pos);
+        const uint64_t b = load_key8(chars, bo, bl, pos);
+        const uint64_t m = __ballot(a != b || (a & 0xFF) == 0);
+        if (m) {
+            const int f = __ffsll((unsigned long long)m) - 1;
+            const uint64_t fa = __shfl((unsigned long long)a, f), fb = __shfl((unsigned long long)b, f);
+            return base + 8 * (uint32_t)f + (fa != fb ? key_lcp(fa, fb) : key_strlen(fa));
+        }
+    }
+}
+
 }  // namespace dss

@@ -665,51 +665,96 @@ k_hp_hist(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv,
 
 // Stable scatter of one tile by the digit (x >> shift) & (2^bits - 1): the
 // first pass reads the candidate list and hashes, the second the first
-// pass's output.
+// pass's output.  The tile is first reordered by digit in LDS (wave ballots
+// rank equal digits, as in the local sort's scatter) and then written out
+// run by run, so consecutive lanes write consecutive addresses of a digit's
+// run instead of one line per record.
 __global__ void __launch_bounds__(kHpT)
 k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv,
              const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ ids_in, uint64_t nh, uint32_t shift,
              uint32_t bits, uint64_t ntiles, const uint64_t* __restrict__ off, uint64_t* __restrict__ keys_out,
              uint32_t* __restrict__ ids_out) {
-    __shared__ uint64_t base[1u << kHpMaxBits];
+    __shared__ uint64_t skey[kHpTile];
+    __shared__ uint32_t sid[kHpTile];
+    __shared__ uint64_t gbase[1u << kHpMaxBits];   // the digit's output offset for this tile
+    __shared__ uint32_t lstart[1u << kHpMaxBits];  // the digit's first slot in the reordered tile
+    __shared__ uint32_t run[1u << kHpMaxBits];     // next free slot of the digit
     __shared__ uint32_t wcnt[kHpT / 64][1u << kHpMaxBits];
+    __shared__ uint32_t scan[kHpT];
     const uint32_t nb = 1u << bits, tid = threadIdx.x, w = tid >> 6;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kHpTile;
     for (uint32_t d = tid; d < nb; d += kHpT) {
-        base[d] = off[(uint64_t)d * ntiles + blockIdx.x];
+        gbase[d] = off[(uint64_t)d * ntiles + blockIdx.x];
+        run[d] = 0;
         for (uint32_t k = 0; k < kHpT / 64; ++k) wcnt[k][d] = 0;
     }
     __syncthreads();
-    const uint64_t lt = lanemask_lt();
-    const uint64_t t0 = (uint64_t)blockIdx.x * kHpTile;
+    // the tile's elements and their digits
+    uint64_t x[kHpI];
+    uint32_t id[kHpI], dg[kHpI];
+#pragma unroll
     for (uint32_t r = 0; r < kHpI; ++r) {
         const uint64_t i = t0 + r * kHpT + tid;
-        const bool valid = i < nh;
-        uint64_t x = 0;
-        uint32_t id = 0, dg = 0;
-        if (valid) {
+        x[r] = 0;
+        id[r] = 0;
+        dg[r] = 0;
+        if (i < nh) {
             if (keys_in) {
-                x = keys_in[i];
-                id = ids_in[i];
+                x[r] = keys_in[i];
+                id[r] = ids_in[i];
             } else {
-                id = list[i];
-                x = mix64(hashv[id]);
+                id[r] = list[i];
+                x[r] = mix64(hashv[id[r]]);
             }
-            dg = (uint32_t)(x >> shift) & (nb - 1);
+            dg[r] = (uint32_t)(x[r] >> shift) & (nb - 1);
+            atomicAdd(&run[dg[r]], 1u);
         }
+    }
+    __syncthreads();
+    // exclusive scan of the digit counts (two digits per thread at most)
+    const uint32_t per = (nb + kHpT - 1) / kHpT;
+    uint32_t mine = 0;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t d = tid * per + k;
+        if (d < nb) mine += run[d];
+    }
+    scan[tid] = mine;
+    __syncthreads();
+    for (uint32_t o = 1; o < kHpT; o <<= 1) {
+        const uint32_t v = tid >= o ? scan[tid - o] : 0;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    uint32_t acc = scan[tid] - mine;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t d = tid * per + k;
+        if (d < nb) {
+            const uint32_t c = run[d];
+            lstart[d] = acc;
+            run[d] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    // stable placement in LDS, in index order (r, wave, lane)
+    const uint64_t lt = lanemask_lt();
+    for (uint32_t r = 0; r < kHpI; ++r) {
+        const bool valid = t0 + r * kHpT + tid < nh;
         uint64_t m = __ballot(valid);
         for (uint32_t b = 0; b < bits; ++b) {
-            const bool bit = (dg >> b) & 1;
+            const bool bit = (dg[r] >> b) & 1;
             const uint64_t bb = __ballot(valid && bit);
             m &= bit ? bb : ~bb;
         }
         const uint32_t rank = (uint32_t)__popcll(m & lt);
-        if (valid && rank == 0) wcnt[w][dg] = (uint32_t)__popcll(m);
+        if (valid && rank == 0) wcnt[w][dg[r]] = (uint32_t)__popcll(m);
         __syncthreads();
         if (valid) {
-            uint64_t q = base[dg] + rank;
-            for (uint32_t ww = 0; ww < w; ++ww) q += wcnt[ww][dg];
-            keys_out[q] = x;
-            ids_out[q] = id;
+            uint32_t q = run[dg[r]] + rank;
+            for (uint32_t ww = 0; ww < w; ++ww) q += wcnt[ww][dg[r]];
+            skey[q] = x[r];
+            sid[q] = id[r];
         }
         __syncthreads();
         for (uint32_t d = tid; d < nb; d += kHpT) {
@@ -718,9 +763,18 @@ k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ has
                 add += wcnt[k][d];
                 wcnt[k][d] = 0;
             }
-            base[d] += add;
+            run[d] += add;
         }
         __syncthreads();
+    }
+    // runs out: slot k of digit d goes to gbase[d] + (k - lstart[d])
+    const uint32_t cnt = (uint32_t)min<uint64_t>(kHpTile, nh - t0);
+    for (uint32_t k = tid; k < cnt; k += kHpT) {
+        const uint64_t xv = skey[k];
+        const uint32_t d = (uint32_t)(xv >> shift) & (nb - 1);
+        const uint64_t q = gbase[d] + (k - lstart[d]);
+        keys_out[q] = xv;
+        ids_out[q] = sid[k];
     }
 }
 

@@ -474,18 +474,51 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
     r_from[dst] = from;
 }
 
+// Common prefix of all strings a PE received: min over its runs r of the
+// LCP of the PE's first string with the first and last string of run r
+// (one wave per receiver and run; 512 characters per wave step).
+__global__ void k_recv_cp(const uint64_t* __restrict__ run_off, const uint64_t* __restrict__ interval, uint32_t p,
+                          const uint64_t* __restrict__ recv_begin, const uint8_t* __restrict__ chars,
+                          const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
+                          uint32_t* __restrict__ cp_out) {
+    const uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // wave-uniform
+    if (t >= (uint64_t)p * p) return;
+    const uint32_t q = (uint32_t)(t / p), r = (uint32_t)(t % p);
+    const uint64_t c = interval[(uint64_t)r * p + q];
+    if (!c) return;
+    const uint64_t s0 = recv_begin[q];
+    const uint64_t f = run_off[(uint64_t)q * p + r], l = f + c - 1;
+    const uint32_t x = wave_lcp(chars, voff[s0], vlen[s0], voff[f], vlen[f]);
+    const uint32_t y = wave_lcp(chars, voff[s0], vlen[s0], voff[l], vlen[l]);
+    if (lane_id() == 0) atomicMin(&cp_out[q], min(x, y));
+}
+
+// Receiving PE of receive position k.
+__device__ inline uint32_t receiver_of(const uint64_t* __restrict__ recv_begin, uint32_t p, uint64_t k) {
+    uint32_t lo = 0, hi = p;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (recv_begin[mid] <= k) lo = mid + 1; else hi = mid;
+    }
+    return lo - 1;
+}
+
 // 2^lg lanes per received string copy its bytes into the receive arena.
 // With lg = 0 (short copies, such as the few characters after the LCP that
 // an LCP-compressed distinguishing prefix sends) every lane copies one
 // string, 32 characters at a time from five aligned 8-byte loads, so a wave
-// has 64 strings in flight.
+// has 64 strings in flight.  The lane (lane 0 of the string's lanes) also
+// stores the merge's cached key: the string's 8 characters at its
+// receiver's common prefix cp, from the lines the copy reads.
 __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* __restrict__ r_voff,
                             const uint32_t* __restrict__ r_vlen, const uint32_t* __restrict__ r_from,
                             const uint64_t* __restrict__ byte_off, int mode, uint64_t n, uint32_t lg,
-                            uint8_t* __restrict__ out) {
+                            uint8_t* __restrict__ out, const uint64_t* __restrict__ recv_begin, uint32_t p,
+                            const uint32_t* __restrict__ cp, uint64_t* __restrict__ keyc) {
     if (lg == 0) {
         for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
              k += (uint64_t)gridDim.x * blockDim.x) {
+            keyc[k] = load_key8(chars, r_voff[k], r_vlen[k], cp[receiver_of(recv_begin, p, k)]);
             const uint32_t from = r_from[k];
             const uint64_t a = r_voff[k] + from;
             const uint32_t m = r_vlen[k] - from;
@@ -514,6 +547,7 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
         const uint32_t from = r_from[k];
         const uint32_t vlen = r_vlen[k];
         uint8_t* dst = out + byte_off[k];
+        if (l == 0) keyc[k] = load_key8(chars, r_voff[k], vlen, cp[receiver_of(recv_begin, p, k)]);
         // modes 0/1: chars [from, vlen] incl. the NUL; mode 2: [from, vlen) + NUL
         const uint32_t m = vlen - from;
         for (uint32_t t = l; t <= m; t += lanes) dst[t] = (t == m) ? 0 : src[from + t];
@@ -603,13 +637,27 @@ void exchange(dss_set_s* s, int mode) {
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                            P.recv_from.as<uint32_t>());
     }
+    // the merge's per-receiver common prefix (from the views, before the copy)
+    P.m_keyc.reserve(nn * 8);
+    const uint64_t cp_bytes = ((uint64_t)(p + 1) * 4 + 7) / 8 * 8;
+    P.m_cp.reserve(cp_bytes + (p + 1) * 8);
+    uint32_t* cpd = P.m_cp.as<uint32_t>();
+    uint64_t* rb_d = reinterpret_cast<uint64_t*>(P.m_cp.as<uint8_t>() + cp_bytes);
+    DSS_HIP(hipMemcpyAsync(rb_d, P.recv_begin.data(), (p + 1) * 8, hipMemcpyHostToDevice, st));
+    DSS_HIP(hipMemsetAsync(cpd, 0xFF, (p + 1) * 4, st));
+    if (n) {
+        KScope ks(s->timer, st, "exchange_index", (double)p * p * 64.0);
+        hipLaunchKernelGGL(k_recv_cp, dim3(((uint64_t)p * p * 64 + 255) / 256), dim3(256), 0, st, ls + (uint64_t)p * p,
+                           P.interval_d.as<uint64_t>(), p, rb_d, s->arena(), P.recv_view_off.as<uint64_t>(),
+                           P.recv_view_len.as<uint32_t>(), cpd);
+    }
     uint64_t total = 0;
     exclusive_scan_u64(P.recv_size.as<uint64_t>(), P.recv_byte_off.as<uint64_t>(), n, st, P, &total);
     P.recv_nbytes = total;
     P.recv_bytes.reserve(total + kArenaPad);
     DSS_HIP(hipMemsetAsync(P.recv_bytes.as<uint8_t>() + total, 0, kArenaPad, st));
     if (n) {
-        KScope ks(s->timer, st, "exchange_copy", 2.0 * total + n * 24.0);
+        KScope ks(s->timer, st, "exchange_copy", 2.0 * total + n * 40.0);
         // a lane per string for copies of at most 32 characters on average,
         // else 8 lanes (short prefix-doubling prefixes) or 16 (whole strings)
         const uint32_t lg = total <= 33 * n ? 0 : mode == 2 ? 3 : 4;
@@ -617,7 +665,7 @@ void exchange(dss_set_s* s, int mode) {
         hipLaunchKernelGGL(k_exch_copy, dim3(stride_grid(threads, 256)), dim3(256), 0, st, s->arena(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                            P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), mode, n, lg,
-                           P.recv_bytes.as<uint8_t>());
+                           P.recv_bytes.as<uint8_t>(), rb_d, p, cpd, P.m_keyc.as<uint64_t>());
     }
     if (n && mode == 3) {
         KScope ks(s->timer, st, "exchange_lcp", 2.0 * total + n * 20.0);
@@ -688,38 +736,6 @@ __device__ inline int elem_cmp(const ViewCtx& V, uint32_t a, uint64_t ka, uint32
         from = V.cp + 8;
     }
     return view_cmp(V.chars, V.voff[a], V.vlen[a], V.voff[b], V.vlen[b], from, &lcp);
-}
-
-// Common prefix of all strings a PE received: min over its runs r of the
-// LCP of the PE's first string with the first and last string of run r.
-__global__ void k_recv_cp(const uint64_t* __restrict__ run_off, const uint64_t* __restrict__ interval, uint32_t p,
-                          const uint64_t* __restrict__ recv_begin, const uint8_t* __restrict__ chars,
-                          const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
-                          uint32_t* __restrict__ cp_out) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)p * p) return;
-    const uint32_t q = (uint32_t)(t / p), r = (uint32_t)(t % p);
-    const uint64_t c = interval[(uint64_t)r * p + q];
-    if (!c) return;
-    const uint64_t s0 = recv_begin[q];
-    const uint64_t f = run_off[(uint64_t)q * p + r], l = f + c - 1;
-    uint32_t x, y;
-    view_cmp(chars, voff[s0], vlen[s0], voff[f], vlen[f], 0, &x);
-    view_cmp(chars, voff[s0], vlen[s0], voff[l], vlen[l], 0, &y);
-    atomicMin(&cp_out[q], min(x, y));
-}
-
-__global__ void k_keycache(const uint64_t* __restrict__ recv_begin, uint32_t p, const uint8_t* __restrict__ chars,
-                           const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
-                           const uint32_t* __restrict__ cp, uint64_t n, uint64_t* __restrict__ keyc) {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    uint32_t lo = 0, hi = p;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (recv_begin[mid] <= k) lo = mid + 1; else hi = mid;
-    }
-    keyc[k] = load_key8(chars, voff[k], vlen[k], cp[lo - 1]);
 }
 
 __device__ inline uint32_t pair_of_tile(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t t) {
@@ -891,25 +907,16 @@ void merge(dss_set_s* s) {
     P.m_order2.reserve(nn * 4);
     P.m_lcp2.reserve(nn * 4);
     P.m_saved.reserve(nn * 4);
-    P.m_keyc.reserve(nn * 8);
-    const uint64_t cp_bytes = ((uint64_t)(p + 1) * 4 + 7) / 8 * 8;
-    P.m_cp.reserve(cp_bytes + (p + 1) * 8);
+    // the receivers' common prefixes and the cached keys at them were
+    // written by the exchange's copy
     uint32_t* cp = P.m_cp.as<uint32_t>();
-    uint64_t* rb_d = reinterpret_cast<uint64_t*>(P.m_cp.as<uint8_t>() + cp_bytes);
-    DSS_HIP(hipMemcpyAsync(rb_d, P.recv_begin.data(), (p + 1) * 8, hipMemcpyHostToDevice, st));
     uint32_t* ord = P.m_order.as<uint32_t>();
     uint32_t* lc = P.m_lcp.as<uint32_t>();
     uint32_t* ord2 = P.m_order2.as<uint32_t>();
     uint32_t* lc2 = P.m_lcp2.as<uint32_t>();
     if (n) {
+        KScope ks(s->timer, st, "merge_init", n * 16.0);
         hipLaunchKernelGGL(k_iota_copy, dim3((n + 255) / 256), dim3(256), 0, st, ord, lc, P.recv_lcp.as<uint32_t>(), n);
-        DSS_HIP(hipMemsetAsync(cp, 0xFF, (p + 1) * 4, st));
-        hipLaunchKernelGGL(k_recv_cp, dim3(((uint64_t)p * p + 255) / 256), dim3(256), 0, st,
-                           P.layout.as<uint64_t>() + (uint64_t)p * p, P.interval_d.as<uint64_t>(), p, rb_d,
-                           s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(), cp);
-        KScope ks(s->timer, st, "merge_keycache", n * 24.0);
-        hipLaunchKernelGGL(k_keycache, dim3((n + 255) / 256), dim3(256), 0, st, rb_d, p, s->arena(),
-                           P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(), cp, n, P.m_keyc.as<uint64_t>());
     }
     // runs per receiver (host), then pairwise rounds; receivers that already
     // hold a single run are carried along as one-run pairs
