@@ -176,6 +176,12 @@ struct PipeState {
     DevBuf pd_cand, pd_next, pd_hash, pd_flags, pd_table, pd_tflag, pd_ctr, pd_slot, pd_list, pd_maybe, pd_dup,
         pd_bits, pd_head, pd_hmax, pd_hpos, pd_hstate;
     DevBuf pd_pkey, pd_pid, pd_phist, pd_pbound;  // partitioned duplicate detection
+    // characters [32(stripes-1), depth) of strings hashed from a run head's
+    // state: one packed row per position and the depth it was read for;
+    // valid for the results of the last prefix doubling (stage_valid)
+    DevBuf pd_stage, pd_sdepth;
+    bool stage_valid = false;
+    uint32_t stage_w = 0;  // words per kept row: 4 + ceil((depth % 32) / 8)
     // partition
     bool partitioned = false;
     std::vector<uint64_t> interval;  // p*p, [r*p+q] = strings PE r sends to PE q
@@ -187,7 +193,7 @@ struct PipeState {
     uint64_t recv_nbytes = 0;
     DevBuf recv_bytes;
     DevBuf recv_byte_off, recv_lcp, recv_src_pe, recv_src_pos, recv_view_off, recv_view_len,
-        recv_from, recv_size;
+        recv_from, recv_size, recv_stage;
     std::vector<uint64_t> recv_begin, byte_begin;  // p+1 each
     DevBuf layout;                                 // send_start / run offsets (u64)
     // merge (per receive-side output position)

@@ -104,12 +104,13 @@ constexpr uint64_t kEmpty = ~0ull;  // never a hash value: hash % m < m <= 2^64-
 // at depth `dlast`) are already applied: in them every position stayed a
 // candidate with results = depth.
 __global__ void k_pd_init(uint8_t* __restrict__ cand, uint8_t* __restrict__ rounds, uint32_t* __restrict__ dist,
-                          uint64_t n, uint32_t skipped, uint32_t dlast) {
+                          uint64_t n, uint32_t skipped, uint32_t dlast, uint32_t* __restrict__ sdepth) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         cand[i] = 1;
         rounds[i] = (uint8_t)min(skipped, 255u);
         dist[i] = skipped ? dlast : 0;
+        if (sdepth) sdepth[i] = 0;
     }
 }
 
@@ -473,7 +474,8 @@ __global__ void __launch_bounds__(256)
 k_pd_hash_shared(const uint32_t* __restrict__ list, const unsigned long long* __restrict__ nhashed,
                  const uint64_t* __restrict__ s_off, const uint32_t* __restrict__ hrank,
                  const uint64_t* __restrict__ state, const uint8_t* __restrict__ chars, uint32_t depth, uint32_t L,
-                 uint64_t m, uint64_t* __restrict__ hashv) {
+                 uint64_t m, uint64_t* __restrict__ hashv, uint64_t* __restrict__ stage, uint32_t* __restrict__ sdepth,
+                 uint32_t stage_w) {
     const uint32_t j = threadIdx.x & 3, g = threadIdx.x >> 2;
     const uint32_t base_lane = threadIdx.x & ~3u;
     const uint32_t stripes = depth / 32;
@@ -483,7 +485,7 @@ k_pd_hash_shared(const uint32_t* __restrict__ list, const unsigned long long* __
     constexpr uint64_t kPer = 64 * kHashU;
     for (uint64_t blk = (uint64_t)blockIdx.x * kPer; blk < nh; blk += (uint64_t)gridDim.x * kPer) {
         uint32_t c[kHashU], rk[kHashU];
-        uint64_t own[kHashU], v[kHashU], tw[kHashU];
+        uint64_t own[kHashU], v[kHashU], tw[kHashU], last[kHashU];  // last: word j of the last full stripe
         bool hashed[kHashU];
 #pragma unroll
         for (uint32_t u = 0; u < kHashU; ++u) {
@@ -500,6 +502,7 @@ k_pd_hash_shared(const uint32_t* __restrict__ list, const unsigned long long* __
         for (uint32_t u = 0; u < kHashU; ++u) {
             v[u] = v0;
             tw[u] = 0;
+            last[u] = 0;
             if (hashed[u]) {
                 uint32_t k0 = 0;
                 if (!(rk[u] & kHeadBit)) {
@@ -507,7 +510,24 @@ k_pd_hash_shared(const uint32_t* __restrict__ list, const unsigned long long* __
                     k0 = L;
                 }
                 if (tw_in) tw[u] = load_word_le(chars, own[u] + 32 * stripes + 8 * j);
-                for (uint32_t k = k0; k < stripes; ++k) v[u] = xround(v[u], load_word_le(chars, own[u] + 32 * k + 8 * j));
+                for (uint32_t k = k0; k < stripes; ++k) {
+                    last[u] = load_word_le(chars, own[u] + 32 * k + 8 * j);
+                    v[u] = xround(v[u], last[u]);
+                }
+            }
+        }
+        if (stage) {
+            // characters [32(stripes-1), depth) for the exchange, stored after
+            // every string's loads are issued
+#pragma unroll
+            for (uint32_t u = 0; u < kHashU; ++u) {
+                if (!hashed[u]) continue;
+                // packed rows of stage_w words: consecutive positions fill
+                // whole lines (a partly written line costs a read as well)
+                uint64_t* row = stage + stage_w * (uint64_t)c[u];
+                row[j] = last[u];
+                if (tw_in) row[4 + j] = tw[u];
+                if (j == 0) sdepth[c[u]] = depth;
             }
         }
 #pragma unroll
@@ -924,6 +944,8 @@ static uint64_t partition_duplicates(dss_set_s* s, uint64_t nh, uint32_t table_l
     return nm;
 }
 
+constexpr uint64_t kStageMax = 16ull << 30;
+
 void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     DSS_REQUIRE(s->sorted, DSS_ESTATE, "dss_sort_lcp has not run");
     DSS_REQUIRE(m >= 1, DSS_EINVAL, "hash range m must be >= 1");
@@ -953,7 +975,14 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     uint64_t depth = start_depth, step = (uint64_t)start_depth * 2;
     // leading rounds in which every string has a neighbour sharing at least
     // the round's depth keep every candidate: they are applied at once
-    uint32_t skipped = 0, dlast = 0;
+    uint32_t skipped = 0, dlast = 0, stage_w = 0;
+    // the characters the last stripe and the tail of shared-prefix hashing
+    // read are kept for the exchange (at most kStageMax bytes)
+    const bool stage_on = n && n * 64 <= kStageMax;
+    if (stage_on) {
+        P.pd_stage.reserve(n * 64);
+        P.pd_sdepth.reserve(n * 4);
+    }
     if (n) {
         unsigned int minrun = 0;
         {
@@ -972,7 +1001,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
         }
         round = skipped;
         hipLaunchKernelGGL(k_pd_init, grid, blk, 0, st, cand, P.rounds.as<uint8_t>(), P.dist.as<uint32_t>(), n,
-                           skipped, dlast);
+                           skipped, dlast, stage_on ? P.pd_sdepth.as<uint32_t>() : nullptr);
     }
     // duplicate detection by partitioning from part_min hashed candidates on;
     // both settings can be lowered by the environment (tests cover the
@@ -1028,6 +1057,12 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
                                nheads, s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(), s->arena(), L,
                                P.pd_hstate.as<uint64_t>());
         }
+        if (shared && stage_on) {
+            // rows of another width would overlap the kept ones: drop those
+            const uint32_t w = 4 + ((uint32_t)(depth & 31) + 7) / 8;
+            if (stage_w && w != stage_w) DSS_HIP(hipMemsetAsync(P.pd_sdepth.p, 0, n * 4, st));
+            stage_w = w;
+        }
         {
             KScope ks(s->timer, st, "pd_hash", 0.0);
             const dim3 g(std::min<uint64_t>((nh + kPdPos - 1) / kPdPos, kPdGrid));
@@ -1035,7 +1070,9 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
             if (shared)
                 hipLaunchKernelGGL(k_pd_hash_shared, gs, blk, 0, st, P.pd_list.as<uint32_t>(), ctr,
                                    s->s_off.as<uint64_t>(), P.pd_head.as<uint32_t>(), P.pd_hstate.as<uint64_t>(),
-                                   s->arena(), (uint32_t)depth, L, m, P.pd_hash.as<uint64_t>());
+                                   s->arena(), (uint32_t)depth, L, m, P.pd_hash.as<uint64_t>(),
+                                   stage_on ? P.pd_stage.as<uint64_t>() : nullptr,
+                                   stage_on ? P.pd_sdepth.as<uint32_t>() : nullptr, stage_w);
             else if (nh)
                 hipLaunchKernelGGL(k_pd_hash, g, blk, 0, st, P.pd_list.as<uint32_t>(), ctr, s->s_off.as<uint64_t>(),
                                    s->arena(), (uint32_t)depth, m, P.pd_hash.as<uint64_t>());
@@ -1100,6 +1137,8 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     }
     P.pd_rounds = round;
     P.has_dist = true;
+    P.stage_valid = stage_on && stage_w;
+    P.stage_w = stage_w;
     P.partitioned = P.exchanged = P.merged = P.extended = false;
 }
 

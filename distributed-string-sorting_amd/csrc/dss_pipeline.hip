@@ -438,7 +438,8 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
                              const uint64_t* __restrict__ send_start, const uint64_t* __restrict__ run_off, int mode,
                              uint64_t n, uint64_t* __restrict__ size, uint32_t* __restrict__ r_lcp,
                              uint32_t* __restrict__ r_pe, uint32_t* __restrict__ r_pos, uint64_t* __restrict__ r_voff,
-                             uint32_t* __restrict__ r_vlen, uint32_t* __restrict__ r_from) {
+                             uint32_t* __restrict__ r_vlen, uint32_t* __restrict__ r_from,
+                             const uint32_t* __restrict__ sdepth, uint32_t* __restrict__ r_stage) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t lo = 0, hi = p;  // PE of position i: last r with pe_begin[r] <= i (non-empty)
@@ -472,6 +473,8 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
     r_voff[dst] = s_off[i];
     r_vlen[dst] = vlen;
     r_from[dst] = from;
+    // mode 2: the prefix doubling kept [32(vlen/32-1), vlen) of this string
+    if (r_stage) r_stage[dst] = (sdepth[i] == vlen && vlen >= 64) ? (uint32_t)i : kNone;
 }
 
 // Common prefix of all strings a PE received: min over its runs r of the
@@ -514,11 +517,50 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
                             const uint32_t* __restrict__ r_vlen, const uint32_t* __restrict__ r_from,
                             const uint64_t* __restrict__ byte_off, int mode, uint64_t n, uint32_t lg,
                             uint8_t* __restrict__ out, const uint64_t* __restrict__ recv_begin, uint32_t p,
-                            const uint32_t* __restrict__ cp, uint64_t* __restrict__ keyc) {
+                            const uint32_t* __restrict__ cp, uint64_t* __restrict__ keyc,
+                            const uint32_t* __restrict__ r_stage, const uint64_t* __restrict__ stage,
+                            uint32_t stage_w) {
     if (lg == 0) {
         for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
              k += (uint64_t)gridDim.x * blockDim.x) {
-            keyc[k] = load_key8(chars, r_voff[k], r_vlen[k], cp[receiver_of(recv_begin, p, k)]);
+            const uint32_t cpk = cp[receiver_of(recv_begin, p, k)];
+            const uint32_t si = r_stage ? r_stage[k] : kNone;
+            if (si != kNone) {
+                // the characters were kept by the prefix doubling's last hash
+                // of this string: row = [B, dist), B = 32 (dist/32 - 1), in
+                // stage_w words (the characters beyond dist are not used)
+                const uint32_t vlen = r_vlen[k], from = r_from[k];
+                const uint32_t B = 32 * (vlen / 32 - 1);
+                if (from >= B && cpk >= B) {
+                    const uint64_t* rw = stage + stage_w * (uint64_t)si;
+                    uint64_t row[9];
+#pragma unroll
+                    for (uint32_t i = 0; i < 9; ++i) row[i] = i < stage_w ? rw[i] : 0;
+                    // 8 characters at row byte x, little endian
+                    auto rword = [&](uint32_t x) -> uint64_t {
+                        const uint32_t w = x >> 3, sh = (x & 7) * 8;
+                        return sh ? (row[w] >> sh) | (row[w + 1] << (64 - sh)) : row[w];
+                    };
+                    uint64_t key = 0;
+                    if (cpk < vlen) {
+                        key = __builtin_bswap64(rword(cpk - B));
+                        const uint32_t rem = vlen - cpk;
+                        if (rem < 8) key &= ~0ull << (8 * (8 - rem));
+                    }
+                    keyc[k] = key;
+                    const uint32_t m = vlen - from;
+                    uint8_t* dst = out + byte_off[k];
+                    for (uint32_t t = 0; t < m; t += 8) {
+                        const uint64_t y = rword(from - B + t);
+#pragma unroll
+                        for (uint32_t b = 0; b < 8; ++b)
+                            if (t + b < m) dst[t + b] = (uint8_t)(y >> (8 * b));
+                    }
+                    dst[m] = 0;
+                    continue;
+                }
+            }
+            keyc[k] = load_key8(chars, r_voff[k], r_vlen[k], cpk);
             const uint32_t from = r_from[k];
             const uint64_t a = r_voff[k] + from;
             const uint32_t m = r_vlen[k] - from;
@@ -626,6 +668,9 @@ void exchange(dss_set_s* s, int mode) {
     P.recv_view_off.reserve(nn * 8);
     P.recv_view_len.reserve(nn * 4);
     P.recv_from.reserve(nn * 4);
+    const bool staged = mode == 2 && P.stage_valid;
+    if (staged) P.recv_stage.reserve(nn * 4);
+    uint32_t* r_stage = staged ? P.recv_stage.as<uint32_t>() : nullptr;
     const uint64_t* ls = P.layout.as<uint64_t>();
     if (n) {
         KScope ks(s->timer, st, "exchange_index", n * 40.0);
@@ -635,7 +680,7 @@ void exchange(dss_set_s* s, int mode) {
                            ls + (uint64_t)p * p, mode, n, P.recv_size.as<uint64_t>(), P.recv_lcp.as<uint32_t>(),
                            P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
-                           P.recv_from.as<uint32_t>());
+                           P.recv_from.as<uint32_t>(), staged ? P.pd_sdepth.as<uint32_t>() : nullptr, r_stage);
     }
     // the merge's per-receiver common prefix (from the views, before the copy)
     P.m_keyc.reserve(nn * 8);
@@ -665,7 +710,8 @@ void exchange(dss_set_s* s, int mode) {
         hipLaunchKernelGGL(k_exch_copy, dim3(stride_grid(threads, 256)), dim3(256), 0, st, s->arena(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                            P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), mode, n, lg,
-                           P.recv_bytes.as<uint8_t>(), rb_d, p, cpd, P.m_keyc.as<uint64_t>());
+                           P.recv_bytes.as<uint8_t>(), rb_d, p, cpd, P.m_keyc.as<uint64_t>(),
+                           lg == 0 ? r_stage : nullptr, staged ? P.pd_stage.as<uint64_t>() : nullptr, P.stage_w);
     }
     if (n && mode == 3) {
         KScope ks(s->timer, st, "exchange_lcp", 2.0 * total + n * 20.0);
@@ -1508,6 +1554,7 @@ extern "C" int dss_set_dist(dss_set s, const uint32_t* dist) {
         if (s->n) DSS_HIP(hipMemcpyAsync(P.dist.p, dist, s->n * 4, hipMemcpyHostToDevice, s->stream));
         DSS_HIP(hipStreamSynchronize(s->stream));
         P.has_dist = true;
+        P.stage_valid = false;  // the kept characters belong to the computed results
         P.partitioned = P.exchanged = P.merged = P.extended = false;
     });
 }

@@ -337,3 +337,29 @@ def test_exact_dist_and_prefix_doubling_bounds(name):
         ds.prefix_doubling_sort(m=64)
         c = ds.check_dist()
         assert c["lower_violations"] == 0
+
+
+@pytest.mark.parametrize("p", [4, 16])
+def test_prefix_doubling_exchange_from_kept_characters(p):
+    """Long shared prefixes: the last rounds hash from run heads' states and
+    keep each string's last stripe and tail, from which the exchange copies
+    the distinguishing prefixes and the merge's cached keys.  Receive bytes,
+    merged LCPs and the permutation match the oracle."""
+    dss = _dss()
+    chars = O.gen_dton(20000, 300, 0.5, 0x5EED0002)
+    ss = O.StringSet(chars, p)
+    res = O.prefix_doubling_sort(ss)
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        ds.prefix_doubling_sort()
+        gres, grounds = ds.get_prefix_doubling()
+        assert (gres == res["results"]).all() and (grounds == res["rounds"]).all()
+        ex = res["exchange"]
+        nbytes = int(ex["byte_begin"][-1])
+        assert ds.get_exchange_bytes(nbytes).tobytes() == ex["bytes"].tobytes()
+        gord, gml, gsv = ds.get_merged()
+        assert (gml == np.concatenate([o["lcp"] for o in res["out"]])).all()
+        si, pi = ds.get_permutation()
+        exp = [(int(a), int(b)) for o in res["out"] for a, b in zip(o["pe_index"], o["string_index"])]
+        assert list(zip(map(int, pi), map(int, si))) == exp  # D/N strings are distinct: no ties
+        chk = ds.check()
+        assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0
