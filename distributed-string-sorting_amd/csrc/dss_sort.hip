@@ -475,29 +475,41 @@ k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint6
 // One block per large segment: digit totals and offsets over the segment's
 // tiles, LCPs at bucket boundaries, and the sub-bucket lists.  List slots
 // are reserved with one global atomic per list per block.
-__global__ void __launch_bounds__(kThreads)
-k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
+// kScGroups segments per block, 256 threads (one per digit) each; the
+// block's small-class and large-list counts reach the global counters with
+// one atomic per list (same-address atomics of many small blocks serialise).
+constexpr uint32_t kScGroups = 4;
+
+__global__ void __launch_bounds__(kThreads * kScGroups)
+k_scan_classify(const Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
                 Seg* __restrict__ next, SmallSeg* __restrict__ small, SmallLists L, uint32_t large_min,
                 uint32_t* __restrict__ ctr, uint32_t* __restrict__ next_large, uint32_t* __restrict__ lcp,
                 uint32_t key_buf) {
-    __shared__ uint32_t sc[256];
-    __shared__ uint32_t first;
+    __shared__ uint32_t sc_all[kScGroups][256];
+    __shared__ uint32_t first_all[kScGroups];
     __shared__ uint32_t lcnt[kClasses + 1], lelem[kClasses], lbase[kClasses + 1];
-    const Seg S = segs[blockIdx.x];
-    uint32_t kd, dep;
-    seg_depths(S, kd, dep);
-    const uint32_t d = threadIdx.x;
-    const uint32_t nt = seg_tiles(S.cnt);
-    const uint32_t nc = seg_chunks(nt);
-    // digit totals over the tiles, or over the chunk sums of a long segment
-    uint32_t* const col = nc ? csum + (uint64_t)S.chunk0 * 256 : hist + (uint64_t)S.tile0 * 256;
-    const uint32_t nrow = nc ? nc : nt;
-    uint32_t tot = 0;
-    for (uint32_t t = 0; t < nrow; ++t) tot += col[(uint64_t)t * 256 + d];
+    const uint32_t grp = threadIdx.x >> 8;
+    const uint32_t d = threadIdx.x & 255;
+    const uint32_t si = blockIdx.x * kScGroups + grp;
+    const bool live = si < nseg;  // group-uniform
+    uint32_t* const sc = sc_all[grp];
+    Seg S{};
+    uint32_t kd = 0, dep = 0, tot = 0, nrow = 0;
+    uint32_t* col = nullptr;
+    if (live) {
+        S = segs[si];
+        seg_depths(S, kd, dep);
+        const uint32_t nt = seg_tiles(S.cnt);
+        const uint32_t nc = seg_chunks(nt);
+        // digit totals over the tiles, or over the chunk sums of a long segment
+        col = nc ? csum + (uint64_t)S.chunk0 * 256 : hist + (uint64_t)S.tile0 * 256;
+        nrow = nc ? nc : nt;
+        for (uint32_t t = 0; t < nrow; ++t) tot += col[(uint64_t)t * 256 + d];
+    }
     sc[d] = tot;
-    if (d == 0) first = 256;
-    if (d <= kClasses) lcnt[d] = 0;
-    if (d < kClasses) lelem[d] = 0;
+    if (d == 0) first_all[grp] = 256;
+    if (threadIdx.x <= kClasses) lcnt[threadIdx.x] = 0;
+    if (threadIdx.x < kClasses) lelem[threadIdx.x] = 0;
     __syncthreads();
     for (uint32_t o = 1; o < 256; o <<= 1) {
         uint32_t x = d >= o ? sc[d - o] : 0;
@@ -506,7 +518,7 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint3
         __syncthreads();
     }
     const uint32_t base = sc[d] - tot;
-    if (tot) atomicMin(&first, d);
+    if (tot) atomicMin(&first_all[grp], d);
     // exclusive offsets per tile (k_chunk_apply expands chunk offsets)
     uint32_t run = S.begin + base;
     for (uint32_t t = 0; t < nrow; ++t) {
@@ -517,7 +529,7 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint3
     }
     __syncthreads();
     const uint32_t b = S.begin + base;
-    if (tot && d != first) lcp[b] = dep;  // boundary between non-empty buckets
+    if (tot && d != first_all[grp]) lcp[b] = dep;  // boundary between non-empty buckets
     // d == 0: equal strings; tot == 1: a single string -- both done
     int list = -1;  // small class c, or kClasses for the large list
     uint32_t slot = 0;
@@ -527,11 +539,11 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t* __restrict__ hist, uint3
         if (list < (int)kClasses) atomicAdd(&lelem[list], tot);
     }
     __syncthreads();
-    if (d < kClasses && lcnt[d]) {
-        lbase[d] = atomicAdd(&ctr[kCtrSmall + d], lcnt[d]);
-        atomicAdd(&ctr[kCtrSmallElems + d], lelem[d]);
+    if (threadIdx.x < kClasses && lcnt[threadIdx.x]) {
+        lbase[threadIdx.x] = atomicAdd(&ctr[kCtrSmall + threadIdx.x], lcnt[threadIdx.x]);
+        atomicAdd(&ctr[kCtrSmallElems + threadIdx.x], lelem[threadIdx.x]);
     }
-    if (d == kClasses && lcnt[kClasses]) lbase[kClasses] = atomicAdd(next_large, lcnt[kClasses]);
+    if (threadIdx.x == kClasses && lcnt[kClasses]) lbase[kClasses] = atomicAdd(next_large, lcnt[kClasses]);
     __syncthreads();
     if (list < 0) return;
     if (list < (int)kClasses) {
@@ -1102,8 +1114,9 @@ void sort_lcp(dss_set_s* s) {
             if (nchunks)
                 hipLaunchKernelGGL(k_chunk_sum, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
                                    X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
-            hipLaunchKernelGGL(k_scan_classify, dim3(nlarge), dim3(kThreads), 0, st, cur, X.hist.as<uint32_t>(),
-                               X.csum.as<uint32_t>(), nxt, X.small.as<SmallSeg>(), L, T.large_min, ctr,
+            hipLaunchKernelGGL(k_scan_classify, dim3((nlarge + kScGroups - 1) / kScGroups), dim3(kThreads * kScGroups),
+                               0, st, cur, nlarge, X.hist.as<uint32_t>(), X.csum.as<uint32_t>(), nxt,
+                               X.small.as<SmallSeg>(), L, T.large_min, ctr,
                                pn + kCtrLarge, lcp, kn == X.keyB.as<uint64_t>() ? 1u : 0u);
             if (nchunks)
                 hipLaunchKernelGGL(k_chunk_apply, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
