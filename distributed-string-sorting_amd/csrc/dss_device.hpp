@@ -122,6 +122,28 @@ __device__ inline int view_cmp(const uint8_t* __restrict__ chars, uint64_t ao, u
     }
 }
 
+// view_cmp from character 0 with 32 characters per step (five aligned loads
+// per string and step): fewer dependent steps for long common prefixes.
+__device__ inline int view_cmp32(const uint8_t* __restrict__ chars, uint64_t ao, uint32_t al, uint64_t bo,
+                                 uint32_t bl, uint32_t* lcp_out) {
+    for (uint32_t d = 0;; d += 32) {
+        uint64_t x[4], y[4];
+        load_keys4(chars, ao, al, d, x);
+        load_keys4(chars, bo, bl, d, y);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (x[j] != y[j]) {
+                if (lcp_out) *lcp_out = d + 8 * j + key_lcp(x[j], y[j]);
+                return x[j] < y[j] ? -1 : 1;
+            }
+            if ((x[j] & 0xFF) == 0) {
+                if (lcp_out) *lcp_out = d + 8 * j + key_strlen(x[j]);
+                return 0;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- wave helpers
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 

@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(256) k_sample_rank(const Sample* __restrict__ 
     for (uint32_t j = threadIdx.x; j < ns; j += blockDim.x) {
         if (j == i) continue;
         const Sample b = smp[j];
-        const int r = view_cmp(chars, b.off, b.tlen, a.off, a.tlen, 0, nullptr);
+        const int r = view_cmp32(chars, b.off, b.tlen, a.off, a.tlen, nullptr);
         if (r < 0) ++c;
         else if (r == 0 && (indexed ? (b.gidx < a.gidx) : (j < i))) ++c;
     }
@@ -1310,7 +1310,7 @@ __global__ void k_check(const uint32_t* __restrict__ ord, const uint32_t* __rest
     }
     const uint32_t b = ord[i - 1];
     uint32_t l;
-    const int c = view_cmp(chars, voff[b], vlen[b], voff[a], vlen[a], 0, &l);
+    const int c = view_cmp32(chars, voff[b], vlen[b], voff[a], vlen[a], &l);
     if (c > 0) atomicAdd(&cnt[0], 1ull);  // order violation
     uint32_t lo = 0, hi = p;
     while (lo < hi) {
@@ -1363,25 +1363,9 @@ __global__ void k_full_glcp(const uint32_t* __restrict__ ord, const uint32_t* __
     }
     const uint32_t b = ord[i - 1];
     const uint64_t pb = pe_begin[rpe[b]] + rpos[b];
-    const uint64_t ao = s_off[pa], bo = s_off[pb];
-    const uint32_t al = s_len[pa], bl = s_len[pb];
-    // 32 characters per step from aligned loads (keys are zero after a
-    // string's end, so equal strings stop at their common length)
-    for (uint32_t d = 0;; d += 32) {
-        uint64_t x[4], y[4];
-        load_keys4(chars, ao, al, d, x);
-        load_keys4(chars, bo, bl, d, y);
-        uint32_t l = kNone;
-#pragma unroll
-        for (int j = 3; j >= 0; --j) {  // the first event wins
-            if (x[j] != y[j]) l = d + 8 * j + key_lcp(x[j], y[j]);
-            else if ((x[j] & 0xFF) == 0) l = d + 8 * j + key_strlen(x[j]);
-        }
-        if (l != kNone) {
-            glcp[i] = l;
-            return;
-        }
-    }
+    uint32_t l = 0;
+    (void)view_cmp32(chars, s_off[pb], s_len[pb], s_off[pa], s_len[pa], &l);
+    glcp[i] = l;
 }
 
 __global__ void k_exact_dist(const uint32_t* __restrict__ glcp, const uint64_t* __restrict__ gpos, uint64_t n,
