@@ -54,6 +54,7 @@ namespace dss {
 namespace {
 
 constexpr uint64_t kEnd = ~0ull;
+constexpr uint64_t kNone64 = ~0ull - 1;  // no exit found yet
 constexpr uint32_t kChunkBits = 4096;  // speculative decode chunk (64 words)
 constexpr uint32_t kChunkWords = kChunkBits / 64;
 constexpr uint32_t kVBytes = 16;      // bytes per lane in the varint decode
@@ -360,58 +361,102 @@ __global__ void k_golomb_spec(const uint64_t* __restrict__ words, const GBlock* 
     E[c] = pos;
 }
 
+// One round of the fixed point of chunk exits: a chunk's exit given the exit
+// of its predecessor (its entry).  Entering at its first bit, the
+// speculative exit holds; otherwise the chunk is decoded from the entry
+// until the walk meets a speculative start (from there on both walks agree)
+// or leaves the chunk.
 __global__ void k_golomb_sync(const uint64_t* __restrict__ words, const GBlock* __restrict__ blk, uint32_t nb,
                               uint64_t nchunks, const uint64_t* __restrict__ S, const uint64_t* __restrict__ E,
-                              const uint64_t* __restrict__ Xin, uint64_t* __restrict__ Xout, uint64_t* __restrict__ T,
+                              const uint64_t* __restrict__ Xin, uint64_t* __restrict__ Xout,
                               uint32_t* __restrict__ changed) {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nchunks) return;
     const GBlock B = blk[find_block(blk, nb, c)];
     const uint64_t* W = words + B.bitw0;
     const uint64_t cs = (c - B.chunk0) * kChunkBits, ce = min<uint64_t>(cs + kChunkBits, B.nbits);
-    const uint64_t w0 = cs / 64, w1 = (ce + 63) / 64;  // the chunk's bitmap words
     const uint64_t* Sb = S + B.bitw0;
-    uint64_t* Tb = T + B.bitw0;
     const uint64_t entry = c == B.chunk0 ? 0 : Xin[c - 1];
     uint64_t x;
     if (entry == cs) {
-        for (uint64_t w = w0; w < w1; ++w) Tb[w] = Sb[w];
         x = E[c];
+    } else if (entry == kEnd || entry >= ce) {
+        x = entry;  // no code starts in this chunk
     } else {
-        for (uint64_t w = w0; w < w1; ++w) Tb[w] = 0;
-        if (entry == kEnd || entry >= ce) {
-            x = entry;  // no code starts in this chunk
-        } else {
-            StartBits tb(Tb);
-            uint64_t pos = entry;
-            bool synced = false;
-            while (pos < ce) {
-                if ((Sb[pos >> 6] >> (63 - (pos & 63))) & 1) {
-                    synced = true;  // from here on the speculative walk is exact
-                    break;
-                }
-                uint64_t v;
-                const uint64_t nx = golomb_next(W, B, pos, &v);
-                if (nx == kEnd) {
-                    pos = kEnd;
-                    break;
-                }
-                tb.set(pos);
-                pos = nx;
+        uint64_t pos = entry;
+        x = kNone64;
+        while (pos < ce) {
+            if ((Sb[pos >> 6] >> (63 - (pos & 63))) & 1) {
+                x = E[c];  // joined the speculative walk
+                break;
             }
-            tb.flush();
-            if (synced) {
-                const uint64_t pw = pos >> 6;
-                Tb[pw] |= Sb[pw] & (~0ull >> (pos & 63));
-                for (uint64_t w = pw + 1; w < w1; ++w) Tb[w] = Sb[w];
-                x = E[c];
-            } else {
-                x = pos;
+            uint64_t v;
+            const uint64_t nx = golomb_next(W, B, pos, &v);
+            if (nx == kEnd) {
+                pos = kEnd;
+                break;
             }
+            pos = nx;
         }
+        if (x == kNone64) x = pos;
     }
     Xout[c] = x;
     if (x != Xin[c]) atomicOr(changed, 1u);
+}
+
+// The exact start bitmap T (a copy of S on entry) after the fixed point:
+// a chunk entered elsewhere than at its first bit replaces its speculative
+// starts before the point where the walks join by the decoded ones.
+__global__ void k_golomb_starts(const uint64_t* __restrict__ words, const GBlock* __restrict__ blk, uint32_t nb,
+                                uint64_t nchunks, const uint64_t* __restrict__ S, const uint64_t* __restrict__ X,
+                                uint64_t* __restrict__ T) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunks) return;
+    const GBlock B = blk[find_block(blk, nb, c)];
+    const uint64_t* W = words + B.bitw0;
+    const uint64_t cs = (c - B.chunk0) * kChunkBits, ce = min<uint64_t>(cs + kChunkBits, B.nbits);
+    const uint64_t* Sb = S + B.bitw0;
+    uint64_t* Tb = T + B.bitw0;
+    const uint64_t entry = c == B.chunk0 ? 0 : X[c - 1];
+    if (entry == cs) return;  // T = S here
+    const uint64_t w0 = cs / 64, w1 = (ce + 63) / 64;
+    if (entry == kEnd || entry >= ce) {
+        for (uint64_t w = w0; w < w1; ++w) Tb[w] = 0;
+        return;
+    }
+    uint64_t pos = entry, w = w0, acc = 0;
+    bool synced = false;
+    while (pos < ce) {
+        if ((Sb[pos >> 6] >> (63 - (pos & 63))) & 1) {
+            synced = true;
+            break;
+        }
+        uint64_t v;
+        const uint64_t nx = golomb_next(W, B, pos, &v);
+        if (nx == kEnd) {
+            pos = kEnd;
+            break;
+        }
+        for (; w < (pos >> 6); ++w) {  // words before the start's word are complete
+            Tb[w] = acc;
+            acc = 0;
+        }
+        acc |= 1ull << (63 - (pos & 63));
+        pos = nx;
+    }
+    if (synced) {
+        const uint64_t pw = pos >> 6;
+        for (; w < pw; ++w) {
+            Tb[w] = acc;
+            acc = 0;
+        }
+        Tb[pw] = acc | (Sb[pw] & (~0ull >> (pos & 63)));  // later words keep S
+    } else {
+        for (; w < w1; ++w) {
+            Tb[w] = acc;
+            acc = 0;
+        }
+    }
 }
 
 // Value slots: block j owns slot chunk0_j + j (its raw value) and one slot
@@ -750,8 +795,8 @@ static void golomb_decode(dss_codec_s* c, int layout, const uint64_t* words, uin
             DSS_REQUIRE(it <= nch + 1, DSS_EHIP, "Golomb decode did not converge");
             DSS_HIP(hipMemsetAsync(dflag + 2, 0, 4, st));
             {
-                KScope ks(c->timer, st, "golomb_sync", nwords * 24.0);
-                hipLaunchKernelGGL(k_golomb_sync, dim3(gch), dim3(256), 0, st, words, dB, nb, nch, S, E, Xa, Xb, T,
+                KScope ks(c->timer, st, "golomb_sync", nch * 24.0);
+                hipLaunchKernelGGL(k_golomb_sync, dim3(gch), dim3(256), 0, st, words, dB, nb, nch, S, E, Xa, Xb,
                                    dflag + 2);
             }
             check_launch();
@@ -761,6 +806,9 @@ static void golomb_decode(dss_codec_s* c, int layout, const uint64_t* words, uin
             std::swap(Xa, Xb);
             if (!ch) break;
         }
+        KScope ks(c->timer, st, "golomb_starts", nwords * 16.0);
+        DSS_HIP(hipMemcpyAsync(T, S, nwords * 8, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_golomb_starts, dim3(gch), dim3(256), 0, st, words, dB, nb, nch, S, Xa, T);
     }
     const uint32_t gcb = (uint32_t)std::max<uint64_t>((std::max<uint64_t>(nch, nb) + 255) / 256, 1);
     uint64_t total = 0;
