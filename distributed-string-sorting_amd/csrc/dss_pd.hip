@@ -894,8 +894,11 @@ static uint64_t partition_duplicates(dss_set_s* s, uint64_t nh, uint32_t table_l
     PipeState& P = s->pp;
     const hipStream_t st = s->stream;
     unsigned long long* ctr = P.pd_ctr.as<unsigned long long>();
-    uint32_t kb = 2;  // bucket bits: about 384 candidates per bucket at most on average
-    while (kb < 2 * kHpMaxBits && (nh >> kb) > 384) ++kb;
+    // bucket bits: at most about 768 candidates per bucket on average (the
+    // LDS table holds kHpCap * 7/8), so the digit passes stay narrow and
+    // their runs per tile long (short runs leave partly written lines)
+    uint32_t kb = 2;
+    while (kb < 2 * kHpMaxBits && (nh >> kb) > 768) ++kb;
     const uint32_t b1 = kb / 2, b0 = kb - b1;  // pass 0: the lower b0 of the top kb bits; pass 1: the top b1
     const uint64_t ntiles = (nh + kHpTile - 1) / kHpTile;
     const uint32_t nbk = 1u << kb;
