@@ -270,8 +270,22 @@ __global__ void k_gen_dton(uint8_t* __restrict__ out, uint64_t n, uint32_t k, ui
     const uint32_t pow26[7] = {1u, 26u, 676u, 17576u, 456976u, 11881376u, 308915776u};
     const uint32_t stride = slen + 1;
     const uint64_t total = n * (uint64_t)stride;
+    const uint32_t dstart = k > 7 ? k - 7 : 0;  // characters before are 'A'
+    const uint64_t wordA = 0x4141414141414141ull, wordF = 0x0101010101010101ull * fill;
     // a wave covers a 4096-byte chunk, lane l its 64 bytes [64l, 64l + 64):
-    // one 64-bit division per chunk, string ids only where a string starts
+    // one 64-bit division per chunk.  A word inside one string's 'A' padding
+    // or fill is constant; only words that touch the last 7 digit characters
+    // or a NUL are built byte by byte, and a string's id (the shuffle) is
+    // computed only for its digit characters.  The words go through LDS so
+    // that the wave stores whole lines (lane l writes words l, l + 64, ...).
+    // The ids of the chunk's first kGenIds strings are computed by the whole
+    // wave at once (one string per lane and round) into LDS: a lane that
+    // computed the shuffle on its own would keep the rest of the wave idle.
+    constexpr uint32_t kGenIds = 256;
+    __shared__ uint64_t stage[4][512];
+    __shared__ uint32_t sid[4][kGenIds];
+    uint64_t* const sw = stage[(threadIdx.x >> 6) & 3];
+    uint32_t* const si = sid[(threadIdx.x >> 6) & 3];
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t nchunks = (words + 511) / 512;
     for (uint64_t ch = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; ch < nchunks;
@@ -279,34 +293,62 @@ __global__ void k_gen_dton(uint8_t* __restrict__ out, uint64_t n, uint32_t k, ui
         const uint64_t a0 = ch * 4096;
         const uint64_t j0 = a0 / stride;
         const uint32_t pos0 = (uint32_t)(a0 - j0 * stride);
+        const uint32_t nstr = (uint32_t)min<uint64_t>((pos0 + 4095) / stride + 1, kGenIds);
+        for (uint32_t t = lane; t < ((nstr + 63) & ~63u); t += 64) {  // wave-uniform trip count
+            const uint64_t jj = j0 + t;
+            const uint32_t v = jj < n ? (uint32_t)shuffle_index(jj, n, seed) : 0u;
+            if (t < nstr) si[t] = v;
+        }
+        __builtin_amdgcn_wave_barrier();
         const uint32_t rel = pos0 + 64 * lane;
         uint64_t j = j0 + rel / stride;
         uint32_t pos = rel % stride;
-        uint32_t id = j < n ? (uint32_t)shuffle_index(j, n, seed) : 0u;
+        uint64_t idj = ~0ull;  // string whose id is in `id`
+        uint32_t id = 0;
         const uint64_t a = a0 + 64 * lane;
         for (uint32_t wd = 0; wd < 8; ++wd) {
             const uint64_t w = (a >> 3) + wd;
             if (w >= words) break;
-            uint64_t word = 0;
-            for (uint32_t b = 0; b < 8; ++b) {
-                uint8_t c = 0;
-                if (a + 8 * wd + b < total) {
-                    if (pos == slen) c = 0;
-                    else if (pos >= k) c = fill;
-                    else {
-                        const uint32_t e = k - 1 - pos;  // digit index from the right
-                        c = (uint8_t)('A' + (e < 7 ? (id / pow26[e]) % 26u : 0u));
+            const uint64_t ab = a + 8 * wd;
+            uint64_t word;
+            if (ab + 8 <= total && pos + 8 <= slen && (pos + 8 <= dstart || pos >= k)) {
+                word = pos >= k ? wordF : wordA;
+                pos += 8;
+            } else {
+                word = 0;
+                for (uint32_t b = 0; b < 8; ++b) {
+                    uint8_t c = 0;
+                    if (ab + b < total) {
+                        if (pos == slen) c = 0;
+                        else if (pos >= k) c = fill;
+                        else {
+                            const uint32_t e = k - 1 - pos;  // digit index from the right
+                            if (e < 7) {
+                                if (idj != j) {
+                                    idj = j;
+                                    id = j - j0 < kGenIds ? si[j - j0]
+                                                          : (j < n ? (uint32_t)shuffle_index(j, n, seed) : 0u);
+                                }
+                                c = (uint8_t)('A' + (id / pow26[e]) % 26u);
+                            } else {
+                                c = 'A';
+                            }
+                        }
+                    }
+                    word |= (uint64_t)c << (8 * b);
+                    if (++pos == stride) {
+                        pos = 0;
+                        ++j;
                     }
                 }
-                word |= (uint64_t)c << (8 * b);
-                if (++pos == stride) {
-                    pos = 0;
-                    ++j;
-                    id = j < n ? (uint32_t)shuffle_index(j, n, seed) : 0u;
-                }
             }
-            reinterpret_cast<uint64_t*>(out)[w] = word;
+            sw[8 * lane + wd] = word;
         }
+        __builtin_amdgcn_wave_barrier();  // the wave's LDS words are written
+        const uint64_t w0 = a0 >> 3;
+        for (uint32_t q = lane; q < 512; q += 64)
+            if (w0 + q < words) reinterpret_cast<uint64_t*>(out)[w0 + q] = sw[q];
+        __builtin_amdgcn_wave_barrier();  // read before the next chunk overwrites
     }
 }
 
