@@ -262,30 +262,43 @@ void compute_pe_split(dss_set_s* s) {
 // DNRatioGenerator::getRawStringsTimoStyle (random_string_generator.hpp:188-
 // 235) with a seeded shuffle: arena position j holds string i = shuffle(j),
 // the base-26 digits of i ('A'..'Z', padded with 'A') in the first k chars,
-// then slen-k copies of the fill character.  One thread per 8-byte word.
-__global__ void k_gen_dton(uint8_t* __restrict__ out, uint64_t n, uint32_t k, uint32_t slen,
-                           uint8_t fill, uint64_t seed, uint64_t words) {
-    // base-26 digit weights; string ids are < 2^32 < 26^7, so digits at
-    // weight 26^7 and above are 'A'
-    const uint32_t pow26[7] = {1u, 26u, 676u, 17576u, 456976u, 11881376u, 308915776u};
+// then slen-k copies of the fill character.
+//
+// First pass: the shuffled id of every arena position, one lane per string
+// (the Feistel walk of one string per lane keeps every lane busy).
+__global__ void k_dton_ids(uint32_t* __restrict__ ids, uint64_t n, uint64_t seed) {
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+         j += (uint64_t)gridDim.x * blockDim.x)
+        ids[j] = (uint32_t)shuffle_index(j, n, seed);
+}
+
+// Character at position p of a string with digit word dw (byte e = base-26
+// digit e of its id, as a letter); positions before dstart = max(k - 7, 0)
+// are 'A' (ids are < 2^32 < 26^7).
+__device__ inline uint8_t dton_char(uint32_t p, uint32_t k, uint32_t dstart, uint32_t slen, uint8_t fill,
+                                    uint64_t dw) {
+    if (p == slen) return 0;
+    if (p >= k) return fill;
+    if (p < dstart) return 'A';
+    return (uint8_t)(dw >> (8 * (k - 1 - p)));
+}
+
+// Second pass: a wave builds a 4096-byte chunk in LDS and stores it as whole
+// lines.  Every word is first written as 'A's or fill characters by the
+// region of its first byte; that is exact except at the digits, the 8 bytes
+// after them, a string's first 8 bytes and its NUL, which one lane per string
+// then rewrites byte by byte.
+__global__ void __launch_bounds__(256)
+k_gen_dton(uint8_t* __restrict__ out, uint64_t n, uint32_t k, uint32_t slen, uint8_t fill,
+           const uint32_t* __restrict__ ids, uint64_t words) {
     const uint32_t stride = slen + 1;
     const uint64_t total = n * (uint64_t)stride;
-    const uint32_t dstart = k > 7 ? k - 7 : 0;  // characters before are 'A'
+    const uint32_t dstart = k > 7 ? k - 7 : 0;
     const uint64_t wordA = 0x4141414141414141ull, wordF = 0x0101010101010101ull * fill;
-    // a wave covers a 4096-byte chunk, lane l its 64 bytes [64l, 64l + 64):
-    // one 64-bit division per chunk.  A word inside one string's 'A' padding
-    // or fill is constant; only words that touch the last 7 digit characters
-    // or a NUL are built byte by byte, and a string's id (the shuffle) is
-    // computed only for its digit characters.  The words go through LDS so
-    // that the wave stores whole lines (lane l writes words l, l + 64, ...).
-    // The ids of the chunk's first kGenIds strings are computed by the whole
-    // wave at once (one string per lane and round) into LDS: a lane that
-    // computed the shuffle on its own would keep the rest of the wave idle.
-    constexpr uint32_t kGenIds = 256;
+    const uint32_t inc = 512u % stride;  // position step between a lane's words
     __shared__ uint64_t stage[4][512];
-    __shared__ uint32_t sid[4][kGenIds];
     uint64_t* const sw = stage[(threadIdx.x >> 6) & 3];
-    uint32_t* const si = sid[(threadIdx.x >> 6) & 3];
+    uint8_t* const sb = reinterpret_cast<uint8_t*>(sw);
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t nchunks = (words + 511) / 512;
     for (uint64_t ch = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; ch < nchunks;
@@ -293,62 +306,54 @@ __global__ void k_gen_dton(uint8_t* __restrict__ out, uint64_t n, uint32_t k, ui
         const uint64_t a0 = ch * 4096;
         const uint64_t j0 = a0 / stride;
         const uint32_t pos0 = (uint32_t)(a0 - j0 * stride);
-        const uint32_t nstr = (uint32_t)min<uint64_t>((pos0 + 4095) / stride + 1, kGenIds);
-        for (uint32_t t = lane; t < ((nstr + 63) & ~63u); t += 64) {  // wave-uniform trip count
-            const uint64_t jj = j0 + t;
-            const uint32_t v = jj < n ? (uint32_t)shuffle_index(jj, n, seed) : 0u;
-            if (t < nstr) si[t] = v;
+        // lane l builds words l, l + 64, ... of the chunk
+        uint32_t pos = (pos0 + 8 * lane) % stride;
+        for (uint32_t q = lane; q < 512; q += 64) {
+            const uint64_t ab = a0 + 8ull * q;
+            uint64_t v = pos < k ? wordA : wordF;
+            if (ab + 8 > total) v = ab >= total ? 0 : v & ((1ull << (8 * (total - ab))) - 1);
+            sw[q] = v;
+            pos += inc;
+            if (pos >= stride) pos -= stride;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
-        const uint32_t rel = pos0 + 64 * lane;
-        uint64_t j = j0 + rel / stride;
-        uint32_t pos = rel % stride;
-        uint64_t idj = ~0ull;  // string whose id is in `id`
-        uint32_t id = 0;
-        const uint64_t a = a0 + 64 * lane;
-        for (uint32_t wd = 0; wd < 8; ++wd) {
-            const uint64_t w = (a >> 3) + wd;
-            if (w >= words) break;
-            const uint64_t ab = a + 8 * wd;
-            uint64_t word;
-            if (ab + 8 <= total && pos + 8 <= slen && (pos + 8 <= dstart || pos >= k)) {
-                word = pos >= k ? wordF : wordA;
-                pos += 8;
-            } else {
-                word = 0;
-                for (uint32_t b = 0; b < 8; ++b) {
-                    uint8_t c = 0;
-                    if (ab + b < total) {
-                        if (pos == slen) c = 0;
-                        else if (pos >= k) c = fill;
-                        else {
-                            const uint32_t e = k - 1 - pos;  // digit index from the right
-                            if (e < 7) {
-                                if (idj != j) {
-                                    idj = j;
-                                    id = j - j0 < kGenIds ? si[j - j0]
-                                                          : (j < n ? (uint32_t)shuffle_index(j, n, seed) : 0u);
-                                }
-                                c = (uint8_t)('A' + (id / pow26[e]) % 26u);
-                            } else {
-                                c = 'A';
-                            }
-                        }
-                    }
-                    word |= (uint64_t)c << (8 * b);
-                    if (++pos == stride) {
-                        pos = 0;
-                        ++j;
-                    }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        // strings that overlap the chunk: j0 .. j0 + nstr - 1
+        const uint64_t aend = min<uint64_t>(a0 + 4096, total);
+        const uint64_t nstr = a0 < total ? (aend - 1) / stride - j0 + 1 : 0;
+        for (uint64_t t = lane; t < nstr; t += 64) {
+            const uint64_t j = j0 + t;
+            const uint64_t sj = j * stride;
+            // the chunk's part of the string: positions [lo, hi)
+            const uint32_t lo = a0 > sj ? (uint32_t)(a0 - sj) : 0u;
+            const uint32_t hi = (uint32_t)min<uint64_t>(aend - sj, stride);
+            uint64_t dw = 0;
+            if (lo < k && hi > dstart) {  // digits inside the chunk
+                uint32_t x = ids[j];
+                for (uint32_t e = 0; e < 7; ++e) {
+                    const uint32_t qd = x / 26u;
+                    dw |= (uint64_t)('A' + (x - 26u * qd)) << (8 * e);
+                    x = qd;
                 }
             }
-            sw[8 * lane + wd] = word;
+            const uint32_t r1 = min(8u, stride);
+            for (uint32_t p = lo; p < min(hi, r1); ++p)
+                sb[sj + p - a0] = dton_char(p, k, dstart, slen, fill, dw);
+            const uint32_t r2e = min(k + 8, stride);
+            for (uint32_t p = max(lo, max(dstart, r1)); p < min(hi, r2e); ++p)
+                sb[sj + p - a0] = dton_char(p, k, dstart, slen, fill, dw);
+            if (slen >= lo && slen < hi) sb[sj + slen - a0] = 0;
         }
-        __builtin_amdgcn_wave_barrier();  // the wave's LDS words are written
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const uint64_t w0 = a0 >> 3;
         for (uint32_t q = lane; q < 512; q += 64)
             if (w0 + q < words) reinterpret_cast<uint64_t*>(out)[w0 + q] = sw[q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();  // read before the next chunk overwrites
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
 }
 
@@ -742,10 +747,13 @@ int dss_set_create_dton(uint64_t n, uint32_t len, double dton, uint64_t seed, ui
             s->chars.reserve(s->nbytes + kArenaPad);
             DSS_HIP(hipMemsetAsync(s->chars.as<uint8_t>() + s->nbytes, 0, kArenaPad, s->stream));
             const uint64_t words = (s->nbytes + 7) / 8;
-            hipLaunchKernelGGL(k_gen_dton, dim3(stride_grid((words + 511) / 512, 4)), dim3(256), 0, s->stream,
-                               s->chars.as<uint8_t>(), n, k, slen, fill, seed, words);
             s->off.reserve(n * sizeof(uint64_t));
             s->len.reserve(n * sizeof(uint32_t));
+            // the lengths array holds the shuffled ids until it is filled
+            hipLaunchKernelGGL(k_dton_ids, dim3(stride_grid(n, 256)), dim3(256), 0, s->stream,
+                               s->len.as<uint32_t>(), n, seed);
+            hipLaunchKernelGGL(k_gen_dton, dim3(stride_grid((words + 511) / 512, 4)), dim3(256), 0, s->stream,
+                               s->chars.as<uint8_t>(), n, k, slen, fill, s->len.as<const uint32_t>(), words);
             hipLaunchKernelGGL(k_fixed_offsets, dim3((n + 255) / 256), dim3(256), 0, s->stream,
                                s->off.as<uint64_t>(), s->len.as<uint32_t>(), n, slen);
             DSS_HIP(hipGetLastError());
@@ -1172,8 +1180,11 @@ int dss_pool_add_job_dton(dss_pool P, uint64_t n, uint32_t length, double dton, 
         const uint32_t id = pool_new_job(P, nbytes, n, b0, s0);
         P->jobs[id].max_len = slen;
         const uint64_t words = (nbytes + 7) / 8;
-        hipLaunchKernelGGL(k_gen_dton, dim3(stride_grid((words + 511) / 512, 4)), dim3(256), 0, P->stream, P->chars.as<uint8_t>() + b0, n, k,
-                           slen, fill, seed, words);
+        // the job's lengths hold the shuffled ids until they are filled
+        hipLaunchKernelGGL(k_dton_ids, dim3(stride_grid(n, 256)), dim3(256), 0, P->stream,
+                           P->len.as<uint32_t>() + s0, n, seed);
+        hipLaunchKernelGGL(k_gen_dton, dim3(stride_grid((words + 511) / 512, 4)), dim3(256), 0, P->stream,
+                           P->chars.as<uint8_t>() + b0, n, k, slen, fill, P->len.as<const uint32_t>() + s0, words);
         hipLaunchKernelGGL(k_offsets_base, dim3((n + 255) / 256), dim3(256), 0, P->stream, P->off.as<uint64_t>() + s0,
                            P->len.as<uint32_t>() + s0, n, slen, b0);
         DSS_HIP(hipGetLastError());
