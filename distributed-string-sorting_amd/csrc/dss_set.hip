@@ -396,37 +396,77 @@ __global__ void k_skew_len(SkewP P, uint64_t seed, uint64_t* __restrict__ size_o
     len_out[j] = L;
 }
 
-// One wave per string.
-__global__ void k_skew_fill(SkewP P, uint64_t seed, const uint64_t* __restrict__ off,
-                            uint8_t* __restrict__ out) {
-    // one wave per string; lane l writes bytes [8l, 8l + 8) of every 512-byte
-    // step, so one mix64 yields its 8 filler bytes
-    const uint32_t lane = threadIdx.x & 63;
-    for (uint64_t j = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; j < P.n;
-         j += (uint64_t)gridDim.x * (blockDim.x / 64)) {
-        const uint64_t i = shuffle_index(j, P.n, seed);
-        uint64_t id;
-        uint32_t L, k;
-        skew_id_len(P, seed, i, id, L, k);
-        const uint64_t h = rng(seed, 5, id);
-        uint8_t* s = out + off[j];
-        for (uint32_t t0 = 8 * lane; t0 <= L; t0 += 512) {
-            const uint64_t fill = mix64(h + (t0 >> 3));
-            for (uint32_t b = 0; b < 8 && t0 + b <= L; ++b) {
-                const uint32_t t = t0 + b;
+// One wave per 64 consecutive strings (contiguous in the arena).  Lane t
+// derives string t's id, digit count k, length and filler hash once; the
+// strings' 8-character pieces (one filler word each) are then dealt to the
+// lanes in rounds of 64, so short strings keep the wave busy.  Ids are below
+// 2^32 < 255^5, so digits 5 and up are 0 (character 1); the first 8 digit
+// characters are kept as one word.
+__global__ void __launch_bounds__(256)
+k_skew_fill(SkewP P, uint64_t seed, const uint64_t* __restrict__ off, uint8_t* __restrict__ out) {
+    __shared__ uint64_t s_off[4][64], s_h[4][64], s_dw[4][64];
+    __shared__ uint32_t s_len[4][64], s_k[4][64], s_pre[4][65];
+    const uint32_t wv = (threadIdx.x >> 6) & 3, lane = threadIdx.x & 63;
+    const uint64_t ngroups = (P.n + 63) / 64;
+    for (uint64_t g = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; g < ngroups;
+         g += (uint64_t)gridDim.x * blockDim.x / 64) {
+        const uint64_t j = g * 64 + lane;
+        uint32_t pieces = 0;
+        if (j < P.n) {
+            const uint64_t i = shuffle_index(j, P.n, seed);
+            uint64_t id;
+            uint32_t L, k;
+            skew_id_len(P, seed, i, id, L, k);
+            uint64_t dw = 0, v = id;
+            for (uint32_t e = 0; e < 8; ++e) {
+                dw |= (uint64_t)(1 + v % 255) << (8 * e);
+                v /= 255;
+            }
+            s_off[wv][lane] = off[j];
+            s_h[wv][lane] = rng(seed, 5, id);
+            s_dw[wv][lane] = dw;
+            s_len[wv][lane] = L;
+            s_k[wv][lane] = k;
+            pieces = L / 8 + 1;  // positions [0, L] in pieces of 8
+        }
+        // inclusive scan of the piece counts over the wave (all lanes active)
+        uint32_t incl = pieces;
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        s_pre[wv][lane + 1] = incl;
+        if (lane == 0) s_pre[wv][0] = 0;
+        const uint32_t total = __shfl(incl, 63, 64);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (uint32_t x = lane; x < total; x += 64) {
+            // string t: the last one with s_pre[t] <= x (s_pre is non-decreasing)
+            uint32_t t = 0;
+            for (uint32_t step = 32; step; step >>= 1)
+                if (s_pre[wv][t + step] <= x) t += step;
+            const uint32_t L = s_len[wv][t], k = s_k[wv][t];
+            const uint32_t t0 = 8 * (x - s_pre[wv][t]);
+            const uint64_t fill = mix64(s_h[wv][t] + (t0 >> 3));
+            const uint64_t dw = s_dw[wv][t];
+            uint8_t* const sp = out + s_off[wv][t];
+            for (uint32_t b = 0; b < 8; ++b) {
+                const uint32_t tt = t0 + b;
+                if (tt > L) break;
                 uint8_t c;
-                if (t == L) c = 0;
-                else if (t >= k) c = (uint8_t)(1 + ((fill >> (8 * b)) & 0xFF) % 255);
+                if (tt == L) c = 0;
+                else if (tt >= k) c = (uint8_t)(1 + ((fill >> (8 * b)) & 0xFF) % 255);
                 else {
-                    const uint32_t e = k - 1 - t;
-                    uint64_t v = id;
-                    if (e >= 9) v = 0;  // 255^8 > 2^64
-                    else for (uint32_t q = 0; q < e && v; ++q) v /= 255;
-                    c = (uint8_t)(1 + v % 255);
+                    const uint32_t e = k - 1 - tt;
+                    c = e < 8 ? (uint8_t)(dw >> (8 * e)) : (uint8_t)1;
                 }
-                s[t] = c;
+                sp[tt] = c;
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();  // LDS read before the next group overwrites
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
 }
 
@@ -789,7 +829,7 @@ int dss_set_create_skewed(uint64_t n, uint32_t long_len, uint32_t short_len, dou
             exclusive_scan_u64(s->off.as<uint64_t>(), s->off.as<uint64_t>(), n, s->stream, s->pp, &s->nbytes);
             s->chars.reserve(s->nbytes + kArenaPad);
             DSS_HIP(hipMemsetAsync(s->chars.as<uint8_t>() + s->nbytes, 0, kArenaPad, s->stream));
-            hipLaunchKernelGGL(k_skew_fill, dim3(stride_grid(n, 4)), dim3(256), 0, s->stream, P, seed,
+            hipLaunchKernelGGL(k_skew_fill, dim3(stride_grid((n + 63) / 64, 4)), dim3(256), 0, s->stream, P, seed,
                                s->off.as<uint64_t>(), s->chars.as<uint8_t>());
             DSS_HIP(hipGetLastError());
             compute_pe_split(s);
@@ -1225,7 +1265,7 @@ int dss_pool_add_job_skewed(dss_pool P, uint64_t n, uint32_t long_len, uint32_t 
         const uint32_t id = pool_register(P, b0, nbytes, asz, s0, n);
         P->jobs[id].max_len = std::max(std::max(long_len, S.k_long), std::max(short_len, S.k_short));
         hipLaunchKernelGGL(k_add_base, dim3((n + 255) / 256), dim3(256), 0, P->stream, off, n, b0);
-        hipLaunchKernelGGL(k_skew_fill, dim3(stride_grid(n, 4)), dim3(256), 0, P->stream, S, seed, off,
+        hipLaunchKernelGGL(k_skew_fill, dim3(stride_grid((n + 63) / 64, 4)), dim3(256), 0, P->stream, S, seed, off,
                            P->chars.as<uint8_t>());
         DSS_HIP(hipGetLastError());
         DSS_HIP(hipStreamSynchronize(P->stream));

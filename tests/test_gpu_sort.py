@@ -92,6 +92,32 @@ def test_device_generators_match_oracle():
     assert chars.tobytes() == exp.tobytes()
 
 
+@pytest.mark.parametrize("n,L,dn,seed", [
+    (1, 1, 0.5, 1), (2, 3, 0.0, 7), (100, 5, 0.5, 3), (1000, 7, 1.0, 9), (3000, 1, 0.0, 4),
+    (20000, 13, 0.7, 8), (700, 4095, 0.01, 6), (9, 4096, 0.5, 10), (26 ** 3 + 5, 2, 0.0, 12)])
+def test_dton_generator_shapes(n, L, dn, seed):
+    # strides of 2..4097 bytes: strings shorter than a word, words across
+    # several strings, strings longer than a 4096-byte chunk, a ragged tail
+    dss = _dss()
+    with dss.DeviceStringSet.dton(n, L, dn, seed, 1) as ds:
+        chars, off, ln = ds.download()
+    assert chars.tobytes() == O.gen_dton(n, L, dn, seed).tobytes()
+
+
+@pytest.mark.parametrize("n,ll,sl,ldn,sdn,lp,dp", [
+    (1, 9, 1, 0.5, 0.0, 500, 0), (63, 16, 3, 0.3, 0.5, 200, 100), (64, 600, 7, 0.75, 0.25, 200, 100),
+    (65, 40, 8, 0.9, 0.1, 900, 300), (4097, 5000, 2, 0.5, 0.0, 20, 500), (30000, 120, 12, 0.6, 0.3, 250, 50)])
+def test_skewed_generator_shapes(n, ll, sl, ldn, sdn, lp, dp):
+    dss = _dss()
+    seed = 0x5EED00A0 + n
+    with dss.DeviceStringSet.skewed(n, seed, 1, long_len=ll, short_len=sl, long_dn=ldn, short_dn=sdn,
+                                    long_permille=lp, dup_permille=dp) as ds:
+        chars, off, ln = ds.download()
+    exp = O.gen_skewed(O.skew_params(n, long_len=ll, short_len=sl, long_dn=ldn, short_dn=sdn,
+                                     long_permille=lp, dup_permille=dp), seed)
+    assert chars.tobytes() == exp.tobytes()
+
+
 PARAM_SETS = [
     {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8, "skip_strings": 1},
     {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 2},
