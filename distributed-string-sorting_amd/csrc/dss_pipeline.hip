@@ -746,8 +746,8 @@ void exchange(dss_set_s* s, int mode) {
 // compared from that LCP on.  Character comparisons first use a cached 8-char
 // key at the receiver's common prefix.  The merged LCP array is exact; the
 // saved LCP of an output is the run LCP it was received with (oldLcps).
-constexpr uint32_t kMT = 256;               // threads per merge block
-constexpr uint32_t kMW = 4;                 // outputs per thread
+constexpr uint32_t kMT = 512;               // threads per merge block
+constexpr uint32_t kMW = 2;                 // outputs per thread
 constexpr uint32_t kTileOut = kMT * kMW;    // outputs per tile
 
 struct MergePair {
