@@ -659,7 +659,7 @@ __global__ void k_pd_flag(const uint32_t* __restrict__ maybe, uint64_t nm, const
 // many distinct hashes for the table hands its candidates to the exact
 // global table instead.  Every access is a streamed read or write except the
 // duplicate flags, which only duplicates write.
-constexpr uint32_t kHpT = 512, kHpI = 8, kHpTile = kHpT * kHpI;
+constexpr uint32_t kHpT = 512, kHpI = 4, kHpTile = kHpT * kHpI;
 constexpr uint32_t kHpMaxBits = 9;     // digit bits per pass
 constexpr uint32_t kHpCap = 2048;      // LDS table slots per bucket
 constexpr uint64_t kHpMin = 1u << 20;  // hashed candidates from which rounds partition
