@@ -153,6 +153,35 @@ struct KScope {
     }
 };
 
+// ------------------------------------------------------------- read-backs
+// Small device-to-host read-backs between launches (counters that size the
+// next launches).  A copy into pageable memory plus a stream synchronisation
+// leaves the device idle for tens of microseconds; instead a one-wave kernel
+// copies the words into pinned, coherent host memory and then publishes a
+// sequence number with a system-scope release store, and the host spins on
+// that number.  Every 1024 polls the host queries the stream, so a failed or
+// drained stream without the number raises an error instead of spinning.
+struct Mailbox {
+    static constexpr size_t kBytes = 4096;  // data bytes per read-back
+    uint64_t* flag = nullptr;               // pinned: [0] sequence, data from byte 64
+    uint64_t seq = 0;
+    Mailbox() = default;
+    Mailbox(const Mailbox&) = delete;
+    Mailbox& operator=(const Mailbox&) = delete;
+    ~Mailbox();
+    // Copies `bytes` (a multiple of 4) from device memory `dev` to `dst` once
+    // every earlier launch on `st` has finished; more than kBytes go through a
+    // copy and a stream synchronisation.
+    void fetch(hipStream_t st, void* dst, const void* dev, size_t bytes);
+    // Returns once every earlier launch on `st` has finished.
+    void wait(hipStream_t st) { fetch(st, nullptr, nullptr, 0); }
+};
+
+// Sets n bytes at device address p to v with one small kernel on st (for
+// counters and short arrays between launches: cheaper to issue than a
+// runtime memset).
+void fill_small(hipStream_t st, void* p, uint8_t v, size_t n);
+
 // ------------------------------------------------------------- sort scratch
 struct SortScratch {
     DevBuf keyA, keyB, pbA, pbB;
@@ -207,6 +236,7 @@ struct PipeState {
     DevBuf exact, exact_glcp, exact_pos, exact_cnt;
     // generic scratch
     DevBuf scan_tmp, scan_tot;
+    Mailbox mbox;  // counter read-backs on the owner's stream
 };
 
 // Device-wide exclusive scan of u64 values (out may alias in); returns the

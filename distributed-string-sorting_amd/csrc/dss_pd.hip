@@ -934,7 +934,7 @@ static uint64_t partition_duplicates(dss_set_s* s, uint64_t nh, uint32_t table_l
         KScope ks(s->timer, st, "pd_dedup", (double)nh * 28.0);
         uint32_t* bstart = P.pd_pbound.as<uint32_t>();
         uint32_t* bend = bstart + nbk;
-        DSS_HIP(hipMemsetAsync(bstart, 0, (uint64_t)nbk * 8, st));
+        fill_small(st, bstart, 0, (uint64_t)nbk * 8);
         hipLaunchKernelGGL(k_hp_bounds, dim3((nh + 255) / 256), dim3(256), 0, st, kB, nh, kb, bstart, bend);
         DSS_HIP(hipMemsetAsync(P.pd_dup.p, 0, s->n, st));
         hipLaunchKernelGGL(k_hp_dedup, dim3(std::min<uint32_t>(nbk, 65536)), dim3(256), 0, st, kB, iB, bstart, bend,
@@ -942,8 +942,7 @@ static uint64_t partition_duplicates(dss_set_s* s, uint64_t nh, uint32_t table_l
     }
     DSS_HIP(hipGetLastError());
     uint64_t nm = 0;
-    DSS_HIP(hipMemcpyAsync(&nm, ctr + 2, 8, hipMemcpyDeviceToHost, st));
-    DSS_HIP(hipStreamSynchronize(st));
+    s->pp.mbox.fetch(st, &nm, ctr + 2, 8);
     return nm;
 }
 
@@ -990,12 +989,11 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
         unsigned int minrun = 0;
         {
             KScope ks(s->timer, st, "pd_skip", (double)n * 4.0);
-            DSS_HIP(hipMemsetAsync(ctr, 0xFF, 4, st));
+            fill_small(st, ctr, 0xFF, 4);
             hipLaunchKernelGGL(k_pd_min_run, dim3(std::min<uint64_t>((n + 255) / 256, 1024)), blk, 0, st,
                                s->lcp.as<uint32_t>(), n, reinterpret_cast<unsigned int*>(ctr));
         }
-        DSS_HIP(hipMemcpyAsync(&minrun, ctr, 4, hipMemcpyDeviceToHost, st));
-        DSS_HIP(hipStreamSynchronize(st));
+        s->pp.mbox.fetch(st, &minrun, ctr, 4);
         while (depth <= minrun) {
             dlast = (uint32_t)depth;
             ++skipped;
@@ -1017,7 +1015,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     while (ncand > 0) {
         DSS_REQUIRE(depth < 0xFFFFFFFFull, DSS_ERANGE, "prefix doubling depth overflow");
         DSS_REQUIRE(2 * ncand < (1ull << 32), DSS_ERANGE, "too many candidates for the hash table");
-        DSS_HIP(hipMemsetAsync(ctr, 0, 32, st));
+        fill_small(st, ctr, 0, 32);
         {
             // every position: candidate flags of c and c-1, length, lcp, the
             // flag written; every candidate: rounds and dist
@@ -1028,8 +1026,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
                                P.pd_list.as<uint32_t>(), ctr);
         }
         uint64_t nh = 0;
-        DSS_HIP(hipMemcpyAsync(&nh, ctr, 8, hipMemcpyDeviceToHost, st));
-        DSS_HIP(hipStreamSynchronize(st));
+        s->pp.mbox.fetch(st, &nh, ctr, 8);
         // shared-prefix hashing: at least 2 full stripes, at least half of the
         // positions hashed, and at most one run head per 4 hashed candidates
         const uint32_t stripes = (uint32_t)(depth / 32);
@@ -1044,8 +1041,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
             hipLaunchKernelGGL(k_head_count, dim3(nbh), dim3(kHsT), 0, st, s->lcp.as<uint32_t>(), n, B,
                                P.pd_hmax.as<uint32_t>());
             hipLaunchKernelGGL(k_head_sums, dim3(1), dim3(1024), 0, st, P.pd_hmax.as<uint32_t>(), nbh, ctr + 3);
-            DSS_HIP(hipMemcpyAsync(&nheads, ctr + 3, 8, hipMemcpyDeviceToHost, st));
-            DSS_HIP(hipStreamSynchronize(st));
+            s->pp.mbox.fetch(st, &nheads, ctr + 3, 8);
             shared = nheads * 4 <= nh;
         }
         if (shared) {
@@ -1105,8 +1101,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
                                    P.pd_hash.as<uint64_t>(), bits, again, P.pd_dup.as<uint8_t>(),
                                    P.pd_maybe.as<uint32_t>(), ctr + 2);
             }
-            DSS_HIP(hipMemcpyAsync(&nm, ctr + 2, 8, hipMemcpyDeviceToHost, st));
-            DSS_HIP(hipStreamSynchronize(st));
+            s->pp.mbox.fetch(st, &nm, ctr + 2, 8);
         }
         if (nm) {
             uint64_t T = 1024;
@@ -1131,8 +1126,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
                                (uint32_t)depth, n, P.pd_dup.as<uint8_t>(), next, ctr + 1);
         }
         DSS_HIP(hipGetLastError());
-        DSS_HIP(hipMemcpyAsync(&ncand, ctr + 1, 8, hipMemcpyDeviceToHost, st));
-        DSS_HIP(hipStreamSynchronize(st));
+        s->pp.mbox.fetch(st, &ncand, ctr + 1, 8);
         std::swap(cand, next);
         ++round;
         depth = start_depth + step;

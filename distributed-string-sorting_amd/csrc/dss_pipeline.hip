@@ -89,21 +89,20 @@ __global__ void __launch_bounds__(kST) k_scan_apply(const uint64_t* __restrict__
     }
 }
 
+__global__ void k_store_u64(uint64_t* __restrict__ dst, uint64_t v) { *dst = v; }
+
 void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t st, PipeState& P,
                         uint64_t* total_host) {
     const uint64_t nb = std::max<uint64_t>((n + kSB - 1) / kSB, 1);
     P.scan_tmp.reserve((nb + 1) * 8);
     P.scan_tot.reserve(8);
     uint64_t* sums = P.scan_tmp.as<uint64_t>();
-    DSS_HIP(hipMemsetAsync(sums, 0, (nb + 1) * 8, st));
+    fill_small(st, sums, 0, (nb + 1) * 8);
     if (n) hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kST), 0, st, in, n, sums);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, nb, P.scan_tot.as<uint64_t>());
     if (n) hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kST), 0, st, in, out, n, sums);
     DSS_HIP(hipGetLastError());
-    if (total_host) {
-        DSS_HIP(hipMemcpyAsync(total_host, P.scan_tot.p, 8, hipMemcpyDeviceToHost, st));
-        DSS_HIP(hipStreamSynchronize(st));
-    }
+    if (total_host) P.mbox.fetch(st, total_host, P.scan_tot.p, 8);
 }
 
 // ================================================================== (a3)
@@ -126,8 +125,7 @@ uint64_t avg_lcp(dss_set_s* s) {
     hipLaunchKernelGGL(k_sum_u32, dim3(std::min<uint64_t>((s->n + 255) / 256, 2048)), dim3(256), 0, s->stream,
                        s->lcp.as<uint32_t>(), s->n, P.scan_tot.as<unsigned long long>());
     uint64_t tot = 0;
-    DSS_HIP(hipMemcpyAsync(&tot, P.scan_tot.p, 8, hipMemcpyDeviceToHost, s->stream));
-    DSS_HIP(hipStreamSynchronize(s->stream));
+    P.mbox.fetch(s->stream, &tot, P.scan_tot.p, 8);
     return tot / s->n;
 }
 
@@ -368,8 +366,7 @@ void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, boo
                            s->n, wprefix);
         uint64_t tot = 0;
         exclusive_scan_u64(wprefix, wprefix, s->n, st, P, &tot);
-        DSS_HIP(hipMemcpyAsync(wprefix + s->n, &tot, 8, hipMemcpyHostToDevice, st));
-        DSS_HIP(hipStreamSynchronize(st));
+        hipLaunchKernelGGL(k_store_u64, dim3(1), dim3(1), 0, st, wprefix + s->n, tot);
     }
     {
         KScope ks(s->timer, st, "partition", 0.0);
@@ -379,8 +376,7 @@ void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, boo
                                by_chars, factor, max_len, nr_max, slots, counts);
             hipLaunchKernelGGL(k_sample_compact, dim3(1), dim3(256), 0, st, slots, counts, p, nr_max, compact, ns_d);
             uint32_t ns = 0;
-            DSS_HIP(hipMemcpyAsync(&ns, ns_d, 4, hipMemcpyDeviceToHost, st));
-            DSS_HIP(hipStreamSynchronize(st));
+            P.mbox.fetch(st, &ns, ns_d, 4);
             if (ns)
                 hipLaunchKernelGGL(k_sample_rank, dim3(ns), dim3(256), 0, st, compact, ns, indexed,
                                    s->arena(), P.sample_sorted.as<uint32_t>());
@@ -408,8 +404,7 @@ void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, boo
     }
     DSS_HIP(hipGetLastError());
     P.interval.assign((uint64_t)p * p, 0);
-    DSS_HIP(hipMemcpyAsync(P.interval.data(), P.interval_d.p, (uint64_t)p * p * 8, hipMemcpyDeviceToHost, st));
-    DSS_HIP(hipStreamSynchronize(st));
+    P.mbox.fetch(st, P.interval.data(), P.interval_d.p, (uint64_t)p * p * 8);
     P.partitioned = true;
     P.exchanged = P.merged = P.extended = false;
 }
@@ -689,7 +684,7 @@ void exchange(dss_set_s* s, int mode) {
     uint32_t* cpd = P.m_cp.as<uint32_t>();
     uint64_t* rb_d = reinterpret_cast<uint64_t*>(P.m_cp.as<uint8_t>() + cp_bytes);
     DSS_HIP(hipMemcpyAsync(rb_d, P.recv_begin.data(), (p + 1) * 8, hipMemcpyHostToDevice, st));
-    DSS_HIP(hipMemsetAsync(cpd, 0xFF, (p + 1) * 4, st));
+    fill_small(st, cpd, 0xFF, (p + 1) * 4);
     if (n) {
         KScope ks(s->timer, st, "exchange_index", (double)p * p * 64.0);
         hipLaunchKernelGGL(k_recv_cp, dim3(((uint64_t)p * p * 64 + 255) / 256), dim3(256), 0, st, ls + (uint64_t)p * p,
@@ -700,7 +695,7 @@ void exchange(dss_set_s* s, int mode) {
     exclusive_scan_u64(P.recv_size.as<uint64_t>(), P.recv_byte_off.as<uint64_t>(), n, st, P, &total);
     P.recv_nbytes = total;
     P.recv_bytes.reserve(total + kArenaPad);
-    DSS_HIP(hipMemsetAsync(P.recv_bytes.as<uint8_t>() + total, 0, kArenaPad, st));
+    fill_small(st, P.recv_bytes.as<uint8_t>() + total, 0, kArenaPad);
     if (n) {
         KScope ks(s->timer, st, "exchange_copy", 2.0 * total + n * 40.0);
         // a lane per string for copies of at most 32 characters on average,
@@ -728,8 +723,7 @@ void exchange(dss_set_s* s, int mode) {
         DSS_HIP(hipMemcpyAsync(rbd, P.recv_begin.data(), (p + 1) * 8, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_byte_begin, dim3((p + 1 + 255) / 256), dim3(256), 0, st, rbd, P.recv_byte_off.as<uint64_t>(),
                            n, total, p, bbd);
-        DSS_HIP(hipMemcpyAsync(P.byte_begin.data(), bbd, (p + 1) * 8, hipMemcpyDeviceToHost, st));
-        DSS_HIP(hipStreamSynchronize(st));
+        P.mbox.fetch(st, P.byte_begin.data(), bbd, (p + 1) * 8);
     }
     P.mode = mode;
     P.exchanged = true;

@@ -72,6 +72,66 @@ KernelTimer::~KernelTimer() {
     for (auto e : pool) (void)hipEventDestroy(e);
 }
 
+// ------------------------------------------------------------- read-backs
+__global__ void __launch_bounds__(64)
+k_mailbox(const uint32_t* __restrict__ src, uint32_t nwords, uint32_t* __restrict__ dst, uint64_t* flag,
+          uint64_t seq) {
+    for (uint32_t i = threadIdx.x; i < nwords; i += 64) dst[i] = src[i];
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ void k_fill_small(uint8_t* __restrict__ p, uint64_t n, uint8_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void fill_small(hipStream_t st, void* p, uint8_t v, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_small, dim3((uint32_t)std::min<uint64_t>((n + 255) / 256, 1024)), dim3(256), 0, st,
+                       static_cast<uint8_t*>(p), (uint64_t)n, v);
+    DSS_HIP(hipGetLastError());
+}
+
+Mailbox::~Mailbox() {
+    if (flag) (void)hipHostFree(flag);
+}
+
+void Mailbox::fetch(hipStream_t st, void* dst, const void* dev, size_t bytes) {
+    DSS_REQUIRE(bytes % 4 == 0, DSS_EINVAL, "read-back size");
+    if (bytes > kBytes) {
+        DSS_HIP(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    if (!flag) {
+        void* h = nullptr;
+        DSS_HIP(hipHostMalloc(&h, kBytes + 64, hipHostMallocCoherent | hipHostMallocMapped));
+        flag = static_cast<uint64_t*>(h);
+        flag[0] = 0;
+    }
+    const uint64_t s = ++seq;
+    uint32_t* data = reinterpret_cast<uint32_t*>(flag + 8);
+    hipLaunchKernelGGL(k_mailbox, dim3(1), dim3(64), 0, st, static_cast<const uint32_t*>(dev),
+                       (uint32_t)(bytes / 4), data, flag, s);
+    DSS_HIP(hipGetLastError());
+    for (uint64_t it = 1;; ++it) {
+        if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == s) break;
+        if ((it & 1023) == 0) {
+            const hipError_t e = hipStreamQuery(st);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == s) break;
+                throw Error{DSS_EHIP, "read-back: the stream finished without publishing"};
+            }
+            if (e != hipErrorNotReady)
+                throw Error{DSS_EHIP, std::string("read-back: ") + hipGetErrorString(e)};
+        }
+        __builtin_ia32_pause();
+    }
+    if (bytes) std::memcpy(dst, data, bytes);
+}
+
 // ------------------------------------------------------------------ set
 dss_set_s* new_set(uint32_t num_pes) {
     DSS_REQUIRE(num_pes >= 1 && num_pes <= (1u << 16), DSS_EINVAL, "num_pes must be in [1, 65536]");

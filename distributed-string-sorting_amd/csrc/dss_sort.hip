@@ -1036,7 +1036,7 @@ void sort_lcp(dss_set_s* s) {
     const uint64_t seg_cap = caps.seg_cap, tile_cap = caps.tile_cap, chunk_cap = caps.chunk_cap;
     const SmallLists L = caps.L;
     uint32_t* ctr = X.counters.as<uint32_t>();
-    DSS_HIP(hipMemsetAsync(ctr, 0, 64 * 4, st));
+    fill_small(st, ctr, 0, 64 * 4);
 
     {
         KScope ks(s->timer, st, "sort_init", n * 8.0);
@@ -1060,8 +1060,7 @@ void sort_lcp(dss_set_s* s) {
     uint32_t h_ctr[64];
     uint32_t par = 0;  // counter set of the current pass
     hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, X.segA.as<Seg>(), ctr + kCtrPass);
-    DSS_HIP(hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, st));
-    DSS_HIP(hipStreamSynchronize(st));
+    s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
     const bool trace = getenv("DSS_SORT_TRACE") != nullptr;
     const uint8_t* chars = s->arena();
     const uint64_t* off = s->off.as<uint64_t>();
@@ -1078,7 +1077,7 @@ void sort_lcp(dss_set_s* s) {
         DSS_REQUIRE(nlarge <= seg_cap, DSS_ERANGE, "segment list overflow");
         DSS_REQUIRE(nt <= tile_cap, DSS_ERANGE, "tile list overflow");
         DSS_REQUIRE(nchunks <= chunk_cap, DSS_ERANGE, "chunk list overflow");
-        DSS_HIP(hipMemsetAsync(pn, 0, 16, st));
+        fill_small(st, pn, 0, 16);
         hipLaunchKernelGGL(k_fill_tiles, dim3((nt + 255) / 256), dim3(256), 0, st, cur, nlarge, nt,
                            X.tiles.as<Tile>());
         Tile* tl = X.tiles.as<Tile>();
@@ -1129,8 +1128,7 @@ void sort_lcp(dss_set_s* s) {
         }
         hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, nxt, pn);
         DSS_HIP(hipGetLastError());
-        DSS_HIP(hipMemcpyAsync(h_ctr, ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, st));
-        DSS_HIP(hipStreamSynchronize(st));
+        s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
         if (s->timer.enabled) {
             // prefix skip: per string of a segment that gathers keys, its
             // characters up to the common prefix + 8 (the key), id, offset
