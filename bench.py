@@ -15,6 +15,8 @@ max(lcp_i, lcp_{i+1}) + 1 (the reference's computeD), in GB/s.
 The JSON line also carries
   roofline:     the dominant kernel's algorithmic bytes per launch over its
                 HIP-event-timed average duration, against MI355X's 8 TB/s;
+  passes:       the same over all timed kernels of a step together, and the
+                whole step's (D + 16 n) bytes over its time;
   cpu_baseline: the CPU oracle (a single-thread C restatement of the same
                 pipeline) timed on a bounded sample of the same workload.
 Only one GPU is used; --gpus N > 1 exits with a message.
@@ -460,6 +462,17 @@ def main():
             if roof["traffic"] is not None:
                 roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
 
+    # all timed kernels together (the sort's passes, prefix doubling,
+    # partition, exchange and merge) and the whole step, against 8 TB/s
+    alg_sum = sum(v["alg_bytes"] for v in stats.values())
+    ms_sum = sum(v["ms"] for v in stats.values())
+    passes = {
+        "alg_bytes_per_step": int(alg_sum / args.steps),
+        "kernel_ms_per_step": round(ms_sum / args.steps, 3),
+        "frac": round(alg_sum / (ms_sum * 1e-3) / (HBM_PEAK_GBS * 1e9), 4) if ms_sum > 0 else None,
+        "end_to_end_bytes": int(D + 16 * args.n),
+        "end_to_end_frac": round((D + 16 * args.n) / step_s / (HBM_PEAK_GBS * 1e9), 4),
+    }
     ds.close()
     batched = None
     if args.serve_jobs > 0:
@@ -503,6 +516,7 @@ def main():
             "strings_per_s": round(args.n / step_s, 1),
         },
         "roofline": roof,
+        "passes": passes,
         "cpu_baseline": cpu,
         "batched": batched,
         "extras": extras,

@@ -141,9 +141,10 @@ struct Sample {
 // One thread per PE: the sampling policies of samplingStrategies.hpp.
 //  NumStrings / IndexedNumStrings (:20-90, :107-187): sample i at
 //    (size_t)(i * (double)n / (double)(nr + 1));
-//  NumChars / IndexedNumChars (:189-389): walk lengths (or distinguishing
-//    prefixes) until splitter_dist = num / (nr + 1) characters were seen;
-//    done by binary searches on the prefix sums w_prefix.
+//  NumChars / IndexedNumChars (:189-389): walk lengths (IndexedNumChars in
+//    the prefix-doubling form: distinguishing prefixes) until splitter_dist
+//    = num / (nr + 1) characters were seen; done by binary searches on the
+//    prefix sums w_prefix.
 // nr = min(factor * (p - 1), n) (getNumberSplitter, :14-18).  A sample is
 // truncated to max_len (merge sort) or to dist[idx] (prefix doubling).
 __global__ void k_sample(const uint64_t* __restrict__ pe_begin, uint32_t p, const uint32_t* __restrict__ perm,
@@ -362,8 +363,10 @@ void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, boo
     if (by_chars && s->n) {
         P.wprefix.reserve((s->n + 1) * 8);
         wprefix = P.wprefix.as<uint64_t>();
-        hipLaunchKernelGGL(k_weights, dim3((s->n + 255) / 256), dim3(256), 0, st, s->s_len.as<uint32_t>(), dist,
-                           s->n, wprefix);
+        // the indexed policy walks distinguishing prefixes, the plain one
+        // lengths (samplingStrategies.hpp:243-273, :345-386)
+        hipLaunchKernelGGL(k_weights, dim3((s->n + 255) / 256), dim3(256), 0, st, s->s_len.as<uint32_t>(),
+                           indexed ? dist : nullptr, s->n, wprefix);
         uint64_t tot = 0;
         exclusive_scan_u64(wprefix, wprefix, s->n, st, P, &tot);
         hipLaunchKernelGGL(k_store_u64, dim3(1), dim3(1), 0, st, wprefix + s->n, tot);

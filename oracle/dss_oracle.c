@@ -485,9 +485,9 @@ ORC_API uint64_t orc_avg_lcp(const uint32_t* lcp, uint64_t n) {
 /* position distinguishing prefix (prefix doubling) or NULL.                   */
 /* Output: interval[r*p + q] = strings PE r sends to PE q.                     */
 /* Pinned by: an independent Python restatement (sorted() sample order, the  */
-/* reference's halving binarySearch) on inputs with and without duplicates,  */
-/* truncated and prefix-doubling splitters (NumStrings, IndexedNumStrings);  */
-/* NumChars samplers by bucket-order invariants only (tests/test_oracle.py). */
+/* reference's halving binarySearch, the NumChars walk) on inputs with and    */
+/* without duplicates, truncated and prefix-doubling splitters, for all four  */
+/* samplers (tests/test_oracle.py).                                           */
 /* ------------------------------------------------------------------------- */
 
 typedef struct {
@@ -536,16 +536,19 @@ ORC_API int orc_partition(const uint8_t* chars, const uint64_t* off,
             const double sd = (double)m / (double)(nr + 1);
             for (uint64_t i = 1; i <= nr; ++i) idx_list[cnt++] = (uint64_t)((double)i * sd);
         } else {
-            /* NumChars policies (:199-236, :299-337); the dist variants count
-             * distinguishing prefixes instead of lengths (:239-296). */
+            /* NumChars policies (:199-236, :299-337).  With distinguishing
+             * prefixes the indexed policy counts those instead of lengths
+             * (:239-296); the plain one still counts lengths and only
+             * truncates the splitter to dist (:339-387). */
+            const int wdist = dist && sampler == 3;
             uint64_t num = 0;
-            for (uint64_t i = 0; i < m; ++i) num += dist ? dist[b + i] : len[perm[b + i]];
+            for (uint64_t i = 0; i < m; ++i) num += wdist ? dist[b + i] : len[perm[b + i]];
             const uint64_t sd = num / (nr + 1);
             uint64_t si = 0;
             for (uint64_t i = 1; i <= nr && si < m; ++i) {
                 uint64_t seen = 0;
                 while (seen < sd && si < m) {
-                    seen += dist ? dist[b + si] : len[perm[b + si]];
+                    seen += wdist ? dist[b + si] : len[perm[b + si]];
                     ++si;
                 }
                 idx_list[cnt++] = si ? si - 1 : 0; /* the reference reads index -1 here

@@ -361,7 +361,25 @@ def _ref_binary_search(strs, key, idx_key=None, local_offset=0):
     return left
 
 
-def _brute_partition(pes, indexed, max_len=None, dist=None, factor=2):
+def _numchars_indices(weights, nr):
+    """The NumChars walk (samplingStrategies.hpp:199-236, :243-273, :307-337,
+    :345-386): the count restarts for every splitter, the splitter is the
+    last string consumed, the walk stops at the end of the PE."""
+    n = len(weights)
+    sd = sum(weights) // (nr + 1)
+    out, si = [], 0
+    for _ in range(nr):
+        if si >= n:
+            break
+        seen = 0
+        while seen < sd and si < n:
+            seen += weights[si]
+            si += 1
+        out.append(si - 1 if si else 0)  # the reference reads index -1 if nothing was consumed
+    return out
+
+
+def _brute_partition(pes, indexed, max_len=None, dist=None, factor=2, chars=False):
     """pes: per PE the sorted list of bytes strings."""
     p = len(pes)
     offsets = [0]
@@ -371,9 +389,15 @@ def _brute_partition(pes, indexed, max_len=None, dist=None, factor=2):
     for r, ss in enumerate(pes):
         n = len(ss)
         nr = min(factor * (p - 1), n)
-        sd = n / (nr + 1)
-        for i in range(1, nr + 1):
-            j = int(i * sd)
+        if chars:
+            # only the indexed policy walks the distinguishing prefixes; the
+            # plain one walks the lengths and truncates to dist
+            w = [int(dist[r][j]) for j in range(n)] if (dist is not None and indexed) else [len(x) for x in ss]
+            idxs = _numchars_indices(w, nr)
+        else:
+            sd = n / (nr + 1)
+            idxs = [int(i * sd) for i in range(1, nr + 1)]
+        for j in idxs:
             lim = dist[r][j] if dist is not None else max_len
             samples.append((ss[j][:int(lim)], offsets[r] + j))
     samples.sort(key=(lambda t: (t[0], t[1])) if indexed else (lambda t: t[0]))
@@ -407,21 +431,41 @@ PART_INPUTS = {
 
 @pytest.mark.parametrize("name", sorted(PART_INPUTS))
 @pytest.mark.parametrize("p", [2, 3, 5])
-@pytest.mark.parametrize("sampler", ["NumStrings", "IndexedNumStrings"])
+@pytest.mark.parametrize("sampler", ["NumStrings", "IndexedNumStrings", "NumChars", "IndexedNumChars"])
 def test_partition_matches_brute_force(name, p, sampler):
     ss = O.StringSet(PART_INPUTS[name](), p)
     perm, lcp, pes = _sorted_pes(ss)
     indexed = sampler.startswith("Indexed")
+    chars = sampler.endswith("Chars")
     for max_len in (1000, 3):
         got = O.partition(ss, perm, sampler, max_len=max_len)
-        exp = _brute_partition(pes, indexed, max_len=max_len)
+        exp = _brute_partition(pes, indexed, max_len=max_len, chars=chars)
         assert (got == exp).all(), (name, p, sampler, max_len, got, exp)
     # prefix-doubling form: splitters truncated to the per-position lengths
     dist = np.array([1 + (i * 13) % 9 for i in range(ss.n)], dtype=np.uint32)
     dl = [list(dist[int(ss.pe_begin[r]):int(ss.pe_begin[r + 1])]) for r in range(p)]
     got = O.partition(ss, perm, sampler, dist=dist)
-    exp = _brute_partition(pes, indexed, dist=dl)
+    exp = _brute_partition(pes, indexed, dist=dl, chars=chars)
     assert (got == exp).all(), (name, p, sampler, "dist", got, exp)
+
+
+def test_numchars_walk_counts_lengths_unless_indexed_with_dist():
+    """The plain NumChars policy walks string lengths even in the
+    prefix-doubling form (dist only truncates the splitter), the indexed one
+    walks the distinguishing prefixes: with lengths and dist chosen so that
+    the two walks pick different strings, the partitions differ."""
+    strs = sorted([bytes([65 + i % 26]) * (1 + (i * 7) % 40) + bytes([66 + i % 5]) for i in range(400)])
+    ss = O.StringSet(arena(strs), 2)
+    perm, lcp, pes = _sorted_pes(ss)
+    dist = np.array([1 + (i * 29) % 3 for i in range(ss.n)], dtype=np.uint32)
+    dl = [list(dist[int(ss.pe_begin[r]):int(ss.pe_begin[r + 1])]) for r in range(2)]
+    by_len = _numchars_indices([len(x) for x in pes[0]], 2)
+    by_dist = _numchars_indices([int(v) for v in dl[0]], 2)
+    assert by_len != by_dist
+    for sampler, indexed in (("NumChars", False), ("IndexedNumChars", True)):
+        got = O.partition(ss, perm, sampler, dist=dist)
+        exp = _brute_partition(pes, indexed, dist=dl, chars=True)
+        assert (got == exp).all(), (sampler, got, exp)
 
 
 @pytest.mark.parametrize("sampler", ["NumChars", "IndexedNumChars"])
