@@ -504,6 +504,8 @@ __device__ inline uint32_t receiver_of(const uint64_t* __restrict__ recv_begin, 
     return lo - 1;
 }
 
+constexpr uint32_t kXcWords = 512;  // LDS words per wave for the short-copy path
+
 // 2^lg lanes per received string copy its bytes into the receive arena.
 // With lg = 0 (short copies, such as the few characters after the LCP that
 // an LCP-compressed distinguishing prefix sends) every lane copies one
@@ -519,63 +521,112 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
                             const uint32_t* __restrict__ r_stage, const uint64_t* __restrict__ stage,
                             uint32_t stage_w) {
     if (lg == 0) {
-        for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-             k += (uint64_t)gridDim.x * blockDim.x) {
-            const uint32_t cpk = cp[receiver_of(recv_begin, p, k)];
-            const uint32_t si = r_stage ? r_stage[k] : kNone;
-            if (si != kNone) {
-                // the characters were kept by the prefix doubling's last hash
-                // of this string: row = [B, dist), B = 32 (dist/32 - 1), in
-                // stage_w words (the characters beyond dist are not used)
-                const uint32_t vlen = r_vlen[k], from = r_from[k];
-                const uint32_t B = 32 * (vlen / 32 - 1);
-                if (from >= B && cpk >= B) {
-                    const uint64_t* rw = stage + stage_w * (uint64_t)si;
-                    uint64_t row[9];
+        // A wave's 64 consecutive receive positions own one contiguous byte
+        // range of the receive arena.  When it fits, the lanes build it in
+        // LDS and the wave stores it as aligned words (byte stores only for
+        // the two edge words, which neighbouring waves share); otherwise the
+        // lanes store their bytes directly.
+        __shared__ uint64_t xw[4][kXcWords];
+        uint64_t* const sw = xw[(threadIdx.x >> 6) & 3];
+        uint8_t* const sb = reinterpret_cast<uint8_t*>(sw);
+        const uint32_t lane = threadIdx.x & 63;
+        const uint64_t nwv = (n + 63) / 64;
+        for (uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; wv < nwv;
+             wv += ((uint64_t)gridDim.x * blockDim.x) >> 6) {  // wave-uniform
+            const uint64_t k = wv * 64 + lane;
+            const bool valid = k < n;
+            const uint32_t last = (uint32_t)min<uint64_t>(63, n - 1 - wv * 64);
+            uint32_t si = kNone, vlen = 0, from = 0, cpk = 0;
+            uint64_t start = 0;
+            if (valid) {
+                cpk = cp[receiver_of(recv_begin, p, k)];
+                si = r_stage ? r_stage[k] : kNone;
+                vlen = r_vlen[k];
+                from = r_from[k];
+                start = byte_off[k];
+            }
+            const uint32_t m = vlen - from;  // characters; a NUL follows
+            const uint64_t A = __shfl(start, 0);
+            const uint64_t E = __shfl(start + m + 1, last);
+            const uint64_t A0 = A & ~7ull;
+            const bool fits = E - A0 <= 8ull * kXcWords;  // wave-uniform
+            uint8_t* const gdst = out + start;
+            uint8_t* const ldst = sb + (start - A0);
+            auto put = [&](uint32_t t, uint8_t c) {
+                if (fits) ldst[t] = c;
+                else gdst[t] = c;
+            };
+            if (valid) {
+                bool done = false;
+                if (si != kNone) {
+                    // the characters were kept by the prefix doubling's last
+                    // hash of this string: row = [B, dist), B = 32 (dist/32 -
+                    // 1), in stage_w words (the characters beyond dist are
+                    // not used)
+                    const uint32_t B = 32 * (vlen / 32 - 1);
+                    if (from >= B && cpk >= B) {
+                        const uint64_t* rw = stage + stage_w * (uint64_t)si;
+                        uint64_t row[9];
 #pragma unroll
-                    for (uint32_t i = 0; i < 9; ++i) row[i] = i < stage_w ? rw[i] : 0;
-                    // 8 characters at row byte x, little endian
-                    auto rword = [&](uint32_t x) -> uint64_t {
-                        const uint32_t w = x >> 3, sh = (x & 7) * 8;
-                        return sh ? (row[w] >> sh) | (row[w + 1] << (64 - sh)) : row[w];
-                    };
-                    uint64_t key = 0;
-                    if (cpk < vlen) {
-                        key = __builtin_bswap64(rword(cpk - B));
-                        const uint32_t rem = vlen - cpk;
-                        if (rem < 8) key &= ~0ull << (8 * (8 - rem));
-                    }
-                    keyc[k] = key;
-                    const uint32_t m = vlen - from;
-                    uint8_t* dst = out + byte_off[k];
-                    for (uint32_t t = 0; t < m; t += 8) {
-                        const uint64_t y = rword(from - B + t);
+                        for (uint32_t i = 0; i < 9; ++i) row[i] = i < stage_w ? rw[i] : 0;
+                        // 8 characters at row byte x, little endian
+                        auto rword = [&](uint32_t x) -> uint64_t {
+                            const uint32_t w = x >> 3, sh = (x & 7) * 8;
+                            return sh ? (row[w] >> sh) | (row[w + 1] << (64 - sh)) : row[w];
+                        };
+                        uint64_t key = 0;
+                        if (cpk < vlen) {
+                            key = __builtin_bswap64(rword(cpk - B));
+                            const uint32_t rem = vlen - cpk;
+                            if (rem < 8) key &= ~0ull << (8 * (8 - rem));
+                        }
+                        keyc[k] = key;
+                        for (uint32_t t = 0; t < m; t += 8) {
+                            const uint64_t y = rword(from - B + t);
 #pragma unroll
-                        for (uint32_t b = 0; b < 8; ++b)
-                            if (t + b < m) dst[t + b] = (uint8_t)(y >> (8 * b));
+                            for (uint32_t b = 0; b < 8; ++b)
+                                if (t + b < m) put(t + b, (uint8_t)(y >> (8 * b)));
+                        }
+                        done = true;
                     }
-                    dst[m] = 0;
-                    continue;
                 }
+                if (!done) {
+                    keyc[k] = load_key8(chars, r_voff[k], vlen, cpk);
+                    const uint64_t a = r_voff[k] + from;
+                    for (uint32_t t = 0; t < m; t += 32) {
+                        const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + ((a + t) & ~7ull));
+                        const unsigned sh = (unsigned)((a + t) & 7) * 8;
+                        uint64_t x[5], y[4];
+#pragma unroll
+                        for (int i = 0; i < 5; ++i) x[i] = w[i];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) y[i] = sh ? (x[i] >> sh) | (x[i + 1] << (64 - sh)) : x[i];
+#pragma unroll
+                        for (uint32_t b = 0; b < 32; ++b)
+                            if (t + b < m) put(t + b, (uint8_t)(y[b >> 3] >> (8 * (b & 7))));
+                    }
+                }
+                put(m, 0);
             }
-            keyc[k] = load_key8(chars, r_voff[k], r_vlen[k], cpk);
-            const uint32_t from = r_from[k];
-            const uint64_t a = r_voff[k] + from;
-            const uint32_t m = r_vlen[k] - from;
-            uint8_t* dst = out + byte_off[k];
-            for (uint32_t t = 0; t < m; t += 32) {
-                const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + ((a + t) & ~7ull));
-                const unsigned sh = (unsigned)((a + t) & 7) * 8;
-                uint64_t x[5], y[4];
-#pragma unroll
-                for (int i = 0; i < 5; ++i) x[i] = w[i];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) y[i] = sh ? (x[i] >> sh) | (x[i + 1] << (64 - sh)) : x[i];
-#pragma unroll
-                for (uint32_t b = 0; b < 32; ++b)
-                    if (t + b < m) dst[t + b] = (uint8_t)(y[b >> 3] >> (8 * (b & 7)));
+            if (fits) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                // word w of the buffer is the arena word at A0 + 8 w
+                const uint32_t nwords = (uint32_t)((E - A0 + 7) / 8);
+                for (uint32_t w = lane; w < nwords; w += 64) {
+                    const uint64_t g = A0 + 8ull * w;
+                    if (g >= A && g + 8 <= E) {
+                        reinterpret_cast<uint64_t*>(out)[g >> 3] = sw[w];
+                    } else {
+                        for (uint32_t b = 0; b < 8; ++b)
+                            if (g + b >= A && g + b < E) out[g + b] = sb[8 * w + b];
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();  // read before the next range is built
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             }
-            dst[m] = 0;
         }
         return;
     }
