@@ -425,6 +425,8 @@ __device__ inline uint32_t bucket_of(const uint64_t* __restrict__ send_start, ui
     return lo - 1;
 }
 
+constexpr uint32_t kXiP = 32;  // PEs up to which k_exch_index stages its tables
+
 // One thread per source position: destination, size and receive metadata.
 // Encoders (src/mpi/byte_encoder.hpp): mode 0 whole string incl. NUL
 // (:81-107), mode 1 chars [lcp, len] incl. NUL (:109-137), mode 2 chars
@@ -438,18 +440,33 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
                              uint32_t* __restrict__ r_pe, uint32_t* __restrict__ r_pos, uint64_t* __restrict__ r_voff,
                              uint32_t* __restrict__ r_vlen, uint32_t* __restrict__ r_from,
                              const uint32_t* __restrict__ sdepth, uint32_t* __restrict__ r_stage) {
+    // up to kXiP PEs the PE begins and the layout tables are staged in LDS,
+    // so the two searches per position read LDS instead of global memory
+    __shared__ uint64_t s_pe[kXiP + 1], s_ss[kXiP * kXiP], s_ro[kXiP * kXiP];
+    const bool staged_tables = p <= kXiP;  // block-uniform
+    if (staged_tables) {
+        for (uint32_t t = threadIdx.x; t <= p; t += blockDim.x) s_pe[t] = pe_begin[t];
+        for (uint32_t t = threadIdx.x; t < p * p; t += blockDim.x) {
+            s_ss[t] = send_start[t];
+            s_ro[t] = run_off[t];
+        }
+        __syncthreads();
+    }
+    const uint64_t* const PB = staged_tables ? s_pe : pe_begin;
+    const uint64_t* const SS = staged_tables ? s_ss : send_start;
+    const uint64_t* const RO = staged_tables ? s_ro : run_off;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t lo = 0, hi = p;  // PE of position i: last r with pe_begin[r] <= i (non-empty)
     while (lo < hi) {
         const uint32_t mid = (lo + hi) / 2;
-        if (pe_begin[mid] <= i) lo = mid + 1; else hi = mid;
+        if (PB[mid] <= i) lo = mid + 1; else hi = mid;
     }
     const uint32_t r = lo - 1;
-    const uint64_t j = i - pe_begin[r];
-    const uint32_t q = bucket_of(send_start, p, r, j);
-    const uint64_t st = send_start[(uint64_t)r * p + q];
-    const uint64_t dst = run_off[(uint64_t)q * p + r] + (j - st);
+    const uint64_t j = i - PB[r];
+    const uint32_t q = bucket_of(SS, p, r, j);
+    const uint64_t st = SS[(uint64_t)r * p + q];
+    const uint64_t dst = RO[(uint64_t)q * p + r] + (j - st);
     const uint32_t l = (j == st) ? 0u : lcp[i];
     const uint32_t L = s_len[i];
     uint64_t sz;

@@ -59,7 +59,7 @@ def canon(keys, eqmask):
 
 
 @pytest.mark.parametrize("name", sorted(INPUTS))
-@pytest.mark.parametrize("p", [3, 8])
+@pytest.mark.parametrize("p", [3, 8, 40])  # 40: above the exchange's staged-table limit of 32 PEs
 def test_merge_sort_stages(name, p):
     dss = _dss()
     chars = INPUTS[name]
@@ -127,7 +127,7 @@ def test_merge_sort_stages(name, p):
 
 
 @pytest.mark.parametrize("name", sorted(INPUTS))
-@pytest.mark.parametrize("p", [2, 5])
+@pytest.mark.parametrize("p", [2, 5, 37])
 def test_prefix_doubling_stages(name, p):
     dss = _dss()
     chars = INPUTS[name]
