@@ -105,28 +105,12 @@ __device__ inline uint32_t key_strlen(uint64_t k) {
 
 // Compares two string views (offset, length) from character `from` on, the
 // way scmp does on NUL-terminated strings (src/strings/stringtools.hpp:108-
-// 115); returns <0, 0, >0 and the LCP (total, from 0) in *lcp_out.
-__device__ inline int view_cmp(const uint8_t* __restrict__ chars, uint64_t ao, uint32_t al,
-                               uint64_t bo, uint32_t bl, uint32_t from, uint32_t* lcp_out) {
-    for (uint32_t pos = from;; pos += 8) {
-        const uint64_t a = load_key8(chars, ao, al, pos);
-        const uint64_t b = load_key8(chars, bo, bl, pos);
-        if (a != b) {
-            if (lcp_out) *lcp_out = pos + key_lcp(a, b);
-            return a < b ? -1 : 1;
-        }
-        if ((a & 0xFF) == 0) {
-            if (lcp_out) *lcp_out = pos + key_strlen(a);
-            return 0;
-        }
-    }
-}
-
-// view_cmp from character 0 with 32 characters per step (five aligned loads
-// per string and step): fewer dependent steps for long common prefixes.
+// 115); returns <0, 0, >0 and the LCP (total, from 0) in *lcp_out.  32
+// characters per step (five aligned loads per string and step), so long
+// common prefixes take few dependent steps.
 __device__ inline int view_cmp32(const uint8_t* __restrict__ chars, uint64_t ao, uint32_t al, uint64_t bo,
-                                 uint32_t bl, uint32_t* lcp_out) {
-    for (uint32_t d = 0;; d += 32) {
+                                 uint32_t bl, uint32_t* lcp_out, uint32_t from = 0) {
+    for (uint32_t d = from;; d += 32) {
         uint64_t x[4], y[4];
         load_keys4(chars, ao, al, d, x);
         load_keys4(chars, bo, bl, d, y);

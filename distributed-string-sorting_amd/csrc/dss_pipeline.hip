@@ -260,7 +260,7 @@ __global__ void k_bucket_search(const uint64_t* __restrict__ pe_begin, uint32_t 
             while (left != right) {
                 const uint64_t d = (right - left) / 2;
                 const uint32_t id = perm[b + left + d];
-                int c = view_cmp(chars, off[id], len[id], sp.off, sp.tlen, 0, nullptr);
+                int c = view_cmp32(chars, off[id], len[id], sp.off, sp.tlen, nullptr);
                 if (c == 0 && indexed) {
                     const uint64_t gi = b + left + d;
                     c = gi < sp.gidx ? -1 : (gi > sp.gidx ? 1 : 0);
@@ -846,7 +846,7 @@ __device__ inline int elem_cmp(const ViewCtx& V, uint32_t a, uint64_t ka, uint32
         }
         from = V.cp + 8;
     }
-    return view_cmp(V.chars, V.voff[a], V.vlen[a], V.voff[b], V.vlen[b], from, &lcp);
+    return view_cmp32(V.chars, V.voff[a], V.vlen[a], V.voff[b], V.vlen[b], &lcp, from);
 }
 
 __device__ inline uint32_t pair_of_tile(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t t) {
