@@ -258,8 +258,10 @@ def extras_leg(args, D_c2):
     """Measurements of the paths beside the headline sort, on the same GPU:
     the wire codecs on C2-shaped streams (device-resident, HIP-event timed),
     the distributed merge sort on the 1M x 500 B D/N input over 4 PEs (the
-    reference's CPU-runnable case) with each string encoder, the text loader
-    and the exact distinguishing-prefix validator."""
+    reference's CPU-runnable case) with each string encoder, the text loader,
+    the exact distinguishing-prefix validator, and both drivers on the
+    skewed 20M-string input and on the 200M capped suffixes of a 200 MB
+    Markov text (16 PEs each)."""
     import numpy as np
     import dss_amd
     from dss_amd import WireCodec
@@ -367,6 +369,34 @@ def extras_leg(args, D_c2):
                                                   "'\\n' -> NUL, index and PE split"}
     finally:
         os.unlink(path)
+
+    # -- the other full-size inputs: skewed D/N (20M strings, 16 PEs) and
+    #    capped suffixes of a 200 MB Markov text (200M views, 16 PEs); one
+    #    untimed sort for D and the check, then 3 timed sorts per driver
+    def timed(ds, run, reps=3):
+        run()
+        ds.sync()
+        chk = ds.check()
+        assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        ds.sync()
+        dt = (time.perf_counter() - t0) / reps
+        return {"ms": round(dt * 1e3, 3), "dp_GBps": round(chk["D"] / dt / 1e9, 2),
+                "strings_per_s": round(chk["num_strings"] / dt, 1), "D": chk["D"]}
+
+    with dss_amd.DeviceStringSet.skewed(20_000_000, 0x5EED0003, 16) as ds:
+        out["c3_skewed_20m_p16"] = {
+            "prefix_doubling_sort": timed(ds, lambda: ds.prefix_doubling_sort()),
+            "merge_sort": timed(ds, lambda: ds.merge_sort()),
+        }
+    with dss_amd.DeviceStringSet.markov_suffixes(200_000_000, 0x5EED0004, 16, repeats=5000, rep_min=1000,
+                                                 rep_max=10000, cap=500) as ds:
+        out["c4_suffixes_200m_p16"] = {
+            "prefix_doubling_sort": timed(ds, lambda: ds.prefix_doubling_sort()),
+            "merge_sort_lcp_compact": timed(ds, lambda: ds.merge_sort(compressed="lcp_compact")),
+        }
     return out
 
 
