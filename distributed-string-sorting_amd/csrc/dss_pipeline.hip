@@ -647,18 +647,32 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
         }
         return;
     }
+    // Longer copies: the string's lanes own the aligned 8-byte words of its
+    // output range [o, o + m + 1) (characters, then a NUL); a word wholly
+    // inside the range is built from two aligned source loads and stored at
+    // once, the two edge words (shared with the neighbouring strings) byte
+    // by byte.
     const uint32_t lanes = 1u << lg;
     const uint32_t l = threadIdx.x & (lanes - 1);
     for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lg; k < n;
          k += ((uint64_t)gridDim.x * blockDim.x) >> lg) {
-        const uint8_t* src = chars + r_voff[k];
         const uint32_t from = r_from[k];
         const uint32_t vlen = r_vlen[k];
-        uint8_t* dst = out + byte_off[k];
+        const uint64_t sa = r_voff[k] + from;  // first source character
         if (l == 0) keyc[k] = load_key8(chars, r_voff[k], vlen, cp[receiver_of(recv_begin, p, k)]);
         // modes 0/1: chars [from, vlen] incl. the NUL; mode 2: [from, vlen) + NUL
         const uint32_t m = vlen - from;
-        for (uint32_t t = l; t <= m; t += lanes) dst[t] = (t == m) ? 0 : src[from + t];
+        const uint64_t o = byte_off[k], e = o + m + 1;
+        for (uint64_t w = (o >> 3) + l; w < (e + 7) >> 3; w += lanes) {
+            const uint64_t g = w << 3;
+            if (g >= o && g + 8 <= e) {
+                uint64_t v = load_word_le(chars, sa + (g - o));
+                if (e == g + 8) v &= ~(0xFFull << 56);  // the NUL is the word's last byte
+                *reinterpret_cast<uint64_t*>(out + g) = v;
+            } else {
+                for (uint64_t x = max(g, o); x < min(g + 8, e); ++x) out[x] = x - o < m ? chars[sa + (x - o)] : 0;
+            }
+        }
     }
 }
 
