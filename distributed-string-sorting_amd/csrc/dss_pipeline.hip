@@ -1201,6 +1201,23 @@ __global__ void k_out_size(const uint32_t* __restrict__ ord, const uint32_t* __r
     if (i < n) out_size[i] = saved[i] + rsize[ord[i]];  // prefix + stored chars incl. NUL
 }
 
+// Copies n bytes from base[sa ..] to out[oa ..] with L lanes (this is lane
+// l): the lanes own the aligned 8-byte words of [oa, oa + n); a word wholly
+// inside is built from two aligned source loads and stored at once, the edge
+// words byte by byte (their other bytes belong to other copies).
+__device__ inline void copy_span(uint8_t* __restrict__ out, uint64_t oa, const uint8_t* __restrict__ base, uint64_t sa,
+                                 uint64_t n, uint32_t l, uint32_t L) {
+    const uint64_t e = oa + n;
+    for (uint64_t w = (oa >> 3) + l; w < (e + 7) >> 3; w += L) {
+        const uint64_t g = w << 3;
+        if (g >= oa && g + 8 <= e) {
+            *reinterpret_cast<uint64_t*>(out + g) = load_word_le(base, sa + (g - oa));
+        } else {
+            for (uint64_t x = max(g, oa); x < min(g + 8, e); ++x) out[x] = base[sa + (x - oa)];
+        }
+    }
+}
+
 // 16 lanes per output string.
 __global__ void k_extend(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ saved,
                          const uint64_t* __restrict__ psv, const uint64_t* __restrict__ rbyte,
@@ -1209,19 +1226,14 @@ __global__ void k_extend(const uint32_t* __restrict__ ord, const uint32_t* __res
     const uint32_t l = threadIdx.x & 15;
     for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < n;
          i += ((uint64_t)gridDim.x * blockDim.x) >> 4) {
-        uint8_t* dst = out + out_off[i];
+        const uint64_t o0 = out_off[i];
         uint32_t cur = saved[i];
         // own stored characters (incl. NUL) after the prefix
-        {
-            const uint8_t* src = rbytes + rbyte[ord[i]];
-            const uint64_t m = rsize[ord[i]];
-            for (uint64_t t = l; t < m; t += 16) dst[cur + t] = src[t];
-        }
+        copy_span(out, o0 + cur, rbytes, rbyte[ord[i]], rsize[ord[i]], l, 16);
         uint64_t k = psv[i];
         while (cur > 0 && k != ~0ull) {
             const uint32_t sk = saved[k];
-            const uint8_t* src = rbytes + rbyte[ord[k]];
-            for (uint32_t t = sk + l; t < cur; t += 16) dst[t] = src[t - sk];
+            copy_span(out, o0 + sk, rbytes, rbyte[ord[k]], cur - sk, l, 16);
             cur = sk;
             k = psv[k];
         }
