@@ -873,7 +873,8 @@ __device__ inline uint32_t pair_of_tile(const MergePair* __restrict__ pairs, uin
 }
 
 // Merge-path split (number of A elements among the first d outputs, A wins
-// ties) at the start of every tile, from global memory.
+// ties) at the start of every tile, from global memory.  ord == nullptr: the
+// first round, whose order is the receive order.
 __global__ void k_merge_split(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t ntiles,
                               const uint32_t* __restrict__ ord, const uint8_t* __restrict__ chars,
                               const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
@@ -888,7 +889,8 @@ __global__ void k_merge_split(const MergePair* __restrict__ pairs, uint32_t npai
     uint64_t lo = d > Pr.nb ? d - Pr.nb : 0, hi = d < Pr.na ? d : Pr.na;
     while (lo < hi) {
         const uint64_t mid = (lo + hi) / 2;
-        const uint32_t a = ord[A + mid], b = ord[B + d - mid - 1];
+        const uint64_t ia = A + mid, ib = B + d - mid - 1;
+        const uint32_t a = ord ? ord[ia] : (uint32_t)ia, b = ord ? ord[ib] : (uint32_t)ib;
         uint32_t x;
         if (elem_cmp(V, a, keyc[a], b, keyc[b], V.cp, x) <= 0) lo = mid + 1;
         else hi = mid;
@@ -901,12 +903,15 @@ __global__ void k_merge_split(const MergePair* __restrict__ pairs, uint32_t npai
 // start by a merge-path search in LDS and merges kMW outputs with the LCP
 // rules of the loser tree (bingmann-lcp_losertree.hpp:280-333): the head
 // with the larger LCP to the last output is smaller, equal LCPs compare
-// characters from there on.
+// characters from there on.  ord == nullptr: the first round (receive
+// order); saved != nullptr: the last round, which also writes every output's
+// saved LCP (the LCP it was received with, rlcp).
 __global__ void __launch_bounds__(kMT)
 k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_t* __restrict__ split,
              const uint32_t* __restrict__ ord, const uint32_t* __restrict__ lcpa, uint32_t* __restrict__ ord_out,
              uint32_t* __restrict__ lcp_out, const uint8_t* __restrict__ chars, const uint64_t* __restrict__ voff,
-             const uint32_t* __restrict__ vlen, const uint64_t* __restrict__ keyc, const uint32_t* __restrict__ rcp) {
+             const uint32_t* __restrict__ vlen, const uint64_t* __restrict__ keyc, const uint32_t* __restrict__ rcp,
+             const uint32_t* __restrict__ rlcp, uint32_t* __restrict__ saved) {
     __shared__ uint32_t sAo[kTileOut + 1], sAl[kTileOut + 1], sBo[kTileOut + 1], sBl[kTileOut + 1];
     __shared__ uint64_t sAk[kTileOut + 1], sBk[kTileOut + 1];
     const uint64_t t = blockIdx.x;
@@ -925,7 +930,7 @@ k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_
     for (uint32_t k = threadIdx.x; k <= la_n; k += kMT) {
         const int64_t g = (int64_t)i0 - 1 + k;
         if (g >= 0 && (uint64_t)g < na) {
-            const uint32_t o = ord[A + g];
+            const uint32_t o = ord ? ord[A + g] : (uint32_t)(A + g);
             sAo[k] = o;
             sAl[k] = lcpa[A + g];
             sAk[k] = keyc[o];
@@ -934,7 +939,7 @@ k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_
     for (uint32_t k = threadIdx.x; k <= lb_n; k += kMT) {
         const int64_t g = (int64_t)j0 - 1 + k;
         if (g >= 0 && (uint64_t)g < nb) {
-            const uint32_t o = ord[B + g];
+            const uint32_t o = ord ? ord[B + g] : (uint32_t)(B + g);
             sBo[k] = o;
             sBl[k] = lcpa[B + g];
             sBk[k] = keyc[o];
@@ -1002,6 +1007,7 @@ k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_
         const uint64_t go = d0 + o;
         ord_out[A + go] = outo;
         lcp_out[A + go] = (go == 0) ? 0u : lout;
+        if (saved) saved[A + go] = rlcp[outo];  // the last round: the LCP it was received with
     }
 }
 
@@ -1039,10 +1045,6 @@ void merge(dss_set_s* s) {
     uint32_t* lc = P.m_lcp.as<uint32_t>();
     uint32_t* ord2 = P.m_order2.as<uint32_t>();
     uint32_t* lc2 = P.m_lcp2.as<uint32_t>();
-    if (n) {
-        KScope ks(s->timer, st, "merge_init", n * 16.0);
-        hipLaunchKernelGGL(k_iota_copy, dim3((n + 255) / 256), dim3(256), 0, st, ord, lc, P.recv_lcp.as<uint32_t>(), n);
-    }
     // runs per receiver (host), then pairwise rounds; receivers that already
     // hold a single run are carried along as one-run pairs
     std::vector<std::vector<std::pair<uint64_t, uint64_t>>> runs(p);
@@ -1094,22 +1096,39 @@ void merge(dss_set_s* s) {
         MergePair* dpairs = P.m_work.as<MergePair>();
         uint64_t* splits = reinterpret_cast<uint64_t*>(P.m_work.as<uint8_t>() + pair_bytes);
         DSS_HIP(hipMemcpyAsync(dpairs, all.data(), all.size() * sizeof(MergePair), hipMemcpyHostToDevice, st));
+        // round 0 reads the receive order and LCPs directly; round r writes
+        // the buffer pair round r - 1 did not
+        const uint32_t* in_ord = nullptr;
+        const uint32_t* in_lc = P.recv_lcp.as<uint32_t>();
         for (size_t r = 0; r + 1 < rbeg.size(); ++r) {
             const uint32_t np = (uint32_t)(rbeg[r + 1] - rbeg[r]);
             const uint64_t tiles = rtiles[r];
             MergePair* rp = dpairs + rbeg[r];
             uint64_t* split = splits + tbeg[r];
+            const bool last = r + 2 == rbeg.size();
             KScope ks(s->timer, st, "merge_pairs", n * 24.0);
-            hipLaunchKernelGGL(k_merge_split, dim3((tiles + 255) / 256), dim3(256), 0, st, rp, np, tiles, ord,
+            hipLaunchKernelGGL(k_merge_split, dim3((tiles + 255) / 256), dim3(256), 0, st, rp, np, tiles, in_ord,
                                s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                                P.m_keyc.as<uint64_t>(), cp, split);
-            hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, rp, np, split, ord, lc, ord2, lc2,
+            hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, rp, np, split, in_ord, in_lc, ord, lc,
                                s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
-                               P.m_keyc.as<uint64_t>(), cp);
+                               P.m_keyc.as<uint64_t>(), cp, P.recv_lcp.as<uint32_t>(),
+                               last ? P.m_saved.as<uint32_t>() : nullptr);
+            in_ord = ord;
+            in_lc = lc;
             std::swap(ord, ord2);
             std::swap(lc, lc2);
         }
         DSS_HIP(hipGetLastError());
+        // results in the last round's output pair
+        ord = const_cast<uint32_t*>(in_ord);
+        lc = const_cast<uint32_t*>(in_lc);
+    } else if (n) {
+        // no receiver holds two runs: the receive order is the merged order
+        KScope ks(s->timer, st, "merge_init", n * 16.0);
+        hipLaunchKernelGGL(k_iota_copy, dim3((n + 255) / 256), dim3(256), 0, st, ord, lc, P.recv_lcp.as<uint32_t>(), n);
+        hipLaunchKernelGGL(k_gather_saved, dim3((n + 255) / 256), dim3(256), 0, st, ord, P.recv_lcp.as<uint32_t>(),
+                           n, P.m_saved.as<uint32_t>());
     }
     // results live in (ord, lc); make them the canonical buffers
     if (ord != P.m_order.as<uint32_t>()) {
@@ -1118,9 +1137,6 @@ void merge(dss_set_s* s) {
         std::swap(P.m_lcp.p, P.m_lcp2.p);
         std::swap(P.m_lcp.bytes, P.m_lcp2.bytes);
     }
-    if (n)
-        hipLaunchKernelGGL(k_gather_saved, dim3((n + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
-                           P.recv_lcp.as<uint32_t>(), n, P.m_saved.as<uint32_t>());
     DSS_HIP(hipGetLastError());
     P.merged = true;
     P.extended = false;
