@@ -227,7 +227,7 @@ struct PipeState {
     DevBuf layout;                                 // send_start / run offsets (u64)
     // merge (per receive-side output position)
     bool merged = false;
-    DevBuf m_order, m_lcp, m_saved, m_order2, m_lcp2, m_keyc, m_cp, m_work;
+    DevBuf m_order, m_lcp, m_saved, m_order2, m_lcp2, m_keyc, m_keyc2, m_cp, m_work;
     // prefix decompression
     bool extended = false;
     uint64_t out_nbytes = 0;

@@ -528,13 +528,14 @@ constexpr uint32_t kXcWords = 512;  // LDS words per wave for the short-copy pat
 // an LCP-compressed distinguishing prefix sends) every lane copies one
 // string, 32 characters at a time from five aligned 8-byte loads, so a wave
 // has 64 strings in flight.  The lane (lane 0 of the string's lanes) also
-// stores the merge's cached key: the string's 8 characters at its
-// receiver's common prefix cp, from the lines the copy reads.
+// stores the merge's cached keys: the string's 16 characters at its
+// receiver's common prefix cp (two words), from the lines the copy reads.
 __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* __restrict__ r_voff,
                             const uint32_t* __restrict__ r_vlen, const uint32_t* __restrict__ r_from,
                             const uint64_t* __restrict__ byte_off, int mode, uint64_t n, uint32_t lg,
                             uint8_t* __restrict__ out, const uint64_t* __restrict__ recv_begin, uint32_t p,
                             const uint32_t* __restrict__ cp, uint64_t* __restrict__ keyc,
+                            uint64_t* __restrict__ keyc2,
                             const uint32_t* __restrict__ r_stage, const uint64_t* __restrict__ stage,
                             uint32_t stage_w) {
     if (lg == 0) {
@@ -598,6 +599,13 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
                             if (rem < 8) key &= ~0ull << (8 * (8 - rem));
                         }
                         keyc[k] = key;
+                        uint64_t key2 = 0;  // the next 8 characters
+                        if (cpk + 8 < vlen) {
+                            key2 = __builtin_bswap64(rword(cpk + 8 - B));
+                            const uint32_t rem = vlen - cpk - 8;
+                            if (rem < 8) key2 &= ~0ull << (8 * (8 - rem));
+                        }
+                        keyc2[k] = key2;
                         for (uint32_t t = 0; t < m; t += 8) {
                             const uint64_t y = rword(from - B + t);
 #pragma unroll
@@ -609,6 +617,7 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
                 }
                 if (!done) {
                     keyc[k] = load_key8(chars, r_voff[k], vlen, cpk);
+                    keyc2[k] = load_key8(chars, r_voff[k], vlen, cpk + 8);
                     const uint64_t a = r_voff[k] + from;
                     for (uint32_t t = 0; t < m; t += 32) {
                         const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + ((a + t) & ~7ull));
@@ -659,7 +668,11 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
         const uint32_t from = r_from[k];
         const uint32_t vlen = r_vlen[k];
         const uint64_t sa = r_voff[k] + from;  // first source character
-        if (l == 0) keyc[k] = load_key8(chars, r_voff[k], vlen, cp[receiver_of(recv_begin, p, k)]);
+        if (l == 0) {
+            const uint32_t cpk = cp[receiver_of(recv_begin, p, k)];
+            keyc[k] = load_key8(chars, r_voff[k], vlen, cpk);
+            keyc2[k] = load_key8(chars, r_voff[k], vlen, cpk + 8);
+        }
         // modes 0/1: chars [from, vlen] incl. the NUL; mode 2: [from, vlen) + NUL
         const uint32_t m = vlen - from;
         const uint64_t o = byte_off[k], e = o + m + 1;
@@ -764,6 +777,7 @@ void exchange(dss_set_s* s, int mode) {
     }
     // the merge's per-receiver common prefix (from the views, before the copy)
     P.m_keyc.reserve(nn * 8);
+    P.m_keyc2.reserve(nn * 8);
     const uint64_t cp_bytes = ((uint64_t)(p + 1) * 4 + 7) / 8 * 8;
     P.m_cp.reserve(cp_bytes + (p + 1) * 8);
     uint32_t* cpd = P.m_cp.as<uint32_t>();
@@ -790,7 +804,7 @@ void exchange(dss_set_s* s, int mode) {
         hipLaunchKernelGGL(k_exch_copy, dim3(stride_grid(threads, 256)), dim3(256), 0, st, s->arena(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                            P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), mode, n, lg,
-                           P.recv_bytes.as<uint8_t>(), rb_d, p, cpd, P.m_keyc.as<uint64_t>(),
+                           P.recv_bytes.as<uint8_t>(), rb_d, p, cpd, P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(),
                            lg == 0 ? r_stage : nullptr, staged ? P.pd_stage.as<uint64_t>() : nullptr, P.stage_w);
     }
     if (n && mode == 3) {
@@ -841,12 +855,15 @@ struct ViewCtx {
     const uint8_t* chars;
     const uint64_t* voff;
     const uint32_t* vlen;
+    const uint64_t* key2;  // characters [cp + 8, cp + 16) per receive index
     uint32_t cp;
 };
 
 // Compares received strings a and b (receive indices, cached keys ka, kb at
 // the receiver's common prefix cp) known to agree on the first `from` >= cp
-// characters; returns the sign and the LCP.
+// characters; returns the sign and the LCP.  Characters up to cp + 16 come
+// from the cached keys (the second word from global memory, only on a tie of
+// the first), the rest from the strings.
 __device__ inline int elem_cmp(const ViewCtx& V, uint32_t a, uint64_t ka, uint32_t b, uint64_t kb, uint32_t from,
                                uint32_t& lcp) {
     if (from < V.cp + 8) {
@@ -859,6 +876,18 @@ __device__ inline int elem_cmp(const ViewCtx& V, uint32_t a, uint64_t ka, uint32
             return 0;
         }
         from = V.cp + 8;
+    }
+    if (from < V.cp + 16) {
+        const uint64_t xa = V.key2[a], xb = V.key2[b];
+        if (xa != xb) {
+            lcp = V.cp + 8 + key_lcp(xa, xb);
+            return xa < xb ? -1 : 1;
+        }
+        if ((xa & 0xFF) == 0) {
+            lcp = V.cp + 8 + key_strlen(xa);
+            return 0;
+        }
+        from = V.cp + 16;
     }
     return view_cmp32(V.chars, V.voff[a], V.vlen[a], V.voff[b], V.vlen[b], &lcp, from);
 }
@@ -878,12 +907,12 @@ __device__ inline uint32_t pair_of_tile(const MergePair* __restrict__ pairs, uin
 __global__ void k_merge_split(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t ntiles,
                               const uint32_t* __restrict__ ord, const uint8_t* __restrict__ chars,
                               const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
-                              const uint64_t* __restrict__ keyc, const uint32_t* __restrict__ rcp,
-                              uint64_t* __restrict__ split) {
+                              const uint64_t* __restrict__ keyc, const uint64_t* __restrict__ keyc2,
+                              const uint32_t* __restrict__ rcp, uint64_t* __restrict__ split) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     const MergePair Pr = pairs[pair_of_tile(pairs, npairs, t)];
-    const ViewCtx V{chars, voff, vlen, rcp[Pr.recv]};
+    const ViewCtx V{chars, voff, vlen, keyc2, rcp[Pr.recv]};
     const uint64_t A = Pr.out, B = Pr.out + Pr.na;
     const uint64_t d = (t - Pr.tile0) * kTileOut;
     uint64_t lo = d > Pr.nb ? d - Pr.nb : 0, hi = d < Pr.na ? d : Pr.na;
@@ -910,14 +939,15 @@ __global__ void __launch_bounds__(kMT)
 k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_t* __restrict__ split,
              const uint32_t* __restrict__ ord, const uint32_t* __restrict__ lcpa, uint32_t* __restrict__ ord_out,
              uint32_t* __restrict__ lcp_out, const uint8_t* __restrict__ chars, const uint64_t* __restrict__ voff,
-             const uint32_t* __restrict__ vlen, const uint64_t* __restrict__ keyc, const uint32_t* __restrict__ rcp,
-             const uint32_t* __restrict__ rlcp, uint32_t* __restrict__ saved) {
+             const uint32_t* __restrict__ vlen, const uint64_t* __restrict__ keyc,
+             const uint64_t* __restrict__ keyc2, const uint32_t* __restrict__ rcp, const uint32_t* __restrict__ rlcp,
+             uint32_t* __restrict__ saved) {
     __shared__ uint32_t sAo[kTileOut + 1], sAl[kTileOut + 1], sBo[kTileOut + 1], sBl[kTileOut + 1];
     __shared__ uint64_t sAk[kTileOut + 1], sBk[kTileOut + 1];
     const uint64_t t = blockIdx.x;
     const uint32_t pi = pair_of_tile(pairs, npairs, t);
     const MergePair Pr = pairs[pi];
-    const ViewCtx V{chars, voff, vlen, rcp[Pr.recv]};
+    const ViewCtx V{chars, voff, vlen, keyc2, rcp[Pr.recv]};
     const uint64_t A = Pr.out, B = Pr.out + Pr.na;
     const uint64_t na = Pr.na, nb = Pr.nb, tot = na + nb;
     const uint64_t d0 = (t - Pr.tile0) * kTileOut;
@@ -1109,10 +1139,10 @@ void merge(dss_set_s* s) {
             KScope ks(s->timer, st, "merge_pairs", n * 24.0);
             hipLaunchKernelGGL(k_merge_split, dim3((tiles + 255) / 256), dim3(256), 0, st, rp, np, tiles, in_ord,
                                s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
-                               P.m_keyc.as<uint64_t>(), cp, split);
+                               P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp, split);
             hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, rp, np, split, in_ord, in_lc, ord, lc,
                                s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
-                               P.m_keyc.as<uint64_t>(), cp, P.recv_lcp.as<uint32_t>(),
+                               P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp, P.recv_lcp.as<uint32_t>(),
                                last ? P.m_saved.as<uint32_t>() : nullptr);
             in_ord = ord;
             in_lc = lc;
