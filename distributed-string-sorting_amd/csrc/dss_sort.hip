@@ -560,49 +560,81 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restric
 }
 
 // ------------------------------------------------------------ scatter
+// Stable scatter of one tile by its digit.  The tile is first put in digit
+// order in LDS (wave ballots rank equal digits, in index order) and then
+// written run by run, so consecutive lanes store consecutive addresses of a
+// digit's run instead of a few records of every run per store.
 __global__ void __launch_bounds__(kThreads)
 k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint32_t* __restrict__ hist,
           const uint64_t* __restrict__ key_in, const uint32_t* __restrict__ pb_in,
           uint64_t* __restrict__ key_out, uint32_t* __restrict__ pb_out, uint32_t* __restrict__ perm,
           uint32_t* __restrict__ lcp) {
-    __shared__ uint32_t run_base[256];
+    __shared__ uint64_t skey[kTile];
+    __shared__ uint32_t sid[kTile];
+    __shared__ uint32_t gbase[256];   // the digit's output offset for this tile
+    __shared__ uint32_t lstart[256];  // the digit's first slot in the reordered tile
+    __shared__ uint32_t run[256];     // next free slot of the digit
     __shared__ uint32_t wcnt[kThreads / 64][256];
+    __shared__ uint32_t scan[kThreads];
     const Tile T = tiles[blockIdx.x];
     const Seg S = segs[T.seg];
     uint32_t kd, dep;
     seg_depths(S, kd, dep);
     const uint32_t sh = 56 - 8 * (dep - kd);
     const uint32_t tid = threadIdx.x, w = tid >> 6;
-    run_base[tid] = hist[(uint64_t)blockIdx.x * 256 + tid];
+    gbase[tid] = hist[(uint64_t)blockIdx.x * 256 + tid];
+    run[tid] = 0;
     for (uint32_t k = 0; k < kThreads / 64; ++k) wcnt[k][tid] = 0;
     __syncthreads();
-    const uint64_t lt = lanemask_lt();
+    // the tile's records and their digits
+    uint64_t kv[kItems];
+    uint32_t idv[kItems], dg[kItems];
+#pragma unroll
     for (uint32_t r = 0; r < kItems; ++r) {
         const uint32_t e = T.begin + r * kThreads + tid;
-        const bool valid = e < T.end;
-        uint64_t k = 0;
-        uint32_t id = 0, dg = 0;
-        if (valid) {
-            k = key_in[e];
-            id = pb_in[e];
-            dg = (uint32_t)(k >> sh) & 0xFF;
+        kv[r] = 0;
+        idv[r] = 0;
+        dg[r] = 0;
+        if (e < T.end) {
+            kv[r] = key_in[e];
+            idv[r] = pb_in[e];
+            dg[r] = (uint32_t)(kv[r] >> sh) & 0xFF;
+            atomicAdd(&run[dg[r]], 1u);
         }
+    }
+    __syncthreads();
+    // exclusive scan of the digit counts (one digit per thread)
+    const uint32_t mine = run[tid];
+    scan[tid] = mine;
+    __syncthreads();
+    for (uint32_t o = 1; o < kThreads; o <<= 1) {
+        const uint32_t v = tid >= o ? scan[tid - o] : 0;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    lstart[tid] = scan[tid] - mine;
+    run[tid] = scan[tid] - mine;
+    __syncthreads();
+    // stable placement in LDS, in index order (r, wave, lane)
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (uint32_t r = 0; r < kItems; ++r) {
+        const bool valid = T.begin + r * kThreads + tid < T.end;
         uint64_t m = __ballot(valid);
         for (uint32_t b = 0; b < 8; ++b) {
-            const bool bit = (dg >> b) & 1;
+            const bool bit = (dg[r] >> b) & 1;
             const uint64_t bb = __ballot(valid && bit);
             m &= bit ? bb : ~bb;
         }
         const uint32_t rank = (uint32_t)__popcll(m & lt);
-        if (valid && rank == 0) wcnt[w][dg] = (uint32_t)__popcll(m);
+        if (valid && rank == 0) wcnt[w][dg[r]] = (uint32_t)__popcll(m);
         __syncthreads();
         if (valid) {
-            uint32_t q = run_base[dg] + rank;
-            for (uint32_t ww = 0; ww < w; ++ww) q += wcnt[ww][dg];
-            key_out[q] = k;
-            pb_out[q] = id;
-            perm[q] = id;
-            if (dg == 0 && q != S.begin) lcp[q] = dep;  // equal strings of length dep
+            uint32_t q = run[dg[r]] + rank;
+            for (uint32_t ww = 0; ww < w; ++ww) q += wcnt[ww][dg[r]];
+            skey[q] = kv[r];
+            sid[q] = idv[r];
         }
         __syncthreads();
         uint32_t add = 0;
@@ -610,8 +642,20 @@ k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const ui
             add += wcnt[ww][tid];
             wcnt[ww][tid] = 0;
         }
-        run_base[tid] += add;
+        run[tid] += add;
         __syncthreads();
+    }
+    // runs out: slot k of digit d goes to gbase[d] + (k - lstart[d])
+    const uint32_t cnt = T.end - T.begin;
+    for (uint32_t k = tid; k < cnt; k += kThreads) {
+        const uint64_t x = skey[k];
+        const uint32_t d = (uint32_t)(x >> sh) & 0xFF;
+        const uint32_t q = gbase[d] + (k - lstart[d]);
+        const uint32_t id = sid[k];
+        key_out[q] = x;
+        pb_out[q] = id;
+        perm[q] = id;
+        if (d == 0 && q != S.begin) lcp[q] = dep;  // equal strings of length dep
     }
 }
 
