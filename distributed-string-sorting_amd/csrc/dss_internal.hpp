@@ -191,6 +191,7 @@ struct SortScratch {
     DevBuf csum;            // per tile chunk x 256 digit sums / offsets
     DevBuf small;           // small segment lists (all classes)
     DevBuf counters;        // device counters
+    DevBuf offlen;          // packed (offset, length) per string for the base cases
     std::vector<uint64_t> small_off, small_cap;  // per class region in `small`
     uint64_t seg_cap = 0, tile_cap = 0;
 };

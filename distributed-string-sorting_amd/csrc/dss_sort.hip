@@ -877,10 +877,24 @@ __device__ inline bool key_ended(uint64_t k, uint32_t w) { return ((k >> (64 - 8
 // advances the depths.  Group extents come from ballots of the leader lanes;
 // every loop runs while a ballot over the wave is non-zero, so all lanes
 // take part in every shuffle.
+// Offset and length of every string in one 16-byte record, so that the base
+// case's random lookup of a string reads one line instead of two.
+struct alignas(16) OffLen {
+    uint64_t off;
+    uint64_t len;
+};
+
+__global__ void k_pack_offlen(const uint64_t* __restrict__ off, const uint32_t* __restrict__ len, uint64_t n,
+                              OffLen* __restrict__ out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = OffLen{off[i], len[i]};
+}
+
 __global__ void __launch_bounds__(kThreads)
 k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2,
                   uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
                   const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                  const OffLen* __restrict__ offlen,
                   const uint64_t* __restrict__ keyA, const uint64_t* __restrict__ keyB,
                   uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len) {
     const uint32_t lane = lane_id();
@@ -903,8 +917,14 @@ k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls
             valid = true;
             my_id = perm[sg.begin + j];
             if (cache) K0 = ((cache & 8) ? keyB : keyA)[sg.begin + j] << (8 * (cache & 7));
-            my_off = off[my_id];
-            my_len = len[my_id];
+            if (offlen) {
+                const OffLen ol = offlen[my_id];
+                my_off = ol.off;
+                my_len = (uint32_t)ol.len;
+            } else {
+                my_off = off[my_id];
+                my_len = len[my_id];
+            }
         }
     }
     // per element (moves with the sort): G, K, src (original lane)
@@ -1212,6 +1232,20 @@ void sort_lcp(dss_set_s* s) {
             if (cls[c]) fprintf(stderr, " c%u=%u segs/%u strings", c, cls[c], cls[kClasses + c]);
         fprintf(stderr, "\n");
     }
+    // when most strings go through the one-wave base cases, their random
+    // offset/length lookups read packed records
+    const OffLen* offlen = nullptr;
+    {
+        uint64_t wave_strings = 0;
+        for (uint32_t c = 1; c < kClasses && c <= T.wave_class_max; ++c) wave_strings += cls[kClasses + c];
+        if (wave_strings >= n / 4 && wave_strings >= (1u << 16)) {
+            X.offlen.reserve(n * sizeof(OffLen));
+            KScope ks(s->timer, st, "offlen_pack", (double)n * 28.0);
+            hipLaunchKernelGGL(k_pack_offlen, dim3(stride_grid(n, 256)), dim3(256), 0, st, off, len, n,
+                               X.offlen.as<OffLen>());
+            offlen = X.offlen.as<OffLen>();
+        }
+    }
     for (uint32_t c = 1; c < kClasses; ++c) {
         const uint32_t cnt = cls[c];
         if (!cnt) continue;
@@ -1224,7 +1258,7 @@ void sort_lcp(dss_set_s* s) {
         if (c <= T.wave_class_max) {
             const uint64_t waves = (cnt + (64u >> c) - 1) / (64u >> c);
             hipLaunchKernelGGL(k_small_sort_wave, dim3((waves + 3) / 4), dim3(kThreads), 0, st,
-                               X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len,
+                               X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len, offlen,
                                X.keyA.as<uint64_t>(), X.keyB.as<uint64_t>(), s->s_off.as<uint64_t>(),
                                s->s_len.as<uint32_t>());
         } else {
