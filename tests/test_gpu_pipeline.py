@@ -21,6 +21,13 @@ def arena(strings):
     return np.frombuffer(b"".join(s + b"\0" for s in strings), dtype=np.uint8).copy()
 
 
+def _short_with_long(rng):
+    strs = [bytes(rng.integers(97, 101, int(l), dtype=np.uint8)) for l in rng.integers(1, 9, 20000)]
+    for pos in rng.choice(len(strs), 6, replace=False):
+        strs[int(pos)] = bytes(rng.integers(97, 101, 5000, dtype=np.uint8))
+    return strs
+
+
 def inputs():
     rng = np.random.default_rng(3)
     small = [bytes(rng.integers(97, 100, int(l), dtype=np.uint8)) for l in rng.integers(0, 12, 3000)]
@@ -29,6 +36,10 @@ def inputs():
         "skewed": O.gen_skewed(O.skew_params(6000, long_len=200, short_len=20), 0x5EED0003),
         "suffix": O.gen_suffixes(5000, 0x5EED0004, repeats=10, rep_min=100, rep_max=800, cap=300),
         "tiny_alphabet": arena(small),
+        # mostly 1-8 characters with a few 5000-character strings: the short
+        # copy path of the exchange, where a wave's byte range can exceed its
+        # LDS buffer
+        "short_with_long": arena(_short_with_long(rng)),
     }
 
 
