@@ -903,28 +903,44 @@ __device__ inline uint32_t pair_of_tile(const MergePair* __restrict__ pairs, uin
 
 // Merge-path split (number of A elements among the first d outputs, A wins
 // ties) at the start of every tile, from global memory.  ord == nullptr: the
-// first round, whose order is the receive order.
+// first round, whose order is the receive order.  Four lanes per tile search
+// 4-ary: lanes 0-2 each test one quartile point of the interval per step, so
+// a split takes about log4 of the run length dependent steps.
 __global__ void k_merge_split(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t ntiles,
                               const uint32_t* __restrict__ ord, const uint8_t* __restrict__ chars,
                               const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
                               const uint64_t* __restrict__ keyc, const uint64_t* __restrict__ keyc2,
                               const uint32_t* __restrict__ rcp, uint64_t* __restrict__ split) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;  // uniform over the 4 lanes
     if (t >= ntiles) return;
+    const uint32_t l = threadIdx.x & 3, g0 = threadIdx.x & ~3u;
     const MergePair Pr = pairs[pair_of_tile(pairs, npairs, t)];
     const ViewCtx V{chars, voff, vlen, keyc2, rcp[Pr.recv]};
     const uint64_t A = Pr.out, B = Pr.out + Pr.na;
     const uint64_t d = (t - Pr.tile0) * kTileOut;
+    // the answer is the first mid in [lo, hi] with A[mid] > B[d - mid - 1]
+    // (hi if none); the test is monotone in mid, so the probes that hold form
+    // a prefix of lanes 0-2
     uint64_t lo = d > Pr.nb ? d - Pr.nb : 0, hi = d < Pr.na ? d : Pr.na;
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) / 2;
-        const uint64_t ia = A + mid, ib = B + d - mid - 1;
-        const uint32_t a = ord ? ord[ia] : (uint32_t)ia, b = ord ? ord[ib] : (uint32_t)ib;
-        uint32_t x;
-        if (elem_cmp(V, a, keyc[a], b, keyc[b], V.cp, x) <= 0) lo = mid + 1;
-        else hi = mid;
+    while (lo < hi) {  // uniform over the 4 lanes (same lo, hi)
+        const uint64_t span = hi - lo;
+        const uint64_t m = lo + span * (l + 1) / 4;  // non-decreasing in l, < hi for l < 3
+        bool le = false;
+        if (l < 3) {
+            const uint64_t ia = A + m, ib = B + d - m - 1;
+            const uint32_t a = ord ? ord[ia] : (uint32_t)ia, b = ord ? ord[ib] : (uint32_t)ib;
+            uint32_t x;
+            le = elem_cmp(V, a, keyc[a], b, keyc[b], V.cp, x) <= 0;
+        }
+        const uint32_t c = (uint32_t)__shfl((int)le, g0 + 0) + (uint32_t)__shfl((int)le, g0 + 1) +
+                           (uint32_t)__shfl((int)le, g0 + 2);
+        const uint64_t m0 = lo + span / 4, m1 = lo + span * 2 / 4, m2 = lo + span * 3 / 4;
+        const uint64_t mc = c == 0 ? m0 : (c == 1 ? m1 : m2);  // the first probe that failed (c < 3)
+        const uint64_t ml = c == 1 ? m0 : (c == 2 ? m1 : m2);  // the last probe that held (c > 0)
+        if (c > 0) lo = ml + 1;
+        if (c < 3) hi = mc;
     }
-    split[t] = lo;
+    if (l == 0) split[t] = lo;
 }
 
 // One tile of kTileOut outputs per block: the tile's windows of A and B
@@ -1137,7 +1153,7 @@ void merge(dss_set_s* s) {
             uint64_t* split = splits + tbeg[r];
             const bool last = r + 2 == rbeg.size();
             KScope ks(s->timer, st, "merge_pairs", n * 24.0);
-            hipLaunchKernelGGL(k_merge_split, dim3((tiles + 255) / 256), dim3(256), 0, st, rp, np, tiles, in_ord,
+            hipLaunchKernelGGL(k_merge_split, dim3((tiles * 4 + 255) / 256), dim3(256), 0, st, rp, np, tiles, in_ord,
                                s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                                P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp, split);
             hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, rp, np, split, in_ord, in_lc, ord, lc,
