@@ -864,24 +864,52 @@ k_hp_dedup(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids, 
     }
 }
 
+// Next round's candidates: LCP duplicates stay, hashed positions stay if
+// they are an LCP-local root (the next position, in the same PE as lcp is 0
+// at PE starts, is a local LCP duplicate) or a hash duplicate.  Eight
+// positions per thread and step, so the byte arrays move as 8-byte words.
+__device__ inline uint32_t pd_keep(uint8_t f, uint32_t lcp_next, uint8_t cand_next, bool has_next, uint8_t d,
+                                   uint32_t depth) {
+    if (f == kLcpDup) return 1;
+    if (f == kHashed) return (has_next && lcp_next >= depth && cand_next) ? 1u : (d ? 1u : 0u);
+    return 0;
+}
+
 __global__ void __launch_bounds__(256)
 k_pd_next(const uint8_t* __restrict__ cand, const uint8_t* __restrict__ flags, const uint64_t* __restrict__ pe_begin,
           uint32_t p, const uint32_t* __restrict__ lcp, uint32_t depth, uint64_t n, const uint8_t* __restrict__ dup,
           uint8_t* __restrict__ next, unsigned long long* __restrict__ ncand) {
     uint32_t kept = 0;
-    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t keep = 0;
-        const uint8_t f = flags[c];
-        if (f == kLcpDup) {
-            keep = 1;
-        } else if (f == kHashed) {
-            // LCP-local root: the next position (same PE, as lcp is 0 at PE
-            // starts) is a local LCP duplicate
-            if (c + 1 < n && lcp[c + 1] >= depth && cand[c + 1]) keep = 1;
-            else keep = dup[c];
+    const uint64_t n8 = n / 8;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n8; w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c0 = 8 * w;
+        const uint64_t F = reinterpret_cast<const uint64_t*>(flags)[w];
+        const uint64_t Dp = reinterpret_cast<const uint64_t*>(dup)[w];
+        const uint64_t Cd = reinterpret_cast<const uint64_t*>(cand)[w];
+        const uint4 la = reinterpret_cast<const uint4*>(lcp)[2 * w];
+        const uint4 lb = reinterpret_cast<const uint4*>(lcp)[2 * w + 1];
+        const bool tail = c0 + 8 < n;
+        const uint32_t l8 = tail ? lcp[c0 + 8] : 0u;
+        const uint8_t c8 = tail ? cand[c0 + 8] : 0;
+        const uint32_t ln[8] = {la.y, la.z, la.w, lb.x, lb.y, lb.z, lb.w, l8};  // lcp of the next position
+        uint64_t out = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            const uint8_t cn = b < 7 ? (uint8_t)(Cd >> (8 * (b + 1))) : c8;
+            const uint32_t k = pd_keep((uint8_t)(F >> (8 * b)), ln[b], cn, b < 7 || tail, (uint8_t)(Dp >> (8 * b)),
+                                       depth);
+            out |= (uint64_t)k << (8 * b);
+            kept += k;
         }
-        next[c] = (uint8_t)keep;
-        kept += keep;
+        reinterpret_cast<uint64_t*>(next)[w] = out;
+    }
+    // the last n % 8 positions
+    for (uint64_t c = 8 * n8 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (uint64_t)gridDim.x * blockDim.x) {
+        const bool has = c + 1 < n;
+        const uint32_t k = pd_keep(flags[c], has ? lcp[c + 1] : 0u, has ? cand[c + 1] : (uint8_t)0, has, dup[c], depth);
+        next[c] = (uint8_t)k;
+        kept += k;
     }
     block_count_add(kept, ncand);
 }
