@@ -224,7 +224,7 @@ struct PipeState {
     DevBuf recv_bytes;
     DevBuf recv_byte_off, recv_lcp, recv_src_pe, recv_src_pos, recv_view_off, recv_view_len,
         recv_from, recv_size, recv_stage;
-    std::vector<uint64_t> recv_begin, byte_begin;  // p+1 each
+    std::vector<uint64_t> recv_begin;  // p+1
     DevBuf layout;                                 // send_start / run offsets (u64)
     // merge (per receive-side output position)
     bool merged = false;

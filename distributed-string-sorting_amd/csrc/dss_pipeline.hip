@@ -720,6 +720,12 @@ __global__ void k_byte_begin(const uint64_t* __restrict__ recv_begin, const uint
     out[q] = rb < n ? byte_off[rb] : total;
 }
 
+// p + 1 byte begins of the receivers, after the layout's 2 p^2 values and p + 1
+// spare slots.
+static inline uint64_t* byte_begin_dev(PipeState& P, uint32_t p) {
+    return P.layout.as<uint64_t>() + (uint64_t)2 * p * p + (p + 1);
+}
+
 void exchange(dss_set_s* s, int mode) {
     PipeState& P = s->pp;
     DSS_REQUIRE(P.partitioned, DSS_ESTATE, "dss_partition has not run");
@@ -814,16 +820,10 @@ void exchange(dss_set_s* s, int mode) {
                            P.recv_lcp.as<uint32_t>());
     }
     DSS_HIP(hipGetLastError());
-    // byte_begin per receiver: the byte offset of its first received string
-    P.byte_begin.assign(p + 1, total);
-    {
-        uint64_t* rbd = P.layout.as<uint64_t>() + (uint64_t)2 * p * p;  // p+1 receive begins
-        uint64_t* bbd = rbd + (p + 1);                                   // p+1 byte begins
-        DSS_HIP(hipMemcpyAsync(rbd, P.recv_begin.data(), (p + 1) * 8, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_byte_begin, dim3((p + 1 + 255) / 256), dim3(256), 0, st, rbd, P.recv_byte_off.as<uint64_t>(),
-                           n, total, p, bbd);
-        P.mbox.fetch(st, P.byte_begin.data(), bbd, (p + 1) * 8);
-    }
+    // byte_begin per receiver (the byte offset of its first received string),
+    // on the device; dss_exchange reads it back when the caller asks for it
+    hipLaunchKernelGGL(k_byte_begin, dim3((p + 1 + 255) / 256), dim3(256), 0, st, rb_d, P.recv_byte_off.as<uint64_t>(),
+                       n, total, p, byte_begin_dev(P, p));
     P.mode = mode;
     P.exchanged = true;
     P.merged = P.extended = false;
@@ -1368,7 +1368,7 @@ int dss_exchange(dss_set s, int mode, uint64_t* recv_begin, uint64_t* byte_begin
         DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
         exchange(s, mode);
         if (recv_begin) memcpy(recv_begin, s->pp.recv_begin.data(), (s->p + 1) * 8);
-        if (byte_begin) memcpy(byte_begin, s->pp.byte_begin.data(), (s->p + 1) * 8);
+        if (byte_begin) s->pp.mbox.fetch(s->stream, byte_begin, byte_begin_dev(s->pp, s->p), (s->p + 1) * 8);
         if (recv_nbytes) *recv_nbytes = s->pp.recv_nbytes;
     });
 }
