@@ -216,10 +216,15 @@ __global__ void k_sample_compact(const Sample* __restrict__ slots, const uint32_
 // Rank of each sample in the global sample order (the result of the sample
 // sort RQuick::sort, misc.hpp:503-514): by string, then (indexed samplers,
 // IndexStringComparator misc.hpp:35-53) by global index, else by sample id.
-__global__ void __launch_bounds__(256) k_sample_rank(const Sample* __restrict__ smp, uint32_t ns, int indexed,
-                                                     const uint8_t* __restrict__ chars, uint32_t* __restrict__ sorted) {
+// One block per sample; the launch may cover more blocks than the *ns_p
+// samples the compaction produced (they return at once).
+__global__ void __launch_bounds__(256) k_sample_rank(const Sample* __restrict__ smp, const uint32_t* __restrict__ ns_p,
+                                                     int indexed, const uint8_t* __restrict__ chars,
+                                                     uint32_t* __restrict__ sorted) {
     __shared__ uint32_t red[4];
+    const uint32_t ns = *ns_p;
     const uint32_t i = blockIdx.x;
+    if (i >= ns) return;  // block-uniform
     const Sample a = smp[i];
     uint32_t c = 0;
     for (uint32_t j = threadIdx.x; j < ns; j += blockDim.x) {
@@ -378,10 +383,16 @@ void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, boo
                                s->perm.as<uint32_t>(), s->off.as<uint64_t>(), s->len.as<uint32_t>(), dist, wprefix,
                                by_chars, factor, max_len, nr_max, slots, counts);
             hipLaunchKernelGGL(k_sample_compact, dim3(1), dim3(256), 0, st, slots, counts, p, nr_max, compact, ns_d);
-            uint32_t ns = 0;
-            P.mbox.fetch(st, &ns, ns_d, 4);
-            if (ns)
-                hipLaunchKernelGGL(k_sample_rank, dim3(ns), dim3(256), 0, st, compact, ns, indexed,
+            // a block per sample: up to 65536 possible samples the grid covers
+            // them all and needs no read-back of the count
+            uint64_t rank_blocks = nsmax;
+            if (nsmax > 65536) {
+                uint32_t ns = 0;
+                P.mbox.fetch(st, &ns, ns_d, 4);
+                rank_blocks = ns;
+            }
+            if (rank_blocks)
+                hipLaunchKernelGGL(k_sample_rank, dim3((uint32_t)rank_blocks), dim3(256), 0, st, compact, ns_d, indexed,
                                    s->arena(), P.sample_sorted.as<uint32_t>());
             if ((uint64_t)p * (p - 1) <= (1u << 16)) {
                 P.splitter_pos.reserve((uint64_t)p * (p - 1) * 8);
