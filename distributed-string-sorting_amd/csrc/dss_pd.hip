@@ -122,7 +122,14 @@ __global__ void k_pd_init(uint8_t* __restrict__ cand, uint8_t* __restrict__ roun
 // candidates (and d <= the LCP <= len(c), so no string ends).
 __global__ void k_pd_min_run(const uint32_t* __restrict__ lcp, uint64_t n, unsigned int* __restrict__ out) {
     uint32_t v = 0xFFFFFFFFu;
-    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (uint64_t)gridDim.x * blockDim.x)
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t n4 = n / 4;  // four positions per 16-byte load
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n4; w += stride) {
+        const uint4 x = reinterpret_cast<const uint4*>(lcp)[w];
+        const uint32_t nx = 4 * w + 4 < n ? lcp[4 * w + 4] : 0u;
+        v = min(v, min(min(max(x.x, x.y), max(x.y, x.z)), min(max(x.z, x.w), max(x.w, nx))));
+    }
+    for (uint64_t c = 4 * n4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride)
         v = min(v, max(lcp[c], c + 1 < n ? lcp[c + 1] : 0u));
     for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_down(v, o));
     __shared__ uint32_t part[4];  // one atomic per block: single-address atomics serialise
@@ -332,8 +339,13 @@ k_head_count(const uint32_t* __restrict__ lcp, uint64_t n, uint32_t B, uint32_t*
     __shared__ uint32_t red[kHsT / 64];
     const uint64_t b = (uint64_t)blockIdx.x * kHsB + (uint64_t)threadIdx.x * kHsI;
     uint32_t cnt = 0;
-    for (uint32_t k = 0; k < kHsI; ++k)
-        if (b + k < n) cnt += lcp[b + k] < B;
+    if (b + kHsI <= n) {  // two 16-byte loads
+        const uint4 x = reinterpret_cast<const uint4*>(lcp + b)[0], y = reinterpret_cast<const uint4*>(lcp + b)[1];
+        cnt = (x.x < B) + (x.y < B) + (x.z < B) + (x.w < B) + (y.x < B) + (y.y < B) + (y.z < B) + (y.w < B);
+    } else {
+        for (uint32_t k = 0; k < kHsI; ++k)
+            if (b + k < n) cnt += lcp[b + k] < B;
+    }
     for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o);
     if (lane_id() == 0) red[threadIdx.x >> 6] = cnt;
     __syncthreads();
@@ -378,10 +390,15 @@ k_head_rank(const uint32_t* __restrict__ lcp, uint64_t n, uint32_t B, const uint
     const uint64_t b = (uint64_t)blockIdx.x * kHsB + (uint64_t)threadIdx.x * kHsI;
     uint32_t f[kHsI];
     uint32_t cnt = 0;
-    for (uint32_t k = 0; k < kHsI; ++k) {
-        f[k] = b + k < n && lcp[b + k] < B;
-        cnt += f[k];
+    const bool whole = b + kHsI <= n;
+    if (whole) {  // two 16-byte loads
+        const uint4 x = reinterpret_cast<const uint4*>(lcp + b)[0], y = reinterpret_cast<const uint4*>(lcp + b)[1];
+        f[0] = x.x < B; f[1] = x.y < B; f[2] = x.z < B; f[3] = x.w < B;
+        f[4] = y.x < B; f[5] = y.y < B; f[6] = y.z < B; f[7] = y.w < B;
+    } else {
+        for (uint32_t k = 0; k < kHsI; ++k) f[k] = b + k < n && lcp[b + k] < B;
     }
+    for (uint32_t k = 0; k < kHsI; ++k) cnt += f[k];
     part[threadIdx.x] = cnt;
     __syncthreads();
     for (uint32_t o = 1; o < kHsT; o <<= 1) {
@@ -391,10 +408,17 @@ k_head_rank(const uint32_t* __restrict__ lcp, uint64_t n, uint32_t B, const uint
         __syncthreads();
     }
     uint32_t run = bcnt[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0u);
+    uint32_t r[kHsI];
     for (uint32_t k = 0; k < kHsI; ++k) {
         if (b + k >= n) break;
         if (f[k]) hpos[run++] = (uint32_t)(b + k);
-        hrank[b + k] = (run - 1) | (f[k] ? kHeadBit : 0u);
+        r[k] = (run - 1) | (f[k] ? kHeadBit : 0u);
+    }
+    if (whole) {  // two 16-byte stores
+        reinterpret_cast<uint4*>(hrank + b)[0] = make_uint4(r[0], r[1], r[2], r[3]);
+        reinterpret_cast<uint4*>(hrank + b)[1] = make_uint4(r[4], r[5], r[6], r[7]);
+    } else {
+        for (uint32_t k = 0; k < kHsI && b + k < n; ++k) hrank[b + k] = r[k];
     }
 }
 
