@@ -173,45 +173,103 @@ __device__ inline uint64_t lds_word(const uint64_t* rows, uint32_t b) {
     return sh ? (lo >> sh) | (rows[w + 1] << (64 - sh)) : lo;
 }
 
-// Classification of the candidates of one round, one thread per position:
-// end of string, local LCP duplicate or hashed.  Positions to hash collect
-// in LDS over kMarkIters block iterations and are appended to `list` with
-// one global atomic per flush.
-constexpr uint32_t kMarkIters = 16;
+// Classification of the candidates of one round, four positions per thread
+// (word loads and stores of the byte and 32-bit arrays): end of string,
+// local LCP duplicate or hashed.  Positions to hash collect in LDS over
+// kMarkIters block iterations and are appended to `list` with one global
+// atomic per flush.
+constexpr uint32_t kMarkIters = 4;
+constexpr uint32_t kMarkPer = 4;  // positions per thread and iteration
+
+// One position: its flag; rounds and dist are updated for candidates.
+__device__ inline uint8_t pd_mark_one(uint8_t cc, uint8_t cprev, uint32_t L, uint32_t lc, uint64_t c, uint32_t depth,
+                                      uint8_t& rnd, uint32_t& d, bool& wd) {
+    uint8_t f = 0;
+    wd = false;
+    if (cc) {
+        // lcp is 0 at every PE start and depth >= 1, so lcp[c] >= depth
+        // implies that c-1 is in the same PE
+        if (depth <= L) f = (c > 0 && lc >= depth && cprev) ? kLcpDup : kHashed;
+        rnd = rnd + (rnd < 255);
+        d = depth > L ? L : depth;  // end of string: results = len
+        wd = true;
+    }
+    return f;
+}
 
 __global__ void __launch_bounds__(256)
 k_pd_mark(const uint8_t* __restrict__ cand, const uint32_t* __restrict__ lcp, const uint32_t* __restrict__ s_len,
           uint32_t depth, uint64_t n, uint8_t* __restrict__ flags, uint8_t* __restrict__ rounds,
           uint32_t* __restrict__ dist, uint32_t* __restrict__ list, unsigned long long* __restrict__ nhashed) {
-    __shared__ uint32_t sl[256 * kMarkIters];
+    __shared__ uint32_t sl[256 * kMarkPer * kMarkIters];
     __shared__ uint32_t scount, base;
     const uint32_t lane = lane_id();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kMarkPer;
     if (threadIdx.x == 0) scount = 0;
     __syncthreads();
     uint32_t it = 0;
-    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
-        const uint64_t c = b0 + threadIdx.x;
-        uint8_t f = 0;
-        if (c < n) {
-            // independent loads first; lcp is 0 at every PE start and depth >= 1,
-            // so lcp[c] >= depth implies that c-1 is in the same PE
-            const uint8_t cc = cand[c];
-            const uint8_t cp = c > 0 ? cand[c - 1] : 0;
-            const uint32_t L = s_len[c];
-            const uint32_t lc = lcp[c];
-            if (cc) {
-                if (depth <= L) f = (c > 0 && lc >= depth && cp) ? kLcpDup : kHashed;
-                rounds[c] = rounds[c] + (rounds[c] < 255);
-                dist[c] = depth > L ? L : depth;  // end of string: results = len
+    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x * kMarkPer; b0 < n; b0 += stride) {  // block-uniform
+        const uint64_t c0 = b0 + (uint64_t)kMarkPer * threadIdx.x;
+        uint8_t f[kMarkPer] = {0, 0, 0, 0};
+        if (c0 + kMarkPer <= n) {
+            const uint32_t C = *reinterpret_cast<const uint32_t*>(cand + c0);
+            const uint8_t cprev = c0 > 0 ? cand[c0 - 1] : 0;
+            const uint4 L = *reinterpret_cast<const uint4*>(s_len + c0);
+            const uint4 LC = *reinterpret_cast<const uint4*>(lcp + c0);
+            const uint32_t Ls[4] = {L.x, L.y, L.z, L.w}, Lc[4] = {LC.x, LC.y, LC.z, LC.w};
+            uint32_t R = C ? *reinterpret_cast<const uint32_t*>(rounds + c0) : 0u;
+            uint32_t d[4] = {0, 0, 0, 0};
+            bool wd[4];
+#pragma unroll
+            for (uint32_t k = 0; k < kMarkPer; ++k) {
+                const uint8_t cc = (uint8_t)(C >> (8 * k));
+                const uint8_t cp = k ? (uint8_t)(C >> (8 * (k - 1))) : cprev;
+                uint8_t rnd = (uint8_t)(R >> (8 * k));
+                f[k] = pd_mark_one(cc, cp, Ls[k], Lc[k], c0 + k, depth, rnd, d[k], wd[k]);
+                R = (R & ~(0xFFu << (8 * k))) | ((uint32_t)rnd << (8 * k));
             }
-            flags[c] = f;
+            if (C) *reinterpret_cast<uint32_t*>(rounds + c0) = R;
+            if (wd[0] && wd[1] && wd[2] && wd[3]) {
+                *reinterpret_cast<uint4*>(dist + c0) = make_uint4(d[0], d[1], d[2], d[3]);
+            } else {
+#pragma unroll
+                for (uint32_t k = 0; k < kMarkPer; ++k)
+                    if (wd[k]) dist[c0 + k] = d[k];
+            }
+            *reinterpret_cast<uint32_t*>(flags + c0) =
+                (uint32_t)f[0] | ((uint32_t)f[1] << 8) | ((uint32_t)f[2] << 16) | ((uint32_t)f[3] << 24);
+        } else {
+            for (uint32_t k = 0; k < kMarkPer && c0 + k < n; ++k) {
+                const uint64_t c = c0 + k;
+                uint8_t rnd = rounds[c];
+                uint32_t dk = 0;
+                bool w = false;
+                f[k] = pd_mark_one(cand[c], c > 0 ? cand[c - 1] : 0, s_len[c], lcp[c], c, depth, rnd, dk, w);
+                if (w) {
+                    rounds[c] = rnd;
+                    dist[c] = dk;
+                }
+                flags[c] = f[k];
+            }
         }
-        const uint64_t bal = __ballot(f == kHashed);
+        // append in position order (the hash kernel writes a row per listed
+        // position; consecutive positions keep those writes in whole lines):
+        // a wave prefix sum of the threads' counts
+        uint32_t cnt = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kMarkPer; ++k) cnt += f[k] == kHashed;
+        uint32_t incl = cnt;
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const uint32_t wtot = __shfl(incl, 63, 64);
         uint32_t wbase = 0;
-        if (lane == 0 && bal) wbase = atomicAdd(&scount, (uint32_t)__popcll(bal));
-        wbase = __shfl(wbase, 0);
-        if (f == kHashed) sl[wbase + (uint32_t)__popcll(bal & lanemask_lt())] = (uint32_t)c;
+        if (lane == 0 && wtot) wbase = atomicAdd(&scount, wtot);
+        wbase = __shfl(wbase, 0, 64) + (incl - cnt);
+#pragma unroll
+        for (uint32_t k = 0; k < kMarkPer; ++k)
+            if (f[k] == kHashed) sl[wbase++] = (uint32_t)(c0 + k);
         if (++it == kMarkIters || b0 + stride >= n) {  // block-uniform
             it = 0;
             __syncthreads();
@@ -1072,7 +1130,8 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
             // every position: candidate flags of c and c-1, length, lcp, the
             // flag written; every candidate: rounds and dist
             KScope ks(s->timer, st, "pd_mark", (double)n * 11.0 + (double)ncand * 6.0);
-            hipLaunchKernelGGL(k_pd_mark, dim3(std::min<uint64_t>((n + 255) / 256, kPdGrid)), blk, 0, st, cand,
+            hipLaunchKernelGGL(k_pd_mark, dim3(std::min<uint64_t>((n + 256 * kMarkPer - 1) / (256 * kMarkPer), kPdGrid)),
+                               blk, 0, st, cand,
                                s->lcp.as<uint32_t>(), s->s_len.as<uint32_t>(), (uint32_t)depth, n,
                                P.pd_flags.as<uint8_t>(), P.rounds.as<uint8_t>(), P.dist.as<uint32_t>(),
                                P.pd_list.as<uint32_t>(), ctr);
