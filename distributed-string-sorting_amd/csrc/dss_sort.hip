@@ -156,41 +156,52 @@ __device__ inline uint32_t seg_chunks(uint32_t nt) {
     return nt > kChunkTiles ? (nt + kChunkTiles - 1) / kChunkTiles : 0u;
 }
 
-__global__ void k_build_tiles(Seg* __restrict__ segs, uint32_t* __restrict__ pc) {
+// 1024 threads: wave-level shuffle scans of the tile and chunk counts, the
+// 16 wave totals combined through LDS (three barriers per 1024 segments).
+__global__ void __launch_bounds__(1024) k_build_tiles(Seg* __restrict__ segs, uint32_t* __restrict__ pc) {
     const uint32_t nseg = pc[kCtrLarge];  // written by the previous pass (or the seed)
-    __shared__ uint32_t part[1024], cpart[1024];
+    __shared__ uint32_t wsum[16], wcsum[16], woff[16], wcoff[16];
     __shared__ uint32_t carry, ccarry;
-    const uint32_t t = threadIdx.x;
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) carry = ccarry = 0;
     __syncthreads();
-    for (uint32_t base = 0; base < nseg; base += 1024) {
+    for (uint32_t base = 0; base < nseg; base += 1024) {  // block-uniform
         const uint32_t i = base + t;
         const uint32_t nt = i < nseg ? seg_tiles(segs[i].cnt) : 0u;
         const uint32_t nc = seg_chunks(nt);
-        part[t] = nt;
-        cpart[t] = nc;
-        __syncthreads();
-        for (uint32_t o = 1; o < 1024; o <<= 1) {
-            const uint32_t x = t >= o ? part[t - o] : 0;
-            const uint32_t y = t >= o ? cpart[t - o] : 0;
-            __syncthreads();
-            part[t] += x;
-            cpart[t] += y;
-            __syncthreads();
+        uint32_t a = nt, c = nc;  // inclusive scans within the wave
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t x = __shfl_up(a, o, 64), y = __shfl_up(c, o, 64);
+            if (lane >= o) {
+                a += x;
+                c += y;
+            }
         }
+        if (lane == 63) {
+            wsum[w] = a;
+            wcsum[w] = c;
+        }
+        __syncthreads();
+        if (t == 0) {
+            uint32_t x = carry, y = ccarry;
+            for (uint32_t k = 0; k < 16; ++k) {
+                woff[k] = x;
+                wcoff[k] = y;
+                x += wsum[k];
+                y += wcsum[k];
+            }
+            carry = x;
+            ccarry = y;
+        }
+        __syncthreads();
         if (i < nseg) {
             Seg s = segs[i];
-            s.tile0 = carry + part[t] - nt;
-            s.chunk0 = ccarry + cpart[t] - nc;
+            s.tile0 = woff[w] + a - nt;
+            s.chunk0 = wcoff[w] + c - nc;
             if (s.kdepth == kNone) s.cp = kNone;
             segs[i] = s;
         }
-        __syncthreads();
-        if (t == 1023) {
-            carry += part[1023];
-            ccarry += cpart[1023];
-        }
-        __syncthreads();
+        __syncthreads();  // woff is rewritten for the next 1024 segments
     }
     if (t == 0) {
         pc[kCtrTiles] = carry;
