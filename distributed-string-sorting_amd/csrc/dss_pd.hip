@@ -885,10 +885,17 @@ k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ has
 __global__ void k_hp_bounds(const uint64_t* __restrict__ keys, uint64_t nh, uint32_t kb, uint32_t* __restrict__ bstart,
                             uint32_t* __restrict__ bend) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nh) return;
-    const uint64_t b = keys[i] >> (64 - kb);
-    if (i == 0 || (keys[i - 1] >> (64 - kb)) != b) bstart[b] = (uint32_t)i;
-    if (i + 1 == nh || (keys[i + 1] >> (64 - kb)) != b) bend[b] = (uint32_t)(i + 1);
+    const uint32_t lane = threadIdx.x & 63;
+    const bool valid = i < nh;
+    const uint64_t x = valid ? keys[i] : 0;
+    // the neighbours' keys from the adjacent lanes (loads at the wave edges)
+    uint64_t prev = __shfl_up(x, 1, 64), next = __shfl_down(x, 1, 64);
+    if (!valid) return;
+    if (lane == 0 && i > 0) prev = keys[i - 1];
+    if (lane == 63 && i + 1 < nh) next = keys[i + 1];
+    const uint64_t b = x >> (64 - kb);
+    if (i == 0 || (prev >> (64 - kb)) != b) bstart[b] = (uint32_t)i;
+    if (i + 1 == nh || (next >> (64 - kb)) != b) bend[b] = (uint32_t)(i + 1);
 }
 
 __global__ void __launch_bounds__(256)
