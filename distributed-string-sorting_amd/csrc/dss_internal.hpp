@@ -224,6 +224,7 @@ struct PipeState {
     DevBuf recv_bytes;
     DevBuf recv_byte_off, recv_lcp, recv_src_pe, recv_src_pos, recv_view_off, recv_view_len,
         recv_from, recv_size, recv_stage;
+    bool origin_valid = false;         // recv_src_pe / recv_src_pos filled (on demand)
     std::vector<uint64_t> recv_begin;  // p+1
     DevBuf layout;                                 // send_start / run offsets (u64)
     // merge (per receive-side output position)
@@ -302,6 +303,7 @@ void sort_lcp(dss_set_s* s);
 uint64_t avg_lcp(dss_set_s* s);
 void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, bool use_dist);
 void exchange(dss_set_s* s, int mode);
+void recv_origin(dss_set_s* s);  // fills recv_src_pe / recv_src_pos once per exchange
 void merge(dss_set_s* s);
 void extend_prefix(dss_set_s* s);
 void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth);

@@ -1363,6 +1363,7 @@ int dss_get_permutation(dss_set s, uint32_t* string_index, uint32_t* pe_index) {
         DSS_REQUIRE(P.merged, DSS_ESTATE, "no merged output");
         const uint64_t n = s->n;
         if (!n) return;
+        recv_origin(s);
         DevBuf tmp;
         tmp.reserve(n * 8);
         uint32_t* si = tmp.as<uint32_t>();

@@ -436,6 +436,60 @@ __device__ inline uint32_t bucket_of(const uint64_t* __restrict__ send_start, ui
     return lo - 1;
 }
 
+// Where source position i goes: its PE r, its position j inside r, its
+// bucket q, the bucket's first position st inside r and its receive
+// position dst (bucket q of PE r lands as run r of receiver q).
+struct ExchPos {
+    uint32_t r, q;
+    uint64_t j, st, dst;
+};
+__device__ inline ExchPos exch_pos(const uint64_t* __restrict__ pe_begin, const uint64_t* __restrict__ send_start,
+                                   const uint64_t* __restrict__ run_off, uint32_t p, uint64_t i) {
+    uint32_t lo = 0, hi = p;  // PE of position i: last r with pe_begin[r] <= i (non-empty)
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pe_begin[mid] <= i) lo = mid + 1; else hi = mid;
+    }
+    ExchPos e;
+    e.r = lo - 1;
+    e.j = i - pe_begin[e.r];
+    e.q = bucket_of(send_start, p, e.r, e.j);
+    e.st = send_start[(uint64_t)e.r * p + e.q];
+    e.dst = run_off[(uint64_t)e.q * p + e.r] + (e.j - e.st);
+    return e;
+}
+
+// Origin (source PE, position inside it) of every received string, computed
+// from the exchange layout when a caller asks for it; the exchange itself
+// does not write it.
+__global__ void k_exch_origin(const uint64_t* __restrict__ pe_begin, uint32_t p,
+                              const uint64_t* __restrict__ send_start, const uint64_t* __restrict__ run_off,
+                              uint64_t n, uint32_t* __restrict__ r_pe, uint32_t* __restrict__ r_pos) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ExchPos e = exch_pos(pe_begin, send_start, run_off, p, i);
+    r_pe[e.dst] = e.r;
+    r_pos[e.dst] = (uint32_t)e.j;
+}
+
+void recv_origin(dss_set_s* s) {
+    PipeState& P = s->pp;
+    DSS_REQUIRE(P.exchanged, DSS_ESTATE, "dss_exchange has not run");
+    if (P.origin_valid) return;
+    const uint64_t n = s->n, nn = std::max<uint64_t>(n, 1);
+    const uint32_t p = s->p;
+    P.recv_src_pe.reserve(nn * 4);
+    P.recv_src_pos.reserve(nn * 4);
+    if (n) {
+        const uint64_t* ls = P.layout.as<uint64_t>();
+        hipLaunchKernelGGL(k_exch_origin, dim3((n + 255) / 256), dim3(256), 0, s->stream,
+                           s->pe_begin_d.as<uint64_t>(), p, ls, ls + (uint64_t)p * p, n,
+                           P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>());
+        DSS_HIP(hipGetLastError());
+    }
+    P.origin_valid = true;
+}
+
 constexpr uint32_t kXiP = 32;  // PEs up to which k_exch_index stages its tables
 
 // One thread per source position: destination, size and receive metadata.
@@ -448,7 +502,7 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
                              const uint32_t* __restrict__ dist,
                              const uint64_t* __restrict__ send_start, const uint64_t* __restrict__ run_off, int mode,
                              uint64_t n, uint64_t* __restrict__ size, uint32_t* __restrict__ r_lcp,
-                             uint32_t* __restrict__ r_pe, uint32_t* __restrict__ r_pos, uint64_t* __restrict__ r_voff,
+                             uint64_t* __restrict__ r_voff,
                              uint32_t* __restrict__ r_vlen, uint32_t* __restrict__ r_from,
                              const uint32_t* __restrict__ sdepth, uint32_t* __restrict__ r_stage) {
     // up to kXiP PEs the PE begins and the layout tables are staged in LDS,
@@ -468,16 +522,8 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
     const uint64_t* const RO = staged_tables ? s_ro : run_off;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t lo = 0, hi = p;  // PE of position i: last r with pe_begin[r] <= i (non-empty)
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (PB[mid] <= i) lo = mid + 1; else hi = mid;
-    }
-    const uint32_t r = lo - 1;
-    const uint64_t j = i - PB[r];
-    const uint32_t q = bucket_of(SS, p, r, j);
-    const uint64_t st = SS[(uint64_t)r * p + q];
-    const uint64_t dst = RO[(uint64_t)q * p + r] + (j - st);
+    const ExchPos e = exch_pos(PB, SS, RO, p, i);
+    const uint64_t j = e.j, st = e.st, dst = e.dst;
     const uint32_t l = (j == st) ? 0u : lcp[i];
     const uint32_t L = s_len[i];
     uint64_t sz;
@@ -494,8 +540,6 @@ __global__ void k_exch_index(const uint64_t* __restrict__ pe_begin, uint32_t p, 
     }
     size[dst] = sz;
     r_lcp[dst] = mode == 3 && j != st ? kNone : l;  // mode 3 sends no LCPs
-    r_pe[dst] = r;
-    r_pos[dst] = (uint32_t)j;
     r_voff[dst] = s_off[i];
     r_vlen[dst] = vlen;
     r_from[dst] = from;
@@ -773,8 +817,6 @@ void exchange(dss_set_s* s, int mode) {
     P.recv_size.reserve(nn * 8);
     P.recv_byte_off.reserve((nn + 1) * 8);
     P.recv_lcp.reserve(nn * 4);
-    P.recv_src_pe.reserve(nn * 4);
-    P.recv_src_pos.reserve(nn * 4);
     P.recv_view_off.reserve(nn * 8);
     P.recv_view_len.reserve(nn * 4);
     P.recv_from.reserve(nn * 4);
@@ -788,7 +830,6 @@ void exchange(dss_set_s* s, int mode) {
                            s->lcp.as<uint32_t>(), s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(),
                            mode == 2 ? P.dist.as<uint32_t>() : nullptr, ls,
                            ls + (uint64_t)p * p, mode, n, P.recv_size.as<uint64_t>(), P.recv_lcp.as<uint32_t>(),
-                           P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                            P.recv_from.as<uint32_t>(), staged ? P.pd_sdepth.as<uint32_t>() : nullptr, r_stage);
     }
@@ -837,6 +878,7 @@ void exchange(dss_set_s* s, int mode) {
                        n, total, p, byte_begin_dev(P, p));
     P.mode = mode;
     P.exchanged = true;
+    P.origin_valid = false;
     P.merged = P.extended = false;
 }
 
@@ -1394,6 +1436,7 @@ int dss_get_exchange(dss_set s, uint8_t* bytes, uint32_t* lcp, uint32_t* src_pe,
             DSS_HIP(hipMemcpyAsync(bytes, P.recv_bytes.p, P.recv_nbytes, hipMemcpyDeviceToHost, st));
         if (s->n) {
             if (lcp) DSS_HIP(hipMemcpyAsync(lcp, P.recv_lcp.p, s->n * 4, hipMemcpyDeviceToHost, st));
+            if (src_pe || src_pos) recv_origin(s);
             if (src_pe) DSS_HIP(hipMemcpyAsync(src_pe, P.recv_src_pe.p, s->n * 4, hipMemcpyDeviceToHost, st));
             if (src_pos) DSS_HIP(hipMemcpyAsync(src_pos, P.recv_src_pos.p, s->n * 4, hipMemcpyDeviceToHost, st));
         }
@@ -1582,6 +1625,7 @@ static void exact_dist(dss_set_s* s) {
     P.exact_pos.reserve(nn * 8);
     if (!n) return;
     const hipStream_t st = s->stream;
+    recv_origin(s);
     KScope ks(s->timer, st, "exact_dist", n * 48.0);
     hipLaunchKernelGGL(k_full_glcp, dim3((n + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
                        P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>(), s->pe_begin_d.as<uint64_t>(),
@@ -1654,6 +1698,7 @@ extern "C" int dss_check(dss_set s, dss_check_t* out) {
         DSS_HIP(hipMemsetAsync(seen, 0, nw * 4, st));
         DSS_HIP(hipMemsetAsync(cnt, 0, 64, st));
         DSS_HIP(hipMemcpyAsync(rb, P.recv_begin.data(), (s->p + 1) * 8, hipMemcpyHostToDevice, st));
+        recv_origin(s);
         hipLaunchKernelGGL(k_check, dim3((n + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
                            P.m_lcp.as<uint32_t>(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                            P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>(), s->pe_begin_d.as<uint64_t>(),
