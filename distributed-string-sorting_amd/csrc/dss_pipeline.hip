@@ -1529,22 +1529,38 @@ __global__ void k_check(const uint32_t* __restrict__ ord, const uint32_t* __rest
     glcp[i] = l;
 }
 
-__global__ void k_dsum(const uint32_t* __restrict__ glcp, uint64_t n, unsigned long long* __restrict__ out) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t t = 0;
-    if (i < n) {
-        if (i + 1 < n) t = (uint64_t)max(glcp[i], glcp[i + 1]) + 1;
-        else t = (uint64_t)glcp[i] + 1;
-    }
+// Sum of per-thread values over a block (256 threads), one atomic per block;
+// every thread of the block calls it.
+__device__ inline void block_add(uint64_t t, unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long part[4];
     for (int o = 32; o > 0; o >>= 1) t += __shfl_down((unsigned long long)t, o);
-    if (lane_id() == 0 && t) atomicAdd(out, (unsigned long long)t);
+    if (lane_id() == 0) part[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long b = part[0] + part[1] + part[2] + part[3];
+        if (b) atomicAdd(out, b);
+    }
+    __syncthreads();  // part[] is reused by the next call
 }
 
-__global__ void k_popc(const unsigned int* __restrict__ w, uint64_t nw, unsigned long long* __restrict__ out) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t t = i < nw ? (uint64_t)__popc(w[i]) : 0;
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_down((unsigned long long)t, o);
-    if (lane_id() == 0 && t) atomicAdd(out, (unsigned long long)t);
+constexpr uint32_t kSumBlocks = 2048;  // grid of the two reductions below (256 threads each)
+
+// Σ (max(glcp[i], glcp[i+1]) + 1): the distinguishing-prefix total implied by
+// the sorted order's LCPs.
+__global__ void __launch_bounds__(256) k_dsum(const uint32_t* __restrict__ glcp, uint64_t n,
+                                              unsigned long long* __restrict__ out) {
+    uint64_t t = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        t += (uint64_t)(i + 1 < n ? max(glcp[i], glcp[i + 1]) : glcp[i]) + 1;
+    block_add(t, out);
+}
+
+__global__ void __launch_bounds__(256) k_popc(const unsigned int* __restrict__ w, uint64_t nw,
+                                              unsigned long long* __restrict__ out) {
+    uint64_t t = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (uint64_t)gridDim.x * blockDim.x)
+        t += (uint64_t)__popc(w[i]);
+    block_add(t, out);
 }
 
 // ExcatDistinguishingPrefix::filter_exact (src/sorter/distributed/
@@ -1590,29 +1606,22 @@ __global__ void k_exact_dist(const uint32_t* __restrict__ glcp, const uint64_t* 
 // is its whole length), cnt[1] strings cut before their end whose result is
 // not below 2 * DIST + 8 (the doubling schedule's bound when no hashes
 // collide), cnt[2] sum of DIST, cnt[3] sum of results.
-__global__ void k_dist_bounds(const uint32_t* __restrict__ exact, const uint32_t* __restrict__ res,
-                              const uint32_t* __restrict__ s_len, uint64_t n, unsigned long long* __restrict__ cnt) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_dist_bounds(const uint32_t* __restrict__ exact,
+                                                     const uint32_t* __restrict__ res,
+                                                     const uint32_t* __restrict__ s_len, uint64_t n,
+                                                     unsigned long long* __restrict__ cnt) {
     uint64_t lo = 0, hi = 0, sd = 0, sr = 0;
-    if (i < n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t d = exact[i], r = res[i], L = s_len[i];
-        lo = d > r + (r >= L ? 1 : 0);
-        hi = r < L && r >= 2 * d + 8;
-        sd = d;
-        sr = r;
+        lo += d > r + (r >= L ? 1 : 0);
+        hi += r < L && r >= 2 * d + 8;
+        sd += d;
+        sr += r;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        lo += __shfl_down((unsigned long long)lo, o);
-        hi += __shfl_down((unsigned long long)hi, o);
-        sd += __shfl_down((unsigned long long)sd, o);
-        sr += __shfl_down((unsigned long long)sr, o);
-    }
-    if (lane_id() == 0) {
-        if (lo) atomicAdd(&cnt[0], (unsigned long long)lo);
-        if (hi) atomicAdd(&cnt[1], (unsigned long long)hi);
-        if (sd) atomicAdd(&cnt[2], (unsigned long long)sd);
-        if (sr) atomicAdd(&cnt[3], (unsigned long long)sr);
-    }
+    block_add(lo, &cnt[0]);
+    block_add(hi, &cnt[1]);
+    block_add(sd, &cnt[2]);
+    block_add(sr, &cnt[3]);
 }
 
 static void exact_dist(dss_set_s* s) {
@@ -1661,8 +1670,8 @@ extern "C" int dss_check_dist(dss_set s, dss_dist_check_t* out) {
         const hipStream_t st = s->stream;
         P.exact_cnt.reserve(32);
         DSS_HIP(hipMemsetAsync(P.exact_cnt.p, 0, 32, st));
-        hipLaunchKernelGGL(k_dist_bounds, dim3((n + 255) / 256), dim3(256), 0, st, P.exact.as<uint32_t>(),
-                           P.dist.as<uint32_t>(), s->s_len.as<uint32_t>(), n,
+        hipLaunchKernelGGL(k_dist_bounds, dim3((uint32_t)std::min<uint64_t>((n + 255) / 256, kSumBlocks)), dim3(256), 0,
+                           st, P.exact.as<uint32_t>(), P.dist.as<uint32_t>(), s->s_len.as<uint32_t>(), n,
                            P.exact_cnt.as<unsigned long long>());
         DSS_HIP(hipGetLastError());
         unsigned long long h[4];
@@ -1703,8 +1712,10 @@ extern "C" int dss_check(dss_set s, dss_check_t* out) {
                            P.m_lcp.as<uint32_t>(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                            P.recv_src_pe.as<uint32_t>(), P.recv_src_pos.as<uint32_t>(), s->pe_begin_d.as<uint64_t>(),
                            rb, s->p, s->arena(), n, glcp, seen, cnt);
-        hipLaunchKernelGGL(k_dsum, dim3((n + 255) / 256), dim3(256), 0, st, glcp, n, cnt + 3);
-        hipLaunchKernelGGL(k_popc, dim3((nw + 255) / 256), dim3(256), 0, st, seen, nw, cnt + 4);
+        hipLaunchKernelGGL(k_dsum, dim3((uint32_t)std::min<uint64_t>((n + 255) / 256, kSumBlocks)), dim3(256), 0, st,
+                           glcp, n, cnt + 3);
+        hipLaunchKernelGGL(k_popc, dim3((uint32_t)std::min<uint64_t>((nw + 255) / 256, kSumBlocks)), dim3(256), 0, st,
+                           seen, nw, cnt + 4);
         DSS_HIP(hipGetLastError());
         unsigned long long h[5];
         DSS_HIP(hipMemcpyAsync(h, cnt, 40, hipMemcpyDeviceToHost, st));
