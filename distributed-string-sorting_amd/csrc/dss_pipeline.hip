@@ -825,7 +825,9 @@ void exchange(dss_set_s* s, int mode) {
     uint32_t* r_stage = staged ? P.recv_stage.as<uint32_t>() : nullptr;
     const uint64_t* ls = P.layout.as<uint64_t>();
     if (n) {
-        KScope ks(s->timer, st, "exchange_index", n * 40.0);
+        // reads lcp, offset, length (+ dist in mode 2), writes size, lcp,
+        // view offset, length and start (+ the kept-row index)
+        KScope ks(s->timer, st, "exchange_index", n * ((mode == 2 ? 48.0 : 44.0) + (staged ? 8.0 : 0.0)));
         hipLaunchKernelGGL(k_exch_index, dim3((n + 255) / 256), dim3(256), 0, st, s->pe_begin_d.as<uint64_t>(), p,
                            s->lcp.as<uint32_t>(), s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(),
                            mode == 2 ? P.dist.as<uint32_t>() : nullptr, ls,
