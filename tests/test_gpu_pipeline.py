@@ -374,3 +374,29 @@ def test_prefix_doubling_exchange_from_kept_characters(p):
         assert list(zip(map(int, pi), map(int, si))) == exp  # D/N strings are distinct: no ties
         chk = ds.check()
         assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0
+
+
+def test_exchange_origin_follows_each_exchange():
+    """The received strings' origin (source PE, position) is computed on the
+    first request after an exchange: repeated exchanges of one set with other
+    send counts report the origins of the latest one (oracle: orc_exchange)."""
+    dss = _dss()
+    rng = np.random.default_rng(77)
+    p = 5
+    chars = O.gen_dton(6000, 40, 0.5, 78)
+    ss = O.StringSet(chars, p)
+    perm, lcp = O.local_sort(ss)
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        ds.set_local_order(perm, lcp)
+        for k in range(3):
+            iv = np.zeros((p, p), np.uint64)
+            for r in range(p):
+                cnt = int(ss.pe_begin[r + 1] - ss.pe_begin[r])
+                cuts = np.sort(rng.integers(0, cnt + 1, p - 1))
+                iv[r] = np.diff(np.concatenate([[0], cuts, [cnt]]))
+            ds.set_interval(iv)
+            ds.exchange("whole")
+            glcp, gpe, gpos = ds.get_exchange()
+            ex = O.exchange(ss, perm, lcp, iv, "whole")
+            assert (glcp == ex["lcp"]).all(), k
+            assert (gpe == ex["src_pe"]).all() and (gpos == ex["src_pos"]).all(), k
