@@ -574,8 +574,13 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restric
 // Stable scatter of one tile by its digit.  The tile is first put in digit
 // order in LDS (wave ballots rank equal digits, in index order) and then
 // written run by run, so consecutive lanes store consecutive addresses of a
-// digit's run instead of a few records of every run per store.
-__global__ void __launch_bounds__(kThreads)
+// digit's run instead of a few records of every run per store.  512 threads
+// x 4 records per tile: 37 KB of LDS per 8 waves, 8 waves per SIMD.
+constexpr uint32_t kScT = 512;           // scatter threads per tile
+constexpr uint32_t kScI = kTile / kScT;  // records per thread
+static_assert(kScI * kScT == kTile && kScT >= 256 && kScT % 64 == 0, "scatter tile shape");
+
+__global__ void __launch_bounds__(kScT)
 k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint32_t* __restrict__ hist,
           const uint64_t* __restrict__ key_in, const uint32_t* __restrict__ pb_in,
           uint64_t* __restrict__ key_out, uint32_t* __restrict__ pb_out, uint32_t* __restrict__ perm,
@@ -585,24 +590,27 @@ k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const ui
     __shared__ uint32_t gbase[256];   // the digit's output offset for this tile
     __shared__ uint32_t lstart[256];  // the digit's first slot in the reordered tile
     __shared__ uint32_t run[256];     // next free slot of the digit
-    __shared__ uint32_t wcnt[kThreads / 64][256];
-    __shared__ uint32_t scan[kThreads];
+    __shared__ uint32_t wcnt[kScT / 64][256];
+    __shared__ uint32_t scan[256];
     const Tile T = tiles[blockIdx.x];
     const Seg S = segs[T.seg];
     uint32_t kd, dep;
     seg_depths(S, kd, dep);
     const uint32_t sh = 56 - 8 * (dep - kd);
     const uint32_t tid = threadIdx.x, w = tid >> 6;
-    gbase[tid] = hist[(uint64_t)blockIdx.x * 256 + tid];
-    run[tid] = 0;
-    for (uint32_t k = 0; k < kThreads / 64; ++k) wcnt[k][tid] = 0;
+    const bool dig = tid < 256;  // the threads that own a digit
+    if (dig) {
+        gbase[tid] = hist[(uint64_t)blockIdx.x * 256 + tid];
+        run[tid] = 0;
+        for (uint32_t k = 0; k < kScT / 64; ++k) wcnt[k][tid] = 0;
+    }
     __syncthreads();
     // the tile's records and their digits
-    uint64_t kv[kItems];
-    uint32_t idv[kItems], dg[kItems];
+    uint64_t kv[kScI];
+    uint32_t idv[kScI], dg[kScI];
 #pragma unroll
-    for (uint32_t r = 0; r < kItems; ++r) {
-        const uint32_t e = T.begin + r * kThreads + tid;
+    for (uint32_t r = 0; r < kScI; ++r) {
+        const uint32_t e = T.begin + r * kScT + tid;
         kv[r] = 0;
         idv[r] = 0;
         dg[r] = 0;
@@ -615,23 +623,25 @@ k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const ui
     }
     __syncthreads();
     // exclusive scan of the digit counts (one digit per thread)
-    const uint32_t mine = run[tid];
-    scan[tid] = mine;
+    const uint32_t mine = dig ? run[tid] : 0;
+    if (dig) scan[tid] = mine;
     __syncthreads();
-    for (uint32_t o = 1; o < kThreads; o <<= 1) {
-        const uint32_t v = tid >= o ? scan[tid - o] : 0;
+    for (uint32_t o = 1; o < 256; o <<= 1) {
+        const uint32_t v = dig && tid >= o ? scan[tid - o] : 0;
         __syncthreads();
-        scan[tid] += v;
+        if (dig) scan[tid] += v;
         __syncthreads();
     }
-    lstart[tid] = scan[tid] - mine;
-    run[tid] = scan[tid] - mine;
+    if (dig) {
+        lstart[tid] = scan[tid] - mine;
+        run[tid] = scan[tid] - mine;
+    }
     __syncthreads();
     // stable placement in LDS, in index order (r, wave, lane)
     const uint64_t lt = lanemask_lt();
 #pragma unroll
-    for (uint32_t r = 0; r < kItems; ++r) {
-        const bool valid = T.begin + r * kThreads + tid < T.end;
+    for (uint32_t r = 0; r < kScI; ++r) {
+        const bool valid = T.begin + r * kScT + tid < T.end;
         uint64_t m = __ballot(valid);
         for (uint32_t b = 0; b < 8; ++b) {
             const bool bit = (dg[r] >> b) & 1;
@@ -648,17 +658,19 @@ k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const ui
             sid[q] = idv[r];
         }
         __syncthreads();
-        uint32_t add = 0;
-        for (uint32_t ww = 0; ww < kThreads / 64; ++ww) {
-            add += wcnt[ww][tid];
-            wcnt[ww][tid] = 0;
+        if (dig) {
+            uint32_t add = 0;
+            for (uint32_t ww = 0; ww < kScT / 64; ++ww) {
+                add += wcnt[ww][tid];
+                wcnt[ww][tid] = 0;
+            }
+            run[tid] += add;
         }
-        run[tid] += add;
         __syncthreads();
     }
     // runs out: slot k of digit d goes to gbase[d] + (k - lstart[d])
     const uint32_t cnt = T.end - T.begin;
-    for (uint32_t k = tid; k < cnt; k += kThreads) {
+    for (uint32_t k = tid; k < cnt; k += kScT) {
         const uint64_t x = skey[k];
         const uint32_t d = (uint32_t)(x >> sh) & 0xFF;
         const uint32_t q = gbase[d] + (k - lstart[d]);
@@ -1198,7 +1210,7 @@ void sort_lcp(dss_set_s* s) {
         }
         {
             KScope ks(s->timer, st, "scatter", (double)cnt_large * 28.0);
-            hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kThreads), 0, st, tl, cur, X.hist.as<uint32_t>(), kc,
+            hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kScT), 0, st, tl, cur, X.hist.as<uint32_t>(), kc,
                                pc_ids, kn, pn_ids, perm, lcp);
         }
         hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, nxt, pn);
