@@ -781,7 +781,7 @@ k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ has
     __shared__ uint64_t gbase[1u << kHpMaxBits];   // the digit's output offset for this tile
     __shared__ uint32_t lstart[1u << kHpMaxBits];  // the digit's first slot in the reordered tile
     __shared__ uint32_t run[1u << kHpMaxBits];     // next free slot of the digit
-    __shared__ uint32_t wcnt[kHpT / 64][1u << kHpMaxBits];
+    __shared__ uint8_t wcnt[kHpT / 64][1u << kHpMaxBits];  // a wave's count of a digit, at most 64
     __shared__ uint32_t scan[kHpT];
     const uint32_t nb = 1u << bits, tid = threadIdx.x, w = tid >> 6;
     const uint64_t t0 = (uint64_t)blockIdx.x * kHpTile;
@@ -850,7 +850,7 @@ k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ has
             m &= bit ? bb : ~bb;
         }
         const uint32_t rank = (uint32_t)__popcll(m & lt);
-        if (valid && rank == 0) wcnt[w][dg[r]] = (uint32_t)__popcll(m);
+        if (valid && rank == 0) wcnt[w][dg[r]] = (uint8_t)__popcll(m);
         __syncthreads();
         if (valid) {
             uint32_t q = run[dg[r]] + rank;
