@@ -17,8 +17,9 @@ The JSON line also carries
                 HIP-event-timed average duration, against MI355X's 8 TB/s;
   passes:       the same over all timed kernels of a step together, and the
                 whole step's (D + 16 n) bytes over its time;
-  cpu_baseline: the CPU oracle (a single-thread C restatement of the same
-                pipeline) timed on a bounded sample of the same workload.
+  cpu_baseline: the CPU restatement of the same pipeline, run PE-parallel
+                (one PE per thread, one thread per available core) on the
+                same input, timed on the host.
 Only one GPU is used; --gpus N > 1 exits with a message.
 """
 from __future__ import annotations
@@ -83,42 +84,50 @@ def measured_traffic(timer_name):
     return None
 
 
-def cpu_baseline(n, length, dton, pes, seed, m):
-    """Times the CPU oracle's prefix-doubling sort on a bounded sample."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, length, dton, pes, seed, m, sampler):
+    """Times the CPU restatement of the whole prefix-doubling sort, run
+    PE-parallel the way the reference's MPI ranks run it (one PE per thread,
+    one thread per available core, at most one per PE), on the same input."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(cores, pes))
+    t0 = time.perf_counter()
     chars = O.gen_dton(n, length, dton, seed)
     ss = O.StringSet(chars, pes)
+    gen_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    res = O.prefix_doubling_sort(ss, m=m)
+    res = O.prefix_doubling_sort_mt(ss, sampler=sampler, m=m, threads=threads)
     dt = time.perf_counter() - t0
-    # D of the sorted sample: merged outputs plus recomputed PE-boundary LCPs
-    lcps = []
-    prev = None
-    for q, o in enumerate(res["out"]):
-        lc = [int(x) for x in o["lcp"]]
-        if lc:
-            first = ss.string(res["perm"][int(ss.pe_begin[int(o["pe_index"][0])]) + int(o["string_index"][0])])
-            if prev is not None:
-                k = 0
-                while k < min(len(prev), len(first)) and prev[k] == first[k]:
-                    k += 1
-                lc[0] = k
-            last_pe, last_si = int(o["pe_index"][-1]), int(o["string_index"][-1])
-            prev = ss.string(res["perm"][int(ss.pe_begin[last_pe]) + last_si])
-        lcps += lc
-    import numpy as np
-    D = O.compute_d(np.array(lcps, dtype=np.uint32))
+    D = res["D"]
+    ph = res["phase_s"]
     return {
         "value": round(D / dt / 1e9, 5),
         "unit": "GB/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"D/N generator, {n} strings x {length} chars, D/N {dton}, {pes} PEs, "
-                  f"same seed; single-thread C oracle of the whole prefix-doubling sort "
-                  f"({dt:.2f} s, D = {D})",
+        "cpu_model": cpu_model(),
+        "cores_available": cores,
+        "sample": f"the full workload: D/N generator, {n} strings x {length} chars, D/N {dton}, {pes} PEs, "
+                  f"same seed; the C restatement of the whole prefix-doubling sort (local sort, "
+                  f"prefix doubling, partition, exchange, merge, permutation) PE-parallel on {threads} "
+                  f"threads (one PE per thread, as one MPI rank per core), {dt:.2f} s, D = {D}",
         "strings_per_s": round(n / dt, 1),
+        "phase_s": {k: round(v, 3) for k, v in zip(
+            ("local_sort", "prefix_doubling", "partition", "exchange", "merge", "permutation"), ph)},
+        "input_s": round(gen_s, 2),
     }
 
 
@@ -414,8 +423,8 @@ def main():
     ap.add_argument("--bloom-bits", type=int, default=0,
                     help="hash range = bits * n (the 'Bloom filter sized b bits/string' variant); "
                          "0 = 2^64-1 as in the reference")
-    ap.add_argument("--cpu-sample", type=int, default=10_000_000,
-                    help="strings in the CPU-oracle sample (0 disables the CPU leg)")
+    ap.add_argument("--cpu-sample", type=int, default=20_000_000,
+                    help="strings of the CPU baseline's input (default: the whole workload; 0 disables it)")
     ap.add_argument("--serve-jobs", type=int, default=2000,
                     help="jobs of the batched/served leg (0 disables it)")
     ap.add_argument("--serve-n-min", type=int, default=1000)
@@ -516,9 +525,9 @@ def main():
 
     cpu = None
     if args.cpu_sample > 0:
-        log(f"CPU oracle on {args.cpu_sample} strings ...")
-        cpu = cpu_baseline(args.cpu_sample, args.len, args.dton, args.pes, args.seed, m)
-        cpu["cores"] = 1
+        log(f"CPU restatement on {args.cpu_sample} strings ...")
+        cpu = cpu_baseline(args.cpu_sample, args.len, args.dton, args.pes, args.seed, m, args.sampler)
+        log(f"CPU: {cpu['value']} GB/s on {cpu['cores']} threads, phases {cpu['phase_s']}")
 
     out = {
         "metric": "distinguishing-prefix GB/s at 1 GPU (batched vs plain); p50/p99 job latency",

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -229,6 +230,9 @@ struct PipeState {
     DevBuf layout;                                 // send_start / run offsets (u64)
     // merge (per receive-side output position)
     bool merged = false;
+    bool merged_by_caller = false;  // merged output from dss_set_merged (no saved LCPs)
+    bool perm_valid = false;        // perm_out holds the merged output's permutation
+    DevBuf perm_out;                // u32 stringIndex[n], then u32 PEIndex[n] from perm_pe_off(n)
     DevBuf m_order, m_lcp, m_saved, m_order2, m_lcp2, m_keyc, m_keyc2, m_cp, m_work;
     // prefix decompression
     bool extended = false;
@@ -304,6 +308,9 @@ uint64_t avg_lcp(dss_set_s* s);
 void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, bool use_dist);
 void exchange(dss_set_s* s, int mode);
 void recv_origin(dss_set_s* s);  // fills recv_src_pe / recv_src_pos once per exchange
+void permutation(dss_set_s* s);  // fills perm_out once per merged output
+// Element offset of the PEIndex array in perm_out (16-byte aligned).
+inline uint64_t perm_pe_off(uint64_t n) { return (std::max<uint64_t>(n, 1) + 3) / 4 * 4; }
 void merge(dss_set_s* s);
 void extend_prefix(dss_set_s* s);
 void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth);

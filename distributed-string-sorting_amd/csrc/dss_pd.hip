@@ -1258,17 +1258,6 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
 }
 
 // ------------------------------------------------------------------ drivers
-__global__ void k_permutation(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ rpos,
-                              const uint32_t* __restrict__ rpe, uint64_t n, uint32_t* __restrict__ si,
-                              uint32_t* __restrict__ pi) {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) {
-        const uint32_t o = ord[k];
-        si[k] = rpos[o];
-        pi[k] = rpe[o];
-    }
-}
-
 // Hash of the first min(depth, len) characters of the given strings.
 __global__ void k_hash_ids(const uint32_t* __restrict__ ids, uint64_t count, const uint64_t* __restrict__ off,
                            const uint32_t* __restrict__ len, const uint8_t* __restrict__ chars, uint64_t n,
@@ -1353,6 +1342,7 @@ int dss_prefix_doubling_sort(dss_set s, int sampler, uint64_t m) {
         partition(s, sampler, 2, 0, true);
         exchange(s, 2);
         merge(s);
+        permutation(s);  // the writeback of prefix_doubling.hpp:282-290, on the device
     });
 }
 
@@ -1363,14 +1353,10 @@ int dss_get_permutation(dss_set s, uint32_t* string_index, uint32_t* pe_index) {
         DSS_REQUIRE(P.merged, DSS_ESTATE, "no merged output");
         const uint64_t n = s->n;
         if (!n) return;
-        recv_origin(s);
-        DevBuf tmp;
-        tmp.reserve(n * 8);
-        uint32_t* si = tmp.as<uint32_t>();
-        hipLaunchKernelGGL(k_permutation, dim3((n + 255) / 256), dim3(256), 0, s->stream, P.m_order.as<uint32_t>(),
-                           P.recv_src_pos.as<uint32_t>(), P.recv_src_pe.as<uint32_t>(), n, si, si + n);
+        permutation(s);  // already written by dss_prefix_doubling_sort
+        const uint32_t* si = P.perm_out.as<uint32_t>();
         DSS_HIP(hipMemcpyAsync(string_index, si, n * 4, hipMemcpyDeviceToHost, s->stream));
-        DSS_HIP(hipMemcpyAsync(pe_index, si + n, n * 4, hipMemcpyDeviceToHost, s->stream));
+        DSS_HIP(hipMemcpyAsync(pe_index, si + perm_pe_off(n), n * 4, hipMemcpyDeviceToHost, s->stream));
         DSS_HIP(hipStreamSynchronize(s->stream));
     });
 }

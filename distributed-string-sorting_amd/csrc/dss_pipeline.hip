@@ -215,7 +215,12 @@ __global__ void k_sample_compact(const Sample* __restrict__ slots, const uint32_
 
 // Rank of each sample in the global sample order (the result of the sample
 // sort RQuick::sort, misc.hpp:503-514): by string, then (indexed samplers,
-// IndexStringComparator misc.hpp:35-53) by global index, else by sample id.
+// IndexStringComparator misc.hpp:35-53) by global index, then by sample id.
+// The last key makes the ranks a permutation: a NumChars walk over a PE with
+// fewer characters than splitters emits its first string several times
+// (identical samples, same string and index), and tied ranks would leave
+// slots of `sorted` unwritten.  Identical samples are interchangeable, so
+// the order among them changes no splitter.
 // One block per sample; the launch may cover more blocks than the *ns_p
 // samples the compaction produced (they return at once).
 __global__ void __launch_bounds__(256) k_sample_rank(const Sample* __restrict__ smp, const uint32_t* __restrict__ ns_p,
@@ -232,7 +237,7 @@ __global__ void __launch_bounds__(256) k_sample_rank(const Sample* __restrict__ 
         const Sample b = smp[j];
         const int r = view_cmp32(chars, b.off, b.tlen, a.off, a.tlen, nullptr);
         if (r < 0) ++c;
-        else if (r == 0 && (indexed ? (b.gidx < a.gidx) : (j < i))) ++c;
+        else if (r == 0 && (indexed ? (b.gidx < a.gidx || (b.gidx == a.gidx && j < i)) : (j < i))) ++c;
     }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
     if (lane_id() == 0) red[threadIdx.x >> 6] = c;
@@ -476,10 +481,9 @@ void recv_origin(dss_set_s* s) {
     PipeState& P = s->pp;
     DSS_REQUIRE(P.exchanged, DSS_ESTATE, "dss_exchange has not run");
     if (P.origin_valid) return;
-    const uint64_t n = s->n, nn = std::max<uint64_t>(n, 1);
+    const uint64_t n = s->n;
     const uint32_t p = s->p;
-    P.recv_src_pe.reserve(nn * 4);
-    P.recv_src_pos.reserve(nn * 4);
+    // both arrays were reserved by the exchange: no allocation here
     if (n) {
         const uint64_t* ls = P.layout.as<uint64_t>();
         hipLaunchKernelGGL(k_exch_origin, dim3((n + 255) / 256), dim3(256), 0, s->stream,
@@ -777,6 +781,12 @@ __global__ void k_byte_begin(const uint64_t* __restrict__ recv_begin, const uint
 
 // p + 1 byte begins of the receivers, after the layout's 2 p^2 values and p + 1
 // spare slots.
+// Receivers' first receive positions (p+1) on the device, in the merge's
+// common-prefix buffer after the p+1 common prefixes.
+static inline uint64_t* recv_begin_dev(PipeState& P, uint32_t p) {
+    return reinterpret_cast<uint64_t*>(P.m_cp.as<uint8_t>() + ((uint64_t)(p + 1) * 4 + 7) / 8 * 8);
+}
+
 static inline uint64_t* byte_begin_dev(PipeState& P, uint32_t p) {
     return P.layout.as<uint64_t>() + (uint64_t)2 * p * p + (p + 1);
 }
@@ -820,6 +830,10 @@ void exchange(dss_set_s* s, int mode) {
     P.recv_view_off.reserve(nn * 8);
     P.recv_view_len.reserve(nn * 4);
     P.recv_from.reserve(nn * 4);
+    // the origin arrays are filled only on request (recv_origin), but their
+    // memory is reserved here so that a later query allocates nothing
+    P.recv_src_pe.reserve(nn * 4);
+    P.recv_src_pos.reserve(nn * 4);
     const bool staged = mode == 2 && P.stage_valid;
     if (staged) P.recv_stage.reserve(nn * 4);
     uint32_t* r_stage = staged ? P.recv_stage.as<uint32_t>() : nullptr;
@@ -841,7 +855,7 @@ void exchange(dss_set_s* s, int mode) {
     const uint64_t cp_bytes = ((uint64_t)(p + 1) * 4 + 7) / 8 * 8;
     P.m_cp.reserve(cp_bytes + (p + 1) * 8);
     uint32_t* cpd = P.m_cp.as<uint32_t>();
-    uint64_t* rb_d = reinterpret_cast<uint64_t*>(P.m_cp.as<uint8_t>() + cp_bytes);
+    uint64_t* rb_d = recv_begin_dev(P, p);
     DSS_HIP(hipMemcpyAsync(rb_d, P.recv_begin.data(), (p + 1) * 8, hipMemcpyHostToDevice, st));
     fill_small(st, cpd, 0xFF, (p + 1) * 4);
     if (n) {
@@ -1240,7 +1254,90 @@ void merge(dss_set_s* s) {
     }
     DSS_HIP(hipGetLastError());
     P.merged = true;
+    P.merged_by_caller = false;
+    P.perm_valid = false;
     P.extended = false;
+}
+
+// StringIndexPEIndex of every merged output position: the permutation
+// writeback of DistributedPrefixDoublingSort::sort (prefix_doubling.hpp:
+// 282-290).  The merged order holds receive indices; receive index a lies in
+// receiver q's range and in its run from source PE r (the last run starting
+// at or before a: empty runs share their successor's start), so its origin
+// is PE r, sorted position send_start[r][q] + (a - run_off[q][r]).  Up to
+// kXiP PEs the tables are staged in LDS.
+__global__ void k_permutation_out(const uint32_t* __restrict__ ord, uint64_t n, uint32_t p,
+                                  const uint64_t* __restrict__ send_start, const uint64_t* __restrict__ run_off,
+                                  const uint64_t* __restrict__ recv_begin, uint32_t* __restrict__ si,
+                                  uint32_t* __restrict__ pi) {
+    __shared__ uint64_t s_rb[kXiP + 1], s_ss[kXiP * kXiP], s_ro[kXiP * kXiP];
+    const bool staged = p <= kXiP;  // block-uniform
+    if (staged) {
+        for (uint32_t t = threadIdx.x; t <= p; t += blockDim.x) s_rb[t] = recv_begin[t];
+        for (uint32_t t = threadIdx.x; t < p * p; t += blockDim.x) {
+            s_ss[t] = send_start[t];
+            s_ro[t] = run_off[t];
+        }
+        __syncthreads();
+    }
+    const uint64_t* rb = staged ? s_rb : recv_begin;
+    const uint64_t* ss = staged ? s_ss : send_start;
+    const uint64_t* ro = staged ? s_ro : run_off;
+    const uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    uint32_t a4[4] = {0, 0, 0, 0};
+    if (base + 4 <= n) {
+        const uint4 v = *reinterpret_cast<const uint4*>(ord + base);
+        a4[0] = v.x; a4[1] = v.y; a4[2] = v.z; a4[3] = v.w;
+    } else {
+        for (uint32_t t = 0; t < 4 && base + t < n; ++t) a4[t] = ord[base + t];
+    }
+    uint32_t o_si[4], o_pe[4];
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t) {
+        const uint64_t a = a4[t];
+        uint32_t lo = 0, hi = p;  // receiver: last q with rb[q] <= a
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (rb[mid] <= a) lo = mid; else hi = mid;
+        }
+        const uint32_t q = lo;
+        const uint64_t* row = ro + (uint64_t)q * p;
+        lo = 0;
+        hi = p;  // run: last r with row[r] <= a
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (row[mid] <= a) lo = mid; else hi = mid;
+        }
+        o_pe[t] = lo;
+        o_si[t] = (uint32_t)(ss[(uint64_t)lo * p + q] + (a - row[lo]));
+    }
+    if (base + 4 <= n) {
+        *reinterpret_cast<uint4*>(si + base) = make_uint4(o_si[0], o_si[1], o_si[2], o_si[3]);
+        *reinterpret_cast<uint4*>(pi + base) = make_uint4(o_pe[0], o_pe[1], o_pe[2], o_pe[3]);
+    } else {
+        for (uint32_t t = 0; t < 4 && base + t < n; ++t) {
+            si[base + t] = o_si[t];
+            pi[base + t] = o_pe[t];
+        }
+    }
+}
+
+void permutation(dss_set_s* s) {
+    PipeState& P = s->pp;
+    DSS_REQUIRE(P.merged, DSS_ESTATE, "no merged output");
+    if (P.perm_valid) return;
+    const uint64_t n = s->n, n4 = perm_pe_off(n);
+    const uint32_t p = s->p;
+    P.perm_out.reserve(n4 * 8);
+    if (n) {
+        const uint64_t* ls = P.layout.as<uint64_t>();
+        KScope ks(s->timer, s->stream, "permutation", n * 12.0);
+        hipLaunchKernelGGL(k_permutation_out, dim3((uint32_t)((n + 1023) / 1024)), dim3(256), 0, s->stream,
+                           P.m_order.as<uint32_t>(), n, p, ls, ls + (uint64_t)p * p, recv_begin_dev(P, p),
+                           P.perm_out.as<uint32_t>(), P.perm_out.as<uint32_t>() + n4);
+        DSS_HIP(hipGetLastError());
+    }
+    P.perm_valid = true;
 }
 
 // ================================================================== (a9)
@@ -1360,6 +1457,7 @@ __global__ void k_extend(const uint32_t* __restrict__ ord, const uint32_t* __res
 void extend_prefix(dss_set_s* s) {
     PipeState& P = s->pp;
     DSS_REQUIRE(P.merged, DSS_ESTATE, "dss_merge has not run");
+    DSS_REQUIRE(!P.merged_by_caller, DSS_ESTATE, "a merged output set by dss_set_merged has no saved LCPs");
     DSS_REQUIRE(P.mode == 1, DSS_ESTATE, "prefix decompression needs the LCP-compressed exchange (mode 1)");
     const hipStream_t st = s->stream;
     const uint64_t n = s->n;
@@ -1748,6 +1846,33 @@ extern "C" int dss_set_interval(dss_set s, const uint64_t* interval) {
         DSS_HIP(hipStreamSynchronize(s->stream));
         P.partitioned = true;
         P.exchanged = P.merged = P.extended = false;
+    });
+}
+
+extern "C" int dss_set_merged(dss_set s, const uint32_t* order, const uint32_t* lcp) {
+    return guarded([&] {
+        DSS_REQUIRE(s && (s->n == 0 || (order && lcp)), DSS_EINVAL, "NULL argument");
+        PipeState& P = s->pp;
+        DSS_REQUIRE(P.exchanged, DSS_ESTATE, "dss_exchange has not run");
+        const uint64_t n = s->n, nn = std::max<uint64_t>(n, 1);
+        // every entry must name a received string (the checker reads its
+        // view and origin); repeats are allowed and reported as incomplete
+        for (uint64_t i = 0; i < n; ++i)
+            DSS_REQUIRE(order[i] < n, DSS_EINVAL, "merged order entry beyond the received strings");
+        P.m_order.reserve(nn * 4);
+        P.m_lcp.reserve(nn * 4);
+        P.m_saved.reserve(nn * 4);
+        const hipStream_t st = s->stream;
+        if (n) {
+            DSS_HIP(hipMemcpyAsync(P.m_order.p, order, n * 4, hipMemcpyHostToDevice, st));
+            DSS_HIP(hipMemcpyAsync(P.m_lcp.p, lcp, n * 4, hipMemcpyHostToDevice, st));
+            fill_small(st, P.m_saved.p, 0, n * 4);
+        }
+        DSS_HIP(hipStreamSynchronize(st));
+        P.merged = true;
+        P.merged_by_caller = true;
+        P.perm_valid = false;
+        P.extended = false;
     });
 }
 

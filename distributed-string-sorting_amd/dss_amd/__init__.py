@@ -251,6 +251,13 @@ class DeviceStringSet:
         check(lib().dss_get_merged(self._h, _p(o, _lib.u32p), _p(l, _lib.u32p), _p(sv, _lib.u32p)))
         return o[:n], l[:n], sv[:n]
 
+    def set_merged(self, order, lcp):
+        """A caller-built merged output (the checker's input as the
+        reference's checker tests build it); entries are receive indices."""
+        order = np.ascontiguousarray(order, dtype=np.uint32)
+        lcp = np.ascontiguousarray(lcp, dtype=np.uint32)
+        check(lib().dss_set_merged(self._h, _p(order, _lib.u32p), _p(lcp, _lib.u32p)))
+
     def extend_prefix(self) -> int:
         nb = C.c_uint64()
         check(lib().dss_extend_prefix(self._h, C.byref(nb)))

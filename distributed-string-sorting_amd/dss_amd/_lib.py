@@ -88,6 +88,7 @@ _SIGS = {
     "dss_get_exchange": (C.c_int, [set_p, u8p, u32p, u32p, u32p]),
     "dss_merge_lcp": (C.c_int, [set_p]),
     "dss_get_merged": (C.c_int, [set_p, u32p, u32p, u32p]),
+    "dss_set_merged": (C.c_int, [set_p, u32p, u32p]),
     "dss_extend_prefix": (C.c_int, [set_p, u64p]),
     "dss_get_extended": (C.c_int, [set_p, u8p, u64p]),
     "dss_prefix_doubling": (C.c_int, [set_p, C.c_uint64, C.c_uint32, u32p]),

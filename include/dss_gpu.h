@@ -210,6 +210,16 @@ int dss_merge_lcp(dss_set s);
  * the merged LCP and the saved LCP (oldLcps); any pointer may be NULL. */
 int dss_get_merged(dss_set s, uint32_t* order, uint32_t* lcp, uint32_t* saved);
 
+/* Sets a caller-built merged output instead of dss_merge_lcp: the input of
+ * the reference's checker (is_sorted / is_complete_and_sorted,
+ * src/mpi/is_sorted.hpp:27-198) as its tests build it
+ * (src/tests/isSorted_tests.cpp:12-143).  Per receive-side output position
+ * the received string (receive index < num_strings; a repeated index is
+ * allowed and dss_check reports the output incomplete) and the merged LCP.
+ * dss_check, dss_get_permutation and the distinguishing-prefix validators
+ * then read this output; dss_extend_prefix refuses it (no saved LCPs). */
+int dss_set_merged(dss_set s, const uint32_t* order, const uint32_t* lcp);
+
 /* ---- (a9) StringLcpContainer::extendPrefix
  *      (src/strings/stringcontainer.hpp:247-319), called at
  *      src/executables/distributed_sorter.cpp:127-131 after a mode-1

@@ -107,6 +107,7 @@ ORC_API void orc_gen_dton(uint64_t n, uint32_t L, double dton, uint64_t seed,
     uint32_t k, slen;
     orc_dton_params(n, L, dton, &k, &slen);
     const uint8_t fill = orc_dton_fill(seed);
+#pragma omp parallel for schedule(static)
     for (uint64_t j = 0; j < n; ++j) {
         uint64_t i = orc_shuffle_index(j, n, seed);
         dton_string(i, k, slen, fill, out + j * (uint64_t)(slen + 1));
@@ -246,7 +247,11 @@ ORC_API uint64_t orc_suffix_arena(const uint8_t* text, uint64_t text_len,
 
 ORC_API uint64_t orc_count_strings(const uint8_t* chars, uint64_t nbytes) {
     uint64_t n = 0;
-    for (uint64_t i = 0; i < nbytes; ++i) n += chars[i] == 0;
+    const uint8_t *p = chars, *e = chars + nbytes;
+    while (p < e && (p = (const uint8_t*)memchr(p, 0, (size_t)(e - p))) != NULL) {
+        ++n;
+        ++p;
+    }
     return n;
 }
 
@@ -255,13 +260,15 @@ ORC_API uint64_t orc_count_strings(const uint8_t* chars, uint64_t nbytes) {
 ORC_API void orc_index(const uint8_t* chars, uint64_t nbytes, uint64_t* off,
                        uint32_t* len) {
     uint64_t s = 0, k = 0;
-    for (uint64_t i = 0; i < nbytes; ++i) {
-        if (chars[i] == 0) {
-            off[k] = s;
-            len[k] = (uint32_t)(i - s);
-            ++k;
-            s = i + 1;
-        }
+    const uint8_t* e = chars + nbytes;
+    while (s < nbytes) {
+        const uint8_t* z = (const uint8_t*)memchr(chars + s, 0, (size_t)(e - (chars + s)));
+        if (!z) break;
+        const uint64_t i = (uint64_t)(z - chars);
+        off[k] = s;
+        len[k] = (uint32_t)(i - s);
+        ++k;
+        s = i + 1;
     }
 }
 
@@ -484,15 +491,21 @@ ORC_API uint64_t orc_avg_lcp(const uint32_t* lcp, uint64_t n) {
 /* max_len: truncation length (merge sort: 2*100*(avgLcp+5)); dist: per-      */
 /* position distinguishing prefix (prefix doubling) or NULL.                   */
 /* Output: interval[r*p + q] = strings PE r sends to PE q.                     */
-/* Pinned by: an independent Python restatement (sorted() sample order, the  */
-/* reference's halving binarySearch, the NumChars walk) on inputs with and    */
-/* without duplicates, truncated and prefix-doubling splitters, for all four  */
-/* samplers (tests/test_oracle.py).                                           */
+/* Pinned by: counting (tests/test_oracle.py): with the indexed samplers the */
+/* boundary before bucket q+1 equals the number of local (string, global      */
+/* index) keys below splitter q's key; without an index it lies between the   */
+/* counts of smaller and of not-greater strings; the splitters are the        */
+/* samples at ranks i*floor(S/(nspl+1)) of the textbook-sorted sample list;   */
+/* NumStrings samples sit at floor(i*n/(nr+1)), NumChars samples where the    */
+/* character count since the previous sample first reaches num/(nr+1); the    */
+/* buckets of all PEs respect the global order.  Also an independent Python  */
+/* restatement of the rules, for all four samplers.                           */
 /* ------------------------------------------------------------------------- */
 
 typedef struct {
     const uint8_t* s; /* chars (NUL-terminated copy) */
     uint64_t index;   /* global index (indexed samplers) */
+    uint32_t tlen;    /* truncated length */
 } sample_t;
 
 static int g_cmp_indexed;
@@ -512,14 +525,23 @@ static int index_cmp(const uint8_t* a, uint64_t ia, const uint8_t* b, uint64_t i
     return ia < ib ? -1 : (ia > ib ? 1 : 0);
 }
 
-ORC_API int orc_partition(const uint8_t* chars, const uint64_t* off,
-                          const uint32_t* len, const uint64_t* pe_begin, uint32_t p,
-                          const uint32_t* perm, int sampler, uint64_t sampling_factor,
-                          uint64_t max_len, const uint32_t* dist, uint64_t* interval) {
+/* orc_partition_ex additionally reports, when the pointers are non-NULL,
+ * every sample in emission order (PE by PE) as (global index, truncated
+ * length) -- capacity p * sampling_factor * (p - 1) -- and the p - 1
+ * splitters the same way (UINT64_MAX index for a missing splitter), so that
+ * tests can check the bucket bounds against the splitters by counting. */
+ORC_API int orc_partition_ex(const uint8_t* chars, const uint64_t* off,
+                             const uint32_t* len, const uint64_t* pe_begin, uint32_t p,
+                             const uint32_t* perm, int sampler, uint64_t sampling_factor,
+                             uint64_t max_len, const uint32_t* dist, uint64_t* interval,
+                             uint64_t* out_sample_gidx, uint32_t* out_sample_tlen,
+                             uint64_t* out_num_samples, uint64_t* out_spl_gidx,
+                             uint32_t* out_spl_tlen) {
     const int indexed = sampler >= 2;
     const int by_chars = sampler == 1 || sampler == 3;
+    if (p == 0 || sampling_factor * (p - 1) > 8192) return -1;
     uint64_t nsamp_cap = 0;
-    for (uint32_t r = 0; r < p; ++r) nsamp_cap += sampling_factor * (p ? p - 1 : 0) + 1;
+    for (uint32_t r = 0; r < p; ++r) nsamp_cap += sampling_factor * (p - 1) + 1;
     sample_t* samp = (sample_t*)calloc(nsamp_cap + 1, sizeof(sample_t));
     uint8_t** bufs = (uint8_t**)calloc(nsamp_cap + 1, sizeof(uint8_t*));
     uint64_t ns = 0;
@@ -530,7 +552,6 @@ ORC_API int orc_partition(const uint8_t* chars, const uint64_t* off,
         uint64_t nr = sampling_factor * (p - 1);
         if (nr > m) nr = m;
         uint64_t idx_list[8192];
-        if (nr > 8192) return -1;
         uint64_t cnt = 0;
         if (!by_chars) {
             const double sd = (double)m / (double)(nr + 1);
@@ -567,14 +588,23 @@ ORC_API int orc_partition(const uint8_t* chars, const uint64_t* off,
             bufs[ns] = copy;
             samp[ns].s = copy;
             samp[ns].index = local_offset + idx;
+            samp[ns].tlen = (uint32_t)used;
+            if (out_sample_gidx) out_sample_gidx[ns] = local_offset + idx;
+            if (out_sample_tlen) out_sample_tlen[ns] = (uint32_t)used;
             ++ns;
         }
         local_offset += m;
     }
+    if (out_num_samples) *out_num_samples = ns;
     g_cmp_indexed = indexed;
     qsort(samp, ns, sizeof(sample_t), sample_cmp);
     uint64_t nspl = (p - 1) < ns ? (p - 1) : ns;
     uint64_t sdist = ns / (nspl + 1);
+    for (uint64_t q = 0; q + 1 < p; ++q) {
+        const int have = q < nspl;
+        if (out_spl_gidx) out_spl_gidx[q] = have ? samp[(q + 1) * sdist].index : UINT64_MAX;
+        if (out_spl_tlen) out_spl_tlen[q] = have ? samp[(q + 1) * sdist].tlen : 0;
+    }
     local_offset = 0;
     for (uint32_t r = 0; r < p; ++r) {
         const uint64_t b = pe_begin[r], m = pe_begin[r + 1] - b;
@@ -610,6 +640,14 @@ ORC_API int orc_partition(const uint8_t* chars, const uint64_t* off,
     return 0;
 }
 
+ORC_API int orc_partition(const uint8_t* chars, const uint64_t* off,
+                          const uint32_t* len, const uint64_t* pe_begin, uint32_t p,
+                          const uint32_t* perm, int sampler, uint64_t sampling_factor,
+                          uint64_t max_len, const uint32_t* dist, uint64_t* interval) {
+    return orc_partition_ex(chars, off, len, pe_begin, p, perm, sampler, sampling_factor, max_len, dist,
+                            interval, NULL, NULL, NULL, NULL, NULL);
+}
+
 /* ------------------------------------------------------------------------- */
 /* (a6, a7, a13) String exchange between virtual PEs.                          */
 /* mode 0: EmptyByteEncoderMemCpy (byte_encoder.hpp:81-107, whole strings);    */
@@ -627,6 +665,44 @@ ORC_API int orc_partition(const uint8_t* chars, const uint64_t* off,
 /* Pinned by: alltoallStrings_tests.cpp:17-232 known answers                  */
 /* (tests/test_oracle.py::test_exchange_reference_vectors).                   */
 /* ------------------------------------------------------------------------- */
+/* Receiver q's part of the exchange: its strings from every source PE in
+ * source order, written from byte `total` and receive index `k` on (outputs
+ * only when out_bytes != NULL); returns q's byte count. */
+static uint64_t exch_receiver(int mode, uint32_t q, const uint8_t* chars, const uint64_t* off,
+                              const uint32_t* len, const uint64_t* pe_begin, uint32_t p,
+                              const uint32_t* perm, const uint32_t* lcp, const uint32_t* dist,
+                              const uint64_t* interval, uint64_t total, uint64_t k,
+                              uint8_t* out_bytes, uint32_t* out_lcp, uint32_t* out_src_pe,
+                              uint32_t* out_src_pos) {
+    const uint64_t total0 = total;
+    for (uint32_t r = 0; r < p; ++r) {
+        uint64_t start = 0;
+        for (uint32_t t = 0; t < q; ++t) start += interval[(uint64_t)r * p + t];
+        uint64_t cnt = interval[(uint64_t)r * p + q];
+        for (uint64_t j = 0; j < cnt; ++j) {
+            uint64_t pos = pe_begin[r] + start + j;
+            uint32_t id = perm[pos];
+            uint32_t l = j == 0 ? 0 : lcp[pos];
+            uint64_t from, to; /* chars copied, then maybe NUL */
+            int add_nul = 0;
+            if (mode == 0) { from = 0; to = (uint64_t)len[id] + 1; }
+            else if (mode == 1) { from = l; to = (uint64_t)len[id] + 1; }
+            else { from = l; to = dist[pos]; add_nul = 1; }
+            uint64_t m = to - from;
+            if (out_bytes) {
+                memcpy(out_bytes + total, chars + off[id] + from, m);
+                if (add_nul) out_bytes[total + m] = 0;
+                out_lcp[k] = l;
+                out_src_pe[k] = r;
+                out_src_pos[k] = (uint32_t)(start + j);
+            }
+            total += m + add_nul;
+            ++k;
+        }
+    }
+    return total - total0;
+}
+
 ORC_API uint64_t orc_exchange(int mode, const uint8_t* chars, const uint64_t* off,
                               const uint32_t* len, const uint64_t* pe_begin, uint32_t p,
                               const uint32_t* perm, const uint32_t* lcp,
@@ -640,31 +716,9 @@ ORC_API uint64_t orc_exchange(int mode, const uint8_t* chars, const uint64_t* of
             out_recv_begin[q] = k;
             out_byte_begin[q] = total;
         }
-        for (uint32_t r = 0; r < p; ++r) {
-            uint64_t start = 0;
-            for (uint32_t t = 0; t < q; ++t) start += interval[(uint64_t)r * p + t];
-            uint64_t cnt = interval[(uint64_t)r * p + q];
-            for (uint64_t j = 0; j < cnt; ++j) {
-                uint64_t pos = pe_begin[r] + start + j;
-                uint32_t id = perm[pos];
-                uint32_t l = j == 0 ? 0 : lcp[pos];
-                uint64_t from, to; /* chars copied, then maybe NUL */
-                int add_nul = 0;
-                if (mode == 0) { from = 0; to = (uint64_t)len[id] + 1; }
-                else if (mode == 1) { from = l; to = (uint64_t)len[id] + 1; }
-                else { from = l; to = dist[pos]; add_nul = 1; }
-                uint64_t m = to - from;
-                if (out_bytes) {
-                    memcpy(out_bytes + total, chars + off[id] + from, m);
-                    if (add_nul) out_bytes[total + m] = 0;
-                    out_lcp[k] = l;
-                    out_src_pe[k] = r;
-                    out_src_pos[k] = (uint32_t)(start + j);
-                }
-                total += m + add_nul;
-                ++k;
-            }
-        }
+        total += exch_receiver(mode, q, chars, off, len, pe_begin, p, perm, lcp, dist, interval, total, k,
+                               out_bytes, out_lcp, out_src_pe, out_src_pos);
+        for (uint32_t r = 0; r < p; ++r) k += interval[(uint64_t)r * p + q];
     }
     if (out_bytes) {
         out_recv_begin[p] = k;
@@ -1257,4 +1311,324 @@ ORC_API int64_t orc_golomb_decode(const uint64_t* words, uint64_t nwords, uint64
     }
 #undef ORC_BIT
     return (int64_t)n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* The whole DistributedPrefixDoublingSort::sort (prefix_doubling.hpp:204-294) */
+/* with p PEs run PE-parallel on CPU threads, as p MPI ranks run it: the      */
+/* local sort, the candidate classification and hashing of every round, the  */
+/* local duplicate search, the exchange and the merge are per-PE work; the   */
+/* global duplicate search of a round is split over hash ranges (the         */
+/* reference's FindDuplicates owners, bloomfilter.hpp:662-765).  The CPU     */
+/* baseline of the benchmark; the same steps as orc_local_sort,              */
+/* orc_prefix_doubling, orc_partition, orc_exchange (mode 2) and             */
+/* orc_merge_losertree, and bit-exact with them (tests/test_oracle.py).      */
+/* Outputs per merged output position (receivers in order): stringIndex,     */
+/* PEIndex and the merged LCP (the PE-boundary LCPs are 0); results per local */
+/* sorted position; phase_s (6 entries, may be NULL): wall seconds of local  */
+/* sort, prefix doubling, partition, exchange, merge and permutation.        */
+/* Returns the number of prefix-doubling rounds.                              */
+/* ------------------------------------------------------------------------- */
+#include <omp.h>
+#include <time.h>
+
+static double wall_s(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+static uint32_t pd_mt(const uint8_t* chars, const uint64_t* off, const uint32_t* len,
+                      const uint64_t* pe_begin, uint32_t p, const uint32_t* perm,
+                      const uint32_t* lcp, uint64_t m, uint32_t start_depth, uint32_t* results,
+                      uint8_t* rounds) {
+    const uint64_t n = pe_begin[p];
+    const uint64_t nn = n ? n : 1;
+    /* per-candidate buffers, grown when the candidate multiset (which may
+     * hold a position twice) outgrows them */
+    uint64_t cap = nn, ccap = nn;
+    uint64_t* cand = (uint64_t*)malloc(ccap * sizeof(uint64_t));
+    uint64_t ncand = n;
+    pd_rec* rec = (pd_rec*)malloc(cap * sizeof(pd_rec));
+    uint8_t* rflag = (uint8_t*)malloc(cap);
+    uint64_t* next = (uint64_t*)malloc((cap * 3 + 2 * (uint64_t)p) * sizeof(uint64_t));
+    sent_t* sent = (sent_t*)malloc(cap * sizeof(sent_t));
+    uint64_t* cb = (uint64_t*)calloc(p + 1, 8);     /* candidate range per PE */
+    uint64_t* nrec = (uint64_t*)calloc(p, 8);
+    uint64_t* nsent = (uint64_t*)calloc(p, 8);
+    uint64_t* nnext = (uint64_t*)calloc(p, 8);
+    uint64_t* nb = (uint64_t*)calloc(p + 1, 8);
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; ++i) {
+        cand[i] = i;
+        results[i] = 0;
+        rounds[i] = 0;
+    }
+    uint32_t round = 0;
+    uint64_t d = start_depth, step = (uint64_t)start_depth * 2;
+    while (ncand > 0) {
+        if (ncand > cap) {
+            cap = ncand;
+            free(rec); free(rflag); free(next); free(sent);
+            rec = (pd_rec*)malloc(cap * sizeof(pd_rec));
+            rflag = (uint8_t*)malloc(cap);
+            next = (uint64_t*)malloc((cap * 3 + 2 * (uint64_t)p) * sizeof(uint64_t));
+            sent = (sent_t*)malloc(cap * sizeof(sent_t));
+        }
+        /* candidates of PE r: cand[cb[r], cb[r+1]) (sorted positions) */
+        for (uint32_t r = 0, ci = 0; r <= p; ++r) {
+            if (r == p) { cb[p] = ncand; break; }
+            cb[r] = ci;
+            while (ci < ncand && cand[ci] < pe_begin[r + 1]) ++ci;
+        }
+#pragma omp parallel for schedule(dynamic, 1)
+        for (uint32_t r = 0; r < p; ++r) {
+            const uint64_t c0 = cb[r], c1 = cb[r + 1];
+            pd_rec* R = rec + c0;       /* at most one record per candidate */
+            uint64_t* X = next + 3 * c0 + 2 * (uint64_t)r;
+            uint64_t nr_ = 0, nx = 0;
+            for (uint64_t t = c0; t < c1; ++t) {
+                uint64_t c = cand[t];
+                if ((t == c0 || cand[t - 1] != c) && rounds[c] < 255) rounds[c]++;
+                results[c] = (uint32_t)d;
+                uint32_t id = perm[c];
+                if (d > len[id]) {
+                    results[c] = len[id];
+                    continue;
+                }
+                if (t > c0 && cand[t - 1] + 1 == c && lcp[c] >= d) {
+                    X[nx++] = c;
+                    if (nr_ > 0 && R[nr_ - 1].pos + 1 == c) R[nr_ - 1].lcp_root = 1;
+                    continue;
+                }
+                pd_rec x;
+                x.hash = orc_pd_hash(chars + off[id], d, m);
+                x.pos = c;
+                x.pe = r;
+                x.local_dup = x.send_anyway = x.lcp_root = 0;
+                R[nr_++] = x;
+            }
+            qsort(R, nr_, sizeof(pd_rec), rec_cmp);
+            /* getIndicesOfLocalDuplicates (the single-thread loop, one PE) */
+            uint64_t i = 0;
+            while (i + 1 < nr_) {
+                uint64_t j = i + 1;
+                if (R[j].hash == R[i].hash) {
+                    R[i].local_dup = R[i].send_anyway = 1;
+                    R[j].local_dup = 1;
+                    X[nx++] = R[i].pos;
+                    X[nx++] = R[j].pos;
+                    ++j;
+                    while (j < nr_ && R[j].hash == R[i].hash) {
+                        R[j].local_dup = 1;
+                        X[nx++] = R[j].pos;
+                        ++j;
+                    }
+                } else if (R[i].lcp_root) {
+                    R[i].local_dup = R[i].send_anyway = 1;
+                    X[nx++] = R[i].pos;
+                }
+                i = j;
+            }
+            if (nr_ > 0 && R[nr_ - 1].lcp_root) {
+                R[nr_ - 1].local_dup = R[nr_ - 1].send_anyway = 1;
+                X[nx++] = R[nr_ - 1].pos;
+            }
+            uint64_t ns = 0;
+            sent_t* Sn = sent + c0;
+            for (uint64_t t = 0; t < nr_; ++t) {
+                rflag[c0 + t] = 0;
+                if (!R[t].local_dup || R[t].send_anyway) {
+                    Sn[ns].hash = R[t].hash;
+                    Sn[ns].rec = c0 + t;
+                    ++ns;
+                }
+            }
+            nrec[r] = nr_;
+            nsent[r] = ns;
+            nnext[r] = nx;
+        }
+        /* global duplicates: hash range k owns hashes h with h / w == k;
+         * each PE's sent list is sorted by hash, so its part of a range is
+         * found by binary search */
+        const uint64_t w = UINT64_MAX / p + 1;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (uint32_t k = 0; k < p; ++k) {
+            uint64_t tot = 0;
+            uint64_t* lo = (uint64_t*)malloc(2 * p * sizeof(uint64_t));
+            uint64_t* hi = lo + p;
+            for (uint32_t r = 0; r < p; ++r) {
+                const sent_t* Sn = sent + cb[r];
+                uint64_t a = 0, b = nsent[r];
+                while (a < b) { uint64_t mid = (a + b) / 2; if (Sn[mid].hash / w < k) a = mid + 1; else b = mid; }
+                lo[r] = a;
+                b = nsent[r];
+                while (a < b) { uint64_t mid = (a + b) / 2; if (Sn[mid].hash / w <= k) a = mid + 1; else b = mid; }
+                hi[r] = a;
+                tot += hi[r] - lo[r];
+            }
+            sent_t* G = (sent_t*)malloc((tot ? tot : 1) * sizeof(sent_t));
+            uint64_t g = 0;
+            for (uint32_t r = 0; r < p; ++r)
+                for (uint64_t t = lo[r]; t < hi[r]; ++t) G[g++] = sent[cb[r] + t];
+            qsort(G, tot, sizeof(sent_t), sent_cmp);
+            for (uint64_t i = 0; i < tot;) {
+                uint64_t j = i + 1;
+                while (j < tot && G[j].hash == G[i].hash) ++j;
+                if (j - i >= 2)
+                    for (uint64_t t = i; t < j; ++t)
+                        if (!rec[G[t].rec].send_anyway) rflag[G[t].rec] = 1;
+                i = j;
+            }
+            free(G);
+            free(lo);
+        }
+        /* next candidates per PE, sorted, concatenated in PE order */
+#pragma omp parallel for schedule(dynamic, 1)
+        for (uint32_t r = 0; r < p; ++r) {
+            const uint64_t c0 = cb[r];
+            uint64_t* X = next + 3 * c0 + 2 * (uint64_t)r;
+            uint64_t nx = nnext[r];
+            for (uint64_t t = 0; t < nrec[r]; ++t)
+                if (rflag[c0 + t]) X[nx++] = rec[c0 + t].pos;
+            qsort(X, nx, sizeof(uint64_t), u64_cmp);
+            nnext[r] = nx;
+        }
+        nb[0] = 0;
+        for (uint32_t r = 0; r < p; ++r) nb[r + 1] = nb[r] + nnext[r];
+        if (nb[p] > ccap) { /* the old candidates are no longer read */
+            ccap = nb[p];
+            free(cand);
+            cand = (uint64_t*)malloc(ccap * sizeof(uint64_t));
+        }
+#pragma omp parallel for schedule(dynamic, 1)
+        for (uint32_t r = 0; r < p; ++r)
+            memcpy(cand + nb[r], next + 3 * cb[r] + 2 * (uint64_t)r, nnext[r] * sizeof(uint64_t));
+        ncand = nb[p];
+        ++round;
+        d = start_depth + step;
+        step *= 2;
+        if (round > 60) break;
+    }
+    free(nb); free(nnext); free(nsent); free(nrec); free(cb);
+    free(sent); free(next); free(rflag); free(rec); free(cand);
+    return round;
+}
+
+ORC_API uint32_t orc_pd_sort_mt(const uint8_t* chars, const uint64_t* off, const uint32_t* len,
+                                const uint64_t* pe_begin, uint32_t p, int sampler, uint64_t m,
+                                int threads, uint32_t* out_si, uint32_t* out_pe, uint32_t* out_lcp,
+                                uint32_t* out_results, double* phase_s, uint64_t* out_D) {
+    if (threads > 0) omp_set_num_threads(threads);
+    const uint64_t n = pe_begin[p];
+    const uint64_t nn = n ? n : 1;
+    double t0 = wall_s(), ph[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t* perm = (uint32_t*)malloc(nn * 4);
+    uint32_t* lcp = (uint32_t*)malloc(nn * 4);
+    /* (a2) local sort, one PE per thread */
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint32_t r = 0; r < p; ++r) {
+        const uint64_t b = pe_begin[r], mm = pe_begin[r + 1] - b;
+        uint32_t* tmp = (uint32_t*)malloc((mm ? mm : 1) * 4);
+        sort_ctx c = {chars, off, lcp};
+        for (uint64_t i = b; i < b + mm; ++i) {
+            perm[i] = (uint32_t)i;
+            lcp[i] = 0;
+        }
+        sort_rec(&c, perm + b, tmp, b, mm, 0);
+        free(tmp);
+    }
+    double t1 = wall_s();
+    ph[0] = t1 - t0;
+    uint32_t nrounds = 0;
+    if (p > 1) {
+        uint8_t* rounds = (uint8_t*)malloc(nn);
+        nrounds = pd_mt(chars, off, len, pe_begin, p, perm, lcp, m, 8, out_results, rounds);
+        free(rounds);
+        double t2 = wall_s();
+        ph[1] = t2 - t1;
+        uint64_t* interval = (uint64_t*)malloc((uint64_t)p * p * 8);
+        orc_partition_ex(chars, off, len, pe_begin, p, perm, sampler, 2, 0, out_results, interval, NULL, NULL,
+                         NULL, NULL, NULL);
+        double t3 = wall_s();
+        ph[2] = t3 - t2;
+        /* (a13) exchange, one receiver per thread after the size pass */
+        uint64_t* rb = (uint64_t*)calloc(p + 1, 8);
+        uint64_t* bb = (uint64_t*)calloc(p + 1, 8);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (uint32_t q = 0; q < p; ++q)
+            bb[q + 1] = exch_receiver(2, q, chars, off, len, pe_begin, p, perm, lcp, out_results, interval, 0, 0,
+                                      NULL, NULL, NULL, NULL);
+        for (uint32_t q = 0; q < p; ++q) {
+            uint64_t c = 0;
+            for (uint32_t r = 0; r < p; ++r) c += interval[(uint64_t)r * p + q];
+            rb[q + 1] = rb[q] + c;
+            bb[q + 1] += bb[q];
+        }
+        uint8_t* bytes = (uint8_t*)malloc(bb[p] + 1);
+        uint32_t* rlcp = (uint32_t*)malloc(nn * 4);
+        uint32_t* rpe = (uint32_t*)malloc(nn * 4);
+        uint32_t* rpos = (uint32_t*)malloc(nn * 4);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (uint32_t q = 0; q < p; ++q)
+            exch_receiver(2, q, chars, off, len, pe_begin, p, perm, lcp, out_results, interval, bb[q], rb[q], bytes,
+                          rlcp, rpe, rpos);
+        double t4 = wall_s();
+        ph[3] = t4 - t3;
+        /* (a8) compressed LCP loser-tree merge, one receiver per thread */
+        uint64_t* order = (uint64_t*)malloc(nn * 8);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (uint32_t q = 0; q < p; ++q) {
+            const uint64_t r0 = rb[q], cnt = rb[q + 1] - r0;
+            if (!cnt) continue;
+            const uint8_t** strs = (const uint8_t**)malloc(cnt * sizeof(uint8_t*));
+            const uint8_t* a = bytes + bb[q];
+            for (uint64_t i = 0; i < cnt; ++i) {
+                strs[i] = a;
+                while (*a) ++a;
+                ++a;
+            }
+            uint64_t* runb = (uint64_t*)malloc((p + 1) * 8);
+            runb[0] = 0;
+            for (uint32_t r = 0; r < p; ++r) runb[r + 1] = runb[r] + interval[(uint64_t)r * p + q];
+            orc_merge_losertree(strs, rlcp + r0, runb, p, 1, order + r0, out_lcp + r0, NULL);
+            free(runb);
+            free((void*)strs);
+        }
+        double t5 = wall_s();
+        ph[4] = t5 - t4;
+        /* permutation writeback (prefix_doubling.hpp:282-290) */
+#pragma omp parallel for schedule(dynamic, 1)
+        for (uint32_t q = 0; q < p; ++q)
+            for (uint64_t k = rb[q]; k < rb[q + 1]; ++k) {
+                const uint64_t a = rb[q] + order[k];
+                out_si[k] = rpos[a];
+                out_pe[k] = rpe[a];
+            }
+        ph[5] = wall_s() - t5;
+        /* D of the global output (untimed): the merged LCPs with the LCP
+         * across every receiver boundary recomputed from the strings */
+        if (out_D) {
+            uint32_t* glcp = (uint32_t*)malloc(nn * 4);
+            memcpy(glcp, out_lcp, n * 4);
+            int64_t prev = -1;
+            for (uint32_t q = 0; q < p; ++q) {
+                if (rb[q + 1] == rb[q]) continue;
+                const uint64_t k = rb[q];
+                if (prev >= 0) {
+                    const uint8_t* x = chars + off[perm[pe_begin[out_pe[prev]] + out_si[prev]]];
+                    const uint8_t* y = chars + off[perm[pe_begin[out_pe[k]] + out_si[k]]];
+                    glcp[k] = calc_lcp(x, y);
+                }
+                prev = (int64_t)rb[q + 1] - 1;
+            }
+            *out_D = orc_compute_d(glcp, n);
+            free(glcp);
+        }
+        free(order); free(rpos); free(rpe); free(rlcp); free(bytes); free(bb); free(rb); free(interval);
+    }
+    free(lcp);
+    free(perm);
+    if (phase_s) memcpy(phase_s, ph, sizeof(ph));
+    return nrounds;
 }

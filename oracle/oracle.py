@@ -61,6 +61,12 @@ def lib():
         L.orc_partition.restype = C.c_int
         L.orc_partition.argtypes = [u8p, u64p, u32p, u64p, C.c_uint32, u32p, C.c_int,
                                     C.c_uint64, C.c_uint64, u32p, u64p]
+        L.orc_partition_ex.restype = C.c_int
+        L.orc_partition_ex.argtypes = [u8p, u64p, u32p, u64p, C.c_uint32, u32p, C.c_int,
+                                       C.c_uint64, C.c_uint64, u32p, u64p, u64p, u32p, u64p, u64p, u32p]
+        L.orc_pd_sort_mt.restype = C.c_uint32
+        L.orc_pd_sort_mt.argtypes = [u8p, u64p, u32p, u64p, C.c_uint32, C.c_int, C.c_uint64, C.c_int,
+                                     u32p, u32p, u32p, u32p, C.POINTER(C.c_double), u64p]
         L.orc_exchange.restype = C.c_uint64
         L.orc_exchange.argtypes = [C.c_int, u8p, u64p, u32p, u64p, C.c_uint32, u32p, u32p,
                                    u32p, u64p, u8p, u64p, u64p, u32p, u32p, u32p]
@@ -256,6 +262,32 @@ def partition(ss: StringSet, perm, sampler, max_len=0, dist=None, factor=2):
     return out.reshape(p, p)
 
 
+def partition_ex(ss: StringSet, perm, sampler, max_len=0, dist=None, factor=2):
+    """partition() plus the samples in emission order and the splitters,
+    each as (global index = PE offset + sorted position, truncated length);
+    a missing splitter has index 2^64 - 1."""
+    p = ss.p
+    out = np.zeros(p * p, dtype=np.uint64)
+    cap = max(p * factor * max(p - 1, 0), 1)
+    sg = np.zeros(cap, dtype=np.uint64)
+    st = np.zeros(cap, dtype=np.uint32)
+    ns = C.c_uint64()
+    pg = np.zeros(max(p - 1, 1), dtype=np.uint64)
+    pt = np.zeros(max(p - 1, 1), dtype=np.uint32)
+    offc = ss.off if ss.n else np.zeros(1, dtype=np.uint64)
+    lenc = ss.len if ss.n else np.zeros(1, dtype=np.uint32)
+    permc = perm if ss.n else np.zeros(1, dtype=np.uint32)
+    d = None if dist is None else np.ascontiguousarray(dist, dtype=np.uint32)
+    rc = lib().orc_partition_ex(P(ss.chars, u8p), P(offc, u64p), P(lenc, u32p),
+                                P(ss.pe_begin, u64p), p, P(np.ascontiguousarray(permc), u32p),
+                                SAMPLERS.get(sampler, sampler), factor, max_len,
+                                P(d, u32p) if d is not None else None, P(out, u64p),
+                                P(sg, u64p), P(st, u32p), C.byref(ns), P(pg, u64p), P(pt, u32p))
+    assert rc == 0
+    k = ns.value
+    return out.reshape(p, p), (sg[:k], st[:k]), (pg[:p - 1], pt[:p - 1])
+
+
 MODES = {"whole": 0, "lcp_compact": 1, "pd_prefix": 2}
 
 
@@ -444,6 +476,30 @@ def prefix_doubling_sort(ss: StringSet, sampler="IndexedNumStrings", m=2**64 - 1
                          lcp=olcp, saved=saved))
     return dict(perm=perm, lcp=lcp, results=res, rounds=rounds, nrounds=nr,
                 interval=interval, exchange=ex, out=outs)
+
+
+def prefix_doubling_sort_mt(ss: StringSet, sampler="IndexedNumStrings", m=2**64 - 1, threads=0):
+    """The same sort PE-parallel on `threads` CPU threads (0: OpenMP's
+    default), one PE per thread at a time (orc_pd_sort_mt): per merged
+    output position stringIndex, PEIndex and merged LCP (0 at receiver
+    starts), results per local sorted position, rounds and per-phase wall
+    seconds (local sort, prefix doubling, partition, exchange, merge,
+    permutation)."""
+    n = ss.n
+    nn = max(n, 1)
+    si = np.zeros(nn, np.uint32)
+    pe = np.zeros(nn, np.uint32)
+    lcp = np.zeros(nn, np.uint32)
+    res = np.zeros(nn, np.uint32)
+    ph = (C.c_double * 6)()
+    D = np.zeros(1, np.uint64)
+    offc = ss.off if n else np.zeros(1, dtype=np.uint64)
+    lenc = ss.len if n else np.zeros(1, dtype=np.uint32)
+    nr = lib().orc_pd_sort_mt(P(ss.chars, u8p), P(offc, u64p), P(lenc, u32p), P(ss.pe_begin, u64p), ss.p,
+                              SAMPLERS.get(sampler, sampler), m, threads, P(si, u32p), P(pe, u32p),
+                              P(lcp, u32p), P(res, u32p), ph, P(D, u64p))
+    return dict(string_index=si[:n], pe_index=pe[:n], lcp=lcp[:n], results=res[:n], nrounds=int(nr),
+                phase_s=list(ph), D=int(D[0]))
 
 
 # ---------------------------------------------------------------- wire codecs

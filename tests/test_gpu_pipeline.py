@@ -239,6 +239,51 @@ def test_few_strings_many_pes():
         assert got.tobytes() == exp
 
 
+def _few_chars_pes(rng, p):
+    """Per-PE inputs where PE 1 has 6 strings of 2 characters in total and
+    PE 2 three strings of 1 character: with p = 4 and factor 2 their
+    NumChars walks have splitter_dist 0 and emit their first string several
+    times (identical samples)."""
+    pes = []
+    for r in range(p):
+        if r == 1:
+            strs = [b"", b"b", b"", b"", b"a", b""]
+        elif r == 2:
+            strs = [b"", b"c", b""]
+        else:
+            strs = [bytes(rng.integers(97, 100, int(l), dtype=np.uint8)) for l in rng.integers(0, 6, 200)]
+        pes.append(arena(strs))
+    return pes
+
+
+@pytest.mark.parametrize("use_dist", [False, True])
+def test_partition_pe_with_fewer_chars_than_splitters(use_dist):
+    """Identical samples (the same string and index, emitted more than once
+    by a NumChars walk) get distinct ranks in the global sample order, so
+    every splitter slot is written; buckets equal the oracle's for all four
+    samplers, with and without prefix-doubling lengths."""
+    dss = _dss()
+    p = 4
+    pes = _few_chars_pes(np.random.default_rng(31), p)
+    ss = O.StringSet.from_pes(pes)
+    operm, olcp = O.local_sort(ss)
+    ores = O.prefix_doubling(ss, operm, olcp)[0] if use_dist else None
+    with dss.DeviceStringSet.from_pes(pes) as ds:
+        ds.sort_lcp()
+        if use_dist:
+            ds.prefix_doubling()
+        for rep in range(2):  # the second call reuses the sample buffers
+            for sampler in SAMPLERS:
+                giv = ds.partition(sampler, max_len=1000, use_dist=use_dist)
+                oiv = O.partition(ss, operm, sampler, max_len=1000, dist=ores)
+                assert (giv == oiv).all(), (sampler, rep, giv, oiv)
+                assert (giv.sum(axis=1) == np.diff(ss.pe_begin)).all()
+        # the whole driver on the same input
+        ds.merge_sort(sampler="IndexedNumChars", compressed=False)
+        chk = ds.check()
+        assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0
+
+
 @pytest.mark.parametrize("m", [2**64 - 1, 1 << 20, 4093])
 def test_prefix_doubling_shared_prefix_rounds(m):
     """D/N 0.5 over 500 chars: in the depth-264 round every string is hashed

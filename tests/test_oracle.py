@@ -491,6 +491,213 @@ def test_partition_numchars_buckets_respect_the_order(sampler):
             assert hi[q] <= lo[q + 1]
 
 
+# ------------------------------------------- partition, pinned by counting
+# Checks that the mathematics fixes, with no halving search and no restated
+# NumChars walk: (i) with the indexed samplers every (string, global index)
+# key is distinct and each PE's sorted strings are sorted by that key, so the
+# boundary before bucket q + 1 is the number of local keys smaller than
+# splitter q's key, whatever search finds it; (ii) without an index it is the
+# number of smaller strings when no local string equals the splitter, and lies
+# inside the run of equal strings otherwise (where inside is implementation
+# defined: binarySearch returns any equal probe, misc.hpp:310-333); (iii) the
+# splitters are the samples at global ranks i * floor(S / (nspl + 1)) of the
+# textbook-sorted sample list (getSplitters(Indexed), misc.hpp:89-189), the
+# samples sit at positions floor(i * n / (nr + 1)) (NumStrings,
+# samplingStrategies.hpp:20-90, 107-187) or where the running character count
+# since the previous sample first reaches num_chars / (nr + 1) (NumChars,
+# :189-389), truncated to the splitter length; (iv) the buckets of all PEs
+# respect the global order.
+
+MISSING = 2**64 - 1
+ALL_SAMPLERS = ["NumStrings", "NumChars", "IndexedNumStrings", "IndexedNumChars"]
+
+
+def _few_chars_set(p):
+    """PE 1 holds 6 strings with 2 characters in total, PE 2 three strings
+    with one: their NumChars walks have splitter_dist 0 (at p = 4, factor 2)."""
+    rng = np.random.default_rng(41)
+    pes = []
+    for r in range(p):
+        if r == 1:
+            strs = [b"", b"b", b"", b"", b"a", b""]
+        elif r == 2:
+            strs = [b"", b"c", b""]
+        else:
+            strs = [bytes(rng.integers(97, 100, int(l), dtype=np.uint8)) for l in rng.integers(0, 6, 150)]
+        pes.append(arena(strs))
+    return O.StringSet.from_pes(pes)
+
+
+PIN_SETS = {
+    "distinct": lambda p: O.StringSet(PART_INPUTS["distinct"](), p),
+    "duplicates": lambda p: O.StringSet(PART_INPUTS["duplicates"](), p),
+    "skewed": lambda p: O.StringSet(PART_INPUTS["skewed"](), p),
+    "few_chars": _few_chars_set,
+}
+
+
+# (the few-characters set is built for 4 PEs)
+PIN_CASES = [(n, p) for n in sorted(PIN_SETS) for p in (2, 3, 4, 7) if n != "few_chars" or p == 4]
+
+
+def _pin_case(name, p, sampler, trunc):
+    ss = PIN_SETS[name](p)
+    perm, lcp = O.local_sort(ss)
+    strs = [ss.string(int(perm[g])) for g in range(ss.n)]  # by global sorted position
+    dist = None
+    kw = {}
+    if trunc == "dist":
+        dist = np.array([1 + (g * 13) % 9 for g in range(ss.n)], dtype=np.uint32)
+        kw["dist"] = dist
+    else:
+        kw["max_len"] = 1000 if trunc == "long" else 3
+    iv, samples, spl = O.partition_ex(ss, perm, sampler, **kw)
+    return ss, strs, dist, kw, iv, samples, spl
+
+
+def _bound_errors(ss, strs, iv, spl, indexed):
+    """Boundaries that contradict the splitters (counting checks i and ii)."""
+    errs = []
+    sg, st = spl
+    p = ss.p
+    for r in range(p):
+        b, e = int(ss.pe_begin[r]), int(ss.pe_begin[r + 1])
+        if int(iv[r].sum()) != e - b:
+            errs.append((r, "row sum"))
+        bounds = np.cumsum(iv[r].astype(np.int64))[:-1]
+        local = strs[b:e]
+        for q in range(p - 1):
+            if int(sg[q]) == MISSING:  # fewer splitters than PEs: later buckets empty
+                if int(bounds[q]) != e - b:
+                    errs.append((r, q, "missing splitter"))
+                continue
+            key = strs[int(sg[q])][:int(st[q])]
+            if indexed:
+                kk = (key, int(sg[q]))
+                exp = sum(1 for j, s in enumerate(local) if (s, b + j) < kk)
+                if int(bounds[q]) != exp:
+                    errs.append((r, q, int(bounds[q]), exp))
+            else:
+                lt = sum(1 for s in local if s < key)
+                le = sum(1 for s in local if s <= key)
+                if not lt <= int(bounds[q]) <= le:
+                    errs.append((r, q, int(bounds[q]), lt, le))
+    return errs
+
+
+@pytest.mark.parametrize("trunc", ["long", "short", "dist"])
+@pytest.mark.parametrize("sampler", ALL_SAMPLERS)
+@pytest.mark.parametrize("name,p", PIN_CASES)
+def test_partition_bounds_count_smaller_keys(name, p, sampler, trunc):
+    ss, strs, dist, kw, iv, samples, spl = _pin_case(name, p, sampler, trunc)
+    assert _bound_errors(ss, strs, iv, spl, sampler.startswith("Indexed")) == []
+
+
+def test_partition_bound_check_is_not_vacuous():
+    """The counting check rejects a boundary moved by one string, and
+    boundaries computed with index ties broken the wrong way (descending)."""
+    ss, strs, dist, kw, iv, samples, spl = _pin_case("duplicates", 3, "IndexedNumStrings", "long")
+    assert _bound_errors(ss, strs, iv, spl, True) == []
+    bad = iv.copy()
+    bad[1, 0] += 1
+    bad[1, 1] -= 1
+    assert _bound_errors(ss, strs, bad, spl, True) != []
+    sg, st = spl
+    wrong = np.zeros_like(iv)
+    for r in range(3):
+        b, e = int(ss.pe_begin[r]), int(ss.pe_begin[r + 1])
+        cut = [0]
+        for q in range(2):
+            key = (strs[int(sg[q])][:int(st[q])], -int(sg[q]))
+            cut.append(sum(1 for j, s in enumerate(strs[b:e]) if (s, -(b + j)) < key))
+        cut.append(e - b)
+        wrong[r] = np.diff(cut)
+    assert (wrong != iv).any()
+    assert _bound_errors(ss, strs, wrong, spl, True) != []
+
+
+@pytest.mark.parametrize("trunc", ["long", "short", "dist"])
+@pytest.mark.parametrize("sampler", ALL_SAMPLERS)
+@pytest.mark.parametrize("name,p", PIN_CASES)
+def test_partition_samples_and_splitters(name, p, sampler, trunc):
+    ss, strs, dist, kw, iv, (sgid, stl), (pg, pt) = _pin_case(name, p, sampler, trunc)
+    indexed = sampler.startswith("Indexed")
+    by_chars = sampler.endswith("Chars")
+    factor = 2
+    # samples come PE by PE, each PE's in ascending position
+    pe_of = np.searchsorted(ss.pe_begin[1:], sgid, side="right")
+    assert (np.diff(pe_of) >= 0).all()
+    for r in range(p):
+        b, e = int(ss.pe_begin[r]), int(ss.pe_begin[r + 1])
+        m = e - b
+        nr = min(factor * (p - 1), m)
+        own = [int(g) - b for g in sgid[pe_of == r]]
+        if not by_chars:
+            assert own == [int(i * (m / (nr + 1))) for i in range(1, nr + 1)]
+        else:
+            # the indexed policy walks distinguishing prefixes in the
+            # prefix-doubling form, the plain one lengths
+            w = [int(dist[b + j]) if (dist is not None and indexed) else len(strs[b + j]) for j in range(m)]
+            sd = sum(w) // (nr + 1)
+            if sd == 0:
+                # the reference reads index -1 here (undefined); the oracle
+                # takes the first string every time
+                assert own == [0] * nr
+                continue
+            prev = -1
+            for t in own:
+                assert t > prev
+                run = sum(w[prev + 1:t + 1])
+                assert sum(w[prev + 1:t]) < sd  # no earlier index reaches sd
+                assert run >= sd or t == m - 1  # t reaches it, or the PE ended
+                prev = t
+            assert len(own) == nr or (own and own[-1] == m - 1)
+        # truncation: max_len, or the string's distinguishing prefix
+        for g, tl in zip(sgid[pe_of == r], stl[pe_of == r]):
+            lim = int(dist[int(g)]) if dist is not None else kw["max_len"]
+            assert int(tl) == min(len(strs[int(g)]), lim)
+    # splitters: ranks i * floor(S / (nspl + 1)) of the textbook-sorted samples
+    smp = [(strs[int(g)][:int(t)], int(g)) for g, t in zip(sgid, stl)]
+    smp.sort()
+    S = len(smp)
+    nspl = min(p - 1, S)
+    step = S // (nspl + 1)
+    for q in range(p - 1):
+        if q >= nspl:
+            assert int(pg[q]) == MISSING
+            continue
+        exp = smp[(q + 1) * step]
+        got = (strs[int(pg[q])][:int(pt[q])], int(pg[q]))
+        if indexed:
+            assert got == exp
+        else:  # the order among equal sample strings is not part of parity
+            assert got[0] == exp[0]
+
+
+@pytest.mark.parametrize("sampler", ALL_SAMPLERS)
+@pytest.mark.parametrize("name", sorted(PIN_SETS))
+def test_partition_buckets_respect_the_order_all_samplers(name, sampler):
+    """Every string of bucket q (any PE) precedes every string of bucket q + 1
+    (any PE): by string, and for the indexed samplers strictly by (string,
+    global index)."""
+    p = 4
+    ss, strs, dist, kw, iv, samples, spl = _pin_case(name, p, sampler, "long")
+    indexed = sampler.startswith("Indexed")
+    buckets = [[] for _ in range(p)]
+    for r in range(p):
+        b = int(ss.pe_begin[r])
+        cut = np.concatenate([[0], np.cumsum(iv[r].astype(np.int64))])
+        for q in range(p):
+            for j in range(int(cut[q]), int(cut[q + 1])):
+                buckets[q].append((strs[b + j], b + j))
+    for q in range(p - 1):
+        if buckets[q] and buckets[q + 1]:
+            if indexed:
+                assert max(buckets[q]) < min(buckets[q + 1])
+            else:
+                assert max(x[0] for x in buckets[q]) <= min(x[0] for x in buckets[q + 1])
+
+
 @pytest.mark.parametrize("p", [1, 3, 4])
 def test_merge_sort_no_lcps_is_textbook_sort(p):
     """NoLcps variant of the oracle: every PE's merged strings concatenated
@@ -531,3 +738,27 @@ def test_exact_dist_sums_to_computed(name):
             if count[s] > 1:
                 assert ex[i] == len(s) + 1
     assert list(O.exact_dist(O.StringSet(arena([b"abc"]), 1), np.zeros(1, np.uint32))) == [0]
+
+
+# ------------------------------------------- PE-parallel CPU restatement
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("p", [2, 5, 16])
+@pytest.mark.parametrize("name", sorted(INPUTS))
+def test_pd_sort_mt_equals_single_thread(name, p, threads):
+    """orc_pd_sort_mt (the benchmark's CPU baseline) gives the single-thread
+    oracle's permutation, merged LCPs, results and round count."""
+    chars = INPUTS[name]()
+    ss = O.StringSet(chars, p)
+    ref = O.prefix_doubling_sort(ss)
+    mt = O.prefix_doubling_sort_mt(ss, threads=threads)
+    assert mt["nrounds"] == ref["nrounds"]
+    assert (mt["results"] == ref["results"]).all()
+    si = np.concatenate([o["string_index"] for o in ref["out"]])
+    pe = np.concatenate([o["pe_index"] for o in ref["out"]])
+    lc = np.concatenate([o["lcp"] for o in ref["out"]])
+    assert (mt["string_index"] == si).all() and (mt["pe_index"] == pe).all()
+    assert (mt["lcp"] == lc).all()
+    # D of the global order: computeD over the textbook-sorted input
+    one = O.StringSet(chars, 1)
+    _, glcp = O.local_sort(one)
+    assert mt["D"] == O.compute_d(glcp)
