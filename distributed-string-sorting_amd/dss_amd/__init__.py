@@ -189,17 +189,25 @@ class DeviceStringSet:
         t = _lib.SortParams(cur["large_min"], cur["wave_class_max"], cur["skip_lanes"], cur["skip_strings"])
         check(lib().dss_set_sort_params(self._h, C.byref(t)))
 
+    def _per_string(self, a, dtype, what):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        if a.size != self.n:
+            raise ValueError(f"{what} must have one entry per string ({self.n}), got {a.size}")
+        return a
+
     def set_local_order(self, perm, lcp):
-        perm = np.ascontiguousarray(perm, dtype=np.uint32)
-        lcp = np.ascontiguousarray(lcp, dtype=np.uint32)
+        perm = self._per_string(perm, np.uint32, "perm")
+        lcp = self._per_string(lcp, np.uint32, "lcp")
         check(lib().dss_set_local_order(self._h, _p(perm, _lib.u32p), _p(lcp, _lib.u32p)))
 
     def set_interval(self, interval):
         iv = np.ascontiguousarray(interval, dtype=np.uint64).reshape(-1)
+        if iv.size != self.p * self.p:
+            raise ValueError(f"interval must have p * p = {self.p * self.p} entries, got {iv.size}")
         check(lib().dss_set_interval(self._h, _p(iv, _lib.u64p)))
 
     def set_dist(self, dist):
-        d = np.ascontiguousarray(dist, dtype=np.uint32)
+        d = self._per_string(dist, np.uint32, "dist")
         check(lib().dss_set_dist(self._h, _p(d, _lib.u32p)))
 
     def avg_lcp(self) -> int:
@@ -254,8 +262,8 @@ class DeviceStringSet:
     def set_merged(self, order, lcp):
         """A caller-built merged output (the checker's input as the
         reference's checker tests build it); entries are receive indices."""
-        order = np.ascontiguousarray(order, dtype=np.uint32)
-        lcp = np.ascontiguousarray(lcp, dtype=np.uint32)
+        order = self._per_string(order, np.uint32, "order")
+        lcp = self._per_string(lcp, np.uint32, "lcp")
         check(lib().dss_set_merged(self._h, _p(order, _lib.u32p), _p(lcp, _lib.u32p)))
 
     def extend_prefix(self) -> int:

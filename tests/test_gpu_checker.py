@@ -38,7 +38,8 @@ def _lcp(a, b):
 def _local_lcps(pes):
     out = []
     for strs in pes:
-        out += [0] + [_lcp(strs[i - 1], strs[i]) for i in range(1, len(strs))]
+        if strs:  # an empty PE has no positions
+            out += [0] + [_lcp(strs[i - 1], strs[i]) for i in range(1, len(strs))]
     return np.array(out, dtype=np.uint32)
 
 
@@ -61,6 +62,7 @@ def _device_check(pes, lcp_fault=None, order_fault=None):
         assert n == sum(len(s) for s in pes)
         pb = ds.pe_begin()
         lcp = _local_lcps(pes)
+        assert lcp.size == n
         if lcp_fault:
             lcp_fault(lcp)
         ds.set_local_order(np.arange(n, dtype=np.uint32), lcp)
