@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "dss_device.hpp"
@@ -559,6 +560,82 @@ void check_launch() { DSS_HIP(hipGetLastError()); }
 
 }  // namespace
 
+// ------------------------------------------------- allocation-free varint
+// dss_varint_write_ranges_noalloc keeps its scratch (per-value byte lengths
+// and offsets, the scan's block sums and total, the range counts and ends)
+// in one fixed device array instead of hipMalloc'd buffers.  Offsets in u64
+// words, each region 256-byte aligned.
+constexpr uint64_t kNaMaxValues = DSS_NOALLOC_VARINT_MAX_VALUES;
+constexpr uint64_t kNaMaxRanges = DSS_NOALLOC_VARINT_MAX_RANGES;
+constexpr uint64_t na_align(uint64_t w) { return (w + 31) / 32 * 32; }
+constexpr uint64_t kNaScanWords = (kNaMaxValues + 1 + 2047) / 2048 + 1;  // scan_scratch_words(n + 1), n <= max
+constexpr uint64_t kNaL = 0;
+constexpr uint64_t kNaO = kNaL + na_align(kNaMaxValues + 1);
+constexpr uint64_t kNaSums = kNaO + na_align(kNaMaxValues + 1);
+constexpr uint64_t kNaTot = kNaSums + na_align(kNaScanWords);
+constexpr uint64_t kNaCounts = kNaTot + 32;
+constexpr uint64_t kNaEnds = kNaCounts + na_align(kNaMaxRanges);
+constexpr uint64_t kNaWords = kNaEnds + na_align(kNaMaxRanges + 1);
+
+// One copy per GPU, for the life of the process.  Host code must not pass
+// this name to a kernel (on the host it is a stand-in, not a device
+// address): noalloc_scratch() resolves each device's address once.
+static __device__ __align__(256) uint64_t scratch[kNaWords];
+
+constexpr int kNaMaxDevices = 64;
+static std::mutex g_na_mutex;
+static uint64_t* g_na_ptr[kNaMaxDevices];
+
+// The device address of `scratch` on `device`, resolved by
+// hipGetSymbolAddress the first time with that device current.
+static uint64_t* noalloc_scratch(int device) {
+    DSS_REQUIRE(device >= 0 && device < kNaMaxDevices, DSS_ERANGE, "device index beyond the supported range");
+    std::lock_guard<std::mutex> g(g_na_mutex);
+    if (!g_na_ptr[device]) {
+        int cur = 0;
+        DSS_HIP(hipGetDevice(&cur));
+        if (cur != device) DSS_HIP(hipSetDevice(device));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr)
+            throw Error{DSS_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+        g_na_ptr[device] = static_cast<uint64_t*>(ptr);
+    }
+    return g_na_ptr[device];
+}
+
+// ends[r] = O[first value of range r] for r < nr, ends[nr] = O[n]: one block,
+// four ranges per thread (nr <= 4096), an LDS scan of the counts.
+__global__ void __launch_bounds__(1024) k_range_ends(const uint64_t* __restrict__ counts, uint32_t nr,
+                                                     const uint64_t* __restrict__ O, uint64_t* __restrict__ ends) {
+    __shared__ uint64_t part[1024];
+    const uint32_t t = threadIdx.x;
+    uint64_t c[4], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t r = 4 * t + k;
+        c[k] = r < nr ? counts[r] : 0;
+        sum += c[k];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+        const uint64_t x = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint64_t run = t ? part[t - 1] : 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t r = 4 * t + k;
+        if (r < nr) ends[r] = O[run];
+        run += c[k];
+    }
+    if (t == 1023) ends[nr] = O[part[1023]];
+}
+
 // ----------------------------------------------------------- entry points
 static void varint_write(dss_codec_s* c, const uint64_t* values, const uint64_t* counts, uint32_t nr, uint8_t* out,
                          uint64_t out_cap, uint64_t* range_bytes, uint64_t* total_bytes) {
@@ -924,6 +1001,51 @@ int dss_varint_write_ranges_dev(dss_codec c, const uint64_t* values, const uint6
         DSS_REQUIRE(c && (counts || !num_ranges) && (range_bytes || !num_ranges) && total_bytes, DSS_EINVAL,
                     "NULL argument");
         varint_write(c, values, counts, num_ranges, out, out_cap, range_bytes, total_bytes);
+    });
+}
+
+int dss_varint_write_ranges_noalloc(dss_codec c, const uint64_t* values, const uint64_t* counts,
+                                    uint32_t num_ranges, uint8_t* out, uint64_t out_cap, uint64_t* range_bytes,
+                                    uint64_t* total_bytes) {
+    return guarded([&] {
+        DSS_REQUIRE(c && (counts || !num_ranges) && (range_bytes || !num_ranges) && total_bytes, DSS_EINVAL,
+                    "NULL argument");
+        DSS_REQUIRE(num_ranges <= kNaMaxRanges, DSS_ERANGE,
+                    "dss_varint_write_ranges_noalloc takes at most " + std::to_string(kNaMaxRanges) +
+                        " ranges, got " + std::to_string(num_ranges));
+        uint64_t n = 0;
+        for (uint32_t r = 0; r < num_ranges; ++r) {
+            n += counts[r];
+            DSS_REQUIRE(n <= kNaMaxValues, DSS_ERANGE,
+                        "dss_varint_write_ranges_noalloc takes at most " + std::to_string(kNaMaxValues) +
+                            " values per call");
+        }
+        DSS_REQUIRE(n == 0 || values, DSS_EINVAL, "NULL values");
+        uint64_t* S = noalloc_scratch(c->device);
+        uint64_t* L = S + kNaL;
+        uint64_t* O = S + kNaO;
+        uint64_t* ends = S + kNaEnds;
+        const hipStream_t st = c->stream;
+        // byte length per value (and 0 after the last), offsets by the scan
+        hipLaunchKernelGGL(k_varint_len, dim3(stride_grid(n + 1, 256)), dim3(256), 0, st, values, n, L);
+        check_launch();
+        exclusive_scan_u64_scratch(L, O, n + 1, st, S + kNaSums, S + kNaTot);
+        uint64_t last = 0;
+        if (num_ranges) {
+            DSS_HIP(hipMemcpyAsync(S + kNaCounts, counts, num_ranges * 8, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_range_ends, dim3(1), dim3(1024), 0, st, S + kNaCounts, num_ranges, O, ends);
+            check_launch();
+            DSS_HIP(hipMemcpyAsync(range_bytes, ends, num_ranges * 8, hipMemcpyDeviceToHost, st));
+        }
+        DSS_HIP(hipMemcpyAsync(&last, O + n, 8, hipMemcpyDeviceToHost, st));
+        DSS_HIP(hipStreamSynchronize(st));
+        for (uint32_t r = 0; r < num_ranges; ++r)  // range ends -> range sizes, in place
+            range_bytes[r] = (r + 1 < num_ranges ? range_bytes[r + 1] : last) - range_bytes[r];
+        *total_bytes = last;
+        DSS_REQUIRE(last <= out_cap, DSS_ERANGE,
+                    "varint output needs " + std::to_string(last) + " bytes, capacity " + std::to_string(out_cap));
+        if (n) hipLaunchKernelGGL(k_varint_put, dim3(stride_grid(n, 256)), dim3(256), 0, st, values, n, O, out);
+        check_launch();
     });
 }
 

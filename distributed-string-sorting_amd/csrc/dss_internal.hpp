@@ -249,6 +249,12 @@ struct PipeState {
 // total when total_host != nullptr (synchronises the stream).
 void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t st,
                         PipeState& P, uint64_t* total_host);
+// The same scan with caller-provided scratch: sums holds scan_scratch_words(n)
+// u64 words, the total is written to the device word total_dev; no
+// allocation and no read-back.
+uint64_t scan_scratch_words(uint64_t n);
+void exclusive_scan_u64_scratch(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t st,
+                                uint64_t* sums, uint64_t* total_dev);
 
 }  // namespace dss
 

@@ -91,17 +91,23 @@ __global__ void __launch_bounds__(kST) k_scan_apply(const uint64_t* __restrict__
 
 __global__ void k_store_u64(uint64_t* __restrict__ dst, uint64_t v) { *dst = v; }
 
-void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t st, PipeState& P,
-                        uint64_t* total_host) {
-    const uint64_t nb = std::max<uint64_t>((n + kSB - 1) / kSB, 1);
-    P.scan_tmp.reserve((nb + 1) * 8);
-    P.scan_tot.reserve(8);
-    uint64_t* sums = P.scan_tmp.as<uint64_t>();
+uint64_t scan_scratch_words(uint64_t n) { return std::max<uint64_t>((n + kSB - 1) / kSB, 1) + 1; }
+
+void exclusive_scan_u64_scratch(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t st, uint64_t* sums,
+                                uint64_t* total_dev) {
+    const uint64_t nb = scan_scratch_words(n) - 1;
     fill_small(st, sums, 0, (nb + 1) * 8);
     if (n) hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kST), 0, st, in, n, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, nb, P.scan_tot.as<uint64_t>());
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, nb, total_dev);
     if (n) hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kST), 0, st, in, out, n, sums);
     DSS_HIP(hipGetLastError());
+}
+
+void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, hipStream_t st, PipeState& P,
+                        uint64_t* total_host) {
+    P.scan_tmp.reserve(scan_scratch_words(n) * 8);
+    P.scan_tot.reserve(8);
+    exclusive_scan_u64_scratch(in, out, n, st, P.scan_tmp.as<uint64_t>(), P.scan_tot.as<uint64_t>());
     if (total_host) P.mbox.fetch(st, total_host, P.scan_tot.p, 8);
 }
 

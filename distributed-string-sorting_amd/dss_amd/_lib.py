@@ -124,6 +124,8 @@ _SIGS = {
     "dss_varint_write_ranges": (C.c_int, [codec_p, u64p, u64p, C.c_uint32, u8p, C.c_uint64, u64p, u64p]),
     "dss_varint_write_ranges_dev": (C.c_int, [codec_p, C.c_void_p, u64p, C.c_uint32, C.c_void_p, C.c_uint64,
                                               u64p, u64p]),
+    "dss_varint_write_ranges_noalloc": (C.c_int, [codec_p, C.c_void_p, u64p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                                  u64p, u64p]),
     "dss_varint_read_ranges": (C.c_int, [codec_p, u8p, C.c_uint64, u64p, C.c_uint32, u64p]),
     "dss_varint_read_ranges_dev": (C.c_int, [codec_p, C.c_void_p, C.c_uint64, u64p, C.c_uint32, C.c_void_p]),
     "dss_golomb_b": (C.c_int, [C.c_uint64, C.c_uint64, u64p]),

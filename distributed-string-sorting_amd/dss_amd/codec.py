@@ -146,6 +146,16 @@ class WireCodec:
                                                 _p(rb, u64p), C.byref(tot)))
         return tot.value, rb[:nr].copy()
 
+    def write_ranges_noalloc(self, values_ptr: int, counts, out_ptr: int, out_cap: int):
+        """write_ranges_dev with the library's fixed scratch array: allocates
+        no device memory (at most 2^21 values and 4096 ranges per call)."""
+        c, nr = _u64(counts)
+        rb = np.zeros(max(1, nr), np.uint64)
+        tot = C.c_uint64()
+        check(lib().dss_varint_write_ranges_noalloc(self._h, values_ptr, _p(c, u64p), nr, out_ptr, out_cap,
+                                                    _p(rb, u64p), C.byref(tot)))
+        return tot.value, rb[:nr].copy()
+
     def read_ranges_dev(self, bytes_ptr: int, nbytes: int, counts, values_ptr: int):
         c, nr = _u64(counts)
         check(lib().dss_varint_read_ranges_dev(self._h, bytes_ptr, nbytes, _p(c, u64p), nr, values_ptr))

@@ -378,6 +378,22 @@ int dss_varint_write_ranges(dss_codec c, const uint64_t* values, const uint64_t*
                             uint8_t* out, uint64_t out_cap, uint64_t* range_bytes, uint64_t* total_bytes);
 int dss_varint_write_ranges_dev(dss_codec c, const uint64_t* values, const uint64_t* counts, uint32_t num_ranges,
                                 uint8_t* out, uint64_t out_cap, uint64_t* range_bytes, uint64_t* total_bytes);
+/* No counterpart in the reference: dss_varint_write_ranges_dev for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and output;
+ * its scratch (per-value lengths and offsets, scan sums, range ends) lives in
+ * one fixed device array of the library (about 34 MB, held on every GPU the
+ * library runs on until the process ends), so a call allocates nothing.
+ * Limits: at most DSS_NOALLOC_VARINT_MAX_VALUES values and
+ * DSS_NOALLOC_VARINT_MAX_RANGES ranges per call (DSS_ERANGE beyond).  The
+ * encoded bytes are complete once the codec's stream has finished
+ * (dss_codec_sync).  All calls on one GPU share the array: calls that overlap
+ * on one GPU (other codecs or host threads) race. */
+#define DSS_NOALLOC_VARINT_MAX_VALUES (1u << 21)
+#define DSS_NOALLOC_VARINT_MAX_RANGES 4096u
+int dss_varint_write_ranges_noalloc(dss_codec c, const uint64_t* values, const uint64_t* counts,
+                                    uint32_t num_ranges, uint8_t* out, uint64_t out_cap, uint64_t* range_bytes,
+                                    uint64_t* total_bytes);
 /* IntegerCompression::readRanges (src/mpi/byte_encoder.hpp:64-78): nbytes must
  * hold exactly sum(counts) varints; DSS_EINVAL where GetVarint
  * (integer_compression.hpp:209-236) throws or the input is short. */

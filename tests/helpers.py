@@ -62,3 +62,65 @@ def file_distributer(text: bytes, p: int):
             s = s[firsts[r] + 1:]
         out.append(bytes(s))
     return out
+
+
+class HipMem:
+    """Device buffers for tests of the library's device-pointer entry points,
+    through the HIP runtime the library itself loaded (libamdhip64.so.7,
+    resolved by soname once dss_amd is imported), so no second runtime is
+    brought into the process."""
+
+    def __init__(self):
+        import ctypes as C
+        import dss_amd  # noqa: F401  (loads libdss_gpu.so and its HIP runtime)
+        self.C = C
+        self.h = C.CDLL("libamdhip64.so.7")
+        self.h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.h.hipFree.argtypes = [C.c_void_p]
+        self.h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        self.h.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        self.h.hipDeviceSynchronize.argtypes = []
+        self.bufs = []
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    def alloc(self, nbytes: int) -> int:
+        p = self.C.c_void_p()
+        self._ok(self.h.hipMalloc(self.C.byref(p), max(int(nbytes), 1)), "hipMalloc")
+        self._ok(self.h.hipMemset(p, 0, max(int(nbytes), 1)), "hipMemset")
+        self.bufs.append(p.value)
+        return p.value
+
+    def upload(self, a) -> int:
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        self._ok(self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
+        return p
+
+    def download(self, ptr: int, nbytes: int, dtype=np.uint8):
+        out = np.zeros(int(nbytes), dtype=np.uint8)
+        if nbytes:
+            self._ok(self.h.hipMemcpy(out.ctypes.data, ptr, int(nbytes), 2), "hipMemcpy D2H")
+        return out.view(dtype)
+
+    def free_bytes(self) -> int:
+        f, t = self.C.c_size_t(), self.C.c_size_t()
+        self._ok(self.h.hipMemGetInfo(self.C.byref(f), self.C.byref(t)), "hipMemGetInfo")
+        return f.value
+
+    def sync(self):
+        self._ok(self.h.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def close(self):
+        for p in self.bufs:
+            self.h.hipFree(p)
+        self.bufs = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
