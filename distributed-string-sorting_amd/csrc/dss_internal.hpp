@@ -265,6 +265,7 @@ struct SortParams {
     uint32_t wave_class_max = 6;   // base-case classes handled one segment per wave
     uint32_t skip_lanes = 8;       // lanes per string in the prefix skip
     uint32_t skip_strings = 4;     // strings per lane group and round in the prefix skip
+    uint32_t merge_fan = 2;        // runs merged per pass (2: the pairwise tree; 4, 8, 16: K-way tiles)
 };
 
 struct dss_set_s {

@@ -905,25 +905,63 @@ void exchange(dss_set_s* s, int mode) {
 }
 
 // ================================================================== (a8)
-// LCP merge of the p received runs of every PE: a binary merge tree of
-// merge-path partitioned two-way LCP merges.  Each thread produces kW outputs
-// of one pair of runs: it finds its start by a merge-path search, derives the
-// LCP of both heads with the previous output, and then merges with the LCP
-// rules of the loser tree (bingmann-lcp_losertree.hpp:280-333): the head with
-// the larger LCP to the last output is smaller; on equal LCPs characters are
-// compared from that LCP on.  Character comparisons first use a cached 8-char
-// key at the receiver's common prefix.  The merged LCP array is exact; the
-// saved LCP of an output is the run LCP it was received with (oldLcps).
-constexpr uint32_t kMT = 512;               // threads per merge block
-constexpr uint32_t kMW = 2;                 // outputs per thread
-constexpr uint32_t kTileOut = kMT * kMW;    // outputs per tile
+// K-way LCP merge of the runs every PE received: choose_merge / merge<K>
+// with LcpStringLoserTree_ (src/sorter/distributed/merging.hpp:23-123,
+// src/merge/bingmann-lcp_losertree.hpp:204-480), K = the merge fan-in F, a
+// runtime parameter: F = 2 runs the pairwise merge tree further below (one
+// global pass per tree level), F = 4, 8 or 16 the K-way passes here; a PE
+// with more runs than F merges them in ceil(log_F runs) passes.  Both give
+// the same output (ties: the earlier run first).
+//
+// A K-way pass merges groups of at most F adjacent lists (a receiver's runs, or a
+// previous pass's outputs) with one read and one write of every element:
+//   1. samples: every S-th element of every list.  F lanes per sample find
+//      its co-rank (the number of elements before it) in each list of its
+//      group by binary search; their sum is its rank in the group's merged
+//      order.  The order is by string, equal strings by list (an earlier
+//      list first: on equality the loser tree keeps its defender).  With
+//      S = floor((G - 1) / lists) + 1 consecutive sample ranks differ by at
+//      most G (between two consecutive samples every list holds fewer than S
+//      elements).
+//   2. tiles: tile k of a group starts at B_k, the largest sample rank
+//      <= k G (B_0 = 0: a list's first element is a sample), so a tile holds
+//      fewer than 2G elements; the co-ranks of its two bounding samples give
+//      every list's part of the tile.
+//   3. one block per tile stages its elements in LDS (receive index, the
+//      cached 8 characters at the receiver's common prefix, the LCP with the
+//      list predecessor) and runs the tournament's levels in LDS: level r
+//      merges adjacent pairs of 2^r-list units by merge-path partitioned
+//      two-way LCP merges with the loser tree's node rule (bingmann-lcp_
+//      losertree.hpp:280-333: the head with the larger LCP to the last output
+//      is smaller; equal LCPs compare characters from that LCP on).  The
+//      merged LCP array is exact; the saved LCP of an output is the run LCP
+//      it was received with (oldLcps).
+//   4. the LCP of every tile's first output with its predecessor.
+// Character comparisons first use the cached keys, then the strings.
+constexpr uint32_t kKT = 512;           // threads per tile block
+constexpr uint32_t kKW = 4;             // outputs per thread and level at most
+constexpr uint32_t kKTile = kKT * kKW;  // LDS capacity of a tile (elements)
+constexpr uint32_t kKG = kKTile / 2;    // tile step: a tile holds fewer than 2 kKG elements
+constexpr uint32_t kKMaxFan = 16;
 
-struct MergePair {
-    uint64_t out;      // first receive position of the pair (A then B)
-    uint64_t na, nb;   // run lengths
-    uint64_t tile0;    // first tile of the pair
-    uint32_t recv;     // receiving PE
-    uint32_t pad;
+struct KGroup {
+    uint64_t out;     // position of the group's first element
+    uint64_t n;       // elements of the group (>= 1)
+    uint64_t tile0;   // first tile
+    uint64_t ntiles;  // ceil(n / kKG)
+    uint32_t list0;   // first list
+    uint32_t nl;      // lists (1..F)
+    uint32_t recv;    // receiving PE
+    uint32_t S;       // sample stride of its lists
+};
+
+struct KList {
+    uint64_t start;   // position of the first element
+    uint64_t size;    // >= 1
+    uint64_t samp0;   // first sample (index into the pass's samples)
+    uint64_t nsamp;   // ceil(size / S)
+    uint32_t group;
+    uint32_t slot;    // index inside the group
 };
 
 struct ViewCtx {
@@ -966,6 +1004,325 @@ __device__ inline int elem_cmp(const ViewCtx& V, uint32_t a, uint64_t ka, uint32
     }
     return view_cmp32(V.chars, V.voff[a], V.vlen[a], V.voff[b], V.vlen[b], &lcp, from);
 }
+
+// Step 1: F lanes per sample (F a power of two <= 16: a sample's lanes share
+// a wave); lane `slot` finds the sample's co-rank in the group's list `slot`.
+// Two launches: coarse (every kKCoarse-th sample of a list) searches the
+// whole list, fine searches only between the co-ranks of the coarse samples
+// around it (co-ranks grow with the sample's position in its list).
+constexpr uint32_t kKCoarse = 16;
+
+__global__ void __launch_bounds__(256)
+k_kmerge_sample(const KGroup* __restrict__ groups, const KList* __restrict__ lists, uint32_t lgF, int coarse, const uint32_t* __restrict__ ord, const uint8_t* __restrict__ chars,
+                const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen, const uint64_t* __restrict__ keyc,
+                const uint64_t* __restrict__ keyc2, const uint32_t* __restrict__ rcp, uint32_t* __restrict__ corank,
+                uint64_t* __restrict__ rankv) {
+    const uint32_t F = 1u << lgF;
+    // block (x, y): list y, its samples from x * blockDim / F on (coarse:
+    // its coarse samples, every kKCoarse-th); F lanes per sample
+    const KList L = lists[blockIdx.y];
+    const uint64_t u = (uint64_t)blockIdx.x * (blockDim.x >> lgF) + (threadIdx.x >> lgF);
+    const uint64_t si = coarse ? u * kKCoarse : u;  // sample index inside the list
+    const uint32_t slot = threadIdx.x & (F - 1);
+    const uint64_t s = L.samp0 + si;
+    uint64_t co = 0;
+    const bool mine = si < L.nsamp && (coarse || si % kKCoarse != 0);  // this launch computes sample s
+    if (mine) {
+        const KGroup G = groups[L.group];
+        const uint64_t j = si * G.S;
+        if (slot == L.slot) {
+            co = j;
+        } else if (slot < G.nl) {
+            const KList M = lists[G.list0 + slot];
+            const ViewCtx V{chars, voff, vlen, keyc2, rcp[G.recv]};
+            const uint32_t x = ord ? ord[L.start + j] : (uint32_t)(L.start + j);
+            const uint64_t kx = keyc[x];
+            const bool later = slot > L.slot;  // a later list's equal elements come after x
+            uint64_t a = 0, b = M.size;
+            if (!coarse) {  // bracket by the coarse samples before and after
+                const uint64_t c0 = L.samp0 + si / kKCoarse * kKCoarse, c1 = c0 + kKCoarse;
+                a = corank[c0 * F + slot];
+                if (c1 < L.samp0 + L.nsamp) b = corank[c1 * F + slot];
+            }
+            while (a < b) {
+                const uint64_t mid = (a + b) / 2;
+                const uint32_t y = ord ? ord[M.start + mid] : (uint32_t)(M.start + mid);
+                uint32_t l;
+                const int c = elem_cmp(V, y, keyc[y], x, kx, V.cp, l);
+                if (c < 0 || (c == 0 && !later)) a = mid + 1; else b = mid;
+            }
+            co = a;
+        }
+        corank[s * F + slot] = (uint32_t)co;
+    }
+    uint64_t sum = co;  // the sample's rank: the sum over its F lanes
+    for (uint32_t o = 1; o < F; o <<= 1) sum += __shfl_xor((unsigned long long)sum, (int)o);
+    if (mine && slot == 0) rankv[s] = sum;
+}
+
+// Step 2: the start of every tile, kKMaxFan lanes per tile: lane l finds the
+// last sample of list l with rank <= k G, the largest of them bounds the
+// tile; its rank and co-ranks are stored (tile_b, tile_co).
+__global__ void __launch_bounds__(256)
+k_kmerge_bounds(const KGroup* __restrict__ groups, uint32_t ngroups, uint64_t ntiles, const KList* __restrict__ lists,
+                uint32_t lgF, const uint32_t* __restrict__ corank, const uint64_t* __restrict__ rankv,
+                uint64_t* __restrict__ tile_b, uint32_t* __restrict__ tile_co, uint32_t* __restrict__ tile_g) {
+    const uint32_t F = 1u << lgF;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t tb = t / kKMaxFan;
+    const uint32_t l = (uint32_t)(t % kKMaxFan);
+    uint64_t best = 0, bsamp = 0;  // rank + 1 of the list's last sample <= target (0: none)
+    uint32_t nl = 0;
+    if (tb < ntiles) {
+        uint32_t lo = 0, hi = ngroups;
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (groups[mid].tile0 <= tb) lo = mid; else hi = mid;
+        }
+        const KGroup G = groups[lo];
+        nl = G.nl;
+        if (l == 0) tile_g[tb] = lo;
+        const uint64_t target = (tb - G.tile0) * kKG;
+        if (l < G.nl) {
+            const KList L = lists[G.list0 + l];
+            uint64_t a = 0, b = L.nsamp;  // samples with rank <= target form a prefix
+            while (a < b) {
+                const uint64_t mid = (a + b) / 2;
+                if (rankv[L.samp0 + mid] <= target) a = mid + 1; else b = mid;
+            }
+            if (a > 0) {
+                bsamp = L.samp0 + a - 1;
+                best = rankv[bsamp] + 1;
+            }
+        }
+    }
+    uint64_t m = best;
+    for (uint32_t o = 1; o < kKMaxFan; o <<= 1) m = max(m, (uint64_t)__shfl_xor((unsigned long long)m, (int)o));
+    if (tb < ntiles && best != 0 && best == m) {  // the bounding sample (ranks are distinct)
+        tile_b[tb] = best - 1;
+        for (uint32_t q = 0; q < nl; ++q) tile_co[tb * F + q] = corank[bsamp * F + q];
+    }
+}
+
+// One element of a tile in LDS: its cached key, receive index and the LCP
+// with its predecessor in its current list (16 bytes: one LDS access).
+struct KElem {
+    uint64_t key;
+    uint32_t id;
+    uint32_t lcp;
+};
+
+// LDS slot of tile element i: one spare record after every 16, so that lanes
+// touching records 2 or 4 apart (a thread's outputs are consecutive) fall on
+// different banks.
+__device__ inline uint32_t kpad(uint32_t i) { return i + (i >> 4); }
+constexpr uint32_t kKTileP = kKTile + kKTile / 16;
+
+// Step 3 helper: merges outputs [dl, dl + cnt) of the unit made of lists A =
+// LDS positions [aS, aS + na) and B = [bS, bS + nb) (bS = aS + na) into the
+// destination buffer at the same positions.
+__device__ inline void kmerge_unit(const ViewCtx& V, const KElem* __restrict__ E, KElem* __restrict__ D, uint32_t aS,
+                                   uint32_t na, uint32_t bS, uint32_t nb, uint32_t dl, uint32_t cnt) {
+    uint32_t lo = dl > nb ? dl - nb : 0, hi = dl < na ? dl : na;
+    while (lo < hi) {  // merge path: A elements among the first dl outputs (A wins ties)
+        const uint32_t mid = (lo + hi) / 2;
+        const KElem x = E[kpad(aS + mid)], y = E[kpad(bS + dl - mid - 1)];
+        uint32_t l;
+        if (elem_cmp(V, x.id, x.key, y.id, y.key, V.cp, l) <= 0) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t i = lo, j = dl - lo;
+    uint32_t la = V.cp, lb = V.cp;  // LCPs of the heads with the last output
+    KElem ha, hb;                   // the heads
+    if (i < na) ha = E[kpad(aS + i)];
+    if (j < nb) hb = E[kpad(bS + j)];
+    if (dl > 0) {
+        bool prev_is_a;
+        uint32_t x;
+        KElem pa, pb;
+        if (i > 0) pa = E[kpad(aS + i - 1)];
+        if (j > 0) pb = E[kpad(bS + j - 1)];
+        if (i == 0) prev_is_a = false;
+        else if (j == 0) prev_is_a = true;
+        else prev_is_a = elem_cmp(V, pa.id, pa.key, pb.id, pb.key, V.cp, x) > 0;
+        const KElem pp = prev_is_a ? pa : pb;
+        if (i < na) {
+            if (prev_is_a) la = ha.lcp;
+            else elem_cmp(V, ha.id, ha.key, pp.id, pp.key, V.cp, la);
+        }
+        if (j < nb) {
+            if (!prev_is_a) lb = hb.lcp;
+            else elem_cmp(V, hb.id, hb.key, pp.id, pp.key, V.cp, lb);
+        }
+    }
+    for (uint32_t c = 0; c < cnt; ++c) {
+        const bool has_a = i < na, has_b = j < nb;
+        bool take_a;
+        if (!has_a) take_a = false;
+        else if (!has_b) take_a = true;
+        else if (la > lb) take_a = true;
+        else if (lb > la) take_a = false;
+        else {
+            uint32_t x;
+            const int cr = elem_cmp(V, ha.id, ha.key, hb.id, hb.key, la, x);
+            take_a = cr <= 0;
+            if (take_a) lb = x; else la = x;
+        }
+        KElem out;
+        if (take_a) {
+            out = ha;
+            out.lcp = la;
+            ++i;
+            if (i < na) {
+                ha = E[kpad(aS + i)];
+                la = ha.lcp;
+            }
+        } else {
+            out = hb;
+            out.lcp = lb;
+            ++j;
+            if (j < nb) {
+                hb = E[kpad(bS + j)];
+                lb = hb.lcp;
+            }
+        }
+        D[kpad(aS + dl + c)] = out;
+    }
+}
+
+// Steps 2 and 3: one block per tile.
+__global__ void __launch_bounds__(kKT)
+k_kmerge_tile(const KGroup* __restrict__ groups, uint32_t ngroups, const KList* __restrict__ lists, uint32_t lgF,
+              const uint64_t* __restrict__ tile_b, const uint32_t* __restrict__ tile_co,
+              const uint32_t* __restrict__ tile_g,
+              const uint32_t* __restrict__ ord, const uint32_t* __restrict__ lcp_in, uint32_t* __restrict__ ord_out,
+              uint32_t* __restrict__ lcp_out, const uint8_t* __restrict__ chars, const uint64_t* __restrict__ voff,
+              const uint32_t* __restrict__ vlen, const uint64_t* __restrict__ keyc,
+              const uint64_t* __restrict__ keyc2, const uint32_t* __restrict__ rcp, const uint32_t* __restrict__ rlcp,
+              uint32_t* __restrict__ saved) {
+    __shared__ KElem sE[2][kKTileP];
+    __shared__ uint64_t sBnd[2];
+    __shared__ uint32_t sCo[2][kKMaxFan];
+    __shared__ uint32_t sOff[kKMaxFan + 1];
+    __shared__ uint64_t sStart[kKMaxFan];  // position of every list's first element of the tile
+    const uint32_t F = 1u << lgF;
+    const uint64_t tb = blockIdx.x;
+    const KGroup G = groups[tile_g[tb]];
+    const uint64_t k = tb - G.tile0;
+    // tile bounds (step 2); the group's end bounds its last tile
+    if (threadIdx.x < 2 * kKMaxFan) {
+        const uint32_t which = threadIdx.x / kKMaxFan, l = threadIdx.x % kKMaxFan;
+        const bool at_end = k + which >= G.ntiles;
+        if (l == 0) sBnd[which] = at_end ? G.n : tile_b[tb + which];
+        if (l < G.nl) sCo[which][l] = at_end ? (uint32_t)lists[G.list0 + l].size : tile_co[(tb + which) * F + l];
+    }
+    __syncthreads();
+    const uint64_t B0 = sBnd[0], B1 = sBnd[1];
+    if (threadIdx.x < G.nl) sStart[threadIdx.x] = lists[G.list0 + threadIdx.x].start + sCo[0][threadIdx.x];
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t l = 0; l < G.nl; ++l) {
+            sOff[l] = acc;
+            acc += sCo[1][l] - sCo[0][l];
+        }
+        sOff[G.nl] = acc;
+    }
+    __syncthreads();
+    const uint32_t tsize = (uint32_t)(B1 - B0);
+    if (B1 < B0 || B1 - B0 > kKTile || sOff[G.nl] != tsize) return;  // cannot happen (gap bound)
+    const ViewCtx V{chars, voff, vlen, keyc2, rcp[G.recv]};
+    // stage the tile: list l's part at LDS [sOff[l], sOff[l+1])
+    for (uint32_t l = 0; l < G.nl; ++l) {
+        const uint32_t o = sOff[l], c = sOff[l + 1] - o;
+        const uint64_t p0 = sStart[l];
+        for (uint32_t i = threadIdx.x; i < c; i += kKT) {
+            const uint64_t pos = p0 + i;
+            const uint32_t id = ord ? ord[pos] : (uint32_t)pos;
+            KElem e;
+            e.key = keyc[id];
+            e.id = id;
+            e.lcp = lcp_in[pos];
+            sE[0][kpad(o + i)] = e;
+        }
+    }
+    __syncthreads();
+    // outputs per thread: as few as the tile allows, so that every thread works
+    const uint32_t kw = (tsize + kKT - 1) / kKT;
+    uint32_t cur = 0;
+    for (uint32_t w = 1; w < G.nl; w <<= 1) {  // level: units of w lists merge pairwise
+        const uint32_t o0 = threadIdx.x * kw, oe = min(o0 + kw, tsize);
+        uint32_t o = o0;
+        while (o < oe) {
+            uint32_t u = 0;  // last unit (2w lists) starting at or before o
+            while ((u + 1) * 2 * w < G.nl && sOff[(u + 1) * 2 * w] <= o) ++u;
+            const uint32_t aL = u * 2 * w, bL = min(aL + w, G.nl), eL = min(aL + 2 * w, G.nl);
+            const uint32_t aS = sOff[aL], bS = sOff[bL], bE = sOff[eL];
+            const uint32_t cnt = min(oe, bE) - o;
+            kmerge_unit(V, sE[cur], sE[cur ^ 1], aS, bS - aS, bS, bE - bS, o - aS, cnt);
+            o += cnt;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (uint32_t i = threadIdx.x; i < tsize; i += kKT) {
+        const KElem e = sE[cur][kpad(i)];
+        const uint32_t id = e.id;
+        const uint64_t pos = G.out + B0 + i;
+        ord_out[pos] = id;
+        lcp_out[pos] = e.lcp;  // the tile's first output: step 4
+        if (saved) saved[pos] = rlcp[id];
+    }
+}
+
+// Step 4: the LCP of every tile's first output with the previous output (0 at
+// a group's start).
+__global__ void k_kmerge_fix(const KGroup* __restrict__ groups, uint32_t ngroups, uint64_t ntiles,
+                             const uint64_t* __restrict__ tile_b, const uint32_t* __restrict__ ord_out,
+                             uint32_t* __restrict__ lcp_out, const uint8_t* __restrict__ chars,
+                             const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
+                             const uint64_t* __restrict__ keyc, const uint64_t* __restrict__ keyc2,
+                             const uint32_t* __restrict__ rcp) {
+    const uint64_t tb = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tb >= ntiles) return;
+    uint32_t lo = 0, hi = ngroups;
+    while (lo + 1 < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (groups[mid].tile0 <= tb) lo = mid; else hi = mid;
+    }
+    const KGroup G = groups[lo];
+    const uint64_t b = tile_b[tb];
+    if (b >= G.n) return;
+    const uint64_t pos = G.out + b;
+    if (b == 0) {
+        lcp_out[pos] = 0;
+        return;
+    }
+    const ViewCtx V{chars, voff, vlen, keyc2, rcp[G.recv]};
+    const uint32_t a = ord_out[pos - 1], c = ord_out[pos];
+    uint32_t l = 0;
+    elem_cmp(V, a, keyc[a], c, keyc[c], V.cp, l);
+    lcp_out[pos] = l;
+}
+
+// Fan-in 2: the binary tree of the same node merges, one global pass per
+// tree level (merge<2> nested log2(runs) deep).  Every pass splits its pairs'
+// outputs into tiles of kTileOut by merge-path searches in global memory
+// (k_merge_split), then each tile's windows of the pair are staged in LDS and
+// merged (k_merge_tile).  No sampling and exact tiles: on one MI355X this is
+// the faster form where the merge is bound by the node merges' dependent LDS
+// latency rather than by HBM traffic (the D/N inputs); the tuner picks the
+// fan-in per input regime.
+constexpr uint32_t kMT = 512;               // threads per merge block
+constexpr uint32_t kMW = 2;                 // outputs per thread
+constexpr uint32_t kTileOut = kMT * kMW;    // outputs per tile
+
+struct MergePair {
+    uint64_t out;      // first receive position of the pair (A then B)
+    uint64_t na, nb;   // run lengths
+    uint64_t tile0;    // first tile of the pair
+    uint32_t recv;     // receiving PE
+    uint32_t pad;
+};
 
 __device__ inline uint32_t pair_of_tile(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t t) {
     uint32_t lo = 0, hi = npairs;
@@ -1132,6 +1489,7 @@ k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_
     }
 }
 
+
 __global__ void k_iota_copy(uint32_t* __restrict__ ord, uint32_t* __restrict__ l, const uint32_t* __restrict__ rl,
                             uint64_t n) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1166,82 +1524,204 @@ void merge(dss_set_s* s) {
     uint32_t* lc = P.m_lcp.as<uint32_t>();
     uint32_t* ord2 = P.m_order2.as<uint32_t>();
     uint32_t* lc2 = P.m_lcp2.as<uint32_t>();
-    // runs per receiver (host), then pairwise rounds; receivers that already
-    // hold a single run are carried along as one-run pairs
-    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> runs(p);
+    const uint32_t F = s->sort_params.merge_fan;
+    uint32_t lgF = 0;
+    while ((1u << lgF) < F) ++lgF;
+    // every receiver's non-empty runs, in source order
+    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> lists(p);
+    bool any = false;
     for (uint32_t q = 0; q < p; ++q) {
         uint64_t pos = P.recv_begin[q];
         for (uint32_t r = 0; r < p; ++r) {
             const uint64_t c = P.interval[(uint64_t)r * p + q];
-            if (c) runs[q].push_back({pos, c});
+            if (c) lists[q].push_back({pos, c});
             pos += c;
         }
+        any |= lists[q].size() > 1;
     }
-    // all rounds' pair lists are built first and uploaded once, so the
-    // rounds run back to back without host synchronisation
-    std::vector<MergePair> all;
-    std::vector<size_t> rbeg;          // first pair of each round
-    std::vector<uint64_t> rtiles, tbeg;  // tiles of each round, first split slot
-    uint64_t tsum = 0;
-    while (true) {
-        bool any = false;
-        for (uint32_t q = 0; q < p; ++q) any |= runs[q].size() > 1;
-        if (!any) break;
-        rbeg.push_back(all.size());
-        tbeg.push_back(tsum);
-        uint64_t tiles = 0;
-        for (uint32_t q = 0; q < p; ++q) {
-            auto& R = runs[q];
-            std::vector<std::pair<uint64_t, uint64_t>> next;
-            for (size_t k = 0; k < R.size(); k += 2) {
-                MergePair mp;
-                mp.out = R[k].first;
-                mp.na = R[k].second;
-                mp.nb = k + 1 < R.size() ? R[k + 1].second : 0;
-                mp.recv = q;
-                mp.pad = 0;
-                mp.tile0 = tiles;
-                tiles += (mp.na + mp.nb + kTileOut - 1) / kTileOut;
-                all.push_back(mp);
-                next.push_back({mp.out, mp.na + mp.nb});
+    if (any && F == 2) {
+        // fan-in 2: all rounds' pair lists are built first and uploaded
+        // once, so the rounds run back to back without host synchronisation
+        auto& runs = lists;
+        std::vector<MergePair> all;
+        std::vector<size_t> rbeg;          // first pair of each round
+        std::vector<uint64_t> rtiles, tbeg;  // tiles of each round, first split slot
+        uint64_t tsum = 0;
+        while (true) {
+            bool more = false;
+            for (uint32_t q = 0; q < p; ++q) more |= runs[q].size() > 1;
+            if (!more) break;
+            rbeg.push_back(all.size());
+            tbeg.push_back(tsum);
+            uint64_t tiles = 0;
+            for (uint32_t q = 0; q < p; ++q) {
+                auto& R = runs[q];
+                std::vector<std::pair<uint64_t, uint64_t>> next;
+                for (size_t k = 0; k < R.size(); k += 2) {
+                    MergePair mp;
+                    mp.out = R[k].first;
+                    mp.na = R[k].second;
+                    mp.nb = k + 1 < R.size() ? R[k + 1].second : 0;
+                    mp.recv = q;
+                    mp.pad = 0;
+                    mp.tile0 = tiles;
+                    tiles += (mp.na + mp.nb + kTileOut - 1) / kTileOut;
+                    all.push_back(mp);
+                    next.push_back({mp.out, mp.na + mp.nb});
+                }
+                R.swap(next);
             }
-            R.swap(next);
+            rtiles.push_back(tiles);
+            tsum += tiles + 1;
         }
-        rtiles.push_back(tiles);
-        tsum += tiles + 1;
-    }
-    rbeg.push_back(all.size());
-    if (!all.empty()) {
-        const uint64_t pair_bytes = (all.size() * sizeof(MergePair) + 15) / 16 * 16;
-        P.m_work.reserve(pair_bytes + (tsum + 1) * 8);
-        MergePair* dpairs = P.m_work.as<MergePair>();
-        uint64_t* splits = reinterpret_cast<uint64_t*>(P.m_work.as<uint8_t>() + pair_bytes);
-        DSS_HIP(hipMemcpyAsync(dpairs, all.data(), all.size() * sizeof(MergePair), hipMemcpyHostToDevice, st));
-        // round 0 reads the receive order and LCPs directly; round r writes
-        // the buffer pair round r - 1 did not
+        rbeg.push_back(all.size());
+        if (!all.empty()) {
+            const uint64_t pair_bytes = (all.size() * sizeof(MergePair) + 15) / 16 * 16;
+            P.m_work.reserve(pair_bytes + (tsum + 1) * 8);
+            MergePair* dpairs = P.m_work.as<MergePair>();
+            uint64_t* splits = reinterpret_cast<uint64_t*>(P.m_work.as<uint8_t>() + pair_bytes);
+            DSS_HIP(hipMemcpyAsync(dpairs, all.data(), all.size() * sizeof(MergePair), hipMemcpyHostToDevice, st));
+            // round 0 reads the receive order and LCPs directly; round r writes
+            // the buffer pair round r - 1 did not
+            const uint32_t* in_ord = nullptr;
+            const uint32_t* in_lc = P.recv_lcp.as<uint32_t>();
+            for (size_t r = 0; r + 1 < rbeg.size(); ++r) {
+                const uint32_t np = (uint32_t)(rbeg[r + 1] - rbeg[r]);
+                const uint64_t tiles = rtiles[r];
+                MergePair* rp = dpairs + rbeg[r];
+                uint64_t* split = splits + tbeg[r];
+                const bool last = r + 2 == rbeg.size();
+                KScope ks(s->timer, st, "merge", n * 24.0);
+                hipLaunchKernelGGL(k_merge_split, dim3((tiles * 4 + 255) / 256), dim3(256), 0, st, rp, np, tiles, in_ord,
+                                   s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                                   P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp, split);
+                hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, rp, np, split, in_ord, in_lc, ord, lc,
+                                   s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                                   P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp, P.recv_lcp.as<uint32_t>(),
+                                   last ? P.m_saved.as<uint32_t>() : nullptr);
+                in_ord = ord;
+                in_lc = lc;
+                std::swap(ord, ord2);
+                std::swap(lc, lc2);
+            }
+            DSS_HIP(hipGetLastError());
+            // results in the last round's output pair
+            ord = const_cast<uint32_t*>(in_ord);
+            lc = const_cast<uint32_t*>(in_lc);
+        }
+    } else if (any) {
+        // all passes' groups and lists are built first and uploaded once, so
+        // the passes run back to back without host synchronisation; a
+        // receiver whose lists are already merged is carried along as a
+        // one-list group (a copy into the pass's output buffers)
+        struct Pass {
+            size_t g0, ng, l0, nl;
+            uint64_t nsamp, ntiles, n, max_list_samp;
+        };
+        std::vector<KGroup> G;
+        std::vector<KList> L;
+        std::vector<Pass> passes;
+        while (any) {
+            Pass ps{G.size(), 0, L.size(), 0, 0, 0, 0, 0};
+            any = false;
+            for (uint32_t q = 0; q < p; ++q) {
+                auto& R = lists[q];
+                std::vector<std::pair<uint64_t, uint64_t>> next;
+                for (size_t k = 0; k < R.size(); k += F) {
+                    const uint32_t nl = (uint32_t)std::min<size_t>(F, R.size() - k);
+                    KGroup g{};
+                    g.out = R[k].first;
+                    g.list0 = (uint32_t)(L.size() - ps.l0);
+                    g.nl = nl;
+                    g.recv = q;
+                    g.S = (kKG - 1) / nl + 1;
+                    g.tile0 = ps.ntiles;
+                    for (uint32_t i = 0; i < nl; ++i) {
+                        KList l{};
+                        l.start = R[k + i].first;
+                        l.size = R[k + i].second;
+                        l.samp0 = (ps.nsamp + kKCoarse - 1) / kKCoarse * kKCoarse;  // coarse-aligned
+                        l.nsamp = (l.size + g.S - 1) / g.S;
+                        l.group = (uint32_t)(G.size() - ps.g0);
+                        l.slot = i;
+                        ps.nsamp = l.samp0 + l.nsamp;
+                        ps.max_list_samp = std::max(ps.max_list_samp, l.nsamp);
+                        g.n += l.size;
+                        L.push_back(l);
+                    }
+                    g.ntiles = (g.n + kKG - 1) / kKG;
+                    ps.ntiles += g.ntiles;
+                    ps.n += g.n;
+                    G.push_back(g);
+                    next.push_back({g.out, g.n});
+                }
+                R.swap(next);
+                any |= R.size() > 1;
+            }
+            ps.ng = G.size() - ps.g0;
+            ps.nl = L.size() - ps.l0;
+            passes.push_back(ps);
+        }
+        uint64_t max_samp = 0, max_tiles = 0;
+        for (const Pass& ps : passes) {
+            max_samp = std::max(max_samp, ps.nsamp);
+            max_tiles = std::max(max_tiles, ps.ntiles);
+        }
+        const uint64_t gbytes = (G.size() * sizeof(KGroup) + 255) / 256 * 256;
+        const uint64_t lbytes = (L.size() * sizeof(KList) + 255) / 256 * 256;
+        const uint64_t cbytes = (max_samp * F * 4 + 255) / 256 * 256;
+        const uint64_t rbytes = (max_samp * 8 + 255) / 256 * 256;
+        const uint64_t tbytes = (max_tiles * 8 + 255) / 256 * 256;
+        const uint64_t cobytes = (max_tiles * F * 4 + 255) / 256 * 256;
+        P.m_work.reserve(gbytes + lbytes + cbytes + rbytes + tbytes + cobytes + max_tiles * 4 + 256);
+        uint8_t* w = P.m_work.as<uint8_t>();
+        KGroup* dG = reinterpret_cast<KGroup*>(w);
+        KList* dL = reinterpret_cast<KList*>(w + gbytes);
+        uint32_t* corank = reinterpret_cast<uint32_t*>(w + gbytes + lbytes);
+        uint64_t* rankv = reinterpret_cast<uint64_t*>(w + gbytes + lbytes + cbytes);
+        uint64_t* tile_b = reinterpret_cast<uint64_t*>(w + gbytes + lbytes + cbytes + rbytes);
+        uint32_t* tile_co = reinterpret_cast<uint32_t*>(w + gbytes + lbytes + cbytes + rbytes + tbytes);
+        uint32_t* tile_g = reinterpret_cast<uint32_t*>(w + gbytes + lbytes + cbytes + rbytes + tbytes + cobytes);
+        DSS_HIP(hipMemcpyAsync(dG, G.data(), G.size() * sizeof(KGroup), hipMemcpyHostToDevice, st));
+        DSS_HIP(hipMemcpyAsync(dL, L.data(), L.size() * sizeof(KList), hipMemcpyHostToDevice, st));
+        // pass 0 reads the receive order and LCPs directly; pass r writes the
+        // buffer pair pass r - 1 did not; the last also writes the saved LCPs
         const uint32_t* in_ord = nullptr;
         const uint32_t* in_lc = P.recv_lcp.as<uint32_t>();
-        for (size_t r = 0; r + 1 < rbeg.size(); ++r) {
-            const uint32_t np = (uint32_t)(rbeg[r + 1] - rbeg[r]);
-            const uint64_t tiles = rtiles[r];
-            MergePair* rp = dpairs + rbeg[r];
-            uint64_t* split = splits + tbeg[r];
-            const bool last = r + 2 == rbeg.size();
-            KScope ks(s->timer, st, "merge_pairs", n * 24.0);
-            hipLaunchKernelGGL(k_merge_split, dim3((tiles * 4 + 255) / 256), dim3(256), 0, st, rp, np, tiles, in_ord,
-                               s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
-                               P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp, split);
-            hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, rp, np, split, in_ord, in_lc, ord, lc,
-                               s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+        for (size_t r = 0; r < passes.size(); ++r) {
+            const Pass& ps = passes[r];
+            const bool last = r + 1 == passes.size();
+            const double bytes = ps.n * (12.0 + (in_ord ? 4.0 : 0.0) + 8.0 + (last ? 8.0 : 0.0));
+            KScope ks(s->timer, st, "merge", bytes);
+            for (int coarse = 1; coarse >= 0; --coarse) {
+                const uint64_t per_list = coarse ? (ps.max_list_samp + kKCoarse - 1) / kKCoarse : ps.max_list_samp;
+                const uint64_t per_block = 256 >> lgF;
+                hipLaunchKernelGGL(k_kmerge_sample, dim3((uint32_t)((per_list + per_block - 1) / per_block),
+                                                         (uint32_t)ps.nl), dim3(256), 0, st,
+                                   dG + ps.g0, dL + ps.l0, lgF, coarse, in_ord,
+                                   s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                                   P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp, corank, rankv);
+            }
+            hipLaunchKernelGGL(k_kmerge_bounds, dim3((uint32_t)((ps.ntiles * kKMaxFan + 255) / 256)), dim3(256), 0,
+                               st, dG + ps.g0, (uint32_t)ps.ng, ps.ntiles, dL + ps.l0, lgF, corank, rankv, tile_b,
+                               tile_co, tile_g);
+            hipLaunchKernelGGL(k_kmerge_tile, dim3((uint32_t)ps.ntiles), dim3(kKT), 0, st, dG + ps.g0,
+                               (uint32_t)ps.ng, dL + ps.l0, lgF, tile_b, tile_co, tile_g, in_ord, in_lc, ord, lc,
+                               s->arena(),
+                               P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
                                P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp, P.recv_lcp.as<uint32_t>(),
                                last ? P.m_saved.as<uint32_t>() : nullptr);
+            hipLaunchKernelGGL(k_kmerge_fix, dim3((uint32_t)((ps.ntiles + 255) / 256)), dim3(256), 0, st, dG + ps.g0,
+                               (uint32_t)ps.ng, ps.ntiles, tile_b, ord, lc, s->arena(),
+                               P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                               P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp);
             in_ord = ord;
             in_lc = lc;
             std::swap(ord, ord2);
             std::swap(lc, lc2);
         }
         DSS_HIP(hipGetLastError());
-        // results in the last round's output pair
+        // results in the last pass's output pair
         ord = const_cast<uint32_t*>(in_ord);
         lc = const_cast<uint32_t*>(in_lc);
     } else if (n) {

@@ -1347,10 +1347,14 @@ int dss_set_sort_params(dss_set s, const dss_sort_params_t* params) {
             DSS_REQUIRE(params->skip_lanes == 8 || params->skip_lanes == 16, DSS_EINVAL, "skip_lanes must be 8 or 16");
             DSS_REQUIRE(params->skip_strings <= 4 && params->skip_strings != 3, DSS_EINVAL,
                         "skip_strings must be 0 (default), 1, 2 or 4");
+            DSS_REQUIRE(params->merge_fan_in == 0 || params->merge_fan_in == 2 || params->merge_fan_in == 4 ||
+                            params->merge_fan_in == 8 || params->merge_fan_in == 16,
+                        DSS_EINVAL, "merge_fan_in must be 0 (default), 2, 4, 8 or 16");
             T.large_min = params->large_min;
             T.wave_class_max = params->wave_class_max;
             T.skip_lanes = params->skip_lanes;
             if (params->skip_strings) T.skip_strings = params->skip_strings;
+            if (params->merge_fan_in) T.merge_fan = params->merge_fan_in;
         }
         s->sort_params = T;
     });
@@ -1363,6 +1367,7 @@ int dss_get_sort_params(dss_set s, dss_sort_params_t* params) {
         params->wave_class_max = s->sort_params.wave_class_max;
         params->skip_lanes = s->sort_params.skip_lanes;
         params->skip_strings = s->sort_params.skip_strings;
+        params->merge_fan_in = s->sort_params.merge_fan;
     });
 }
 

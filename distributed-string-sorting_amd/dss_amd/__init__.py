@@ -175,7 +175,7 @@ class DeviceStringSet:
         t = _lib.SortParams()
         check(lib().dss_get_sort_params(self._h, C.byref(t)))
         return dict(large_min=t.large_min, wave_class_max=t.wave_class_max, skip_lanes=t.skip_lanes,
-                    skip_strings=t.skip_strings)
+                    skip_strings=t.skip_strings, merge_fan_in=t.merge_fan_in)
 
     def set_sort_params(self, params=None):
         """Local-sort tuning (large_min, wave_class_max, skip_lanes,
@@ -186,7 +186,8 @@ class DeviceStringSet:
             return
         cur = self.sort_params()
         cur.update(params)
-        t = _lib.SortParams(cur["large_min"], cur["wave_class_max"], cur["skip_lanes"], cur["skip_strings"])
+        t = _lib.SortParams(cur["large_min"], cur["wave_class_max"], cur["skip_lanes"], cur["skip_strings"],
+                            cur.get("merge_fan_in", 0))
         check(lib().dss_set_sort_params(self._h, C.byref(t)))
 
     def _per_string(self, a, dtype, what):

@@ -1,12 +1,15 @@
-"""Autotuner of the local sort's parameters, keyed by input regime.
+"""Autotuner of the sort's runtime parameters, keyed by input regime.
 
 A regime is (log2 of the strings per PE, log2 of the mean string length,
 tenths of the mean LCP over the mean length, number of PEs): inputs in one
-regime share their MSD pass structure and base-case sizes.  `autotune` times
-the sort under each candidate setting on the given set (every setting yields
-the same output, so any set of the regime serves as the sample) and records
-the fastest in a cache that can be saved as JSON; `apply` sets the cached
-setting for a new set of a known regime.
+regime share their MSD pass structure, base-case sizes and merge shape.
+`autotune` times the local sort under each candidate setting (base-case
+threshold, one-wave base-case classes, prefix-skip lanes and strings), then,
+with more than one PE, the LCP merge of the exchanged runs under each merge
+fan-in (2, 4, 8, 16 runs per pass); every setting yields the same output, so
+any set of the regime serves as the sample.  The fastest setting is recorded
+in a cache that can be saved as JSON; `apply` sets the cached setting for a
+new set of a known regime.
 """
 from __future__ import annotations
 
@@ -15,13 +18,14 @@ import math
 import os
 import time
 
-DEFAULT = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4}
+DEFAULT = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 2}
 
 CANDIDATES = [
     {"large_min": lm, "wave_class_max": wc, "skip_lanes": sk, "skip_strings": ss}
     for lm in (256, 512, 1024) for wc in (5, 6)
     for sk, ss in ((8, 1), (8, 2), (8, 4), (16, 1), (16, 2))
 ]
+MERGE_FANS = (2, 4, 8, 16)
 
 
 def regime(info: dict, avg_lcp: int | None = None) -> str:
@@ -54,10 +58,23 @@ class Tuner:
             avg = None
         return regime(info, avg)
 
-    def autotune(self, ds, candidates=None, repeats: int = 2) -> dict:
-        """Times ds.sort_lcp() under every candidate; keeps the fastest for
-        the set's regime.  Returns {"regime", "best", "times_ms"}."""
-        candidates = candidates or CANDIDATES
+    @staticmethod
+    def _time(ds, run, repeats):
+        best = math.inf
+        for _ in range(repeats):
+            ds.sync()
+            t0 = time.perf_counter()
+            run()
+            ds.sync()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    def autotune(self, ds, candidates=None, repeats: int = 2, fans=MERGE_FANS) -> dict:
+        """Times ds.sort_lcp() under every candidate, then (more than one PE)
+        ds.merge_lcp() of the exchanged runs under every merge fan-in; keeps
+        the fastest for the set's regime.  Returns {"regime", "best",
+        "times_ms", "merge_times_ms"}."""
+        candidates = [dict(c, merge_fan_in=DEFAULT["merge_fan_in"]) for c in (candidates or CANDIDATES)]
         ds.set_sort_params(DEFAULT)
         ds.sort_lcp()  # warm up buffers; also gives the LCPs for the regime key
         ds.sync()
@@ -65,19 +82,26 @@ class Tuner:
         times = []
         for cand in candidates:
             ds.set_sort_params(cand)
-            best = math.inf
-            for _ in range(repeats):
-                ds.sync()
-                t0 = time.perf_counter()
-                ds.sort_lcp()
-                ds.sync()
-                best = min(best, time.perf_counter() - t0)
-            times.append((best, cand))
+            times.append((self._time(ds, ds.sort_lcp, repeats), cand))
         times.sort(key=lambda t: t[0])
-        self.table[key] = dict(times[0][1])
-        ds.set_sort_params(times[0][1])
-        return {"regime": key, "best": dict(times[0][1]),
-                "times_ms": [(round(t * 1e3, 3), c) for t, c in times]}
+        best = dict(times[0][1])
+        merge_times = []
+        if ds.p > 1 and fans:
+            ds.set_sort_params(best)
+            ds.sort_lcp()
+            ds.partition("IndexedNumStrings", max_len=2 * 100 * (ds.avg_lcp() + 5))
+            ds.exchange("whole")
+            for fan in fans:
+                ds.set_sort_params(dict(best, merge_fan_in=fan))
+                ds.merge_lcp()  # first call of a shape: buffers
+                merge_times.append((self._time(ds, ds.merge_lcp, repeats), fan))
+            merge_times.sort()
+            best["merge_fan_in"] = merge_times[0][1]
+        self.table[key] = best
+        ds.set_sort_params(best)
+        return {"regime": key, "best": dict(best),
+                "times_ms": [(round(t * 1e3, 3), c) for t, c in times],
+                "merge_times_ms": [(round(t * 1e3, 3), f) for t, f in merge_times]}
 
     def apply(self, ds) -> bool:
         """Sets the cached parameters of the set's regime (after one sort the

@@ -139,13 +139,18 @@ int dss_get_sorted(dss_set s, uint32_t* perm, uint32_t* lcp);
  * case; base-case classes up to 2^wave_class_max strings (0..6, default 6)
  * are sorted one segment per wave; skip_lanes (8 or 16, default 8) lanes
  * compare each string in the common-prefix skip, skip_strings (1, 2 or 4,
- * default 4; 0 selects the default) strings per lane group at a time.  Every setting
- * produces the same output.  NULL restores the defaults. */
+ * default 4; 0 selects the default) strings per lane group at a time;
+ * merge_fan_in (2, 4, 8 or 16, default 2; 0 selects the default) runs are
+ * merged per pass of dss_merge_lcp (the K of merge<K>, merging.hpp:23-63:
+ * more runs than that take several passes; 2 is the pairwise merge tree, one
+ * pass per level, larger values merge K runs per pass in LDS tiles).  Every setting produces the
+ * same output.  NULL restores the defaults. */
 typedef struct {
     uint32_t large_min;
     uint32_t wave_class_max;
     uint32_t skip_lanes;
     uint32_t skip_strings;
+    uint32_t merge_fan_in;
 } dss_sort_params_t;
 
 int dss_set_sort_params(dss_set s, const dss_sort_params_t* params);

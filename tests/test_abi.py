@@ -73,6 +73,6 @@ def test_tuner_regime_and_cache(tmp_path):
     assert regime(info, avg_lcp=249) == "n2^17/len2^8/lcp4/p4"
     path = tmp_path / "tune.json"
     t = Tuner(str(path))
-    t.table["x"] = {"large_min": 256, "wave_class_max": 5, "skip_lanes": 16, "skip_strings": 2}
+    t.table["x"] = {"large_min": 256, "wave_class_max": 5, "skip_lanes": 16, "skip_strings": 2, "merge_fan_in": 8}
     t.save()
     assert Tuner(str(path)).table == t.table

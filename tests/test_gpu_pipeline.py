@@ -227,6 +227,44 @@ def test_drivers(name):
     assert canon(got_pairs, eqm) == canon(exp_pairs, eqm)
 
 
+@pytest.mark.parametrize("name", sorted(INPUTS))
+@pytest.mark.parametrize("p", [5, 16, 37])
+def test_merge_fan_in_gives_the_same_output(name, p):
+    """The K of the K-way merge (runs merged per pass: 2, 4, 8, 16) changes
+    the number of passes, not the output: merged order, LCPs and saved LCPs
+    are identical for every fan-in (ties go to the earlier run), the merged
+    strings equal the oracle's loser tree and the checker passes."""
+    dss = _dss()
+    chars = INPUTS[name]
+    ss = O.StringSet(chars, p)
+    operm, olcp = O.local_sort(ss)
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        ds.sort_lcp()
+        iv = ds.partition("IndexedNumStrings", max_len=1000)
+        for mode in ("whole", "lcp_compact"):
+            rb, bb, nbytes = ds.exchange(mode)
+            ex = O.exchange(ss, operm, olcp, iv, mode)
+            ref = None
+            for fan in (16, 8, 4, 2):
+                ds.set_sort_params({"merge_fan_in": fan})
+                ds.merge_lcp()
+                got = ds.get_merged()
+                if ref is None:
+                    ref = got
+                    for q in range(p):
+                        r0, r1 = int(rb[q]), int(rb[q + 1])
+                        b0, b1 = int(bb[q]), int(bb[q + 1])
+                        oo, ol, osv = O.merge_losertree(ex["bytes"][b0:b1], ex["lcp"][r0:r1],
+                                                        iv[:, q].astype(np.uint64), mode == "lcp_compact")
+                        assert (got[1][r0:r1] == ol).all(), (mode, q)
+                else:
+                    for a, b in zip(got, ref):
+                        assert (a == b).all(), (mode, fan)
+                chk = ds.check()
+                assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0, (mode, fan)
+        ds.set_sort_params(None)
+
+
 def test_few_strings_many_pes():
     dss = _dss()
     chars = arena([b"b", b"a", b"c", b"a"])

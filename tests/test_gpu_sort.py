@@ -119,14 +119,14 @@ def test_skewed_generator_shapes(n, ll, sl, ldn, sdn, lp, dp):
 
 
 PARAM_SETS = [
-    {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8, "skip_strings": 1},
-    {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 2},
-    {"large_min": 256, "wave_class_max": 5, "skip_lanes": 8, "skip_strings": 4},
-    {"large_min": 1024, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 1},
-    {"large_min": 128, "wave_class_max": 4, "skip_lanes": 8, "skip_strings": 2},
-    {"large_min": 32, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 4},
+    {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8, "skip_strings": 1, "merge_fan_in": 2},
+    {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 2, "merge_fan_in": 4},
+    {"large_min": 256, "wave_class_max": 5, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 8},
+    {"large_min": 1024, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 1, "merge_fan_in": 16},
+    {"large_min": 128, "wave_class_max": 4, "skip_lanes": 8, "skip_strings": 2, "merge_fan_in": 2},
+    {"large_min": 32, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 4, "merge_fan_in": 16},
 ]
-DEFAULT_PARAMS = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4}
+DEFAULT_PARAMS = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 2}
 
 
 @pytest.mark.parametrize("name", ["dton", "skewed", "suffix"])
@@ -160,7 +160,10 @@ def test_autotuner_picks_and_applies_a_setting(tmp_path):
     tuner = Tuner(str(tmp_path / "t.json"))
     with dss.DeviceStringSet.dton(200000, 100, 0.5, 34, 4) as ds:
         out = tuner.autotune(ds, repeats=1)
-        assert out["best"] in [c for _, c in out["times_ms"]]
+        sort_best = {k: v for k, v in out["best"].items() if k != "merge_fan_in"}
+        assert sort_best in [{k: v for k, v in c.items() if k != "merge_fan_in"} for _, c in out["times_ms"]]
+        assert out["best"]["merge_fan_in"] == out["merge_times_ms"][0][1]
+        assert sorted(f for _, f in out["merge_times_ms"]) == [2, 4, 8, 16]
         assert ds.sort_params() == out["best"]
         tuner.save()
     with dss.DeviceStringSet.dton(200000, 100, 0.5, 35, 4) as ds:
@@ -174,3 +177,5 @@ def test_autotuner_picks_and_applies_a_setting(tmp_path):
             ds.set_sort_params({"skip_lanes": 12})
         with pytest.raises(dss.DssError):
             ds.set_sort_params({"skip_strings": 3})
+        with pytest.raises(dss.DssError):
+            ds.set_sort_params({"merge_fan_in": 3})
