@@ -270,7 +270,8 @@ struct SortParams {
 
 struct dss_set_s {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream all work is queued on
+    hipStream_t own_stream = nullptr;  // the set's own stream (stream unless the caller set one)
     uint64_t n = 0;        // strings
     uint64_t nbytes = 0;   // arena bytes (incl. NULs)
     uint64_t slice_bytes = 0;  // bytes the PE split slices (0: nbytes); a text file's size

@@ -138,12 +138,21 @@ dss_set_s* new_set(uint32_t num_pes) {
     dss_set_s* s = new dss_set_s();
     s->p = num_pes;
     DSS_HIP(hipGetDevice(&s->device));
-    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    hipError_t e = hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete s;
         throw Error{DSS_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
     }
+    s->stream = s->own_stream;
     return s;
+}
+
+// Frees a set and its own stream (error paths of the constructors).
+void free_set(dss_set_s* s) {
+    hipStream_t st = s->own_stream;
+    (void)hipStreamSynchronize(s->stream);
+    delete s;
+    (void)hipStreamDestroy(st);
 }
 
 // ------------------------------------------------------ arena indexing
@@ -645,7 +654,7 @@ int dss_set_create_suffixes(const uint8_t* text, uint64_t text_len, uint32_t cap
             if (text_len) DSS_HIP(hipMemcpyAsync(s->chars.p, text, text_len, hipMemcpyHostToDevice, s->stream));
             make_suffix_views(s, text_len, cap);
         } catch (...) {
-            delete s;
+            free_set(s);
             throw;
         }
         *out = s;
@@ -679,7 +688,7 @@ int dss_set_create_markov_suffixes(uint64_t text_len, uint32_t repeats, uint32_t
             }
             make_suffix_views(s, text_len, cap);
         } catch (...) {
-            delete s;
+            free_set(s);
             throw;
         }
         *out = s;
@@ -715,7 +724,7 @@ int dss_set_create(const uint8_t* chars, uint64_t nbytes, uint32_t num_pes, dss_
             index_arena(s);
             compute_pe_split(s);
         } catch (...) {
-            delete s;
+            free_set(s);
             throw;
         }
         *out = s;
@@ -769,7 +778,7 @@ int dss_set_create_text(const uint8_t* text, uint64_t text_len, uint64_t max_siz
             if (L) DSS_HIP(hipMemcpyAsync(s->chars.p, text, L, hipMemcpyHostToDevice, s->stream));
             finish_text(s, L, L ? text[L - 1] : 0);
         } catch (...) {
-            delete s;
+            free_set(s);
             throw;
         }
         *out = s;
@@ -821,7 +830,7 @@ int dss_set_load_file(const char* path, uint64_t max_size, uint32_t num_pes, dss
             finish_text(s, L, last);
         } catch (...) {
             cleanup();
-            delete s;
+            free_set(s);
             throw;
         }
         cleanup();
@@ -859,7 +868,7 @@ int dss_set_create_dton(uint64_t n, uint32_t len, double dton, uint64_t seed, ui
             DSS_HIP(hipGetLastError());
             compute_pe_split(s);
         } catch (...) {
-            delete s;
+            free_set(s);
             throw;
         }
         *out = s;
@@ -894,7 +903,7 @@ int dss_set_create_skewed(uint64_t n, uint32_t long_len, uint32_t short_len, dou
             DSS_HIP(hipGetLastError());
             compute_pe_split(s);
         } catch (...) {
-            delete s;
+            free_set(s);
             throw;
         }
         *out = s;
@@ -905,9 +914,17 @@ int dss_set_destroy(dss_set s) {
     return guarded([&] {
         if (!s) return;
         (void)hipStreamSynchronize(s->stream);
-        hipStream_t st = s->stream;
+        hipStream_t st = s->own_stream;  // a caller's stream stays the caller's
         delete s;
         (void)hipStreamDestroy(st);
+    });
+}
+
+int dss_set_stream(dss_set s, void* stream) {
+    return guarded([&] {
+        DSS_REQUIRE(s, DSS_EINVAL, "NULL set");
+        DSS_HIP(hipStreamSynchronize(s->stream));  // work queued so far completes on its stream
+        s->stream = stream ? static_cast<hipStream_t>(stream) : s->own_stream;
     });
 }
 
@@ -1038,7 +1055,7 @@ extern "C" int dss_set_create_pes(const uint8_t* chars, const uint64_t* pe_byte_
             for (auto v : lens) mx = std::max(mx, v);
             s->max_len = mx;
         } catch (...) {
-            delete s;
+            free_set(s);
             throw;
         }
         *out = s;
@@ -1374,9 +1391,7 @@ int dss_pool_batch(dss_pool P, const uint32_t* jobs, uint32_t count, dss_set* ou
         try {
             pool_fill_batch(P, s, jobs, count);
         } catch (...) {
-            hipStream_t st = s->stream;
-            delete s;
-            (void)hipStreamDestroy(st);
+            free_set(s);
             throw;
         }
         *out = s;

@@ -155,6 +155,11 @@ class DeviceStringSet:
                                      _p(ln, _lib.u32p)))
         return chars[:info["num_bytes"]], off[:info["num_strings"]], ln[:info["num_strings"]]
 
+    def set_stream(self, stream: int | None):
+        """Queue the set's work on a caller's HIP stream (an int handle), or
+        on its own stream again (None)."""
+        check(lib().dss_set_stream(self._h, stream))
+
     # ------------------------------------------------------------ stages
     def sort_lcp(self):
         """Local LCP sort of every PE (tlx radixsort_CI3 call sites)."""

@@ -72,6 +72,7 @@ _SIGS = {
     "dss_set_create_markov_suffixes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                                  C.c_uint64, C.c_uint32, C.POINTER(set_p)]),
     "dss_set_destroy": (C.c_int, [set_p]),
+    "dss_set_stream": (C.c_int, [set_p, C.c_void_p]),
     "dss_set_info": (C.c_int, [set_p, C.POINTER(SetInfo)]),
     "dss_set_pe_begin": (C.c_int, [set_p, u64p]),
     "dss_set_download": (C.c_int, [set_p, u8p, u64p, u32p]),

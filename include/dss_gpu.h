@@ -107,6 +107,14 @@ int dss_set_create_markov_suffixes(uint64_t text_len, uint32_t repeats, uint32_t
 
 int dss_set_destroy(dss_set s);
 
+/* The stream the set queues its work on: a caller's hipStream_t (passed as
+ * void*), or NULL for the set's own stream (the default).  Work queued so far
+ * finishes first.  The set never destroys a caller's stream; the caller keeps
+ * it alive while the set uses it.  No counterpart in the reference (one MPI
+ * rank is one thread); the explicit-stream form of the ABI, for callers that
+ * order the set's work with their own. */
+int dss_set_stream(dss_set s, void* stream);
+
 typedef struct {
     uint64_t num_strings;
     uint64_t num_bytes;   /* arena bytes incl. NULs */

@@ -81,7 +81,17 @@ class HipMem:
         self.h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         self.h.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         self.h.hipDeviceSynchronize.argtypes = []
+        self.h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipStreamDestroy.argtypes = [C.c_void_p]
+        self.h.hipStreamSynchronize.argtypes = [C.c_void_p]
         self.bufs = []
+        self.streams = []
+
+    def stream(self) -> int:
+        s = self.C.c_void_p()
+        self._ok(self.h.hipStreamCreate(self.C.byref(s)), "hipStreamCreate")
+        self.streams.append(s.value)
+        return s.value
 
     def _ok(self, rc, what):
         if rc != 0:
@@ -118,6 +128,10 @@ class HipMem:
         for p in self.bufs:
             self.h.hipFree(p)
         self.bufs = []
+        for st in self.streams:
+            self.h.hipStreamSynchronize(st)
+            self.h.hipStreamDestroy(st)
+        self.streams = []
 
     def __enter__(self):
         return self

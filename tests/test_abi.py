@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "distributed-string-sorting_amd", "dss_amd", "libdss_gpu.so")
 
@@ -76,3 +78,50 @@ def test_tuner_regime_and_cache(tmp_path):
     t.table["x"] = {"large_min": 256, "wave_class_max": 5, "skip_lanes": 16, "skip_strings": 2, "merge_fan_in": 8}
     t.save()
     assert Tuner(str(path)).table == t.table
+
+
+REFERENCE = "/root/reference"
+ADAPTOR_TU = r'''
+#include <cstdint>
+#include <vector>
+#include "strings/stringcontainer.hpp"
+#include "dss_gpu_adaptors.hpp"
+
+// the shape of StringIndexPEIndex (src/util/structs.hpp:9-14, whose header
+// pulls in MPI)
+struct Entry {
+    size_t stringIndex, PEIndex;
+    Entry(size_t s, size_t p) : stringIndex(s), PEIndex(p) {}
+};
+using SSet = dss_schimek::UCharLengthStringSet;
+
+int main(int argc, char**) {
+    if (argc < 100) return 0;  // built and linked here, never run (no GPU)
+    std::vector<unsigned char> raw = {'b', 0, 'a', 0};
+    dss_schimek::StringLcpContainer<SSet> cont(std::move(raw));
+    auto ptr = cont.make_string_lcp_ptr();
+    dss_gpu::radixsort_CI3(ptr, 0, 0);  // the call of lcp_aware...:218-224
+    std::vector<uint64_t> hashes;
+    dss_gpu::hash_round(nullptr, {0, 1}, 8, hashes);
+    auto perm = dss_gpu::prefix_doubling_sort<Entry>(cont.raw_strings().data(), {0, cont.char_size()});
+    return (int)perm.size();
+}
+'''
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "src")), reason="reference checkout not present")
+def test_reference_side_adaptors_compile_and_link(tmp_path):
+    """include/dss_gpu_adaptors.hpp (the drop-ins of INTEGRATION.md)
+    instantiated with the reference's own StringLcpContainer / StringLcpPtr
+    compiles against the reference headers and links against libdss_gpu.so,
+    so a change of either side's interface fails here."""
+    import subprocess
+    tlx = os.path.join(REFERENCE, "NeuKaDiS/KaDiS/external/RBC/external/tlx")
+    src = tmp_path / "use_adaptors.cpp"
+    src.write_text(ADAPTOR_TU)
+    libdir = os.path.dirname(LIB)
+    cmd = ["g++", "-std=c++17", "-DNDEBUG", "-w", "-I", os.path.join(REFERENCE, "src"), "-I", tlx,
+           "-I", os.path.join(ROOT, "include"), str(src), "-o", str(tmp_path / "use_adaptors"),
+           f"-L{libdir}", "-ldss_gpu", f"-Wl,-rpath,{libdir}", "-Wl,--allow-shlib-undefined"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]

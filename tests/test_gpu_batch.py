@@ -194,3 +194,39 @@ def test_reserved_batches_give_the_same_results():
                     D, _, _, (pp, pl) = plain[r.jid]
                     assert r.D == D and (r.perm == pp).all() and (r.lcp == pl).all()
         assert pool.usage()["bytes_used"] == 0
+
+
+def _spec_chars(s):
+    """Host copy of a job spec's strings from the oracle's generators."""
+    if s.kind == "dton":
+        return O.gen_dton(s.n, s.length, s.dton, s.seed)
+    return O.gen_skewed(O.skew_params(s.n, long_len=s.length, short_len=s.short_len, long_dn=s.dton,
+                                      short_dn=s.short_dn), s.seed)
+
+
+def test_server_large_jobs_alone_path_matches_plain_and_oracle():
+    """With the large-job threshold lowered to 20k strings, jobs above it
+    take worker 0's alone path while the smaller ones are batched on the
+    other worker; every job's sorted order, LCP array and D are identical to
+    its plain (alone) sort, and the large jobs' equal the oracle's."""
+    dss = _dss()
+    from dss_amd.serve import Server, job_stream, run_plain
+    specs = job_stream(30, 1000, 150000, 10, 300, rate=300.0, seed=0x5EED0015)
+    large = 20000
+    assert sum(s.n >= large for s in specs) >= 3 and sum(s.n < large for s in specs) >= 3
+    with dss.JobPool(512 << 20, 4 << 20) as pool:
+        plain = run_plain(pool, specs, keep_results=True)
+        with Server(pool, inflight=2, max_batch_strings=100000, large_strings=large) as srv:
+            res = srv.run(specs, realtime=True, keep_results=True)
+        assert pool.usage()["bytes_used"] == 0
+    by = {s.jid: s for s in specs}
+    for r in res:
+        D, _, _, (pp, pl) = plain[r.jid]
+        assert r.D == D, r.jid
+        assert (r.perm == pp).all() and (r.lcp == pl).all(), f"job {r.jid} differs when served"
+        if by[r.jid].n >= large:
+            assert r.batch_jobs == 1, f"large job {r.jid} shared a batch"
+            ss, operm, olcp, od = oracle_job(_spec_chars(by[r.jid]))
+            pb = np.array([0, ss.n], dtype=np.uint64)
+            assert (r.lcp == olcp).all() and r.D == od
+            assert (canonical_ties(r.perm, r.lcp, ss.len, pb) == canonical_ties(operm, olcp, ss.len, pb)).all()

@@ -59,6 +59,26 @@ def test_c2_dton_20m_prefix_doubling():
         assert (res == 264).all() and (rounds == 6).all()
 
 
+def test_c2_dton_20m_bloom_8_bits_per_string():
+    """The same input with the hash range sized 8 bits per string (m = 8 n,
+    the Bloom-filter-sized variant of the configuration; the reference uses
+    2^64 - 1, bloomfilter.hpp:1135-1136): collisions keep strings candidates
+    longer, so results may exceed the exact distinguishing prefixes but never
+    fall below them; the sorted output is exact."""
+    dss = _dss()
+    n = 20_000_000
+    with dss.DeviceStringSet.dton(n, 500, 0.5, 0x5EED0002, 16) as ds:
+        ds.prefix_doubling_sort(m=8 * n)
+        chk = ds.check()
+        check_ok(chk)
+        assert chk["D"] == 250 * n
+        dc = ds.check_dist()
+        assert dc["lower_violations"] == 0
+        assert dc["exact_sum"] == chk["D"]
+        res, rounds = ds.get_prefix_doubling()
+        assert (res >= 250).all()
+
+
 def test_c1_dton_1m_merge_sort_compressed():
     """1M x 500 chars, D/N 0.5, 4 PEs, merge sort with LCP compression."""
     dss = _dss()

@@ -177,6 +177,29 @@ def test_prefix_doubling_small_hash_range(m):
     assert nr == onr and (gres == ores).all() and (grounds == orounds).all()
 
 
+@pytest.mark.parametrize("name", ["dton", "skewed"])
+def test_prefix_doubling_bloom_8_bits_per_string_200k(name):
+    """Hash range m = 8 n (8 bits per string) on 200k strings: results,
+    rounds and round count equal the oracle's with the same m, collisions
+    included (with 2^64 - 1 there are none in practice)."""
+    dss = _dss()
+    n = 200_000
+    chars = (O.gen_dton(n, 120, 0.5, 0x5EED0012) if name == "dton" else
+             O.gen_skewed(O.skew_params(n, long_len=300, short_len=30), 0x5EED0013))
+    ss = O.StringSet(chars, 8)
+    m = 8 * ss.n
+    operm, olcp = O.local_sort(ss)
+    ores, orounds, onr = O.prefix_doubling(ss, operm, olcp, m=m)
+    with dss.DeviceStringSet.from_arena(chars, 8) as ds:
+        ds.sort_lcp()
+        nr = ds.prefix_doubling(m=m)
+        gres, grounds = ds.get_prefix_doubling()
+    assert nr == onr and (gres == ores).all() and (grounds == orounds).all()
+    # m = 8 n does collide here: some results exceed the 2^64 - 1 ones
+    eres, _, _ = O.prefix_doubling(ss, operm, olcp)
+    assert (ores >= eres).all() and (ores > eres).any()
+
+
 def test_hash_prefixes_match_reference_hash():
     dss = _dss()
     rng = np.random.default_rng(9)
@@ -263,6 +286,30 @@ def test_merge_fan_in_gives_the_same_output(name, p):
                 chk = ds.check()
                 assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0, (mode, fan)
         ds.set_sort_params(None)
+
+
+def test_caller_stream_gives_the_same_results():
+    """dss_set_stream: the whole prefix-doubling sort queued on a caller's
+    HIP stream gives the own-stream results; the set returns to its own
+    stream with NULL and never destroys the caller's."""
+    from helpers import HipMem
+    dss = _dss()
+    chars = INPUTS["skewed"]
+    with dss.DeviceStringSet.from_arena(chars, 5) as ds, HipMem() as hm:
+        ds.prefix_doubling_sort()
+        ref = ds.get_permutation(), ds.get_prefix_doubling()
+        st = hm.stream()
+        ds.set_stream(st)
+        ds.prefix_doubling_sort()
+        got = ds.get_permutation(), ds.get_prefix_doubling()
+        chk = ds.check()
+        ds.set_stream(None)
+        ds.merge_sort(compressed=True)  # back on its own stream
+        assert ds.check()["sorted"]
+    for a, b in zip(ref, got):
+        for x, y in zip(a, b):
+            assert (x == y).all()
+    assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0
 
 
 def test_few_strings_many_pes():
