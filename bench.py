@@ -436,6 +436,11 @@ def main():
     ap.add_argument("--pes", type=int, default=16)
     ap.add_argument("--seed", type=lambda x: int(x, 0), default=0x5EED0002)
     ap.add_argument("--sampler", default="IndexedNumStrings")
+    ap.add_argument("--workload", choices=["dton", "skewed", "suffixes"], default="dton",
+                    help="dton: the D/N-ratio input (--n, --len, --dton, --pes, --seed); skewed: 20M skewed strings "
+                         "(20%% of 1000 chars D/N .75, 80%% of 100 chars D/N .25, 10%% duplicates, bytes 1-255, "
+                         "seed 0x5EED0003); suffixes: capped suffixes (500 chars) of a 200 MB order-3 Markov "
+                         "ACGT text with 5000 planted repeats of 1-10 KB (seed 0x5EED0004); both 16 PEs")
     ap.add_argument("--bloom-bits", type=int, default=0,
                     help="hash range = bits * n (the 'Bloom filter sized b bits/string' variant); "
                          "0 = 2^64-1 as in the reference")
@@ -471,7 +476,15 @@ def main():
     m = (2**64 - 1) if args.bloom_bits == 0 else args.bloom_bits * args.n
     log("device:", dss_amd.device_info())
     t0 = time.time()
-    ds = dss_amd.DeviceStringSet.dton(args.n, args.len, args.dton, args.seed, args.pes)
+    if args.workload == "skewed":
+        args.n, args.pes = 20_000_000, 16
+        ds = dss_amd.DeviceStringSet.skewed(args.n, 0x5EED0003, args.pes)
+    elif args.workload == "suffixes":
+        args.n, args.pes = 200_000_000, 16
+        ds = dss_amd.DeviceStringSet.markov_suffixes(args.n, 0x5EED0004, args.pes, repeats=5000, rep_min=1000,
+                                                     rep_max=10000, cap=500)
+    else:
+        ds = dss_amd.DeviceStringSet.dton(args.n, args.len, args.dton, args.seed, args.pes)
     ds.sync()
     info = ds.info()
     log(f"generated {info['num_strings']} strings, {info['num_bytes']} bytes in {time.time() - t0:.2f} s")
@@ -513,8 +526,8 @@ def main():
     roof = roofline_from_stats(stats)
     if roof:
         roof["launches_per_step"] = stats[roof["kernel"]]["launches"] / args.steps
-        default_workload = (args.n, args.len, args.dton, args.pes, args.seed, args.bloom_bits) == \
-            (20_000_000, 500, 0.5, 16, 0x5EED0002, 0)
+        default_workload = args.workload == "dton" and (args.n, args.len, args.dton, args.pes, args.seed,
+                                                        args.bloom_bits) == (20_000_000, 500, 0.5, 16, 0x5EED0002, 0)
         if default_workload:
             roof["traffic"] = measured_traffic(roof["kernel"])
             if roof["traffic"] is not None:
@@ -543,11 +556,16 @@ def main():
         extras = extras_leg(args, D)
 
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_sample > 0 and args.workload == "dton":
         log(f"CPU restatement on {args.cpu_sample} strings ...")
         cpu = cpu_baseline(args.cpu_sample, args.len, args.dton, args.pes, args.seed, m, args.sampler)
         log(f"CPU: {cpu['value']} GB/s on {cpu['cores']} threads, phases {cpu['phase_s']}")
 
+    wl = {"dton": f"D/N-ratio input, {args.n} strings x {args.len} chars, D/N {args.dton}",
+          "skewed": f"skewed D/N input, {args.n} strings (20% x 1000 chars D/N 0.75, 80% x 100 chars D/N 0.25, "
+                    f"10% exact duplicates, bytes 1-255)",
+          "suffixes": f"{args.n} suffixes capped at 500 chars of a 200 MB order-3 Markov ACGT text with 5000 "
+                      f"planted repeats of 1-10 KB"}[args.workload]
     out = {
         "metric": "distinguishing-prefix GB/s at 1 GPU (batched vs plain); p50/p99 job latency",
         "value": round(D / step_s / 1e9, 3),
@@ -562,8 +580,7 @@ def main():
         "dtype": "u8",
         "data": "synthetic (seeded D/N-ratio generator, generated in HBM)",
         "config": {
-            "workload": f"D/N-ratio input, {args.n} strings x {args.len} chars, D/N {args.dton}, "
-                        f"{args.pes} virtual PEs, prefix-doubling sort (XXH64-stream, "
+            "workload": f"{wl}, {args.pes} virtual PEs, prefix-doubling sort (XXH64-stream, "
                         f"hash range {'2^64-1' if args.bloom_bits == 0 else f'{args.bloom_bits}*n'}, "
                         f"start depth 8), sampler {args.sampler}",
             "strings": args.n,
