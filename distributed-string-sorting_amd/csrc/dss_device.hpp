@@ -157,11 +157,12 @@ __device__ inline int wave_cmp(const uint8_t* __restrict__ chars, uint64_t ao, u
 }
 
 // LCP of two strings by a whole wave (same window scheme as wave_cmp; every
-// lane of the wave must call it with the same arguments).
+// lane of the wave must call it with the same arguments), for strings known
+// to agree on their first `from` characters.
 __device__ inline uint32_t wave_lcp(const uint8_t* __restrict__ chars, uint64_t ao, uint32_t al, uint64_t bo,
-                                    uint32_t bl) {
+                                    uint32_t bl, uint32_t from = 0) {
     const uint32_t lane = lane_id();
-    for (uint32_t base = 0;; base += 512) {
+    for (uint32_t base = from;; base += 512) {
         const uint32_t pos = base + 8 * lane;
         const uint64_t a = load_key8(chars, ao, al, pos);
         const uint64_t b = load_key8(chars, bo, bl, pos);

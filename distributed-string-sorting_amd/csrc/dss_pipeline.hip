@@ -1312,8 +1312,8 @@ __global__ void k_kmerge_fix(const KGroup* __restrict__ groups, uint32_t ngroups
 // the faster form where the merge is bound by the node merges' dependent LDS
 // latency rather than by HBM traffic (the D/N inputs); the tuner picks the
 // fan-in per input regime.
-constexpr uint32_t kMT = 512;               // threads per merge block
-constexpr uint32_t kMW = 2;                 // outputs per thread
+constexpr uint32_t kMT = 256;               // threads per merge block
+constexpr uint32_t kMW = 4;                 // outputs per thread
 constexpr uint32_t kTileOut = kMT * kMW;    // outputs per tile
 
 struct MergePair {
@@ -1375,14 +1375,114 @@ __global__ void k_merge_split(const MergePair* __restrict__ pairs, uint32_t npai
     if (l == 0) split[t] = lo;
 }
 
+// A tile's strings in LDS: slot x holds the receive index o[x], the LCP l[x]
+// with its predecessor in its run, and the 16 characters at `base` (k1[x],
+// k2[x]).  Every two strings the tile compares agree on their first `base`
+// characters.  k2s: the second keys are staged; otherwise they are read from
+// the global keys at the receiver's common prefix (base is that prefix then).
+struct TileView {
+    const uint32_t* o;
+    const uint64_t* k1;
+    const uint64_t* k2;
+    const uint8_t* chars;
+    const uint64_t* voff;
+    const uint32_t* vlen;
+    const uint64_t* gk2;
+    uint32_t base;
+    bool k2s;
+};
+
+// Compares, for every lane of the wave with `act` set, the strings of slots
+// x and y, known to agree on their first `from` characters; returns the sign
+// and the LCP.  All lanes of the wave call it together.  The staged keys are
+// compared per lane; the lanes they leave undecided are then served one at a
+// time by the whole wave, lane l comparing the 8 characters at from + 8 l of both strings (512 characters
+// per dependent step), so a lane whose strings agree on hundreds of
+// characters costs the wave one or two loads instead of a loop of
+// 32-character steps that the other lanes wait out.
+__device__ inline int wave_tile_cmp(const TileView& T, bool act, uint32_t x, uint32_t y, uint32_t from,
+                                    uint32_t& lcp) {
+    int c = 0;
+    bool pend = false;
+    if (act) {
+        pend = true;
+        if (from < T.base + 8) {
+            const uint64_t a = T.k1[x], b = T.k1[y];
+            if (a != b) {
+                lcp = T.base + key_lcp(a, b);
+                c = a < b ? -1 : 1;
+                pend = false;
+            } else if ((a & 0xFF) == 0) {
+                lcp = T.base + key_strlen(a);
+                pend = false;
+            }
+            from = T.base + 8;
+        }
+        if (pend && from < T.base + 16) {
+            const uint64_t a = T.k2s ? T.k2[x] : T.gk2[T.o[x]], b = T.k2s ? T.k2[y] : T.gk2[T.o[y]];
+            if (a != b) {
+                lcp = T.base + 8 + key_lcp(a, b);
+                c = a < b ? -1 : 1;
+                pend = false;
+            } else if ((a & 0xFF) == 0) {
+                lcp = T.base + 8 + key_strlen(a);
+                pend = false;
+            }
+            from = T.base + 16;
+        }
+    }
+    uint64_t ao = 0, bo = 0;
+    uint32_t al = 0, bl = 0;
+    if (pend) {
+        const uint32_t oa = T.o[x], ob = T.o[y];
+        ao = T.voff[oa];
+        al = T.vlen[oa];
+        bo = T.voff[ob];
+        bl = T.vlen[ob];
+    }
+    const uint32_t lane = lane_id();
+    for (uint64_t m = __ballot(pend); m; m &= m - 1) {
+        const int ld = __ffsll((unsigned long long)m) - 1;
+        const uint64_t pa = __shfl((unsigned long long)ao, ld), pb = __shfl((unsigned long long)bo, ld);
+        const uint32_t pal = (uint32_t)__shfl((int)al, ld), pbl = (uint32_t)__shfl((int)bl, ld);
+        const uint32_t pf = (uint32_t)__shfl((int)from, ld);
+        for (uint32_t base = pf;; base += 512) {
+            const uint32_t pos = base + 8 * lane;
+            const uint64_t ka = load_key8(T.chars, pa, pal, pos), kb = load_key8(T.chars, pb, pbl, pos);
+            const uint64_t mm = __ballot(ka != kb || (ka & 0xFF) == 0);
+            if (mm) {
+                const int f = __ffsll((unsigned long long)mm) - 1;
+                const uint64_t fa = __shfl((unsigned long long)ka, f), fb = __shfl((unsigned long long)kb, f);
+                if (lane == (uint32_t)ld) {
+                    c = fa < fb ? -1 : (fa > fb ? 1 : 0);
+                    lcp = base + 8 * (uint32_t)f + (fa != fb ? key_lcp(fa, fb) : key_strlen(fa));
+                }
+                break;
+            }
+        }
+    }
+    return c;
+}
+
 // One tile of kTileOut outputs per block: the tile's windows of A and B
-// (plus the element before each) are staged in LDS; every thread finds its
-// start by a merge-path search in LDS and merges kMW outputs with the LCP
-// rules of the loser tree (bingmann-lcp_losertree.hpp:280-333): the head
-// with the larger LCP to the last output is smaller, equal LCPs compare
-// characters from there on.  ord == nullptr: the first round (receive
-// order); saved != nullptr: the last round, which also writes every output's
-// saved LCP (the LCP it was received with, rlcp).
+// (plus the element before each) are staged in LDS, A's at slots 0..la_n and
+// B's after them; every thread finds its start by a merge-path search in LDS
+// and merges kMW outputs with the LCP rules of the loser tree
+// (bingmann-lcp_losertree.hpp:280-333): the head with the larger LCP to the
+// last output is smaller, equal LCPs compare characters from there on.
+//
+// Characters: the staged run LCPs bound the LCP of any two strings of one
+// window from below (their minimum over the window's adjacent pairs); with
+// the LCP of the two windows' first strings (one wave compares them) they
+// bound every pair the tile compares.  When that bound T is at least 8
+// characters past the receiver's common prefix cp, the cached keys at cp are
+// equal throughout the tile and the strings' 16 characters at T are staged
+// instead; otherwise the keys at cp are staged, the second one only when some
+// adjacent pair of a window agrees beyond cp + 8.
+//
+// ord == nullptr: the first round (receive order); saved != nullptr: the last
+// round, which also writes every output's saved LCP (the LCP it was received
+// with, rlcp).
 __global__ void __launch_bounds__(kMT)
 k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_t* __restrict__ split,
              const uint32_t* __restrict__ ord, const uint32_t* __restrict__ lcpa, uint32_t* __restrict__ ord_out,
@@ -1390,12 +1490,14 @@ k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_
              const uint32_t* __restrict__ vlen, const uint64_t* __restrict__ keyc,
              const uint64_t* __restrict__ keyc2, const uint32_t* __restrict__ rcp, const uint32_t* __restrict__ rlcp,
              uint32_t* __restrict__ saved) {
-    __shared__ uint32_t sAo[kTileOut + 1], sAl[kTileOut + 1], sBo[kTileOut + 1], sBl[kTileOut + 1];
-    __shared__ uint64_t sAk[kTileOut + 1], sBk[kTileOut + 1];
+    constexpr uint32_t kSlots = kTileOut + 2;
+    __shared__ uint32_t sO[kSlots], sL[kSlots];
+    __shared__ uint64_t sK1[kSlots], sK2[kSlots];
+    __shared__ uint32_t sWMin[kMT / 64], sWMax[kMT / 64], sCross;
     const uint64_t t = blockIdx.x;
     const uint32_t pi = pair_of_tile(pairs, npairs, t);
     const MergePair Pr = pairs[pi];
-    const ViewCtx V{chars, voff, vlen, keyc2, rcp[Pr.recv]};
+    const uint32_t cp = rcp[Pr.recv];
     const uint64_t A = Pr.out, B = Pr.out + Pr.na;
     const uint64_t na = Pr.na, nb = Pr.nb, tot = na + nb;
     const uint64_t d0 = (t - Pr.tile0) * kTileOut;
@@ -1404,83 +1506,146 @@ k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_
     const uint64_t i1 = (d1 == tot) ? na : split[t + 1];
     const uint64_t j0 = d0 - i0, j1 = d1 - i1;
     const uint32_t la_n = (uint32_t)(i1 - i0), lb_n = (uint32_t)(j1 - j0);
-    // stage A[i0-1 .. i1) and B[j0-1 .. j1) at LDS index (global - start + 1)
-    for (uint32_t k = threadIdx.x; k <= la_n; k += kMT) {
-        const int64_t g = (int64_t)i0 - 1 + k;
-        if (g >= 0 && (uint64_t)g < na) {
-            const uint32_t o = ord ? ord[A + g] : (uint32_t)(A + g);
-            sAo[k] = o;
-            sAl[k] = lcpa[A + g];
-            sAk[k] = keyc[o];
+    const uint32_t bs = la_n + 1;  // slot of B's window position 0 (B[j0 - 1])
+    // stage A[i0-1 .. i1) at slots 0..la_n and B[j0-1 .. j1) at bs..bs+lb_n;
+    // the LCP of a window position with a predecessor inside the window
+    // enters the bounds
+    uint32_t lmin = 0xFFFFFFFFu, lmax = 0;
+    for (uint32_t k = threadIdx.x; k <= la_n + 1 + lb_n; k += kMT) {
+        const bool in_a = k <= la_n;
+        const uint32_t w = in_a ? k : k - bs;
+        const int64_t g = (int64_t)(in_a ? i0 : j0) - 1 + w;
+        const uint64_t len = in_a ? na : nb;
+        if (g >= 0 && (uint64_t)g < len) {
+            const uint64_t pos = (in_a ? A : B) + g;
+            const uint32_t o = ord ? ord[pos] : (uint32_t)pos;
+            const uint32_t l = lcpa[pos];
+            sO[k] = o;
+            sL[k] = l;
+            sK1[k] = keyc[o];
+            if (w >= 1 && g >= 1) {
+                lmin = min(lmin, l);
+                lmax = max(lmax, l);
+            }
         }
     }
-    for (uint32_t k = threadIdx.x; k <= lb_n; k += kMT) {
-        const int64_t g = (int64_t)j0 - 1 + k;
-        if (g >= 0 && (uint64_t)g < nb) {
-            const uint32_t o = ord ? ord[B + g] : (uint32_t)(B + g);
-            sBo[k] = o;
-            sBl[k] = lcpa[B + g];
-            sBk[k] = keyc[o];
-        }
+    for (uint32_t m = 32; m; m >>= 1) {
+        lmin = min(lmin, (uint32_t)__shfl_xor((int)lmin, m));
+        lmax = max(lmax, (uint32_t)__shfl_xor((int)lmax, m));
+    }
+    if (lane_id() == 0) {
+        sWMin[threadIdx.x / 64] = lmin;
+        sWMax[threadIdx.x / 64] = lmax;
     }
     __syncthreads();
+    uint32_t sMin = sWMin[0], sMax = sWMax[0];
+#pragma unroll
+    for (uint32_t w = 1; w < kMT / 64; ++w) {
+        sMin = min(sMin, sWMin[w]);
+        sMax = max(sMax, sWMax[w]);
+    }
+    // first staged slot of each window (none: kNone)
+    const uint32_t fa = i0 > 0 ? 0u : (la_n > 0 ? 1u : kNone);
+    const uint32_t fb = j0 > 0 ? bs : (lb_n > 0 ? bs + 1 : kNone);
+    const bool try_restage = fa != kNone && fb != kNone && sMin != 0xFFFFFFFFu && sMin >= cp + 8;  // block-uniform
+    if (try_restage) {
+        if (threadIdx.x < 64) {
+            const uint32_t oa = sO[fa], ob = sO[fb];
+            const uint32_t x = wave_lcp(chars, voff[oa], vlen[oa], voff[ob], vlen[ob], cp);
+            if (lane_id() == 0) sCross = x;
+        }
+        __syncthreads();
+    }
+    const uint32_t T = try_restage ? min(sMin, sCross) : cp;
+    const bool restage = T >= cp + 8;
+    const bool k2s = restage || sMax >= cp + 8;
+    if (k2s) {
+        for (uint32_t k = threadIdx.x; k <= la_n + 1 + lb_n; k += kMT) {
+            const bool in_a = k <= la_n;
+            const int64_t g = (int64_t)(in_a ? i0 : j0) - 1 + (in_a ? k : k - bs);
+            if (g >= 0 && (uint64_t)g < (in_a ? na : nb)) {
+                const uint32_t o = sO[k];
+                if (restage) {
+                    const uint64_t so = voff[o];
+                    const uint32_t sl = vlen[o];
+                    sK1[k] = load_key8(chars, so, sl, T);
+                    sK2[k] = load_key8(chars, so, sl, T + 8);
+                } else {
+                    sK2[k] = keyc2[o];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const TileView V{sO, sK1, sK2, chars, voff, vlen, keyc2, restage ? T : cp, k2s};
     const uint32_t dl = threadIdx.x * kMW;
-    if (d0 + dl >= d1) return;
-    // local merge path
-    uint32_t lo = dl > lb_n ? dl - lb_n : 0, hi = dl < la_n ? dl : la_n;
-    while (lo < hi) {
+    const bool live = d0 + dl < d1;
+    // local merge path (window positions; A's position w is slot w, B's is
+    // bs + w); every lane runs the wave's iterations, the finished ones idle
+    uint32_t lo = 0, hi = 0;
+    if (live) {
+        lo = dl > lb_n ? dl - lb_n : 0;
+        hi = dl < la_n ? dl : la_n;
+    }
+    while (__ballot(lo < hi)) {
+        const bool act = lo < hi;
         const uint32_t mid = (lo + hi) / 2;
-        const uint32_t bi = dl - mid - 1;
         uint32_t x;
-        if (elem_cmp(V, sAo[mid + 1], sAk[mid + 1], sBo[bi + 1], sBk[bi + 1], V.cp, x) <= 0) lo = mid + 1;
-        else hi = mid;
+        const int c = wave_tile_cmp(V, act, mid + 1, act ? bs + dl - mid : 0u, cp, x);
+        if (act) {
+            if (c <= 0) lo = mid + 1;
+            else hi = mid;
+        }
     }
-    uint32_t il = lo, jl = dl - lo;          // local indices; LDS index = local + 1
+    uint32_t il = lo, jl = dl - lo;             // heads: slots il + 1 and bs + jl + 1
     const uint64_t gi = i0 + il, gj = j0 + jl;  // global indices in A and B
-    uint32_t la = V.cp, lb = V.cp;
-    if (d0 + dl > 0) {
-        bool prev_is_a;
-        uint32_t x;
-        if (gi == 0) prev_is_a = false;
-        else if (gj == 0) prev_is_a = true;
-        else prev_is_a = elem_cmp(V, sAo[il], sAk[il], sBo[jl], sBk[jl], V.cp, x) > 0;
-        const uint32_t po = prev_is_a ? sAo[il] : sBo[jl];
-        const uint64_t pk = prev_is_a ? sAk[il] : sBk[jl];
-        // heads outside this tile's windows are never compared in this tile
-        if (il < la_n) {
-            if (prev_is_a) la = sAl[il + 1];
-            else elem_cmp(V, sAo[il + 1], sAk[il + 1], po, pk, V.cp, la);
-        }
-        if (jl < lb_n) {
-            if (!prev_is_a) lb = sBl[jl + 1];
-            else elem_cmp(V, sBo[jl + 1], sBk[jl + 1], po, pk, V.cp, lb);
-        }
+    uint32_t la = cp, lb = cp;
+    // the previous output is the later of A[gi - 1] and B[gj - 1] (B on
+    // equality); the head of the other list gets its LCP with it by a
+    // comparison, the head of the same list has its run LCP
+    const bool has_prev = live && d0 + dl > 0;
+    const bool cmp_prev = has_prev && gi != 0 && gj != 0;
+    uint32_t x;
+    const int pc = wave_tile_cmp(V, cmp_prev, il, bs + jl, cp, x);
+    const bool prev_is_a = has_prev && gi != 0 && (gj == 0 || pc > 0);
+    const uint32_t ps = prev_is_a ? il : bs + jl;
+    const bool cmp_a = has_prev && !prev_is_a && il < la_n;
+    const bool cmp_b = has_prev && prev_is_a && jl < lb_n;
+    uint32_t lh = cp;
+    wave_tile_cmp(V, cmp_a || cmp_b, cmp_a ? il + 1 : bs + jl + 1, ps, cp, lh);
+    if (has_prev) {
+        if (il < la_n) la = prev_is_a ? sL[il + 1] : lh;
+        if (jl < lb_n) lb = prev_is_a ? lh : sL[bs + jl + 1];
     }
-    const uint64_t oend = min((uint64_t)dl + kMW, d1 - d0);
-    for (uint64_t o = dl; o < oend; ++o) {
+    const uint32_t oend = live ? (uint32_t)min((uint64_t)dl + kMW, d1 - d0) : dl;
+    for (uint32_t q = 0; q < kMW; ++q) {
+        const uint32_t o = dl + q;
+        const bool valid = o < oend;
         const bool has_a = il < la_n, has_b = jl < lb_n;
+        const bool tie = valid && has_a && has_b && la == lb;
+        uint32_t y;
+        const int cr = wave_tile_cmp(V, tie, il + 1, bs + jl + 1, la, y);
+        if (!valid) continue;
         bool take_a;
         if (!has_a) take_a = false;
         else if (!has_b) take_a = true;
         else if (la > lb) take_a = true;
         else if (lb > la) take_a = false;
         else {
-            uint32_t x;
-            const int cr = elem_cmp(V, sAo[il + 1], sAk[il + 1], sBo[jl + 1], sBk[jl + 1], la, x);
             take_a = cr <= 0;
-            if (take_a) lb = x; else la = x;
+            if (take_a) lb = y; else la = y;
         }
         uint32_t lout, outo;
         if (take_a) {
             lout = la;
-            outo = sAo[il + 1];
+            outo = sO[il + 1];
             ++il;
-            if (il < la_n) la = sAl[il + 1];
+            if (il < la_n) la = sL[il + 1];
         } else {
             lout = lb;
-            outo = sBo[jl + 1];
+            outo = sO[bs + jl + 1];
             ++jl;
-            if (jl < lb_n) lb = sBl[jl + 1];
+            if (jl < lb_n) lb = sL[bs + jl + 1];
         }
         const uint64_t go = d0 + o;
         ord_out[A + go] = outo;

@@ -47,7 +47,6 @@ constexpr uint32_t kSmallMax = 1024;  // base-case capacity (slots per block)
 constexpr uint32_t kClasses = 12;     // class c holds sizes (2^(c-1), 2^c], c = 1..11
 constexpr uint32_t kTile = 2048;      // strings per tile in the global passes
 constexpr uint32_t kThreads = 256;
-constexpr uint32_t kItems = kTile / kThreads;
 constexpr uint32_t kChunkTiles = 64;  // tiles per chunk of the two-level digit-offset scan
 constexpr uint32_t kWaveClassMax = 6;  // default: classes up to 64 strings use the one-wave base case
 
