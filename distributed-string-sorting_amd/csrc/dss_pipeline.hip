@@ -587,6 +587,7 @@ __device__ inline uint32_t receiver_of(const uint64_t* __restrict__ recv_begin, 
 }
 
 constexpr uint32_t kXcWords = 512;  // LDS words per wave for the short-copy path
+constexpr uint32_t kXcP = 64;        // receivers up to which the copy stages its tables
 
 // 2^lg lanes per received string copy its bytes into the receive arena.
 // With lg = 0 (short copies, such as the few characters after the LCP that
@@ -603,6 +604,18 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
                             uint64_t* __restrict__ keyc2,
                             const uint32_t* __restrict__ r_stage, const uint64_t* __restrict__ stage,
                             uint32_t stage_w) {
+    // up to kXcP receivers their first receive positions and common prefixes
+    // are staged in LDS for the receiver search of every string
+    __shared__ uint64_t s_rb[kXcP + 1];
+    __shared__ uint32_t s_cp[kXcP];
+    const bool staged_tab = p <= kXcP;  // grid-uniform
+    if (staged_tab) {
+        for (uint32_t t = threadIdx.x; t <= p; t += blockDim.x) s_rb[t] = recv_begin[t];
+        for (uint32_t t = threadIdx.x; t < p; t += blockDim.x) s_cp[t] = cp[t];
+        __syncthreads();
+    }
+    const uint64_t* const RB = staged_tab ? s_rb : recv_begin;
+    const uint32_t* const CP = staged_tab ? s_cp : cp;
     if (lg == 0) {
         // A wave's 64 consecutive receive positions own one contiguous byte
         // range of the receive arena.  When it fits, the lanes build it in
@@ -622,7 +635,7 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
             uint32_t si = kNone, vlen = 0, from = 0, cpk = 0;
             uint64_t start = 0;
             if (valid) {
-                cpk = cp[receiver_of(recv_begin, p, k)];
+                cpk = CP[receiver_of(RB, p, k)];
                 si = r_stage ? r_stage[k] : kNone;
                 vlen = r_vlen[k];
                 from = r_from[k];
@@ -721,11 +734,11 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
         }
         return;
     }
-    // Longer copies: the string's lanes own the aligned 8-byte words of its
-    // output range [o, o + m + 1) (characters, then a NUL); a word wholly
-    // inside the range is built from two aligned source loads and stored at
-    // once, the two edge words (shared with the neighbouring strings) byte
-    // by byte.
+    // Longer copies: 2^lg lanes per string.  An output word belongs to the
+    // string that holds its first byte; the string's lanes build each of its
+    // words whole (its characters, its NUL, then the first bytes of the
+    // strings that follow when it ends inside the word) and store it as one
+    // aligned 8-byte word, so every word of the receive arena is written once.
     const uint32_t lanes = 1u << lg;
     const uint32_t l = threadIdx.x & (lanes - 1);
     for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lg; k < n;
@@ -734,22 +747,33 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
         const uint32_t vlen = r_vlen[k];
         const uint64_t sa = r_voff[k] + from;  // first source character
         if (l == 0) {
-            const uint32_t cpk = cp[receiver_of(recv_begin, p, k)];
+            const uint32_t cpk = CP[receiver_of(RB, p, k)];
             keyc[k] = load_key8(chars, r_voff[k], vlen, cpk);
             keyc2[k] = load_key8(chars, r_voff[k], vlen, cpk + 8);
         }
         // modes 0/1: chars [from, vlen] incl. the NUL; mode 2: [from, vlen) + NUL
         const uint32_t m = vlen - from;
         const uint64_t o = byte_off[k], e = o + m + 1;
-        for (uint64_t w = (o >> 3) + l; w < (e + 7) >> 3; w += lanes) {
+        for (uint64_t w = ((o + 7) >> 3) + l; w < (e + 7) >> 3; w += lanes) {
             const uint64_t g = w << 3;
-            if (g >= o && g + 8 <= e) {
-                uint64_t v = load_word_le(chars, sa + (g - o));
-                if (e == g + 8) v &= ~(0xFFull << 56);  // the NUL is the word's last byte
-                *reinterpret_cast<uint64_t*>(out + g) = v;
+            uint64_t v;
+            if (g + 8 <= e) {
+                v = load_word_le(chars, sa + (g - o));
+                if (g + 8 == e) v &= ~(0xFFull << 56);  // the NUL is the word's last byte
             } else {
-                for (uint64_t x = max(g, o); x < min(g + 8, e); ++x) out[x] = x - o < m ? chars[sa + (x - o)] : 0;
+                const uint32_t nk = (uint32_t)(e - g);  // 1..7 bytes of this string, the last its NUL
+                v = nk > 1 ? load_word_le(chars, sa + (g - o)) & ((1ull << (8 * (nk - 1))) - 1) : 0;
+                uint32_t filled = nk;
+                for (uint64_t kk = k + 1; filled < 8 && kk < n; ++kk) {
+                    const uint32_t f2 = r_from[kk], m2 = r_vlen[kk] - f2;
+                    const uint32_t take = min(8u - filled, m2 + 1);  // its characters, then its NUL
+                    const uint32_t nch = min(take, m2);              // < 8
+                    if (nch) v |= (load_word_le(chars, r_voff[kk] + f2) & ((1ull << (8 * nch)) - 1)) << (8 * filled);
+                    filled += take;
+                }
+                // bytes past the last string stay zero (the arena padding)
             }
+            *reinterpret_cast<uint64_t*>(out + g) = v;
         }
     }
 }

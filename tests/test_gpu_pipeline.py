@@ -28,6 +28,13 @@ def _short_with_long(rng):
     return strs
 
 
+def _tiny_with_long(rng):
+    strs = [bytes(rng.integers(97, 100, int(l), dtype=np.uint8)) for l in rng.integers(0, 8, 20000)]
+    for pos in rng.choice(len(strs), 300, replace=False):
+        strs[int(pos)] = bytes(rng.integers(97, 100, 3000, dtype=np.uint8))
+    return strs
+
+
 def inputs():
     rng = np.random.default_rng(3)
     small = [bytes(rng.integers(97, 100, int(l), dtype=np.uint8)) for l in rng.integers(0, 12, 3000)]
@@ -40,6 +47,10 @@ def inputs():
         # copy path of the exchange, where a wave's byte range can exceed its
         # LDS buffer
         "short_with_long": arena(_short_with_long(rng)),
+        # 0-7 characters with 300 strings of 3000: more than 33 bytes per
+        # string on average, so the exchange takes its multi-lane path, where
+        # one output word can hold the ends of several short strings
+        "tiny_with_long": arena(_tiny_with_long(np.random.default_rng(5))),
     }
 
 
