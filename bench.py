@@ -207,6 +207,7 @@ def batched_leg(args):
     if mism:
         log(f"batched D differs from plain for jobs {mism[:10]}")
         sys.exit(1)
+    small = small_jobs_leg(args, specs, Dp, plain_t)
     D_total = sum(Dp.values())
     plain_gbs = D_total / sum(plain_t.values()) / 1e9
     batched_gbs = D_total / batched_s / 1e9
@@ -248,6 +249,10 @@ def batched_leg(args):
         "plain_strings_per_s": round(n_total / sum(plain_t.values()), 1),
         "batched_strings_per_s": round(n_total / batched_s, 1),
         "mean_jobs_per_batch": round(float(np.mean(batch_sizes)), 2),
+        "small_jobs": small,
+        # where the plain time goes: jobs above the batch budget sort alone in
+        # both forms, so their share bounds what batching can gain
+        "plain_s_by_size": plain_by_size(specs, plain_t, args.serve_small_max, args.serve_batch_strings),
         "served": {
             "lambda_jobs_per_s": round(lam, 3),
             "load": args.serve_load,
@@ -261,6 +266,59 @@ def batched_leg(args):
             "plain_fifo": percentiles(plain_lat),
         },
     }
+
+
+def plain_by_size(specs, plain_t, small_max, batch_max):
+    bins = {f"<={small_max}": 0.0, f"<={batch_max}": 0.0, f">{batch_max}": 0.0}
+    keys = list(bins)
+    for s in specs:
+        k = keys[0] if s.n <= small_max else keys[1] if s.n <= batch_max else keys[2]
+        bins[k] += plain_t[s.jid]
+    tot = sum(bins.values())
+    return {k: {"s": round(v, 4), "share": round(v / tot, 4) if tot else 0.0} for k, v in bins.items()}
+
+
+def small_jobs_leg(args, specs, Dp, plain_t):
+    """Plain vs batched on the small-job subset of the same stream (jobs of at
+    most --serve-small-max strings), all resident at once: the plain times
+    are those of the mixed run (each job alone, sort only); the batched run
+    sorts the subset through a FIFO server of its own."""
+    import numpy as np
+    from dss_amd import JobPool
+    from dss_amd.serve import Server, run_plain
+
+    small = [s for s in specs if s.n <= args.serve_small_max]
+    if not small:
+        return None
+    cap_bytes = sum(s.max_bytes() + 8 for s in small) + (64 << 20)
+    cap_strings = sum(s.n for s in small) + (1 << 20)
+    server = Server(None, inflight=args.serve_inflight, max_batch_strings=args.serve_batch_strings,
+                    large_strings=None, reserve_strings=args.serve_batch_strings)
+    with JobPool(cap_bytes, cap_strings) as pool:
+        server.pool = pool
+        w = [s for s in small][:1]
+        j = w[0].add_to(pool)
+        run_plain(pool, w, resident={w[0].jid: j})
+        server.prepare(j)
+        pool.free(j)
+        ids = {s.jid: s.add_to(pool) for s in small}
+        t0 = time.perf_counter()
+        res = server.run(small, realtime=False, resident=ids)
+        batched_s = time.perf_counter() - t0
+        sizes = [r.batch_jobs for r in res]
+        server.close()
+    if any(r.D != Dp[r.jid] for r in res):
+        log("small-job batched D differs from plain")
+        sys.exit(1)
+    plain_s = sum(plain_t[s.jid] for s in small)
+    D = sum(Dp[s.jid] for s in small)
+    n = sum(s.n for s in small)
+    return {"max_strings": args.serve_small_max, "jobs": len(small), "strings": n, "D_bytes": D,
+            "plain_s": round(plain_s, 4), "batched_s": round(batched_s, 4),
+            "plain_us_per_job": round(plain_s / len(small) * 1e6, 1),
+            "batched_us_per_job": round(batched_s / len(small) * 1e6, 1),
+            "batched_over_plain": round(plain_s / batched_s, 3),
+            "mean_jobs_per_batch": round(float(np.mean(sizes)), 2)}
 
 
 def extras_leg(args, D_c2):
@@ -450,6 +508,8 @@ def main():
                     help="jobs of the batched/served leg (0 disables it)")
     ap.add_argument("--serve-n-min", type=int, default=1000)
     ap.add_argument("--serve-n-max", type=int, default=50_000_000)
+    ap.add_argument("--serve-small-max", type=int, default=100_000,
+                    help="jobs of at most this many strings form the small-job subset")
     ap.add_argument("--serve-len-min", type=int, default=10)
     ap.add_argument("--serve-len-max", type=int, default=1000)
     ap.add_argument("--serve-seed", type=lambda x: int(x, 0), default=0x5EED0005)

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -388,7 +389,16 @@ void partition(dss_set_s* s, int sampler, uint64_t factor, uint64_t max_len, boo
         hipLaunchKernelGGL(k_store_u64, dim3(1), dim3(1), 0, st, wprefix + s->n, tot);
     }
     {
-        KScope ks(s->timer, st, "partition", 0.0);
+        // algorithmic bytes: every possible sample's id, offset and length and
+        // its characters (at most max_len, at most the mean string length),
+        // the splitters' binary searches over every PE (log2 of its strings
+        // probes of id, offset, length and an 8-character key each) and the
+        // p x p interval counts
+        const double mean_len = s->n ? (double)(s->nbytes - s->n) / (double)s->n : 0.0;
+        const double probes = std::log2((double)s->n / std::max<uint32_t>(p, 1) + 1.0);
+        const double part_bytes = (double)nsmax * (16.0 + std::min<double>((double)max_len, mean_len)) +
+                                  (double)p * (p - 1) * probes * 24.0 + (double)p * p * 8.0;
+        KScope ks(s->timer, st, "partition", p > 1 && s->n ? part_bytes : 0.0);
         if (p > 1 && s->n) {
             hipLaunchKernelGGL(k_sample, dim3((p + 63) / 64), dim3(64), 0, st, s->pe_begin_d.as<uint64_t>(), p,
                                s->perm.as<uint32_t>(), s->off.as<uint64_t>(), s->len.as<uint32_t>(), dist, wprefix,

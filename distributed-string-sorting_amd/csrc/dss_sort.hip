@@ -1218,14 +1218,14 @@ void sort_lcp(dss_set_s* s) {
         if (s->timer.enabled) {
             // prefix skip: per string of a segment that gathers keys, its
             // characters up to the common prefix + 8 (the key), id, offset
-            // and length reads, key and id writes; gather: per string of a
-            // tile whose keys are re-gathered, id, offset, length, 8 chars
-            // and the key write
+            // and length reads, key and id writes; gather: the tile records
+            // it reads, and per string of a tile whose keys are re-gathered,
+            // id, offset, length, 8 chars and the key write
             std::vector<Seg> after(nlarge);
             DSS_HIP(hipMemcpy(after.data(), cur, nlarge * sizeof(Seg), hipMemcpyDeviceToHost));
             std::vector<Tile> ht(nt);
             DSS_HIP(hipMemcpy(ht.data(), tl, nt * sizeof(Tile), hipMemcpyDeviceToHost));
-            double skip = 0, gath = 0;
+            double skip = 0, gath = (double)nt * sizeof(Tile);
             for (uint32_t k = 0; k < nlarge; ++k)
                 if (hseg[k].kdepth == kNone) skip += (double)after[k].cnt * ((double)after[k].cp + 36.0);
             for (const Tile& q : ht)
