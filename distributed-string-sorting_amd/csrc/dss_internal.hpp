@@ -184,6 +184,79 @@ struct Mailbox {
 void fill_small(hipStream_t st, void* p, uint8_t v, size_t n);
 
 // ------------------------------------------------------------- sort scratch
+// Key of the local sort's captured launch sequence: the capacities and
+// parameters it was captured for and every buffer address it reads or writes.
+struct SortGraphKey {
+    uint64_t ncap;
+    uint32_t pcap, large_min, wave_class_max, skip_lanes, skip_strings;
+    const void* ptr[19];
+    hipStream_t stream;
+    SortGraphKey() : ncap(0), pcap(0), large_min(0), wave_class_max(0), skip_lanes(0), skip_strings(0), ptr{},
+                     stream(nullptr) {}
+    template <class... P>
+    SortGraphKey(uint64_t n, uint32_t p, uint32_t lm, uint32_t wc, uint32_t sl, uint32_t ss, P... ps)
+        : ncap(n), pcap(p), large_min(lm), wave_class_max(wc), skip_lanes(sl), skip_strings(ss), ptr{},
+          stream(nullptr) {
+        const void* a[] = {static_cast<const void*>(ps)...};
+        static_assert(sizeof...(P) == 20, "19 buffers and the stream");
+        for (int k = 0; k < 19; ++k) ptr[k] = a[k];
+        stream = (hipStream_t)const_cast<void*>(a[19]);
+    }
+    bool operator==(const SortGraphKey& o) const {
+        if (ncap != o.ncap || pcap != o.pcap || large_min != o.large_min || wave_class_max != o.wave_class_max ||
+            skip_lanes != o.skip_lanes || skip_strings != o.skip_strings || stream != o.stream)
+            return false;
+        for (int k = 0; k < 19; ++k)
+            if (ptr[k] != o.ptr[k]) return false;
+        return true;
+    }
+};
+
+// The two executable graphs of the device-driven local sort (passes; base
+// cases) for one key.
+struct SortGraph {
+    hipGraphExec_t passes = nullptr, base = nullptr;
+    SortGraphKey key;
+    uint64_t last_use = 0;
+    void reset() {
+        if (passes) (void)hipGraphExecDestroy(passes);
+        if (base) (void)hipGraphExecDestroy(base);
+        passes = base = nullptr;
+        key = SortGraphKey();
+        last_use = 0;
+    }
+    SortGraph() = default;
+    SortGraph(const SortGraph&) = delete;
+    SortGraph& operator=(const SortGraph&) = delete;
+    ~SortGraph() { reset(); }
+};
+
+// A set's captured graphs, one per size class (strings and PEs rounded up to
+// powers of two), the least recently used replaced when all are taken.
+struct SortGraphCache {
+    static constexpr int kSlots = 6;
+    SortGraph slot[kSlots];
+    uint64_t clock = 0;
+    SortGraph& find(const SortGraphKey& k, bool& hit) {
+        SortGraph* lru = &slot[0];
+        for (auto& g : slot) {
+            if (g.passes && g.base && g.key == k) {
+                hit = true;
+                g.last_use = ++clock;
+                return g;
+            }
+            if (g.last_use < lru->last_use) lru = &g;
+        }
+        hit = false;
+        lru->reset();
+        lru->last_use = ++clock;
+        return *lru;
+    }
+    void reset() {
+        for (auto& g : slot) g.reset();
+    }
+};
+
 struct SortScratch {
     DevBuf keyA, keyB, pbA, pbB;
     DevBuf segA, segB;      // large segment lists
@@ -195,6 +268,7 @@ struct SortScratch {
     DevBuf offlen;          // packed (offset, length) per string for the base cases
     std::vector<uint64_t> small_off, small_cap;  // per class region in `small`
     uint64_t seg_cap = 0, tile_cap = 0;
+    SortGraphCache graphs;  // the device-driven form's captured launch sequences
 };
 
 // ------------------------------------------------------------- pipeline state
@@ -266,6 +340,7 @@ struct SortParams {
     uint32_t skip_lanes = 8;       // lanes per string in the prefix skip
     uint32_t skip_strings = 4;     // strings per lane group and round in the prefix skip
     uint32_t merge_fan = 2;        // runs merged per pass (2: the pairwise tree; 4, 8, 16: K-way tiles)
+    uint32_t pass_mode = 0;        // local sort: 0 auto, 1 host-driven passes, 2 device-driven (graph)
 };
 
 struct dss_set_s {
@@ -278,6 +353,8 @@ struct dss_set_s {
     uint32_t p = 1;        // virtual PEs
     uint32_t max_len = 0;
     std::vector<uint64_t> pe_begin;  // host copy, p+1
+    uint64_t sort_cap_n = 0;  // strings and PEs the local sort's buffers are reserved for
+    uint32_t sort_cap_p = 0;
 
     dss::DevBuf chars;  // nbytes + padding (owned arena)
     const uint8_t* borrowed = nullptr;  // arena owned by a job pool (batches)

@@ -98,50 +98,69 @@ __device__ inline void push_small(SmallSeg* small, const SmallLists& L, uint32_t
 // position after its last string contributes no LCP.
 constexpr uint64_t kDistChunk = 1 << 16;
 
-__global__ void k_pe_dsum(const uint64_t* __restrict__ chunks, const uint32_t* __restrict__ lcp,
-                          const uint64_t* __restrict__ pe_begin, unsigned long long* __restrict__ d) {
-    const uint64_t a = chunks[2 * blockIdx.x], b = chunks[2 * blockIdx.x + 1];
-    const uint32_t pe = (uint32_t)(a >> 40);
-    const uint64_t c0 = a & ((1ull << 40) - 1), cnt = b;
-    const uint64_t pb = pe_begin[pe], pe_end = pe_begin[pe + 1];
-    uint64_t t = 0;
-    for (uint64_t i = c0 + threadIdx.x; i < c0 + cnt; i += blockDim.x) {
-        const uint32_t x = i == pb ? 0u : lcp[i];
-        const uint32_t y = i + 1 < pe_end ? lcp[i + 1] : 0u;
-        t += (uint64_t)max(x, y) + 1;
-    }
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_down((unsigned long long)t, o);
+// Block b covers positions [b kDistChunk, (b + 1) kDistChunk) and adds the
+// contribution of each PE it overlaps (PEs of fewer than two strings
+// contribute nothing) to that PE's sum.
+__global__ void __launch_bounds__(256) k_pe_dsum(const uint32_t* __restrict__ lcp, const uint64_t* __restrict__ pe_begin,
+                                                 uint32_t p, uint64_t n, unsigned long long* __restrict__ d) {
     __shared__ unsigned long long part[4];
-    if (lane_id() == 0) part[threadIdx.x >> 6] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(d + pe, part[0] + part[1] + part[2] + part[3]);
+    const uint64_t b0 = (uint64_t)blockIdx.x * kDistChunk, b1 = min(n, b0 + kDistChunk);
+    uint32_t lo = 0, hi = p;  // the last PE that begins at or before b0
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (pe_begin[mid] <= b0) lo = mid;
+        else hi = mid;
+    }
+    for (uint32_t r = lo; r < p && pe_begin[r] < b1; ++r) {  // block-uniform
+        const uint64_t pb = pe_begin[r], pe = pe_begin[r + 1];
+        unsigned long long t = 0;
+        if (pe - pb >= 2)
+            for (uint64_t i = max(pb, b0) + threadIdx.x; i < min(pe, b1); i += blockDim.x) {
+                const uint32_t x = i == pb ? 0u : lcp[i];
+                const uint32_t y = i + 1 < pe ? lcp[i + 1] : 0u;
+                t += (uint64_t)max(x, y) + 1;
+            }
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o);
+        if (lane_id() == 0) part[threadIdx.x >> 6] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long sum = part[0] + part[1] + part[2] + part[3];
+            if (sum) atomicAdd(d + r, sum);
+        }
+        __syncthreads();  // part is rewritten for the next PE
+    }
 }
 
 // Identity order, zero LCPs, and "no metadata yet" (kNone) in the sorted
 // length array: base cases fill offset/length of the strings they place,
 // k_sorted_meta the rest.
+// kDev (device-driven form): the string count comes from dn and the grid
+// strides over it.
+template <bool kDev>
 __global__ void k_sort_init(uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, uint32_t* __restrict__ out_len,
-                            uint64_t n) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+                            uint64_t n, const uint64_t* __restrict__ dn) {
+    if (kDev) n = *dn;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         perm[i] = (uint32_t)i;
         lcp[i] = 0;
         out_len[i] = kNone;
     }
 }
 
+template <bool kDev>
 __global__ void k_sort_seed(const uint64_t* __restrict__ pe_begin, uint32_t p, uint32_t large_min,
                             Seg* __restrict__ segs, SmallSeg* __restrict__ small, SmallLists L,
-                            uint32_t* __restrict__ ctr) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= p) return;
-    const uint32_t b = (uint32_t)pe_begin[r];
-    const uint32_t cnt = (uint32_t)(pe_begin[r + 1] - pe_begin[r]);
-    if (cnt > large_min) {
-        const uint32_t k = atomicAdd(&ctr[kCtrPass + kCtrLarge], 1u);
-        segs[k] = Seg{b, cnt, 0, kNone, kNone, 0, 0};
-    } else if (cnt >= 2) {
-        push_small(small, L, ctr, b, cnt, 0);
+                            uint32_t* __restrict__ ctr, const uint32_t* __restrict__ dp) {
+    if (kDev) p = *dp;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < p; r += gridDim.x * blockDim.x) {
+        const uint32_t b = (uint32_t)pe_begin[r];
+        const uint32_t cnt = (uint32_t)(pe_begin[r + 1] - pe_begin[r]);
+        if (cnt > large_min) {
+            const uint32_t k = atomicAdd(&ctr[kCtrPass + kCtrLarge], 1u);
+            segs[k] = Seg{b, cnt, 0, kNone, kNone, 0, 0};
+        } else if (cnt >= 2) {
+            push_small(small, L, ctr, b, cnt, 0);
+        }
     }
 }
 
@@ -222,43 +241,59 @@ __device__ inline uint32_t seg_of(const Seg* __restrict__ segs, uint32_t nseg, u
     return lo;
 }
 
+// kDev: segment and tile counts from the pass counters hp, grid-stride.
+template <bool kDev>
 __global__ void k_fill_tiles(const Seg* __restrict__ segs, uint32_t nseg, uint32_t ntiles,
-                             Tile* __restrict__ tiles) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    const uint32_t i = seg_of<false>(segs, nseg, t);
-    const Seg s = segs[i];
-    const uint32_t b = s.begin + (t - s.tile0) * kTile;
-    tiles[t] = Tile{i, b, min(s.begin + s.cnt, b + kTile), kNone};
+                             Tile* __restrict__ tiles, const uint32_t* __restrict__ hp) {
+    if (kDev) {
+        nseg = hp[kCtrLarge];
+        ntiles = hp[kCtrTiles];
+    }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+        const uint32_t i = seg_of<false>(segs, nseg, t);
+        const Seg s = segs[i];
+        const uint32_t b = s.begin + (t - s.tile0) * kTile;
+        tiles[t] = Tile{i, b, min(s.begin + s.cnt, b + kTile), kNone};
+    }
 }
 
 // Per chunk and digit: the sum of the chunk's tile histograms.
+template <bool kDev>
 __global__ void __launch_bounds__(kThreads)
 k_chunk_sum(const Seg* __restrict__ segs, uint32_t nseg, const uint32_t* __restrict__ hist,
-            uint32_t* __restrict__ csum) {
-    const uint32_t k = blockIdx.x, d = threadIdx.x;
-    const Seg s = segs[seg_of<true>(segs, nseg, k)];
-    const uint32_t t0 = s.tile0 + (k - s.chunk0) * kChunkTiles;
-    const uint32_t t1 = min(s.tile0 + seg_tiles(s.cnt), t0 + kChunkTiles);
-    uint32_t tot = 0;
-    for (uint32_t t = t0; t < t1; ++t) tot += hist[(uint64_t)t * 256 + d];
-    csum[(uint64_t)k * 256 + d] = tot;
+            uint32_t* __restrict__ csum, const uint32_t* __restrict__ hp) {
+    const uint32_t d = threadIdx.x;
+    if (kDev) nseg = hp[kCtrLarge];
+    const uint32_t nch = kDev ? hp[kCtrChunks] : blockIdx.x + 1;
+    for (uint32_t k = blockIdx.x; k < nch; k += gridDim.x) {
+        const Seg s = segs[seg_of<true>(segs, nseg, k)];
+        const uint32_t t0 = s.tile0 + (k - s.chunk0) * kChunkTiles;
+        const uint32_t t1 = min(s.tile0 + seg_tiles(s.cnt), t0 + kChunkTiles);
+        uint32_t tot = 0;
+        for (uint32_t t = t0; t < t1; ++t) tot += hist[(uint64_t)t * 256 + d];
+        csum[(uint64_t)k * 256 + d] = tot;
+    }
 }
 
 // Per chunk and digit: tile offsets from the chunk's base offset.
+template <bool kDev>
 __global__ void __launch_bounds__(kThreads)
 k_chunk_apply(const Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restrict__ hist,
-              const uint32_t* __restrict__ csum) {
-    const uint32_t k = blockIdx.x, d = threadIdx.x;
-    const Seg s = segs[seg_of<true>(segs, nseg, k)];
-    const uint32_t t0 = s.tile0 + (k - s.chunk0) * kChunkTiles;
-    const uint32_t t1 = min(s.tile0 + seg_tiles(s.cnt), t0 + kChunkTiles);
-    uint32_t run = csum[(uint64_t)k * 256 + d];
-    for (uint32_t t = t0; t < t1; ++t) {
-        uint32_t* hp = &hist[(uint64_t)t * 256 + d];
-        const uint32_t x = *hp;
-        *hp = run;
-        run += x;
+              const uint32_t* __restrict__ csum, const uint32_t* __restrict__ hp) {
+    const uint32_t d = threadIdx.x;
+    if (kDev) nseg = hp[kCtrLarge];
+    const uint32_t nch = kDev ? hp[kCtrChunks] : blockIdx.x + 1;
+    for (uint32_t k = blockIdx.x; k < nch; k += gridDim.x) {
+        const Seg s = segs[seg_of<true>(segs, nseg, k)];
+        const uint32_t t0 = s.tile0 + (k - s.chunk0) * kChunkTiles;
+        const uint32_t t1 = min(s.tile0 + seg_tiles(s.cnt), t0 + kChunkTiles);
+        uint32_t run = csum[(uint64_t)k * 256 + d];
+        for (uint32_t t = t0; t < t1; ++t) {
+            uint32_t* h = &hist[(uint64_t)t * 256 + d];
+            const uint32_t x = *h;
+            *h = run;
+            run += x;
+        }
     }
 }
 
@@ -294,15 +329,16 @@ __device__ inline void seg_depths(const Seg& s, uint32_t& kd, uint32_t& dep) {
 // The key gather keeps the keys if the block minimum is the segment's, which
 // the whole grid decides.
 template <uint32_t kSkipLanes, uint32_t kU>
-__global__ void __launch_bounds__(kThreads)
-k_prefix_skip(Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
-              const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
-              const uint32_t* __restrict__ len, uint64_t* __restrict__ key, uint32_t* __restrict__ pb) {
+__device__ __forceinline__ void prefix_skip_tile(uint32_t tb, Tile* __restrict__ tiles, Seg* __restrict__ segs,
+                                                 const uint32_t* __restrict__ perm,
+                                                 const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ len, uint64_t* __restrict__ key,
+                                                 uint32_t* __restrict__ pb) {
     __shared__ uint64_t t_off[kTile];  // offset; after a string's event its kept characters
     __shared__ uint32_t t_len[kTile];
     __shared__ uint32_t t_res[kTile];  // event depth - segment depth if characters were kept
     __shared__ uint32_t bmin;
-    const Tile T = tiles[blockIdx.x];
+    const Tile T = tiles[tb];
     const Seg S = segs[T.seg];
     if (S.kdepth != kNone) return;
     const uint32_t cnt = T.end - T.begin;
@@ -429,7 +465,7 @@ k_prefix_skip(Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* 
     if (bm == kNone) return;
     if (threadIdx.x == 0) {
         atomicMin(&segs[T.seg].cp, bm);
-        tiles[blockIdx.x].keyed = bm;
+        tiles[tb].keyed = bm;
     }
     // a string equals the reference before its event depth r >= bm: its key
     // at bm is the reference's r - bm characters, then its kept ones
@@ -447,12 +483,31 @@ k_prefix_skip(Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* 
     }
 }
 
-// ------------------------------------------------------------ key gather
+// kDev (device-driven form, persistent grid): the tile count comes from the
+// pass counters hp and every block loops over tiles.
+template <uint32_t kSkipLanes, uint32_t kU, bool kDev>
 __global__ void __launch_bounds__(kThreads)
-k_gather(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
-         const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
-         const uint32_t* __restrict__ len, uint64_t* __restrict__ key) {
-    const Tile T = tiles[blockIdx.x];
+k_prefix_skip(Tile* __restrict__ tiles, Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
+              const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
+              const uint32_t* __restrict__ len, uint64_t* __restrict__ key, uint32_t* __restrict__ pb,
+              const uint32_t* __restrict__ hp) {
+    if constexpr (!kDev) {
+        prefix_skip_tile<kSkipLanes, kU>(blockIdx.x, tiles, segs, perm, chars, off, len, key, pb);
+    } else {
+        const uint32_t nb = hp[kCtrTiles];
+        for (uint32_t tb = blockIdx.x; tb < nb; tb += gridDim.x) {
+            prefix_skip_tile<kSkipLanes, kU>(tb, tiles, segs, perm, chars, off, len, key, pb);
+            __syncthreads();  // LDS of this tile read before the next tile's
+        }
+    }
+}
+
+// ------------------------------------------------------------ key gather
+__device__ __forceinline__ void gather_tile(uint32_t tb, const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                            const uint32_t* __restrict__ perm, const uint8_t* __restrict__ chars,
+                                            const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                                            uint64_t* __restrict__ key) {
+    const Tile T = tiles[tb];
     const Seg S = segs[T.seg];
     // the prefix skip wrote the ids, and the keys if its block minimum is the segment's
     if (S.kdepth != kNone || T.keyed == S.cp) return;
@@ -463,12 +518,27 @@ k_gather(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uin
     }
 }
 
-// ------------------------------------------------------------ histogram
+template <bool kDev>
 __global__ void __launch_bounds__(kThreads)
-k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint64_t* __restrict__ key,
-       uint32_t* __restrict__ hist) {
+k_gather(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint32_t* __restrict__ perm,
+         const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
+         const uint32_t* __restrict__ len, uint64_t* __restrict__ key, const uint32_t* __restrict__ hp) {
+    if constexpr (!kDev) {
+        gather_tile(blockIdx.x, tiles, segs, perm, chars, off, len, key);
+    } else {
+        const uint32_t nb = hp[kCtrTiles];
+        for (uint32_t tb = blockIdx.x; tb < nb; tb += gridDim.x) {
+            gather_tile(tb, tiles, segs, perm, chars, off, len, key);
+            __syncthreads();  // LDS of this tile read before the next tile's
+        }
+    }
+}
+
+// ------------------------------------------------------------ histogram
+__device__ __forceinline__ void hist_tile(uint32_t tb, const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                          const uint64_t* __restrict__ key, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[256];
-    const Tile T = tiles[blockIdx.x];
+    const Tile T = tiles[tb];
     const Seg S = segs[T.seg];
     uint32_t kd, dep;
     seg_depths(S, kd, dep);
@@ -478,7 +548,22 @@ k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint6
     for (uint32_t e = T.begin + threadIdx.x; e < T.end; e += kThreads)
         atomicAdd(&h[(uint32_t)(key[e] >> sh) & 0xFF], 1u);
     __syncthreads();
-    hist[(uint64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+    hist[(uint64_t)tb * 256 + threadIdx.x] = h[threadIdx.x];
+}
+
+template <bool kDev>
+__global__ void __launch_bounds__(kThreads)
+k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint64_t* __restrict__ key,
+       uint32_t* __restrict__ hist, const uint32_t* __restrict__ hp) {
+    if constexpr (!kDev) {
+        hist_tile(blockIdx.x, tiles, segs, key, hist);
+    } else {
+        const uint32_t nb = hp[kCtrTiles];
+        for (uint32_t tb = blockIdx.x; tb < nb; tb += gridDim.x) {
+            hist_tile(tb, tiles, segs, key, hist);
+            __syncthreads();  // LDS of this tile read before the next tile's
+        }
+    }
 }
 
 // ------------------------------------------------------------ scan/classify
@@ -490,17 +575,18 @@ k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint6
 // one atomic per list (same-address atomics of many small blocks serialise).
 constexpr uint32_t kScGroups = 4;
 
-__global__ void __launch_bounds__(kThreads * kScGroups)
-k_scan_classify(const Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
-                Seg* __restrict__ next, SmallSeg* __restrict__ small, SmallLists L, uint32_t large_min,
-                uint32_t* __restrict__ ctr, uint32_t* __restrict__ next_large, uint32_t* __restrict__ lcp,
-                uint32_t key_buf) {
+__device__ __forceinline__ void scan_classify_block(uint32_t tb, const Seg* __restrict__ segs, uint32_t nseg,
+                                                    uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
+                                                    Seg* __restrict__ next, SmallSeg* __restrict__ small,
+                                                    const SmallLists& L, uint32_t large_min, uint32_t* __restrict__ ctr,
+                                                    uint32_t* __restrict__ next_large, uint32_t* __restrict__ lcp,
+                                                    uint32_t key_buf) {
     __shared__ uint32_t sc_all[kScGroups][256];
     __shared__ uint32_t first_all[kScGroups];
     __shared__ uint32_t lcnt[kClasses + 1], lelem[kClasses], lbase[kClasses + 1];
     const uint32_t grp = threadIdx.x >> 8;
     const uint32_t d = threadIdx.x & 255;
-    const uint32_t si = blockIdx.x * kScGroups + grp;
+    const uint32_t si = tb * kScGroups + grp;
     const bool live = si < nseg;  // group-uniform
     uint32_t* const sc = sc_all[grp];
     Seg S{};
@@ -569,6 +655,27 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restric
     }
 }
 
+// kDev: the segment count comes from the pass counters hp; every block loops
+// over groups of kScGroups segments.
+template <bool kDev>
+__global__ void __launch_bounds__(kThreads * kScGroups)
+k_scan_classify(const Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
+                Seg* __restrict__ next, SmallSeg* __restrict__ small, SmallLists L, uint32_t large_min,
+                uint32_t* __restrict__ ctr, uint32_t* __restrict__ next_large, uint32_t* __restrict__ lcp,
+                uint32_t key_buf, const uint32_t* __restrict__ hp) {
+    if constexpr (!kDev) {
+        scan_classify_block(blockIdx.x, segs, nseg, hist, csum, next, small, L, large_min, ctr, next_large, lcp,
+                            key_buf);
+    } else {
+        nseg = hp[kCtrLarge];
+        const uint32_t nb = (nseg + kScGroups - 1) / kScGroups;
+        for (uint32_t tb = blockIdx.x; tb < nb; tb += gridDim.x) {
+            scan_classify_block(tb, segs, nseg, hist, csum, next, small, L, large_min, ctr, next_large, lcp, key_buf);
+            __syncthreads();
+        }
+    }
+}
+
 // ------------------------------------------------------------ scatter
 // Stable scatter of one tile by its digit.  The tile is first put in digit
 // order in LDS (wave ballots rank equal digits, in index order) and then
@@ -579,11 +686,11 @@ constexpr uint32_t kScT = 512;           // scatter threads per tile
 constexpr uint32_t kScI = kTile / kScT;  // records per thread
 static_assert(kScI * kScT == kTile && kScT >= 256 && kScT % 64 == 0, "scatter tile shape");
 
-__global__ void __launch_bounds__(kScT)
-k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint32_t* __restrict__ hist,
-          const uint64_t* __restrict__ key_in, const uint32_t* __restrict__ pb_in,
-          uint64_t* __restrict__ key_out, uint32_t* __restrict__ pb_out, uint32_t* __restrict__ perm,
-          uint32_t* __restrict__ lcp) {
+__device__ __forceinline__ void scatter_tile(uint32_t tb, const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                             const uint32_t* __restrict__ hist, const uint64_t* __restrict__ key_in,
+                                             const uint32_t* __restrict__ pb_in, uint64_t* __restrict__ key_out,
+                                             uint32_t* __restrict__ pb_out, uint32_t* __restrict__ perm,
+                                             uint32_t* __restrict__ lcp) {
     __shared__ uint64_t skey[kTile];
     __shared__ uint32_t sid[kTile];
     __shared__ uint32_t gbase[256];   // the digit's output offset for this tile
@@ -591,7 +698,7 @@ k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const ui
     __shared__ uint32_t run[256];     // next free slot of the digit
     __shared__ uint32_t wcnt[kScT / 64][256];
     __shared__ uint32_t scan[256];
-    const Tile T = tiles[blockIdx.x];
+    const Tile T = tiles[tb];
     const Seg S = segs[T.seg];
     uint32_t kd, dep;
     seg_depths(S, kd, dep);
@@ -599,7 +706,7 @@ k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const ui
     const uint32_t tid = threadIdx.x, w = tid >> 6;
     const bool dig = tid < 256;  // the threads that own a digit
     if (dig) {
-        gbase[tid] = hist[(uint64_t)blockIdx.x * 256 + tid];
+        gbase[tid] = hist[(uint64_t)tb * 256 + tid];
         run[tid] = 0;
         for (uint32_t k = 0; k < kScT / 64; ++k) wcnt[k][tid] = 0;
     }
@@ -681,6 +788,23 @@ k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const ui
     }
 }
 
+template <bool kDev>
+__global__ void __launch_bounds__(kScT)
+k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint32_t* __restrict__ hist,
+          const uint64_t* __restrict__ key_in, const uint32_t* __restrict__ pb_in,
+          uint64_t* __restrict__ key_out, uint32_t* __restrict__ pb_out, uint32_t* __restrict__ perm,
+          uint32_t* __restrict__ lcp, const uint32_t* __restrict__ hp) {
+    if constexpr (!kDev) {
+        scatter_tile(blockIdx.x, tiles, segs, hist, key_in, pb_in, key_out, pb_out, perm, lcp);
+    } else {
+        const uint32_t nb = hp[kCtrTiles];
+        for (uint32_t tb = blockIdx.x; tb < nb; tb += gridDim.x) {
+            scatter_tile(tb, tiles, segs, hist, key_in, pb_in, key_out, pb_out, perm, lcp);
+            __syncthreads();  // LDS of this tile read before the next tile's
+        }
+    }
+}
+
 // ------------------------------------------------------------ small sort
 // Block-wide exclusive max-scan helper over kThreads threads.
 __device__ inline uint32_t block_excl_max(uint32_t v, uint32_t* tmp) {
@@ -714,11 +838,11 @@ __device__ inline uint32_t lcp_walk(const uint8_t* __restrict__ chars, uint64_t 
 constexpr uint32_t kCap = kSmallMax;
 constexpr uint32_t kPer = kCap / kThreads;  // elements per thread
 
-__global__ void __launch_bounds__(kThreads)
-k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2,
-             uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
-             const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
-             uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len) {
+__device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __restrict__ segs, uint32_t nseg,
+                                                 uint32_t cls_log2, uint32_t* __restrict__ perm,
+                                                 uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
+                                                 const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                                                 uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len) {
     __shared__ uint64_t K[kCap];
     __shared__ uint32_t G[kCap];
     __shared__ uint32_t I[kCap];
@@ -729,7 +853,7 @@ k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2
     __shared__ uint32_t tmp[kThreads / 64];
     const uint32_t CLS = 1u << cls_log2;
     const uint32_t spb = kCap >> cls_log2;
-    const uint32_t s0 = blockIdx.x * spb;
+    const uint32_t s0 = tb * spb;
     const uint32_t tid = threadIdx.x;
 
     for (uint32_t e = tid; e < kCap; e += kThreads) {
@@ -880,6 +1004,27 @@ k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2
     }
 }
 
+// kDev: the class's segment count comes from dcnt; every block loops over
+// groups of kCap >> cls_log2 segments.
+template <bool kDev>
+__global__ void __launch_bounds__(kThreads)
+k_small_sort(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2,
+             uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
+             const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+             uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len, const uint32_t* __restrict__ dcnt) {
+    if constexpr (!kDev) {
+        small_sort_block(blockIdx.x, segs, nseg, cls_log2, perm, lcp, chars, off, len, out_off, out_len);
+    } else {
+        nseg = *dcnt;
+        const uint32_t spb = kCap >> cls_log2;
+        const uint32_t nb = (nseg + spb - 1) / spb;
+        for (uint32_t tb = blockIdx.x; tb < nb; tb += gridDim.x) {
+            small_sort_block(tb, segs, nseg, cls_log2, perm, lcp, chars, off, len, out_off, out_len);
+            __syncthreads();
+        }
+    }
+}
+
 // True if a key whose first w characters are valid reaches the end of its
 // string within them (the key is zero after a string's end).
 __device__ inline bool key_ended(uint64_t k, uint32_t w) { return ((k >> (64 - 8 * w)) & 0xFF) == 0; }
@@ -912,17 +1057,17 @@ __global__ void k_pack_offlen(const uint64_t* __restrict__ off, const uint32_t* 
         out[i] = OffLen{off[i], len[i]};
 }
 
-__global__ void __launch_bounds__(kThreads)
-k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2,
-                  uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
-                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
-                  const OffLen* __restrict__ offlen,
-                  const uint64_t* __restrict__ keyA, const uint64_t* __restrict__ keyB,
-                  uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len) {
+__device__ __forceinline__ void small_sort_wave_unit(uint64_t wave, const SmallSeg* __restrict__ segs, uint32_t nseg,
+                                                     uint32_t cls_log2, uint32_t* __restrict__ perm,
+                                                     uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
+                                                     const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                                                     const OffLen* __restrict__ offlen,
+                                                     const uint64_t* __restrict__ keyA,
+                                                     const uint64_t* __restrict__ keyB,
+                                                     uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len) {
     const uint32_t lane = lane_id();
     const uint32_t CLS = 1u << cls_log2;
     const uint32_t spw = 64u >> cls_log2;
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t slot_start = lane & ~(CLS - 1), j = lane & (CLS - 1);
     const uint64_t sidx = wave * spw + (lane >> cls_log2);
     bool valid = false;
@@ -1046,18 +1191,94 @@ k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls
     }
 }
 
+// kDev: the class's segment count comes from dcnt; every wave loops over
+// groups of 64 >> cls_log2 segments.
+template <bool kDev>
+__global__ void __launch_bounds__(kThreads)
+k_small_sort_wave(const SmallSeg* __restrict__ segs, uint32_t nseg, uint32_t cls_log2,
+                  uint32_t* __restrict__ perm, uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
+                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                  const OffLen* __restrict__ offlen,
+                  const uint64_t* __restrict__ keyA, const uint64_t* __restrict__ keyB,
+                  uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len, const uint32_t* __restrict__ dcnt) {
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // wave-uniform
+    if constexpr (!kDev) {
+        small_sort_wave_unit(wave, segs, nseg, cls_log2, perm, lcp, chars, off, len, offlen, keyA, keyB, out_off,
+                             out_len);
+    } else {
+        nseg = *dcnt;
+        const uint32_t spw = 64u >> cls_log2;
+        const uint64_t nw = (nseg + spw - 1) / spw;
+        for (uint64_t w = wave; w < nw; w += ((uint64_t)gridDim.x * blockDim.x) >> 6)
+            small_sort_wave_unit(w, segs, nseg, cls_log2, perm, lcp, chars, off, len, offlen, keyA, keyB, out_off,
+                                 out_len);
+    }
+}
+
 // Offset and length per sorted position: every later stage streams these
 // instead of gathering through the permutation.
 // With only_missing, positions the base cases already filled (s_len not
 // kNone) are skipped.
+template <bool kDev>
 __global__ void k_sorted_meta(const uint32_t* __restrict__ perm, const uint64_t* __restrict__ off,
                               const uint32_t* __restrict__ len, uint64_t n, uint64_t* __restrict__ s_off,
-                              uint32_t* __restrict__ s_len, int only_missing) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && (!only_missing || s_len[i] == kNone)) {
-        const uint32_t id = perm[i];
-        s_off[i] = off[id];
-        s_len[i] = len[id];
+                              uint32_t* __restrict__ s_len, int only_missing, const uint64_t* __restrict__ dn) {
+    if (kDev) n = *dn;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (!only_missing || s_len[i] == kNone) {
+            const uint32_t id = perm[i];
+            s_off[i] = off[id];
+            s_len[i] = len[id];
+        }
+    }
+}
+
+// Device-driven base cases, every class in one launch: the one-wave classes
+// 1..wave_class_max share one kernel, the block classes above share another.
+// A unit (a wave's or a block's group of segments) of class c is found from
+// the per-class unit counts, read from the device counters.
+__global__ void __launch_bounds__(kThreads)
+k_small_sort_wave_all(const SmallSeg* __restrict__ small, SmallLists L, uint32_t cmax, uint32_t* __restrict__ perm,
+                      uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
+                      const uint32_t* __restrict__ len, const uint64_t* __restrict__ keyA,
+                      const uint64_t* __restrict__ keyB, uint64_t* __restrict__ out_off,
+                      uint32_t* __restrict__ out_len, const uint32_t* __restrict__ ctr) {
+    uint64_t ubeg[kWaveClassMax + 2];
+    uint64_t tot = 0;
+    for (uint32_t c = 1; c <= cmax; ++c) {
+        ubeg[c] = tot;
+        const uint32_t spw = 64u >> c;
+        tot += (ctr[kCtrSmall + c] + spw - 1) / spw;
+    }
+    ubeg[cmax + 1] = tot;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t u = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; u < tot; u += nw) {  // wave-uniform
+        uint32_t c = 1;
+        while (u >= ubeg[c + 1]) ++c;
+        small_sort_wave_unit(u - ubeg[c], small + L.off[c], ctr[kCtrSmall + c], c, perm, lcp, chars, off, len,
+                             nullptr, keyA, keyB, out_off, out_len);
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_small_sort_all(const SmallSeg* __restrict__ small, SmallLists L, uint32_t cmin, uint32_t* __restrict__ perm,
+                 uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars, const uint64_t* __restrict__ off,
+                 const uint32_t* __restrict__ len, uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len,
+                 const uint32_t* __restrict__ ctr) {
+    uint32_t ubeg[kClasses + 1];
+    uint32_t tot = 0;
+    for (uint32_t c = cmin; c < kClasses; ++c) {
+        ubeg[c] = tot;
+        const uint32_t spb = kCap >> c;
+        tot += (ctr[kCtrSmall + c] + spb - 1) / spb;
+    }
+    ubeg[kClasses] = tot;
+    for (uint32_t u = blockIdx.x; u < tot; u += gridDim.x) {  // block-uniform
+        uint32_t c = cmin;
+        while (u >= ubeg[c + 1]) ++c;
+        small_sort_block(u - ubeg[c], small + L.off[c], ctr[kCtrSmall + c], c, perm, lcp, chars, off, len, out_off,
+                         out_len);
+        __syncthreads();
     }
 }
 
@@ -1104,68 +1325,68 @@ static SortCaps reserve_sort(dss_set_s* s, uint64_t n, uint32_t p) {
     X.hist.reserve(c.tile_cap * 256 * 4);
     X.csum.reserve(c.chunk_cap * 256 * 4);
     X.small.reserve(std::max<uint64_t>(c.small_total, 1) * sizeof(SmallSeg));
-    X.counters.reserve(64 * 4);
+    X.counters.reserve(64 * 4 + 16);  // 64 counters, then the device-driven form's string and PE counts
     return c;
 }
 
-void sort_lcp(dss_set_s* s) {
-    const uint64_t n = s->n;
-    const hipStream_t st = s->stream;
-    s->sorted = false;
-    const SortCaps caps = reserve_sort(s, n, s->p);
-    if (n == 0) {
-        s->sorted = true;
-        return;
+// Pass list buffers: the current and next segment lists, key and id
+// buffers, and which of the two per-pass counter sets is the current one.
+struct PassBufs {
+    Seg* cur;
+    Seg* nxt;
+    uint64_t* kc;
+    uint64_t* kn;
+    uint32_t* pc_ids;
+    uint32_t* pn_ids;
+    uint32_t par;
+    void swap_pass() {
+        par ^= 1;
+        std::swap(cur, nxt);
+        std::swap(kc, kn);
+        std::swap(pc_ids, pn_ids);
     }
+};
+
+static PassBufs first_pass_bufs(SortScratch& X) {
+    return PassBufs{X.segA.as<Seg>(), X.segB.as<Seg>(), X.keyA.as<uint64_t>(), X.keyB.as<uint64_t>(),
+                    X.pbA.as<uint32_t>(), X.pbB.as<uint32_t>(), 0};
+}
+
+static const char* const kClassNames[kClasses] = {
+    "small_sort_c0", "small_sort_c1", "small_sort_c2", "small_sort_c3", "small_sort_c4", "small_sort_c5",
+    "small_sort_c6", "small_sort_c7", "small_sort_c8", "small_sort_c9", "small_sort_c10", "small_sort_c11"};
+
+// Host-driven MSD passes from the state in `b` until the segment list is
+// empty: every pass sizes its launches from the counters read back after the
+// previous one (h_ctr holds them on entry and on return).
+static void host_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint32_t* h_ctr, PassBufs& b,
+                        bool trace) {
+    const hipStream_t st = s->stream;
     SortScratch& X = s->ss;
     const SortParams& T = s->sort_params;
-    const uint64_t seg_cap = caps.seg_cap, tile_cap = caps.tile_cap, chunk_cap = caps.chunk_cap;
     const SmallLists L = caps.L;
-    uint32_t* ctr = X.counters.as<uint32_t>();
-    fill_small(st, ctr, 0, 64 * 4);
-
-    {
-        KScope ks(s->timer, st, "sort_init", n * 8.0);
-        hipLaunchKernelGGL(k_sort_init, dim3((n + 255) / 256), dim3(256), 0, st, s->perm.as<uint32_t>(),
-                           s->lcp.as<uint32_t>(), s->s_len.as<uint32_t>(), n);
-    }
-    hipLaunchKernelGGL(k_sort_seed, dim3((s->p + 255) / 256), dim3(256), 0, st,
-                       s->pe_begin_d.as<uint64_t>(), s->p, T.large_min, X.segA.as<Seg>(), X.small.as<SmallSeg>(), L,
-                       ctr);
-    DSS_HIP(hipGetLastError());
-
-    Seg* cur = X.segA.as<Seg>();
-    Seg* nxt = X.segB.as<Seg>();
-    uint64_t* kc = X.keyA.as<uint64_t>();
-    uint64_t* kn = X.keyB.as<uint64_t>();
-    uint32_t* pc_ids = X.pbA.as<uint32_t>();
-    uint32_t* pn_ids = X.pbB.as<uint32_t>();
-    // every pass ends with one read-back of all counters: the next list's
-    // segment count and its tile/chunk counts (k_build_tiles of the next pass
-    // runs before the read-back), and the small-class counts
-    uint32_t h_ctr[64];
-    uint32_t par = 0;  // counter set of the current pass
-    hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, X.segA.as<Seg>(), ctr + kCtrPass);
-    s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
-    const bool trace = getenv("DSS_SORT_TRACE") != nullptr;
     const uint8_t* chars = s->arena();
     const uint64_t* off = s->off.as<uint64_t>();
     const uint32_t* len = s->len.as<uint32_t>();
     uint32_t* perm = s->perm.as<uint32_t>();
     uint32_t* lcp = s->lcp.as<uint32_t>();
     for (;;) {
-        uint32_t* pn = ctr + kCtrPass + 4 * (par ^ 1);  // the next pass
-        const uint32_t* hp = h_ctr + kCtrPass + 4 * par;
+        uint32_t* pn = ctr + kCtrPass + 4 * (b.par ^ 1);  // the next pass
+        const uint32_t* hp = h_ctr + kCtrPass + 4 * b.par;
         const uint32_t nlarge = hp[kCtrLarge];
         if (!nlarge) break;
         const uint32_t nt = hp[kCtrTiles];
         const uint32_t nchunks = hp[kCtrChunks];
-        DSS_REQUIRE(nlarge <= seg_cap, DSS_ERANGE, "segment list overflow");
-        DSS_REQUIRE(nt <= tile_cap, DSS_ERANGE, "tile list overflow");
-        DSS_REQUIRE(nchunks <= chunk_cap, DSS_ERANGE, "chunk list overflow");
+        DSS_REQUIRE(nlarge <= caps.seg_cap, DSS_ERANGE, "segment list overflow");
+        DSS_REQUIRE(nt <= caps.tile_cap, DSS_ERANGE, "tile list overflow");
+        DSS_REQUIRE(nchunks <= caps.chunk_cap, DSS_ERANGE, "chunk list overflow");
+        Seg* const cur = b.cur;
+        Seg* const nxt = b.nxt;
+        uint64_t* const kc = b.kc;
+        uint64_t* const kn = b.kn;
         fill_small(st, pn, 0, 16);
-        hipLaunchKernelGGL(k_fill_tiles, dim3((nt + 255) / 256), dim3(256), 0, st, cur, nlarge, nt,
-                           X.tiles.as<Tile>());
+        hipLaunchKernelGGL(k_fill_tiles<false>, dim3((nt + 255) / 256), dim3(256), 0, st, cur, nlarge, nt,
+                           X.tiles.as<Tile>(), nullptr);
         Tile* tl = X.tiles.as<Tile>();
         // elements in large segments of this pass (for the algorithmic byte counts)
         std::vector<Seg> hseg;
@@ -1177,44 +1398,47 @@ void sort_lcp(dss_set_s* s) {
         }
         {
             KScope ks(s->timer, st, "prefix_skip", 0.0);
-            decltype(&k_prefix_skip<8, 1>) kern = &k_prefix_skip<8, 1>;
+            decltype(&k_prefix_skip<8, 1, false>) kern = &k_prefix_skip<8, 1, false>;
             if (T.skip_lanes == 16)
-                kern = T.skip_strings == 4 ? &k_prefix_skip<16, 4>
-                       : T.skip_strings == 2 ? &k_prefix_skip<16, 2> : &k_prefix_skip<16, 1>;
+                kern = T.skip_strings == 4 ? &k_prefix_skip<16, 4, false>
+                       : T.skip_strings == 2 ? &k_prefix_skip<16, 2, false> : &k_prefix_skip<16, 1, false>;
             else
-                kern = T.skip_strings == 4 ? &k_prefix_skip<8, 4>
-                       : T.skip_strings == 2 ? &k_prefix_skip<8, 2> : &k_prefix_skip<8, 1>;
-            hipLaunchKernelGGL(kern, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc, pc_ids);
+                kern = T.skip_strings == 4 ? &k_prefix_skip<8, 4, false>
+                       : T.skip_strings == 2 ? &k_prefix_skip<8, 2, false> : &k_prefix_skip<8, 1, false>;
+            hipLaunchKernelGGL(kern, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc, b.pc_ids,
+                               nullptr);
         }
         {
             KScope ks(s->timer, st, "gather", 0.0);  // bytes amended after the pass
-            hipLaunchKernelGGL(k_gather, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc);
+            hipLaunchKernelGGL(k_gather<false>, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc,
+                               nullptr);
         }
         {
             KScope ks(s->timer, st, "hist", (double)cnt_large * 8.0);
-            hipLaunchKernelGGL(k_hist, dim3(nt), dim3(kThreads), 0, st, tl, cur, kc, X.hist.as<uint32_t>());
+            hipLaunchKernelGGL(k_hist<false>, dim3(nt), dim3(kThreads), 0, st, tl, cur, kc, X.hist.as<uint32_t>(),
+                               nullptr);
         }
         {
             KScope ks(s->timer, st, "scan_classify", (double)nt * 256 * 4 * 3);
             if (nchunks)
-                hipLaunchKernelGGL(k_chunk_sum, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
-                                   X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
-            hipLaunchKernelGGL(k_scan_classify, dim3((nlarge + kScGroups - 1) / kScGroups), dim3(kThreads * kScGroups),
-                               0, st, cur, nlarge, X.hist.as<uint32_t>(), X.csum.as<uint32_t>(), nxt,
-                               X.small.as<SmallSeg>(), L, T.large_min, ctr,
-                               pn + kCtrLarge, lcp, kn == X.keyB.as<uint64_t>() ? 1u : 0u);
+                hipLaunchKernelGGL(k_chunk_sum<false>, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
+                                   X.hist.as<uint32_t>(), X.csum.as<uint32_t>(), nullptr);
+            hipLaunchKernelGGL(k_scan_classify<false>, dim3((nlarge + kScGroups - 1) / kScGroups),
+                               dim3(kThreads * kScGroups), 0, st, cur, nlarge, X.hist.as<uint32_t>(),
+                               X.csum.as<uint32_t>(), nxt, X.small.as<SmallSeg>(), L, T.large_min, ctr,
+                               pn + kCtrLarge, lcp, kn == X.keyB.as<uint64_t>() ? 1u : 0u, nullptr);
             if (nchunks)
-                hipLaunchKernelGGL(k_chunk_apply, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
-                                   X.hist.as<uint32_t>(), X.csum.as<uint32_t>());
+                hipLaunchKernelGGL(k_chunk_apply<false>, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
+                                   X.hist.as<uint32_t>(), X.csum.as<uint32_t>(), nullptr);
         }
         {
             KScope ks(s->timer, st, "scatter", (double)cnt_large * 28.0);
-            hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kScT), 0, st, tl, cur, X.hist.as<uint32_t>(), kc,
-                               pc_ids, kn, pn_ids, perm, lcp);
+            hipLaunchKernelGGL(k_scatter<false>, dim3(nt), dim3(kScT), 0, st, tl, cur, X.hist.as<uint32_t>(), kc,
+                               b.pc_ids, kn, b.pn_ids, perm, lcp, nullptr);
         }
         hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, nxt, pn);
         DSS_HIP(hipGetLastError());
-        s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
+        s->pp.mbox.fetch(st, h_ctr, ctr, 64 * 4);
         if (s->timer.enabled) {
             // prefix skip: per string of a segment that gathers keys, its
             // characters up to the common prefix + 8 (the key), id, offset
@@ -1237,16 +1461,28 @@ void sort_lcp(dss_set_s* s) {
             std::vector<Seg> hs(std::min<uint32_t>(nlarge, 4));
             DSS_HIP(hipMemcpy(hs.data(), cur, hs.size() * sizeof(Seg), hipMemcpyDeviceToHost));
             fprintf(stderr, "[dss sort] pass nlarge=%u ntiles=%u next=%u", nlarge, nt,
-                    h_ctr[kCtrPass + 4 * (par ^ 1) + kCtrLarge]);
+                    h_ctr[kCtrPass + 4 * (b.par ^ 1) + kCtrLarge]);
             for (auto& q : hs)
                 fprintf(stderr, " {b=%u c=%u d=%u kd=%d cp=%d}", q.begin, q.cnt, q.depth, (int)q.kdepth, (int)q.cp);
             fprintf(stderr, "\n");
         }
-        par ^= 1;
-        std::swap(cur, nxt);
-        std::swap(kc, kn);
-        std::swap(pc_ids, pn_ids);
+        b.swap_pass();
     }
+}
+
+// Host-driven base cases (from the small-class counts in h_ctr) and the
+// sorted offset/length arrays.
+static void host_base_cases(dss_set_s* s, const SortCaps& caps, const uint32_t* h_ctr, bool trace) {
+    const hipStream_t st = s->stream;
+    const uint64_t n = s->n;
+    SortScratch& X = s->ss;
+    const SortParams& T = s->sort_params;
+    const SmallLists L = caps.L;
+    const uint8_t* chars = s->arena();
+    const uint64_t* off = s->off.as<uint64_t>();
+    const uint32_t* len = s->len.as<uint32_t>();
+    uint32_t* perm = s->perm.as<uint32_t>();
+    uint32_t* lcp = s->lcp.as<uint32_t>();
     const uint32_t* cls = h_ctr + kCtrSmall;  // small-class segment and string counts
     if (trace) {
         fprintf(stderr, "[dss sort] small classes:");
@@ -1273,32 +1509,266 @@ void sort_lcp(dss_set_s* s) {
         if (!cnt) continue;
         DSS_REQUIRE(cnt <= L.cap[c], DSS_ERANGE, "small list overflow");
         // per string: id read, offset/length, one 8-char key, id + lcp writes
-        static const char* const kClassNames[kClasses] = {
-            "small_sort_c0", "small_sort_c1", "small_sort_c2", "small_sort_c3", "small_sort_c4", "small_sort_c5",
-            "small_sort_c6", "small_sort_c7", "small_sort_c8", "small_sort_c9", "small_sort_c10", "small_sort_c11"};
         KScope ks(s->timer, st, trace ? kClassNames[c] : "small_sort", (double)cls[kClasses + c] * 32.0);
         if (c <= T.wave_class_max) {
             const uint64_t waves = (cnt + (64u >> c) - 1) / (64u >> c);
-            hipLaunchKernelGGL(k_small_sort_wave, dim3((waves + 3) / 4), dim3(kThreads), 0, st,
+            hipLaunchKernelGGL(k_small_sort_wave<false>, dim3((waves + 3) / 4), dim3(kThreads), 0, st,
                                X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len, offlen,
                                X.keyA.as<uint64_t>(), X.keyB.as<uint64_t>(), s->s_off.as<uint64_t>(),
-                               s->s_len.as<uint32_t>());
+                               s->s_len.as<uint32_t>(), nullptr);
         } else {
             const uint32_t spb = kCap >> c;
             const uint32_t blocks = (cnt + spb - 1) / spb;
-            hipLaunchKernelGGL(k_small_sort, dim3(blocks), dim3(kThreads), 0, st,
+            hipLaunchKernelGGL(k_small_sort<false>, dim3(blocks), dim3(kThreads), 0, st,
                                X.small.as<SmallSeg>() + L.off[c], cnt, c, perm, lcp, chars, off, len,
-                               s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>());
+                               s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(), nullptr);
         }
     }
     {
         // the sorted-length check per position; positions the base cases did
         // not place (normally few) add an id/offset/length gather
         KScope ks(s->timer, st, "sort_meta", n * 4.0);
-        hipLaunchKernelGGL(k_sorted_meta, dim3((n + 255) / 256), dim3(256), 0, st, perm, off, len, n,
-                           s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(), 1);
+        hipLaunchKernelGGL(k_sorted_meta<false>, dim3((n + 255) / 256), dim3(256), 0, st, perm, off, len, n,
+                           s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(), 1, nullptr);
     }
     DSS_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ device-driven
+// The device-driven form of the local sort: every kernel reads its work
+// counts from the device counters and strides a fixed grid over them, so the
+// launch sequence does not depend on the data and is captured once per set
+// (for its buffer capacities and parameters) in two hipGraphs: (1)
+// initialisation and kGraphPasses MSD passes, (2) the base cases of all
+// classes (one launch for the one-wave classes, one for the block classes)
+// and the sorted offset/length arrays.  Between them one read-back
+// checks whether the list of large segments is empty; if not, the remaining
+// passes run host-driven.  String and PE counts are device parameters written
+// before each replay, and the list capacities follow the set's reservation,
+// so one capture serves every batch that fits it.
+constexpr uint32_t kGraphPasses = 2;
+// pass_mode 0 takes the device-driven form for sets of at most
+// kGraphMaxStrings strings over at least kGraphMinPes PEs (batches of jobs),
+// where it measured at least as fast as the host-driven form; single-PE sorts
+// of 10K-4M strings measured 5-8 % slower (a graph node costs about as much
+// to dispatch as an eager launch, and the speculative empty passes add nodes).
+constexpr uint64_t kGraphMaxStrings = 1ull << 22;
+constexpr uint32_t kGraphMinPes = 16;
+constexpr uint32_t kDevGrid = 1024;  // blocks of the persistent tile loops
+
+__global__ void k_sort_params(uint64_t* __restrict__ dn, uint32_t* __restrict__ dp, uint64_t n, uint32_t p) {
+    *dn = n;
+    *dp = p;
+}
+
+static uint32_t grid_cap(uint64_t items, uint32_t per_block, uint32_t cap) {
+    const uint64_t b = (items + per_block - 1) / per_block;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap));
+}
+
+template <uint32_t kL, uint32_t kU>
+static void launch_dev_pass(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, PassBufs& b) {
+    const hipStream_t st = s->stream;
+    SortScratch& X = s->ss;
+    const SortParams& T = s->sort_params;
+    const uint8_t* chars = s->arena();
+    const uint64_t* off = s->off.as<uint64_t>();
+    const uint32_t* len = s->len.as<uint32_t>();
+    uint32_t* perm = s->perm.as<uint32_t>();
+    uint32_t* lcp = s->lcp.as<uint32_t>();
+    const uint32_t* hp = ctr + kCtrPass + 4 * b.par;
+    uint32_t* pn = ctr + kCtrPass + 4 * (b.par ^ 1);
+    Tile* tl = X.tiles.as<Tile>();
+    const uint32_t gt = grid_cap(caps.tile_cap, 1, kDevGrid);
+    const uint32_t gc = grid_cap(caps.chunk_cap, 1, 256);
+    fill_small(st, pn, 0, 16);
+    hipLaunchKernelGGL(k_fill_tiles<true>, dim3(grid_cap(caps.tile_cap, 256, kDevGrid)), dim3(256), 0, st, b.cur, 0u,
+                       0u, tl, hp);
+    hipLaunchKernelGGL((k_prefix_skip<kL, kU, true>), dim3(gt), dim3(kThreads), 0, st, tl, b.cur, perm, chars, off,
+                       len, b.kc, b.pc_ids, hp);
+    hipLaunchKernelGGL(k_gather<true>, dim3(gt), dim3(kThreads), 0, st, tl, b.cur, perm, chars, off, len, b.kc, hp);
+    hipLaunchKernelGGL(k_hist<true>, dim3(gt), dim3(kThreads), 0, st, tl, b.cur, b.kc, X.hist.as<uint32_t>(), hp);
+    hipLaunchKernelGGL(k_chunk_sum<true>, dim3(gc), dim3(kThreads), 0, st, b.cur, 0u, X.hist.as<uint32_t>(),
+                       X.csum.as<uint32_t>(), hp);
+    hipLaunchKernelGGL(k_scan_classify<true>, dim3(grid_cap(caps.seg_cap, kScGroups, 512)),
+                       dim3(kThreads * kScGroups), 0, st, b.cur, 0u, X.hist.as<uint32_t>(), X.csum.as<uint32_t>(),
+                       b.nxt, X.small.as<SmallSeg>(), caps.L, T.large_min, ctr, pn + kCtrLarge, lcp,
+                       b.kn == X.keyB.as<uint64_t>() ? 1u : 0u, hp);
+    hipLaunchKernelGGL(k_chunk_apply<true>, dim3(gc), dim3(kThreads), 0, st, b.cur, 0u, X.hist.as<uint32_t>(),
+                       X.csum.as<uint32_t>(), hp);
+    hipLaunchKernelGGL(k_scatter<true>, dim3(gt), dim3(kScT), 0, st, tl, b.cur, X.hist.as<uint32_t>(), b.kc,
+                       b.pc_ids, b.kn, b.pn_ids, perm, lcp, hp);
+    hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, b.nxt, pn);
+    b.swap_pass();
+}
+
+// Enqueues (1): initialisation and kGraphPasses passes.
+static void enqueue_dev_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint64_t ncap, uint32_t pcap) {
+    const hipStream_t st = s->stream;
+    SortScratch& X = s->ss;
+    const SortParams& T = s->sort_params;
+    uint64_t* dn = reinterpret_cast<uint64_t*>(ctr + 64);
+    uint32_t* dp = ctr + 66;
+    fill_small(st, ctr, 0, 64 * 4);
+    hipLaunchKernelGGL(k_sort_init<true>, dim3(grid_cap(ncap, 256, 2048)), dim3(256), 0, st, s->perm.as<uint32_t>(),
+                       s->lcp.as<uint32_t>(), s->s_len.as<uint32_t>(), (uint64_t)0, dn);
+    hipLaunchKernelGGL(k_sort_seed<true>, dim3(grid_cap(pcap, 256, 256)), dim3(256), 0, st,
+                       s->pe_begin_d.as<uint64_t>(), 0u, T.large_min, X.segA.as<Seg>(), X.small.as<SmallSeg>(),
+                       caps.L, ctr, dp);
+    hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, X.segA.as<Seg>(), ctr + kCtrPass);
+    PassBufs b = first_pass_bufs(X);
+    for (uint32_t k = 0; k < kGraphPasses; ++k) {
+        if (T.skip_lanes == 16)
+            T.skip_strings == 4 ? launch_dev_pass<16, 4>(s, caps, ctr, b)
+            : T.skip_strings == 2 ? launch_dev_pass<16, 2>(s, caps, ctr, b) : launch_dev_pass<16, 1>(s, caps, ctr, b);
+        else
+            T.skip_strings == 4 ? launch_dev_pass<8, 4>(s, caps, ctr, b)
+            : T.skip_strings == 2 ? launch_dev_pass<8, 2>(s, caps, ctr, b) : launch_dev_pass<8, 1>(s, caps, ctr, b);
+    }
+}
+
+// Enqueues (2): the base cases of every class and the sorted offset/length
+// arrays.
+static void enqueue_dev_base_cases(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint64_t ncap) {
+    const hipStream_t st = s->stream;
+    SortScratch& X = s->ss;
+    const SortParams& T = s->sort_params;
+    const SmallLists L = caps.L;
+    const uint8_t* chars = s->arena();
+    const uint64_t* off = s->off.as<uint64_t>();
+    const uint32_t* len = s->len.as<uint32_t>();
+    uint32_t* perm = s->perm.as<uint32_t>();
+    uint32_t* lcp = s->lcp.as<uint32_t>();
+    const uint64_t* dn = reinterpret_cast<const uint64_t*>(ctr + 64);
+    // grids: the larger class capacity of each kind (every class has at most
+    // ncap / 2^(c-1) segments)
+    const uint32_t cmax = std::min<uint32_t>(T.wave_class_max, kClasses - 1);
+    if (cmax >= 1) {
+        uint64_t waves = 0;
+        for (uint32_t c = 1; c <= cmax; ++c) waves = std::max<uint64_t>(waves, (L.cap[c] + (64u >> c) - 1) / (64u >> c));
+        hipLaunchKernelGGL(k_small_sort_wave_all, dim3(grid_cap(waves, 4, kDevGrid)), dim3(kThreads), 0, st,
+                           X.small.as<SmallSeg>(), L, cmax, perm, lcp, chars, off, len, X.keyA.as<uint64_t>(),
+                           X.keyB.as<uint64_t>(), s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(), ctr);
+    }
+    if (cmax + 1 < kClasses) {
+        uint64_t blocks = 0;
+        for (uint32_t c = cmax + 1; c < kClasses; ++c)
+            blocks = std::max<uint64_t>(blocks, (L.cap[c] + (kCap >> c) - 1) / (kCap >> c));
+        hipLaunchKernelGGL(k_small_sort_all, dim3(grid_cap(blocks, 1, kDevGrid)), dim3(kThreads), 0, st,
+                           X.small.as<SmallSeg>(), L, cmax + 1, perm, lcp, chars, off, len, s->s_off.as<uint64_t>(),
+                           s->s_len.as<uint32_t>(), ctr);
+    }
+    hipLaunchKernelGGL(k_sorted_meta<true>, dim3(grid_cap(ncap, 256, 2048)), dim3(256), 0, st, perm, off, len,
+                       (uint64_t)0, s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(), 1, dn);
+}
+
+// Captures the launches of `fill` on the set's stream into an executable graph.
+template <class F>
+static hipGraphExec_t capture(hipStream_t st, F&& fill) {
+    hipGraph_t g = nullptr;
+    DSS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        fill();
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        throw Error{DSS_EHIP, std::string("graph capture: ") + hipGetErrorString(e)};
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) throw Error{DSS_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ei)};
+    return x;
+}
+
+static bool use_device_driven(const dss_set_s* s) {
+    if (s->stream == nullptr || s->timer.enabled) return false;  // no capture on the null stream; timing is per launch
+    const uint32_t mode = s->sort_params.pass_mode;
+    return mode == 2 || (mode == 0 && s->n <= kGraphMaxStrings && s->p >= kGraphMinPes);
+}
+
+static void sort_lcp_device_driven(dss_set_s* s) {
+    const hipStream_t st = s->stream;
+    const uint64_t n = s->n;
+    const uint32_t p = s->p;
+    // the size class: grids and list capacities are those of the next powers
+    // of two (at least 4096 strings), so small sorts replay small grids
+    uint64_t ncap = 4096;
+    while (ncap < n) ncap <<= 1;
+    uint32_t pcap = 1;
+    while (pcap < p) pcap <<= 1;
+    // buffers grow to the reservation or the class, whichever is larger
+    reserve_sort(s, std::max<uint64_t>(ncap, s->sort_cap_n), std::max<uint32_t>(pcap, s->sort_cap_p));
+    const SortCaps caps = sort_caps(ncap, pcap, s->sort_params);
+    SortScratch& X = s->ss;
+    uint32_t* ctr = X.counters.as<uint32_t>();
+    const SortGraphKey key{ncap, pcap, s->sort_params.large_min, s->sort_params.wave_class_max,
+                           s->sort_params.skip_lanes, s->sort_params.skip_strings, s->arena(), s->off.p, s->len.p,
+                           s->pe_begin_d.p, s->perm.p, s->lcp.p, s->s_off.p, s->s_len.p, X.keyA.p, X.keyB.p,
+                           X.pbA.p, X.pbB.p, X.segA.p, X.segB.p, X.tiles.p, X.hist.p, X.csum.p, X.small.p,
+                           X.counters.p, st};
+    bool hit = false;
+    SortGraph& G = X.graphs.find(key, hit);
+    if (!hit) {
+        G.passes = capture(st, [&] { enqueue_dev_passes(s, caps, ctr, ncap, pcap); });
+        G.base = capture(st, [&] { enqueue_dev_base_cases(s, caps, ctr, ncap); });
+        G.key = key;
+    }
+    hipLaunchKernelGGL(k_sort_params, dim3(1), dim3(1), 0, st, reinterpret_cast<uint64_t*>(ctr + 64), ctr + 66, n,
+                       p);
+    DSS_HIP(hipGetLastError());
+    DSS_HIP(hipGraphLaunch(G.passes, st));
+    uint32_t h_ctr[64];
+    s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
+    PassBufs b = first_pass_bufs(X);
+    for (uint32_t k = 0; k < kGraphPasses; ++k) b.swap_pass();
+    if (h_ctr[kCtrPass + 4 * b.par + kCtrLarge]) host_passes(s, caps, ctr, h_ctr, b, false);
+    DSS_HIP(hipGraphLaunch(G.base, st));
+}
+
+void sort_lcp(dss_set_s* s) {
+    const uint64_t n = s->n;
+    const hipStream_t st = s->stream;
+    s->sorted = false;
+    if (n == 0) {
+        reserve_sort(s, n, s->p);
+        s->sorted = true;
+        return;
+    }
+    if (use_device_driven(s)) {
+        sort_lcp_device_driven(s);
+        s->sorted = true;
+        return;
+    }
+    const SortCaps caps = reserve_sort(s, n, s->p);
+    SortScratch& X = s->ss;
+    const SortParams& T = s->sort_params;
+    uint32_t* ctr = X.counters.as<uint32_t>();
+    fill_small(st, ctr, 0, 64 * 4);
+    {
+        KScope ks(s->timer, st, "sort_init", n * 8.0);
+        hipLaunchKernelGGL(k_sort_init<false>, dim3((n + 255) / 256), dim3(256), 0, st, s->perm.as<uint32_t>(),
+                           s->lcp.as<uint32_t>(), s->s_len.as<uint32_t>(), n, nullptr);
+    }
+    hipLaunchKernelGGL(k_sort_seed<false>, dim3((s->p + 255) / 256), dim3(256), 0, st,
+                       s->pe_begin_d.as<uint64_t>(), s->p, T.large_min, X.segA.as<Seg>(), X.small.as<SmallSeg>(),
+                       caps.L, ctr, nullptr);
+    DSS_HIP(hipGetLastError());
+    // every pass ends with one read-back of all counters: the next list's
+    // segment count and its tile/chunk counts (k_build_tiles of the next pass
+    // runs before the read-back), and the small-class counts
+    uint32_t h_ctr[64];
+    hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, X.segA.as<Seg>(), ctr + kCtrPass);
+    s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
+    const bool trace = getenv("DSS_SORT_TRACE") != nullptr;
+    PassBufs b = first_pass_bufs(X);
+    host_passes(s, caps, ctr, h_ctr, b, trace);
+    host_base_cases(s, caps, h_ctr, trace);
     s->sorted = true;
 }
 
@@ -1323,6 +1793,8 @@ int dss_set_reserve(dss_set s, uint64_t num_strings, uint32_t num_pes) {
         const uint64_t nn = std::max<uint64_t>(num_strings, 1);
         DSS_HIP(hipStreamSynchronize(s->stream));
         reserve_sort(s, num_strings, num_pes);
+        s->sort_cap_n = std::max<uint64_t>(s->sort_cap_n, num_strings);
+        s->sort_cap_p = std::max<uint32_t>(s->sort_cap_p, num_pes);
         s->pe_begin_d.reserve_keep((uint64_t)(num_pes + 1) * 8, s->stream);
         s->aux.reserve((uint64_t)num_pes * 8 + (num_strings / 65536 + num_pes) * 16 + 8);
         if (s->borrowed) {  // a batch set: its offset/length copies too
@@ -1349,11 +1821,13 @@ int dss_set_sort_params(dss_set s, const dss_sort_params_t* params) {
             DSS_REQUIRE(params->merge_fan_in == 0 || params->merge_fan_in == 2 || params->merge_fan_in == 4 ||
                             params->merge_fan_in == 8 || params->merge_fan_in == 16,
                         DSS_EINVAL, "merge_fan_in must be 0 (default), 2, 4, 8 or 16");
+            DSS_REQUIRE(params->pass_mode <= 2, DSS_EINVAL, "pass_mode must be 0 (auto), 1 or 2");
             T.large_min = params->large_min;
             T.wave_class_max = params->wave_class_max;
             T.skip_lanes = params->skip_lanes;
             if (params->skip_strings) T.skip_strings = params->skip_strings;
             if (params->merge_fan_in) T.merge_fan = params->merge_fan_in;
+            T.pass_mode = params->pass_mode;
         }
         s->sort_params = T;
     });
@@ -1367,6 +1841,7 @@ int dss_get_sort_params(dss_set s, dss_sort_params_t* params) {
         params->skip_lanes = s->sort_params.skip_lanes;
         params->skip_strings = s->sort_params.skip_strings;
         params->merge_fan_in = s->sort_params.merge_fan;
+        params->pass_mode = s->sort_params.pass_mode;
     });
 }
 
@@ -1375,28 +1850,17 @@ int dss_pe_dist_sizes(dss_set s, uint64_t* d_per_pe) {
         DSS_REQUIRE(s && d_per_pe, DSS_EINVAL, "NULL argument");
         DSS_REQUIRE(s->sorted, DSS_ESTATE, "dss_sort_lcp has not run");
         const uint32_t p = s->p;
-        std::vector<uint64_t> chunks;  // pairs (pe << 40 | begin, count)
-        for (uint32_t r = 0; r < p; ++r) {
-            const uint64_t b = s->pe_begin[r], e = s->pe_begin[r + 1];
-            if (e - b < 2) continue;
-            for (uint64_t c = b; c < e; c += kDistChunk) {
-                chunks.push_back(((uint64_t)r << 40) | c);
-                chunks.push_back(std::min(e, c + kDistChunk) - c);
-            }
-        }
+        const uint64_t n = s->n;
         DevBuf& buf = s->aux;
-        buf.reserve(p * 8 + chunks.size() * 8 + 8);
+        buf.reserve((uint64_t)std::max<uint32_t>(p, 1) * 8);
         unsigned long long* d = buf.as<unsigned long long>();
-        DSS_HIP(hipMemsetAsync(d, 0, p * 8, s->stream));
-        if (!chunks.empty()) {
-            uint64_t* dc = buf.as<uint64_t>() + p;
-            DSS_HIP(hipMemcpyAsync(dc, chunks.data(), chunks.size() * 8, hipMemcpyHostToDevice, s->stream));
-            hipLaunchKernelGGL(k_pe_dsum, dim3(chunks.size() / 2), dim3(256), 0, s->stream, dc,
-                               s->lcp.as<uint32_t>(), s->pe_begin_d.as<uint64_t>(), d);
+        fill_small(s->stream, d, 0, (size_t)p * 8);
+        if (n) {
+            hipLaunchKernelGGL(k_pe_dsum, dim3((uint32_t)((n + kDistChunk - 1) / kDistChunk)), dim3(256), 0,
+                               s->stream, s->lcp.as<uint32_t>(), s->pe_begin_d.as<uint64_t>(), p, n, d);
             DSS_HIP(hipGetLastError());
         }
-        DSS_HIP(hipMemcpyAsync(d_per_pe, d, p * 8, hipMemcpyDeviceToHost, s->stream));
-        DSS_HIP(hipStreamSynchronize(s->stream));
+        s->pp.mbox.fetch(s->stream, d_per_pe, d, (size_t)p * 8);
     });
 }
 
@@ -1445,9 +1909,9 @@ extern "C" int dss_set_local_order(dss_set s, const uint32_t* perm, const uint32
                                 "perm must permute every PE's own strings");
             DSS_HIP(hipMemcpyAsync(s->perm.p, perm, n * 4, hipMemcpyHostToDevice, s->stream));
             DSS_HIP(hipMemcpyAsync(s->lcp.p, lcp, n * 4, hipMemcpyHostToDevice, s->stream));
-            hipLaunchKernelGGL(k_sorted_meta, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->perm.as<uint32_t>(),
-                               s->off.as<uint64_t>(), s->len.as<uint32_t>(), n, s->s_off.as<uint64_t>(),
-                               s->s_len.as<uint32_t>(), 0);
+            hipLaunchKernelGGL(k_sorted_meta<false>, dim3((n + 255) / 256), dim3(256), 0, s->stream,
+                               s->perm.as<uint32_t>(), s->off.as<uint64_t>(), s->len.as<uint32_t>(), n,
+                               s->s_off.as<uint64_t>(), s->s_len.as<uint32_t>(), 0, nullptr);
             DSS_HIP(hipGetLastError());
             DSS_HIP(hipStreamSynchronize(s->stream));
         }

@@ -4,9 +4,10 @@ A regime is (log2 of the strings per PE, log2 of the mean string length,
 tenths of the mean LCP over the mean length, number of PEs): inputs in one
 regime share their MSD pass structure, base-case sizes and merge shape.
 `autotune` times the local sort under each candidate setting (base-case
-threshold, one-wave base-case classes, prefix-skip lanes and strings), then,
-with more than one PE, the LCP merge of the exchanged runs under each merge
-fan-in (2, 4, 8, 16 runs per pass); every setting yields the same output, so
+threshold, one-wave base-case classes, prefix-skip lanes and strings), then
+the best of them with host-driven and with device-driven passes, then, with
+more than one PE, the LCP merge of the exchanged runs under each merge fan-in
+(2, 4, 8, 16 runs per pass); every setting yields the same output, so
 any set of the regime serves as the sample.  The fastest setting is recorded
 in a cache that can be saved as JSON; `apply` sets the cached setting for a
 new set of a known regime.
@@ -18,7 +19,8 @@ import math
 import os
 import time
 
-DEFAULT = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 2}
+DEFAULT = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 2,
+           "pass_mode": 0}
 
 CANDIDATES = [
     {"large_min": lm, "wave_class_max": wc, "skip_lanes": sk, "skip_strings": ss}
@@ -26,6 +28,7 @@ CANDIDATES = [
     for sk, ss in ((8, 1), (8, 2), (8, 4), (16, 1), (16, 2))
 ]
 MERGE_FANS = (2, 4, 8, 16)
+PASS_MODES = (1, 2)  # host-driven passes; device-driven passes replayed from hipGraphs
 
 
 def regime(info: dict, avg_lcp: int | None = None) -> str:
@@ -69,12 +72,14 @@ class Tuner:
             best = min(best, time.perf_counter() - t0)
         return best
 
-    def autotune(self, ds, candidates=None, repeats: int = 2, fans=MERGE_FANS) -> dict:
-        """Times ds.sort_lcp() under every candidate, then (more than one PE)
-        ds.merge_lcp() of the exchanged runs under every merge fan-in; keeps
-        the fastest for the set's regime.  Returns {"regime", "best",
-        "times_ms", "merge_times_ms"}."""
-        candidates = [dict(c, merge_fan_in=DEFAULT["merge_fan_in"]) for c in (candidates or CANDIDATES)]
+    def autotune(self, ds, candidates=None, repeats: int = 2, fans=MERGE_FANS, modes=PASS_MODES) -> dict:
+        """Times ds.sort_lcp() under every candidate, the best candidate under
+        every pass mode, then (more than one PE) ds.merge_lcp() of the
+        exchanged runs under every merge fan-in; keeps the fastest for the
+        set's regime.  Returns {"regime", "best", "times_ms", "pass_times_ms",
+        "merge_times_ms"}."""
+        candidates = [dict(c, merge_fan_in=DEFAULT["merge_fan_in"], pass_mode=DEFAULT["pass_mode"])
+                      for c in (candidates or CANDIDATES)]
         ds.set_sort_params(DEFAULT)
         ds.sort_lcp()  # warm up buffers; also gives the LCPs for the regime key
         ds.sync()
@@ -85,6 +90,14 @@ class Tuner:
             times.append((self._time(ds, ds.sort_lcp, repeats), cand))
         times.sort(key=lambda t: t[0])
         best = dict(times[0][1])
+        pass_times = []
+        for mode in modes:
+            ds.set_sort_params(dict(best, pass_mode=mode))
+            ds.sort_lcp()  # first sort in a mode: its capture
+            pass_times.append((self._time(ds, ds.sort_lcp, repeats), mode))
+        if pass_times:
+            pass_times.sort()
+            best["pass_mode"] = pass_times[0][1]
         merge_times = []
         if ds.p > 1 and fans:
             ds.set_sort_params(best)
@@ -101,6 +114,7 @@ class Tuner:
         ds.set_sort_params(best)
         return {"regime": key, "best": dict(best),
                 "times_ms": [(round(t * 1e3, 3), c) for t, c in times],
+                "pass_times_ms": [(round(t * 1e3, 3), m) for t, m in pass_times],
                 "merge_times_ms": [(round(t * 1e3, 3), f) for t, f in merge_times]}
 
     def apply(self, ds) -> bool:

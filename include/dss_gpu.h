@@ -151,14 +151,23 @@ int dss_get_sorted(dss_set s, uint32_t* perm, uint32_t* lcp);
  * merge_fan_in (2, 4, 8 or 16, default 2; 0 selects the default) runs are
  * merged per pass of dss_merge_lcp (the K of merge<K>, merging.hpp:23-63:
  * more runs than that take several passes; 2 is the pairwise merge tree, one
- * pass per level, larger values merge K runs per pass in LDS tiles).  Every setting produces the
- * same output.  NULL restores the defaults. */
+ * pass per level, larger values merge K runs per pass in LDS tiles);
+ * pass_mode selects how the local sort's MSD passes are driven: 1 from the
+ * host (each pass sized from counters read back after the previous one), 2
+ * from the device (persistent kernels that read their work counts from
+ * device counters, the launch sequence captured once per set in hipGraphs
+ * and replayed: one read-back per sort), 0 (default) the device-driven form
+ * for sets of at most 2^22 strings over at least 16 PEs (batches of jobs),
+ * the host-driven one otherwise (the host-driven form is also used while
+ * kernel timing is on, and on the null stream).  Every setting produces the same output.  NULL restores the
+ * defaults. */
 typedef struct {
     uint32_t large_min;
     uint32_t wave_class_max;
     uint32_t skip_lanes;
     uint32_t skip_strings;
     uint32_t merge_fan_in;
+    uint32_t pass_mode;
 } dss_sort_params_t;
 
 int dss_set_sort_params(dss_set s, const dss_sort_params_t* params);

@@ -119,14 +119,16 @@ def test_skewed_generator_shapes(n, ll, sl, ldn, sdn, lp, dp):
 
 
 PARAM_SETS = [
-    {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8, "skip_strings": 1, "merge_fan_in": 2},
-    {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 2, "merge_fan_in": 4},
-    {"large_min": 256, "wave_class_max": 5, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 8},
-    {"large_min": 1024, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 1, "merge_fan_in": 16},
-    {"large_min": 128, "wave_class_max": 4, "skip_lanes": 8, "skip_strings": 2, "merge_fan_in": 2},
-    {"large_min": 32, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 4, "merge_fan_in": 16},
+    {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8, "skip_strings": 1, "merge_fan_in": 2, "pass_mode": 1},
+    {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 2, "merge_fan_in": 4, "pass_mode": 2},
+    {"large_min": 256, "wave_class_max": 5, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 8, "pass_mode": 0},
+    {"large_min": 1024, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 1, "merge_fan_in": 16, "pass_mode": 2},
+    {"large_min": 128, "wave_class_max": 4, "skip_lanes": 8, "skip_strings": 2, "merge_fan_in": 2, "pass_mode": 1},
+    {"large_min": 32, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 4, "merge_fan_in": 16, "pass_mode": 2},
+    {"large_min": 16, "wave_class_max": 3, "skip_lanes": 8, "skip_strings": 1, "merge_fan_in": 2, "pass_mode": 2},
 ]
-DEFAULT_PARAMS = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 2}
+DEFAULT_PARAMS = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 2,
+                  "pass_mode": 0}
 
 
 @pytest.mark.parametrize("name", ["dton", "skewed", "suffix"])
@@ -160,8 +162,11 @@ def test_autotuner_picks_and_applies_a_setting(tmp_path):
     tuner = Tuner(str(tmp_path / "t.json"))
     with dss.DeviceStringSet.dton(200000, 100, 0.5, 34, 4) as ds:
         out = tuner.autotune(ds, repeats=1)
-        sort_best = {k: v for k, v in out["best"].items() if k != "merge_fan_in"}
-        assert sort_best in [{k: v for k, v in c.items() if k != "merge_fan_in"} for _, c in out["times_ms"]]
+        sort_best = {k: v for k, v in out["best"].items() if k not in ("merge_fan_in", "pass_mode")}
+        assert sort_best in [{k: v for k, v in c.items() if k not in ("merge_fan_in", "pass_mode")}
+                             for _, c in out["times_ms"]]
+        assert out["best"]["pass_mode"] == out["pass_times_ms"][0][1]
+        assert sorted(m for _, m in out["pass_times_ms"]) == [1, 2]
         assert out["best"]["merge_fan_in"] == out["merge_times_ms"][0][1]
         assert sorted(f for _, f in out["merge_times_ms"]) == [2, 4, 8, 16]
         assert ds.sort_params() == out["best"]
@@ -179,3 +184,91 @@ def test_autotuner_picks_and_applies_a_setting(tmp_path):
             ds.set_sort_params({"skip_strings": 3})
         with pytest.raises(dss.DssError):
             ds.set_sort_params({"merge_fan_in": 3})
+        with pytest.raises(dss.DssError):
+            ds.set_sort_params({"pass_mode": 3})
+
+
+def _two_letter_strings(n, length, seed):
+    """Strings over {a, b}: every MSD pass halves the segments, so sorting
+    2^16 of them takes more passes than the device-driven form captures."""
+    rng = np.random.default_rng(seed)
+    return [bytes(rng.integers(97, 99, int(l), dtype=np.uint8)) for l in rng.integers(0, length + 1, n)]
+
+
+DEVICE_DRIVEN_INPUTS = {
+    "dton": lambda: O.gen_dton(30000, 120, 0.5, 41),
+    "skewed": lambda: O.gen_skewed(O.skew_params(30000, long_len=300, short_len=30), 42),
+    "suffix": lambda: O.gen_suffixes(30000, 43, repeats=20, rep_min=100, rep_max=1500, cap=400),
+    "two_letters_deep": lambda: arena(_two_letter_strings(1 << 16, 40, 44)),
+    "tiny": lambda: arena([b"", b"b", b"a", b"", b"ab", b"a"]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DEVICE_DRIVEN_INPUTS))
+@pytest.mark.parametrize("p", [1, 5, 64])
+def test_device_driven_passes_match_host_driven(name, p):
+    """The device-driven local sort (persistent kernels, two hipGraphs
+    replayed, one read-back) gives the host-driven form's permutation, LCPs
+    and sorted offsets/lengths and the oracle's order; three replays in a row,
+    with eager work and synchronisation between them, give the same result
+    each time (p = 64 also leaves PEs empty or with one string)."""
+    dss = _dss()
+    chars = DEVICE_DRIVEN_INPUTS[name]()
+    ss = O.StringSet(chars, p)
+    operm, olcp = O.local_sort(ss)
+    def exchanged(ds):
+        # the exchange reads the sorted offsets/lengths the sort wrote
+        ds.partition("IndexedNumStrings", max_len=2 * 100 * (ds.avg_lcp() + 5))
+        rb, bb, nbytes = ds.exchange("whole")
+        return ds.get_exchange_bytes(nbytes).tobytes()
+
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        ds.set_sort_params({"pass_mode": 1})
+        ds.sort_lcp()
+        hperm, hlcp = ds.get_sorted()
+        hbytes = exchanged(ds)
+        ds.set_sort_params({"pass_mode": 2})
+        for rep in range(3):
+            ds.sort_lcp()
+            perm, lcp = ds.get_sorted()
+            assert (perm == hperm).all() and (lcp == hlcp).all(), rep
+            assert ds.avg_lcp() == O.avg_lcp(olcp)
+            assert exchanged(ds) == hbytes, rep
+            ds.sync()
+    assert (hlcp == olcp).all()
+    assert (canonical_ties(hperm, hlcp, ss.len, ss.pe_begin) == canonical_ties(operm, olcp, ss.len, ss.pe_begin)).all()
+
+
+def test_device_driven_replays_across_rebatched_jobs():
+    """One capture of the device-driven sort serves batches of other sizes:
+    a batch set reserved once is re-pointed at different jobs (other string
+    and PE counts) and each replay equals the host-driven sort of the same
+    jobs and the oracle."""
+    dss = _dss()
+    from dss_amd import JobPool
+    rng = np.random.default_rng(45)
+    arenas = [O.gen_dton(int(rng.integers(500, 20000)), int(rng.integers(20, 200)), float(rng.uniform(0.2, 0.8)),
+                         int(100 + k)) for k in range(12)]
+    groups = [[0, 1, 2], [3], [4, 5, 6, 7, 8], [9, 10, 11], [2, 7]]
+    with JobPool(1 << 28, 1 << 22) as pool:
+        ids = [pool.add(a) for a in arenas]
+        b = pool.batch([ids[k] for k in groups[0]])
+        b.set.reserve(sum(int((a == 0).sum()) for a in arenas), 16)
+        for g in groups:
+            b.rebatch([ids[k] for k in g])
+            b.set.set_sort_params({"pass_mode": 1})
+            b.sort()
+            host = [b.result(k) for k in range(len(g))]
+            hd = b.dist_sizes()
+            b.set.set_sort_params({"pass_mode": 2})
+            b.sort()
+            for k, jk in enumerate(g):
+                perm, lcp = b.result(k)
+                assert (perm == host[k][0]).all() and (lcp == host[k][1]).all(), (g, k)
+                oss = O.StringSet(arenas[jk], 1)
+                operm, olcp = O.local_sort(oss)
+                assert (lcp == olcp).all()
+                assert (canonical_ties(perm, lcp, oss.len, oss.pe_begin) ==
+                        canonical_ties(operm, olcp, oss.len, oss.pe_begin)).all()
+            assert (b.dist_sizes() == hd).all()
+        b.close()
