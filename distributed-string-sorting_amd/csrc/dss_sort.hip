@@ -824,15 +824,13 @@ __device__ inline uint32_t block_excl_max(uint32_t v, uint32_t* tmp) {
     return max(pre, ex);
 }
 
-// LCP of two strings from depth d, by 8-char words (one lane).
+// LCP of two strings from depth d (one lane), 32 characters per dependent
+// step (five aligned loads per string and step).
 __device__ inline uint32_t lcp_walk(const uint8_t* __restrict__ chars, uint64_t ao, uint32_t al,
                                     uint64_t bo, uint32_t bl, uint32_t d) {
-    for (uint32_t pos = d;; pos += 8) {
-        const uint64_t a = load_key8(chars, ao, al, pos);
-        const uint64_t b = load_key8(chars, bo, bl, pos);
-        if (a != b) return pos - d + key_lcp(a, b);
-        if ((a & 0xFF) == 0) return pos - d + key_strlen(a);
-    }
+    uint32_t l = 0;
+    (void)view_cmp32(chars, ao, al, bo, bl, &l, d);
+    return l - d;
 }
 
 constexpr uint32_t kCap = kSmallMax;
