@@ -348,7 +348,7 @@ __device__ __forceinline__ void prefix_skip_tile(uint32_t tb, Tile* __restrict__
         t_off[k] = off[id];
         t_len[k] = len[id];
         t_res[k] = kNone;
-        pb[T.begin + k] = id;
+        if (pb) pb[T.begin + k] = id;  // no copy in the first pass: the order is the identity
     }
     __syncthreads();
     constexpr uint32_t kSkipGroups = kThreads / kSkipLanes;
@@ -722,7 +722,7 @@ __device__ __forceinline__ void scatter_tile(uint32_t tb, const Tile* __restrict
         dg[r] = 0;
         if (e < T.end) {
             kv[r] = key_in[e];
-            idv[r] = pb_in[e];
+            idv[r] = pb_in ? pb_in[e] : e;  // first pass: ids are positions
             dg[r] = (uint32_t)(kv[r] >> sh) & 0xFF;
             atomicAdd(&run[dg[r]], 1u);
         }
@@ -1357,8 +1357,10 @@ static const char* const kClassNames[kClasses] = {
 // Host-driven MSD passes from the state in `b` until the segment list is
 // empty: every pass sizes its launches from the counters read back after the
 // previous one (h_ctr holds them on entry and on return).
+// first: the passes start from the seed, where every id equals its position
+// (the first pass neither copies nor reads the id buffer).
 static void host_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint32_t* h_ctr, PassBufs& b,
-                        bool trace) {
+                        bool trace, bool first) {
     const hipStream_t st = s->stream;
     SortScratch& X = s->ss;
     const SortParams& T = s->sort_params;
@@ -1403,8 +1405,8 @@ static void host_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint3
             else
                 kern = T.skip_strings == 4 ? &k_prefix_skip<8, 4, false>
                        : T.skip_strings == 2 ? &k_prefix_skip<8, 2, false> : &k_prefix_skip<8, 1, false>;
-            hipLaunchKernelGGL(kern, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc, b.pc_ids,
-                               nullptr);
+            hipLaunchKernelGGL(kern, dim3(nt), dim3(kThreads), 0, st, tl, cur, perm, chars, off, len, kc,
+                               first ? nullptr : b.pc_ids, nullptr);
         }
         {
             KScope ks(s->timer, st, "gather", 0.0);  // bytes amended after the pass
@@ -1432,7 +1434,7 @@ static void host_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint3
         {
             KScope ks(s->timer, st, "scatter", (double)cnt_large * 28.0);
             hipLaunchKernelGGL(k_scatter<false>, dim3(nt), dim3(kScT), 0, st, tl, cur, X.hist.as<uint32_t>(), kc,
-                               b.pc_ids, kn, b.pn_ids, perm, lcp, nullptr);
+                               first ? nullptr : b.pc_ids, kn, b.pn_ids, perm, lcp, nullptr);
         }
         hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, nxt, pn);
         DSS_HIP(hipGetLastError());
@@ -1465,6 +1467,7 @@ static void host_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint3
             fprintf(stderr, "\n");
         }
         b.swap_pass();
+        first = false;
     }
 }
 
@@ -1565,7 +1568,7 @@ static uint32_t grid_cap(uint64_t items, uint32_t per_block, uint32_t cap) {
 }
 
 template <uint32_t kL, uint32_t kU>
-static void launch_dev_pass(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, PassBufs& b) {
+static void launch_dev_pass(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, PassBufs& b, bool first) {
     const hipStream_t st = s->stream;
     SortScratch& X = s->ss;
     const SortParams& T = s->sort_params;
@@ -1583,7 +1586,7 @@ static void launch_dev_pass(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, P
     hipLaunchKernelGGL(k_fill_tiles<true>, dim3(grid_cap(caps.tile_cap, 256, kDevGrid)), dim3(256), 0, st, b.cur, 0u,
                        0u, tl, hp);
     hipLaunchKernelGGL((k_prefix_skip<kL, kU, true>), dim3(gt), dim3(kThreads), 0, st, tl, b.cur, perm, chars, off,
-                       len, b.kc, b.pc_ids, hp);
+                       len, b.kc, first ? nullptr : b.pc_ids, hp);
     hipLaunchKernelGGL(k_gather<true>, dim3(gt), dim3(kThreads), 0, st, tl, b.cur, perm, chars, off, len, b.kc, hp);
     hipLaunchKernelGGL(k_hist<true>, dim3(gt), dim3(kThreads), 0, st, tl, b.cur, b.kc, X.hist.as<uint32_t>(), hp);
     hipLaunchKernelGGL(k_chunk_sum<true>, dim3(gc), dim3(kThreads), 0, st, b.cur, 0u, X.hist.as<uint32_t>(),
@@ -1595,7 +1598,7 @@ static void launch_dev_pass(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, P
     hipLaunchKernelGGL(k_chunk_apply<true>, dim3(gc), dim3(kThreads), 0, st, b.cur, 0u, X.hist.as<uint32_t>(),
                        X.csum.as<uint32_t>(), hp);
     hipLaunchKernelGGL(k_scatter<true>, dim3(gt), dim3(kScT), 0, st, tl, b.cur, X.hist.as<uint32_t>(), b.kc,
-                       b.pc_ids, b.kn, b.pn_ids, perm, lcp, hp);
+                       first ? nullptr : b.pc_ids, b.kn, b.pn_ids, perm, lcp, hp);
     hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, b.nxt, pn);
     b.swap_pass();
 }
@@ -1616,12 +1619,15 @@ static void enqueue_dev_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr
     hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, X.segA.as<Seg>(), ctr + kCtrPass);
     PassBufs b = first_pass_bufs(X);
     for (uint32_t k = 0; k < kGraphPasses; ++k) {
+        const bool first = k == 0;
         if (T.skip_lanes == 16)
-            T.skip_strings == 4 ? launch_dev_pass<16, 4>(s, caps, ctr, b)
-            : T.skip_strings == 2 ? launch_dev_pass<16, 2>(s, caps, ctr, b) : launch_dev_pass<16, 1>(s, caps, ctr, b);
+            T.skip_strings == 4 ? launch_dev_pass<16, 4>(s, caps, ctr, b, first)
+            : T.skip_strings == 2 ? launch_dev_pass<16, 2>(s, caps, ctr, b, first)
+                                  : launch_dev_pass<16, 1>(s, caps, ctr, b, first);
         else
-            T.skip_strings == 4 ? launch_dev_pass<8, 4>(s, caps, ctr, b)
-            : T.skip_strings == 2 ? launch_dev_pass<8, 2>(s, caps, ctr, b) : launch_dev_pass<8, 1>(s, caps, ctr, b);
+            T.skip_strings == 4 ? launch_dev_pass<8, 4>(s, caps, ctr, b, first)
+            : T.skip_strings == 2 ? launch_dev_pass<8, 2>(s, caps, ctr, b, first)
+                                  : launch_dev_pass<8, 1>(s, caps, ctr, b, first);
     }
 }
 
@@ -1725,7 +1731,7 @@ static void sort_lcp_device_driven(dss_set_s* s) {
     s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
     PassBufs b = first_pass_bufs(X);
     for (uint32_t k = 0; k < kGraphPasses; ++k) b.swap_pass();
-    if (h_ctr[kCtrPass + 4 * b.par + kCtrLarge]) host_passes(s, caps, ctr, h_ctr, b, false);
+    if (h_ctr[kCtrPass + 4 * b.par + kCtrLarge]) host_passes(s, caps, ctr, h_ctr, b, false, false);
     DSS_HIP(hipGraphLaunch(G.base, st));
 }
 
@@ -1765,7 +1771,7 @@ void sort_lcp(dss_set_s* s) {
     s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
     const bool trace = getenv("DSS_SORT_TRACE") != nullptr;
     PassBufs b = first_pass_bufs(X);
-    host_passes(s, caps, ctr, h_ctr, b, trace);
+    host_passes(s, caps, ctr, h_ctr, b, trace, true);
     host_base_cases(s, caps, h_ctr, trace);
     s->sorted = true;
 }
