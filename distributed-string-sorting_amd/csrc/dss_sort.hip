@@ -343,18 +343,20 @@ __device__ __forceinline__ void prefix_skip_tile(uint32_t tb, Tile* __restrict__
     if (S.kdepth != kNone) return;
     const uint32_t cnt = T.end - T.begin;
     if (threadIdx.x == 0) bmin = kNone;
+    // the first pass (pb == nullptr) starts from the identity order: ids are
+    // positions, and neither the order is read nor the id buffer written
     for (uint32_t k = threadIdx.x; k < cnt; k += kThreads) {
-        const uint32_t id = perm[T.begin + k];
+        const uint32_t id = pb ? perm[T.begin + k] : T.begin + k;
         t_off[k] = off[id];
         t_len[k] = len[id];
         t_res[k] = kNone;
-        if (pb) pb[T.begin + k] = id;  // no copy in the first pass: the order is the identity
+        if (pb) pb[T.begin + k] = id;
     }
     __syncthreads();
     constexpr uint32_t kSkipGroups = kThreads / kSkipLanes;
     constexpr uint32_t kPerRound = kSkipGroups * kU;
     const uint32_t g = threadIdx.x / kSkipLanes, l = threadIdx.x % kSkipLanes;
-    const uint32_t ref = perm[S.begin];
+    const uint32_t ref = pb ? perm[S.begin] : S.begin;
     const uint64_t roff = off[ref];
     const uint32_t rlen = len[ref];
     const uint32_t d0 = S.depth;
