@@ -912,8 +912,10 @@ void exchange(dss_set_s* s, int mode) {
     if (n) {
         KScope ks(s->timer, st, "exchange_copy", 2.0 * total + n * 40.0);
         // a lane per string for copies of at most 32 characters on average,
-        // else 8 lanes (short prefix-doubling prefixes) or 16 (whole strings)
-        const uint32_t lg = total <= 33 * n ? 0 : mode == 2 ? 3 : 4;
+        // 4 lanes up to 256 (distinguishing prefixes, LCP-compressed strings:
+        // more strings in flight per wave beat more lanes per string), 16
+        // lanes above (whole strings)
+        const uint32_t lg = total <= 33 * n ? 0 : total <= 257 * n ? 2 : 4;
         const uint64_t threads = n << lg;
         hipLaunchKernelGGL(k_exch_copy, dim3(stride_grid(threads, 256)), dim3(256), 0, st, s->arena(),
                            P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
