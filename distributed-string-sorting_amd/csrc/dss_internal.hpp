@@ -341,6 +341,7 @@ struct SortParams {
     uint32_t skip_strings = 4;     // strings per lane group and round in the prefix skip
     uint32_t merge_fan = 2;        // runs merged per pass (2: the pairwise tree; 4, 8, 16: K-way tiles)
     uint32_t pass_mode = 0;        // local sort: 0 auto, 1 host-driven passes, 2 device-driven (graph)
+    uint32_t merge_tile = 1024;    // outputs per tile of the pairwise merge (512, 1024, 2048)
 };
 
 struct dss_set_s {

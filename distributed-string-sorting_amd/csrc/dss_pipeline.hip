@@ -1348,9 +1348,8 @@ __global__ void k_kmerge_fix(const KGroup* __restrict__ groups, uint32_t ngroups
 // the faster form where the merge is bound by the node merges' dependent LDS
 // latency rather than by HBM traffic (the D/N inputs); the tuner picks the
 // fan-in per input regime.
-constexpr uint32_t kMT = 256;               // threads per merge block
-constexpr uint32_t kMW = 4;                 // outputs per thread
-constexpr uint32_t kTileOut = kMT * kMW;    // outputs per tile
+constexpr uint32_t kMT = 256;  // threads per merge block; kMW outputs per thread (a template
+                               // parameter: 2, 4 or 8, i.e. 512, 1024 or 2048 outputs per tile)
 
 struct MergePair {
     uint64_t out;      // first receive position of the pair (A then B)
@@ -1374,6 +1373,7 @@ __device__ inline uint32_t pair_of_tile(const MergePair* __restrict__ pairs, uin
 // first round, whose order is the receive order.  Four lanes per tile search
 // 4-ary: lanes 0-2 each test one quartile point of the interval per step, so
 // a split takes about log4 of the run length dependent steps.
+template <uint32_t kMW>
 __global__ void k_merge_split(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t ntiles,
                               const uint32_t* __restrict__ ord, const uint8_t* __restrict__ chars,
                               const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
@@ -1385,6 +1385,7 @@ __global__ void k_merge_split(const MergePair* __restrict__ pairs, uint32_t npai
     const MergePair Pr = pairs[pair_of_tile(pairs, npairs, t)];
     const ViewCtx V{chars, voff, vlen, keyc2, rcp[Pr.recv]};
     const uint64_t A = Pr.out, B = Pr.out + Pr.na;
+    constexpr uint32_t kTileOut = kMT * kMW;
     const uint64_t d = (t - Pr.tile0) * kTileOut;
     // the answer is the first mid in [lo, hi] with A[mid] > B[d - mid - 1]
     // (hi if none); the test is monotone in mid, so the probes that hold form
@@ -1519,6 +1520,7 @@ __device__ inline int wave_tile_cmp(const TileView& T, bool act, uint32_t x, uin
 // ord == nullptr: the first round (receive order); saved != nullptr: the last
 // round, which also writes every output's saved LCP (the LCP it was received
 // with, rlcp).
+template <uint32_t kMW>
 __global__ void __launch_bounds__(kMT)
 k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_t* __restrict__ split,
              const uint32_t* __restrict__ ord, const uint32_t* __restrict__ lcpa, uint32_t* __restrict__ ord_out,
@@ -1526,6 +1528,7 @@ k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_
              const uint32_t* __restrict__ vlen, const uint64_t* __restrict__ keyc,
              const uint64_t* __restrict__ keyc2, const uint32_t* __restrict__ rcp, const uint32_t* __restrict__ rlcp,
              uint32_t* __restrict__ saved) {
+    constexpr uint32_t kTileOut = kMT * kMW;
     constexpr uint32_t kSlots = kTileOut + 2;
     __shared__ uint32_t sO[kSlots], sL[kSlots];
     __shared__ uint64_t sK1[kSlots], sK2[kSlots];
@@ -1744,6 +1747,9 @@ void merge(dss_set_s* s) {
         // fan-in 2: all rounds' pair lists are built first and uploaded
         // once, so the rounds run back to back without host synchronisation
         auto& runs = lists;
+        // outputs per tile: 512, 1024 (default) or 2048 (kMW 2, 4, 8)
+        const uint32_t mw = s->sort_params.merge_tile / kMT;
+        const uint64_t tile_out = (uint64_t)kMT * mw;
         std::vector<MergePair> all;
         std::vector<size_t> rbeg;          // first pair of each round
         std::vector<uint64_t> rtiles, tbeg;  // tiles of each round, first split slot
@@ -1766,7 +1772,7 @@ void merge(dss_set_s* s) {
                     mp.recv = q;
                     mp.pad = 0;
                     mp.tile0 = tiles;
-                    tiles += (mp.na + mp.nb + kTileOut - 1) / kTileOut;
+                    tiles += (mp.na + mp.nb + tile_out - 1) / tile_out;
                     all.push_back(mp);
                     next.push_back({mp.out, mp.na + mp.nb});
                 }
@@ -1793,13 +1799,20 @@ void merge(dss_set_s* s) {
                 uint64_t* split = splits + tbeg[r];
                 const bool last = r + 2 == rbeg.size();
                 KScope ks(s->timer, st, "merge", n * 24.0);
-                hipLaunchKernelGGL(k_merge_split, dim3((tiles * 4 + 255) / 256), dim3(256), 0, st, rp, np, tiles, in_ord,
-                                   s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
-                                   P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp, split);
-                hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kMT), 0, st, rp, np, split, in_ord, in_lc, ord, lc,
-                                   s->arena(), P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
-                                   P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(), cp, P.recv_lcp.as<uint32_t>(),
-                                   last ? P.m_saved.as<uint32_t>() : nullptr);
+                auto launch = [&](auto split_kern, auto tile_kern) {
+                    hipLaunchKernelGGL(split_kern, dim3((tiles * 4 + 255) / 256), dim3(256), 0, st, rp, np, tiles,
+                                       in_ord, s->arena(), P.recv_view_off.as<uint64_t>(),
+                                       P.recv_view_len.as<uint32_t>(), P.m_keyc.as<uint64_t>(),
+                                       P.m_keyc2.as<uint64_t>(), cp, split);
+                    hipLaunchKernelGGL(tile_kern, dim3(tiles), dim3(kMT), 0, st, rp, np, split, in_ord, in_lc, ord,
+                                       lc, s->arena(), P.recv_view_off.as<uint64_t>(),
+                                       P.recv_view_len.as<uint32_t>(), P.m_keyc.as<uint64_t>(),
+                                       P.m_keyc2.as<uint64_t>(), cp, P.recv_lcp.as<uint32_t>(),
+                                       last ? P.m_saved.as<uint32_t>() : nullptr);
+                };
+                if (mw == 2) launch(k_merge_split<2>, k_merge_tile<2>);
+                else if (mw == 8) launch(k_merge_split<8>, k_merge_tile<8>);
+                else launch(k_merge_split<4>, k_merge_tile<4>);
                 in_ord = ord;
                 in_lc = lc;
                 std::swap(ord, ord2);

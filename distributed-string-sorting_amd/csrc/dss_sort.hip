@@ -1828,12 +1828,16 @@ int dss_set_sort_params(dss_set s, const dss_sort_params_t* params) {
                             params->merge_fan_in == 8 || params->merge_fan_in == 16,
                         DSS_EINVAL, "merge_fan_in must be 0 (default), 2, 4, 8 or 16");
             DSS_REQUIRE(params->pass_mode <= 2, DSS_EINVAL, "pass_mode must be 0 (auto), 1 or 2");
+            DSS_REQUIRE(params->merge_tile == 0 || params->merge_tile == 512 || params->merge_tile == 1024 ||
+                            params->merge_tile == 2048,
+                        DSS_EINVAL, "merge_tile must be 0 (default), 512, 1024 or 2048");
             T.large_min = params->large_min;
             T.wave_class_max = params->wave_class_max;
             T.skip_lanes = params->skip_lanes;
             if (params->skip_strings) T.skip_strings = params->skip_strings;
             if (params->merge_fan_in) T.merge_fan = params->merge_fan_in;
             T.pass_mode = params->pass_mode;
+            if (params->merge_tile) T.merge_tile = params->merge_tile;
         }
         s->sort_params = T;
     });
@@ -1848,6 +1852,7 @@ int dss_get_sort_params(dss_set s, dss_sort_params_t* params) {
         params->skip_strings = s->sort_params.skip_strings;
         params->merge_fan_in = s->sort_params.merge_fan;
         params->pass_mode = s->sort_params.pass_mode;
+        params->merge_tile = s->sort_params.merge_tile;
     });
 }
 

@@ -7,7 +7,8 @@ regime share their MSD pass structure, base-case sizes and merge shape.
 threshold, one-wave base-case classes, prefix-skip lanes and strings), then
 the best of them with host-driven and with device-driven passes, then, with
 more than one PE, the LCP merge of the exchanged runs under each merge fan-in
-(2, 4, 8, 16 runs per pass); every setting yields the same output, so
+(2, 4, 8, 16 runs per pass) and, with fan-in 2, each merge tile size (512,
+1024, 2048 outputs); every setting yields the same output, so
 any set of the regime serves as the sample.  The fastest setting is recorded
 in a cache that can be saved as JSON; `apply` sets the cached setting for a
 new set of a known regime.
@@ -20,7 +21,7 @@ import os
 import time
 
 DEFAULT = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 2,
-           "pass_mode": 0}
+           "pass_mode": 0, "merge_tile": 1024}
 
 CANDIDATES = [
     {"large_min": lm, "wave_class_max": wc, "skip_lanes": sk, "skip_strings": ss}
@@ -29,6 +30,7 @@ CANDIDATES = [
 ]
 MERGE_FANS = (2, 4, 8, 16)
 PASS_MODES = (1, 2)  # host-driven passes; device-driven passes replayed from hipGraphs
+MERGE_TILES = (512, 1024, 2048)  # outputs per tile of the pairwise merge
 
 
 def regime(info: dict, avg_lcp: int | None = None) -> str:
@@ -72,14 +74,16 @@ class Tuner:
             best = min(best, time.perf_counter() - t0)
         return best
 
-    def autotune(self, ds, candidates=None, repeats: int = 2, fans=MERGE_FANS, modes=PASS_MODES) -> dict:
+    def autotune(self, ds, candidates=None, repeats: int = 2, fans=MERGE_FANS, modes=PASS_MODES,
+                 tiles=MERGE_TILES) -> dict:
         """Times ds.sort_lcp() under every candidate, the best candidate under
         every pass mode, then (more than one PE) ds.merge_lcp() of the
-        exchanged runs under every merge fan-in; keeps the fastest for the
-        set's regime.  Returns {"regime", "best", "times_ms", "pass_times_ms",
-        "merge_times_ms"}."""
-        candidates = [dict(c, merge_fan_in=DEFAULT["merge_fan_in"], pass_mode=DEFAULT["pass_mode"])
-                      for c in (candidates or CANDIDATES)]
+        exchanged runs under every merge fan-in and, with fan-in 2, every
+        merge tile size; keeps the fastest for the set's regime.  Returns
+        {"regime", "best", "times_ms", "pass_times_ms", "merge_times_ms",
+        "merge_tile_times_ms"}."""
+        candidates = [dict(c, merge_fan_in=DEFAULT["merge_fan_in"], pass_mode=DEFAULT["pass_mode"],
+                           merge_tile=DEFAULT["merge_tile"]) for c in (candidates or CANDIDATES)]
         ds.set_sort_params(DEFAULT)
         ds.sort_lcp()  # warm up buffers; also gives the LCPs for the regime key
         ds.sync()
@@ -99,6 +103,7 @@ class Tuner:
             pass_times.sort()
             best["pass_mode"] = pass_times[0][1]
         merge_times = []
+        tile_times = []
         if ds.p > 1 and fans:
             ds.set_sort_params(best)
             ds.sort_lcp()
@@ -110,12 +115,21 @@ class Tuner:
                 merge_times.append((self._time(ds, ds.merge_lcp, repeats), fan))
             merge_times.sort()
             best["merge_fan_in"] = merge_times[0][1]
+            if best["merge_fan_in"] == 2:
+                for tile in tiles:
+                    ds.set_sort_params(dict(best, merge_tile=tile))
+                    ds.merge_lcp()
+                    tile_times.append((self._time(ds, ds.merge_lcp, repeats), tile))
+                if tile_times:
+                    tile_times.sort()
+                    best["merge_tile"] = tile_times[0][1]
         self.table[key] = best
         ds.set_sort_params(best)
         return {"regime": key, "best": dict(best),
                 "times_ms": [(round(t * 1e3, 3), c) for t, c in times],
                 "pass_times_ms": [(round(t * 1e3, 3), m) for t, m in pass_times],
-                "merge_times_ms": [(round(t * 1e3, 3), f) for t, f in merge_times]}
+                "merge_times_ms": [(round(t * 1e3, 3), f) for t, f in merge_times],
+                "merge_tile_times_ms": [(round(t * 1e3, 3), m) for t, m in tile_times]}
 
     def apply(self, ds) -> bool:
         """Sets the cached parameters of the set's regime (after one sort the

@@ -159,8 +159,10 @@ int dss_get_sorted(dss_set s, uint32_t* perm, uint32_t* lcp);
  * and replayed: one read-back per sort), 0 (default) the device-driven form
  * for sets of at most 2^22 strings over at least 16 PEs (batches of jobs),
  * the host-driven one otherwise (the host-driven form is also used while
- * kernel timing is on, and on the null stream).  Every setting produces the same output.  NULL restores the
- * defaults. */
+ * kernel timing is on, and on the null stream); merge_tile (512, 1024 or
+ * 2048, default 1024; 0 selects the default) is the number of outputs per
+ * tile of the pairwise merge (fan-in 2).  Every setting produces the same
+ * output.  NULL restores the defaults. */
 typedef struct {
     uint32_t large_min;
     uint32_t wave_class_max;
@@ -168,6 +170,7 @@ typedef struct {
     uint32_t skip_strings;
     uint32_t merge_fan_in;
     uint32_t pass_mode;
+    uint32_t merge_tile;
 } dss_sort_params_t;
 
 int dss_set_sort_params(dss_set s, const dss_sort_params_t* params);

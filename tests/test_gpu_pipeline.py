@@ -265,8 +265,9 @@ def test_drivers(name):
 @pytest.mark.parametrize("p", [5, 16, 37])
 def test_merge_fan_in_gives_the_same_output(name, p):
     """The K of the K-way merge (runs merged per pass: 2, 4, 8, 16) changes
-    the number of passes, not the output: merged order, LCPs and saved LCPs
-    are identical for every fan-in (ties go to the earlier run), the merged
+    the number of passes, and the pairwise merge's tile size (512, 1024, 2048
+    outputs) its tiling, not the output: merged order, LCPs and saved LCPs
+    are identical for every setting (ties go to the earlier run), the merged
     strings equal the oracle's loser tree and the checker passes."""
     dss = _dss()
     chars = INPUTS[name]
@@ -279,8 +280,8 @@ def test_merge_fan_in_gives_the_same_output(name, p):
             rb, bb, nbytes = ds.exchange(mode)
             ex = O.exchange(ss, operm, olcp, iv, mode)
             ref = None
-            for fan in (16, 8, 4, 2):
-                ds.set_sort_params({"merge_fan_in": fan})
+            for fan, tile in ((16, 1024), (8, 1024), (4, 1024), (2, 512), (2, 2048), (2, 1024)):
+                ds.set_sort_params({"merge_fan_in": fan, "merge_tile": tile})
                 ds.merge_lcp()
                 got = ds.get_merged()
                 if ref is None:

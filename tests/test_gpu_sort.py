@@ -119,16 +119,23 @@ def test_skewed_generator_shapes(n, ll, sl, ldn, sdn, lp, dp):
 
 
 PARAM_SETS = [
-    {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8, "skip_strings": 1, "merge_fan_in": 2, "pass_mode": 1},
-    {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 2, "merge_fan_in": 4, "pass_mode": 2},
-    {"large_min": 256, "wave_class_max": 5, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 8, "pass_mode": 0},
-    {"large_min": 1024, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 1, "merge_fan_in": 16, "pass_mode": 2},
-    {"large_min": 128, "wave_class_max": 4, "skip_lanes": 8, "skip_strings": 2, "merge_fan_in": 2, "pass_mode": 1},
-    {"large_min": 32, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 4, "merge_fan_in": 16, "pass_mode": 2},
-    {"large_min": 16, "wave_class_max": 3, "skip_lanes": 8, "skip_strings": 1, "merge_fan_in": 2, "pass_mode": 2},
+    {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8, "skip_strings": 1, "merge_fan_in": 2, "pass_mode": 1,
+     "merge_tile": 512},
+    {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 2, "merge_fan_in": 4, "pass_mode": 2,
+     "merge_tile": 2048},
+    {"large_min": 256, "wave_class_max": 5, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 8, "pass_mode": 0,
+     "merge_tile": 1024},
+    {"large_min": 1024, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 1, "merge_fan_in": 16, "pass_mode": 2,
+     "merge_tile": 512},
+    {"large_min": 128, "wave_class_max": 4, "skip_lanes": 8, "skip_strings": 2, "merge_fan_in": 2, "pass_mode": 1,
+     "merge_tile": 2048},
+    {"large_min": 32, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 4, "merge_fan_in": 16, "pass_mode": 2,
+     "merge_tile": 1024},
+    {"large_min": 16, "wave_class_max": 3, "skip_lanes": 8, "skip_strings": 1, "merge_fan_in": 2, "pass_mode": 2,
+     "merge_tile": 512},
 ]
 DEFAULT_PARAMS = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 2,
-                  "pass_mode": 0}
+                  "pass_mode": 0, "merge_tile": 1024}
 
 
 @pytest.mark.parametrize("name", ["dton", "skewed", "suffix"])
@@ -162,9 +169,12 @@ def test_autotuner_picks_and_applies_a_setting(tmp_path):
     tuner = Tuner(str(tmp_path / "t.json"))
     with dss.DeviceStringSet.dton(200000, 100, 0.5, 34, 4) as ds:
         out = tuner.autotune(ds, repeats=1)
-        sort_best = {k: v for k, v in out["best"].items() if k not in ("merge_fan_in", "pass_mode")}
-        assert sort_best in [{k: v for k, v in c.items() if k not in ("merge_fan_in", "pass_mode")}
-                             for _, c in out["times_ms"]]
+        later = ("merge_fan_in", "pass_mode", "merge_tile")
+        sort_best = {k: v for k, v in out["best"].items() if k not in later}
+        assert sort_best in [{k: v for k, v in c.items() if k not in later} for _, c in out["times_ms"]]
+        if out["best"]["merge_fan_in"] == 2:
+            assert out["best"]["merge_tile"] == out["merge_tile_times_ms"][0][1]
+            assert sorted(m for _, m in out["merge_tile_times_ms"]) == [512, 1024, 2048]
         assert out["best"]["pass_mode"] == out["pass_times_ms"][0][1]
         assert sorted(m for _, m in out["pass_times_ms"]) == [1, 2]
         assert out["best"]["merge_fan_in"] == out["merge_times_ms"][0][1]
@@ -186,6 +196,8 @@ def test_autotuner_picks_and_applies_a_setting(tmp_path):
             ds.set_sort_params({"merge_fan_in": 3})
         with pytest.raises(dss.DssError):
             ds.set_sort_params({"pass_mode": 3})
+        with pytest.raises(dss.DssError):
+            ds.set_sort_params({"merge_tile": 768})
 
 
 def _two_letter_strings(n, length, seed):
