@@ -826,15 +826,6 @@ __device__ inline uint32_t block_excl_max(uint32_t v, uint32_t* tmp) {
     return max(pre, ex);
 }
 
-// LCP of two strings from depth d (one lane), 32 characters per dependent
-// step (five aligned loads per string and step).
-__device__ inline uint32_t lcp_walk(const uint8_t* __restrict__ chars, uint64_t ao, uint32_t al,
-                                    uint64_t bo, uint32_t bl, uint32_t d) {
-    uint32_t l = 0;
-    (void)view_cmp32(chars, ao, al, bo, bl, &l, d);
-    return l - d;
-}
-
 constexpr uint32_t kCap = kSmallMax;
 constexpr uint32_t kPer = kCap / kThreads;  // elements per thread
 
@@ -885,14 +876,58 @@ __device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __
         for (uint32_t e = tid; e < kCap; e += kThreads)
             if (I[e] != kNone && ACT[G[e]] && e == G[e]) RC[e] = kNone;
         __syncthreads();
-        for (uint32_t e = tid; e < kCap; e += kThreads) {
-            const uint32_t id = I[e];
-            if (id == kNone) continue;
-            const uint32_t g = G[e];
-            if (!ACT[g] || e == g) continue;  // groups have >= 2 members
-            const uint32_t r = I[g];
-            const uint32_t l = lcp_walk(chars, off[id], len[id], off[r], len[r], RD[g]);
-            atomicMin(&RC[g], l);
+        // every member's LCP with its group's first member, from the group
+        // depth: one 32-character step per member in its lane, then the
+        // walks still open are served one at a time by the whole wave (512
+        // characters per step), so a lane whose strings share hundreds of
+        // characters does not hold the other lanes through a long loop
+        {
+            uint64_t wa[kPer], wb[kPer];
+            uint32_t wal[kPer], wbl[kPer], wfrom[kPer], wg[kPer];
+            bool open[kPer];
+#pragma unroll
+            for (uint32_t q = 0; q < kPer; ++q) {
+                const uint32_t e = tid + q * kThreads;
+                open[q] = false;
+                const uint32_t id = I[e];
+                if (id == kNone) continue;
+                const uint32_t g = G[e];
+                if (!ACT[g] || e == g) continue;  // groups have >= 2 members
+                const uint32_t r = I[g];
+                const uint32_t d = RD[g];
+                wa[q] = off[id];
+                wal[q] = len[id];
+                wb[q] = off[r];
+                wbl[q] = len[r];
+                wg[q] = g;
+                uint64_t x[4], y[4];
+                load_keys4(chars, wa[q], wal[q], d, x);
+                load_keys4(chars, wb[q], wbl[q], d, y);
+                uint32_t l = kNone;
+#pragma unroll
+                for (int j = 3; j >= 0; --j) {  // the first difference or end wins
+                    if (x[j] != y[j]) l = 8 * j + key_lcp(x[j], y[j]);
+                    else if ((x[j] & 0xFF) == 0) l = 8 * j + key_strlen(x[j]);
+                }
+                if (l != kNone) {
+                    atomicMin(&RC[g], l);
+                } else {
+                    open[q] = true;
+                    wfrom[q] = d + 32;
+                }
+            }
+            const uint32_t lane = lane_id();
+#pragma unroll
+            for (uint32_t q = 0; q < kPer; ++q) {
+                for (uint64_t m = __ballot(open[q]); m; m &= m - 1) {  // wave-uniform
+                    const int ld = __ffsll((unsigned long long)m) - 1;
+                    const uint64_t ao = __shfl((unsigned long long)wa[q], ld), bo = __shfl((unsigned long long)wb[q], ld);
+                    const uint32_t al = (uint32_t)__shfl((int)wal[q], ld), bl = (uint32_t)__shfl((int)wbl[q], ld);
+                    const uint32_t from = (uint32_t)__shfl((int)wfrom[q], ld);
+                    const uint32_t l = wave_lcp(chars, ao, al, bo, bl, from);
+                    if (lane == (uint32_t)ld) atomicMin(&RC[wg[q]], l - (from - 32));
+                }
+            }
         }
         __syncthreads();
         for (uint32_t e = tid; e < kCap; e += kThreads)
