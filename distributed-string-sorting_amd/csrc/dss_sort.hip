@@ -44,7 +44,7 @@
 namespace dss {
 
 constexpr uint32_t kSmallMax = 1024;  // base-case capacity (slots per block)
-constexpr uint32_t kClasses = 12;     // class c holds sizes (2^(c-1), 2^c], c = 1..11
+constexpr uint32_t kClasses = 11;     // class c holds sizes (2^(c-1), 2^c], c = 1..10
 constexpr uint32_t kTile = 2048;      // strings per tile in the global passes
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kChunkTiles = 64;  // tiles per chunk of the two-level digit-offset scan
@@ -827,7 +827,13 @@ __device__ inline uint32_t block_excl_max(uint32_t v, uint32_t* tmp) {
 }
 
 constexpr uint32_t kCap = kSmallMax;
+// large_min <= kSmallMax keeps every small segment in classes 1..kClasses-1,
+// and a block holds at least one segment of each of them
+static_assert((kCap >> (kClasses - 1)) == 1, "the largest class fills one block");
 constexpr uint32_t kPer = kCap / kThreads;  // elements per thread
+constexpr uint16_t kNoSlot = 0xFFFF;        // empty slot position
+static_assert(kCap < kNoSlot, "slot positions are 16-bit");
+constexpr uint32_t kRankMax = 32;           // rank sort when no active group is larger
 
 __device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __restrict__ segs, uint32_t nseg,
                                                  uint32_t cls_log2, uint32_t* __restrict__ perm,
@@ -835,17 +841,21 @@ __device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __
                                                  const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
                                                  uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_len) {
     __shared__ uint64_t K[kCap];
-    __shared__ uint32_t G[kCap];
-    __shared__ uint32_t I[kCap];
+    __shared__ uint64_t OFF[kCap];  // offset and length of the string loaded at
+    __shared__ uint32_t LEN[kCap];  // slot position e (gathered once per block)
+    __shared__ uint16_t G[kCap];
+    __shared__ uint16_t I[kCap];   // first slot position of the string now at e
     __shared__ uint32_t LO[kCap];  // lcp per slot position
     __shared__ uint32_t RD[kCap];  // group depth, indexed by group start
     __shared__ uint32_t RC[kCap];  // group common prefix / scratch
     __shared__ uint8_t ACT[kCap];  // group active, indexed by group start
     __shared__ uint32_t tmp[kThreads / 64];
+    __shared__ uint32_t sMaxG;  // largest group activated by the round's step 4
     const uint32_t CLS = 1u << cls_log2;
     const uint32_t spb = kCap >> cls_log2;
     const uint32_t s0 = tb * spb;
     const uint32_t tid = threadIdx.x;
+    if (tid == 0) sMaxG = 0;
 
     for (uint32_t e = tid; e < kCap; e += kThreads) {
         const uint32_t slot = e >> cls_log2, j = e & (CLS - 1);
@@ -853,15 +863,18 @@ __device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __
         LO[e] = 0;
         ACT[e] = 0;
         K[e] = ~0ull;
-        I[e] = kNone;
-        G[e] = sbase;
+        I[e] = kNoSlot;
+        G[e] = (uint16_t)sbase;
         if (s0 + slot < nseg) {
             const SmallSeg sg = segs[s0 + slot];
             if (j < sg.cnt) {
-                I[e] = perm[sg.begin + j];
+                const uint32_t id = perm[sg.begin + j];
+                OFF[e] = off[id];
+                LEN[e] = len[id];
+                I[e] = (uint16_t)e;
                 K[e] = 0;
             } else {
-                G[e] = sbase + sg.cnt;
+                G[e] = (uint16_t)(sbase + sg.cnt);
             }
             if (j == 0) {
                 RD[e] = sg.depth;
@@ -871,11 +884,14 @@ __device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __
     }
     __syncthreads();
 
+    uint32_t maxg = kNone;  // largest active group of the round
+    bool first = true;
     for (;;) {
         // 1. prefix skip inside every active group
         for (uint32_t e = tid; e < kCap; e += kThreads)
-            if (I[e] != kNone && ACT[G[e]] && e == G[e]) RC[e] = kNone;
+            if (I[e] != kNoSlot && ACT[G[e]] && e == G[e]) RC[e] = kNone;
         __syncthreads();
+        if (tid == 0) sMaxG = 0;  // every thread read it before the barrier
         // every member's LCP with its group's first member, from the group
         // depth: one 32-character step per member in its lane, then the
         // walks still open are served one at a time by the whole wave (512
@@ -889,16 +905,16 @@ __device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __
             for (uint32_t q = 0; q < kPer; ++q) {
                 const uint32_t e = tid + q * kThreads;
                 open[q] = false;
-                const uint32_t id = I[e];
-                if (id == kNone) continue;
+                const uint32_t sl = I[e];
+                if (sl == kNoSlot) continue;
                 const uint32_t g = G[e];
                 if (!ACT[g] || e == g) continue;  // groups have >= 2 members
                 const uint32_t r = I[g];
                 const uint32_t d = RD[g];
-                wa[q] = off[id];
-                wal[q] = len[id];
-                wb[q] = off[r];
-                wbl[q] = len[r];
+                wa[q] = OFF[sl];
+                wal[q] = LEN[sl];
+                wb[q] = OFF[r];
+                wbl[q] = LEN[r];
                 wg[q] = g;
                 uint64_t x[4], y[4];
                 load_keys4(chars, wa[q], wal[q], d, x);
@@ -931,40 +947,95 @@ __device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __
         }
         __syncthreads();
         for (uint32_t e = tid; e < kCap; e += kThreads)
-            if (I[e] != kNone && e == G[e] && ACT[e]) RD[e] += RC[e];
+            if (I[e] != kNoSlot && e == G[e] && ACT[e]) RD[e] += RC[e];
         __syncthreads();
         // 2. keys of active groups
         for (uint32_t e = tid; e < kCap; e += kThreads) {
-            const uint32_t id = I[e];
-            if (id != kNone && ACT[G[e]]) K[e] = load_key8(chars, off[id], len[id], RD[G[e]]);
+            const uint32_t sl = I[e];
+            if (sl != kNoSlot && ACT[G[e]]) K[e] = load_key8(chars, OFF[sl], LEN[sl], RD[G[e]]);
         }
         __syncthreads();
-        // 3. bitonic sort of (G, K, I) inside every aligned CLS slot.  The
-        // string id makes the order total: a sorting network that is not
-        // stable then still returns equal strings in ascending id order, the
-        // order the stable radix passes keep, however many further rounds a
-        // block runs for other slots' groups.
-        for (uint32_t k = 2; k <= CLS; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t t = tid; t < kCap / 2; t += kThreads) {
-                    const uint32_t i = 2 * t - (t & (j - 1));
-                    const uint32_t q = i + j;
-                    const bool asc = (k == CLS) || ((i & k) == 0);
-                    const uint32_t gi = G[i], gq = G[q];
-                    const uint64_t ki = K[i], kq = K[q];
-                    const uint32_t ii = I[i], iq = I[q];
-                    const bool gt = gi > gq || (gi == gq && (ki > kq || (ki == kq && ii > iq)));
-                    const bool lt = gi < gq || (gi == gq && (ki < kq || (ki == kq && ii < iq)));
-                    if (asc ? gt : lt) {
-                        G[i] = gq;
-                        G[q] = gi;
-                        K[i] = kq;
-                        K[q] = ki;
-                        I[i] = iq;
-                        I[q] = ii;
-                    }
+        // 3. sort of (G, K, I) inside every aligned CLS slot.  The first slot
+        // position makes the order total, so equal strings keep their order
+        // on entry, however many further rounds a block runs for other
+        // slots' groups.  The MSD passes are stable from the identity
+        // permutation, so that entry order of equal strings is ascending id
+        // order.
+        if (maxg <= kRankMax) {
+            // every active group is small: each member counts the members
+            // before it and moves there (inactive groups stay in place)
+            uint64_t nk[kPer];
+            uint32_t npos[kPer], ni[kPer];
+#pragma unroll
+            for (uint32_t q = 0; q < kPer; ++q) {
+                const uint32_t e = tid + q * kThreads;
+                npos[q] = kNone;
+                const uint32_t sl = I[e];
+                const uint32_t g = G[e];
+                if (sl == kNoSlot || !ACT[g]) continue;
+                const uint64_t k = K[e];
+                uint32_t rank = 0;
+                for (uint32_t m = g; m < kCap && G[m] == g; ++m) {
+                    const uint64_t km = K[m];
+                    rank += (km < k || (km == k && I[m] < sl));
                 }
-                __syncthreads();
+                npos[q] = g + rank;
+                nk[q] = k;
+                ni[q] = sl;
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t q = 0; q < kPer; ++q) {
+                if (npos[q] == kNone) continue;
+                K[npos[q]] = nk[q];
+                I[npos[q]] = (uint16_t)ni[q];
+            }
+            __syncthreads();
+        } else if (first) {
+            // one group per slot: empty positions (key ~0, slot kNoSlot) end
+            // every slot, so (K, I) alone orders it
+            for (uint32_t k = 2; k <= CLS; k <<= 1) {
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t t = tid; t < kCap / 2; t += kThreads) {
+                        const uint32_t i = 2 * t - (t & (j - 1));
+                        const uint32_t q = i + j;
+                        const bool asc = (k == CLS) || ((i & k) == 0);
+                        const uint64_t ki = K[i], kq = K[q];
+                        const uint32_t ii = I[i], iq = I[q];
+                        const bool gt = ki > kq || (ki == kq && ii > iq);
+                        if (asc == gt && (ki != kq || ii != iq)) {
+                            K[i] = kq;
+                            K[q] = ki;
+                            I[i] = (uint16_t)iq;
+                            I[q] = (uint16_t)ii;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        } else {
+            for (uint32_t k = 2; k <= CLS; k <<= 1) {
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t t = tid; t < kCap / 2; t += kThreads) {
+                        const uint32_t i = 2 * t - (t & (j - 1));
+                        const uint32_t q = i + j;
+                        const bool asc = (k == CLS) || ((i & k) == 0);
+                        const uint32_t gi = G[i], gq = G[q];
+                        const uint64_t ki = K[i], kq = K[q];
+                        const uint32_t ii = I[i], iq = I[q];
+                        const bool gt = gi > gq || (gi == gq && (ki > kq || (ki == kq && ii > iq)));
+                        const bool lt = gi < gq || (gi == gq && (ki < kq || (ki == kq && ii < iq)));
+                        if (asc ? gt : lt) {
+                            G[i] = (uint16_t)gq;
+                            G[q] = (uint16_t)gi;
+                            K[i] = kq;
+                            K[q] = ki;
+                            I[i] = (uint16_t)iq;
+                            I[q] = (uint16_t)ii;
+                        }
+                    }
+                    __syncthreads();
+                }
             }
         }
         // 4. boundaries, LCPs and new groups (thread owns kPer consecutive slots)
@@ -976,7 +1047,7 @@ __device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __
             const uint32_t g = G[e];
             oldg[q] = g;
             uint32_t f = (e == g);
-            if (!f && I[e] != kNone && ACT[g]) {
+            if (!f && I[e] != kNoSlot && ACT[g]) {
                 const uint64_t a = K[e - 1], b = K[e];
                 const uint32_t d = RD[g];
                 if (a != b) {
@@ -1006,7 +1077,7 @@ __device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __
         __syncthreads();
         for (uint32_t q = 0; q < kPer; ++q) {
             const uint32_t e = tid * kPer + q;
-            G[e] = newg[q];
+            G[e] = (uint16_t)newg[q];
             if (flag[q]) {
                 ACT[e] = 0;
                 if (oact[q]) RD[e] = od[q] + 8;
@@ -1015,27 +1086,47 @@ __device__ __forceinline__ void small_sort_block(uint32_t tb, const SmallSeg* __
         __syncthreads();
         // a new group stays active if it came from an active group, has a
         // second member and its key did not reach the end of the strings
+        // (the condition holds for all members but the first of such a
+        // group, so the last member's position gives the group size)
         int any = 0;
+        uint32_t gmax = 0;
         for (uint32_t q = 0; q < kPer; ++q) {
             const uint32_t e = tid * kPer + q;
-            if (oact[q] && !flag[q] && I[e] != kNone && (K[e] & 0xFF) != 0) {
+            if (oact[q] && !flag[q] && I[e] != kNoSlot && (K[e] & 0xFF) != 0) {
                 ACT[newg[q]] = 1;
                 any = 1;
+                gmax = max(gmax, e - newg[q] + 1);
             }
         }
+        if (gmax) atomicMax(&sMaxG, gmax);
         if (!__syncthreads_or(any)) break;
+        maxg = sMaxG;
+        first = false;
     }
-    // write back
-    for (uint32_t e = tid; e < kCap; e += kThreads) {
+    // write back: the ids are read from the entry permutation at their first
+    // slot positions (same segment) before any position is overwritten
+    uint32_t wid[kPer], wpos[kPer];
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t e = tid + q * kThreads;
         const uint32_t slot = e >> cls_log2, j = e & (CLS - 1);
+        wid[q] = kNone;
         if (s0 + slot >= nseg) continue;
         const SmallSeg sg = segs[s0 + slot];
         if (j >= sg.cnt) continue;
-        const uint32_t id = I[e];
-        perm[sg.begin + j] = id;
-        out_off[sg.begin + j] = off[id];
-        out_len[sg.begin + j] = len[id];
-        if (j) lcp[sg.begin + j] = LO[e];
+        wpos[q] = sg.begin + j;
+        wid[q] = perm[sg.begin + (I[e] & (CLS - 1))];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+        if (wid[q] == kNone) continue;
+        const uint32_t e = tid + q * kThreads;
+        const uint32_t sl = I[e];
+        perm[wpos[q]] = wid[q];
+        out_off[wpos[q]] = OFF[sl];
+        out_len[wpos[q]] = LEN[sl];
+        if (e & (CLS - 1)) lcp[wpos[q]] = LO[e];
     }
 }
 
@@ -1389,7 +1480,7 @@ static PassBufs first_pass_bufs(SortScratch& X) {
 
 static const char* const kClassNames[kClasses] = {
     "small_sort_c0", "small_sort_c1", "small_sort_c2", "small_sort_c3", "small_sort_c4", "small_sort_c5",
-    "small_sort_c6", "small_sort_c7", "small_sort_c8", "small_sort_c9", "small_sort_c10", "small_sort_c11"};
+    "small_sort_c6", "small_sort_c7", "small_sort_c8", "small_sort_c9", "small_sort_c10"};
 
 // Host-driven MSD passes from the state in `b` until the segment list is
 // empty: every pass sizes its launches from the counters read back after the
