@@ -233,6 +233,7 @@ def batched_leg(args):
         log("served D differs from plain")
         sys.exit(1)
     lat = [r.latency for r in res]
+    admit_wait = [r.admitted - r.arrival for r in res]
     span = max(r.done for r in res) - min(r.arrival for r in res)
     plain_lat = fifo_latencies(sspecs, {j: plain_t[j] + gen_t[j] for j in plain_t})
     n_total = sum(s.n for s in specs)
@@ -263,6 +264,10 @@ def batched_leg(args):
             "gbs": round(D_total / span / 1e9, 3),
             "wall_s": round(wall, 3),
             **percentiles(lat),
+            # arrival to admission (the job's characters fit the pool): jobs
+            # behind a large one waiting for memory are admitted past it
+            # (backfill), so this stays short for the small jobs
+            "admission_wait": percentiles(admit_wait),
             "plain_fifo": percentiles(plain_lat),
         },
     }

@@ -1230,9 +1230,13 @@ int dss_pool_create(uint64_t char_capacity, uint64_t string_capacity, dss_pool* 
             P->off.reserve(std::max<uint64_t>(string_capacity, 1) * 8);
             P->len.reserve(std::max<uint64_t>(string_capacity, 1) * 4);
             P->tot.reserve(8);
+            // the generators' scan scratch and read-back mailbox, so that adding
+            // a job allocates nothing
+            P->scan.scan_tmp.reserve(scan_scratch_words(std::max<uint64_t>(string_capacity, 1)) * 8);
+            P->scan.scan_tot.reserve(8);
             P->bytes.init(char_capacity);
             P->strings.init(string_capacity);
-            DSS_HIP(hipStreamSynchronize(P->stream));
+            P->scan.mbox.wait(P->stream);
         } catch (...) {
             if (P->stream) (void)hipStreamDestroy(P->stream);
             delete P;

@@ -1925,6 +1925,10 @@ int dss_set_reserve(dss_set s, uint64_t num_strings, uint32_t num_pes) {
         const uint64_t nn = std::max<uint64_t>(num_strings, 1);
         DSS_HIP(hipStreamSynchronize(s->stream));
         reserve_sort(s, num_strings, num_pes);
+        // the base cases' packed offset/length records are otherwise grown on
+        // first use, an allocation inside a sort (which can wait on the
+        // driver, e.g. for freed memory to be cleared)
+        s->ss.offlen.reserve(nn * sizeof(OffLen));
         s->sort_cap_n = std::max<uint64_t>(s->sort_cap_n, num_strings);
         s->sort_cap_p = std::max<uint32_t>(s->sort_cap_p, num_pes);
         s->pe_begin_d.reserve_keep((uint64_t)(num_pes + 1) * 8, s->stream);

@@ -105,14 +105,22 @@ class Server:
     """`inflight` workers, each with its own batch set and stream.  Worker 0
     takes the oldest waiting large job (at least `large_strings` strings) on
     its own, else a batch of small jobs; the other workers only batch small
-    jobs, so small jobs never wait behind a large one in the same batch."""
+    jobs, so small jobs never wait behind a large one in the same batch.
+
+    Admission is in arrival order, with backfill: while an arrived job does
+    not fit the pool, later arrivals whose characters take at most
+    `backfill_bytes` (default: 1/16 of the pool's character capacity) are
+    admitted past it, so a large job waiting for memory does not hold up the
+    small jobs behind it; the large job is admitted as soon as it fits."""
 
     def __init__(self, pool: JobPool, inflight: int = 2, max_batch_strings: int = 32 << 20,
-                 large_strings: int | None = 4 << 20, reserve_strings: int = 0):
+                 large_strings: int | None = 4 << 20, reserve_strings: int = 0,
+                 backfill_bytes: int | None = None):
         self.pool = pool
         self.inflight = inflight
         self.max_batch_strings = max_batch_strings
         self.large_strings = large_strings
+        self.backfill_bytes = backfill_bytes
         # device buffers of every worker's batch set are grown to this many
         # strings when the set is created, so batches allocate nothing later
         self.reserve_strings = reserve_strings
@@ -194,36 +202,65 @@ class Server:
         def now():
             return time.perf_counter() - t0
 
+        backfill = self.backfill_bytes
+        if backfill is None and resident is None:
+            backfill = pool.usage()["bytes_capacity"] // 16
+
         def admit():
             try:
-                for s in order:
-                    if realtime:
-                        delay = s.arrival - now()
-                        if delay > 0:
-                            time.sleep(delay)
-                    r = res[s.jid]
-                    r.arrival = s.arrival if realtime else 0.0
+                pending = []  # arrived, not yet admitted, in arrival order
+                k = 0
+                while k < len(order) or pending:
+                    # arrivals up to now join the pending list
+                    while k < len(order) and (not realtime or order[k].arrival <= now()):
+                        s = order[k]
+                        res[s.jid].arrival = s.arrival if realtime else 0.0
+                        pending.append(s)
+                        k += 1
                     if resident is not None:
-                        r.admitted = now()
-                        with cv:
-                            ready.append((s, resident[s.jid]))
-                            cv.notify_all()
+                        for s in pending:
+                            res[s.jid].admitted = now()
+                            with cv:
+                                ready.append((s, resident[s.jid]))
+                                cv.notify_all()
+                        pending = []
                         continue
-                    while True:
+                    # admit in arrival order; behind a job that does not fit,
+                    # only jobs of at most `backfill` characters
+                    blocked, keep = None, []
+                    for s in pending:
+                        if blocked is not None and s.max_bytes() > backfill:
+                            keep.append(s)
+                            continue
                         try:
                             job = s.add_to(pool)
-                            break
                         except DssError as e:
                             if e.code != DSS_ENOMEM:
                                 raise
                             with cv:
-                                if not ready and state["inflight_jobs"] == 0:
+                                if not ready and state["inflight_jobs"] == 0 and blocked is None \
+                                        and s is pending[0]:
                                     raise DssError(e.code, f"job {s.jid} does not fit the empty pool") from e
-                                cv.wait(timeout=0.05)
-                    r.admitted = now()
-                    with cv:
-                        ready.append((s, job))
-                        cv.notify_all()
+                            if blocked is None:
+                                blocked = s
+                            keep.append(s)
+                            continue
+                        res[s.jid].admitted = now()
+                        with cv:
+                            ready.append((s, job))
+                            cv.notify_all()
+                    pending = keep
+                    # wait for the next arrival, or for memory to free
+                    nxt = order[k].arrival - now() if (realtime and k < len(order)) else None
+                    if pending or (nxt is not None and nxt > 0):
+                        timeout = 0.05 if pending else nxt
+                        if pending and nxt is not None:
+                            timeout = min(timeout, max(nxt, 0.0))
+                        if pending:
+                            with cv:
+                                cv.wait(timeout=timeout)
+                        else:
+                            time.sleep(timeout)
             except BaseException as e:  # noqa: BLE001 -- reported by run()
                 state["error"] = e
             finally:

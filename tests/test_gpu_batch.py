@@ -196,6 +196,34 @@ def test_reserved_batches_give_the_same_results():
         assert pool.usage()["bytes_used"] == 0
 
 
+def test_reserved_set_sorts_without_allocating():
+    """After dss_set_reserve a batch set sorts jobs up to the reserved size
+    without allocating device memory: a 1M-string job whose base cases run
+    one segment per wave (where the sort packs every string's offset and
+    length into 16-byte records) leaves the free device memory unchanged, as
+    does re-pointing the set at a smaller job."""
+    dss = _dss()
+    from helpers import HipMem
+    with dss.JobPool(256 << 20, 4 << 20) as pool, HipMem() as hm:
+        big = pool.add_dton(1_000_000, 20, 0.1, 7)
+        small = pool.add_dton(300_000, 20, 0.1, 8)
+        b = pool.batch([big])
+        b.set.reserve(2_000_000, 4)
+        b.sort()  # code objects and the mailbox are set up by the first sort
+        b.dist_sizes()
+        hm.sync()
+        free0 = hm.free_bytes()
+        D = []
+        for jobs in ([big], [small], [big, small]):
+            b.rebatch(jobs)
+            b.sort()
+            D.append([int(x) for x in b.dist_sizes()])
+        hm.sync()
+        assert hm.free_bytes() == free0
+        assert D[2] == D[0] + D[1]
+        b.close()
+
+
 def _spec_chars(s):
     """Host copy of a job spec's strings from the oracle's generators."""
     if s.kind == "dton":

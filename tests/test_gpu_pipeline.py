@@ -518,6 +518,55 @@ def test_prefix_doubling_exchange_from_kept_characters(p):
         assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0
 
 
+def _rows_inputs():
+    rng = np.random.default_rng(11)
+    base = [bytes(s) for s in O.gen_dton(8000, 500, 0.5, 0x5EED0009).tobytes().split(b"\0")[:-1]]
+    # every string twice, the copies in the other half of the PEs: a
+    # duplicate on another PE keeps each string a candidate past the round
+    # after its local LCPs, up to its end
+    dup = base + [base[int(i)] for i in rng.permutation(len(base))]
+    # D/N strings cut at 200-300 characters: some end before the round
+    # after their local LCPs
+    cut = [x[:int(l)] for x, l in zip(base, rng.integers(200, 301, len(base)))]
+    return {"dton500": arena(base), "cross_pe_duplicates": arena(dup), "cut_200_300": arena(cut)}
+
+
+ROWS_INPUTS = _rows_inputs()
+
+
+@pytest.mark.parametrize("pass_mode", [1, 2])
+@pytest.mark.parametrize("m", [2**64 - 1, 4093])
+@pytest.mark.parametrize("name", sorted(ROWS_INPUTS))
+def test_prefix_doubling_sort_kept_rows_edge_cases(name, m, pass_mode):
+    """The composite driver on 500-character D/N strings, where the depth-264
+    round hashes every string from its run head's state and keeps its last
+    stripe and tail for the exchange: rows that are the last ones (distinct
+    strings), rows overwritten by later rounds (exact duplicates on other
+    PEs, small hash ranges) and strings that end before that round.  Results,
+    rounds, receive bytes and merged LCPs match the oracle, twice in a row
+    (the second sort reuses the first one's buffers), for host- and
+    device-driven local sorts (16 PEs)."""
+    dss = _dss()
+    chars = ROWS_INPUTS[name]
+    p = 4 if pass_mode == 1 else 16
+    ss = O.StringSet(chars, p)
+    res = O.prefix_doubling_sort(ss, m=m)
+    assert max(res["rounds"]) >= 6  # the depth-264 round ran
+    with dss.DeviceStringSet.from_arena(chars, p) as ds:
+        ds.set_sort_params({"pass_mode": pass_mode})
+        for _ in range(2):
+            ds.prefix_doubling_sort(m=m)
+            gres, grounds = ds.get_prefix_doubling()
+            assert (gres == res["results"]).all() and (grounds == res["rounds"]).all()
+            ex = res["exchange"]
+            nbytes = int(ex["byte_begin"][-1])
+            assert ds.get_exchange_bytes(nbytes).tobytes() == ex["bytes"].tobytes()
+            gord, gml, gsv = ds.get_merged()
+            assert (gml == np.concatenate([o["lcp"] for o in res["out"]])).all()
+            chk = ds.check()
+            assert chk["sorted"] and chk["complete"] and chk["lcp_errors"] == 0
+
+
 def test_exchange_origin_follows_each_exchange():
     """The received strings' origin (source PE, position) is computed on the
     first request after an exchange: repeated exchanges of one set with other
