@@ -656,12 +656,29 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
             const uint64_t E = __shfl(start + m + 1, last);
             const uint64_t A0 = A & ~7ull;
             const bool fits = E - A0 <= 8ull * kXcWords;  // wave-uniform
-            uint8_t* const gdst = out + start;
-            uint8_t* const ldst = sb + (start - A0);
-            auto put = [&](uint32_t t, uint8_t c) {
-                if (fits) ldst[t] = c;
-                else gdst[t] = c;
+            uint8_t* const gdst = out + start;  // strings of a wave that does not fit: byte stores
+            if (fits) {
+                // the wave's buffer words start at zero (the NULs and any
+                // padding), every string ORs its characters in
+                const uint32_t nwords = (uint32_t)((E - A0 + 7) / 8);
+                for (uint32_t w = lane; w < nwords; w += 64) sw[w] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            }
+            // ORs bytes [lo, hi) of the word W, whose byte 0 belongs at buffer
+            // byte db (< 0 for bytes before this string's), into the buffer:
+            // two word ORs instead of a store per byte
+            auto or_word = [&](int64_t db, uint64_t W, int lo, int hi) {
+                if (lo >= hi) return;
+                W &= (hi - lo == 8 ? ~0ull : ((1ull << (8 * (hi - lo))) - 1)) << (8 * lo);
+                const int64_t q = db >> 3;
+                const unsigned sh = (unsigned)(db & 7) * 8;
+                const uint64_t wl = W << sh, wh = sh ? W >> (64 - sh) : 0ull;
+                if (wl) atomicOr(reinterpret_cast<unsigned long long*>(sw + q), (unsigned long long)wl);
+                if (wh) atomicOr(reinterpret_cast<unsigned long long*>(sw + q + 1), (unsigned long long)wh);
             };
+            const int64_t d0 = (int64_t)(start - A0);  // this string's first buffer byte
             if (valid) {
                 bool done = false;
                 if (si != kNone) {
@@ -694,11 +711,20 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
                             if (rem < 8) key2 &= ~0ull << (8 * (8 - rem));
                         }
                         keyc2[k] = key2;
-                        for (uint32_t t = 0; t < m; t += 8) {
-                            const uint64_t y = rword(from - B + t);
+                        // row bytes [x0, x1) are the characters to copy
+                        const int x0 = (int)(from - B), x1 = x0 + (int)m;
+                        if (fits) {
 #pragma unroll
-                            for (uint32_t b = 0; b < 8; ++b)
-                                if (t + b < m) put(t + b, (uint8_t)(y >> (8 * b)));
+                            for (int j = 0; j < 9; ++j)
+                                if (j < (int)stage_w)
+                                    or_word(d0 - x0 + 8 * j, row[j], max(0, x0 - 8 * j), min(8, x1 - 8 * j));
+                        } else {
+                            for (uint32_t t = 0; t < m; t += 8) {
+                                const uint64_t y = rword(from - B + t);
+#pragma unroll
+                                for (uint32_t b = 0; b < 8; ++b)
+                                    if (t + b < m) gdst[t + b] = (uint8_t)(y >> (8 * b));
+                            }
                         }
                         done = true;
                     }
@@ -707,20 +733,37 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
                     keyc[k] = load_key8(chars, r_voff[k], vlen, cpk);
                     keyc2[k] = load_key8(chars, r_voff[k], vlen, cpk + 8);
                     const uint64_t a = r_voff[k] + from;
-                    for (uint32_t t = 0; t < m; t += 32) {
-                        const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + ((a + t) & ~7ull));
-                        const unsigned sh = (unsigned)((a + t) & 7) * 8;
-                        uint64_t x[5], y[4];
+                    if (fits) {
+                        // the arena's aligned words from the one holding
+                        // character `from`, four at a time
+                        const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + (a & ~7ull));
+                        const int r = (int)(a & 7), end = r + (int)m;
+                        for (int i = 0; i < end; i += 32) {
+                            uint64_t x[4];
 #pragma unroll
-                        for (int i = 0; i < 5; ++i) x[i] = w[i];
+                            for (int j = 0; j < 4; ++j) x[j] = i + 8 * j < end ? w[(i >> 3) + j] : 0ull;
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) y[i] = sh ? (x[i] >> sh) | (x[i + 1] << (64 - sh)) : x[i];
+                            for (int j = 0; j < 4; ++j) {
+                                const int o = i + 8 * j;
+                                or_word(d0 - r + o, x[j], max(0, r - o), min(8, end - o));
+                            }
+                        }
+                    } else {
+                        for (uint32_t t = 0; t < m; t += 32) {
+                            const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + ((a + t) & ~7ull));
+                            const unsigned sh = (unsigned)((a + t) & 7) * 8;
+                            uint64_t x[5], y[4];
 #pragma unroll
-                        for (uint32_t b = 0; b < 32; ++b)
-                            if (t + b < m) put(t + b, (uint8_t)(y[b >> 3] >> (8 * (b & 7))));
+                            for (int i = 0; i < 5; ++i) x[i] = w[i];
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) y[i] = sh ? (x[i] >> sh) | (x[i + 1] << (64 - sh)) : x[i];
+#pragma unroll
+                            for (uint32_t b = 0; b < 32; ++b)
+                                if (t + b < m) gdst[t + b] = (uint8_t)(y[b >> 3] >> (8 * (b & 7)));
+                        }
                     }
                 }
-                put(m, 0);
+                if (!fits) gdst[m] = 0;
             }
             if (fits) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
