@@ -680,10 +680,11 @@ k_scan_classify(const Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restric
 
 // ------------------------------------------------------------ scatter
 // Stable scatter of one tile by its digit.  The tile is first put in digit
-// order in LDS (wave ballots rank equal digits, in index order) and then
-// written run by run, so consecutive lanes store consecutive addresses of a
-// digit's run instead of a few records of every run per store.  512 threads
-// x 4 records per tile: 37 KB of LDS per 8 waves, 8 waves per SIMD.
+// order in LDS (wave ballots and per-wave digit counts rank equal digits, in
+// index order) and then written run by run, so consecutive lanes store
+// consecutive addresses of a digit's run instead of a few records of every
+// run per store.  512 threads x 4 records per tile, 30 KB of LDS per 8 waves;
+// four block barriers per tile.
 constexpr uint32_t kScT = 512;           // scatter threads per tile
 constexpr uint32_t kScI = kTile / kScT;  // records per thread
 static_assert(kScI * kScT == kTile && kScT >= 256 && kScT % 64 == 0, "scatter tile shape");
@@ -693,63 +694,43 @@ __device__ __forceinline__ void scatter_tile(uint32_t tb, const Tile* __restrict
                                              const uint32_t* __restrict__ pb_in, uint64_t* __restrict__ key_out,
                                              uint32_t* __restrict__ pb_out, uint32_t* __restrict__ perm,
                                              uint32_t* __restrict__ lcp) {
+    constexpr uint32_t kW = kScT / 64;         // waves per tile
+    constexpr uint32_t kWR = kScI * 64;        // records per wave (contiguous)
     __shared__ uint64_t skey[kTile];
     __shared__ uint32_t sid[kTile];
     __shared__ uint32_t gbase[256];   // the digit's output offset for this tile
     __shared__ uint32_t lstart[256];  // the digit's first slot in the reordered tile
-    __shared__ uint32_t run[256];     // next free slot of the digit
-    __shared__ uint32_t wcnt[kScT / 64][256];
-    __shared__ uint32_t scan[256];
+    __shared__ uint16_t wcnt[kW][256];  // a wave's count of a digit, then the waves before it
+    __shared__ uint32_t wtot[256 / 64];
     const Tile T = tiles[tb];
     const Seg S = segs[T.seg];
     uint32_t kd, dep;
     seg_depths(S, kd, dep);
     const uint32_t sh = 56 - 8 * (dep - kd);
-    const uint32_t tid = threadIdx.x, w = tid >> 6;
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const bool dig = tid < 256;  // the threads that own a digit
-    if (dig) {
-        gbase[tid] = hist[(uint64_t)tb * 256 + tid];
-        run[tid] = 0;
-        for (uint32_t k = 0; k < kScT / 64; ++k) wcnt[k][tid] = 0;
-    }
+    if (dig) gbase[tid] = hist[(uint64_t)tb * 256 + tid];
+    for (uint32_t k = tid; k < kW * 256; k += kScT) (&wcnt[0][0])[k] = 0;
     __syncthreads();
-    // the tile's records and their digits
-    uint64_t kv[kScI];
-    uint32_t idv[kScI], dg[kScI];
-#pragma unroll
-    for (uint32_t r = 0; r < kScI; ++r) {
-        const uint32_t e = T.begin + r * kScT + tid;
-        kv[r] = 0;
-        idv[r] = 0;
-        dg[r] = 0;
-        if (e < T.end) {
-            kv[r] = key_in[e];
-            idv[r] = pb_in ? pb_in[e] : e;  // first pass: ids are positions
-            dg[r] = (uint32_t)(kv[r] >> sh) & 0xFF;
-            atomicAdd(&run[dg[r]], 1u);
-        }
-    }
-    __syncthreads();
-    // exclusive scan of the digit counts (one digit per thread)
-    const uint32_t mine = dig ? run[tid] : 0;
-    if (dig) scan[tid] = mine;
-    __syncthreads();
-    for (uint32_t o = 1; o < 256; o <<= 1) {
-        const uint32_t v = dig && tid >= o ? scan[tid - o] : 0;
-        __syncthreads();
-        if (dig) scan[tid] += v;
-        __syncthreads();
-    }
-    if (dig) {
-        lstart[tid] = scan[tid] - mine;
-        run[tid] = scan[tid] - mine;
-    }
-    __syncthreads();
-    // stable placement in LDS, in index order (r, wave, lane)
+    // wave w holds records [w kWR, (w + 1) kWR) of the tile, kScI per lane in
+    // (round, lane) order; a record's rank among the wave's earlier records
+    // of its digit comes from ballots (lanes of the round) and the wave's own
+    // running digit counts (earlier rounds): no block barrier between rounds
     const uint64_t lt = lanemask_lt();
+    uint64_t kv[kScI];
+    uint32_t idv[kScI], dg[kScI], off[kScI];
 #pragma unroll
     for (uint32_t r = 0; r < kScI; ++r) {
-        const bool valid = T.begin + r * kScT + tid < T.end;
+        const uint32_t i = w * kWR + r * 64 + lane;
+        const uint32_t e = T.begin + i;
+        const bool valid = e < T.end;
+        kv[r] = valid ? key_in[e] : 0ull;
+        idv[r] = valid ? (pb_in ? pb_in[e] : e) : 0u;  // first pass: ids are positions
+        dg[r] = (uint32_t)(kv[r] >> sh) & 0xFF;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kScI; ++r) {
+        const bool valid = T.begin + w * kWR + r * 64 + lane < T.end;
         uint64_t m = __ballot(valid);
         for (uint32_t b = 0; b < 8; ++b) {
             const bool bit = (dg[r] >> b) & 1;
@@ -757,25 +738,48 @@ __device__ __forceinline__ void scatter_tile(uint32_t tb, const Tile* __restrict
             m &= bit ? bb : ~bb;
         }
         const uint32_t rank = (uint32_t)__popcll(m & lt);
-        if (valid && rank == 0) wcnt[w][dg[r]] = (uint32_t)__popcll(m);
-        __syncthreads();
-        if (valid) {
-            uint32_t q = run[dg[r]] + rank;
-            for (uint32_t ww = 0; ww < w; ++ww) q += wcnt[ww][dg[r]];
+        // every lane reads its digit's count before the digit's first lane
+        // adds the round's (one wave: its LDS accesses keep program order)
+        off[r] = valid ? wcnt[w][dg[r]] + rank : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) wcnt[w][dg[r]] = (uint16_t)(wcnt[w][dg[r]] + __popcll(m));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per digit: the waves' counts become their exclusive prefix, and the
+    // digits' totals an exclusive scan (wave shuffles, then the 4 wave sums)
+    uint32_t tot = 0;
+    if (dig) {
+#pragma unroll
+        for (uint32_t k = 0; k < kW; ++k) {
+            const uint32_t c = wcnt[k][tid];
+            wcnt[k][tid] = (uint16_t)tot;
+            tot += c;
+        }
+    }
+    uint32_t inc = tot;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (dig && lane == 63) wtot[w] = inc;
+    __syncthreads();
+    if (dig) {
+        uint32_t pre = 0;
+        for (uint32_t k = 0; k < w; ++k) pre += wtot[k];
+        lstart[tid] = pre + inc - tot;
+    }
+    __syncthreads();
+    // stable placement in LDS, in index order
+#pragma unroll
+    for (uint32_t r = 0; r < kScI; ++r) {
+        if (T.begin + w * kWR + r * 64 + lane < T.end) {
+            const uint32_t q = lstart[dg[r]] + wcnt[w][dg[r]] + off[r];
             skey[q] = kv[r];
             sid[q] = idv[r];
         }
-        __syncthreads();
-        if (dig) {
-            uint32_t add = 0;
-            for (uint32_t ww = 0; ww < kScT / 64; ++ww) {
-                add += wcnt[ww][tid];
-                wcnt[ww][tid] = 0;
-            }
-            run[tid] += add;
-        }
-        __syncthreads();
     }
+    __syncthreads();
     // runs out: slot k of digit d goes to gbase[d] + (k - lstart[d])
     const uint32_t cnt = T.end - T.begin;
     for (uint32_t k = tid; k < cnt; k += kScT) {
