@@ -767,39 +767,41 @@ k_hp_hist(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv,
 
 // Stable scatter of one tile by the digit (x >> shift) & (2^bits - 1): the
 // first pass reads the candidate list and hashes, the second the first
-// pass's output.  The tile is first reordered by digit in LDS (wave ballots
-// rank equal digits, as in the local sort's scatter) and then written out
-// run by run, so consecutive lanes write consecutive addresses of a digit's
-// run instead of one line per record.
+// pass's output.  The tile is first reordered by digit in LDS and then
+// written out run by run, so consecutive lanes write consecutive addresses
+// of a digit's run instead of one line per record.  Wave w holds the tile's
+// records [w kHpI 64, (w + 1) kHpI 64), kHpI per lane in (round, lane)
+// order; a record's rank among the wave's earlier records of its digit comes
+// from ballots (lanes of one round) and the wave's own running digit counts
+// (earlier rounds), the waves' and digits' offsets from per-digit scans.
 __global__ void __launch_bounds__(kHpT)
 k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv,
              const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ ids_in, uint64_t nh, uint32_t shift,
              uint32_t bits, uint64_t ntiles, const uint64_t* __restrict__ off, uint64_t* __restrict__ keys_out,
              uint32_t* __restrict__ ids_out) {
+    constexpr uint32_t kW = kHpT / 64, kWR = kHpI * 64;
+    constexpr uint32_t kMaxD = 1u << kHpMaxBits;
+    static_assert(kMaxD <= kHpT, "one digit per thread");
     __shared__ uint64_t skey[kHpTile];
     __shared__ uint32_t sid[kHpTile];
-    __shared__ uint64_t gbase[1u << kHpMaxBits];   // the digit's output offset for this tile
-    __shared__ uint32_t lstart[1u << kHpMaxBits];  // the digit's first slot in the reordered tile
-    __shared__ uint32_t run[1u << kHpMaxBits];     // next free slot of the digit
-    __shared__ uint8_t wcnt[kHpT / 64][1u << kHpMaxBits];  // a wave's count of a digit, at most 64
-    __shared__ uint32_t scan[kHpT];
-    const uint32_t nb = 1u << bits, tid = threadIdx.x, w = tid >> 6;
+    __shared__ uint64_t gbase[kMaxD];    // the digit's output offset for this tile
+    __shared__ uint32_t lstart[kMaxD];   // the digit's first slot in the reordered tile
+    __shared__ uint16_t wcnt[kW][kMaxD]; // a wave's count of a digit, then the waves before it
+    __shared__ uint32_t wtot[kW];
+    const uint32_t nb = 1u << bits, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const uint64_t t0 = (uint64_t)blockIdx.x * kHpTile;
-    for (uint32_t d = tid; d < nb; d += kHpT) {
-        gbase[d] = off[(uint64_t)d * ntiles + blockIdx.x];
-        run[d] = 0;
-        for (uint32_t k = 0; k < kHpT / 64; ++k) wcnt[k][d] = 0;
-    }
+    const bool dig = tid < nb;
+    if (dig) gbase[tid] = off[(uint64_t)tid * ntiles + blockIdx.x];
+    for (uint32_t k = tid; k < kW * kMaxD; k += kHpT) (&wcnt[0][0])[k] = 0;
     __syncthreads();
-    // the tile's elements and their digits
+    const uint64_t lt = lanemask_lt();
     uint64_t x[kHpI];
-    uint32_t id[kHpI], dg[kHpI];
+    uint32_t id[kHpI], dg[kHpI], rk[kHpI];
 #pragma unroll
     for (uint32_t r = 0; r < kHpI; ++r) {
-        const uint64_t i = t0 + r * kHpT + tid;
+        const uint64_t i = t0 + w * kWR + r * 64 + lane;
         x[r] = 0;
         id[r] = 0;
-        dg[r] = 0;
         if (i < nh) {
             if (keys_in) {
                 x[r] = keys_in[i];
@@ -808,41 +810,12 @@ k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ has
                 id[r] = list[i];
                 x[r] = mix64(hashv[id[r]]);
             }
-            dg[r] = (uint32_t)(x[r] >> shift) & (nb - 1);
-            atomicAdd(&run[dg[r]], 1u);
         }
+        dg[r] = (uint32_t)(x[r] >> shift) & (nb - 1);
     }
-    __syncthreads();
-    // exclusive scan of the digit counts (two digits per thread at most)
-    const uint32_t per = (nb + kHpT - 1) / kHpT;
-    uint32_t mine = 0;
-    for (uint32_t k = 0; k < per; ++k) {
-        const uint32_t d = tid * per + k;
-        if (d < nb) mine += run[d];
-    }
-    scan[tid] = mine;
-    __syncthreads();
-    for (uint32_t o = 1; o < kHpT; o <<= 1) {
-        const uint32_t v = tid >= o ? scan[tid - o] : 0;
-        __syncthreads();
-        scan[tid] += v;
-        __syncthreads();
-    }
-    uint32_t acc = scan[tid] - mine;
-    for (uint32_t k = 0; k < per; ++k) {
-        const uint32_t d = tid * per + k;
-        if (d < nb) {
-            const uint32_t c = run[d];
-            lstart[d] = acc;
-            run[d] = acc;
-            acc += c;
-        }
-    }
-    __syncthreads();
-    // stable placement in LDS, in index order (r, wave, lane)
-    const uint64_t lt = lanemask_lt();
+#pragma unroll
     for (uint32_t r = 0; r < kHpI; ++r) {
-        const bool valid = t0 + r * kHpT + tid < nh;
+        const bool valid = t0 + w * kWR + r * 64 + lane < nh;
         uint64_t m = __ballot(valid);
         for (uint32_t b = 0; b < bits; ++b) {
             const bool bit = (dg[r] >> b) & 1;
@@ -850,25 +823,47 @@ k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ has
             m &= bit ? bb : ~bb;
         }
         const uint32_t rank = (uint32_t)__popcll(m & lt);
-        if (valid && rank == 0) wcnt[w][dg[r]] = (uint8_t)__popcll(m);
-        __syncthreads();
-        if (valid) {
-            uint32_t q = run[dg[r]] + rank;
-            for (uint32_t ww = 0; ww < w; ++ww) q += wcnt[ww][dg[r]];
+        // every lane reads its digit's count before the digit's first lane
+        // adds the round's (one wave: its LDS accesses keep program order)
+        rk[r] = valid ? wcnt[w][dg[r]] + rank : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) wcnt[w][dg[r]] = (uint16_t)(wcnt[w][dg[r]] + __popcll(m));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per digit: the waves' counts become their exclusive prefix, and the
+    // digits' totals an exclusive scan (wave shuffles, then the wave sums)
+    uint32_t tot = 0;
+    if (dig) {
+#pragma unroll
+        for (uint32_t k = 0; k < kW; ++k) {
+            const uint32_t c = wcnt[k][tid];
+            wcnt[k][tid] = (uint16_t)tot;
+            tot += c;
+        }
+    }
+    uint32_t inc = tot;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wtot[w] = inc;
+    __syncthreads();
+    if (dig) {
+        uint32_t pre = 0;
+        for (uint32_t k = 0; k < w; ++k) pre += wtot[k];
+        lstart[tid] = pre + inc - tot;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kHpI; ++r) {
+        if (t0 + w * kWR + r * 64 + lane < nh) {
+            const uint32_t q = lstart[dg[r]] + wcnt[w][dg[r]] + rk[r];
             skey[q] = x[r];
             sid[q] = id[r];
         }
-        __syncthreads();
-        for (uint32_t d = tid; d < nb; d += kHpT) {
-            uint32_t add = 0;
-            for (uint32_t k = 0; k < kHpT / 64; ++k) {
-                add += wcnt[k][d];
-                wcnt[k][d] = 0;
-            }
-            run[d] += add;
-        }
-        __syncthreads();
     }
+    __syncthreads();
     // runs out: slot k of digit d goes to gbase[d] + (k - lstart[d])
     const uint32_t cnt = (uint32_t)min<uint64_t>(kHpTile, nh - t0);
     for (uint32_t k = tid; k < cnt; k += kHpT) {
