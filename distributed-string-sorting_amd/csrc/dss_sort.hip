@@ -1234,6 +1234,10 @@ __device__ __forceinline__ void small_sort_wave_unit(uint64_t wave, const SmallS
     uint32_t rd = depth0;
     uint32_t rw = cache ? 8 - (cache & 7) : 8;
     bool act = valid && j == 0 && scnt >= 2;
+    // until the first network has run every slot is one group of its strings
+    // (padding lanes after them, key ~0 and a larger source lane), so the
+    // first network orders (key, source lane) and leaves G in place
+    bool one_group = true;
     for (;;) {
         const uint64_t leaders = __ballot(lane == G);
         const uint32_t grd = __shfl(rd, G);
@@ -1270,19 +1274,36 @@ __device__ __forceinline__ void small_sort_wave_unit(uint64_t wave, const SmallS
             continue;
         }
         // bitonic sort of (G, K, src) inside every aligned CLS-lane slot
-        for (uint32_t k = 2; k <= CLS; k <<= 1) {
-            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-                const uint32_t pG = __shfl_xor(G, jj);
-                const uint64_t pK = __shfl_xor((unsigned long long)K, jj);
-                const uint32_t pS = __shfl_xor(src, jj);
-                const bool asc = (k == CLS) || ((lane & k) == 0);
-                const bool lower = (lane & jj) == 0;
-                const bool pless = pG < G || (pG == G && (pK < K || (pK == K && pS < src)));
-                // the lower lane keeps the smaller element when ascending
-                if (lower == asc ? pless : !pless) {
-                    G = pG;
-                    K = pK;
-                    src = pS;
+        if (one_group) {
+            for (uint32_t k = 2; k <= CLS; k <<= 1) {
+                for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+                    const uint64_t pK = __shfl_xor((unsigned long long)K, jj);
+                    const uint32_t pS = __shfl_xor(src, jj);
+                    const bool asc = (k == CLS) || ((lane & k) == 0);
+                    const bool lower = (lane & jj) == 0;
+                    const bool pless = pK < K || (pK == K && pS < src);
+                    if (lower == asc ? pless : !pless) {
+                        K = pK;
+                        src = pS;
+                    }
+                }
+            }
+            one_group = false;
+        } else {
+            for (uint32_t k = 2; k <= CLS; k <<= 1) {
+                for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+                    const uint32_t pG = __shfl_xor(G, jj);
+                    const uint64_t pK = __shfl_xor((unsigned long long)K, jj);
+                    const uint32_t pS = __shfl_xor(src, jj);
+                    const bool asc = (k == CLS) || ((lane & k) == 0);
+                    const bool lower = (lane & jj) == 0;
+                    const bool pless = pG < G || (pG == G && (pK < K || (pK == K && pS < src)));
+                    // the lower lane keeps the smaller element when ascending
+                    if (lower == asc ? pless : !pless) {
+                        G = pG;
+                        K = pK;
+                        src = pS;
+                    }
                 }
             }
         }
