@@ -1187,6 +1187,50 @@ __global__ void k_pack_offlen(const uint64_t* __restrict__ off, const uint32_t* 
         out[i] = OffLen{off[i], len[i]};
 }
 
+// Bitonic sort of (G, K, src) inside every aligned kCls-lane slot of the
+// wave, the network unrolled (constant lane masks).  With one_group every
+// slot is one group of strings followed by padding lanes (key ~0, larger
+// source lanes), so (K, src) decides and G stays in place.
+template <uint32_t kCls>
+__device__ __forceinline__ void slot_network(uint32_t lane, bool one_group, uint32_t& G, uint64_t& K, uint32_t& src) {
+    if (one_group) {
+#pragma unroll
+        for (uint32_t k = 2; k <= kCls; k <<= 1) {
+#pragma unroll
+            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+                const uint64_t pK = __shfl_xor((unsigned long long)K, jj);
+                const uint32_t pS = __shfl_xor(src, jj);
+                const bool asc = (k == kCls) || ((lane & k) == 0);
+                const bool lower = (lane & jj) == 0;
+                const bool pless = pK < K || (pK == K && pS < src);
+                if (lower == asc ? pless : !pless) {
+                    K = pK;
+                    src = pS;
+                }
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (uint32_t k = 2; k <= kCls; k <<= 1) {
+#pragma unroll
+        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+            const uint32_t pG = __shfl_xor(G, jj);
+            const uint64_t pK = __shfl_xor((unsigned long long)K, jj);
+            const uint32_t pS = __shfl_xor(src, jj);
+            const bool asc = (k == kCls) || ((lane & k) == 0);
+            const bool lower = (lane & jj) == 0;
+            const bool pless = pG < G || (pG == G && (pK < K || (pK == K && pS < src)));
+            // the lower lane keeps the smaller element when ascending
+            if (lower == asc ? pless : !pless) {
+                G = pG;
+                K = pK;
+                src = pS;
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ void small_sort_wave_unit(uint64_t wave, const SmallSeg* __restrict__ segs, uint32_t nseg,
                                                      uint32_t cls_log2, uint32_t* __restrict__ perm,
                                                      uint32_t* __restrict__ lcp, const uint8_t* __restrict__ chars,
@@ -1274,39 +1318,15 @@ __device__ __forceinline__ void small_sort_wave_unit(uint64_t wave, const SmallS
             continue;
         }
         // bitonic sort of (G, K, src) inside every aligned CLS-lane slot
-        if (one_group) {
-            for (uint32_t k = 2; k <= CLS; k <<= 1) {
-                for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-                    const uint64_t pK = __shfl_xor((unsigned long long)K, jj);
-                    const uint32_t pS = __shfl_xor(src, jj);
-                    const bool asc = (k == CLS) || ((lane & k) == 0);
-                    const bool lower = (lane & jj) == 0;
-                    const bool pless = pK < K || (pK == K && pS < src);
-                    if (lower == asc ? pless : !pless) {
-                        K = pK;
-                        src = pS;
-                    }
-                }
-            }
-            one_group = false;
-        } else {
-            for (uint32_t k = 2; k <= CLS; k <<= 1) {
-                for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-                    const uint32_t pG = __shfl_xor(G, jj);
-                    const uint64_t pK = __shfl_xor((unsigned long long)K, jj);
-                    const uint32_t pS = __shfl_xor(src, jj);
-                    const bool asc = (k == CLS) || ((lane & k) == 0);
-                    const bool lower = (lane & jj) == 0;
-                    const bool pless = pG < G || (pG == G && (pK < K || (pK == K && pS < src)));
-                    // the lower lane keeps the smaller element when ascending
-                    if (lower == asc ? pless : !pless) {
-                        G = pG;
-                        K = pK;
-                        src = pS;
-                    }
-                }
-            }
+        switch (cls_log2) {
+            case 1: slot_network<2>(lane, one_group, G, K, src); break;
+            case 2: slot_network<4>(lane, one_group, G, K, src); break;
+            case 3: slot_network<8>(lane, one_group, G, K, src); break;
+            case 4: slot_network<16>(lane, one_group, G, K, src); break;
+            case 5: slot_network<32>(lane, one_group, G, K, src); break;
+            default: slot_network<64>(lane, one_group, G, K, src); break;
         }
+        one_group = false;
         // boundaries and LCPs inside active groups
         const uint64_t pK = __shfl_up((unsigned long long)K, 1);
         bool flag = lane == G;
