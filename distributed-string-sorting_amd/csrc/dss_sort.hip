@@ -1187,48 +1187,58 @@ __global__ void k_pack_offlen(const uint64_t* __restrict__ off, const uint32_t* 
         out[i] = OffLen{off[i], len[i]};
 }
 
+// The value of lane (lane ^ kMask): masks 1 and 2 as DPP quad permutations,
+// 4 to 16 as LDS swizzles (no address operand), 32 by a permute.
+template <uint32_t kMask>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+    if constexpr (kMask == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (kMask == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (kMask < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (kMask << 10));
+    else return (uint32_t)__shfl_xor((int)v, (int)kMask);
+}
+template <uint32_t kMask>
+__device__ __forceinline__ uint64_t xor_lane64(uint64_t v) {
+    return (uint64_t)xor_lane<kMask>((uint32_t)v) | ((uint64_t)xor_lane<kMask>((uint32_t)(v >> 32)) << 32);
+}
+
+// One compare-exchange step of the network: partner lane ^ kJ, blocks of kK.
+template <uint32_t kCls, uint32_t kK, uint32_t kJ, bool kOneGroup>
+__device__ __forceinline__ void slot_step(uint32_t lane, uint32_t& G, uint64_t& K, uint32_t& src) {
+    const uint64_t pK = xor_lane64<kJ>(K);
+    const uint32_t pS = xor_lane<kJ>(src);
+    const uint32_t pG = kOneGroup ? G : xor_lane<kJ>(G);
+    const bool asc = (kK == kCls) || ((lane & kK) == 0);
+    const bool lower = (lane & kJ) == 0;
+    const bool pless = kOneGroup ? (pK < K || (pK == K && pS < src))
+                                 : (pG < G || (pG == G && (pK < K || (pK == K && pS < src))));
+    // the lower lane keeps the smaller element when ascending
+    if (lower == asc ? pless : !pless) {
+        G = pG;
+        K = pK;
+        src = pS;
+    }
+}
+
+template <uint32_t kCls, uint32_t kK, uint32_t kJ, bool kOneGroup>
+__device__ __forceinline__ void slot_merge(uint32_t lane, uint32_t& G, uint64_t& K, uint32_t& src) {
+    slot_step<kCls, kK, kJ, kOneGroup>(lane, G, K, src);
+    if constexpr (kJ > 1) slot_merge<kCls, kK, kJ / 2, kOneGroup>(lane, G, K, src);
+}
+
+template <uint32_t kCls, uint32_t kK, bool kOneGroup>
+__device__ __forceinline__ void slot_sort(uint32_t lane, uint32_t& G, uint64_t& K, uint32_t& src) {
+    slot_merge<kCls, kK, kK / 2, kOneGroup>(lane, G, K, src);
+    if constexpr (kK < kCls) slot_sort<kCls, kK * 2, kOneGroup>(lane, G, K, src);
+}
+
 // Bitonic sort of (G, K, src) inside every aligned kCls-lane slot of the
 // wave, the network unrolled (constant lane masks).  With one_group every
 // slot is one group of strings followed by padding lanes (key ~0, larger
 // source lanes), so (K, src) decides and G stays in place.
 template <uint32_t kCls>
 __device__ __forceinline__ void slot_network(uint32_t lane, bool one_group, uint32_t& G, uint64_t& K, uint32_t& src) {
-    if (one_group) {
-#pragma unroll
-        for (uint32_t k = 2; k <= kCls; k <<= 1) {
-#pragma unroll
-            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-                const uint64_t pK = __shfl_xor((unsigned long long)K, jj);
-                const uint32_t pS = __shfl_xor(src, jj);
-                const bool asc = (k == kCls) || ((lane & k) == 0);
-                const bool lower = (lane & jj) == 0;
-                const bool pless = pK < K || (pK == K && pS < src);
-                if (lower == asc ? pless : !pless) {
-                    K = pK;
-                    src = pS;
-                }
-            }
-        }
-        return;
-    }
-#pragma unroll
-    for (uint32_t k = 2; k <= kCls; k <<= 1) {
-#pragma unroll
-        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-            const uint32_t pG = __shfl_xor(G, jj);
-            const uint64_t pK = __shfl_xor((unsigned long long)K, jj);
-            const uint32_t pS = __shfl_xor(src, jj);
-            const bool asc = (k == kCls) || ((lane & k) == 0);
-            const bool lower = (lane & jj) == 0;
-            const bool pless = pG < G || (pG == G && (pK < K || (pK == K && pS < src)));
-            // the lower lane keeps the smaller element when ascending
-            if (lower == asc ? pless : !pless) {
-                G = pG;
-                K = pK;
-                src = pS;
-            }
-        }
-    }
+    if (one_group) slot_sort<kCls, 2, true>(lane, G, K, src);
+    else slot_sort<kCls, 2, false>(lane, G, K, src);
 }
 
 __device__ __forceinline__ void small_sort_wave_unit(uint64_t wave, const SmallSeg* __restrict__ segs, uint32_t nseg,
