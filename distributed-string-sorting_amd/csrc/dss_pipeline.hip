@@ -1413,46 +1413,57 @@ __device__ inline uint32_t pair_of_tile(const MergePair* __restrict__ pairs, uin
 
 // Merge-path split (number of A elements among the first d outputs, A wins
 // ties) at the start of every tile, from global memory.  ord == nullptr: the
-// first round, whose order is the receive order.  Four lanes per tile search
-// 4-ary: lanes 0-2 each test one quartile point of the interval per step, so
-// a split takes about log4 of the run length dependent steps.
+// first round, whose order is the receive order.  kSL lanes per tile search
+// kSL-ary: lanes 0..kSL-2 each test one of the interval's kSL-1 inner points
+// per step, so a split takes about log_kSL of the run length dependent steps
+// (a string comparison that the cached keys leave open walks the strings in
+// its lane, so fewer steps matter most where runs share long prefixes).
+constexpr uint32_t kSL = 8;
+
 template <uint32_t kMW>
 __global__ void k_merge_split(const MergePair* __restrict__ pairs, uint32_t npairs, uint64_t ntiles,
                               const uint32_t* __restrict__ ord, const uint8_t* __restrict__ chars,
                               const uint64_t* __restrict__ voff, const uint32_t* __restrict__ vlen,
                               const uint64_t* __restrict__ keyc, const uint64_t* __restrict__ keyc2,
                               const uint32_t* __restrict__ rcp, uint64_t* __restrict__ split) {
-    const uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;  // uniform over the 4 lanes
-    if (t >= ntiles) return;
-    const uint32_t l = threadIdx.x & 3, g0 = threadIdx.x & ~3u;
-    const MergePair Pr = pairs[pair_of_tile(pairs, npairs, t)];
-    const ViewCtx V{chars, voff, vlen, keyc2, rcp[Pr.recv]};
-    const uint64_t A = Pr.out, B = Pr.out + Pr.na;
-    constexpr uint32_t kTileOut = kMT * kMW;
-    const uint64_t d = (t - Pr.tile0) * kTileOut;
+    const uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kSL;  // uniform over the group
+    const uint32_t l = threadIdx.x % kSL, g0 = (threadIdx.x & 63) & ~(kSL - 1);
+    const bool live = t < ntiles;
+    uint64_t lo = 0, hi = 0, A = 0, B = 0, d = 0;
+    ViewCtx V{chars, voff, vlen, keyc2, 0};
+    if (live) {
+        const MergePair Pr = pairs[pair_of_tile(pairs, npairs, t)];
+        V.cp = rcp[Pr.recv];
+        A = Pr.out;
+        B = Pr.out + Pr.na;
+        constexpr uint32_t kTileOut = kMT * kMW;
+        d = (t - Pr.tile0) * kTileOut;
+        lo = d > Pr.nb ? d - Pr.nb : 0;
+        hi = d < Pr.na ? d : Pr.na;
+    }
     // the answer is the first mid in [lo, hi] with A[mid] > B[d - mid - 1]
     // (hi if none); the test is monotone in mid, so the probes that hold form
-    // a prefix of lanes 0-2
-    uint64_t lo = d > Pr.nb ? d - Pr.nb : 0, hi = d < Pr.na ? d : Pr.na;
-    while (lo < hi) {  // uniform over the 4 lanes (same lo, hi)
+    // a prefix of lanes 0..kSL-2
+    while (__ballot(lo < hi)) {  // every lane of the wave takes part in the ballot
         const uint64_t span = hi - lo;
-        const uint64_t m = lo + span * (l + 1) / 4;  // non-decreasing in l, < hi for l < 3
+        const uint64_t m = lo + span * (l + 1) / kSL;  // non-decreasing in l, < hi for l < kSL - 1
         bool le = false;
-        if (l < 3) {
+        if (lo < hi && l < kSL - 1) {
             const uint64_t ia = A + m, ib = B + d - m - 1;
             const uint32_t a = ord ? ord[ia] : (uint32_t)ia, b = ord ? ord[ib] : (uint32_t)ib;
             uint32_t x;
             le = elem_cmp(V, a, keyc[a], b, keyc[b], V.cp, x) <= 0;
         }
-        const uint32_t c = (uint32_t)__shfl((int)le, g0 + 0) + (uint32_t)__shfl((int)le, g0 + 1) +
-                           (uint32_t)__shfl((int)le, g0 + 2);
-        const uint64_t m0 = lo + span / 4, m1 = lo + span * 2 / 4, m2 = lo + span * 3 / 4;
-        const uint64_t mc = c == 0 ? m0 : (c == 1 ? m1 : m2);  // the first probe that failed (c < 3)
-        const uint64_t ml = c == 1 ? m0 : (c == 2 ? m1 : m2);  // the last probe that held (c > 0)
-        if (c > 0) lo = ml + 1;
-        if (c < 3) hi = mc;
+        const uint64_t held = (__ballot(le) >> g0) & ((1ull << (kSL - 1)) - 1);
+        const uint32_t c = (uint32_t)__popcll(held);  // probes that held
+        if (lo < hi) {
+            const uint64_t mc = lo + span * (c + 1) / kSL;  // the first probe that failed (c < kSL - 1)
+            const uint64_t ml = lo + span * c / kSL;        // the last probe that held (c > 0)
+            if (c > 0) lo = ml + 1;
+            if (c < kSL - 1) hi = mc;
+        }
     }
-    if (l == 0) split[t] = lo;
+    if (live && l == 0) split[t] = lo;
 }
 
 // A tile's strings in LDS: slot x holds the receive index o[x], the LCP l[x]
@@ -1843,7 +1854,7 @@ void merge(dss_set_s* s) {
                 const bool last = r + 2 == rbeg.size();
                 KScope ks(s->timer, st, "merge", n * 24.0);
                 auto launch = [&](auto split_kern, auto tile_kern) {
-                    hipLaunchKernelGGL(split_kern, dim3((tiles * 4 + 255) / 256), dim3(256), 0, st, rp, np, tiles,
+                    hipLaunchKernelGGL(split_kern, dim3((tiles * kSL + 255) / 256), dim3(256), 0, st, rp, np, tiles,
                                        in_ord, s->arena(), P.recv_view_off.as<uint64_t>(),
                                        P.recv_view_len.as<uint32_t>(), P.m_keyc.as<uint64_t>(),
                                        P.m_keyc2.as<uint64_t>(), cp, split);
