@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -290,7 +291,9 @@ k_pd_mark(const uint8_t* __restrict__ cand, const uint32_t* __restrict__ lcp, co
 // chunks of one string); lane j of a group then accumulates XXH64 lane v_j+1
 // over the window's 32-byte stripes and lane 0 merges and hashes the tail.
 // All loops are block-uniform (the hashed length is the round's depth), so
-// every shuffle runs with all lanes active.
+// every shuffle runs with all lanes active.  Hashes are stored in list order
+// (these rounds list few positions: later passes then read them as a
+// stream).
 __global__ void __launch_bounds__(256)
 k_pd_hash(const uint32_t* __restrict__ list, const unsigned long long* __restrict__ nhashed,
           const uint64_t* __restrict__ s_off, const uint8_t* __restrict__ chars, uint32_t depth, uint64_t m,
@@ -375,7 +378,7 @@ k_pd_hash(const uint32_t* __restrict__ list, const unsigned long long* __restric
             h ^= h >> 29;
             h *= P3;
             h ^= h >> 32;
-            hashv[c] = mod_range(h, m);
+            hashv[i] = mod_range(h, m);  // in list order
         }
         __syncthreads();  // rows and s_so are rewritten by the next iteration
     }
@@ -549,7 +552,7 @@ __device__ inline uint64_t xxh64_digest(uint64_t v1, uint64_t v2, uint64_t v3, u
 // any other position continues from its head's state after L stripes.  Lane
 // j loads word j of every stripe it hashes and word j of the tail up front,
 // so a string waits for its list entry, its offset and rank, then its words
-// and head state.
+// and head state.  Hashes are stored by position.
 constexpr uint32_t kHashU = 2;
 
 __global__ void __launch_bounds__(256)
@@ -638,25 +641,28 @@ __device__ inline uint64_t slot_of(uint64_t h, uint64_t mask) {
     return (h >> 17) & mask;
 }
 
-__global__ void k_pd_insert(const uint32_t* __restrict__ list, uint64_t nh, const uint64_t* __restrict__ hashv,
-                            uint64_t* __restrict__ keys, uint8_t* __restrict__ dup, uint64_t mask,
-                            uint32_t* __restrict__ slot) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nh) return;
-    const uint32_t c = list[i];
-    const uint64_t h = hashv[c];
+// The exact table takes the nm candidates listed in `maybe`: entry k names
+// key keyv[maybe[k]] (keyv[kidx[maybe[k]]] when kidx is given), and slot[k]
+// receives that key's slot.  `empty` is a value no key takes.
+__global__ void k_pd_insert(const uint32_t* __restrict__ maybe, uint64_t nm, const uint64_t* __restrict__ keyv,
+                            const uint32_t* __restrict__ kidx, uint64_t empty, uint64_t* __restrict__ keys,
+                            uint8_t* __restrict__ dup, uint64_t mask, uint32_t* __restrict__ slot) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    const uint32_t e = maybe[k];
+    const uint64_t h = keyv[kidx ? kidx[e] : e];
     uint64_t s = slot_of(h, mask);
     for (uint64_t probe = 0; probe <= mask; ++probe) {
         const unsigned long long old =
-            atomicCAS(reinterpret_cast<unsigned long long*>(&keys[s]), (unsigned long long)kEmpty, (unsigned long long)h);
-        if (old == kEmpty) break;
+            atomicCAS(reinterpret_cast<unsigned long long*>(&keys[s]), (unsigned long long)empty, (unsigned long long)h);
+        if (old == empty) break;
         if (old == h) {
             dup[s] = 1;
             break;
         }
         s = (s + 1) & mask;
     }
-    slot[c] = (uint32_t)s;
+    slot[k] = (uint32_t)s;
 }
 
 // ------------------------------------------------ duplicate prefilter
@@ -668,19 +674,21 @@ __global__ void k_pd_insert(const uint32_t* __restrict__ list, uint64_t nh, cons
 __device__ inline uint64_t pd_bit(uint64_t h, uint32_t bits) { return mix64(h) >> (64 - bits); }
 
 __global__ void __launch_bounds__(256)
-k_pd_bits(const uint32_t* __restrict__ list, uint64_t nh, const uint64_t* __restrict__ hashv, uint32_t bits,
-          uint32_t* __restrict__ seen, uint32_t* __restrict__ again) {
+k_pd_bits(const uint32_t* __restrict__ list, bool by_pos, uint64_t nh, const uint64_t* __restrict__ hashv,
+          uint32_t bits, uint32_t* __restrict__ seen, uint32_t* __restrict__ again) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t x = pd_bit(hashv[list[i]], bits);
+        const uint64_t x = pd_bit(hashv[by_pos ? list[i] : i], bits);
         const uint32_t b = 1u << (x & 31);
         if (atomicOr(&seen[x >> 5], b) & b) atomicOr(&again[x >> 5], b);
     }
 }
 
-// Unique hashes get dup = 0; the others are appended to `maybe` (collected
-// in LDS over kMarkIters iterations, one global atomic per flush).
+// Unique hashes get dup = 0; the others' list indices are appended to
+// `maybe` (collected in LDS over kMarkIters iterations, one global atomic
+// per flush).
 __global__ void __launch_bounds__(256)
-k_pd_maybe(const uint32_t* __restrict__ list, uint64_t nh, const uint64_t* __restrict__ hashv, uint32_t bits,
+k_pd_maybe(const uint32_t* __restrict__ list, bool by_pos, uint64_t nh, const uint64_t* __restrict__ hashv,
+           uint32_t bits,
            const uint32_t* __restrict__ again, uint8_t* __restrict__ dup, uint32_t* __restrict__ maybe,
            unsigned long long* __restrict__ nmaybe) {
     __shared__ uint32_t sl[256 * kMarkIters];
@@ -693,10 +701,9 @@ k_pd_maybe(const uint32_t* __restrict__ list, uint64_t nh, const uint64_t* __res
     for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < nh; b0 += stride) {
         const uint64_t i = b0 + threadIdx.x;
         bool m = false;
-        uint32_t c = 0;
         if (i < nh) {
-            c = list[i];
-            const uint64_t x = pd_bit(hashv[c], bits);
+            const uint32_t c = list[i];
+            const uint64_t x = pd_bit(hashv[by_pos ? c : (uint32_t)i], bits);
             m = (again[x >> 5] >> (x & 31)) & 1;
             if (!m) dup[c] = 0;
         }
@@ -704,7 +711,7 @@ k_pd_maybe(const uint32_t* __restrict__ list, uint64_t nh, const uint64_t* __res
         uint32_t wb = 0;
         if (lane == 0 && bal) wb = atomicAdd(&scount, (uint32_t)__popcll(bal));
         wb = __shfl(wb, 0);
-        if (m) sl[wb + (uint32_t)__popcll(bal & lanemask_lt())] = c;
+        if (m) sl[wb + (uint32_t)__popcll(bal & lanemask_lt())] = (uint32_t)i;
         if (++it == kMarkIters || b0 + stride >= nh) {  // block-uniform
             it = 0;
             __syncthreads();
@@ -719,14 +726,13 @@ k_pd_maybe(const uint32_t* __restrict__ list, uint64_t nh, const uint64_t* __res
     }
 }
 
-// Exact result for the prefiltered hashes: their table slot's flag.
-__global__ void k_pd_flag(const uint32_t* __restrict__ maybe, uint64_t nm, const uint32_t* __restrict__ slot,
-                          const uint8_t* __restrict__ tdup, uint8_t* __restrict__ dup) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nm) {
-        const uint32_t c = maybe[i];
-        dup[c] = tdup[slot[c]];
-    }
+// Exact result for the table's candidates: their slot's flag, written at
+// position pos[maybe[k]].
+__global__ void k_pd_flag(const uint32_t* __restrict__ maybe, uint64_t nm, const uint32_t* __restrict__ pos,
+                          const uint32_t* __restrict__ slot, const uint8_t* __restrict__ tdup,
+                          uint8_t* __restrict__ dup) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nm) dup[pos[maybe[k]]] = tdup[slot[k]];
 }
 
 // ------------------------------------------------ partitioned duplicates
@@ -738,8 +744,9 @@ __global__ void k_pd_flag(const uint32_t* __restrict__ maybe, uint64_t nm, const
 // nh / 2^kb candidates.  A block per bucket then inserts the bucket's hashes
 // into an LDS open-addressing table, flags slots that receive a hash twice,
 // and marks the candidates of flagged slots as duplicates.  A bucket with too
-// many distinct hashes for the table hands its candidates to the exact
-// global table instead.  Every access is a streamed read or write except the
+// many distinct hashes for the table hands its records (indices into the
+// partitioned keys) to the exact global table instead, which then takes x as
+// the key.  Every access is a streamed read or write except the
 // duplicate flags, which only duplicates write.
 constexpr uint32_t kHpT = 512, kHpI = 4, kHpTile = kHpT * kHpI;
 constexpr uint32_t kHpMaxBits = 9;     // digit bits per pass
@@ -747,7 +754,8 @@ constexpr uint32_t kHpCap = 2048;      // LDS table slots per bucket
 constexpr uint64_t kHpMin = 1u << 20;  // hashed candidates from which rounds partition
 
 __global__ void __launch_bounds__(kHpT)
-k_hp_hist(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv, const uint64_t* __restrict__ keys,
+k_hp_hist(const uint32_t* __restrict__ list, bool by_pos, const uint64_t* __restrict__ hashv,
+          const uint64_t* __restrict__ keys,
           uint64_t nh, uint32_t shift, uint32_t bits, uint64_t ntiles, uint64_t* __restrict__ hist) {
     __shared__ uint32_t h[1u << kHpMaxBits];
     const uint32_t nb = 1u << bits, tid = threadIdx.x;
@@ -757,7 +765,7 @@ k_hp_hist(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv,
     for (uint32_t r = 0; r < kHpI; ++r) {
         const uint64_t i = t0 + r * kHpT + tid;
         if (i < nh) {
-            const uint64_t x = keys ? keys[i] : mix64(hashv[list[i]]);
+            const uint64_t x = keys ? keys[i] : mix64(hashv[by_pos ? list[i] : (uint32_t)i]);
             atomicAdd(&h[(uint32_t)(x >> shift) & (nb - 1)], 1u);
         }
     }
@@ -766,8 +774,8 @@ k_hp_hist(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv,
 }
 
 // Stable scatter of one tile by the digit (x >> shift) & (2^bits - 1): the
-// first pass reads the candidate list and hashes, the second the first
-// pass's output.  The tile is first reordered by digit in LDS and then
+// first pass reads the candidate list and hashes (a record's id is its
+// position), the second the first pass's output.  The tile is first reordered by digit in LDS and then
 // written out run by run, so consecutive lanes write consecutive addresses
 // of a digit's run instead of one line per record.  Wave w holds the tile's
 // records [w kHpI 64, (w + 1) kHpI 64), kHpI per lane in (round, lane)
@@ -775,7 +783,7 @@ k_hp_hist(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv,
 // from ballots (lanes of one round) and the wave's own running digit counts
 // (earlier rounds), the waves' and digits' offsets from per-digit scans.
 __global__ void __launch_bounds__(kHpT)
-k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ hashv,
+k_hp_scatter(const uint32_t* __restrict__ list, bool by_pos, const uint64_t* __restrict__ hashv,
              const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ ids_in, uint64_t nh, uint32_t shift,
              uint32_t bits, uint64_t ntiles, const uint64_t* __restrict__ off, uint64_t* __restrict__ keys_out,
              uint32_t* __restrict__ ids_out) {
@@ -808,7 +816,7 @@ k_hp_scatter(const uint32_t* __restrict__ list, const uint64_t* __restrict__ has
                 id[r] = ids_in[i];
             } else {
                 id[r] = list[i];
-                x[r] = mix64(hashv[id[r]]);
+                x[r] = mix64(hashv[by_pos ? id[r] : (uint32_t)i]);
             }
         }
         dg[r] = (uint32_t)(x[r] >> shift) & (nb - 1);
@@ -935,7 +943,7 @@ k_hp_dedup(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ ids, 
         if (over) {
             if (tid == 0) mbase = (uint32_t)atomicAdd(nmaybe, (unsigned long long)(e - s));
             __syncthreads();
-            for (uint32_t i = s + tid; i < e; i += 256) maybe[mbase + (i - s)] = ids[i];
+            for (uint32_t i = s + tid; i < e; i += 256) maybe[mbase + (i - s)] = i;  // key index
         } else {
             for (uint32_t i = s + tid; i < e; i += 256) {
                 const unsigned long long x = keys[i];
@@ -1000,9 +1008,11 @@ k_pd_next(const uint8_t* __restrict__ cand, const uint8_t* __restrict__ flags, c
 
 // Duplicate flags of a round's nh hashed candidates by partitioning (see
 // k_hp_dedup): dup is cleared and set for the repeats found in LDS; the
-// candidates of buckets with more than table_limit distinct hashes are
-// listed in pd_maybe for the exact table.  Returns their number.
-static uint64_t partition_duplicates(dss_set_s* s, uint64_t nh, uint32_t table_limit) {
+// records of buckets with more than table_limit distinct hashes are listed
+// in pd_maybe (indices into the partitioned keys and positions) for the
+// exact table.  Returns their number.  by_pos: hashes are stored by
+// position, else in list order.
+static uint64_t partition_duplicates(dss_set_s* s, uint64_t nh, uint32_t table_limit, bool by_pos) {
     PipeState& P = s->pp;
     const hipStream_t st = s->stream;
     unsigned long long* ctr = P.pd_ctr.as<unsigned long long>();
@@ -1030,15 +1040,16 @@ static uint64_t partition_duplicates(dss_set_s* s, uint64_t nh, uint32_t table_l
         // (read and write) and the second histogram's key read
         KScope ks(s->timer, st, "pd_partition", (double)nh * 68.0);
         const uint32_t sh0 = 64 - kb, sh1 = 64 - b1;
-        hipLaunchKernelGGL(k_hp_hist, dim3(ntiles), dim3(kHpT), 0, st, list, hash, nullptr, nh, sh0, b0, ntiles, hist);
+        hipLaunchKernelGGL(k_hp_hist, dim3(ntiles), dim3(kHpT), 0, st, list, by_pos, hash, nullptr, nh, sh0, b0, ntiles,
+                           hist);
         exclusive_scan_u64(hist, hist, (1ull << b0) * ntiles, st, P, nullptr);
-        hipLaunchKernelGGL(k_hp_scatter, dim3(ntiles), dim3(kHpT), 0, st, list, hash, nullptr, nullptr, nh, sh0, b0,
-                           ntiles, hist, kA, iA);
-        hipLaunchKernelGGL(k_hp_hist, dim3(ntiles), dim3(kHpT), 0, st, nullptr, nullptr, kA, nh, sh1, b1, ntiles,
+        hipLaunchKernelGGL(k_hp_scatter, dim3(ntiles), dim3(kHpT), 0, st, list, by_pos, hash, nullptr, nullptr, nh,
+                           sh0, b0, ntiles, hist, kA, iA);
+        hipLaunchKernelGGL(k_hp_hist, dim3(ntiles), dim3(kHpT), 0, st, nullptr, false, nullptr, kA, nh, sh1, b1, ntiles,
                            hist);
         exclusive_scan_u64(hist, hist, (1ull << b1) * ntiles, st, P, nullptr);
-        hipLaunchKernelGGL(k_hp_scatter, dim3(ntiles), dim3(kHpT), 0, st, nullptr, nullptr, kA, iA, nh, sh1, b1,
-                           ntiles, hist, kB, iB);
+        hipLaunchKernelGGL(k_hp_scatter, dim3(ntiles), dim3(kHpT), 0, st, nullptr, false, nullptr, kA, iA, nh, sh1,
+                           b1, ntiles, hist, kB, iB);
     }
     {
         // per candidate: bucket bounds (key read), table insert and lookup
@@ -1121,6 +1132,7 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
     // partitioned path and its exact-table fallback on small inputs)
     uint64_t part_min = kHpMin;
     uint32_t table_limit = kHpCap / 8 * 7;
+    const bool trace = getenv("DSS_PD_TRACE") != nullptr;  // per-round counts to stderr
     if (const char* e = getenv("DSS_PD_PARTITION_MIN")) part_min = std::max<uint64_t>(strtoull(e, nullptr, 10), 1);
     if (const char* e = getenv("DSS_PD_TABLE_LIMIT"))
         table_limit = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(strtoull(e, nullptr, 10), 1), kHpCap / 8 * 7);
@@ -1189,12 +1201,30 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
                 hipLaunchKernelGGL(k_pd_hash, g, blk, 0, st, P.pd_list.as<uint32_t>(), ctr, s->s_off.as<uint64_t>(),
                                    s->arena(), (uint32_t)depth, m, P.pd_hash.as<uint64_t>());
         }
+        if (trace)
+            fprintf(stderr, "[dss pd] round %u depth %llu candidates %llu hashed %llu shared %d heads %llu\n", round,
+                    (unsigned long long)depth, (unsigned long long)ncand, (unsigned long long)nh, (int)shared,
+                    (unsigned long long)nheads);
         // every hashed candidate: list entry, offset, the characters read
         // from its own string (all of them, or beyond B), hash
         s->timer.amend_last("pd_hash", (double)nh * ((shared ? depth - B : depth) + 24.0));
+        // the shared-prefix kernel stores hashes by position (its rounds hash
+        // at least half of the positions), the plain kernel in list order
+        const bool by_pos = shared;
         uint64_t nm = 0;  // possible repeats for the exact table
+        // the exact table's keys: keyv[maybe[k]] (keyv[kidx[maybe[k]]]), at
+        // position pos[maybe[k]]
+        const uint64_t* keyv = P.pd_hash.as<uint64_t>();
+        const uint32_t* kidx = by_pos ? P.pd_list.as<uint32_t>() : nullptr;
+        const uint32_t* kpos = P.pd_list.as<uint32_t>();
+        uint64_t kempty = kEmpty;
         if (nh >= part_min) {
-            nm = partition_duplicates(s, nh, table_limit);
+            nm = partition_duplicates(s, nh, table_limit, by_pos);
+            // keys x = mix64(hash): a bijection, and no hash is kEmpty
+            keyv = P.pd_pkey.as<uint64_t>() + nh;
+            kidx = nullptr;
+            kpos = P.pd_pid.as<uint32_t>() + nh;
+            kempty = mix64(kEmpty);
         } else if (nh) {
             uint32_t bits = 16;
             while (bits < 32 && (1ull << bits) < 16 * nh) ++bits;
@@ -1208,9 +1238,9 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
                 // hashed candidates: list entry, hash, two bitmap atomics
                 // (seen, rarely again), one bitmap read, the flag
                 KScope ks(s->timer, st, "pd_prefilter", (double)nh * 25.0);
-                hipLaunchKernelGGL(k_pd_bits, g, blk, 0, st, P.pd_list.as<uint32_t>(), nh, P.pd_hash.as<uint64_t>(),
-                                   bits, seen, again);
-                hipLaunchKernelGGL(k_pd_maybe, g, blk, 0, st, P.pd_list.as<uint32_t>(), nh,
+                hipLaunchKernelGGL(k_pd_bits, g, blk, 0, st, P.pd_list.as<uint32_t>(), by_pos, nh,
+                                   P.pd_hash.as<uint64_t>(), bits, seen, again);
+                hipLaunchKernelGGL(k_pd_maybe, g, blk, 0, st, P.pd_list.as<uint32_t>(), by_pos, nh,
                                    P.pd_hash.as<uint64_t>(), bits, again, P.pd_dup.as<uint8_t>(),
                                    P.pd_maybe.as<uint32_t>(), ctr + 2);
             }
@@ -1223,12 +1253,12 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
             P.pd_tflag.reserve(T);
             // table fill, per possible repeat: hash, slot, flags
             KScope ks(s->timer, st, "pd_table", (double)T * 9.0 + (double)nm * 30.0);
-            hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T, kEmpty);
+            hipLaunchKernelGGL(k_fill_u64, dim3((T + 255) / 256), blk, 0, st, P.pd_table.as<uint64_t>(), T, kempty);
             DSS_HIP(hipMemsetAsync(P.pd_tflag.p, 0, T, st));
-            hipLaunchKernelGGL(k_pd_insert, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
-                               P.pd_hash.as<uint64_t>(), P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(), T - 1,
+            hipLaunchKernelGGL(k_pd_insert, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm, keyv,
+                               kidx, kempty, P.pd_table.as<uint64_t>(), P.pd_tflag.as<uint8_t>(), T - 1,
                                P.pd_slot.as<uint32_t>());
-            hipLaunchKernelGGL(k_pd_flag, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm,
+            hipLaunchKernelGGL(k_pd_flag, dim3((nm + 255) / 256), blk, 0, st, P.pd_maybe.as<uint32_t>(), nm, kpos,
                                P.pd_slot.as<uint32_t>(), P.pd_tflag.as<uint8_t>(), P.pd_dup.as<uint8_t>());
         }
         {

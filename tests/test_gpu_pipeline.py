@@ -401,28 +401,56 @@ def test_prefix_doubling_shared_prefix_rounds(m):
     assert nr == onr and (gres == ores).all() and (grounds == orounds).all()
 
 
+def _pd_trace(text):
+    """(depth, hashed, shared) of every round the library's round trace
+    (DSS_PD_TRACE) printed."""
+    out = []
+    for line in text.splitlines():
+        if line.startswith("[dss pd] round"):
+            f = line.split()
+            out.append((int(f[5]), int(f[9]), int(f[11])))
+    return out
+
+
 @pytest.mark.parametrize("limit", [None, "8"])
 @pytest.mark.parametrize("m", [2**64 - 1, 4093, 64])
-@pytest.mark.parametrize("name", ["skewed", "dton"])
-def test_prefix_doubling_partitioned_duplicates(name, m, limit, monkeypatch):
+@pytest.mark.parametrize("name", ["skewed", "dton", "dton500"])
+def test_prefix_doubling_partitioned_duplicates(name, m, limit, monkeypatch, capfd):
     """Rounds with many hashed candidates find repeated hashes by partitioning
     them into buckets and checking each bucket in an LDS table; buckets with
     more distinct hashes than the table allows (a limit of 8 forces that)
     fall back to the exact global table.  Forced on for every round here,
-    both must give the oracle's results, collisions of small ranges included."""
+    both must give the oracle's results, collisions of small ranges included.
+    The round trace shows which hash layout each round used: shared-prefix
+    rounds store hashes by position (the 500-character D/N input's depth-264
+    round), the others in list order (the skewed input's rounds)."""
     dss = _dss()
-    chars = INPUTS[name] if name == "skewed" else O.gen_dton(20000, 200, 0.5, 0x5EED0008)
+    if name == "skewed":
+        chars = INPUTS[name]
+    elif name == "dton":
+        chars = O.gen_dton(20000, 200, 0.5, 0x5EED0008)
+    else:
+        chars = O.gen_dton(20000, 500, 0.5, 0x5EED0007)
     ss = O.StringSet(chars, 4)
     operm, olcp = O.local_sort(ss)
     ores, orounds, onr = O.prefix_doubling(ss, operm, olcp, m=m)
     monkeypatch.setenv("DSS_PD_PARTITION_MIN", "1")
+    monkeypatch.setenv("DSS_PD_TRACE", "1")
     if limit:
         monkeypatch.setenv("DSS_PD_TABLE_LIMIT", limit)
+    capfd.readouterr()
     with dss.DeviceStringSet.from_arena(chars, 4) as ds:
         ds.sort_lcp()
         nr = ds.prefix_doubling(m=m)
         gres, grounds = ds.get_prefix_doubling()
+    trace = _pd_trace(capfd.readouterr().err)
     assert nr == onr and (gres == ores).all() and (grounds == orounds).all()
+    assert trace and len(trace) <= nr, trace  # rounds skipped up front print nothing
+    layouts = {shared for _, hashed, shared in trace if hashed}
+    if name == "skewed":
+        assert 0 in layouts, trace
+    elif name == "dton500":
+        assert 1 in layouts, trace
 
 
 @pytest.mark.parametrize("name", sorted(INPUTS))
