@@ -1532,7 +1532,68 @@ __device__ inline int wave_tile_cmp(const TileView& T, bool act, uint32_t x, uin
         bl = T.vlen[ob];
     }
     const uint32_t lane = lane_id();
-    for (uint64_t m = __ballot(pend); m; m &= m - 1) {
+    uint64_t m = __ballot(pend);
+    if (__popcll(m) >= 2) {
+        // several open lanes: each first takes one step on its own group of
+        // W = 64 / kG lanes (up to kG = 8 lanes at a time, 8 W characters),
+        // so short walks cost one dependent load per group of lanes instead
+        // of one per lane; those still open continue one at a time below
+        uint64_t todo = m;
+        while (todo) {  // wave-uniform
+            const uint32_t k = min(8u, (uint32_t)__popcll(todo));
+            const uint32_t lgk = k <= 1 ? 0u : (k <= 2 ? 1u : (k <= 4 ? 2u : 3u));
+            const uint32_t W = 64u >> lgk, g = lane >> (6 - lgk);
+            uint64_t sel = 0;
+            uint32_t myld = 0;
+            bool gact = false;
+            for (uint32_t r = 0; r < k; ++r) {
+                const uint32_t ldr = (uint32_t)__ffsll((unsigned long long)todo) - 1;
+                todo &= todo - 1;
+                sel |= 1ull << ldr;
+                if (g == r) {
+                    myld = ldr;
+                    gact = true;
+                }
+            }
+            const uint64_t pa = __shfl((unsigned long long)ao, myld), pb = __shfl((unsigned long long)bo, myld);
+            const uint32_t pal = (uint32_t)__shfl((int)al, myld), pbl = (uint32_t)__shfl((int)bl, myld);
+            const uint32_t pf = (uint32_t)__shfl((int)from, myld);
+            const uint32_t sub = lane & (W - 1);
+            uint64_t ka = 0, kb = 0;
+            bool ev = false;
+            if (gact) {
+                ka = load_key8(T.chars, pa, pal, pf + 8 * sub);
+                kb = load_key8(T.chars, pb, pbl, pf + 8 * sub);
+                ev = ka != kb || (ka & 0xFF) == 0;
+            }
+            const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
+            const uint64_t gm = (__ballot(ev) >> (g * W)) & wmask;
+            const uint32_t f = g * W + (gm ? (uint32_t)__ffsll((unsigned long long)gm) - 1 : 0u);
+            const uint64_t fa = __shfl((unsigned long long)ka, f), fb = __shfl((unsigned long long)kb, f);
+            int gc = 0;
+            uint32_t glcp = 0;
+            if (gm) {
+                gc = fa < fb ? -1 : (fa > fb ? 1 : 0);
+                glcp = pf + 8 * (f - g * W) + (fa != fb ? key_lcp(fa, fb) : key_strlen(fa));
+            }
+            // a selected lane's group is its rank among the selected lanes
+            const uint32_t src = min((uint32_t)__popcll(sel & lanemask_lt()), (1u << lgk) - 1) * W;
+            const int rc = __shfl(gc, src);
+            const uint32_t rl = (uint32_t)__shfl((int)glcp, src);
+            const bool rd = __shfl((int)(gm != 0), src) != 0;
+            if ((sel >> lane) & 1) {
+                if (rd) {
+                    c = rc;
+                    lcp = rl;
+                    pend = false;
+                } else {
+                    from += 8 * W;
+                }
+            }
+        }
+        m = __ballot(pend);
+    }
+    for (; m; m &= m - 1) {
         const int ld = __ffsll((unsigned long long)m) - 1;
         const uint64_t pa = __shfl((unsigned long long)ao, ld), pb = __shfl((unsigned long long)bo, ld);
         const uint32_t pal = (uint32_t)__shfl((int)al, ld), pbl = (uint32_t)__shfl((int)bl, ld);
