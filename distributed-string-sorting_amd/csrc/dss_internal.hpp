@@ -266,6 +266,7 @@ struct SortScratch {
     DevBuf small;           // small segment lists (all classes)
     DevBuf counters;        // device counters
     DevBuf offlen;          // packed (offset, length) per string for the base cases
+    DevBuf digit;           // the radix digit map of the sort and the arena's byte values
     std::vector<uint64_t> small_off, small_cap;  // per class region in `small`
     uint64_t seg_cap = 0, tile_cap = 0;
     SortGraphCache graphs;  // the device-driven form's captured launch sequences
@@ -342,6 +343,7 @@ struct SortParams {
     uint32_t merge_fan = 2;        // runs merged per pass (2: the pairwise tree; 4, 8, 16: K-way tiles)
     uint32_t pass_mode = 0;        // local sort: 0 auto, 1 host-driven passes, 2 device-driven (graph)
     uint32_t merge_tile = 1024;    // outputs per tile of the pairwise merge (512, 1024, 2048)
+    uint32_t digit_chars = 0;      // characters per radix digit: 0 auto (small alphabets), 1, or up to 2 / 3
 };
 
 struct dss_set_s {

@@ -17,13 +17,17 @@
 //                     first string, 32 bytes per lane and step, loads bounded
 //                     by the minimum found so far;
 //     key gather   -- 8 chars at the new depth per string;
-//     histogram    -- 256 byte-digit counters per tile in LDS;
+//     histogram    -- 256 digit counters per tile in LDS (a digit is one
+//                     byte, or on small alphabets up to 3 ranked
+//                     characters: see DigitMap);
 //     scan/classify-- digit offsets (two-level over 64-tile chunks for long
 //                     segments), sub-bucket lists, LCP at sub-bucket
 //                     boundaries;
 //     scatter      -- stable rank inside the tile by wave ballots, writes the
 //                     next key/id buffers, the id array and LCPs of equal
-//                     strings.  Up to 8 digit passes reuse one gathered key.
+//                     strings.  The digit passes over one gathered key
+//                     cover its 8 characters (8 passes of one character, 3
+//                     of up to three).
 //   small segments (2..large_min strings), batched by power-of-two class:
 //     up to 64 strings one segment per wave (register bitonic sort), larger
 //     ones in 1024-slot LDS blocks; both sort (group, key, id) and refine
@@ -308,6 +312,147 @@ __device__ inline void seg_depths(const Seg& s, uint32_t& kd, uint32_t& dep) {
     }
 }
 
+// ------------------------------------------------------------ digits
+// A radix digit covers kc characters.  With a small alphabet (sigma values:
+// the string end, rank 0, and the byte values that occur, ranked in byte
+// order) kc characters c_0..c_{kc-1} form the digit sum rank(c_i)
+// sigma^(kc-1-i) < 256, which orders like the characters; kc = 1 is the
+// byte itself.  A digit whose last rank is 0 holds strings that end inside
+// it (the ranks after an end are 0 too); digit 0 holds strings that end at
+// its first character.  The map is computed on the device per sort from the
+// bytes of the arena (k_alphabet, k_digit_map).
+struct DigitMap {
+    uint32_t kc, sigma;
+    uint8_t rank[256];
+};
+
+constexpr uint64_t kDigitBitsAt = 512;   // the present-byte bits after the map in SortScratch::digit
+constexpr uint64_t kDigitBytes = 1024;
+
+struct DigitCtx {
+    uint32_t kc, sigma;
+    const uint8_t* rank;  // LDS copy of DigitMap::rank (kc > 1)
+};
+
+// The digit at byte `pos` of a big-endian 8-character key (pos + kc <= 8).
+__device__ inline uint32_t digit_of(uint64_t key, uint32_t pos, const DigitCtx& D) {
+    if (D.kc == 1) return (uint32_t)(key >> (56 - 8 * pos)) & 0xFF;
+    uint32_t d = 0;
+    for (uint32_t i = 0; i < D.kc; ++i) d = d * D.sigma + D.rank[(uint32_t)(key >> (56 - 8 * (pos + i))) & 0xFF];
+    return d;
+}
+
+// Characters two different digits share at their start.
+__device__ inline uint32_t digit_lead(uint32_t a, uint32_t b, const DigitCtx& D) {
+    if (D.kc == 1) return 0;
+    uint32_t pw = 1;
+    for (uint32_t i = 1; i < D.kc; ++i) pw *= D.sigma;
+    for (uint32_t i = 0; i < D.kc; ++i, pw /= D.sigma)
+        if ((a / pw) % D.sigma != (b / pw) % D.sigma) return i;
+    return D.kc;
+}
+
+// Characters of a (non-zero) digit before the end of its strings: kc unless
+// they end inside it.
+__device__ inline uint32_t digit_chars(uint32_t d, const DigitCtx& D) {
+    if (D.kc == 1) return 1;
+    uint32_t pw = 1;
+    for (uint32_t i = 1; i < D.kc; ++i) pw *= D.sigma;
+    for (uint32_t i = 0; i < D.kc; ++i, pw /= D.sigma)
+        if ((d / pw) % D.sigma == 0) return i;
+    return D.kc;
+}
+
+// A segment's digit context: the digit starting at byte pos of its keys
+// covers at most the 8 - pos characters the keys still hold, so that keys
+// are re-gathered every 8 characters whatever the digit width.
+__device__ inline DigitCtx seg_digits(const DigitCtx& D, uint32_t pos) {
+    return D.kc == 1 ? D : DigitCtx{min(D.kc, 8u - pos), D.sigma, D.rank};
+}
+
+// The block's digit context: kMulti false is one character per digit (a
+// compile-time constant, so the byte digits compile as before); otherwise
+// the ranks are staged in `lds` (the caller synchronises).
+template <bool kMulti>
+__device__ inline DigitCtx digit_ctx(const DigitMap* __restrict__ dm, uint8_t* lds) {
+    if (!kMulti) return DigitCtx{1u, 256u, lds};
+    for (uint32_t t = threadIdx.x; t < 256; t += blockDim.x) lds[t] = dm->rank[t];
+    return DigitCtx{dm->kc, dm->sigma, lds};
+}
+
+// Byte values present in chars[lo, hi): a block marks them in LDS with plain
+// stores (16 bytes per thread and step) and ORs them into bits[8].  With
+// `gate`, blocks return at once when bits already holds more than 15
+// non-zero byte values (more than one character per digit is then ruled
+// out).
+constexpr uint64_t kAlphaSample = 1ull << 20;  // bytes scanned before the gated rest
+
+__global__ void __launch_bounds__(256)
+k_alphabet(const uint8_t* __restrict__ chars, uint64_t lo, uint64_t hi, uint32_t* __restrict__ bits, int gate) {
+    __shared__ uint8_t present[256];
+    __shared__ int stop;
+    const uint32_t t = threadIdx.x;
+    present[t] = 0;
+    if (t == 0) {
+        int c = 0;
+        if (gate) {
+            for (int w = 0; w < 8; ++w) c += __popc(bits[w]);
+            c -= (int)(bits[0] & 1);  // byte 0 is the end marker
+        }
+        stop = c > 15;
+    }
+    __syncthreads();
+    if (stop) return;  // block-uniform
+    const uint64_t a0 = (lo + 15) & ~15ull, a1 = max(a0, hi & ~15ull);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t a = a0 + 16ull * ((uint64_t)blockIdx.x * blockDim.x + t); a < a1; a += 16 * stride) {
+        const uint4 x = *reinterpret_cast<const uint4*>(chars + a);
+        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) present[(w[i] >> (8 * k)) & 0xFF] = 1;
+    }
+    // the unaligned edges
+    if (blockIdx.x == 0) {
+        for (uint64_t a = lo + t; a < min(a0, hi); a += blockDim.x) present[chars[a]] = 1;
+        for (uint64_t a = max(a1, lo) + t; a < hi; a += blockDim.x) present[chars[a]] = 1;
+    }
+    __syncthreads();
+    const uint64_t m = __ballot(present[t] != 0);
+    if ((t & 31) == 0) {
+        const uint32_t word = (uint32_t)(m >> (t & 32));
+        if (word) atomicOr(&bits[t >> 5], word);
+    }
+}
+
+// The digit map from the present byte values: sigma = 1 + present non-zero
+// values, kc the most characters (at most max_kc) with sigma^kc <= 256.
+__global__ void __launch_bounds__(256)
+k_digit_map(const uint32_t* __restrict__ bits, uint32_t max_kc, DigitMap* __restrict__ dm) {
+    __shared__ uint32_t wc[4];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const bool pres = t > 0 && ((bits[t >> 5] >> (t & 31)) & 1);
+    const uint64_t m = __ballot(pres);
+    if (lane == 0) wc[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = (uint32_t)__popcll(m & lanemask_lt());
+    for (uint32_t k = 0; k < w; ++k) before += wc[k];
+    dm->rank[t] = pres ? (uint8_t)(1 + before) : (uint8_t)0;
+    if (t == 0) {
+        const uint32_t sigma = 1 + wc[0] + wc[1] + wc[2] + wc[3];
+        uint32_t kc = 1;
+        while (kc < max_kc) {
+            uint32_t pw = 1;
+            for (uint32_t i = 0; i <= kc; ++i) pw *= sigma;
+            if (pw > 256) break;
+            ++kc;
+        }
+        dm->kc = kc;
+        dm->sigma = sigma;
+    }
+}
+
 // ------------------------------------------------------------ prefix skip
 // Groups of 8 (or 16) lanes compare kU strings at a time against the
 // segment's first string, 32 bytes per lane and step (four 8-char keys from
@@ -538,31 +683,35 @@ k_gather(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uin
 
 // ------------------------------------------------------------ histogram
 __device__ __forceinline__ void hist_tile(uint32_t tb, const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
-                                          const uint64_t* __restrict__ key, uint32_t* __restrict__ hist) {
+                                          const uint64_t* __restrict__ key, uint32_t* __restrict__ hist,
+                                          const DigitCtx& D) {
     __shared__ uint32_t h[256];
     const Tile T = tiles[tb];
     const Seg S = segs[T.seg];
     uint32_t kd, dep;
     seg_depths(S, kd, dep);
-    const uint32_t sh = 56 - 8 * (dep - kd);
+    const DigitCtx Ds = seg_digits(D, dep - kd);
     h[threadIdx.x] = 0;
     __syncthreads();
     for (uint32_t e = T.begin + threadIdx.x; e < T.end; e += kThreads)
-        atomicAdd(&h[(uint32_t)(key[e] >> sh) & 0xFF], 1u);
+        atomicAdd(&h[digit_of(key[e], dep - kd, Ds)], 1u);
     __syncthreads();
     hist[(uint64_t)tb * 256 + threadIdx.x] = h[threadIdx.x];
 }
 
-template <bool kDev>
+// kMulti: digits from the digit map dm (else one byte each).
+template <bool kDev, bool kMulti>
 __global__ void __launch_bounds__(kThreads)
 k_hist(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint64_t* __restrict__ key,
-       uint32_t* __restrict__ hist, const uint32_t* __restrict__ hp) {
+       uint32_t* __restrict__ hist, const DigitMap* __restrict__ dm, const uint32_t* __restrict__ hp) {
+    __shared__ uint8_t rk[256];
+    const DigitCtx D = digit_ctx<kMulti>(dm, rk);  // hist_tile synchronises before its first use
     if constexpr (!kDev) {
-        hist_tile(blockIdx.x, tiles, segs, key, hist);
+        hist_tile(blockIdx.x, tiles, segs, key, hist, D);
     } else {
         const uint32_t nb = hp[kCtrTiles];
         for (uint32_t tb = blockIdx.x; tb < nb; tb += gridDim.x) {
-            hist_tile(tb, tiles, segs, key, hist);
+            hist_tile(tb, tiles, segs, key, hist, D);
             __syncthreads();  // LDS of this tile read before the next tile's
         }
     }
@@ -582,8 +731,9 @@ __device__ __forceinline__ void scan_classify_block(uint32_t tb, const Seg* __re
                                                     Seg* __restrict__ next, SmallSeg* __restrict__ small,
                                                     const SmallLists& L, uint32_t large_min, uint32_t* __restrict__ ctr,
                                                     uint32_t* __restrict__ next_large, uint32_t* __restrict__ lcp,
-                                                    uint32_t key_buf) {
+                                                    uint32_t key_buf, const DigitMap* __restrict__ dm) {
     __shared__ uint32_t sc_all[kScGroups][256];
+    __shared__ uint32_t pv_all[kScGroups][256];
     __shared__ uint32_t first_all[kScGroups];
     __shared__ uint32_t lcnt[kClasses + 1], lelem[kClasses], lbase[kClasses + 1];
     const uint32_t grp = threadIdx.x >> 8;
@@ -626,8 +776,28 @@ __device__ __forceinline__ void scan_classify_block(uint32_t tb, const Seg* __re
         run += x;
     }
     __syncthreads();
+    // digits of kc characters: a bucket's first string shares dep characters
+    // and the leading characters of the two digits with the last string of
+    // the previous non-empty bucket (found by a max-scan)
+    const DigitCtx D0{dm ? dm->kc : 1u, dm ? dm->sigma : 256u, nullptr};
+    const DigitCtx D = seg_digits(D0, dep - kd);
+    uint32_t prev = 0;       // the previous non-empty digit + 1, 0 if none
+    if (dm && D0.kc > 1) {   // block-uniform
+        uint32_t* const pv = pv_all[grp];
+        pv[d] = tot ? d + 1 : 0;
+        __syncthreads();
+        for (uint32_t o = 1; o < 256; o <<= 1) {
+            const uint32_t x = d >= o ? pv[d - o] : 0;
+            __syncthreads();
+            pv[d] = max(pv[d], x);
+            __syncthreads();
+        }
+        prev = d ? pv[d - 1] : 0u;
+    }
     const uint32_t b = S.begin + base;
-    if (tot && d != first_all[grp]) lcp[b] = dep;  // boundary between non-empty buckets
+    if (tot && d != first_all[grp])  // boundary between non-empty buckets
+        lcp[b] = dep + (prev ? digit_lead(prev - 1, d, D) : 0u);
+    const uint32_t nd = dep + digit_chars(d, D);  // the sub-bucket's depth (d != 0)
     // d == 0: equal strings; tot == 1: a single string -- both done
     int list = -1;  // small class c, or kClasses for the large list
     uint32_t slot = 0;
@@ -646,33 +816,34 @@ __device__ __forceinline__ void scan_classify_block(uint32_t tb, const Seg* __re
     if (list < 0) return;
     if (list < (int)kClasses) {
         // the scatter of this pass leaves the bucket's keys (depth kd) in key_buf
-        const uint32_t sh = dep + 1 - kd;
-        small[L.off[list] + lbase[list] + slot] = SmallSeg{b, tot, dep + 1, sh < 8 ? (sh | (key_buf << 3)) : 0u};
+        const uint32_t sh = nd - kd;
+        small[L.off[list] + lbase[list] + slot] = SmallSeg{b, tot, nd, sh < 8 ? (sh | (key_buf << 3)) : 0u};
     } else {
         // keys run out after 8 characters; a bucket that holds the whole
         // segment (one digit shared by all) also re-runs the prefix skip,
         // which jumps over the rest of the shared prefix at once
-        const uint32_t nkd = (dep + 1 - kd >= 8 || tot == S.cnt) ? kNone : kd;
-        next[lbase[kClasses] + slot] = Seg{b, tot, dep + 1, nkd, kNone, 0, 0};
+        const uint32_t nkd = (nd - kd >= 8 || tot == S.cnt) ? kNone : kd;
+        next[lbase[kClasses] + slot] = Seg{b, tot, nd, nkd, kNone, 0, 0};
     }
 }
 
 // kDev: the segment count comes from the pass counters hp; every block loops
-// over groups of kScGroups segments.
-template <bool kDev>
+// over groups of kScGroups segments.  kMulti: digits from the digit map dm.
+template <bool kDev, bool kMulti>
 __global__ void __launch_bounds__(kThreads * kScGroups)
 k_scan_classify(const Seg* __restrict__ segs, uint32_t nseg, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
                 Seg* __restrict__ next, SmallSeg* __restrict__ small, SmallLists L, uint32_t large_min,
                 uint32_t* __restrict__ ctr, uint32_t* __restrict__ next_large, uint32_t* __restrict__ lcp,
-                uint32_t key_buf, const uint32_t* __restrict__ hp) {
+                uint32_t key_buf, const DigitMap* __restrict__ dm, const uint32_t* __restrict__ hp) {
     if constexpr (!kDev) {
         scan_classify_block(blockIdx.x, segs, nseg, hist, csum, next, small, L, large_min, ctr, next_large, lcp,
-                            key_buf);
+                            key_buf, kMulti ? dm : nullptr);
     } else {
         nseg = hp[kCtrLarge];
         const uint32_t nb = (nseg + kScGroups - 1) / kScGroups;
         for (uint32_t tb = blockIdx.x; tb < nb; tb += gridDim.x) {
-            scan_classify_block(tb, segs, nseg, hist, csum, next, small, L, large_min, ctr, next_large, lcp, key_buf);
+            scan_classify_block(tb, segs, nseg, hist, csum, next, small, L, large_min, ctr, next_large, lcp, key_buf,
+                                kMulti ? dm : nullptr);
             __syncthreads();
         }
     }
@@ -693,7 +864,7 @@ __device__ __forceinline__ void scatter_tile(uint32_t tb, const Tile* __restrict
                                              const uint32_t* __restrict__ hist, const uint64_t* __restrict__ key_in,
                                              const uint32_t* __restrict__ pb_in, uint64_t* __restrict__ key_out,
                                              uint32_t* __restrict__ pb_out, uint32_t* __restrict__ perm,
-                                             uint32_t* __restrict__ lcp) {
+                                             uint32_t* __restrict__ lcp, const DigitCtx& D) {
     constexpr uint32_t kW = kScT / 64;         // waves per tile
     constexpr uint32_t kWR = kScI * 64;        // records per wave (contiguous)
     __shared__ uint64_t skey[kTile];
@@ -706,7 +877,8 @@ __device__ __forceinline__ void scatter_tile(uint32_t tb, const Tile* __restrict
     const Seg S = segs[T.seg];
     uint32_t kd, dep;
     seg_depths(S, kd, dep);
-    const uint32_t sh = 56 - 8 * (dep - kd);
+    const uint32_t pos = dep - kd;  // the digit's first byte in the keys
+    const DigitCtx Ds = seg_digits(D, pos);
     const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const bool dig = tid < 256;  // the threads that own a digit
     if (dig) gbase[tid] = hist[(uint64_t)tb * 256 + tid];
@@ -726,7 +898,7 @@ __device__ __forceinline__ void scatter_tile(uint32_t tb, const Tile* __restrict
         const bool valid = e < T.end;
         kv[r] = valid ? key_in[e] : 0ull;
         idv[r] = valid ? (pb_in ? pb_in[e] : e) : 0u;  // first pass: ids are positions
-        dg[r] = (uint32_t)(kv[r] >> sh) & 0xFF;
+        dg[r] = digit_of(kv[r], pos, Ds);
     }
 #pragma unroll
     for (uint32_t r = 0; r < kScI; ++r) {
@@ -784,7 +956,7 @@ __device__ __forceinline__ void scatter_tile(uint32_t tb, const Tile* __restrict
     const uint32_t cnt = T.end - T.begin;
     for (uint32_t k = tid; k < cnt; k += kScT) {
         const uint64_t x = skey[k];
-        const uint32_t d = (uint32_t)(x >> sh) & 0xFF;
+        const uint32_t d = digit_of(x, pos, Ds);
         const uint32_t q = gbase[d] + (k - lstart[d]);
         const uint32_t id = sid[k];
         key_out[q] = x;
@@ -794,18 +966,21 @@ __device__ __forceinline__ void scatter_tile(uint32_t tb, const Tile* __restrict
     }
 }
 
-template <bool kDev>
+// kMulti: digits from the digit map dm (else one byte each).
+template <bool kDev, bool kMulti>
 __global__ void __launch_bounds__(kScT)
 k_scatter(const Tile* __restrict__ tiles, const Seg* __restrict__ segs, const uint32_t* __restrict__ hist,
           const uint64_t* __restrict__ key_in, const uint32_t* __restrict__ pb_in,
           uint64_t* __restrict__ key_out, uint32_t* __restrict__ pb_out, uint32_t* __restrict__ perm,
-          uint32_t* __restrict__ lcp, const uint32_t* __restrict__ hp) {
+          uint32_t* __restrict__ lcp, const DigitMap* __restrict__ dm, const uint32_t* __restrict__ hp) {
+    __shared__ uint8_t rk[256];
+    const DigitCtx D = digit_ctx<kMulti>(dm, rk);  // scatter_tile synchronises before its first use
     if constexpr (!kDev) {
-        scatter_tile(blockIdx.x, tiles, segs, hist, key_in, pb_in, key_out, pb_out, perm, lcp);
+        scatter_tile(blockIdx.x, tiles, segs, hist, key_in, pb_in, key_out, pb_out, perm, lcp, D);
     } else {
         const uint32_t nb = hp[kCtrTiles];
         for (uint32_t tb = blockIdx.x; tb < nb; tb += gridDim.x) {
-            scatter_tile(tb, tiles, segs, hist, key_in, pb_in, key_out, pb_out, perm, lcp);
+            scatter_tile(tb, tiles, segs, hist, key_in, pb_in, key_out, pb_out, perm, lcp, D);
             __syncthreads();  // LDS of this tile read before the next tile's
         }
     }
@@ -1507,7 +1682,34 @@ static SortCaps reserve_sort(dss_set_s* s, uint64_t n, uint32_t p) {
     X.csum.reserve(c.chunk_cap * 256 * 4);
     X.small.reserve(std::max<uint64_t>(c.small_total, 1) * sizeof(SmallSeg));
     X.counters.reserve(64 * 4 + 16);  // 64 counters, then the device-driven form's string and PE counts
+    X.digit.reserve(kDigitBytes);
     return c;
+}
+
+// The digit map of a host-driven sort, or nullptr for one character per
+// digit.  digit_chars 0 (auto) scans sets of at least 2^22 strings whose
+// arena holds at most 2 GiB (a sample of the first 1 MiB first: arenas with
+// more than 15 byte values stop there), 2 or 3 scan any set and allow up to
+// that many characters.  Batch sets over a job pool keep one character.
+static const DigitMap* digit_map(dss_set_s* s) {
+    const uint32_t want = s->sort_params.digit_chars;
+    if (want == 1 || s->borrowed || s->nbytes == 0) return nullptr;
+    if (want == 0 && (s->n < (1ull << 22) || s->nbytes > (1ull << 31))) return nullptr;
+    const hipStream_t st = s->stream;
+    SortScratch& X = s->ss;
+    DigitMap* dm = X.digit.as<DigitMap>();
+    uint32_t* bits = reinterpret_cast<uint32_t*>(X.digit.as<uint8_t>() + kDigitBitsAt);
+    const uint64_t nb = s->nbytes, ns = std::min<uint64_t>(nb, kAlphaSample);
+    KScope ks(s->timer, st, "alphabet", (double)nb);
+    fill_small(st, bits, 0, 32);
+    hipLaunchKernelGGL(k_alphabet, dim3(stride_grid(ns / 16 + 1, 256)), dim3(256), 0, st, s->arena(), (uint64_t)0, ns,
+                       bits, 0);
+    if (nb > ns)
+        hipLaunchKernelGGL(k_alphabet, dim3(stride_grid((nb - ns) / 16 + 1, 256 * 16)), dim3(256), 0, st, s->arena(),
+                           ns, nb, bits, 1);
+    hipLaunchKernelGGL(k_digit_map, dim3(1), dim3(256), 0, st, bits, want ? want : 3u, dm);
+    DSS_HIP(hipGetLastError());
+    return dm;
 }
 
 // Pass list buffers: the current and next segment lists, key and id
@@ -1520,6 +1722,7 @@ struct PassBufs {
     uint32_t* pc_ids;
     uint32_t* pn_ids;
     uint32_t par;
+    const DigitMap* dm;  // characters per digit (nullptr: one)
     void swap_pass() {
         par ^= 1;
         std::swap(cur, nxt);
@@ -1530,7 +1733,7 @@ struct PassBufs {
 
 static PassBufs first_pass_bufs(SortScratch& X) {
     return PassBufs{X.segA.as<Seg>(), X.segB.as<Seg>(), X.keyA.as<uint64_t>(), X.keyB.as<uint64_t>(),
-                    X.pbA.as<uint32_t>(), X.pbB.as<uint32_t>(), 0};
+                    X.pbA.as<uint32_t>(), X.pbB.as<uint32_t>(), 0, nullptr};
 }
 
 static const char* const kClassNames[kClasses] = {
@@ -1598,7 +1801,8 @@ static void host_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint3
         }
         {
             KScope ks(s->timer, st, "hist", (double)cnt_large * 8.0);
-            hipLaunchKernelGGL(k_hist<false>, dim3(nt), dim3(kThreads), 0, st, tl, cur, kc, X.hist.as<uint32_t>(),
+            decltype(&k_hist<false, false>) kern = b.dm ? &k_hist<false, true> : &k_hist<false, false>;
+            hipLaunchKernelGGL(kern, dim3(nt), dim3(kThreads), 0, st, tl, cur, kc, X.hist.as<uint32_t>(), b.dm,
                                nullptr);
         }
         {
@@ -1606,18 +1810,21 @@ static void host_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint3
             if (nchunks)
                 hipLaunchKernelGGL(k_chunk_sum<false>, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
                                    X.hist.as<uint32_t>(), X.csum.as<uint32_t>(), nullptr);
-            hipLaunchKernelGGL(k_scan_classify<false>, dim3((nlarge + kScGroups - 1) / kScGroups),
+            decltype(&k_scan_classify<false, false>) kern =
+                b.dm ? &k_scan_classify<false, true> : &k_scan_classify<false, false>;
+            hipLaunchKernelGGL(kern, dim3((nlarge + kScGroups - 1) / kScGroups),
                                dim3(kThreads * kScGroups), 0, st, cur, nlarge, X.hist.as<uint32_t>(),
                                X.csum.as<uint32_t>(), nxt, X.small.as<SmallSeg>(), L, T.large_min, ctr,
-                               pn + kCtrLarge, lcp, kn == X.keyB.as<uint64_t>() ? 1u : 0u, nullptr);
+                               pn + kCtrLarge, lcp, kn == X.keyB.as<uint64_t>() ? 1u : 0u, b.dm, nullptr);
             if (nchunks)
                 hipLaunchKernelGGL(k_chunk_apply<false>, dim3(nchunks), dim3(kThreads), 0, st, cur, nlarge,
                                    X.hist.as<uint32_t>(), X.csum.as<uint32_t>(), nullptr);
         }
         {
             KScope ks(s->timer, st, "scatter", (double)cnt_large * 28.0);
-            hipLaunchKernelGGL(k_scatter<false>, dim3(nt), dim3(kScT), 0, st, tl, cur, X.hist.as<uint32_t>(), kc,
-                               first ? nullptr : b.pc_ids, kn, b.pn_ids, perm, lcp, nullptr);
+            decltype(&k_scatter<false, false>) kern = b.dm ? &k_scatter<false, true> : &k_scatter<false, false>;
+            hipLaunchKernelGGL(kern, dim3(nt), dim3(kScT), 0, st, tl, cur, X.hist.as<uint32_t>(), kc,
+                               first ? nullptr : b.pc_ids, kn, b.pn_ids, perm, lcp, b.dm, nullptr);
         }
         hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, nxt, pn);
         DSS_HIP(hipGetLastError());
@@ -1771,17 +1978,18 @@ static void launch_dev_pass(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, P
     hipLaunchKernelGGL((k_prefix_skip<kL, kU, true>), dim3(gt), dim3(kThreads), 0, st, tl, b.cur, perm, chars, off,
                        len, b.kc, first ? nullptr : b.pc_ids, hp);
     hipLaunchKernelGGL(k_gather<true>, dim3(gt), dim3(kThreads), 0, st, tl, b.cur, perm, chars, off, len, b.kc, hp);
-    hipLaunchKernelGGL(k_hist<true>, dim3(gt), dim3(kThreads), 0, st, tl, b.cur, b.kc, X.hist.as<uint32_t>(), hp);
+    hipLaunchKernelGGL((k_hist<true, false>), dim3(gt), dim3(kThreads), 0, st, tl, b.cur, b.kc, X.hist.as<uint32_t>(), b.dm,
+                       hp);
     hipLaunchKernelGGL(k_chunk_sum<true>, dim3(gc), dim3(kThreads), 0, st, b.cur, 0u, X.hist.as<uint32_t>(),
                        X.csum.as<uint32_t>(), hp);
-    hipLaunchKernelGGL(k_scan_classify<true>, dim3(grid_cap(caps.seg_cap, kScGroups, 512)),
+    hipLaunchKernelGGL((k_scan_classify<true, false>), dim3(grid_cap(caps.seg_cap, kScGroups, 512)),
                        dim3(kThreads * kScGroups), 0, st, b.cur, 0u, X.hist.as<uint32_t>(), X.csum.as<uint32_t>(),
                        b.nxt, X.small.as<SmallSeg>(), caps.L, T.large_min, ctr, pn + kCtrLarge, lcp,
-                       b.kn == X.keyB.as<uint64_t>() ? 1u : 0u, hp);
+                       b.kn == X.keyB.as<uint64_t>() ? 1u : 0u, b.dm, hp);
     hipLaunchKernelGGL(k_chunk_apply<true>, dim3(gc), dim3(kThreads), 0, st, b.cur, 0u, X.hist.as<uint32_t>(),
                        X.csum.as<uint32_t>(), hp);
-    hipLaunchKernelGGL(k_scatter<true>, dim3(gt), dim3(kScT), 0, st, tl, b.cur, X.hist.as<uint32_t>(), b.kc,
-                       first ? nullptr : b.pc_ids, b.kn, b.pn_ids, perm, lcp, hp);
+    hipLaunchKernelGGL((k_scatter<true, false>), dim3(gt), dim3(kScT), 0, st, tl, b.cur, X.hist.as<uint32_t>(), b.kc,
+                       first ? nullptr : b.pc_ids, b.kn, b.pn_ids, perm, lcp, b.dm, hp);
     hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, b.nxt, pn);
     b.swap_pass();
 }
@@ -1954,6 +2162,7 @@ void sort_lcp(dss_set_s* s) {
     s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
     const bool trace = getenv("DSS_SORT_TRACE") != nullptr;
     PassBufs b = first_pass_bufs(X);
+    b.dm = digit_map(s);
     host_passes(s, caps, ctr, h_ctr, b, trace, true);
     host_base_cases(s, caps, h_ctr, trace);
     s->sorted = true;
@@ -2016,6 +2225,7 @@ int dss_set_sort_params(dss_set s, const dss_sort_params_t* params) {
             DSS_REQUIRE(params->merge_tile == 0 || params->merge_tile == 512 || params->merge_tile == 1024 ||
                             params->merge_tile == 2048,
                         DSS_EINVAL, "merge_tile must be 0 (default), 512, 1024 or 2048");
+            DSS_REQUIRE(params->digit_chars <= 3, DSS_EINVAL, "digit_chars must be 0 (auto), 1, 2 or 3");
             T.large_min = params->large_min;
             T.wave_class_max = params->wave_class_max;
             T.skip_lanes = params->skip_lanes;
@@ -2023,6 +2233,7 @@ int dss_set_sort_params(dss_set s, const dss_sort_params_t* params) {
             if (params->merge_fan_in) T.merge_fan = params->merge_fan_in;
             T.pass_mode = params->pass_mode;
             if (params->merge_tile) T.merge_tile = params->merge_tile;
+            T.digit_chars = params->digit_chars;
         }
         s->sort_params = T;
     });
@@ -2038,6 +2249,7 @@ int dss_get_sort_params(dss_set s, dss_sort_params_t* params) {
         params->merge_fan_in = s->sort_params.merge_fan;
         params->pass_mode = s->sort_params.pass_mode;
         params->merge_tile = s->sort_params.merge_tile;
+        params->digit_chars = s->sort_params.digit_chars;
     });
 }
 

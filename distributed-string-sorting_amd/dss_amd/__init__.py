@@ -181,21 +181,24 @@ class DeviceStringSet:
         check(lib().dss_get_sort_params(self._h, C.byref(t)))
         return dict(large_min=t.large_min, wave_class_max=t.wave_class_max, skip_lanes=t.skip_lanes,
                     skip_strings=t.skip_strings, merge_fan_in=t.merge_fan_in, pass_mode=t.pass_mode,
-                    merge_tile=t.merge_tile)
+                    merge_tile=t.merge_tile, digit_chars=t.digit_chars)
 
     def set_sort_params(self, params=None):
         """Local-sort tuning (large_min, wave_class_max, skip_lanes,
         skip_strings, merge_fan_in, pass_mode: 0 auto, 1 host-driven
         passes, 2 device-driven passes replayed from hipGraphs, merge_tile:
-        512, 1024 or 2048 outputs per pairwise merge tile); None restores
-        the defaults.  Every setting gives the same output."""
+        512, 1024 or 2048 outputs per pairwise merge tile, digit_chars: 0
+        auto, 1, or up to 2 / 3 characters per radix digit on small
+        alphabets); None restores the defaults.  Every setting gives the
+        same output."""
         if params is None:
             check(lib().dss_set_sort_params(self._h, None))
             return
         cur = self.sort_params()
         cur.update(params)
         t = _lib.SortParams(cur["large_min"], cur["wave_class_max"], cur["skip_lanes"], cur["skip_strings"],
-                            cur.get("merge_fan_in", 0), cur.get("pass_mode", 0), cur.get("merge_tile", 0))
+                            cur.get("merge_fan_in", 0), cur.get("pass_mode", 0), cur.get("merge_tile", 0),
+                            cur.get("digit_chars", 0))
         check(lib().dss_set_sort_params(self._h, C.byref(t)))
 
     def _per_string(self, a, dtype, what):

@@ -32,7 +32,8 @@ class KernelStat(C.Structure):
 class SortParams(C.Structure):
     _fields_ = [("large_min", C.c_uint32), ("wave_class_max", C.c_uint32),
                 ("skip_lanes", C.c_uint32), ("skip_strings", C.c_uint32),
-                ("merge_fan_in", C.c_uint32), ("pass_mode", C.c_uint32), ("merge_tile", C.c_uint32)]
+                ("merge_fan_in", C.c_uint32), ("pass_mode", C.c_uint32), ("merge_tile", C.c_uint32),
+                ("digit_chars", C.c_uint32)]
 
 
 class CheckResult(C.Structure):

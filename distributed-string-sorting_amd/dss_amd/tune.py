@@ -21,7 +21,7 @@ import os
 import time
 
 DEFAULT = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 2,
-           "pass_mode": 0, "merge_tile": 1024}
+           "pass_mode": 0, "merge_tile": 1024, "digit_chars": 0}
 
 CANDIDATES = [
     {"large_min": lm, "wave_class_max": wc, "skip_lanes": sk, "skip_strings": ss}
@@ -31,6 +31,7 @@ CANDIDATES = [
 MERGE_FANS = (2, 4, 8, 16)
 PASS_MODES = (1, 2)  # host-driven passes; device-driven passes replayed from hipGraphs
 MERGE_TILES = (512, 1024, 2048)  # outputs per tile of the pairwise merge
+DIGIT_CHARS = (1, 2, 3)  # characters per radix digit (more than one on small alphabets only)
 
 
 def regime(info: dict, avg_lcp: int | None = None) -> str:
@@ -75,15 +76,17 @@ class Tuner:
         return best
 
     def autotune(self, ds, candidates=None, repeats: int = 2, fans=MERGE_FANS, modes=PASS_MODES,
-                 tiles=MERGE_TILES) -> dict:
+                 tiles=MERGE_TILES, digits=DIGIT_CHARS) -> dict:
         """Times ds.sort_lcp() under every candidate, the best candidate under
-        every pass mode, then (more than one PE) ds.merge_lcp() of the
+        every radix digit width (characters per digit, host-driven passes)
+        and every pass mode, then (more than one PE) ds.merge_lcp() of the
         exchanged runs under every merge fan-in and, with fan-in 2, every
         merge tile size; keeps the fastest for the set's regime.  Returns
-        {"regime", "best", "times_ms", "pass_times_ms", "merge_times_ms",
-        "merge_tile_times_ms"}."""
+        {"regime", "best", "times_ms", "digit_times_ms", "pass_times_ms",
+        "merge_times_ms", "merge_tile_times_ms"}."""
         candidates = [dict(c, merge_fan_in=DEFAULT["merge_fan_in"], pass_mode=DEFAULT["pass_mode"],
-                           merge_tile=DEFAULT["merge_tile"]) for c in (candidates or CANDIDATES)]
+                           merge_tile=DEFAULT["merge_tile"], digit_chars=DEFAULT["digit_chars"])
+                      for c in (candidates or CANDIDATES)]
         ds.set_sort_params(DEFAULT)
         ds.sort_lcp()  # warm up buffers; also gives the LCPs for the regime key
         ds.sync()
@@ -94,6 +97,13 @@ class Tuner:
             times.append((self._time(ds, ds.sort_lcp, repeats), cand))
         times.sort(key=lambda t: t[0])
         best = dict(times[0][1])
+        digit_times = []
+        for dc in digits:
+            ds.set_sort_params(dict(best, digit_chars=dc, pass_mode=1))
+            digit_times.append((self._time(ds, ds.sort_lcp, repeats), dc))
+        if digit_times:
+            digit_times.sort()
+            best["digit_chars"] = digit_times[0][1]
         pass_times = []
         for mode in modes:
             ds.set_sort_params(dict(best, pass_mode=mode))
@@ -127,6 +137,7 @@ class Tuner:
         ds.set_sort_params(best)
         return {"regime": key, "best": dict(best),
                 "times_ms": [(round(t * 1e3, 3), c) for t, c in times],
+                "digit_times_ms": [(round(t * 1e3, 3), d) for t, d in digit_times],
                 "pass_times_ms": [(round(t * 1e3, 3), m) for t, m in pass_times],
                 "merge_times_ms": [(round(t * 1e3, 3), f) for t, f in merge_times],
                 "merge_tile_times_ms": [(round(t * 1e3, 3), m) for t, m in tile_times]}

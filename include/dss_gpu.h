@@ -161,7 +161,12 @@ int dss_get_sorted(dss_set s, uint32_t* perm, uint32_t* lcp);
  * the host-driven one otherwise (the host-driven form is also used while
  * kernel timing is on, and on the null stream); merge_tile (512, 1024 or
  * 2048, default 1024; 0 selects the default) is the number of outputs per
- * tile of the pairwise merge (fan-in 2).  Every setting produces the same
+ * tile of the pairwise merge (fan-in 2); digit_chars is the number of
+ * characters one radix digit of the host-driven passes may cover: 1 one
+ * (the byte), 2 or 3 up to that many for small alphabets (the sort scans the
+ * arena's byte values and packs kc characters into a digit when (values + 1)
+ * ^ kc <= 256), 0 (default) up to 3 for sets of at least 2^22 strings whose
+ * arena holds at most 2 GiB, one otherwise.  Every setting produces the same
  * output.  NULL restores the defaults. */
 typedef struct {
     uint32_t large_min;
@@ -171,6 +176,7 @@ typedef struct {
     uint32_t merge_fan_in;
     uint32_t pass_mode;
     uint32_t merge_tile;
+    uint32_t digit_chars;
 } dss_sort_params_t;
 
 int dss_set_sort_params(dss_set s, const dss_sort_params_t* params);

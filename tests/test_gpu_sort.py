@@ -120,22 +120,81 @@ def test_skewed_generator_shapes(n, ll, sl, ldn, sdn, lp, dp):
 
 PARAM_SETS = [
     {"large_min": 16, "wave_class_max": 0, "skip_lanes": 8, "skip_strings": 1, "merge_fan_in": 2, "pass_mode": 1,
-     "merge_tile": 512},
+     "merge_tile": 512, "digit_chars": 3},
     {"large_min": 64, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 2, "merge_fan_in": 4, "pass_mode": 2,
-     "merge_tile": 2048},
+     "merge_tile": 2048, "digit_chars": 3},
     {"large_min": 256, "wave_class_max": 5, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 8, "pass_mode": 0,
-     "merge_tile": 1024},
+     "merge_tile": 1024, "digit_chars": 0},
     {"large_min": 1024, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 1, "merge_fan_in": 16, "pass_mode": 2,
-     "merge_tile": 512},
+     "merge_tile": 512, "digit_chars": 2},
     {"large_min": 128, "wave_class_max": 4, "skip_lanes": 8, "skip_strings": 2, "merge_fan_in": 2, "pass_mode": 1,
-     "merge_tile": 2048},
+     "merge_tile": 2048, "digit_chars": 2},
     {"large_min": 32, "wave_class_max": 6, "skip_lanes": 16, "skip_strings": 4, "merge_fan_in": 16, "pass_mode": 2,
-     "merge_tile": 1024},
+     "merge_tile": 1024, "digit_chars": 1},
     {"large_min": 16, "wave_class_max": 3, "skip_lanes": 8, "skip_strings": 1, "merge_fan_in": 2, "pass_mode": 2,
-     "merge_tile": 512},
+     "merge_tile": 512, "digit_chars": 2},
 ]
 DEFAULT_PARAMS = {"large_min": 512, "wave_class_max": 6, "skip_lanes": 8, "skip_strings": 4, "merge_fan_in": 2,
-                  "pass_mode": 0, "merge_tile": 1024}
+                  "pass_mode": 0, "merge_tile": 1024, "digit_chars": 0}
+
+
+def _small_alphabet(n, alphabet, max_len, seed, late=b""):
+    """n strings over `alphabet` of lengths 0..max_len, a tenth of them
+    repeated and a tenth cut to prefixes of others (ends inside digits), then
+    the strings of `late` at the end of the arena."""
+    rng = np.random.default_rng(seed)
+    sym = np.frombuffer(alphabet, dtype=np.uint8)
+    out = []
+    for _ in range(n):
+        k = int(rng.integers(0, max_len + 1))
+        out.append(sym[rng.integers(0, sym.size, k)].tobytes())
+    for i in rng.integers(0, n, n // 10):
+        out.append(out[int(i)])
+    for i in rng.integers(0, n, n // 10):
+        t = out[int(i)]
+        out.append(t[: int(rng.integers(0, len(t) + 1))])
+    order = rng.permutation(len(out))
+    return arena([out[int(i)] for i in order] + [t for t in late.split(b",") if late])
+
+
+MULTICHAR_INPUTS = {
+    # sigma = values + 1 (the end): 5 -> 3 characters per digit, 3 -> 3, 11 -> 2
+    "acgt": lambda: _small_alphabet(60000, b"ACGT", 30, 41),
+    "binary": lambda: _small_alphabet(60000, b"01", 40, 42),
+    "ten": lambda: _small_alphabet(60000, b"abcdefghij", 20, 43),
+    "suffixes": lambda: O.gen_suffixes(30000, 44, repeats=20, rep_min=200, rep_max=2000, cap=300),
+    # more than the 1 MiB the alphabet scan samples first; the two strings
+    # after it add 'Z' and 'a' (sigma 7: 2 characters per digit)
+    "late_symbols": lambda: _small_alphabet(90000, b"ACGT", 24, 45, late=b"ACGZT,ZZ,aAC,ACGTa"),
+}
+
+
+@pytest.mark.parametrize("large_min", [16, 512])
+@pytest.mark.parametrize("name", sorted(MULTICHAR_INPUTS))
+def test_multichar_digits_match_one_char_digits(name, large_min):
+    """Radix digits of two or three characters on small alphabets (the byte
+    values the arena holds, ranked, plus the end as rank 0) give the same
+    permutation and LCP array as one-character digits, and the oracle's LCPs
+    and order: bucket boundaries inside a digit, strings that end inside a
+    digit (duplicates, prefixes, empty strings), keys re-gathered when the
+    next digit leaves the 8 cached characters, and byte values that appear
+    only after the scanned sample."""
+    dss = _dss()
+    chars = MULTICHAR_INPUTS[name]()
+    assert name != "late_symbols" or chars.size > (1 << 20)
+    ss = O.StringSet(chars, 4)
+    operm, olcp = O.local_sort(ss)
+    with dss.DeviceStringSet.from_arena(chars, 4) as ds:
+        out = {}
+        for dc in (1, 2, 3):
+            ds.set_sort_params({"large_min": large_min, "pass_mode": 1, "digit_chars": dc})
+            ds.sort_lcp()
+            out[dc] = ds.get_sorted()
+    for dc in (2, 3):
+        assert (out[dc][0] == out[1][0]).all() and (out[dc][1] == out[1][1]).all(), dc
+    assert (out[1][1] == olcp).all()
+    a = canonical_ties(out[1][0], out[1][1], ss.len, ss.pe_begin)
+    assert (a == canonical_ties(operm, olcp, ss.len, ss.pe_begin)).all()
 
 
 @pytest.mark.parametrize("name", ["dton", "skewed", "suffix"])
@@ -169,12 +228,14 @@ def test_autotuner_picks_and_applies_a_setting(tmp_path):
     tuner = Tuner(str(tmp_path / "t.json"))
     with dss.DeviceStringSet.dton(200000, 100, 0.5, 34, 4) as ds:
         out = tuner.autotune(ds, repeats=1)
-        later = ("merge_fan_in", "pass_mode", "merge_tile")
+        later = ("merge_fan_in", "pass_mode", "merge_tile", "digit_chars")
         sort_best = {k: v for k, v in out["best"].items() if k not in later}
         assert sort_best in [{k: v for k, v in c.items() if k not in later} for _, c in out["times_ms"]]
         if out["best"]["merge_fan_in"] == 2:
             assert out["best"]["merge_tile"] == out["merge_tile_times_ms"][0][1]
             assert sorted(m for _, m in out["merge_tile_times_ms"]) == [512, 1024, 2048]
+        assert out["best"]["digit_chars"] == out["digit_times_ms"][0][1]
+        assert sorted(d for _, d in out["digit_times_ms"]) == [1, 2, 3]
         assert out["best"]["pass_mode"] == out["pass_times_ms"][0][1]
         assert sorted(m for _, m in out["pass_times_ms"]) == [1, 2]
         assert out["best"]["merge_fan_in"] == out["merge_times_ms"][0][1]
@@ -198,6 +259,8 @@ def test_autotuner_picks_and_applies_a_setting(tmp_path):
             ds.set_sort_params({"pass_mode": 3})
         with pytest.raises(dss.DssError):
             ds.set_sort_params({"merge_tile": 768})
+        with pytest.raises(dss.DssError):
+            ds.set_sort_params({"digit_chars": 4})
 
 
 def _two_letter_strings(n, length, seed):
