@@ -547,6 +547,54 @@ __device__ inline uint64_t xxh64_digest(uint64_t v1, uint64_t v2, uint64_t v3, u
     return h;
 }
 
+// Rounds of depth at most kLaneDepth hash one string per lane: the lane
+// loads the aligned words that cover the string's first depth characters
+// (independent loads, issued together) and runs XXH64 in registers.  Hashes
+// are stored in list order.
+constexpr uint32_t kLaneDepth = 64;
+constexpr uint32_t kLaneWords = kLaneDepth / 8 + 1;  // aligned words covering kLaneDepth characters
+
+__global__ void __launch_bounds__(256)
+k_pd_hash_lane(const uint32_t* __restrict__ list, const unsigned long long* __restrict__ nhashed,
+               const uint64_t* __restrict__ s_off, const uint8_t* __restrict__ chars, uint32_t depth, uint64_t m,
+               uint64_t* __restrict__ hashv) {
+    const uint64_t nh = *nhashed;
+    const uint32_t nw = (depth + 7) / 8 + 1;  // <= kLaneWords
+    const uint32_t stripes = depth / 32;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t o = s_off[list[i]];
+        const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + (o & ~7ull));
+        const uint32_t sh = (uint32_t)(o & 7) * 8;
+        uint64_t x[kLaneWords + 1];
+#pragma unroll
+        for (uint32_t k = 0; k < kLaneWords; ++k) x[k] = k < nw ? w[k] : 0ull;
+        x[kLaneWords] = 0;
+        // the little-endian word at character 8 k of the string (k < nw)
+        auto word = [&](uint32_t k) -> uint64_t {
+            uint64_t lo = 0, hi = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < kLaneWords; ++q)
+                if (q == k) {
+                    lo = x[q];
+                    hi = x[q + 1];
+                }
+            return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+        };
+        uint64_t v1 = P1 + P2, v2 = P2, v3 = 0, v4 = (uint64_t)0 - P1;
+        for (uint32_t st = 0; st < stripes; ++st) {
+            v1 = xround(v1, word(4 * st));
+            v2 = xround(v2, word(4 * st + 1));
+            v3 = xround(v3, word(4 * st + 2));
+            v4 = xround(v4, word(4 * st + 3));
+        }
+        // tail words past the loaded ones are never read by the digest
+        // (it reads depth % 32 characters)
+        const uint32_t t = 4 * stripes;
+        const uint64_t t0 = word(t), t1 = word(t + 1), t2 = word(t + 2), t3 = word(t + 3);
+        hashv[i] = mod_range(xxh64_digest(v1, v2, v3, v4, depth, t0, t1, t2, t3), m);
+    }
+}
+
 // XXH64 of the listed candidates, four lanes per string and kHashU strings
 // per lane group at a time: a run head hashes its own bytes from the start,
 // any other position continues from its head's state after L stripes.  Lane
@@ -1197,6 +1245,10 @@ void prefix_doubling(dss_set_s* s, uint64_t m, uint32_t start_depth) {
                                    s->arena(), (uint32_t)depth, L, m, P.pd_hash.as<uint64_t>(),
                                    stage_on ? P.pd_stage.as<uint64_t>() : nullptr,
                                    stage_on ? P.pd_sdepth.as<uint32_t>() : nullptr, stage_w);
+            else if (nh && depth <= kLaneDepth)
+                hipLaunchKernelGGL(k_pd_hash_lane, dim3(stride_grid(nh, 256)), blk, 0, st, P.pd_list.as<uint32_t>(),
+                                   ctr, s->s_off.as<uint64_t>(), s->arena(), (uint32_t)depth, m,
+                                   P.pd_hash.as<uint64_t>());
             else if (nh)
                 hipLaunchKernelGGL(k_pd_hash, g, blk, 0, st, P.pd_list.as<uint32_t>(), ctr, s->s_off.as<uint64_t>(),
                                    s->arena(), (uint32_t)depth, m, P.pd_hash.as<uint64_t>());
