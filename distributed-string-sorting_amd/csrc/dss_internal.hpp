@@ -286,6 +286,7 @@ struct PipeState {
     // state: one packed row per position and the depth it was read for;
     // valid for the results of the last prefix doubling (stage_valid)
     DevBuf pd_stage, pd_sdepth;
+    DevBuf copy_chunks;  // exchange copy: the string holding each 4 KiB chunk's first byte
     bool stage_valid = false;
     uint32_t stage_w = 0;  // words per kept row: 4 + ceil((depth % 32) / 8)
     // partition

@@ -599,16 +599,16 @@ __device__ inline uint32_t receiver_of(const uint64_t* __restrict__ recv_begin, 
 constexpr uint32_t kXcWords = 512;  // LDS words per wave for the short-copy path
 constexpr uint32_t kXcP = 64;        // receivers up to which the copy stages its tables
 
-// 2^lg lanes per received string copy its bytes into the receive arena.
-// With lg = 0 (short copies, such as the few characters after the LCP that
-// an LCP-compressed distinguishing prefix sends) every lane copies one
-// string, 32 characters at a time from five aligned 8-byte loads, so a wave
-// has 64 strings in flight.  The lane (lane 0 of the string's lanes) also
-// stores the merge's cached keys: the string's 16 characters at its
-// receiver's common prefix cp (two words), from the lines the copy reads.
+// Short copies (at most 33 characters on average, such as the few
+// characters after the LCP that an LCP-compressed distinguishing prefix
+// sends): every lane copies one received string into the receive arena, 32
+// characters at a time from five aligned 8-byte loads, so a wave has 64
+// strings in flight.  The lane also stores the merge's cached keys: the
+// string's 16 characters at its receiver's common prefix cp (two words),
+// from the lines the copy reads.
 __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* __restrict__ r_voff,
                             const uint32_t* __restrict__ r_vlen, const uint32_t* __restrict__ r_from,
-                            const uint64_t* __restrict__ byte_off, int mode, uint64_t n, uint32_t lg,
+                            const uint64_t* __restrict__ byte_off, uint64_t n,
                             uint8_t* __restrict__ out, const uint64_t* __restrict__ recv_begin, uint32_t p,
                             const uint32_t* __restrict__ cp, uint64_t* __restrict__ keyc,
                             uint64_t* __restrict__ keyc2,
@@ -626,7 +626,7 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
     }
     const uint64_t* const RB = staged_tab ? s_rb : recv_begin;
     const uint32_t* const CP = staged_tab ? s_cp : cp;
-    if (lg == 0) {
+    {
         // A wave's 64 consecutive receive positions own one contiguous byte
         // range of the receive arena.  When it fits, the lanes build it in
         // LDS and the wave stores it as aligned words (byte stores only for
@@ -785,46 +785,161 @@ __global__ void k_exch_copy(const uint8_t* __restrict__ chars, const uint64_t* _
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             }
         }
-        return;
     }
-    // Longer copies: 2^lg lanes per string.  An output word belongs to the
-    // string that holds its first byte; the string's lanes build each of its
-    // words whole (its characters, its NUL, then the first bytes of the
-    // strings that follow when it ends inside the word) and store it as one
-    // aligned 8-byte word, so every word of the receive arena is written once.
-    const uint32_t lanes = 1u << lg;
-    const uint32_t l = threadIdx.x & (lanes - 1);
-    for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lg; k < n;
-         k += ((uint64_t)gridDim.x * blockDim.x) >> lg) {
-        const uint32_t from = r_from[k];
-        const uint32_t vlen = r_vlen[k];
-        const uint64_t sa = r_voff[k] + from;  // first source character
-        if (l == 0) {
+}
+
+// Longer copies, balanced over output words: wave c builds the receive
+// arena's bytes [kXbChunk c, kXbChunk (c + 1)), lane l its words l + 64 j,
+// so every store instruction writes 512 contiguous bytes and neighbouring
+// lanes read neighbouring source words.  A word's owner is the last string
+// that starts at or before its first byte (a max-scan over the chunk's
+// words of where each string starts); the word holds that string's
+// characters from there, its NUL, then the heads of the strings after it.
+// The lane that walks a string whose start lies in the chunk also stores
+// its 16 cached merge characters at its receiver's common prefix.
+constexpr uint32_t kXbChunk = 4096;            // output bytes per wave
+constexpr uint32_t kXbWords = kXbChunk / 8;    // output words per wave
+constexpr uint32_t kXbPer = kXbWords / 64;     // output words per lane
+constexpr uint32_t kXbMeta = 128;              // strings per chunk whose layout is staged in LDS
+
+// chunk_first[c]: the string that holds output byte kXbChunk c.
+__global__ void k_copy_chunks(const uint64_t* __restrict__ byte_off, const uint32_t* __restrict__ r_vlen,
+                              const uint32_t* __restrict__ r_from, uint64_t n, uint32_t* __restrict__ chunk_first) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t o = byte_off[k], e = o + (r_vlen[k] - r_from[k]) + 1;
+        for (uint64_t c = (o + kXbChunk - 1) / kXbChunk; c * kXbChunk < e; ++c) chunk_first[c] = (uint32_t)k;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_exch_copy_bal(const uint8_t* __restrict__ chars, const uint64_t* __restrict__ r_voff,
+                const uint32_t* __restrict__ r_vlen, const uint32_t* __restrict__ r_from,
+                const uint64_t* __restrict__ byte_off, uint64_t n, uint64_t total,
+                const uint32_t* __restrict__ chunk_first, uint64_t nchunks, uint8_t* __restrict__ out,
+                const uint64_t* __restrict__ recv_begin, uint32_t p, const uint32_t* __restrict__ cp,
+                uint64_t* __restrict__ keyc, uint64_t* __restrict__ keyc2) {
+    __shared__ uint64_t s_rb[kXcP + 1];
+    __shared__ uint32_t s_cp[kXcP];
+    __shared__ uint32_t mark_all[4][kXbWords];
+    __shared__ uint64_t mo_all[4][kXbMeta], ms_all[4][kXbMeta];  // output start, source start
+    __shared__ uint32_t mm_all[4][kXbMeta];                      // characters sent
+    const bool staged_tab = p <= kXcP;  // grid-uniform
+    if (staged_tab) {
+        for (uint32_t t = threadIdx.x; t <= p; t += blockDim.x) s_rb[t] = recv_begin[t];
+        for (uint32_t t = threadIdx.x; t < p; t += blockDim.x) s_cp[t] = cp[t];
+        __syncthreads();
+    }
+    const uint64_t* const RB = staged_tab ? s_rb : recv_begin;
+    const uint32_t* const CP = staged_tab ? s_cp : cp;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wv = (threadIdx.x >> 6) & 3;
+    uint32_t* const mk = mark_all[wv];
+    uint64_t* const mo = mo_all[wv];
+    uint64_t* const msrc = ms_all[wv];
+    uint32_t* const mm = mm_all[wv];
+    const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < nchunks; c += waves) {
+        const uint64_t A = c * kXbChunk, E = min(A + kXbChunk, total);
+        const uint32_t k0 = chunk_first[c];
+        uint32_t k1 = (uint32_t)(n - 1);  // the last string that starts before E
+        if (c + 1 < nchunks) {
+            k1 = chunk_first[c + 1];
+            if (byte_off[k1] >= E) --k1;
+        }
+        // mk[w] = the last string (relative to k0) that starts at or before
+        // byte A + 8 w: marks where each string starts, then a max-scan
+        for (uint32_t w = lane; w < kXbWords; w += 64) mk[w] = 0;
+        // the layout of the chunk's first kXbMeta strings
+        for (uint32_t r = lane; r < kXbMeta && k0 + r <= k1; r += 64) {
+            const uint32_t k = k0 + r, f = r_from[k];
+            mo[r] = byte_off[k];
+            mm[r] = r_vlen[k] - f;
+            msrc[r] = r_voff[k] + f;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (uint32_t k = k0 + 1 + lane; k <= k1; k += 64) {
+            const uint64_t ws = ((k - k0 < kXbMeta ? mo[k - k0] : byte_off[k]) - A + 7) / 8;
+            if (ws < kXbWords) atomicMax(&mk[ws], k - k0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        uint32_t loc[kXbPer];
+#pragma unroll
+        for (uint32_t i = 0; i < kXbPer; ++i) {
+            loc[i] = mk[kXbPer * lane + i];
+            if (i) loc[i] = max(loc[i], loc[i - 1]);
+        }
+        uint32_t incl = loc[kXbPer - 1];
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl = max(incl, y);
+        }
+        uint32_t excl = __shfl_up(incl, 1, 64);
+        if (lane == 0) excl = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < kXbPer; ++i) mk[kXbPer * lane + i] = max(loc[i], excl);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        // the cached merge keys of the strings that start in this chunk
+        for (uint32_t k = k0 + lane; k <= k1; k += 64) {
+            if (byte_off[k] < A) continue;  // k0 started in an earlier chunk
             const uint32_t cpk = CP[receiver_of(RB, p, k)];
+            const uint32_t vlen = r_vlen[k];
             keyc[k] = load_key8(chars, r_voff[k], vlen, cpk);
             keyc2[k] = load_key8(chars, r_voff[k], vlen, cpk + 8);
         }
-        // modes 0/1: chars [from, vlen] incl. the NUL; mode 2: [from, vlen) + NUL
-        const uint32_t m = vlen - from;
-        const uint64_t o = byte_off[k], e = o + m + 1;
-        for (uint64_t w = ((o + 7) >> 3) + l; w < (e + 7) >> 3; w += lanes) {
-            const uint64_t g = w << 3;
-            uint64_t v;
-            if (g + 8 <= e) {
-                v = load_word_le(chars, sa + (g - o));
-                if (g + 8 == e) v &= ~(0xFFull << 56);  // the NUL is the word's last byte
-            } else {
-                const uint32_t nk = (uint32_t)(e - g);  // 1..7 bytes of this string, the last its NUL
-                v = nk > 1 ? load_word_le(chars, sa + (g - o)) & ((1ull << (8 * (nk - 1))) - 1) : 0;
-                uint32_t filled = nk;
-                for (uint64_t kk = k + 1; filled < 8 && kk < n; ++kk) {
-                    const uint32_t f2 = r_from[kk], m2 = r_vlen[kk] - f2;
-                    const uint32_t take = min(8u - filled, m2 + 1);  // its characters, then its NUL
-                    const uint32_t nch = min(take, m2);              // < 8
-                    if (nch) v |= (load_word_le(chars, r_voff[kk] + f2) & ((1ull << (8 * nch)) - 1)) << (8 * filled);
-                    filled += take;
+        // the words: characters of the owner from the word's first byte on,
+        // the owner's NUL, then the strings after it
+        const uint64_t nw = (E - A + 7) / 8;
+#pragma unroll
+        for (uint32_t j = 0; j < kXbPer; ++j) {
+            const uint32_t w = lane + 64 * j;
+            if (w >= nw) continue;
+            const uint64_t g = A + 8ull * w;
+            uint64_t k = k0 + mk[w];
+            uint64_t v = 0;
+            uint32_t filled = 0;
+            // string k's layout: from LDS for the chunk's first kXbMeta strings
+            auto layout = [&](uint64_t kk, uint64_t& o, uint32_t& m, uint64_t& src) {
+                const uint64_t r = kk - k0;
+                if (r < kXbMeta) {
+                    o = mo[r];
+                    m = mm[r];
+                    src = msrc[r];
+                } else {
+                    const uint32_t f = r_from[kk];
+                    o = byte_off[kk];
+                    m = r_vlen[kk] - f;
+                    src = r_voff[kk] + f;
                 }
-                // bytes past the last string stay zero (the arena padding)
+            };
+            uint64_t o, src;
+            uint32_t m;
+            layout(k, o, m, src);
+            // the owner's part (every word of a long string's middle ends
+            // here), then the strings that follow while the word has room
+            uint32_t at = (uint32_t)(g - o);  // output byte of k; byte m is its NUL
+            uint32_t take = min(8u, m + 1 - at);
+            uint32_t nch = at < m ? min(take, m - at) : 0u;  // characters before the NUL
+            if (nch) {
+                v = load_word_le(chars, src + at);
+                if (nch < 8) v &= (1ull << (8 * nch)) - 1;
+            }
+            filled = take;
+            while (filled < 8 && g + filled < total) {
+                layout(++k, o, m, src);
+                take = min(8u - filled, m + 1);
+                nch = min(take, m);
+                if (nch) {
+                    uint64_t x = load_word_le(chars, src);
+                    if (nch < 8) x &= (1ull << (8 * nch)) - 1;
+                    v |= x << (8 * filled);
+                }
+                filled += take;
             }
             *reinterpret_cast<uint64_t*>(out + g) = v;
         }
@@ -954,17 +1069,28 @@ void exchange(dss_set_s* s, int mode) {
     fill_small(st, P.recv_bytes.as<uint8_t>() + total, 0, kArenaPad);
     if (n) {
         KScope ks(s->timer, st, "exchange_copy", 2.0 * total + n * 40.0);
-        // a lane per string for copies of at most 32 characters on average,
-        // 4 lanes up to 256 (distinguishing prefixes, LCP-compressed strings:
-        // more strings in flight per wave beat more lanes per string), 16
-        // lanes above (whole strings)
-        const uint32_t lg = total <= 33 * n ? 0 : total <= 257 * n ? 2 : 4;
-        const uint64_t threads = n << lg;
-        hipLaunchKernelGGL(k_exch_copy, dim3(stride_grid(threads, 256)), dim3(256), 0, st, s->arena(),
-                           P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
-                           P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), mode, n, lg,
-                           P.recv_bytes.as<uint8_t>(), rb_d, p, cpd, P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>(),
-                           lg == 0 ? r_stage : nullptr, staged ? P.pd_stage.as<uint64_t>() : nullptr, P.stage_w);
+        // a lane per string for copies of at most 32 characters on average
+        // (the wave assembles its range in LDS), otherwise waves over 4 KiB
+        // chunks of the receive arena, a lane per output word
+        if (total <= 33 * n) {
+            hipLaunchKernelGGL(k_exch_copy, dim3(stride_grid(n, 256)), dim3(256), 0, st, s->arena(),
+                               P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                               P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), n,
+                               P.recv_bytes.as<uint8_t>(), rb_d, p, cpd, P.m_keyc.as<uint64_t>(),
+                               P.m_keyc2.as<uint64_t>(), r_stage, staged ? P.pd_stage.as<uint64_t>() : nullptr,
+                               P.stage_w);
+        } else {
+            const uint64_t nchunks = (total + kXbChunk - 1) / kXbChunk;
+            P.copy_chunks.reserve(nchunks * 4);
+            hipLaunchKernelGGL(k_copy_chunks, dim3(stride_grid(n, 256)), dim3(256), 0, st,
+                               P.recv_byte_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                               P.recv_from.as<uint32_t>(), n, P.copy_chunks.as<uint32_t>());
+            hipLaunchKernelGGL(k_exch_copy_bal, dim3(stride_grid(nchunks * 64, 256)), dim3(256), 0, st, s->arena(),
+                               P.recv_view_off.as<uint64_t>(), P.recv_view_len.as<uint32_t>(),
+                               P.recv_from.as<uint32_t>(), P.recv_byte_off.as<uint64_t>(), n, total,
+                               P.copy_chunks.as<uint32_t>(), nchunks, P.recv_bytes.as<uint8_t>(), rb_d, p, cpd,
+                               P.m_keyc.as<uint64_t>(), P.m_keyc2.as<uint64_t>());
+        }
     }
     if (n && mode == 3) {
         KScope ks(s->timer, st, "exchange_lcp", 2.0 * total + n * 20.0);
