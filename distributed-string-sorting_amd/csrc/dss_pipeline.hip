@@ -2292,15 +2292,23 @@ void permutation(dss_set_s* s) {
 // psv is computed per 1024-position block with a stack, falling back to a
 // backward walk over earlier blocks (block minima) for open positions.
 constexpr uint32_t kPsvB = 1024;
+constexpr uint32_t kPsvG = 32;  // positions per group minimum
 
-__global__ void k_block_min(const uint32_t* __restrict__ saved, const uint64_t* __restrict__ recv_begin, uint32_t p,
-                            uint64_t n, uint32_t* __restrict__ bmin) {
+// Per block of kPsvB positions its minimum (bmin) and the minima of its
+// groups of kPsvG positions (gmin), for the look-back of k_psv.
+__global__ void __launch_bounds__(256)
+k_block_min(const uint32_t* __restrict__ saved, const uint64_t* __restrict__ recv_begin, uint32_t p, uint64_t n,
+            uint32_t* __restrict__ bmin, uint32_t* __restrict__ gmin) {
     __shared__ uint32_t red[4];
     const uint64_t b = (uint64_t)blockIdx.x * kPsvB;
     uint32_t m = 0xFFFFFFFFu;
     for (uint32_t t = threadIdx.x; t < kPsvB; t += blockDim.x) {
         const uint64_t i = b + t;
-        if (i < n) m = min(m, saved[i]);
+        uint32_t x = i < n ? saved[i] : 0xFFFFFFFFu;
+        // the group's minimum over its kPsvG consecutive lanes
+        for (uint32_t o = kPsvG / 2; o > 0; o >>= 1) x = min(x, (uint32_t)__shfl_xor(x, o, kPsvG));
+        if ((t & (kPsvG - 1)) == 0) gmin[(uint64_t)blockIdx.x * (kPsvB / kPsvG) + t / kPsvG] = x;
+        m = min(m, x);
     }
     for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_down(m, o));
     if (lane_id() == 0) red[threadIdx.x >> 6] = m;
@@ -2309,45 +2317,68 @@ __global__ void k_block_min(const uint32_t* __restrict__ saved, const uint64_t* 
 }
 
 // psv[i] = max{k < i in the same receiver : saved[k] < saved[i]}, or ~0
-// when saved[i] == 0 or no such k exists (the receiver's first string).
-__global__ void k_psv(const uint32_t* __restrict__ saved, const uint64_t* __restrict__ recv_begin, uint32_t p,
-                      uint64_t n, const uint32_t* __restrict__ bmin, uint64_t* __restrict__ psv) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t lo = 0, hi = p;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (recv_begin[mid] <= i) lo = mid + 1; else hi = mid;
+// when saved[i] == 0 or no such k exists (the receiver's first string).  A
+// block of kPsvB positions builds a min-tree of its values in LDS; each
+// position climbs from its leaf until a left sibling holds a smaller value
+// and descends into its rightmost smaller leaf (2 log2 kPsvB steps).
+// Positions whose block holds none look back over earlier blocks by their
+// minima and scan the first block that holds one from its end.
+__global__ void __launch_bounds__(256)
+k_psv(const uint32_t* __restrict__ saved, const uint64_t* __restrict__ recv_begin, uint32_t p, uint64_t n,
+      const uint32_t* __restrict__ bmin, const uint32_t* __restrict__ gmin, uint64_t* __restrict__ psv) {
+    __shared__ uint32_t tree[2 * kPsvB];  // node j: min of its subtree; leaves kPsvB + t
+    const uint64_t b = (uint64_t)blockIdx.x * kPsvB;
+    for (uint32_t t = threadIdx.x; t < kPsvB; t += blockDim.x) tree[kPsvB + t] = b + t < n ? saved[b + t] : 0u;
+    __syncthreads();
+    for (uint32_t lvl = kPsvB / 2; lvl >= 1; lvl >>= 1) {
+        for (uint32_t j = lvl + threadIdx.x; j < 2 * lvl; j += blockDim.x) tree[j] = min(tree[2 * j], tree[2 * j + 1]);
+        __syncthreads();
     }
-    const uint64_t rb = recv_begin[lo - 1];
-    const uint32_t v = saved[i];
-    uint64_t res = ~0ull;
-    if (v != 0 && i > rb) {
-        uint64_t k = i;
-        // walk back inside the current block
-        const uint64_t blk0 = (i / kPsvB) * kPsvB;
-        while (k > blk0 && k > rb) {
-            --k;
-            if (saved[k] < v) { res = k; break; }
+    for (uint32_t t = threadIdx.x; t < kPsvB; t += blockDim.x) {
+        const uint64_t i = b + t;
+        if (i >= n) break;
+        uint32_t lo = 0, hi = p;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (recv_begin[mid] <= i) lo = mid + 1; else hi = mid;
         }
-        if (res == ~0ull && k > rb) {
-            // earlier blocks: skip those whose minimum is not smaller
-            int64_t blk = (int64_t)(blk0 / kPsvB) - 1;
-            while (blk >= 0 && (uint64_t)(blk + 1) * kPsvB > rb) {
-                if (bmin[blk] < v) {
-                    uint64_t e = (uint64_t)(blk + 1) * kPsvB;
-                    const uint64_t s0 = std::max<uint64_t>((uint64_t)blk * kPsvB, rb);
-                    while (e > s0) {
-                        --e;
-                        if (saved[e] < v) { res = e; break; }
-                    }
-                    if (res != ~0ull) break;
+        const uint64_t rb = recv_begin[lo - 1];
+        const uint32_t v = tree[kPsvB + t];
+        uint64_t res = ~0ull;
+        if (v != 0 && i > rb) {
+            uint32_t node = kPsvB + t;
+            while (node > 1) {
+                if ((node & 1) && tree[node - 1] < v) {  // the left sibling holds a smaller value
+                    node -= 1;
+                    while (node < kPsvB) node = tree[2 * node + 1] < v ? 2 * node + 1 : 2 * node;
+                    res = b + (node - kPsvB);
+                    break;
                 }
-                --blk;
+                node >>= 1;
             }
+            if (res == ~0ull) {
+                // earlier blocks: skip those whose minimum is not smaller
+                int64_t blk = (int64_t)(b / kPsvB) - 1;
+                while (blk >= 0 && (uint64_t)(blk + 1) * kPsvB > rb) {
+                    if (bmin[blk] < v) {
+                        // the last group holding a smaller value, then its last such position
+                        for (int32_t gi = (int32_t)(kPsvB / kPsvG) - 1; gi >= 0 && res == ~0ull; --gi) {
+                            if (gmin[(uint64_t)blk * (kPsvB / kPsvG) + gi] >= v) continue;
+                            const uint64_t g0 = (uint64_t)blk * kPsvB + (uint64_t)gi * kPsvG;
+                            for (uint64_t e = g0 + kPsvG; e > g0;) {
+                                --e;
+                                if (saved[e] < v) { res = e; break; }
+                            }
+                        }
+                        break;  // the block holds one: found
+                    }
+                    --blk;
+                }
+            }
+            if (res != ~0ull && res < rb) res = ~0ull;  // nearest smaller lies in an earlier receiver
         }
+        psv[i] = res;
     }
-    psv[i] = res;
 }
 
 __global__ void k_out_size(const uint32_t* __restrict__ ord, const uint32_t* __restrict__ saved,
@@ -2407,15 +2438,18 @@ void extend_prefix(dss_set_s* s) {
     const uint64_t nblk = (n + kPsvB - 1) / kPsvB;
     P.out_psv.reserve(nn * 8);
     P.out_off.reserve((nn + 1) * 8);
-    P.out_aux.reserve((nblk + 1) * 4 + (p + 1) * 8 + 64);
+    P.out_aux.reserve((nblk + 1) * 4 * (1 + kPsvB / kPsvG) + (p + 1) * 8 + 64);
     uint32_t* bmin = P.out_aux.as<uint32_t>();
-    uint64_t* rb_d = reinterpret_cast<uint64_t*>(P.out_aux.as<uint8_t>() + ((nblk + 1) * 4 + 7) / 8 * 8);
+    uint32_t* gmin = bmin + (nblk + 1);
+    uint64_t* rb_d = reinterpret_cast<uint64_t*>(
+        P.out_aux.as<uint8_t>() + ((nblk + 1) * 4 * (1 + kPsvB / kPsvG) + 7) / 8 * 8);
     DSS_HIP(hipMemcpyAsync(rb_d, P.recv_begin.data(), (p + 1) * 8, hipMemcpyHostToDevice, st));
     uint64_t total = 0;
     if (n) {
         KScope ks(s->timer, st, "extend_psv", n * 16.0);
-        hipLaunchKernelGGL(k_block_min, dim3(nblk), dim3(256), 0, st, P.m_saved.as<uint32_t>(), rb_d, p, n, bmin);
-        hipLaunchKernelGGL(k_psv, dim3((n + 255) / 256), dim3(256), 0, st, P.m_saved.as<uint32_t>(), rb_d, p, n, bmin,
+        hipLaunchKernelGGL(k_block_min, dim3(nblk), dim3(256), 0, st, P.m_saved.as<uint32_t>(), rb_d, p, n, bmin,
+                           gmin);
+        hipLaunchKernelGGL(k_psv, dim3(nblk), dim3(256), 0, st, P.m_saved.as<uint32_t>(), rb_d, p, n, bmin, gmin,
                            P.out_psv.as<uint64_t>());
         hipLaunchKernelGGL(k_out_size, dim3((n + 255) / 256), dim3(256), 0, st, P.m_order.as<uint32_t>(),
                            P.m_saved.as<uint32_t>(), P.recv_size.as<uint64_t>(), n, P.out_off.as<uint64_t>());
