@@ -1898,6 +1898,7 @@ k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_
         if (jl < lb_n) lb = prev_is_a ? lh : sL[bs + jl + 1];
     }
     const uint32_t oend = live ? (uint32_t)min((uint64_t)dl + kMW, d1 - d0) : dl;
+    uint64_t res[kMW];  // (receive index << 32) | LCP of the thread's outputs
     for (uint32_t q = 0; q < kMW; ++q) {
         const uint32_t o = dl + q;
         const bool valid = o < oend;
@@ -1927,10 +1928,21 @@ k_merge_tile(const MergePair* __restrict__ pairs, uint32_t npairs, const uint64_
             ++jl;
             if (jl < lb_n) lb = sL[bs + jl + 1];
         }
-        const uint64_t go = d0 + o;
-        ord_out[A + go] = outo;
-        lcp_out[A + go] = (go == 0) ? 0u : lout;
-        if (saved) saved[A + go] = rlcp[outo];  // the last round: the LCP it was received with
+        res[q] = ((uint64_t)outo << 32) | ((d0 + o == 0) ? 0u : lout);
+    }
+    // the tile's outputs leave through LDS, so that consecutive lanes store
+    // consecutive outputs (a thread's kMW outputs are adjacent)
+    __syncthreads();  // every comparison has read the staged keys
+    for (uint32_t q = 0; q < kMW; ++q)
+        if (dl + q < oend) sK1[dl + q] = res[q];
+    __syncthreads();
+    const uint32_t cnt = (uint32_t)(d1 - d0);
+    for (uint32_t o = threadIdx.x; o < cnt; o += kMT) {
+        const uint64_t r = sK1[o];
+        const uint32_t outo = (uint32_t)(r >> 32);
+        ord_out[A + d0 + o] = outo;
+        lcp_out[A + d0 + o] = (uint32_t)r;
+        if (saved) saved[A + d0 + o] = rlcp[outo];  // the last round: the LCP it was received with
     }
 }
 
