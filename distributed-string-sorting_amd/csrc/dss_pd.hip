@@ -1068,7 +1068,7 @@ static uint64_t partition_duplicates(dss_set_s* s, uint64_t nh, uint32_t table_l
     // LDS table holds kHpCap * 7/8), so the digit passes stay narrow and
     // their runs per tile long (short runs leave partly written lines)
     uint32_t kb = 2;
-    while (kb < 2 * kHpMaxBits && (nh >> kb) > 768) ++kb;
+    while (kb < 2 * kHpMaxBits && (nh >> kb) > kHpCap / 8 * 3) ++kb;
     const uint32_t b1 = kb / 2, b0 = kb - b1;  // pass 0: the lower b0 of the top kb bits; pass 1: the top b1
     const uint64_t ntiles = (nh + kHpTile - 1) / kHpTile;
     const uint32_t nbk = 1u << kb;
