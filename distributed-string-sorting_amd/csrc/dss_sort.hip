@@ -246,9 +246,12 @@ __device__ inline uint32_t seg_of(const Seg* __restrict__ segs, uint32_t nseg, u
 }
 
 // kDev: segment and tile counts from the pass counters hp, grid-stride.
+// pn: the next pass's counters, cleared here (before this pass's
+// classification counts into them).
 template <bool kDev>
 __global__ void k_fill_tiles(const Seg* __restrict__ segs, uint32_t nseg, uint32_t ntiles,
-                             Tile* __restrict__ tiles, const uint32_t* __restrict__ hp) {
+                             Tile* __restrict__ tiles, const uint32_t* __restrict__ hp, uint32_t* __restrict__ pn) {
+    if (blockIdx.x == 0 && threadIdx.x < 4) pn[threadIdx.x] = 0;
     if (kDev) {
         nseg = hp[kCtrLarge];
         ntiles = hp[kCtrTiles];
@@ -1770,9 +1773,8 @@ static void host_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint3
         Seg* const nxt = b.nxt;
         uint64_t* const kc = b.kc;
         uint64_t* const kn = b.kn;
-        fill_small(st, pn, 0, 16);
         hipLaunchKernelGGL(k_fill_tiles<false>, dim3((nt + 255) / 256), dim3(256), 0, st, cur, nlarge, nt,
-                           X.tiles.as<Tile>(), nullptr);
+                           X.tiles.as<Tile>(), nullptr, pn);
         Tile* tl = X.tiles.as<Tile>();
         // elements in large segments of this pass (for the algorithmic byte counts)
         std::vector<Seg> hseg;
@@ -1972,9 +1974,8 @@ static void launch_dev_pass(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, P
     Tile* tl = X.tiles.as<Tile>();
     const uint32_t gt = grid_cap(caps.tile_cap, 1, kDevGrid);
     const uint32_t gc = grid_cap(caps.chunk_cap, 1, 256);
-    fill_small(st, pn, 0, 16);
     hipLaunchKernelGGL(k_fill_tiles<true>, dim3(grid_cap(caps.tile_cap, 256, kDevGrid)), dim3(256), 0, st, b.cur, 0u,
-                       0u, tl, hp);
+                       0u, tl, hp, pn);
     hipLaunchKernelGGL((k_prefix_skip<kL, kU, true>), dim3(gt), dim3(kThreads), 0, st, tl, b.cur, perm, chars, off,
                        len, b.kc, first ? nullptr : b.pc_ids, hp);
     hipLaunchKernelGGL(k_gather<true>, dim3(gt), dim3(kThreads), 0, st, tl, b.cur, perm, chars, off, len, b.kc, hp);
@@ -2159,7 +2160,28 @@ void sort_lcp(dss_set_s* s) {
     // runs before the read-back), and the small-class counts
     uint32_t h_ctr[64];
     hipLaunchKernelGGL(k_build_tiles, dim3(1), dim3(1024), 0, st, X.segA.as<Seg>(), ctr + kCtrPass);
-    s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
+    // the first pass's segment, tile and chunk counts follow from the PE
+    // sizes (one segment per PE above large_min); without large segments the
+    // small-class counts the seed made are read back instead
+    {
+        uint64_t nl = 0, nt = 0, nc = 0;
+        for (uint32_t r = 0; r < s->p; ++r) {
+            const uint64_t cnt = s->pe_begin[r + 1] - s->pe_begin[r];
+            if (cnt <= T.large_min) continue;
+            const uint64_t t = (cnt + kTile - 1) / kTile;
+            ++nl;
+            nt += t;
+            nc += t > kChunkTiles ? (t + kChunkTiles - 1) / kChunkTiles : 0;
+        }
+        if (nl) {
+            std::memset(h_ctr, 0, sizeof(h_ctr));
+            h_ctr[kCtrPass + kCtrLarge] = (uint32_t)nl;
+            h_ctr[kCtrPass + kCtrTiles] = (uint32_t)nt;
+            h_ctr[kCtrPass + kCtrChunks] = (uint32_t)nc;
+        } else {
+            s->pp.mbox.fetch(st, h_ctr, ctr, sizeof(h_ctr));
+        }
+    }
     const bool trace = getenv("DSS_SORT_TRACE") != nullptr;
     PassBufs b = first_pass_bufs(X);
     b.dm = digit_map(s);
