@@ -48,8 +48,9 @@ def inputs():
         # LDS buffer
         "short_with_long": arena(_short_with_long(rng)),
         # 0-7 characters with 300 strings of 3000: more than 33 bytes per
-        # string on average, so the exchange takes its multi-lane path, where
-        # one output word can hold the ends of several short strings
+        # string on average, so the exchange copy is balanced over output
+        # words, where one word can hold the ends of several short strings and
+        # a 4 KiB chunk holds more strings than it stages in LDS
         "tiny_with_long": arena(_tiny_with_long(np.random.default_rng(5))),
     }
 
