@@ -128,6 +128,34 @@ __device__ inline int view_cmp32(const uint8_t* __restrict__ chars, uint64_t ao,
     }
 }
 
+// kN consecutive keys (8 kN characters) from `depth`; words past the
+// string's end are not read (the arena padding covers only the five words
+// of load_keys4), keys past it are zero.
+template <int kN>
+__device__ inline void load_keys_n(const uint8_t* __restrict__ chars, uint64_t off, uint32_t len, uint32_t depth,
+                                   uint64_t k[kN]) {
+    if (depth >= len) {
+#pragma unroll
+        for (int j = 0; j < kN; ++j) k[j] = 0;
+        return;
+    }
+    const uint64_t a = off + depth, wa = a & ~7ull, end = off + len;
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(chars + wa);
+    const unsigned sh = (unsigned)(a & 7) * 8;
+    uint64_t x[kN + 1];
+#pragma unroll
+    for (int i = 0; i <= kN; ++i) x[i] = wa + 8 * (uint64_t)i < end ? w[i] : 0ull;
+#pragma unroll
+    for (int j = 0; j < kN; ++j) {
+        uint64_t v = sh ? (x[j] >> sh) | (x[j + 1] << (64 - sh)) : x[j];
+        v = __builtin_bswap64(v);
+        const uint32_t d = depth + 8 * j;
+        if (d >= len) v = 0;
+        else if (len - d < 8) v &= ~0ull << (8 * (8 - (len - d)));
+        k[j] = v;
+    }
+}
+
 // ---------------------------------------------------------------- wave helpers
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 

@@ -1612,11 +1612,15 @@ struct TileView {
 // Compares, for every lane of the wave with `act` set, the strings of slots
 // x and y, known to agree on their first `from` characters; returns the sign
 // and the LCP.  All lanes of the wave call it together.  The staged keys are
-// compared per lane; the lanes they leave undecided are then served one at a
-// time by the whole wave, lane l comparing the 8 characters at from + 8 l of both strings (512 characters
-// per dependent step), so a lane whose strings agree on hundreds of
-// characters costs the wave one or two loads instead of a loop of
-// 32-character steps that the other lanes wait out.
+// compared per lane; the lanes they leave undecided walk their strings on
+// lane groups, up to eight at a time, each lane of a group comparing 32
+// characters (256 to 1024 characters per group and dependent step); the
+// last undecided lane is served by the whole wave, lane l comparing the 8
+// characters at from + 8 l (512 characters per dependent step).  A lane
+// whose strings agree on hundreds of characters thus costs the wave one or
+// two loads instead of a loop of 32-character steps that the other lanes
+// wait out.
+constexpr int kWideKeys = 4;  // keys (8 characters each) per lane and step of a grouped walk
 __device__ inline int wave_tile_cmp(const TileView& T, bool act, uint32_t x, uint32_t y, uint32_t from,
                                     uint32_t& lcp) {
     int c = 0;
@@ -1659,62 +1663,71 @@ __device__ inline int wave_tile_cmp(const TileView& T, bool act, uint32_t x, uin
     }
     const uint32_t lane = lane_id();
     uint64_t m = __ballot(pend);
-    if (__popcll(m) >= 2) {
-        // several open lanes: each first takes one step on its own group of
-        // W = 64 / kG lanes (up to kG = 8 lanes at a time, 8 W characters),
-        // so short walks cost one dependent load per group of lanes instead
-        // of one per lane; those still open continue one at a time below
+    // several open lanes: up to kG = 8 at a time, each on its own group of
+    // W = 64 / kG lanes whose every lane reads kWideKeys keys, so a group
+    // reaches 32 W characters (256 to 1024) per dependent step; a lane still
+    // open after its step is taken again, the last open lane is served by
+    // the whole wave below
+    while (__popcll(m) >= 2) {  // wave-uniform
         uint64_t todo = m;
-        while (todo) {  // wave-uniform
-            const uint32_t k = min(8u, (uint32_t)__popcll(todo));
-            const uint32_t lgk = k <= 1 ? 0u : (k <= 2 ? 1u : (k <= 4 ? 2u : 3u));
-            const uint32_t W = 64u >> lgk, g = lane >> (6 - lgk);
-            uint64_t sel = 0;
-            uint32_t myld = 0;
-            bool gact = false;
-            for (uint32_t r = 0; r < k; ++r) {
-                const uint32_t ldr = (uint32_t)__ffsll((unsigned long long)todo) - 1;
-                todo &= todo - 1;
-                sel |= 1ull << ldr;
-                if (g == r) {
-                    myld = ldr;
-                    gact = true;
+        const uint32_t k = min(8u, (uint32_t)__popcll(todo));
+        const uint32_t lgk = k <= 2 ? 1u : (k <= 4 ? 2u : 3u);
+        const uint32_t W = 64u >> lgk, g = lane >> (6 - lgk);
+        uint64_t sel = 0;
+        uint32_t myld = 0;
+        bool gact = false;
+        for (uint32_t r = 0; r < k; ++r) {
+            const uint32_t ldr = (uint32_t)__ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            sel |= 1ull << ldr;
+            if (g == r) {
+                myld = ldr;
+                gact = true;
+            }
+        }
+        const uint64_t pa = __shfl((unsigned long long)ao, myld), pb = __shfl((unsigned long long)bo, myld);
+        const uint32_t pal = (uint32_t)__shfl((int)al, myld), pbl = (uint32_t)__shfl((int)bl, myld);
+        const uint32_t pf = (uint32_t)__shfl((int)from, myld);
+        const uint32_t sub = lane & (W - 1);
+        uint64_t ea = 0, eb = 0;
+        uint32_t ej = 0;
+        bool ev = false;
+        if (gact) {
+            uint64_t ka[kWideKeys], kb[kWideKeys];
+            load_keys_n<kWideKeys>(T.chars, pa, pal, pf + 8 * kWideKeys * sub, ka);
+            load_keys_n<kWideKeys>(T.chars, pb, pbl, pf + 8 * kWideKeys * sub, kb);
+#pragma unroll
+            for (int j = kWideKeys - 1; j >= 0; --j) {  // the lane's first event wins
+                if (ka[j] != kb[j] || (ka[j] & 0xFF) == 0) {
+                    ev = true;
+                    ea = ka[j];
+                    eb = kb[j];
+                    ej = (uint32_t)j;
                 }
             }
-            const uint64_t pa = __shfl((unsigned long long)ao, myld), pb = __shfl((unsigned long long)bo, myld);
-            const uint32_t pal = (uint32_t)__shfl((int)al, myld), pbl = (uint32_t)__shfl((int)bl, myld);
-            const uint32_t pf = (uint32_t)__shfl((int)from, myld);
-            const uint32_t sub = lane & (W - 1);
-            uint64_t ka = 0, kb = 0;
-            bool ev = false;
-            if (gact) {
-                ka = load_key8(T.chars, pa, pal, pf + 8 * sub);
-                kb = load_key8(T.chars, pb, pbl, pf + 8 * sub);
-                ev = ka != kb || (ka & 0xFF) == 0;
-            }
-            const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
-            const uint64_t gm = (__ballot(ev) >> (g * W)) & wmask;
-            const uint32_t f = g * W + (gm ? (uint32_t)__ffsll((unsigned long long)gm) - 1 : 0u);
-            const uint64_t fa = __shfl((unsigned long long)ka, f), fb = __shfl((unsigned long long)kb, f);
-            int gc = 0;
-            uint32_t glcp = 0;
-            if (gm) {
-                gc = fa < fb ? -1 : (fa > fb ? 1 : 0);
-                glcp = pf + 8 * (f - g * W) + (fa != fb ? key_lcp(fa, fb) : key_strlen(fa));
-            }
-            // a selected lane's group is its rank among the selected lanes
-            const uint32_t src = min((uint32_t)__popcll(sel & lanemask_lt()), (1u << lgk) - 1) * W;
-            const int rc = __shfl(gc, src);
-            const uint32_t rl = (uint32_t)__shfl((int)glcp, src);
-            const bool rd = __shfl((int)(gm != 0), src) != 0;
-            if ((sel >> lane) & 1) {
-                if (rd) {
-                    c = rc;
-                    lcp = rl;
-                    pend = false;
-                } else {
-                    from += 8 * W;
-                }
+        }
+        const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
+        const uint64_t gm = (__ballot(ev) >> (g * W)) & wmask;
+        const uint32_t f = g * W + (gm ? (uint32_t)__ffsll((unsigned long long)gm) - 1 : 0u);
+        const uint64_t fa = __shfl((unsigned long long)ea, f), fb = __shfl((unsigned long long)eb, f);
+        const uint32_t fj = (uint32_t)__shfl((int)ej, f);
+        int gc = 0;
+        uint32_t glcp = 0;
+        if (gm) {
+            gc = fa < fb ? -1 : (fa > fb ? 1 : 0);
+            glcp = pf + 8 * kWideKeys * (f - g * W) + 8 * fj + (fa != fb ? key_lcp(fa, fb) : key_strlen(fa));
+        }
+        const uint32_t src = min((uint32_t)__popcll(sel & lanemask_lt()), (1u << lgk) - 1) * W;
+        const int rc = __shfl(gc, src);
+        const uint32_t rl = (uint32_t)__shfl((int)glcp, src);
+        const bool rd = __shfl((int)(gm != 0), src) != 0;
+        if ((sel >> lane) & 1) {
+            if (rd) {
+                c = rc;
+                lcp = rl;
+                pend = false;
+            } else {
+                from += 8 * kWideKeys * W;
             }
         }
         m = __ballot(pend);
