@@ -1780,8 +1780,11 @@ static void host_passes(dss_set_s* s, const SortCaps& caps, uint32_t* ctr, uint3
         std::vector<Seg> hseg;
         uint64_t cnt_large = 0;
         if (s->timer.enabled) {
+            // read in stream order: the set's stream does not block on the
+            // null stream, and the first pass's segments may still be in flight
             hseg.resize(nlarge);
-            DSS_HIP(hipMemcpy(hseg.data(), cur, nlarge * sizeof(Seg), hipMemcpyDeviceToHost));
+            DSS_HIP(hipMemcpyAsync(hseg.data(), cur, nlarge * sizeof(Seg), hipMemcpyDeviceToHost, st));
+            DSS_HIP(hipStreamSynchronize(st));
             for (auto& q : hseg) cnt_large += q.cnt;
         }
         {
